@@ -1,0 +1,172 @@
+/*
+ * kspn/scc.h -- C ABI of libkspn, a single-GPU (MI355X, gfx950) strongly
+ * connected components solver with the staged method of rawbby/KaSpan.
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes. `stream` is a
+ * hipStream_t passed as void* (NULL = the default stream). Every call
+ * returns 0 on success or a negative KSPN_E_* code; nothing aborts and no
+ * C++ exception crosses this boundary.
+ *
+ * Graph contract (kaspan/include/kaspan/graph/bidi_graph_part.hpp:12-27,
+ * graph/base.hpp:17-28 with KASPAN_INDEX64): n vertices with uint32 ids,
+ * m directed edges; fw_off[n+1] / fw_adj[m] is the out-edge CSR and
+ * bw_off[n+1] / bw_adj[m] its exact transpose (kspn_transpose builds it);
+ * offsets are uint64 with off[0] = 0 and off[n] = m. Neighbours inside a
+ * row need not be sorted; self-loops and duplicate edges are tolerated.
+ *
+ * Output convention: labels[v] is the canonical component id, the minimum
+ * vertex id of v's SCC -- the reference's ids after normalise_scc_id
+ * (kaspan/include/kaspan/test/normalise_scc.hpp:10-25).
+ */
+#ifndef KSPN_SCC_H
+#define KSPN_SCC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KSPN_OK 0
+#define KSPN_E_INVALID (-1)       /* bad argument or malformed CSR */
+#define KSPN_E_OOM (-2)           /* device or host allocation failed */
+#define KSPN_E_HIP (-3)           /* a HIP runtime call failed */
+#define KSPN_E_ITER_CAP (-4)      /* max_rounds reached before every vertex was decided */
+#define KSPN_E_UNIMPLEMENTED (-5) /* option combination not built */
+
+#define KSPN_UNDECIDED 0xFFFFFFFFu
+
+/* opaque per-device context: scratch memory, HIP events, pinned counters */
+typedef struct kspn_ctx kspn_ctx;
+
+typedef struct kspn_graph {
+  uint32_t n;
+  uint64_t m;
+  const uint64_t* fw_off; /* device, n+1 */
+  const uint32_t* fw_adj; /* device, m */
+  const uint64_t* bw_off; /* device, n+1 */
+  const uint32_t* bw_adj; /* device, m */
+} kspn_graph;
+
+/* pipeline shapes; all give identical canonical labels */
+#define KSPN_METHOD_KASPAN 0  /* scc.hpp:21-87: trim-1 first pass + pivot, FW-BW, trim-1 normal pass, colour rounds */
+#define KSPN_METHOD_TRIM_EX 1 /* variant/scc_trim_ex*.hpp: exhaustive trim-1 (+trim-2), FW-BW rounds, colour rounds with trims, residual Tarjan */
+
+#define KSPN_PIVOT_PLAIN 0   /* max in*out degree product, smallest id on ties (trim_1_local.hpp:32-45) */
+#define KSPN_PIVOT_LEARNED 1 /* MLP score (kspn_mlp_*), plain rule as fallback */
+
+#define KSPN_FLAG_PROFILE 1u  /* time every kernel launch with HIP events (slower; fills kspn_stats.kern) */
+#define KSPN_FLAG_NO_TRIM2 2u /* KSPN_METHOD_TRIM_EX without the trim-2 pass */
+
+typedef struct kspn_opts {
+  uint32_t method;             /* KSPN_METHOD_* */
+  uint32_t pivot;              /* KSPN_PIVOT_* */
+  uint32_t fwbw_rounds;        /* FW-BW rounds before colouring (TRIM_EX); 0 -> 1 */
+  uint32_t residual_threshold; /* hand the rest to residual Tarjan once live <= this (0: never) */
+  uint32_t max_rounds;         /* colour-round cap (0 -> unlimited); exceeding it -> KSPN_E_ITER_CAP */
+  uint32_t flags;              /* KSPN_FLAG_* */
+  float alpha;                 /* push->pull switch: edges(frontier) * alpha > edges(unvisited); 0 -> 14 */
+  float beta;                  /* pull->push switch: frontier * beta < live; 0 -> 24 */
+  const float* mlp_params;     /* device, KSPN_MLP_PARAMS floats, for KSPN_PIVOT_LEARNED */
+} kspn_opts;
+
+/* per-kernel profile line, filled with KSPN_FLAG_PROFILE */
+#define KSPN_KERN_MAX 24
+typedef struct kspn_kern_stat {
+  char name[32];
+  uint32_t launches;
+  double ms;    /* summed launch durations (HIP events around each launch) */
+  double bytes; /* summed algorithmic bytes (each array element touched once, stored width) */
+} kspn_kern_stat;
+
+typedef struct kspn_stats {
+  double ms_total, ms_trim, ms_fwbw, ms_color, ms_residual, ms_canon;
+  uint32_t decided_trim, decided_trim2, decided_fwbw, decided_color, decided_residual;
+  uint32_t pivot;          /* first FW-BW pivot */
+  uint32_t fwbw_levels_push, fwbw_levels_pull;
+  uint32_t color_rounds, color_iters;
+  uint32_t residual_n;
+  uint64_t residual_m;
+  uint64_t edges_scanned;  /* adjacency entries read, all phases */
+  double bytes_trim, bytes_fwbw, bytes_color, bytes_residual, bytes_canon; /* algorithmic bytes */
+  uint32_t n_kern;
+  kspn_kern_stat kern[KSPN_KERN_MAX];
+} kspn_stats;
+
+/* defaults: KASPAN method, plain pivot, 1 FW-BW round, no residual, no cap */
+void kspn_opts_default(kspn_opts* o);
+const char* kspn_strerror(int code);
+
+/* ---- context ------------------------------------------------------------ */
+/* Creates a context on `device`; scratch grows on demand to the largest graph seen. */
+int kspn_ctx_create(kspn_ctx** out, int device);
+int kspn_ctx_destroy(kspn_ctx* ctx);
+/* pre-size scratch for graphs up to n vertices and m edges */
+int kspn_ctx_reserve(kspn_ctx* ctx, uint32_t n, uint64_t m);
+
+/* ---- the hot path ------------------------------------------------------- */
+/* kaspan::scc(bidi_graph_part_view<single_part_view>, vertex_t* scc_id)
+ * (kaspan/include/kaspan/scc/scc.hpp:21-25): fills labels[n] (device) with
+ * canonical ids. Stream-ordered; returns after the labels are complete
+ * (the host reads phase counters). `stats` may be NULL. */
+int kspn_scc(kspn_ctx* ctx, const kspn_graph* g, uint32_t* labels, const kspn_opts* opts,
+             kspn_stats* stats, void* stream);
+
+/* the same with host-resident CSR arrays and host labels (copies in and out) */
+int kspn_scc_host(kspn_ctx* ctx, uint32_t n, uint64_t m, const uint64_t* fw_off, const uint32_t* fw_adj,
+                  const uint64_t* bw_off, const uint32_t* bw_adj, uint32_t* labels, const kspn_opts* opts,
+                  kspn_stats* stats);
+
+/* ---- stages callable on their own ---------------------------------------- */
+/* backward_complement_graph (kaspan/include/kaspan/scc/backward_complement.hpp:22-68):
+ * reverse CSR with ascending sources inside each row. Device pointers. */
+int kspn_transpose(kspn_ctx* ctx, uint32_t n, uint64_t m, const uint64_t* fw_off, const uint32_t* fw_adj,
+                   uint64_t* bw_off, uint32_t* bw_adj, void* stream);
+
+/* normalise_scc_id (kaspan/include/kaspan/test/normalise_scc.hpp:10-25) for
+ * ids that are vertex ids: labels[v] <- min { u : labels[u] == labels[v] }. */
+int kspn_canonicalize(kspn_ctx* ctx, uint32_t n, uint32_t* labels, void* stream);
+
+/* allgather_fw_sub_graph at one rank (kaspan/include/kaspan/scc/allgather_sub_graph.hpp:253-301):
+ * induced subgraph on mask[v] != 0; sub ids ascend with global ids; an edge
+ * is kept iff both ends are in the subgraph, in CSR order. super_ids[n],
+ * sub_off[n+1], sub_adj[m] are caller buffers (upper bounds); *sub_n and
+ * *sub_m are written on the host. */
+int kspn_subgraph(kspn_ctx* ctx, uint32_t n, const uint64_t* off, const uint32_t* adj, const uint8_t* mask,
+                  uint32_t* super_ids, uint64_t* sub_off, uint32_t* sub_adj, uint32_t* sub_n,
+                  uint64_t* sub_m, void* stream);
+
+/* tarjan (kaspan/include/kaspan/scc/tarjan.hpp:20-136) on the GPU, one
+ * wavefront, labels = minimum member id. For small graphs (the residual). */
+int kspn_tarjan(kspn_ctx* ctx, uint32_t n, uint64_t m, const uint64_t* off, const uint32_t* adj,
+                uint32_t* labels, void* stream);
+
+/* ---- seeded graph construction (inputs for tests and benchmarks) --------- */
+#define KSPN_GEN_GNM 0     /* p0 = n, p1 = draws */
+#define KSPN_GEN_RMAT 1    /* p0 = scale, p1 = draws, a,b,c, permute */
+#define KSPN_GEN_LATTICE 2 /* p0 = side, a = p_bidirectional */
+
+typedef struct kspn_gen_params {
+  uint32_t kind;
+  uint64_t seed;
+  uint64_t p0, p1;
+  double a, b, c;
+  uint32_t permute;
+} kspn_gen_params;
+
+/* Generates the edge list on the device, removes self-loops and duplicates
+ * and builds both CSRs. The library allocates the four arrays with
+ * hipMalloc; free them with kspn_free. */
+int kspn_generate(kspn_ctx* ctx, const kspn_gen_params* p, uint32_t* n, uint64_t* m, uint64_t** fw_off,
+                  uint32_t** fw_adj, uint64_t** bw_off, uint32_t** bw_adj, void* stream);
+/* CSR + reverse CSR from a device edge list (src[i] -> dst[i]); drops self-loops and duplicates */
+int kspn_build_csr(kspn_ctx* ctx, uint32_t n, uint64_t count, const uint32_t* src, const uint32_t* dst,
+                   uint64_t* m, uint64_t** fw_off, uint32_t** fw_adj, uint64_t** bw_off, uint32_t** bw_adj,
+                   void* stream);
+int kspn_free(void* device_ptr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,307 @@
+"""kaspan_amd -- KaSpan's ``scc(graph) -> label per vertex`` on one MI355X.
+
+Host-side mirror of the reference's interface for this path (paths are in
+the rawbby/KaSpan tree):
+
+=====================================  ===========================================================
+kaspan_amd                             reference
+=====================================  ===========================================================
+``BidiGraph``                          ``bidi_graph`` / ``bidi_graph_view`` (graph/bidi_graph.hpp:17,250)
+``scc(graph, scc_id)``                 ``scc(bidi_graph_part_view, vertex_t* scc_id)`` (scc/scc.hpp:21-87)
+``scc_trim_ex(graph, scc_id)``         ``scc_trim_ex`` (scc/variant/scc_trim_ex.hpp:16-100)
+``scc_trim_ex_residual(graph, ...)``   ``scc_trim_ex_residual`` (scc/variant/scc_trim_ex_residual.hpp:17-128)
+``backward_complement_graph``          ``backward_complement_graph`` (scc/backward_complement.hpp:22-68)
+``allgather_fw_sub_graph``             ``allgather_fw_sub_graph`` at one rank (scc/allgather_sub_graph.hpp:253-301)
+``tarjan``                             ``tarjan`` with min-id callback (scc/tarjan.hpp:124-136)
+``normalise_scc_id``                   ``normalise_scc_id`` (test/normalise_scc.hpp:10-25)
+=====================================  ===========================================================
+
+Every call goes through libkspn's C ABI (``include/kspn/scc.h``); device
+memory is held in torch tensors (offsets as int64 = uint64 bits, ids as
+int32 = uint32 bits). Differences: ``scc_id`` holds canonical ids (minimum
+vertex id per component, the reference's ids after ``normalise_scc_id``);
+errors raise ``KspnError`` where the reference aborts.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import KspnError, Stats
+
+__all__ = [
+    "BidiGraph", "Context", "KspnError", "Stats", "scc", "scc_trim_ex", "scc_trim_ex_residual",
+    "backward_complement_graph", "allgather_fw_sub_graph", "tarjan", "normalise_scc_id", "generate",
+    "METHOD_KASPAN", "METHOD_TRIM_EX", "UNDECIDED",
+]
+
+METHOD_KASPAN = _lib.METHOD_KASPAN
+METHOD_TRIM_EX = _lib.METHOD_TRIM_EX
+UNDECIDED = _lib.UNDECIDED
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise KspnError(-1, "kaspan_amd needs a HIP device (torch.cuda.is_available() is False)")
+
+
+class Context:
+    """Owns a kspn_ctx (scratch memory, events, pinned counters) on one device."""
+
+    _default: dict = {}
+
+    def __init__(self, device: int = 0):
+        _require_gpu()
+        self.device = int(device)
+        self.ptr = C.c_void_p()
+        _lib.check(_lib.lib().kspn_ctx_create(C.byref(self.ptr), self.device), "kspn_ctx_create")
+
+    @classmethod
+    def default(cls, device: int = 0) -> "Context":
+        ctx = cls._default.get(device)
+        if ctx is None:
+            ctx = cls._default[device] = cls(device)
+        return ctx
+
+    def reserve(self, n: int, m: int):
+        _lib.check(_lib.lib().kspn_ctx_reserve(self.ptr, n, m), "kspn_ctx_reserve")
+
+    def close(self):
+        if self.ptr:
+            _lib.lib().kspn_ctx_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _stream_ptr(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _as_u64_tensor(a, device):
+    if isinstance(a, torch.Tensor):
+        return a.to(device=device, dtype=torch.int64).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint64).view(np.int64)).to(device)
+
+
+def _as_u32_tensor(a, device):
+    if isinstance(a, torch.Tensor):
+        return a.to(device=device, dtype=torch.int32).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint32).view(np.int32)).to(device)
+
+
+class BidiGraph:
+    """CSR + reverse CSR resident in HBM: uint64 offsets, uint32 ids.
+
+    ``fw_off[n+1]``/``fw_adj[m]`` hold out-edges, ``bw_off``/``bw_adj`` the exact
+    transpose (graph/bidi_graph_part.hpp:12-27 with KASPAN_INDEX64).
+    """
+
+    def __init__(self, n, fw_off, fw_adj, bw_off=None, bw_adj=None, device="cuda", ctx: Context | None = None):
+        _require_gpu()
+        self.n = int(n)
+        self.device = torch.device(device)
+        self.fw_off = _as_u64_tensor(fw_off, self.device)
+        self.fw_adj = _as_u32_tensor(fw_adj, self.device)
+        self.m = int(self.fw_adj.numel())
+        if self.fw_off.numel() != self.n + 1:
+            raise KspnError(-1, "fw_off must have n+1 entries")
+        if bw_off is None:
+            bw_off, bw_adj = backward_complement_graph(self.n, self.fw_off, self.fw_adj, ctx=ctx)
+        self.bw_off = _as_u64_tensor(bw_off, self.device)
+        self.bw_adj = _as_u32_tensor(bw_adj, self.device)
+        if self.bw_off.numel() != self.n + 1 or self.bw_adj.numel() != self.m:
+            raise KspnError(-1, "bw CSR does not match the fw CSR's shape")
+
+    @classmethod
+    def from_host(cls, n, fw_off, fw_adj, bw_off=None, bw_adj=None, device="cuda"):
+        return cls(n, fw_off, fw_adj, bw_off, bw_adj, device=device)
+
+    @classmethod
+    def from_edges(cls, n, src, dst, device="cuda", ctx: Context | None = None, stream=None):
+        """Build both CSRs on the GPU from an edge list; self-loops and duplicates removed."""
+        ctx = ctx or Context.default(torch.device(device).index or 0)
+        s = _as_u32_tensor(src, device)
+        d = _as_u32_tensor(dst, device)
+        m = C.c_uint64()
+        ptrs = [C.c_void_p() for _ in range(4)]
+        _lib.check(_lib.lib().kspn_build_csr(ctx.ptr, n, s.numel(), C.c_void_p(s.data_ptr()), C.c_void_p(d.data_ptr()),
+                                             C.byref(m), *(C.byref(p) for p in ptrs), _stream_ptr(stream)),
+                   "kspn_build_csr")
+        return cls._adopt(n, m.value, ptrs, device)
+
+    @classmethod
+    def _adopt(cls, n, m, ptrs, device):
+        """copy library-allocated CSR arrays into torch tensors and free them"""
+        torch.cuda.synchronize()
+        fo = torch.empty(n + 1, dtype=torch.int64, device=device)
+        fa = torch.empty(max(m, 0), dtype=torch.int32, device=device)
+        bo = torch.empty(n + 1, dtype=torch.int64, device=device)
+        ba = torch.empty(max(m, 0), dtype=torch.int32, device=device)
+        try:
+            _lib.memcpy_d2d(fo.data_ptr(), ptrs[0].value, 8 * (n + 1))
+            _lib.memcpy_d2d(fa.data_ptr(), ptrs[1].value, 4 * m)
+            _lib.memcpy_d2d(bo.data_ptr(), ptrs[2].value, 8 * (n + 1))
+            _lib.memcpy_d2d(ba.data_ptr(), ptrs[3].value, 4 * m)
+        finally:
+            for p in ptrs:
+                _lib.lib().kspn_free(p)
+        g = cls.__new__(cls)
+        g.n, g.m, g.device = int(n), int(m), torch.device(device)
+        g.fw_off, g.fw_adj, g.bw_off, g.bw_adj = fo, fa, bo, ba
+        return g
+
+    def view(self) -> _lib.Graph:
+        return _lib.Graph(self.n, self.m, self.fw_off.data_ptr(), self.fw_adj.data_ptr() if self.m else None,
+                          self.bw_off.data_ptr(), self.bw_adj.data_ptr() if self.m else None)
+
+    def to_host(self):
+        """(fw_off, fw_adj, bw_off, bw_adj) as numpy uint64/uint32"""
+        return (self.fw_off.cpu().numpy().view(np.uint64), self.fw_adj.cpu().numpy().view(np.uint32),
+                self.bw_off.cpu().numpy().view(np.uint64), self.bw_adj.cpu().numpy().view(np.uint32))
+
+    def nbytes(self):
+        return 2 * 8 * (self.n + 1) + 2 * 4 * self.m
+
+
+def generate(kind: str, seed: int, device="cuda", ctx: Context | None = None, stream=None, **kw) -> BidiGraph:
+    """Seeded graph built on the GPU; the CPU oracle reproduces the same edge stream.
+
+    kind="gnm": n, draws; kind="rmat": scale, draws, a, b, c, permute;
+    kind="lattice": side, p_bidir.
+    """
+    ctx = ctx or Context.default(torch.device(device).index or 0)
+    p = _lib.GenParams()
+    p.seed = seed
+    if kind == "gnm":
+        p.kind, p.p0, p.p1 = _lib.GEN_GNM, kw["n"], kw["draws"]
+    elif kind == "rmat":
+        p.kind, p.p0, p.p1 = _lib.GEN_RMAT, kw["scale"], kw["draws"]
+        p.a, p.b, p.c = kw.get("a", 0.57), kw.get("b", 0.19), kw.get("c", 0.19)
+        p.permute = int(kw.get("permute", True))
+    elif kind == "lattice":
+        p.kind, p.p0, p.a = _lib.GEN_LATTICE, kw["side"], kw.get("p_bidir", 0.1)
+    else:
+        raise KspnError(-1, f"unknown generator {kind!r}")
+    n = C.c_uint32()
+    m = C.c_uint64()
+    ptrs = [C.c_void_p() for _ in range(4)]
+    _lib.check(_lib.lib().kspn_generate(ctx.ptr, C.byref(p), C.byref(n), C.byref(m), *(C.byref(q) for q in ptrs),
+                                        _stream_ptr(stream)), "kspn_generate")
+    return BidiGraph._adopt(n.value, m.value, ptrs, device)
+
+
+def _opts(method=METHOD_KASPAN, pivot=0, fwbw_rounds=1, residual_threshold=0, max_rounds=0, flags=0, alpha=14.0,
+          beta=24.0, mlp_params=None):
+    o = _lib.Opts()
+    _lib.lib().kspn_opts_default(C.byref(o))
+    o.method, o.pivot, o.fwbw_rounds = method, pivot, fwbw_rounds
+    o.residual_threshold, o.max_rounds, o.flags = residual_threshold, max_rounds, flags
+    o.alpha, o.beta = alpha, beta
+    o.mlp_params = mlp_params.data_ptr() if mlp_params is not None else None
+    return o
+
+
+def scc(graph: BidiGraph, scc_id: torch.Tensor | None = None, *, ctx: Context | None = None, stream=None,
+        profile=False, **opts) -> tuple[torch.Tensor, Stats]:
+    """kaspan::scc(graph, scc_id) (scc/scc.hpp:21-87) on the GPU.
+
+    Fills ``scc_id`` (int32 device tensor of n entries, allocated if None)
+    with canonical ids and returns ``(scc_id, stats)``. Keyword options map
+    to ``kspn_opts`` (method, pivot, fwbw_rounds, residual_threshold,
+    max_rounds, flags, alpha, beta, mlp_params).
+    """
+    ctx = ctx or Context.default(graph.device.index or 0)
+    if scc_id is None:
+        scc_id = torch.empty(graph.n, dtype=torch.int32, device=graph.device)
+    if scc_id.numel() != graph.n or scc_id.dtype != torch.int32 or not scc_id.is_contiguous():
+        raise KspnError(-1, "scc_id must be a contiguous int32 tensor with n entries")
+    o = _opts(**opts)
+    if profile:
+        o.flags |= _lib.FLAG_PROFILE
+    st = Stats()
+    g = graph.view()
+    rc = _lib.lib().kspn_scc(ctx.ptr, C.byref(g), C.c_void_p(scc_id.data_ptr()), C.byref(o), C.byref(st),
+                             _stream_ptr(stream))
+    _lib.check(rc, "kspn_scc")
+    return scc_id, st
+
+
+def scc_trim_ex(graph: BidiGraph, scc_id=None, **kw):
+    """scc_trim_ex (scc/variant/scc_trim_ex.hpp:16-100): exhaustive trims around FW-BW and colouring."""
+    return scc(graph, scc_id, method=METHOD_TRIM_EX, **kw)
+
+
+def scc_trim_ex_residual(graph: BidiGraph, scc_id=None, **kw):
+    """scc_trim_ex_residual at one rank (scc/variant/scc_trim_ex_residual.hpp:17-128): exhaustive
+    trim, then Tarjan on everything left (the one-rank decided threshold is 0, :47)."""
+    kw.setdefault("residual_threshold", 0xFFFFFFFF)
+    return scc(graph, scc_id, method=METHOD_TRIM_EX, **kw)
+
+
+def backward_complement_graph(n, fw_off: torch.Tensor, fw_adj: torch.Tensor, ctx: Context | None = None, stream=None):
+    """Reverse CSR, ascending sources per row (scc/backward_complement.hpp:22-68)."""
+    fw_off = _as_u64_tensor(fw_off, "cuda")
+    fw_adj = _as_u32_tensor(fw_adj, "cuda")
+    ctx = ctx or Context.default(fw_off.device.index or 0)
+    m = fw_adj.numel()
+    bw_off = torch.empty(n + 1, dtype=torch.int64, device=fw_off.device)
+    bw_adj = torch.empty(m, dtype=torch.int32, device=fw_off.device)
+    _lib.check(_lib.lib().kspn_transpose(ctx.ptr, n, m, C.c_void_p(fw_off.data_ptr()),
+                                         C.c_void_p(fw_adj.data_ptr() if m else None), C.c_void_p(bw_off.data_ptr()),
+                                         C.c_void_p(bw_adj.data_ptr() if m else None), _stream_ptr(stream)),
+               "kspn_transpose")
+    return bw_off, bw_adj
+
+
+def tarjan(n, off: torch.Tensor, adj: torch.Tensor, labels: torch.Tensor | None = None, ctx: Context | None = None,
+           stream=None):
+    """GPU Tarjan (scc/tarjan.hpp:20-136), one wavefront; labels = minimum member id."""
+    off = _as_u64_tensor(off, "cuda")
+    adj = _as_u32_tensor(adj, "cuda")
+    ctx = ctx or Context.default(off.device.index or 0)
+    if labels is None:
+        labels = torch.empty(n, dtype=torch.int32, device=off.device)
+    m = adj.numel()
+    _lib.check(_lib.lib().kspn_tarjan(ctx.ptr, n, m, C.c_void_p(off.data_ptr()), C.c_void_p(adj.data_ptr() if m else None),
+                                      C.c_void_p(labels.data_ptr()), _stream_ptr(stream)), "kspn_tarjan")
+    return labels
+
+
+def normalise_scc_id(labels: torch.Tensor, ctx: Context | None = None, stream=None):
+    """In place: each id becomes the smallest vertex carrying it (test/normalise_scc.hpp:10-25)."""
+    ctx = ctx or Context.default(labels.device.index or 0)
+    _lib.check(_lib.lib().kspn_canonicalize(ctx.ptr, labels.numel(), C.c_void_p(labels.data_ptr()), _stream_ptr(stream)),
+               "kspn_canonicalize")
+    return labels
+
+
+def allgather_fw_sub_graph(n, off: torch.Tensor, adj: torch.Tensor, mask: torch.Tensor, ctx: Context | None = None,
+                           stream=None):
+    """Induced subgraph on mask != 0 (scc/allgather_sub_graph.hpp:253-301 at one rank).
+
+    Returns (super_ids, sub_off, sub_adj) with sub ids ascending in global id.
+    """
+    off = _as_u64_tensor(off, "cuda")
+    adj = _as_u32_tensor(adj, "cuda")
+    ctx = ctx or Context.default(off.device.index or 0)
+    mask = mask.to(device=off.device, dtype=torch.uint8).contiguous()
+    m = adj.numel()
+    super_ids = torch.empty(max(n, 1), dtype=torch.int32, device=off.device)
+    sub_off = torch.empty(n + 1, dtype=torch.int64, device=off.device)
+    sub_adj = torch.empty(max(m, 1), dtype=torch.int32, device=off.device)
+    sn = C.c_uint32()
+    sm = C.c_uint64()
+    _lib.check(_lib.lib().kspn_subgraph(ctx.ptr, n, C.c_void_p(off.data_ptr()), C.c_void_p(adj.data_ptr() if m else None),
+                                        C.c_void_p(mask.data_ptr()), C.c_void_p(super_ids.data_ptr()),
+                                        C.c_void_p(sub_off.data_ptr()), C.c_void_p(sub_adj.data_ptr()), C.byref(sn),
+                                        C.byref(sm), _stream_ptr(stream)), "kspn_subgraph")
+    return super_ids[: sn.value], sub_off[: sn.value + 1], sub_adj[: sm.value]

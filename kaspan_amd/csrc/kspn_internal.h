@@ -1,0 +1,159 @@
+// kspn_internal.h -- device helpers and the context layout shared by the
+// libkspn translation units. gfx950 only: wave64, 256-thread workgroups.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/kspn/scc.h"
+
+namespace kspn {
+
+constexpr int WG = 256;                 // threads per workgroup (4 waves)
+constexpr int WAVE = 64;                // CDNA wavefront
+constexpr uint32_t UNDEC = KSPN_UNDECIDED;
+
+#define KSPN_TRY(expr)                 \
+  do {                                 \
+    int _rc = (expr);                  \
+    if (_rc != KSPN_OK) return _rc;    \
+  } while (0)
+
+#define KSPN_HIP(expr)                        \
+  do {                                        \
+    hipError_t _e = (expr);                   \
+    if (_e != hipSuccess) return KSPN_E_HIP;  \
+  } while (0)
+
+// ---------------------------------------------------------------- bitmaps
+// One bit per vertex in uint32 words; arrays are padded to whole 64-bit
+// groups so a wavefront can write its 64 ballot bits as two words.
+__host__ __device__ inline uint64_t bm_words(uint64_t n) { return ((n + 63) / 64) * 2; }
+
+__device__ inline bool bm_get(const uint32_t* bm, uint32_t v) { return (bm[v >> 5] >> (v & 31)) & 1u; }
+
+// set a bit; true for the one thread that turned it on
+__device__ inline bool bm_claim(uint32_t* bm, uint32_t v) {
+  uint32_t mask = 1u << (v & 31);
+  if (bm[v >> 5] & mask) return false;
+  return (atomicOr(&bm[v >> 5], mask) & mask) == 0;
+}
+
+// clear a bit; true for the one thread that turned it off
+__device__ inline bool bm_unclaim(uint32_t* bm, uint32_t v) {
+  uint32_t mask = 1u << (v & 31);
+  if (!(bm[v >> 5] & mask)) return false;
+  return (atomicAnd(&bm[v >> 5], ~mask) & mask) != 0;
+}
+
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+// Wavefront-aggregated append: every active lane calls it; lanes with
+// pred append val at consecutive slots after one atomicAdd per wave.
+__device__ inline void wave_append(bool pred, uint32_t val, uint32_t* queue, uint32_t* count) {
+  uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  uint32_t lane = lane_id();
+  uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  if (pred) {
+    uint64_t below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
+    queue[base + __popcll(below)] = val;
+  }
+}
+
+// wave-aggregated counter add
+__device__ inline void wave_count(bool pred, uint32_t* count) {
+  uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  uint32_t lane = lane_id();
+  uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+  if (lane == leader) atomicAdd(count, (uint32_t)__popcll(mask));
+}
+
+__device__ inline uint32_t rotr32(uint32_t x, uint32_t r) { r &= 31; return r ? (x >> r) | (x << (32 - r)) : x; }
+__device__ inline uint32_t rotl32(uint32_t x, uint32_t r) { r &= 31; return r ? (x << r) | (x >> (32 - r)) : x; }
+
+// ---------------------------------------------------------------- counters
+// Device-side counters, read back through a pinned mirror.
+enum Counter : int {
+  C_Q0 = 0,   // size of queue written by a launch (ping)
+  C_Q1,       // pong
+  C_DECIDED,  // decisions made by a launch
+  C_AUX,
+  C_TRIM2,
+  C_RES_N,
+  C_FLAG,
+  C_COUNT_ = 16
+};
+
+struct KernTimer {
+  const char* name;
+  uint32_t launches = 0;
+  double ms = 0, bytes = 0;
+};
+
+struct EventPair {
+  hipEvent_t a, b;
+  int kern;
+  double bytes;
+};
+
+// ---------------------------------------------------------------- context
+struct Ctx {
+  int device = 0;
+  uint32_t cap_n = 0;
+  uint64_t cap_m = 0;
+  // bitmaps (bm_words(n) words each)
+  uint32_t* live = nullptr;  // 1 = undecided
+  uint32_t* fwb = nullptr;   // forward blocked: decided or reached
+  uint32_t* bwb = nullptr;   // backward blocked
+  uint32_t* inq = nullptr;   // queue membership
+  // vertex arrays
+  uint32_t* qa = nullptr;
+  uint32_t* qb = nullptr;
+  uint32_t* qc = nullptr;
+  uint32_t* ulist = nullptr;  // live vertices, ascending
+  uint32_t* col = nullptr;    // colours; reused as min-id table
+  uint64_t* deg = nullptr;    // frontier degrees (n+1)
+  uint64_t* scan = nullptr;   // exclusive scan of deg (n+1)
+  uint64_t* part = nullptr;   // per-workgroup partials
+  uint32_t* cnt = nullptr;    // device counters
+  uint32_t* h_cnt = nullptr;  // pinned mirror
+  uint64_t* h_u64 = nullptr;  // pinned scratch for 64-bit reads
+  void* cub_tmp = nullptr;
+  size_t cub_bytes = 0;
+  // residual
+  uint64_t* sub_off = nullptr;
+  uint32_t* sub_adj = nullptr;
+  uint64_t cap_sub_m = 0;
+  uint32_t* tj = nullptr;  // Tarjan work arrays (5 * n words)
+  // profiling
+  bool profile = false;
+  std::vector<KernTimer> kt;
+  std::vector<EventPair> pending;
+  std::vector<hipEvent_t> free_events;
+  hipStream_t stream = nullptr;
+  int num_cu = 256;
+};
+
+// host helpers implemented in kspn_scc.hip
+int ctx_reserve(Ctx* c, uint32_t n, uint64_t m);
+int read_counters(Ctx* c, hipStream_t s, int count);
+int exclusive_scan_u64(Ctx* c, const uint64_t* in, uint64_t* out, uint64_t items, hipStream_t s);
+int exclusive_scan_u32(Ctx* c, const uint32_t* in, uint32_t* out, uint64_t items, hipStream_t s);
+int build_live_list(Ctx* c, uint32_t n, const uint32_t* bitmap, uint32_t* list, uint32_t* count_out, hipStream_t s);
+
+inline uint32_t grid_for(uint64_t items, uint32_t per_block = WG, uint32_t cap = 1u << 20) {
+  uint64_t g = (items + per_block - 1) / per_block;
+  if (g == 0) g = 1;
+  if (g > cap) g = cap;
+  return (uint32_t)g;
+}
+
+}  // namespace kspn

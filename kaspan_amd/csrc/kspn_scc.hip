@@ -1,0 +1,1672 @@
+// kspn_scc.hip -- the SCC pipeline on one MI355X (gfx950, wave64).
+//
+// Stage map (paths are in the rawbby/KaSpan tree):
+//   k_trim_first + k_pivot_reduce  trim_1_first, scc/trim_1_local.hpp:12-46
+//   fwbw()                         forward_backward_search, scc/pivot_search.hpp:15-85
+//                                  (push levels: k_expand; pull levels: k_pull_*)
+//   k_trim_sweep (NORMAL)          trim_1_normal, scc/trim_1_local.hpp:48-73
+//   k_trim_sweep (TRIM1) loop      trim_1_exhaustive[_first], scc/trim_1_exhaustive.hpp:14-131
+//   k_trim2_*                      trim-2 (no reference counterpart)
+//   color_round()                  cache_rot_label_search_pull, scc/cache_multi_pivot_search_pull.hpp:40-266
+//   residual()                     allgather_fw_sub_graph + tarjan, scc/variant/scc_trim_ex_residual.hpp:110-124
+//   canonicalize()                 normalise_scc_id, test/normalise_scc.hpp:10-25
+//
+// Device state lives in bitmaps (1 bit per vertex, L2-resident at 2 MiB per
+// 16.7M vertices) and uint32 vertex queues; frontier expansion is load
+// balanced over edges (exclusive scan of frontier degrees, 2048-edge tiles,
+// LDS-staged frontier window) so hub vertices split across workgroups.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <new>
+
+#include "kspn_internal.h"
+
+namespace kspn {
+
+// =====================================================================
+// generic kernels
+// =====================================================================
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t count, uint32_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i < count; i += (uint64_t)gridDim.x * WG) p[i] = v;
+}
+
+// words of `out` = op(a, b); op 0: ~a ; op 1: ~(a & b)
+__global__ void k_bm_not(uint32_t* out, const uint32_t* a, uint64_t words) {
+  for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i < words; i += (uint64_t)gridDim.x * WG) out[i] = ~a[i];
+}
+__global__ void k_bm_nand(uint32_t* out, const uint32_t* a, const uint32_t* b, uint64_t words) {
+  for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i < words; i += (uint64_t)gridDim.x * WG) out[i] = ~(a[i] & b[i]);
+}
+
+// *p = v (small device writes without pageable-memory copies)
+__global__ void k_set_u32(uint32_t* p, uint32_t v) {
+  if (threadIdx.x == 0) *p = v;
+}
+__global__ void k_copy_u32(uint32_t* dst, const uint32_t* src) {
+  if (threadIdx.x == 0) *dst = *src;
+}
+
+// set bit v of bm
+__global__ void k_bm_set_one(uint32_t* bm, uint32_t v) {
+  if (threadIdx.x == 0) atomicOr(&bm[v >> 5], 1u << (v & 31));
+}
+
+// clear queue members' bits
+__global__ void k_bm_clear_list(uint32_t* bm, const uint32_t* list, const uint32_t* count_ptr) {
+  uint32_t cnt = *count_ptr;
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < cnt; i += gridDim.x * WG) {
+    uint32_t v = list[i];
+    atomicAnd(&bm[v >> 5], ~(1u << (v & 31)));
+  }
+}
+
+// popcount per 32-bit word; entry `words` is 0 so the scan's last entry is the total
+__global__ void k_bm_popc(const uint32_t* bm, uint64_t words, uint32_t* pc) {
+  for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i <= words; i += (uint64_t)gridDim.x * WG)
+    pc[i] = (i < words) ? __popc(bm[i]) : 0u;
+}
+
+// write set-bit positions ascending, word i starting at pos[i]
+__global__ void k_bm_scatter(const uint32_t* bm, uint64_t words, const uint32_t* pos, uint32_t* list) {
+  for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i < words; i += (uint64_t)gridDim.x * WG) {
+    uint32_t w = bm[i];
+    uint32_t p = pos[i];
+    while (w) {
+      uint32_t b = __ffs(w) - 1;
+      list[p++] = (uint32_t)(i * 32 + b);
+      w &= w - 1;
+    }
+  }
+}
+
+// degree of each frontier vertex; deg[f] = 0 so the scan yields the total
+__global__ void k_frontier_deg(const uint32_t* __restrict__ F, const uint32_t* fcount, const uint64_t* __restrict__ off,
+                               uint64_t* __restrict__ deg) {
+  uint32_t f = *fcount;
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i <= f; i += gridDim.x * WG) {
+    if (i < f) {
+      uint32_t v = F[i];
+      deg[i] = off[v + 1] - off[v];
+    } else {
+      deg[i] = 0;
+    }
+  }
+}
+
+// =====================================================================
+// edge-balanced frontier expansion
+// =====================================================================
+
+enum ExpandOp : int {
+  OP_FW = 0,        // claim fwb bit of the target (forward reachability)
+  OP_BW = 1,        // claim bwb bit of the target, label it with the pivot
+  OP_NEIGH = 2,     // collect live targets once (trim candidates)
+  OP_COLOR = 3,     // push colour max into live targets
+  OP_COLOR_BW = 4,  // backward closure: decide live in-neighbours of the same colour
+};
+
+struct OpArgs {
+  uint32_t* live;
+  uint32_t* fwb;
+  uint32_t* bwb;
+  uint32_t* inq;
+  uint32_t* col;
+  uint32_t* labels;
+  uint32_t value;  // pivot or rotation
+};
+
+template <int OP>
+__device__ inline void expand_apply(uint32_t u, uint32_t w, const OpArgs& a, uint32_t* next, uint32_t* next_cnt) {
+  bool push = false;
+  if constexpr (OP == OP_FW) {
+    push = bm_claim(a.fwb, w);
+  } else if constexpr (OP == OP_BW) {
+    push = bm_claim(a.bwb, w);
+    if (push) a.labels[w] = a.value;
+  } else if constexpr (OP == OP_NEIGH) {
+    push = bm_get(a.live, w) && bm_claim(a.inq, w);
+  } else if constexpr (OP == OP_COLOR) {
+    uint32_t c = a.col[u];
+    uint32_t cw = a.col[w];
+    if (cw != 0u && cw < c) {
+      uint32_t old = atomicMax(&a.col[w], c);
+      push = (old < c) && bm_claim(a.inq, w);
+    }
+  } else if constexpr (OP == OP_COLOR_BW) {
+    uint32_t c = a.col[u];
+    if (a.col[w] == c && bm_unclaim(a.live, w)) {
+      a.labels[w] = rotl32(~c, a.value);
+      push = true;
+    }
+  }
+  wave_append(push, w, next, next_cnt);
+}
+
+constexpr int EXP_TILE = WG * 8;  // edges per workgroup tile
+constexpr int EXP_WIN = 1024;     // frontier entries staged in LDS per tile
+
+// Each workgroup takes 2048-edge tiles of the concatenated adjacency of the
+// frontier. Thread 0 finds the frontier window covering the tile (two
+// binary searches over the scanned degrees), the window (vertex, row start,
+// scan value) is staged in LDS, and each thread maps its edges to their
+// source by a forward-only LDS search. Consecutive threads read
+// consecutive adjacency entries.
+template <int OP>
+__global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, const uint32_t* fcount,
+                                               const uint64_t* __restrict__ S, const uint64_t* __restrict__ off,
+                                               const uint32_t* __restrict__ adj, OpArgs a, uint32_t* __restrict__ next,
+                                               uint32_t* next_cnt) {
+  __shared__ uint64_t s_S[EXP_WIN];
+  __shared__ uint64_t s_beg[EXP_WIN];
+  __shared__ uint32_t s_v[EXP_WIN];
+  __shared__ uint32_t s_i0, s_w;
+  const uint32_t f = *fcount;
+  if (f == 0) return;
+  const uint64_t total = S[f];
+  const uint64_t ntiles = (total + EXP_TILE - 1) / EXP_TILE;
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint64_t e0 = t * EXP_TILE;
+    const uint64_t e1 = min(e0 + (uint64_t)EXP_TILE, total);
+    if (threadIdx.x == 0) {
+      uint32_t lo = 0, hi = f;
+      while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (S[mid] <= e0) lo = mid; else hi = mid;
+      }
+      uint32_t i0 = lo;
+      hi = f;
+      while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (S[mid] <= e1 - 1) lo = mid; else hi = mid;
+      }
+      s_i0 = i0;
+      s_w = lo - i0 + 1;
+    }
+    __syncthreads();
+    const uint32_t i0 = s_i0, w = s_w;
+    if (w <= (uint32_t)EXP_WIN) {
+      for (uint32_t j = threadIdx.x; j < w; j += WG) {
+        uint32_t v = F[i0 + j];
+        s_v[j] = v;
+        s_S[j] = S[i0 + j];
+        s_beg[j] = off[v];
+      }
+      __syncthreads();
+      uint32_t lo = 0;
+      for (uint64_t e = e0 + threadIdx.x; e < e1; e += WG) {
+        uint32_t hi = w;
+        while (hi - lo > 1) {
+          uint32_t mid = (lo + hi) >> 1;
+          if (s_S[mid] <= e) lo = mid; else hi = mid;
+        }
+        uint32_t u = s_v[lo];
+        uint32_t x = adj[s_beg[lo] + (e - s_S[lo])];
+        expand_apply<OP>(u, x, a, next, next_cnt);
+      }
+    } else {
+      // window larger than LDS (many zero-degree frontier entries): search the scan in global memory
+      uint32_t lo = i0;
+      for (uint64_t e = e0 + threadIdx.x; e < e1; e += WG) {
+        uint32_t hi = i0 + w;
+        while (hi - lo > 1) {
+          uint32_t mid = (lo + hi) >> 1;
+          if (S[mid] <= e) lo = mid; else hi = mid;
+        }
+        uint32_t u = F[lo];
+        uint32_t x = adj[off[u] + (e - S[lo])];
+        expand_apply<OP>(u, x, a, next, next_cnt);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// =====================================================================
+// trim-1 first pass + degree-product pivot
+// =====================================================================
+
+// One pass over all vertices: in- or out-degree 0 -> decided with its own
+// id; the rest form the live bitmap (written as two words per wavefront
+// ballot). Each workgroup reduces its (max in*out, min id) pair.
+__global__ __launch_bounds__(WG) void k_trim_first(uint32_t n, const uint64_t* __restrict__ fw_off,
+                                                   const uint64_t* __restrict__ bw_off, uint32_t* __restrict__ live,
+                                                   uint32_t* __restrict__ labels, uint64_t* __restrict__ part,
+                                                   uint32_t* cnt) {
+  __shared__ uint64_t s_p[WG];
+  __shared__ uint32_t s_id[WG];
+  uint64_t best = 0;
+  uint32_t best_id = 0xFFFFFFFFu;
+  const uint64_t n_pad = ((uint64_t)n + WG - 1) / WG * WG;
+  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < n_pad; base += (uint64_t)gridDim.x * WG) {
+    uint64_t v64 = base + threadIdx.x;
+    bool is_live = false;
+    if (v64 < n) {
+      uint32_t v = (uint32_t)v64;
+      uint64_t out = fw_off[v + 1] - fw_off[v];
+      uint64_t in = bw_off[v + 1] - bw_off[v];
+      is_live = in != 0 && out != 0;
+      labels[v] = is_live ? UNDEC : v;
+      if (is_live) {
+        uint64_t p = in * out;
+        if (p > best || (p == best && v < best_id)) { best = p; best_id = v; }
+      }
+    }
+    uint64_t mask = __ballot(is_live);
+    uint32_t lane = lane_id();
+    uint64_t word0 = (base + threadIdx.x - lane) >> 5;
+    if (lane == 0) live[word0] = (uint32_t)mask;
+    if (lane == 32) live[word0 + 1] = (uint32_t)(mask >> 32);
+    wave_count(v64 < n && !is_live, &cnt[C_DECIDED]);
+  }
+  s_p[threadIdx.x] = best;
+  s_id[threadIdx.x] = best_id;
+  __syncthreads();
+  for (int s = WG / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      uint64_t p2 = s_p[threadIdx.x + s];
+      uint32_t i2 = s_id[threadIdx.x + s];
+      if (p2 > s_p[threadIdx.x] || (p2 == s_p[threadIdx.x] && i2 < s_id[threadIdx.x])) {
+        s_p[threadIdx.x] = p2;
+        s_id[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s_p[0];
+    part[2 * blockIdx.x + 1] = s_id[0];
+  }
+}
+
+// (max product, smallest id) over workgroup partials; id 0xFFFFFFFF if none
+__global__ __launch_bounds__(WG) void k_pivot_reduce(const uint64_t* part, uint32_t nparts, uint32_t* out) {
+  __shared__ uint64_t s_p[WG];
+  __shared__ uint32_t s_id[WG];
+  uint64_t best = 0;
+  uint32_t best_id = 0xFFFFFFFFu;
+  for (uint32_t i = threadIdx.x; i < nparts; i += WG) {
+    uint64_t p = part[2 * i];
+    uint32_t id = (uint32_t)part[2 * i + 1];
+    if (id != 0xFFFFFFFFu && (p > best || (p == best && id < best_id))) { best = p; best_id = id; }
+  }
+  s_p[threadIdx.x] = best;
+  s_id[threadIdx.x] = best_id;
+  __syncthreads();
+  for (int s = WG / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      uint64_t p2 = s_p[threadIdx.x + s];
+      uint32_t i2 = s_id[threadIdx.x + s];
+      if (i2 != 0xFFFFFFFFu && (s_id[threadIdx.x] == 0xFFFFFFFFu || p2 > s_p[threadIdx.x] ||
+                                (p2 == s_p[threadIdx.x] && i2 < s_id[threadIdx.x]))) {
+        s_p[threadIdx.x] = p2;
+        s_id[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s_id[0];
+}
+
+// =====================================================================
+// pull (bottom-up) reachability levels
+// =====================================================================
+
+// forward: a live unreached v is reached if any in-neighbour is reached
+// (reached = fwb & live); claims are visible within the sweep
+__global__ __launch_bounds__(WG) void k_pull_fw(uint32_t n, const uint32_t* __restrict__ live, uint32_t* fwb,
+                                                const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
+                                                uint32_t* __restrict__ next, uint32_t* next_cnt) {
+  const uint64_t n_pad = ((uint64_t)n + WG - 1) / WG * WG;
+  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < n_pad; base += (uint64_t)gridDim.x * WG) {
+    uint64_t v64 = base + threadIdx.x;
+    bool hit = false;
+    if (v64 < n) {
+      uint32_t v = (uint32_t)v64;
+      if (!bm_get(fwb, v)) {
+        for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it) {
+          uint32_t u = bw_adj[it];
+          if (bm_get(fwb, u) && bm_get(live, u)) { hit = true; break; }
+        }
+        if (hit) hit = bm_claim(fwb, v);
+      }
+    }
+    wave_append(hit, (uint32_t)v64, next, next_cnt);
+  }
+}
+
+// backward: a reached, not yet backward-visited v (bwb bit 0) is visited if
+// any out-neighbour is visited (bwb & fwb & live); it joins the pivot's SCC
+__global__ __launch_bounds__(WG) void k_pull_bw(uint32_t n, const uint32_t* __restrict__ live,
+                                                const uint32_t* __restrict__ fwb, uint32_t* bwb,
+                                                const uint64_t* __restrict__ fw_off, const uint32_t* __restrict__ fw_adj,
+                                                uint32_t* __restrict__ labels, uint32_t pivot,
+                                                uint32_t* __restrict__ next, uint32_t* next_cnt) {
+  const uint64_t n_pad = ((uint64_t)n + WG - 1) / WG * WG;
+  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < n_pad; base += (uint64_t)gridDim.x * WG) {
+    uint64_t v64 = base + threadIdx.x;
+    bool hit = false;
+    if (v64 < n) {
+      uint32_t v = (uint32_t)v64;
+      if (!bm_get(bwb, v)) {
+        for (uint64_t it = fw_off[v], end = fw_off[v + 1]; it < end; ++it) {
+          uint32_t w = fw_adj[it];
+          if (bm_get(bwb, w) && bm_get(fwb, w) && bm_get(live, w)) { hit = true; break; }
+        }
+        if (hit) {
+          hit = bm_claim(bwb, v);
+          if (hit) labels[v] = pivot;
+        }
+      }
+    }
+    wave_append(hit, (uint32_t)v64, next, next_cnt);
+  }
+}
+
+// after FW-BW: scc = bwb & fwb & live; live &= ~scc; count
+__global__ void k_fwbw_finish(uint32_t* live, const uint32_t* fwb, const uint32_t* bwb, uint64_t words, uint32_t* cnt) {
+  uint32_t local = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i < words; i += (uint64_t)gridDim.x * WG) {
+    uint32_t l = live[i];
+    uint32_t scc = bwb[i] & fwb[i] & l;
+    live[i] = l & ~scc;
+    local += __popc(scc);
+  }
+  if (local) atomicAdd(&cnt[C_DECIDED], local);
+}
+
+// =====================================================================
+// trimming
+// =====================================================================
+
+enum TrimMode : int { TRIM_ONE = 0, TRIM_NORMAL = 1 };
+
+// TRIM_ONE: decide v when it has no live in-neighbour or no live
+// out-neighbour (trim_1_exhaustive's degree-0 test). TRIM_NORMAL: decide v
+// when it has neither (trim_1_normal's has_degree test). Decided vertices
+// get their own id and go to `out`. Members of Q first drop their inq bit.
+template <int MODE>
+__global__ __launch_bounds__(WG) void k_trim_sweep(const uint32_t* __restrict__ Q, const uint32_t* qcount,
+                                                   uint32_t* live, uint32_t* inq, const uint64_t* __restrict__ fw_off,
+                                                   const uint32_t* __restrict__ fw_adj, const uint64_t* __restrict__ bw_off,
+                                                   const uint32_t* __restrict__ bw_adj, uint32_t* __restrict__ labels,
+                                                   uint32_t* __restrict__ out, uint32_t* out_cnt) {
+  const uint32_t q = *qcount;
+  const uint32_t q_pad = (q + WG - 1) / WG * WG;
+  for (uint32_t base = blockIdx.x * WG; base < q_pad; base += gridDim.x * WG) {
+    uint32_t i = base + threadIdx.x;
+    bool dec = false;
+    uint32_t v = 0;
+    if (i < q) {
+      v = Q[i];
+      if (inq) atomicAnd(&inq[v >> 5], ~(1u << (v & 31)));
+      if (bm_get(live, v)) {
+        bool has_in = false, has_out = false;
+        for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it)
+          if (bm_get(live, bw_adj[it])) { has_in = true; break; }
+        if (MODE == TRIM_NORMAL ? !has_in : has_in) {
+          for (uint64_t it = fw_off[v], end = fw_off[v + 1]; it < end; ++it)
+            if (bm_get(live, fw_adj[it])) { has_out = true; break; }
+        }
+        bool trim = (MODE == TRIM_NORMAL) ? (!has_in && !has_out) : (!has_in || !has_out);
+        if (trim && bm_unclaim(live, v)) {
+          labels[v] = v;
+          dec = true;
+        }
+      }
+    }
+    wave_append(dec, v, out, out_cnt);
+  }
+}
+
+// live neighbour (excluding v) count in one direction, up to 2; *only = the single one
+__device__ inline int live_nbrs_upto2(uint32_t v, const uint64_t* off, const uint32_t* adj, const uint32_t* live,
+                                      uint32_t* only) {
+  int c = 0;
+  uint32_t first = 0xFFFFFFFFu;
+  for (uint64_t it = off[v], end = off[v + 1]; it < end; ++it) {
+    uint32_t w = adj[it];
+    if (w == v || w == first) continue;
+    if (bm_get(live, w)) {
+      if (c == 0) { first = w; c = 1; }
+      else return 2;
+    }
+  }
+  *only = first;
+  return c;
+}
+
+// Trim-2: {v,u} is an SCC when u is v's only live neighbour and v is u's
+// only live neighbour in one direction (in-edges for DIR 0, out-edges for
+// DIR 1) -- nothing else live can enter (leave) the pair, and the edges
+// between them make it strongly connected. The pair is claimed through the
+// smaller id; one direction per launch keeps concurrent claims consistent.
+template <int DIR>
+__global__ __launch_bounds__(WG) void k_trim2(const uint32_t* __restrict__ Q, const uint32_t* qcount, uint32_t* live,
+                                              const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                                              uint32_t* __restrict__ labels, uint32_t* __restrict__ out,
+                                              uint32_t* out_cnt) {
+  const uint32_t q = *qcount;
+  const uint32_t q_pad = (q + WG - 1) / WG * WG;
+  for (uint32_t base = blockIdx.x * WG; base < q_pad; base += gridDim.x * WG) {
+    uint32_t i = base + threadIdx.x;
+    bool dec = false;
+    uint32_t lo = 0, hi = 0;
+    if (i < q) {
+      uint32_t v = Q[i];
+      uint32_t u = 0, back = 0;
+      if (bm_get(live, v) && live_nbrs_upto2(v, off, adj, live, &u) == 1 &&
+          live_nbrs_upto2(u, off, adj, live, &back) == 1 && back == v) {
+        lo = min(u, v);
+        hi = max(u, v);
+        if (v == lo && bm_unclaim(live, lo)) {
+          if (bm_unclaim(live, hi)) {
+            labels[lo] = lo;
+            labels[hi] = lo;
+            dec = true;
+          } else {
+            // cannot happen when one direction runs per launch; restore
+            atomicOr(&live[lo >> 5], 1u << (lo & 31));
+          }
+        }
+      }
+    }
+    wave_append(dec, lo, out, out_cnt);
+    wave_append(dec, hi, out, out_cnt);
+  }
+}
+
+// =====================================================================
+// colour propagation
+// =====================================================================
+
+// key of v in round r: ~rotr(v, r). Taking the max of keys selects the
+// same root as the reference's min of rotr(v, r); 0 marks decided vertices.
+__device__ inline uint32_t color_key(uint32_t v, uint32_t r) { return ~rotr32(v, r); }
+
+__global__ void k_color_init_all(uint32_t n, const uint32_t* __restrict__ live, uint32_t* __restrict__ col, uint32_t r) {
+  for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG)
+    col[v] = bm_get(live, (uint32_t)v) ? color_key((uint32_t)v, r) : 0u;
+}
+
+// re-key the previous round's live list: still live -> new key, decided -> 0
+__global__ void k_color_init_list(const uint32_t* __restrict__ U, const uint32_t* ucount, const uint32_t* __restrict__ live,
+                                  uint32_t* __restrict__ col, uint32_t r) {
+  uint32_t u = *ucount;
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < u; i += gridDim.x * WG) {
+    uint32_t v = U[i];
+    col[v] = bm_get(live, v) ? color_key(v, r) : 0u;
+  }
+}
+
+// pull sweep: v takes the max colour over in-neighbours (decided ones hold
+// 0); only v's thread writes col[v]; changed vertices go to `out`
+__global__ __launch_bounds__(WG) void k_color_pull(const uint32_t* __restrict__ U, const uint32_t* ucount,
+                                                   const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
+                                                   uint32_t* col, uint32_t* __restrict__ out, uint32_t* out_cnt) {
+  const uint32_t u = *ucount;
+  const uint32_t u_pad = (u + WG - 1) / WG * WG;
+  for (uint32_t base = blockIdx.x * WG; base < u_pad; base += gridDim.x * WG) {
+    uint32_t i = base + threadIdx.x;
+    bool changed = false;
+    uint32_t v = 0;
+    if (i < u) {
+      v = U[i];
+      uint32_t c0 = col[v];
+      uint32_t c = c0;
+      for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it) c = max(c, col[bw_adj[it]]);
+      if (c > c0 && c0 != 0u) {
+        atomicMax(&col[v], c);
+        changed = true;
+      }
+    }
+    wave_append(changed, v, out, out_cnt);
+  }
+}
+
+// roots: live v whose colour is its own key; decided with id v
+__global__ __launch_bounds__(WG) void k_color_roots(const uint32_t* __restrict__ U, const uint32_t* ucount, uint32_t* live,
+                                                    const uint32_t* __restrict__ col, uint32_t* __restrict__ labels,
+                                                    uint32_t r, uint32_t* __restrict__ out, uint32_t* out_cnt) {
+  const uint32_t u = *ucount;
+  const uint32_t u_pad = (u + WG - 1) / WG * WG;
+  for (uint32_t base = blockIdx.x * WG; base < u_pad; base += gridDim.x * WG) {
+    uint32_t i = base + threadIdx.x;
+    bool root = false;
+    uint32_t v = 0;
+    if (i < u) {
+      v = U[i];
+      if (col[v] == color_key(v, r) && bm_unclaim(live, v)) {
+        labels[v] = v;
+        root = true;
+      }
+    }
+    wave_append(root, v, out, out_cnt);
+  }
+}
+
+// =====================================================================
+// canonical labels
+// =====================================================================
+
+// minid[rep] = min { v : labels[v] == rep }, wavefront-aggregated when all
+// lanes share a representative (the common case inside large SCCs)
+__global__ __launch_bounds__(WG) void k_canon_min(uint32_t n, const uint32_t* __restrict__ labels, uint32_t* minid) {
+  const uint64_t n_pad = ((uint64_t)n + WG - 1) / WG * WG;
+  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < n_pad; base += (uint64_t)gridDim.x * WG) {
+    uint64_t v64 = base + threadIdx.x;
+    bool valid = v64 < n;
+    uint32_t rep = valid ? labels[v64] : 0xFFFFFFFFu;
+    if (rep >= n) valid = false;  // not a vertex id: left as is
+    uint32_t v = valid ? (uint32_t)v64 : 0xFFFFFFFFu;
+    uint32_t rep0 = __shfl(rep, 0);
+    bool same = __all(!valid || rep == rep0);
+    if (same) {
+      // min over lanes of v (lane 0 holds the smallest valid v of the wave)
+      if (lane_id() == 0 && valid && rep0 != v) atomicMin(&minid[rep0], v);
+    } else if (valid && rep != v) {
+      atomicMin(&minid[rep], v);
+    }
+  }
+}
+
+// labels[v] <- min(minid[rep], rep)
+__global__ void k_canon_apply(uint32_t n, uint32_t* __restrict__ labels, const uint32_t* __restrict__ minid) {
+  for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG) {
+    uint32_t rep = labels[v];
+    if (rep >= n) continue;
+    uint32_t m = minid[rep];
+    labels[v] = m < rep ? m : rep;
+  }
+}
+
+// =====================================================================
+// residual: induced subgraph + Tarjan on one wavefront
+// =====================================================================
+
+__global__ void k_sub_ids(const uint32_t* __restrict__ U, const uint32_t* ucount, uint32_t* __restrict__ subid) {
+  uint32_t u = *ucount;
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < u; i += gridDim.x * WG) subid[U[i]] = i;
+}
+
+// out-degree inside the subgraph (mask given as a bitmap or a byte array)
+__global__ void k_sub_deg(const uint32_t* __restrict__ U, const uint32_t* ucount, const uint64_t* __restrict__ off,
+                          const uint32_t* __restrict__ adj, const uint32_t* __restrict__ live_bm,
+                          const uint8_t* __restrict__ mask8, uint64_t* __restrict__ deg) {
+  uint32_t u = *ucount;
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i <= u; i += gridDim.x * WG) {
+    if (i == u) { deg[i] = 0; continue; }
+    uint32_t v = U[i];
+    uint64_t d = 0;
+    for (uint64_t it = off[v], end = off[v + 1]; it < end; ++it) {
+      uint32_t w = adj[it];
+      d += live_bm ? bm_get(live_bm, w) : (mask8[w] != 0);
+    }
+    deg[i] = d;
+  }
+}
+
+__global__ void k_sub_fill(const uint32_t* __restrict__ U, const uint32_t* ucount, const uint64_t* __restrict__ off,
+                           const uint32_t* __restrict__ adj, const uint32_t* __restrict__ live_bm,
+                           const uint8_t* __restrict__ mask8, const uint32_t* __restrict__ subid,
+                           const uint64_t* __restrict__ sub_off, uint32_t* __restrict__ sub_adj) {
+  uint32_t u = *ucount;
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < u; i += gridDim.x * WG) {
+    uint32_t v = U[i];
+    uint64_t p = sub_off[i];
+    for (uint64_t it = off[v], end = off[v + 1]; it < end; ++it) {
+      uint32_t w = adj[it];
+      bool in = live_bm ? bm_get(live_bm, w) : (mask8[w] != 0);
+      if (in) sub_adj[p++] = subid[w];
+    }
+  }
+}
+
+// Iterative Tarjan (scc/tarjan.hpp:20-122) by lane 0 of one wavefront over
+// a device CSR; work = 6n words: index, low, stack, frame vertex, frame
+// iterator (u64). labels = minimum member id (callers take min_element).
+__global__ void k_tarjan(uint32_t n, const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                         uint32_t* __restrict__ work, uint32_t* __restrict__ labels) {
+  if (threadIdx.x != 0) return;
+  uint32_t* index = work;
+  uint32_t* low = work + n;
+  uint32_t* st = work + 2 * (uint64_t)n;
+  uint32_t* fu = work + 3 * (uint64_t)n;
+  uint64_t* fit = reinterpret_cast<uint64_t*>(work + 4 * (uint64_t)n);
+  const uint32_t ON = 0x80000000u;  // low[] high bit: vertex is on the stack
+  for (uint32_t i = 0; i < n; ++i) index[i] = 0xFFFFFFFFu;
+  uint32_t count = 0, sp = 0, fp = 0;
+  for (uint32_t root = 0; root < n; ++root) {
+    if (index[root] != 0xFFFFFFFFu) continue;
+    index[root] = count;
+    low[root] = count | ON;
+    ++count;
+    st[sp++] = root;
+    fu[fp] = root;
+    fit[fp] = off[root];
+    ++fp;
+    while (fp) {
+      uint32_t u = fu[fp - 1];
+      uint64_t it = fit[fp - 1];
+      if (it < off[u + 1]) {
+        uint32_t v = adj[it];
+        fit[fp - 1] = it + 1;
+        if (index[v] == 0xFFFFFFFFu) {
+          index[v] = count;
+          low[v] = count | ON;
+          ++count;
+          st[sp++] = v;
+          fu[fp] = v;
+          fit[fp] = off[v];
+          ++fp;
+        } else if ((low[v] & ON) && index[v] < (low[u] & ~ON)) {
+          low[u] = index[v] | ON;
+        }
+        continue;
+      }
+      uint32_t lu = low[u] & ~ON;
+      if (lu == index[u]) {
+        uint32_t end = sp, beg;
+        do {
+          --sp;
+          low[st[sp]] &= ~ON;
+        } while (st[sp] != u);
+        beg = sp;
+        uint32_t mn = 0xFFFFFFFFu;
+        for (uint32_t i = beg; i < end; ++i) mn = min(mn, st[i]);
+        for (uint32_t i = beg; i < end; ++i) labels[st[i]] = mn;
+      }
+      --fp;
+      if (fp) {
+        uint32_t p = fu[fp - 1];
+        uint32_t lp = low[p];
+        if (lu < (lp & ~ON)) low[p] = lu | (lp & ON);
+      }
+    }
+  }
+}
+
+// labels[U[i]] = U[sub_label[i]]
+__global__ void k_res_apply(const uint32_t* __restrict__ U, const uint32_t* ucount, const uint32_t* __restrict__ sublab,
+                            uint32_t* __restrict__ labels) {
+  uint32_t u = *ucount;
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < u; i += gridDim.x * WG) labels[U[i]] = U[sublab[i]];
+}
+
+// pivot among live vertices: max original in*out, smallest id on ties
+// (select_pivot_from_head, scc/pivot.hpp:28-47, restricted to one rank)
+__global__ __launch_bounds__(WG) void k_pivot_live(const uint32_t* __restrict__ U, const uint32_t* ucount,
+                                                   const uint64_t* __restrict__ fw_off, const uint64_t* __restrict__ bw_off,
+                                                   uint64_t* __restrict__ part) {
+  __shared__ uint64_t s_p[WG];
+  __shared__ uint32_t s_id[WG];
+  uint64_t best = 0;
+  uint32_t best_id = 0xFFFFFFFFu;
+  uint32_t u = *ucount;
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < u; i += gridDim.x * WG) {
+    uint32_t v = U[i];
+    uint64_t p = (fw_off[v + 1] - fw_off[v]) * (bw_off[v + 1] - bw_off[v]);
+    if (best_id == 0xFFFFFFFFu || p > best || (p == best && v < best_id)) { best = p; best_id = v; }
+  }
+  s_p[threadIdx.x] = best;
+  s_id[threadIdx.x] = best_id;
+  __syncthreads();
+  for (int s = WG / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      uint64_t p2 = s_p[threadIdx.x + s];
+      uint32_t i2 = s_id[threadIdx.x + s];
+      if (i2 != 0xFFFFFFFFu && (s_id[threadIdx.x] == 0xFFFFFFFFu || p2 > s_p[threadIdx.x] ||
+                                (p2 == s_p[threadIdx.x] && i2 < s_id[threadIdx.x]))) {
+        s_p[threadIdx.x] = p2;
+        s_id[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s_p[0];
+    part[2 * blockIdx.x + 1] = s_id[0];
+  }
+}
+
+__global__ void k_mask_to_bitmap(uint32_t n, const uint8_t* mask, uint32_t* bm) {
+  const uint64_t n_pad = ((uint64_t)n + WG - 1) / WG * WG;
+  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < n_pad; base += (uint64_t)gridDim.x * WG) {
+    uint64_t v = base + threadIdx.x;
+    bool in = v < n && mask[v] != 0;
+    uint64_t bits = __ballot(in);
+    uint32_t lane = lane_id();
+    uint64_t word0 = (base + threadIdx.x - lane) >> 5;
+    if (lane == 0) bm[word0] = (uint32_t)bits;
+    if (lane == 32) bm[word0 + 1] = (uint32_t)(bits >> 32);
+  }
+}
+
+// =====================================================================
+// host side
+// =====================================================================
+
+namespace {
+
+template <class T>
+int dalloc(T** p, uint64_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (count == 0) count = 1;
+  if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    return KSPN_E_OOM;
+  }
+  return KSPN_OK;
+}
+
+enum KernId : int {
+  K_TRIM_FIRST = 0, K_PIVOT, K_BITMAP, K_FRONTIER_DEG, K_SCAN, K_EXPAND_FW, K_EXPAND_BW, K_PULL_FW, K_PULL_BW,
+  K_TRIM_SWEEP, K_TRIM2, K_EXPAND_NEIGH, K_COLOR_INIT, K_COLOR_PULL, K_EXPAND_COLOR, K_COLOR_ROOTS,
+  K_EXPAND_COLOR_BW, K_LIVE_LIST, K_CANON, K_RESIDUAL, K_TARJAN, K_COUNT
+};
+const char* kKernNames[K_COUNT] = {"trim_first", "pivot_reduce", "bitmap_ops", "frontier_deg", "scan", "expand_fw",
+                                   "expand_bw", "pull_fw", "pull_bw", "trim_sweep", "trim2", "expand_neigh",
+                                   "color_init", "color_pull", "expand_color", "color_roots", "expand_color_bw",
+                                   "live_list", "canonicalize", "residual_build", "tarjan"};
+
+struct Prof {
+  Ctx* c;
+  hipStream_t s;
+  int idx = -1;
+  Prof(Ctx* c_, hipStream_t s_, int kern, double bytes) : c(c_), s(s_) {
+    if (!c->profile) return;
+    EventPair ep{};
+    if (c->free_events.size() >= 2) {
+      ep.a = c->free_events.back();
+      c->free_events.pop_back();
+      ep.b = c->free_events.back();
+      c->free_events.pop_back();
+    } else {
+      if (hipEventCreate(&ep.a) != hipSuccess || hipEventCreate(&ep.b) != hipSuccess) return;
+    }
+    ep.kern = kern;
+    ep.bytes = bytes;
+    (void)hipEventRecord(ep.a, s);
+    c->pending.push_back(ep);
+    idx = (int)c->pending.size() - 1;
+  }
+  void add_bytes(double b) {
+    if (idx >= 0) c->pending[idx].bytes += b;
+  }
+  ~Prof() {
+    if (idx >= 0) (void)hipEventRecord(c->pending[idx].b, s);
+  }
+};
+
+void prof_collect(Ctx* c) {
+  for (auto& ep : c->pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ep.a, ep.b) == hipSuccess) {
+      c->kt[ep.kern].launches++;
+      c->kt[ep.kern].ms += ms;
+      c->kt[ep.kern].bytes += ep.bytes;
+    }
+    c->free_events.push_back(ep.a);
+    c->free_events.push_back(ep.b);
+  }
+  c->pending.clear();
+}
+
+inline uint32_t expand_grid(Ctx* c, uint64_t total) {
+  uint64_t tiles = (total + EXP_TILE - 1) / EXP_TILE;
+  uint64_t cap = (uint64_t)c->num_cu * 16;
+  if (tiles < 1) tiles = 1;
+  return (uint32_t)std::min<uint64_t>(tiles, cap);
+}
+
+inline uint32_t stream_grid(Ctx* c, uint64_t items) {
+  uint64_t g = (items + WG - 1) / WG;
+  uint64_t cap = (uint64_t)c->num_cu * 16;
+  if (g < 1) g = 1;
+  return (uint32_t)std::min<uint64_t>(g, cap);
+}
+
+}  // namespace
+
+int ctx_reserve(Ctx* c, uint32_t n, uint64_t m) {
+  if (n <= c->cap_n && m <= c->cap_m && c->live) return KSPN_OK;
+  uint32_t nn = std::max(n, c->cap_n);
+  uint64_t mm = std::max(m, c->cap_m);
+  if (nn < 64) nn = 64;
+  uint64_t words = ((uint64_t)nn + 255) / 256 * 8 + 16;  // covers the 256-vertex padding of ballot writers
+  KSPN_TRY(dalloc(&c->live, words));
+  KSPN_TRY(dalloc(&c->fwb, words));
+  KSPN_TRY(dalloc(&c->bwb, words));
+  KSPN_TRY(dalloc(&c->inq, words));
+  KSPN_TRY(dalloc(&c->qa, (uint64_t)nn + 64));
+  KSPN_TRY(dalloc(&c->qb, (uint64_t)nn + 64));
+  KSPN_TRY(dalloc(&c->qc, (uint64_t)nn + 64));
+  KSPN_TRY(dalloc(&c->ulist, (uint64_t)nn + 64));
+  KSPN_TRY(dalloc(&c->col, (uint64_t)nn + 64));
+  KSPN_TRY(dalloc(&c->deg, (uint64_t)nn + 64));
+  KSPN_TRY(dalloc(&c->scan, (uint64_t)nn + 64));
+  KSPN_TRY(dalloc(&c->part, (uint64_t)c->num_cu * 64 * 2 + 64));
+  size_t b1 = 0, b2 = 0;
+  if (hipcub::DeviceScan::ExclusiveSum(nullptr, b1, (uint64_t*)nullptr, (uint64_t*)nullptr, (int)(nn + 1)) != hipSuccess)
+    return KSPN_E_HIP;
+  if (hipcub::DeviceScan::ExclusiveSum(nullptr, b2, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)(words + 1)) != hipSuccess)
+    return KSPN_E_HIP;
+  size_t need = std::max(b1, b2) + 256;
+  if (need > c->cub_bytes) {
+    if (c->cub_tmp) (void)hipFree(c->cub_tmp);
+    c->cub_tmp = nullptr;
+    if (hipMalloc(&c->cub_tmp, need) != hipSuccess) return KSPN_E_OOM;
+    c->cub_bytes = need;
+  }
+  c->cap_n = nn;
+  c->cap_m = mm;
+  return KSPN_OK;
+}
+
+int read_counters(Ctx* c, hipStream_t s, int count) {
+  KSPN_HIP(hipGetLastError());
+  KSPN_HIP(hipMemcpyAsync(c->h_cnt, c->cnt, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, s));
+  KSPN_HIP(hipStreamSynchronize(s));
+  return KSPN_OK;
+}
+
+int exclusive_scan_u64(Ctx* c, const uint64_t* in, uint64_t* out, uint64_t items, hipStream_t s) {
+  Prof p(c, s, K_SCAN, 16.0 * items);
+  size_t bytes = c->cub_bytes;
+  KSPN_HIP(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp, bytes, in, out, (int)items, s));
+  return KSPN_OK;
+}
+
+int exclusive_scan_u32(Ctx* c, const uint32_t* in, uint32_t* out, uint64_t items, hipStream_t s) {
+  Prof p(c, s, K_SCAN, 8.0 * items);
+  size_t bytes = c->cub_bytes;
+  KSPN_HIP(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp, bytes, in, out, (int)items, s));
+  return KSPN_OK;
+}
+
+// ascending list of set bits of `bitmap` into `list`; count to *count_out
+// (host) and to cnt[C_AUX] (device)
+int build_live_list(Ctx* c, uint32_t n, const uint32_t* bitmap, uint32_t* list, uint32_t* count_out, hipStream_t s) {
+  uint64_t words = (n + 31) / 32;
+  uint32_t* pc = reinterpret_cast<uint32_t*>(c->deg);   // words+1 popcounts
+  uint32_t* pos = reinterpret_cast<uint32_t*>(c->scan); // scanned
+  {
+    Prof p(c, s, K_LIVE_LIST, 4.0 * words);
+    k_bm_popc<<<stream_grid(c, words + 1), WG, 0, s>>>(bitmap, words, pc);
+  }
+  KSPN_TRY(exclusive_scan_u32(c, pc, pos, words + 1, s));
+  {
+    Prof p(c, s, K_LIVE_LIST, 8.0 * words);
+    k_bm_scatter<<<stream_grid(c, words), WG, 0, s>>>(bitmap, words, pos, list);
+  }
+  k_copy_u32<<<1, 64, 0, s>>>(&c->cnt[C_AUX], pos + words);
+  KSPN_HIP(hipMemcpyAsync(c->h_cnt + C_AUX, pos + words, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  KSPN_HIP(hipStreamSynchronize(s));
+  KSPN_HIP(hipGetLastError());
+  *count_out = c->h_cnt[C_AUX];
+  return KSPN_OK;
+}
+
+namespace {
+
+struct Run {
+  Ctx* c;
+  const kspn_graph* g;
+  uint32_t* labels;
+  kspn_opts o;
+  kspn_stats* st;
+  hipStream_t s;
+  uint32_t n;
+  uint64_t words;
+  uint32_t live_n = 0;  // host view of the live count
+  uint32_t rotation = 0;
+  bool col_valid = false;  // col is 0 for every vertex outside ulist (prev_u entries)
+  uint32_t prev_u = 0;
+};
+
+int select_pivot_live(Run& R, uint32_t* pivot);
+
+// Edge-balanced expansion of queue F (count on device at fcount, host copy
+// f) over CSR (off, adj). Appends to `next` with its counter at next_cnt.
+// Returns the expanded edge count through *edges.
+template <int OP>
+int expand(Run& R, const uint32_t* F, const uint32_t* fcount_dev, uint32_t f, const uint64_t* off, const uint32_t* adj,
+           const OpArgs& a, uint32_t* next, uint32_t* next_cnt, int kern, double state_bytes_per_edge,
+           uint64_t* edges, bool known_total = false, uint64_t total_hint = 0) {
+  Ctx* c = R.c;
+  if (f == 0) {
+    if (edges) *edges = 0;
+    return KSPN_OK;
+  }
+  {
+    Prof p(c, R.s, K_FRONTIER_DEG, 4.0 * f + 16.0 * f + 8.0 * f);
+    k_frontier_deg<<<stream_grid(c, (uint64_t)f + 1), WG, 0, R.s>>>(F, fcount_dev, off, c->deg);
+  }
+  KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)f + 1, R.s));
+  uint64_t total = total_hint;
+  if (!known_total) {
+    KSPN_HIP(hipMemcpyAsync(c->h_u64, c->scan + f, sizeof(uint64_t), hipMemcpyDeviceToHost, R.s));
+    KSPN_HIP(hipStreamSynchronize(R.s));
+    total = *c->h_u64;
+  }
+  if (edges) *edges = total;
+  if (total == 0) return KSPN_OK;
+  {
+    Prof p(c, R.s, kern, 4.0 * f + 8.0 * f + 8.0 * f + (4.0 + state_bytes_per_edge) * total);
+    k_expand<OP><<<expand_grid(c, total), WG, 0, R.s>>>(F, fcount_dev, c->scan, off, adj, a, next, next_cnt);
+  }
+  if (R.st) R.st->edges_scanned += total;
+  return KSPN_OK;
+}
+
+double elapsed_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+  return ms;
+}
+
+// ---------------------------------------------------------------- FW-BW
+// Reachability from the pivot forward (restricted to live vertices), then
+// backward (restricted to reached ones); the backward set is the pivot's
+// SCC and is decided with id = pivot. Each level is push (edge-balanced
+// expansion of the frontier) or pull (sweep over all vertices), switched
+// by Beamer's rule: pull when frontier edges * alpha > remaining edges,
+// back to push when frontier * beta < live.
+int reach(Run& R, int dir, uint32_t pivot, uint32_t* fw_levels_push, uint32_t* fw_levels_pull) {
+  Ctx* c = R.c;
+  const kspn_graph* g = R.g;
+  const uint64_t* off = dir == 0 ? g->fw_off : g->bw_off;
+  const uint32_t* adj = dir == 0 ? g->fw_adj : g->bw_adj;
+  const uint64_t* roff = dir == 0 ? g->bw_off : g->fw_off;
+  const uint32_t* radj = dir == 0 ? g->bw_adj : g->fw_adj;
+  OpArgs a{c->live, c->fwb, c->bwb, c->inq, c->col, R.labels, pivot};
+  uint32_t* F = c->qa;
+  uint32_t* N = c->qb;
+  int fc = C_Q0, nc = C_Q1;
+  // frontier = { pivot }
+  k_set_u32<<<1, 64, 0, R.s>>>(F, pivot);
+  k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[fc], 1u);
+  uint32_t f = 1;
+  bool pull = false;
+  double alpha = R.o.alpha > 0 ? R.o.alpha : 14.0;
+  double beta = R.o.beta > 0 ? R.o.beta : 24.0;
+  double m_rest = (double)g->m;
+  while (f > 0) {
+    KSPN_HIP(hipMemsetAsync(&c->cnt[nc], 0, sizeof(uint32_t), R.s));
+    uint64_t edges = 0;
+    if (!pull) {
+      // degrees + scan first: their total decides the direction
+      {
+        Prof p(c, R.s, K_FRONTIER_DEG, 28.0 * f);
+        k_frontier_deg<<<stream_grid(c, (uint64_t)f + 1), WG, 0, R.s>>>(F, &c->cnt[fc], off, c->deg);
+      }
+      KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)f + 1, R.s));
+      KSPN_HIP(hipMemcpyAsync(c->h_u64, c->scan + f, sizeof(uint64_t), hipMemcpyDeviceToHost, R.s));
+      KSPN_HIP(hipStreamSynchronize(R.s));
+      edges = *c->h_u64;
+      if ((double)edges * alpha > m_rest && f > 64) {
+        pull = true;
+      } else if (edges > 0) {
+        int kern = dir == 0 ? K_EXPAND_FW : K_EXPAND_BW;
+        Prof p(c, R.s, kern, 20.0 * f + (4.0 + 0.125) * edges);
+        if (dir == 0)
+          k_expand<OP_FW><<<expand_grid(c, edges), WG, 0, R.s>>>(F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
+        else
+          k_expand<OP_BW><<<expand_grid(c, edges), WG, 0, R.s>>>(F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
+        m_rest -= (double)edges;
+        if (R.st) R.st->edges_scanned += edges;
+        if (fw_levels_push) (*fw_levels_push)++;
+      }
+    }
+    if (pull) {
+      int kern = dir == 0 ? K_PULL_FW : K_PULL_BW;
+      Prof p(c, R.s, kern, (double)R.n / 8.0 + 16.0 * R.live_n);
+      if (dir == 0)
+        k_pull_fw<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->live, c->fwb, roff, radj, N, &c->cnt[nc]);
+      else
+        k_pull_bw<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->live, c->fwb, c->bwb, roff, radj, R.labels, pivot, N,
+                                                       &c->cnt[nc]);
+      if (fw_levels_pull) (*fw_levels_pull)++;
+    }
+    KSPN_TRY(read_counters(c, R.s, 2));
+    uint32_t nf = c->h_cnt[nc];
+    if (pull && (double)nf * beta < (double)R.live_n) pull = false;
+    std::swap(F, N);
+    std::swap(fc, nc);
+    f = nf;
+  }
+  return KSPN_OK;
+}
+
+int fwbw(Run& R, uint32_t pivot, uint32_t* decided) {
+  Ctx* c = R.c;
+  {
+    Prof p(c, R.s, K_BITMAP, 8.0 * R.words);
+    k_bm_not<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->fwb, c->live, R.words);
+    k_bm_set_one<<<1, 64, 0, R.s>>>(c->fwb, pivot);
+  }
+  uint32_t lp = 0, lq = 0;
+  KSPN_TRY(reach(R, 0, pivot, &lp, &lq));
+  {
+    Prof p(c, R.s, K_BITMAP, 12.0 * R.words);
+    k_bm_nand<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwb, c->fwb, c->live, R.words);
+    k_bm_set_one<<<1, 64, 0, R.s>>>(c->bwb, pivot);
+  }
+  k_set_u32<<<1, 64, 0, R.s>>>(R.labels + pivot, pivot);
+  KSPN_TRY(reach(R, 1, pivot, &lp, &lq));
+  KSPN_HIP(hipMemsetAsync(&c->cnt[C_DECIDED], 0, sizeof(uint32_t), R.s));
+  {
+    Prof p(c, R.s, K_BITMAP, 16.0 * R.words);
+    k_fwbw_finish<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->live, c->fwb, c->bwb, R.words, c->cnt);
+  }
+  KSPN_TRY(read_counters(c, R.s, C_DECIDED + 1));
+  *decided = c->h_cnt[C_DECIDED];
+  R.live_n -= *decided;
+  if (R.st) {
+    R.st->fwbw_levels_push += lp;
+    R.st->fwbw_levels_pull += lq;
+  }
+  return KSPN_OK;
+}
+
+// ---------------------------------------------------------------- trims
+// exhaustive trim-1 to a fixpoint: sweep a candidate queue, then the live
+// neighbours (both directions) of what was decided become the candidates
+int trim_exhaustive(Run& R, uint32_t* decided) {
+  Ctx* c = R.c;
+  const kspn_graph* g = R.g;
+  uint32_t q = 0;
+  KSPN_TRY(build_live_list(c, R.n, c->live, c->qa, &q, R.s));
+  k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], &c->cnt[C_AUX]);
+  uint32_t* Q = c->qa;
+  uint32_t* T = c->qb;
+  uint32_t* NQ = c->qc;
+  bool first = true;
+  uint32_t total = 0;
+  OpArgs a{c->live, c->fwb, c->bwb, c->inq, c->col, R.labels, 0};
+  while (q > 0) {
+    KSPN_HIP(hipMemsetAsync(&c->cnt[C_Q1], 0, sizeof(uint32_t), R.s));
+    {
+      Prof p(c, R.s, K_TRIM_SWEEP, 4.0 * q + 16.0 * q);
+      k_trim_sweep<TRIM_ONE><<<stream_grid(c, q), WG, 0, R.s>>>(Q, &c->cnt[C_Q0], c->live, first ? nullptr : c->inq,
+                                                                g->fw_off, g->fw_adj, g->bw_off, g->bw_adj, R.labels, T,
+                                                                &c->cnt[C_Q1]);
+    }
+    first = false;
+    KSPN_TRY(read_counters(c, R.s, 2));
+    uint32_t t = c->h_cnt[C_Q1];
+    total += t;
+    if (t == 0) break;
+    // candidates: live neighbours of the trimmed vertices, deduplicated by inq
+    KSPN_HIP(hipMemsetAsync(&c->cnt[C_Q0], 0, sizeof(uint32_t), R.s));
+    uint64_t e1 = 0, e2 = 0;
+    KSPN_TRY(expand<OP_NEIGH>(R, T, &c->cnt[C_Q1], t, g->fw_off, g->fw_adj, a, NQ, &c->cnt[C_Q0], K_EXPAND_NEIGH, 0.25, &e1));
+    KSPN_TRY(expand<OP_NEIGH>(R, T, &c->cnt[C_Q1], t, g->bw_off, g->bw_adj, a, NQ, &c->cnt[C_Q0], K_EXPAND_NEIGH, 0.25, &e2));
+    KSPN_TRY(read_counters(c, R.s, 1));
+    q = c->h_cnt[C_Q0];
+    std::swap(Q, NQ);
+  }
+  // leave no inq bit set: clear the last candidate queue
+  if (q > 0) {
+    k_bm_clear_list<<<stream_grid(c, q), WG, 0, R.s>>>(c->inq, Q, &c->cnt[C_Q0]);
+  }
+  *decided = total;
+  R.live_n -= total;
+  return KSPN_OK;
+}
+
+// one trim_1_normal-style sweep over the live list
+int trim_normal(Run& R, uint32_t* decided) {
+  Ctx* c = R.c;
+  const kspn_graph* g = R.g;
+  uint32_t q = 0;
+  KSPN_TRY(build_live_list(c, R.n, c->live, c->qa, &q, R.s));
+  k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], &c->cnt[C_AUX]);
+  KSPN_HIP(hipMemsetAsync(&c->cnt[C_Q1], 0, sizeof(uint32_t), R.s));
+  if (q) {
+    Prof p(c, R.s, K_TRIM_SWEEP, 20.0 * q);
+    k_trim_sweep<TRIM_NORMAL><<<stream_grid(c, q), WG, 0, R.s>>>(c->qa, &c->cnt[C_Q0], c->live, nullptr, g->fw_off,
+                                                                 g->fw_adj, g->bw_off, g->bw_adj, R.labels, c->qb,
+                                                                 &c->cnt[C_Q1]);
+  }
+  KSPN_TRY(read_counters(c, R.s, 2));
+  *decided = c->h_cnt[C_Q1];
+  R.live_n -= *decided;
+  return KSPN_OK;
+}
+
+// trim-2 over all live vertices, in-direction launch then out-direction
+int trim_two(Run& R, uint32_t* decided) {
+  Ctx* c = R.c;
+  const kspn_graph* g = R.g;
+  uint32_t q = 0;
+  KSPN_TRY(build_live_list(c, R.n, c->live, c->qa, &q, R.s));
+  if (q == 0) {
+    *decided = 0;
+    return KSPN_OK;
+  }
+  k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], &c->cnt[C_AUX]);
+  KSPN_HIP(hipMemsetAsync(&c->cnt[C_Q1], 0, sizeof(uint32_t), R.s));
+  {
+    Prof p(c, R.s, K_TRIM2, 40.0 * q);
+    k_trim2<0><<<stream_grid(c, q), WG, 0, R.s>>>(c->qa, &c->cnt[C_Q0], c->live, g->bw_off, g->bw_adj, R.labels, c->qb,
+                                                  &c->cnt[C_Q1]);
+  }
+  {
+    Prof p(c, R.s, K_TRIM2, 40.0 * q);
+    k_trim2<1><<<stream_grid(c, q), WG, 0, R.s>>>(c->qa, &c->cnt[C_Q0], c->live, g->fw_off, g->fw_adj, R.labels, c->qb,
+                                                  &c->cnt[C_Q1]);
+  }
+  KSPN_TRY(read_counters(c, R.s, 2));
+  *decided = c->h_cnt[C_Q1];
+  R.live_n -= *decided;
+  return KSPN_OK;
+}
+
+// ---------------------------------------------------------------- colour
+// One round: keys ~rotr(v, r) on live vertices, forward max-propagation to
+// a fixpoint (pull sweep, then push from changed vertices), roots where the
+// colour is the own key, backward closure over in-edges within a colour.
+int color_round(Run& R, uint32_t* decided) {
+  Ctx* c = R.c;
+  const kspn_graph* g = R.g;
+  uint32_t r = ++R.rotation;
+  // keys on live vertices, 0 elsewhere: a full pass the first time, then
+  // only over the previous round's live list (everything else is already 0)
+  if (!R.col_valid) {
+    Prof p(c, R.s, K_COLOR_INIT, 4.0 * R.n + R.n / 8.0);
+    k_color_init_all<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->live, c->col, r);
+  } else if (R.prev_u) {
+    Prof p(c, R.s, K_COLOR_INIT, 8.0 * R.prev_u);
+    k_color_init_list<<<stream_grid(c, R.prev_u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], c->live, c->col, r);
+  }
+  uint32_t u = 0;
+  KSPN_TRY(build_live_list(c, R.n, c->live, c->ulist, &u, R.s));
+  k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_RES_N], &c->cnt[C_AUX]);
+  R.prev_u = u;
+  OpArgs a{c->live, c->fwb, c->bwb, c->inq, c->col, R.labels, r};
+  // forward: pull sweep over all live vertices
+  KSPN_HIP(hipMemsetAsync(&c->cnt[C_Q0], 0, sizeof(uint32_t), R.s));
+  {
+    Prof p(c, R.s, K_COLOR_PULL, 24.0 * u);
+    k_color_pull<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->col, c->qa,
+                                                    &c->cnt[C_Q0]);
+  }
+  KSPN_TRY(read_counters(c, R.s, 1));
+  uint32_t f = c->h_cnt[C_Q0];
+  uint32_t* F = c->qa;
+  uint32_t* N = c->qb;
+  int fc = C_Q0, nc = C_Q1;
+  uint32_t iters = 1;
+  while (f > 0) {
+    // drop F's queue bits so its members can be queued again
+    k_bm_clear_list<<<stream_grid(c, f), WG, 0, R.s>>>(c->inq, F, &c->cnt[fc]);
+    KSPN_HIP(hipMemsetAsync(&c->cnt[nc], 0, sizeof(uint32_t), R.s));
+    if ((double)f * 4.0 > (double)u) {
+      // large frontier: another pull sweep over every live vertex
+      Prof p(c, R.s, K_COLOR_PULL, 24.0 * u);
+      k_color_pull<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->col, N,
+                                                      &c->cnt[nc]);
+    } else {
+      uint64_t e = 0;
+      KSPN_TRY(expand<OP_COLOR>(R, F, &c->cnt[fc], f, g->fw_off, g->fw_adj, a, N, &c->cnt[nc], K_EXPAND_COLOR, 8.0, &e));
+    }
+    KSPN_TRY(read_counters(c, R.s, 2));
+    f = c->h_cnt[nc];
+    std::swap(F, N);
+    std::swap(fc, nc);
+    ++iters;
+  }
+  // queue bits of the last (empty) frontier are clear; pull-sweep outputs
+  // never set inq, push outputs were cleared before each expansion
+  // roots
+  KSPN_HIP(hipMemsetAsync(&c->cnt[C_Q0], 0, sizeof(uint32_t), R.s));
+  {
+    Prof p(c, R.s, K_COLOR_ROOTS, 12.0 * u);
+    k_color_roots<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], c->live, c->col, R.labels, r, c->qa,
+                                                     &c->cnt[C_Q0]);
+  }
+  KSPN_TRY(read_counters(c, R.s, 1));
+  f = c->h_cnt[C_Q0];
+  uint32_t total = f;
+  F = c->qa;
+  N = c->qb;
+  fc = C_Q0;
+  nc = C_Q1;
+  while (f > 0) {
+    KSPN_HIP(hipMemsetAsync(&c->cnt[nc], 0, sizeof(uint32_t), R.s));
+    uint64_t e = 0;
+    KSPN_TRY(expand<OP_COLOR_BW>(R, F, &c->cnt[fc], f, g->bw_off, g->bw_adj, a, N, &c->cnt[nc], K_EXPAND_COLOR_BW, 4.0, &e));
+    KSPN_TRY(read_counters(c, R.s, 2));
+    f = c->h_cnt[nc];
+    total += f;
+    std::swap(F, N);
+    std::swap(fc, nc);
+  }
+  *decided = total;
+  R.live_n -= total;
+  R.col_valid = true;  // vertices decided from now on are in ulist and get re-keyed
+  if (R.st) {
+    R.st->color_rounds++;
+    R.st->color_iters += iters;
+  }
+  return KSPN_OK;
+}
+
+// ---------------------------------------------------------------- residual
+int residual(Run& R, uint32_t* decided) {
+  Ctx* c = R.c;
+  const kspn_graph* g = R.g;
+  uint32_t u = 0;
+  KSPN_TRY(build_live_list(c, R.n, c->live, c->ulist, &u, R.s));
+  if (u == 0) {
+    *decided = 0;
+    return KSPN_OK;
+  }
+  k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_RES_N], &c->cnt[C_AUX]);
+  uint32_t* subid = c->col;
+  {
+    Prof p(c, R.s, K_RESIDUAL, 8.0 * u);
+    k_sub_ids<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], subid);
+    k_sub_deg<<<stream_grid(c, (uint64_t)u + 1), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, c->live,
+                                                               nullptr, c->deg);
+  }
+  KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)u + 1, R.s));
+  KSPN_HIP(hipMemcpyAsync(c->h_u64, c->scan + u, sizeof(uint64_t), hipMemcpyDeviceToHost, R.s));
+  KSPN_HIP(hipStreamSynchronize(R.s));
+  uint64_t sm = *c->h_u64;
+  if (sm + 1 > c->cap_sub_m) {
+    KSPN_TRY(dalloc(&c->sub_adj, sm + 1));
+    c->cap_sub_m = sm + 1;
+  }
+  if (!c->tj) KSPN_TRY(dalloc(&c->tj, 6ull * c->cap_n + 64));
+  {
+    Prof p(c, R.s, K_RESIDUAL, 16.0 * u + 8.0 * sm);
+    k_sub_fill<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, c->live, nullptr,
+                                                  subid, c->scan, c->sub_adj);
+  }
+  uint32_t* sublab = c->qb;
+  {
+    Prof p(c, R.s, K_TARJAN, 24.0 * u + 4.0 * sm);
+    k_tarjan<<<1, 64, 0, R.s>>>(u, c->scan, c->sub_adj, c->tj, sublab);
+  }
+  k_res_apply<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], sublab, R.labels);
+  KSPN_HIP(hipMemsetAsync(c->live, 0, R.words * sizeof(uint32_t), R.s));
+  if (R.st) {
+    R.st->residual_n = u;
+    R.st->residual_m = sm;
+  }
+  *decided = u;
+  R.live_n -= u;
+  R.col_valid = false;  // col held sub ids
+  return KSPN_OK;
+}
+
+// ---------------------------------------------------------------- canonical ids
+int canonicalize(Ctx* c, uint32_t n, uint32_t* labels, hipStream_t s) {
+  Prof p(c, s, K_CANON, 16.0 * n);
+  k_fill_u32<<<stream_grid(c, n), WG, 0, s>>>(c->col, n, 0xFFFFFFFFu);
+  k_canon_min<<<stream_grid(c, n), WG, 0, s>>>(n, labels, c->col);
+  k_canon_apply<<<stream_grid(c, n), WG, 0, s>>>(n, labels, c->col);
+  return KSPN_OK;
+}
+
+int select_pivot_live(Run& R, uint32_t* pivot) {
+  Ctx* c = R.c;
+  uint32_t u = c->h_cnt[C_AUX];
+  uint32_t nparts = std::min<uint32_t>(stream_grid(c, u), (uint32_t)c->num_cu * 16);
+  {
+    Prof p(c, R.s, K_PIVOT, 20.0 * u);
+    k_pivot_live<<<nparts, WG, 0, R.s>>>(c->ulist, &c->cnt[C_AUX], R.g->fw_off, R.g->bw_off, c->part);
+    k_pivot_reduce<<<1, WG, 0, R.s>>>(c->part, nparts, &c->cnt[C_FLAG]);
+  }
+  KSPN_TRY(read_counters(c, R.s, C_FLAG + 1));
+  *pivot = c->h_cnt[C_FLAG];
+  if (*pivot == 0xFFFFFFFFu) *pivot = R.n;
+  return KSPN_OK;
+}
+
+int run_pipeline(Run& R) {
+  Ctx* c = R.c;
+  const kspn_graph* g = R.g;
+  kspn_stats* st = R.st;
+  hipEvent_t ev[8];
+  bool timing = st != nullptr;
+  if (timing)
+    for (auto& e : ev) KSPN_HIP(hipEventCreate(&e));
+  auto mark = [&](int i) {
+    if (timing) (void)hipEventRecord(ev[i], R.s);
+  };
+  mark(0);
+  // ---- trim-1 first pass + pivot
+  KSPN_HIP(hipMemsetAsync(c->cnt, 0, sizeof(uint32_t) * C_COUNT_, R.s));
+  KSPN_HIP(hipMemsetAsync(c->live, 0, R.words * sizeof(uint32_t), R.s));
+  KSPN_HIP(hipMemsetAsync(c->inq, 0, R.words * sizeof(uint32_t), R.s));
+  uint32_t nparts = std::min<uint32_t>(stream_grid(c, R.n), (uint32_t)c->num_cu * 16);
+  {
+    Prof p(c, R.s, K_TRIM_FIRST, 20.0 * R.n + R.n / 8.0);
+    k_trim_first<<<nparts, WG, 0, R.s>>>(R.n, g->fw_off, g->bw_off, c->live, R.labels, c->part, c->cnt);
+  }
+  {
+    Prof p(c, R.s, K_PIVOT, 16.0 * nparts);
+    k_pivot_reduce<<<1, WG, 0, R.s>>>(c->part, nparts, &c->cnt[C_FLAG]);
+  }
+  KSPN_TRY(read_counters(c, R.s, C_FLAG + 1));
+  uint32_t decided = c->h_cnt[C_DECIDED];
+  uint32_t pivot = c->h_cnt[C_FLAG];
+  R.live_n = R.n - decided;
+  if (st) {
+    st->decided_trim = decided;
+    st->pivot = pivot;
+  }
+  const bool trim_ex = R.o.method == KSPN_METHOD_TRIM_EX;
+  if (trim_ex && R.live_n > 0) {
+    uint32_t d = 0;
+    KSPN_TRY(trim_exhaustive(R, &d));
+    if (st) st->decided_trim += d;
+    if (!(R.o.flags & KSPN_FLAG_NO_TRIM2) && R.live_n > 0) {
+      KSPN_TRY(trim_two(R, &d));
+      if (st) st->decided_trim2 += d;
+      if (d && R.live_n > 0) {
+        KSPN_TRY(trim_exhaustive(R, &d));
+        if (st) st->decided_trim += d;
+      }
+    }
+  }
+  mark(1);
+  // ---- FW-BW rounds
+  uint32_t rounds = R.o.fwbw_rounds ? R.o.fwbw_rounds : 1;
+  for (uint32_t k = 0; k < rounds && R.live_n > 0; ++k) {
+    if (R.o.residual_threshold && R.live_n <= R.o.residual_threshold) break;
+    if (k > 0 || trim_ex) {
+      // pivot among live vertices by original degree product
+      KSPN_HIP(hipMemsetAsync(c->cnt, 0, sizeof(uint32_t) * C_COUNT_, R.s));
+      // reuse k_trim_first's reduction on a live-masked pass
+      uint32_t u = 0;
+      KSPN_TRY(build_live_list(c, R.n, c->live, c->ulist, &u, R.s));
+      (void)u;
+      KSPN_TRY(select_pivot_live(R, &pivot));
+      if (k == 0 && st) st->pivot = pivot;
+    }
+    if (pivot >= R.n) break;
+    uint32_t d = 0;
+    KSPN_TRY(fwbw(R, pivot, &d));
+    if (st) st->decided_fwbw += d;
+    if (R.live_n == 0) break;
+    if (trim_ex) {
+      KSPN_TRY(trim_exhaustive(R, &d));
+      if (st) st->decided_trim += d;
+    } else {
+      KSPN_TRY(trim_normal(R, &d));
+      if (st) st->decided_trim += d;
+    }
+  }
+  mark(2);
+  // ---- colour rounds
+  uint32_t cap = R.o.max_rounds;
+  uint32_t nrounds = 0;
+  while (R.live_n > 0) {
+    if (R.o.residual_threshold && R.live_n <= R.o.residual_threshold) break;
+    if (cap && nrounds >= cap) return KSPN_E_ITER_CAP;
+    uint32_t d = 0;
+    KSPN_TRY(color_round(R, &d));
+    ++nrounds;
+    if (st) st->decided_color += d;
+    if (d == 0) return KSPN_E_ITER_CAP;  // cannot happen: the max key is always a root
+    if (trim_ex && R.live_n > 0) {
+      KSPN_TRY(trim_exhaustive(R, &d));
+      if (st) st->decided_trim += d;
+    }
+  }
+  mark(3);
+  // ---- residual Tarjan
+  if (R.live_n > 0) {
+    uint32_t d = 0;
+    KSPN_TRY(residual(R, &d));
+    if (st) st->decided_residual += d;
+  }
+  mark(4);
+  KSPN_TRY(canonicalize(c, R.n, R.labels, R.s));
+  mark(5);
+  KSPN_HIP(hipStreamSynchronize(R.s));
+  if (timing) {
+    st->ms_trim = elapsed_ms(ev[0], ev[1]);
+    st->ms_fwbw = elapsed_ms(ev[1], ev[2]);
+    st->ms_color = elapsed_ms(ev[2], ev[3]);
+    st->ms_residual = elapsed_ms(ev[3], ev[4]);
+    st->ms_canon = elapsed_ms(ev[4], ev[5]);
+    st->ms_total = elapsed_ms(ev[0], ev[5]);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  }
+  return KSPN_OK;
+}
+
+}  // namespace
+
+}  // namespace kspn
+
+// =====================================================================
+// C ABI
+// =====================================================================
+
+using namespace kspn;
+
+struct kspn_ctx {
+  Ctx c;
+};
+
+extern "C" {
+
+void kspn_opts_default(kspn_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->method = KSPN_METHOD_KASPAN;
+  o->pivot = KSPN_PIVOT_PLAIN;
+  o->fwbw_rounds = 1;
+  o->alpha = 14.0f;
+  o->beta = 24.0f;
+}
+
+const char* kspn_strerror(int code) {
+  switch (code) {
+    case KSPN_OK: return "ok";
+    case KSPN_E_INVALID: return "invalid argument";
+    case KSPN_E_OOM: return "out of memory";
+    case KSPN_E_HIP: return "HIP runtime error";
+    case KSPN_E_ITER_CAP: return "iteration cap reached";
+    case KSPN_E_UNIMPLEMENTED: return "not implemented";
+    default: return "unknown error";
+  }
+}
+
+int kspn_ctx_create(kspn_ctx** out, int device) {
+  if (!out) return KSPN_E_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return KSPN_E_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return KSPN_E_HIP;
+  kspn_ctx* k = new (std::nothrow) kspn_ctx();
+  if (!k) return KSPN_E_OOM;
+  Ctx* c = &k->c;
+  c->device = device;
+  int cu = 0;
+  if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0) c->num_cu = cu;
+  if (hipMalloc((void**)&c->cnt, sizeof(uint32_t) * C_COUNT_) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_cnt, sizeof(uint32_t) * C_COUNT_, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_u64, sizeof(uint64_t) * 4, hipHostMallocDefault) != hipSuccess) {
+    kspn_ctx_destroy(k);
+    return KSPN_E_OOM;
+  }
+  std::memset(c->h_cnt, 0, sizeof(uint32_t) * C_COUNT_);
+  c->kt.resize(K_COUNT);
+  for (int i = 0; i < K_COUNT; ++i) c->kt[i].name = kKernNames[i];
+  *out = k;
+  return KSPN_OK;
+}
+
+int kspn_ctx_destroy(kspn_ctx* k) {
+  if (!k) return KSPN_OK;
+  Ctx* c = &k->c;
+  (void)hipSetDevice(c->device);
+  void* ptrs[] = {c->live, c->fwb, c->bwb, c->inq, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
+                  c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->h_cnt) (void)hipHostFree(c->h_cnt);
+  if (c->h_u64) (void)hipHostFree(c->h_u64);
+  for (auto& ep : c->pending) {
+    (void)hipEventDestroy(ep.a);
+    (void)hipEventDestroy(ep.b);
+  }
+  for (auto e : c->free_events) (void)hipEventDestroy(e);
+  delete k;
+  return KSPN_OK;
+}
+
+int kspn_ctx_reserve(kspn_ctx* k, uint32_t n, uint64_t m) {
+  if (!k) return KSPN_E_INVALID;
+  if (hipSetDevice(k->c.device) != hipSuccess) return KSPN_E_HIP;
+  return ctx_reserve(&k->c, n, m);
+}
+
+int kspn_scc(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* opts, kspn_stats* stats, void* stream) {
+  if (!k || !g || (g->n > 0 && (!labels || !g->fw_off || !g->bw_off))) return KSPN_E_INVALID;
+  if (g->m > 0 && (!g->fw_adj || !g->bw_adj)) return KSPN_E_INVALID;
+  if (g->n == 0xFFFFFFFFu) return KSPN_E_INVALID;  // 0xFFFFFFFF is the undecided sentinel
+  kspn_opts o;
+  if (opts) o = *opts;
+  else kspn_opts_default(&o);
+  if (o.method > KSPN_METHOD_TRIM_EX) return KSPN_E_INVALID;
+  if (o.pivot == KSPN_PIVOT_LEARNED) return KSPN_E_UNIMPLEMENTED;
+  if (o.pivot > KSPN_PIVOT_LEARNED) return KSPN_E_INVALID;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (g->n == 0) return KSPN_OK;
+  Ctx* c = &k->c;
+  if (hipSetDevice(c->device) != hipSuccess) return KSPN_E_HIP;
+  KSPN_TRY(ctx_reserve(c, g->n, g->m));
+  hipStream_t s = (hipStream_t)stream;
+  c->profile = stats && (o.flags & KSPN_FLAG_PROFILE);
+  if (c->profile)
+    for (auto& t : c->kt) { t.launches = 0; t.ms = 0; t.bytes = 0; }
+  Run R{c, g, labels, o, stats, s, g->n, bm_words(g->n)};
+  int rc = run_pipeline(R);
+  if (c->profile) {
+    (void)hipStreamSynchronize(s);
+    prof_collect(c);
+    uint32_t nk = 0;
+    for (int i = 0; i < K_COUNT && nk < KSPN_KERN_MAX; ++i) {
+      if (!c->kt[i].launches) continue;
+      kspn_kern_stat& ks = stats->kern[nk++];
+      std::snprintf(ks.name, sizeof(ks.name), "%s", c->kt[i].name);
+      ks.launches = c->kt[i].launches;
+      ks.ms = c->kt[i].ms;
+      ks.bytes = c->kt[i].bytes;
+    }
+    stats->n_kern = nk;
+    c->profile = false;
+  }
+  return rc;
+}
+
+int kspn_scc_host(kspn_ctx* k, uint32_t n, uint64_t m, const uint64_t* fw_off, const uint32_t* fw_adj,
+                  const uint64_t* bw_off, const uint32_t* bw_adj, uint32_t* labels, const kspn_opts* opts,
+                  kspn_stats* stats) {
+  if (!k || (n > 0 && (!fw_off || !bw_off || !labels)) || (m > 0 && (!fw_adj || !bw_adj))) return KSPN_E_INVALID;
+  if (n == 0) return KSPN_OK;
+  if (fw_off[0] != 0 || bw_off[0] != 0 || fw_off[n] != m || bw_off[n] != m) return KSPN_E_INVALID;
+  if (hipSetDevice(k->c.device) != hipSuccess) return KSPN_E_HIP;
+  uint64_t *dfo = nullptr, *dbo = nullptr;
+  uint32_t *dfa = nullptr, *dba = nullptr, *dl = nullptr;
+  int rc = KSPN_OK;
+  if (hipMalloc((void**)&dfo, (n + 1ull) * 8) != hipSuccess || hipMalloc((void**)&dbo, (n + 1ull) * 8) != hipSuccess ||
+      hipMalloc((void**)&dfa, (m ? m : 1) * 4) != hipSuccess || hipMalloc((void**)&dba, (m ? m : 1) * 4) != hipSuccess ||
+      hipMalloc((void**)&dl, (uint64_t)n * 4) != hipSuccess) {
+    rc = KSPN_E_OOM;
+  }
+  if (rc == KSPN_OK) {
+    if (hipMemcpy(dfo, fw_off, (n + 1ull) * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dbo, bw_off, (n + 1ull) * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        (m && hipMemcpy(dfa, fw_adj, m * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+        (m && hipMemcpy(dba, bw_adj, m * 4, hipMemcpyHostToDevice) != hipSuccess))
+      rc = KSPN_E_HIP;
+  }
+  if (rc == KSPN_OK) {
+    kspn_graph g{n, m, dfo, dfa, dbo, dba};
+    rc = kspn_scc(k, &g, dl, opts, stats, nullptr);
+  }
+  if (rc == KSPN_OK && hipMemcpy(labels, dl, (uint64_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = KSPN_E_HIP;
+  for (void* p : {(void*)dfo, (void*)dbo, (void*)dfa, (void*)dba, (void*)dl})
+    if (p) (void)hipFree(p);
+  return rc;
+}
+
+int kspn_canonicalize(kspn_ctx* k, uint32_t n, uint32_t* labels, void* stream) {
+  if (!k || (n > 0 && !labels)) return KSPN_E_INVALID;
+  if (n == 0) return KSPN_OK;
+  Ctx* c = &k->c;
+  if (hipSetDevice(c->device) != hipSuccess) return KSPN_E_HIP;
+  KSPN_TRY(ctx_reserve(c, n, 0));
+  hipStream_t s = (hipStream_t)stream;
+  KSPN_TRY(canonicalize(c, n, labels, s));
+  KSPN_HIP(hipStreamSynchronize(s));
+  return KSPN_OK;
+}
+
+int kspn_tarjan(kspn_ctx* k, uint32_t n, uint64_t m, const uint64_t* off, const uint32_t* adj, uint32_t* labels,
+                void* stream) {
+  if (!k || (n > 0 && (!off || !labels)) || (m > 0 && !adj)) return KSPN_E_INVALID;
+  if (n == 0) return KSPN_OK;
+  Ctx* c = &k->c;
+  if (hipSetDevice(c->device) != hipSuccess) return KSPN_E_HIP;
+  KSPN_TRY(ctx_reserve(c, n, m));
+  if (!c->tj) KSPN_TRY(dalloc(&c->tj, 6ull * c->cap_n + 64));
+  hipStream_t s = (hipStream_t)stream;
+  k_tarjan<<<1, 64, 0, s>>>(n, off, adj, c->tj, labels);
+  KSPN_HIP(hipGetLastError());
+  KSPN_HIP(hipStreamSynchronize(s));
+  return KSPN_OK;
+}
+
+int kspn_subgraph(kspn_ctx* k, uint32_t n, const uint64_t* off, const uint32_t* adj, const uint8_t* mask,
+                  uint32_t* super_ids, uint64_t* sub_off, uint32_t* sub_adj, uint32_t* sub_n, uint64_t* sub_m,
+                  void* stream) {
+  if (!k || !sub_n || !sub_m || (n > 0 && (!off || !mask || !super_ids || !sub_off))) return KSPN_E_INVALID;
+  *sub_n = 0;
+  *sub_m = 0;
+  if (n == 0) return KSPN_OK;
+  Ctx* c = &k->c;
+  if (hipSetDevice(c->device) != hipSuccess) return KSPN_E_HIP;
+  KSPN_TRY(ctx_reserve(c, n, 0));
+  hipStream_t s = (hipStream_t)stream;
+  // byte mask -> bitmap -> ascending list of members
+  uint64_t words = bm_words(n);
+  KSPN_HIP(hipMemsetAsync(c->fwb, 0, words * 4, s));
+  k_mask_to_bitmap<<<stream_grid(c, n), WG, 0, s>>>(n, mask, c->fwb);
+  uint32_t sn = 0;
+  KSPN_TRY(build_live_list(c, n, c->fwb, super_ids, &sn, s));
+  if (sn == 0) {
+    KSPN_HIP(hipMemsetAsync(sub_off, 0, 8, s));
+    KSPN_HIP(hipStreamSynchronize(s));
+    return KSPN_OK;
+  }
+  k_copy_u32<<<1, 64, 0, s>>>(&c->cnt[C_RES_N], &c->cnt[C_AUX]);
+  k_sub_ids<<<stream_grid(c, sn), WG, 0, s>>>(super_ids, &c->cnt[C_RES_N], c->col);
+  k_sub_deg<<<stream_grid(c, (uint64_t)sn + 1), WG, 0, s>>>(super_ids, &c->cnt[C_RES_N], off, adj, nullptr, mask, c->deg);
+  KSPN_TRY(exclusive_scan_u64(c, c->deg, sub_off, (uint64_t)sn + 1, s));
+  k_sub_fill<<<stream_grid(c, sn), WG, 0, s>>>(super_ids, &c->cnt[C_RES_N], off, adj, nullptr, mask, c->col, sub_off,
+                                              sub_adj);
+  KSPN_HIP(hipMemcpyAsync(c->h_u64, sub_off + sn, 8, hipMemcpyDeviceToHost, s));
+  KSPN_HIP(hipStreamSynchronize(s));
+  *sub_n = sn;
+  *sub_m = *c->h_u64;
+  return KSPN_OK;
+}
+
+}  // extern "C"
+

@@ -1,0 +1,275 @@
+"""ctypes view of the CPU oracle (libkspn_oracle.so). TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` -- as the checker or the timed CPU baseline, never as the product.
+Every function mirrors a C function in ``kspn_oracle.c`` whose header cites the
+KaSpan source it restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkspn_oracle.so")
+UNDECIDED = 0xFFFFFFFF
+
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class SccStats(C.Structure):
+    _fields_ = [
+        ("pivot", C.c_uint32),
+        ("decided_trim_first", C.c_uint32),
+        ("decided_fwbw", C.c_uint32),
+        ("decided_trim_normal", C.c_uint32),
+        ("decided_color", C.c_uint32),
+        ("color_rounds", C.c_uint32),
+        ("decided_residual", C.c_uint32),
+        ("residual_n", C.c_uint32),
+        ("residual_m", C.c_uint64),
+    ]
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc, -O3)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        u32, u64, dbl, i32 = C.c_uint32, C.c_uint64, C.c_double, C.c_int
+        sig = {
+            "orc_splitmix64_at": (u64, [u64, u64]),
+            "orc_perm_pow2": (u32, [u64, u32, u32]),
+            "orc_gen_gnm": (u64, [u64, u32, u64, _u32p, _u32p]),
+            "orc_gen_rmat": (u64, [u64, u32, u64, dbl, dbl, dbl, i32, _u32p, _u32p]),
+            "orc_gen_lattice": (u64, [u64, u32, dbl, _u32p, _u32p]),
+            "orc_lattice_max_edges": (u64, [u32]),
+            "orc_gen_fuzzy": (u64, [u64, u32, dbl, _u32p, _u32p, u64, _u32p]),
+            "orc_gen_planted": (u64, [u64, u32, dbl, u32, dbl, dbl, _u32p, _u32p, u64, _u32p]),
+            "orc_planted_sizes": (u64, [u64, u32, dbl, u32, _u32p, u64]),
+            "orc_csr_from_edges": (u64, [u32, u64, _u32p, _u32p, _u64p, _u32p]),
+            "orc_transpose": (None, [u32, _u64p, _u32p, _u64p, _u32p]),
+            "orc_subgraph": (u64, [u32, _u64p, _u32p, _u8p, _u32p, C.POINTER(u32), _u64p, _u32p]),
+            "orc_tarjan": (u32, [u32, _u64p, _u32p, _u32p]),
+            "orc_canonicalize": (None, [u32, _u32p]),
+            "orc_brute_force": (i32, [u32, _u64p, _u32p, _u32p]),
+            "orc_trim_1_first": (u32, [u32, _u64p, _u64p, _u8p, _u32p]),
+            "orc_fwbw": (u32, [u32, _u64p, _u32p, _u64p, _u32p, _u8p, u32, _u32p]),
+            "orc_color_round": (u32, [u32, _u64p, _u32p, _u64p, _u32p, _u8p, u32, _u32p, _u32p]),
+            "orc_kaspan_scc": (i32, [u32, _u64p, _u32p, _u64p, _u32p, _u32p, i32, C.POINTER(SccStats)]),
+            "orc_scc_trim_ex_residual": (i32, [u32, _u64p, _u32p, _u64p, _u32p, _u32p, C.POINTER(SccStats)]),
+            "orc_scc_trim_ex": (i32, [u32, _u64p, _u32p, _u64p, _u32p, _u32p, C.POINTER(SccStats)]),
+            "orc_pivot_features": (None, [u32, _u64p, _u32p, _u64p, _u32p, _f32p]),
+            "orc_mlp_forward": (None, [u32, _f32p, _f64p, _f64p, _f64p, _f64p, _f64p, dbl, _f64p]),
+            "orc_mlp_loss_grad": (dbl, [u32, _f32p, _f32p, _f64p, _f64p, _f64p, _f64p, _f64p, dbl,
+                                        _f64p, _f64p, _f64p, _f64p, _f64p, C.POINTER(dbl)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+# --------------------------------------------------------------------------- graphs
+class HostGraph:
+    """CSR + reverse CSR on the host: uint64 offsets, uint32 ids."""
+
+    def __init__(self, n, fw_off, fw_adj, bw_off=None, bw_adj=None):
+        self.n = int(n)
+        self.fw_off = np.ascontiguousarray(fw_off, dtype=np.uint64)
+        self.fw_adj = np.ascontiguousarray(fw_adj, dtype=np.uint32)
+        if bw_off is None:
+            bw_off, bw_adj = transpose(self.n, self.fw_off, self.fw_adj)
+        self.bw_off = np.ascontiguousarray(bw_off, dtype=np.uint64)
+        self.bw_adj = np.ascontiguousarray(bw_adj, dtype=np.uint32)
+
+    @property
+    def m(self):
+        return int(self.fw_off[-1]) if self.n else 0
+
+
+def csr_from_edges(n, src, dst) -> HostGraph:
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    dst = np.ascontiguousarray(dst, dtype=np.uint32)
+    off = np.zeros(n + 1, np.uint64)
+    adj = np.zeros(max(len(src), 1), np.uint32)
+    m = lib().orc_csr_from_edges(n, len(src), src, dst, off, adj)
+    return HostGraph(n, off, adj[:m].copy())
+
+
+def transpose(n, fw_off, fw_adj):
+    m = int(fw_off[-1]) if n else 0
+    bw_off = np.zeros(n + 1, np.uint64)
+    bw_adj = np.zeros(max(m, 1), np.uint32)
+    lib().orc_transpose(n, np.ascontiguousarray(fw_off, np.uint64), np.ascontiguousarray(fw_adj if m else np.zeros(1, np.uint32), np.uint32), bw_off, bw_adj)
+    return bw_off, bw_adj[:m].copy()
+
+
+def _nonempty(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a if a.size else np.zeros(1, dtype)
+
+
+def gen_gnm(seed, n, draws):
+    src = np.zeros(max(draws, 1), np.uint32)
+    dst = np.zeros(max(draws, 1), np.uint32)
+    k = lib().orc_gen_gnm(seed, n, draws, src, dst)
+    return src[:k], dst[:k]
+
+
+def gen_rmat(seed, scale, draws, a=0.57, b=0.19, c=0.19, permute=True):
+    src = np.zeros(max(draws, 1), np.uint32)
+    dst = np.zeros(max(draws, 1), np.uint32)
+    k = lib().orc_gen_rmat(seed, scale, draws, a, b, c, int(permute), src, dst)
+    return src[:k], dst[:k]
+
+
+def gen_lattice(seed, side, p_bidir=0.1):
+    cap = max(int(lib().orc_lattice_max_edges(side)), 1)
+    src = np.zeros(cap, np.uint32)
+    dst = np.zeros(cap, np.uint32)
+    k = lib().orc_gen_lattice(seed, side, p_bidir, src, dst)
+    return src[:k], dst[:k]
+
+
+def gen_fuzzy(seed, n, d=-1.0):
+    """Planted-SCC graph shaped like test/fuzzy.hpp; returns (HostGraph, labels)."""
+    cap = max(int(n * max(n - 1, 1)), 1) if n < 3000 else int(n * 64)
+    src = np.zeros(cap, np.uint32)
+    dst = np.zeros(cap, np.uint32)
+    labels = np.zeros(max(n, 1), np.uint32)
+    k = lib().orc_gen_fuzzy(seed, n, float(d), src, dst, cap, labels)
+    return csr_from_edges(n, src[:k], dst[:k]), labels[:n].copy()
+
+
+def gen_planted(seed, n, expo=2.1, max_size=10_000_000, intra_deg=4.0, inter_deg=3.0):
+    cap = int(n * (round(intra_deg) + round(inter_deg) + 1)) + 1
+    src = np.zeros(cap, np.uint32)
+    dst = np.zeros(cap, np.uint32)
+    labels = np.zeros(max(n, 1), np.uint32)
+    k = lib().orc_gen_planted(seed, n, expo, max_size, intra_deg, inter_deg, src, dst, cap, labels)
+    return csr_from_edges(n, src[:k], dst[:k]), labels[:n].copy()
+
+
+def planted_sizes(seed, n, expo=2.1, max_size=10_000_000):
+    sizes = np.zeros(max(n, 1), np.uint32)
+    k = lib().orc_planted_sizes(seed, n, expo, max_size, sizes, n)
+    return sizes[:k].copy()
+
+
+# --------------------------------------------------------------------------- SCC
+def tarjan(g: HostGraph) -> np.ndarray:
+    labels = np.zeros(max(g.n, 1), np.uint32)
+    lib().orc_tarjan(g.n, g.fw_off, _nonempty(g.fw_adj, np.uint32), labels)
+    return labels[: g.n].copy()
+
+
+def canonicalize(labels) -> np.ndarray:
+    lab = np.array(labels, dtype=np.uint32, copy=True)
+    if lab.size:
+        lib().orc_canonicalize(lab.size, lab)
+    return lab
+
+
+def brute_force(g: HostGraph) -> np.ndarray:
+    labels = np.zeros(max(g.n, 1), np.uint32)
+    rc = lib().orc_brute_force(g.n, g.fw_off, _nonempty(g.fw_adj, np.uint32), labels)
+    if rc:
+        raise ValueError("brute force is limited to n <= 4096")
+    return labels[: g.n].copy()
+
+
+def _pipeline(fn, g: HostGraph, *extra):
+    labels = np.full(max(g.n, 1), UNDECIDED, np.uint32)
+    st = SccStats()
+    rc = fn(g.n, g.fw_off, _nonempty(g.fw_adj, np.uint32), g.bw_off, _nonempty(g.bw_adj, np.uint32),
+            labels, *extra, C.byref(st))
+    if rc:
+        raise RuntimeError(f"oracle pipeline failed with {rc}")
+    return labels[: g.n].copy(), st
+
+
+def kaspan_scc(g: HostGraph, faithful=False):
+    """Restated kaspan::scc (scc/scc.hpp:21-87): raw ids, stats."""
+    return _pipeline(lib().orc_kaspan_scc, g, int(faithful))
+
+
+def scc_trim_ex_residual(g: HostGraph):
+    return _pipeline(lib().orc_scc_trim_ex_residual, g)
+
+
+def scc_trim_ex(g: HostGraph):
+    return _pipeline(lib().orc_scc_trim_ex, g)
+
+
+def trim_1_first(g: HostGraph):
+    live = np.ones(max(g.n, 1), np.uint8)
+    scc_id = np.full(max(g.n, 1), UNDECIDED, np.uint32)
+    pivot = lib().orc_trim_1_first(g.n, g.fw_off, g.bw_off, live, scc_id)
+    return int(pivot), live[: g.n].copy(), scc_id[: g.n].copy()
+
+
+def fwbw(g: HostGraph, live, pivot):
+    live = np.array(live, np.uint8, copy=True)
+    scc_id = np.full(max(g.n, 1), UNDECIDED, np.uint32)
+    d = lib().orc_fwbw(g.n, g.fw_off, _nonempty(g.fw_adj, np.uint32), g.bw_off, _nonempty(g.bw_adj, np.uint32), live, pivot, scc_id)
+    return int(d), live, scc_id[: g.n].copy()
+
+
+def color_round(g: HostGraph, live, rotation):
+    live = np.array(live, np.uint8, copy=True)
+    color = np.zeros(max(g.n, 1), np.uint32)
+    scc_id = np.full(max(g.n, 1), UNDECIDED, np.uint32)
+    d = lib().orc_color_round(g.n, g.fw_off, _nonempty(g.fw_adj, np.uint32), g.bw_off, _nonempty(g.bw_adj, np.uint32), live, rotation, color, scc_id)
+    return int(d), live, color[: g.n].copy(), scc_id[: g.n].copy()
+
+
+def subgraph(g: HostGraph, mask):
+    mask = np.ascontiguousarray(mask, dtype=np.uint8)
+    super_ids = np.zeros(max(g.n, 1), np.uint32)
+    sub_off = np.zeros(g.n + 1, np.uint64)
+    sub_adj = np.zeros(max(g.m, 1), np.uint32)
+    sn = C.c_uint32(0)
+    sm = lib().orc_subgraph(g.n, g.fw_off, _nonempty(g.fw_adj, np.uint32), mask, super_ids, C.byref(sn), sub_off, sub_adj)
+    return super_ids[: sn.value].copy(), sub_off[: sn.value + 1].copy(), sub_adj[:sm].copy()
+
+
+def pivot_features(g: HostGraph) -> np.ndarray:
+    feat = np.zeros((max(g.n, 1), 6), np.float32)
+    lib().orc_pivot_features(g.n, g.fw_off, _nonempty(g.fw_adj, np.uint32), g.bw_off, _nonempty(g.bw_adj, np.uint32), feat)
+    return feat[: g.n].copy()
+
+
+def mlp_forward(x, w1, b1, w2, b2, w3, b3):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(max(len(x), 1), np.float64)
+    lib().orc_mlp_forward(len(x), x, *(np.ascontiguousarray(a, np.float64) for a in (w1, b1, w2, b2, w3)), float(b3), out)
+    return out[: len(x)]
+
+
+def mlp_loss_grad(x, t, w1, b1, w2, b2, w3, b3):
+    x = np.ascontiguousarray(x, np.float32)
+    t = np.ascontiguousarray(t, np.float32)
+    g = [np.zeros(192), np.zeros(32), np.zeros(1024), np.zeros(32), np.zeros(32)]
+    gb3 = C.c_double(0)
+    loss = lib().orc_mlp_loss_grad(len(x), x, t, *(np.ascontiguousarray(a, np.float64) for a in (w1, b1, w2, b2, w3)), float(b3), *g, C.byref(gb3))
+    return loss, g + [gb3.value]
