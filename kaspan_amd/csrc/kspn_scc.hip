@@ -565,21 +565,20 @@ __global__ __launch_bounds__(WG) void k_canon_min(uint32_t n, const uint32_t* __
     uint32_t rep0 = __shfl(rep, 0);
     bool same = __all(!valid || rep == rep0);
     if (same) {
-      // min over lanes of v (lane 0 holds the smallest valid v of the wave)
-      if (lane_id() == 0 && valid && rep0 != v) atomicMin(&minid[rep0], v);
-    } else if (valid && rep != v) {
+      // lane 0 holds the smallest valid v of the wavefront
+      if (lane_id() == 0 && valid) atomicMin(&minid[rep0], v);
+    } else if (valid) {
       atomicMin(&minid[rep], v);
     }
   }
 }
 
-// labels[v] <- min(minid[rep], rep)
+// labels[v] <- minid[rep] (the smallest vertex carrying rep)
 __global__ void k_canon_apply(uint32_t n, uint32_t* __restrict__ labels, const uint32_t* __restrict__ minid) {
   for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG) {
     uint32_t rep = labels[v];
     if (rep >= n) continue;
-    uint32_t m = minid[rep];
-    labels[v] = m < rep ? m : rep;
+    labels[v] = minid[rep];
   }
 }
 
@@ -635,7 +634,7 @@ __global__ void k_tarjan(uint32_t n, const uint64_t* __restrict__ off, const uin
   uint32_t* low = work + n;
   uint32_t* st = work + 2 * (uint64_t)n;
   uint32_t* fu = work + 3 * (uint64_t)n;
-  uint64_t* fit = reinterpret_cast<uint64_t*>(work + 4 * (uint64_t)n);
+  uint64_t* fit = reinterpret_cast<uint64_t*>(work + ((4 * (uint64_t)n + 1) & ~1ull));  // 8-byte aligned
   const uint32_t ON = 0x80000000u;  // low[] high bit: vertex is on the stack
   for (uint32_t i = 0; i < n; ++i) index[i] = 0xFFFFFFFFu;
   uint32_t count = 0, sp = 0, fp = 0;
@@ -863,6 +862,10 @@ int ctx_reserve(Ctx* c, uint32_t n, uint64_t m) {
     c->cub_tmp = nullptr;
     if (hipMalloc(&c->cub_tmp, need) != hipSuccess) return KSPN_E_OOM;
     c->cub_bytes = need;
+  }
+  if (c->tj) {  // Tarjan work area is sized by cap_n; reallocated on demand
+    (void)hipFree(c->tj);
+    c->tj = nullptr;
   }
   c->cap_n = nn;
   c->cap_m = mm;

@@ -1,0 +1,252 @@
+"""GPU parity: libkspn through its C ABI against the CPU oracle and the
+reference's golden vectors. Integer work, so every comparison is bit-exact."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+C1_SEED = 0x6B5350AA00000000
+C2_SEED = 0x6B5350AA00000001
+
+METHODS = [
+    dict(method=0),
+    dict(method=1),
+    dict(method=1, flags=2),
+    dict(method=1, fwbw_rounds=4),
+    dict(method=1, residual_threshold=300),
+    dict(method=0, residual_threshold=0xFFFFFFFF),
+    dict(method=0, alpha=1e9, beta=1.0),   # pull levels as soon as possible
+    dict(method=0, alpha=1e-9),            # push levels only
+]
+MIDS = ["kaspan", "trim_ex", "trim_ex_no_trim2", "fwbw4", "residual300", "kaspan_residual_all", "pull", "push"]
+
+
+def host_graph(O, g):
+    fo, fa, bo, ba = g.to_host()
+    return O.HostGraph(g.n, fo, fa, bo, ba)
+
+
+def gpu_labels(ka, g, **opts):
+    labels, st = ka.scc(g, **opts)
+    return labels.cpu().numpy().view(np.uint32), st
+
+
+def test_ispan_tiny(ka, O):
+    g = ka.BidiGraph.from_host(7, H.ISPAN_TINY_FW_OFF, H.ISPAN_TINY_FW_ADJ, H.ISPAN_TINY_BW_OFF, H.ISPAN_TINY_BW_ADJ)
+    for opts in METHODS:
+        lab, st = gpu_labels(ka, g, **opts)
+        assert lab.tolist() == H.ISPAN_TINY_LABELS, opts
+
+
+def test_transpose_golden(ka):
+    bo, ba = ka.backward_complement_graph(6, np.array(H.SUB_FW_OFF, np.uint64), np.array(H.SUB_FW_ADJ, np.uint32))
+    assert bo.cpu().tolist() == H.SUB_BW_OFF
+    assert ba.cpu().tolist() == H.SUB_BW_ADJ
+
+
+def test_subgraph_golden(ka):
+    import torch
+
+    mask = torch.tensor(H.SUB_MASK, dtype=torch.uint8)
+    sid, so, sa = ka.allgather_fw_sub_graph(6, np.array(H.SUB_FW_OFF, np.uint64), np.array(H.SUB_FW_ADJ, np.uint32), mask)
+    assert sid.cpu().tolist() == H.SUB_SUPER_IDS
+    assert so.cpu().tolist() == H.SUB_SUB_FW_OFF and sa.cpu().tolist() == H.SUB_SUB_FW_ADJ
+    sid, so, sa = ka.allgather_fw_sub_graph(6, np.array(H.SUB_BW_OFF, np.uint64), np.array(H.SUB_BW_ADJ, np.uint32), mask)
+    assert so.cpu().tolist() == H.SUB_SUB_BW_OFF and sa.cpu().tolist() == H.SUB_SUB_BW_ADJ
+
+
+def test_subgraph_random_vs_oracle(ka, O):
+    import torch
+
+    s, d = O.gen_gnm(21, 3000, 12000)
+    hg = O.csr_from_edges(3000, s, d)
+    mask = (np.random.default_rng(1).random(3000) < 0.3).astype(np.uint8)
+    sid, so, sa = ka.allgather_fw_sub_graph(3000, hg.fw_off, hg.fw_adj, torch.from_numpy(mask))
+    esid, eso, esa = O.subgraph(hg, mask)
+    assert np.array_equal(sid.cpu().numpy().view(np.uint32), esid)
+    assert np.array_equal(so.cpu().numpy().view(np.uint64), eso)
+    assert np.array_equal(sa.cpu().numpy().view(np.uint32), esa)
+
+
+def test_generators_match_oracle(ka, O):
+    for case in json.load(open(os.path.join(GOLDEN, "generators.json"))):
+        kw = {k: case[k] for k in ("n", "draws", "scale", "side", "p_bidir", "permute") if k in case}
+        g = ka.generate(case["kind"], case["seed"], **kw)
+        assert g.n == case["n_vertices"] and g.m == case["m"], case["kind"]
+        fo, fa, bo, ba = g.to_host()
+        assert H.csr_digest(fo, fa) == case["fw_digest"], case
+        assert H.csr_digest(bo, ba) == case["bw_digest"], case
+
+
+def test_build_csr_from_edges(ka, O):
+    rng = np.random.default_rng(5)
+    src = rng.integers(0, 500, 4000).astype(np.uint32)
+    dst = rng.integers(0, 500, 4000).astype(np.uint32)
+    g = ka.BidiGraph.from_edges(500, src, dst)
+    hg = O.csr_from_edges(500, src, dst)
+    fo, fa, bo, ba = g.to_host()
+    assert np.array_equal(fo, hg.fw_off) and np.array_equal(fa, hg.fw_adj)
+    assert np.array_equal(bo, hg.bw_off) and np.array_equal(ba, hg.bw_adj)
+
+
+def test_tarjan_kernel(ka, O):
+    for seed, n, draws in [(1, 7, 0), (2, 2000, 2500), (3, 5000, 9000)]:
+        if draws == 0:
+            hg = O.HostGraph(7, H.ISPAN_TINY_FW_OFF, H.ISPAN_TINY_FW_ADJ)
+        else:
+            s, d = O.gen_gnm(seed, n, draws)
+            hg = O.csr_from_edges(n, s, d)
+        lab = ka.tarjan(hg.n, hg.fw_off, hg.fw_adj)
+        assert np.array_equal(lab.cpu().numpy().view(np.uint32), O.tarjan(hg))
+
+
+def test_canonicalize_kernel(ka, O):
+    import torch
+
+    rng = np.random.default_rng(9)
+    for n, k in [(1, 1), (100, 7), (100000, 300), (300000, 3)]:
+        reps = rng.choice(n, size=k, replace=False).astype(np.uint32)
+        lab = reps[rng.integers(0, k, n)]
+        t = torch.from_numpy(lab.view(np.int32)).cuda()
+        ka.normalise_scc_id(t)
+        assert np.array_equal(t.cpu().numpy().view(np.uint32), O.canonicalize(lab))
+
+
+@pytest.mark.parametrize("n", [1, 6, 8, 16, 33, 64, 300])
+def test_fuzzy_grid(ka, O, n):
+    """test_mpi/test_scc_fuzzy.cpp:31-80 grid (fixed seeds) for every method"""
+    for run in range(40 if n < 300 else 8):
+        hg, planted = O.gen_fuzzy(1000 * n + run, n)
+        g = ka.BidiGraph.from_host(n, hg.fw_off, hg.fw_adj, hg.bw_off, hg.bw_adj)
+        for opts in METHODS:
+            lab, st = gpu_labels(ka, g, **opts)
+            assert np.array_equal(lab, planted), (n, run, opts)
+
+
+@pytest.mark.parametrize("opts", METHODS, ids=MIDS)
+@pytest.mark.parametrize("kind", ["gnm_sparse", "gnm_dense", "rmat14", "rmat16", "lattice24", "planted"])
+def test_random_graphs(ka, O, kind, opts):
+    if kind == "gnm_sparse":
+        g = ka.generate("gnm", 11, n=30000, draws=33000)
+    elif kind == "gnm_dense":
+        g = ka.generate("gnm", 12, n=20000, draws=160000)
+    elif kind == "rmat14":
+        g = ka.generate("rmat", 13, scale=14, draws=16 << 14)
+    elif kind == "rmat16":
+        g = ka.generate("rmat", 14, scale=16, draws=16 << 16)
+    elif kind == "lattice24":
+        g = ka.generate("lattice", 15, side=24, p_bidir=0.1)
+    else:
+        hg0, _ = O.gen_planted(16, 40000)
+        g = ka.BidiGraph.from_host(hg0.n, hg0.fw_off, hg0.fw_adj, hg0.bw_off, hg0.bw_adj)
+    hg = host_graph(O, g)
+    lab, st = gpu_labels(ka, g, **opts)
+    assert np.array_equal(lab, O.tarjan(hg))
+    total = st.decided_trim + st.decided_trim2 + st.decided_fwbw + st.decided_color + st.decided_residual
+    assert total == g.n
+
+
+def test_c1_gnm_full(ka, O):
+    """config 1: G(n=65536, m=524288), degree-product pivot, oracle = Tarjan"""
+    g = ka.generate("gnm", C1_SEED, n=65536, draws=524288)
+    hg = host_graph(O, g)
+    ref_raw, ref_st = O.kaspan_scc(hg)
+    tar = O.tarjan(hg)
+    assert np.array_equal(O.canonicalize(ref_raw), tar)
+    lab, st = gpu_labels(ka, g, method=0)
+    assert np.array_equal(lab, tar)
+    # stage parity with the restated reference: same pivot, same first trim, same FW-BW set
+    assert st.pivot == ref_st.pivot
+    assert st.decided_fwbw == ref_st.decided_fwbw
+    lab2, _ = gpu_labels(ka, g, method=1)
+    assert np.array_equal(lab2, tar)
+
+
+def test_stage_parity_kaspan(ka, O):
+    for seed in range(4):
+        g = ka.generate("rmat", 100 + seed, scale=13, draws=8 << 13)
+        hg = host_graph(O, g)
+        _, ref_st = O.kaspan_scc(hg)
+        lab, st = gpu_labels(ka, g, method=0)
+        assert st.pivot == ref_st.pivot
+        assert st.decided_fwbw == ref_st.decided_fwbw
+        assert np.array_equal(lab, O.tarjan(hg))
+
+
+def test_edge_cases(ka, O):
+    import torch
+
+    # n = 0
+    g = ka.BidiGraph.from_host(0, [0], [], [0], [])
+    lab, st = ka.scc(g)
+    assert lab.numel() == 0
+    # isolated vertices
+    g = ka.BidiGraph.from_host(5, [0] * 6, [], [0] * 6, [])
+    assert gpu_labels(ka, g)[0].tolist() == [0, 1, 2, 3, 4]
+    # self-loops only: singletons
+    off = np.arange(6, dtype=np.uint64)
+    g = ka.BidiGraph.from_host(5, off, np.arange(5, dtype=np.uint32), off, np.arange(5, dtype=np.uint32))
+    for opts in METHODS:
+        assert gpu_labels(ka, g, **opts)[0].tolist() == [0, 1, 2, 3, 4]
+    # a long directed cycle and a long path
+    for n in (2, 1000):
+        off, adj = H.cycle(n)
+        g = ka.BidiGraph.from_host(n, off, adj)
+        for opts in METHODS:
+            assert set(gpu_labels(ka, g, **opts)[0].tolist()) == {0}
+    off, adj = H.csr_from_pairs(500, [(i, i + 1) for i in range(499)])
+    g = ka.BidiGraph.from_host(500, off, adj)
+    for opts in METHODS:
+        assert gpu_labels(ka, g, **opts)[0].tolist() == list(range(500))
+    # bad arguments
+    with pytest.raises(ka.KspnError):
+        ka.scc(g, torch.empty(3, dtype=torch.int32, device="cuda"))
+
+
+def test_iteration_cap(ka, O):
+    # a forward DAG with every vertex live after trim-1's one pass needs many colour rounds
+    hg0, _ = O.gen_planted(77, 5000)
+    g = ka.BidiGraph.from_host(hg0.n, hg0.fw_off, hg0.fw_adj, hg0.bw_off, hg0.bw_adj)
+    with pytest.raises(ka.KspnError) as ei:
+        ka.scc(g, method=0, max_rounds=1)
+    assert ei.value.code == -4
+    lab, _ = gpu_labels(ka, g, method=0, max_rounds=0)
+    assert np.array_equal(lab, O.tarjan(hg0))
+
+
+def test_profile_stats(ka, O):
+    g = ka.generate("rmat", 3, scale=15, draws=16 << 15)
+    lab, st = ka.scc(g, profile=True)
+    d = st.as_dict()
+    names = {k["name"] for k in d["kernels"]}
+    assert "trim_first" in names and "canonicalize" in names
+    assert all(k["ms"] >= 0 and k["launches"] > 0 for k in d["kernels"])
+    assert st.ms_total > 0
+
+
+def test_c2_rmat24_full(ka, O):
+    """config 2 at full size: R-MAT scale 24, edge factor 16, permuted ids;
+    labels equal the oracle's Tarjan labels bit for bit."""
+    g = ka.generate("rmat", C2_SEED, scale=24, draws=16 << 24)
+    lab, st = gpu_labels(ka, g, method=0)
+    hg = host_graph(O, g)
+    tar = O.tarjan(hg)
+    assert np.array_equal(lab, tar)
+    lab2, _ = gpu_labels(ka, g, method=1)
+    assert np.array_equal(lab2, tar)
+
+
+def test_cpp_mirror(ka):
+    """include/kspn/kaspan.hpp from C++: golden labels and error -> exception"""
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "test_scc_cpp")
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "cpp ok" in out.stdout
