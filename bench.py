@@ -138,6 +138,8 @@ def main():
         "phases_ms": {k: getattr(stats, k) for k in ("ms_trim", "ms_fwbw", "ms_color", "ms_residual", "ms_canon")},
         "decided": {k: getattr(stats, k) for k in ("decided_trim", "decided_trim2", "decided_fwbw", "decided_color",
                                                    "decided_residual")},
+        "solve": {k: getattr(stats, k) for k in ("pivot", "fwbw_levels_push", "fwbw_levels_pull", "color_rounds",
+                                                 "color_iters", "edges_scanned", "residual_n")},
         "kernels": kernels[:8],
     }
     print(json.dumps(out))

@@ -76,6 +76,104 @@ __device__ inline void wave_count(bool pred, uint32_t* count) {
   if (lane == leader) atomicAdd(count, (uint32_t)__popcll(mask));
 }
 
+// ---------------------------------------------------------------- reductions
+__device__ inline uint32_t wave_sum(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  return v;  // valid in lane 0
+}
+__device__ inline uint32_t wave_min(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_down(v, off));
+  return v;
+}
+__device__ inline uint64_t wave_sum64(uint64_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  return v;
+}
+// block-wide 64-bit sum; every thread calls; result in thread 0
+__device__ inline uint64_t block_sum64(uint64_t v, uint64_t* s_red) {
+  v = wave_sum64(v);
+  if (lane_id() == 0) s_red[threadIdx.x / WAVE] = v;
+  __syncthreads();
+  uint64_t r = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < WG / WAVE; ++w) r += s_red[w];
+  __syncthreads();
+  return r;
+}
+
+// block-wide sum / min; every thread of the block calls; result in thread 0
+__device__ inline uint32_t block_sum(uint32_t v, uint32_t* s_red) {
+  v = wave_sum(v);
+  if (lane_id() == 0) s_red[threadIdx.x / WAVE] = v;
+  __syncthreads();
+  uint32_t r = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < WG / WAVE; ++w) r += s_red[w];
+  __syncthreads();
+  return r;
+}
+__device__ inline uint32_t block_min(uint32_t v, uint32_t* s_red) {
+  v = wave_min(v);
+  if (lane_id() == 0) s_red[threadIdx.x / WAVE] = v;
+  __syncthreads();
+  uint32_t r = 0xFFFFFFFFu;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < WG / WAVE; ++w) r = min(r, s_red[w]);
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------- LDS-staged queue
+// Appends are collected in LDS (one LDS atomic per wavefront) and written
+// out with one global atomicAdd per flush, so a launch issues a few
+// thousand global atomics instead of one per wavefront.
+struct BlockQueue {
+  uint32_t* buf;  // __shared__, cap entries
+  uint32_t* n;    // __shared__
+  uint32_t* base; // __shared__
+  uint32_t cap;
+};
+
+__device__ inline void bq_init(BlockQueue& q) {
+  if (threadIdx.x == 0) *q.n = 0;
+  __syncthreads();
+}
+
+__device__ inline void bq_push(BlockQueue& q, bool pred, uint32_t val) {
+  uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  uint32_t lane = lane_id();
+  uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t b = 0;
+  if (lane == leader) b = atomicAdd(q.n, (uint32_t)__popcll(mask));
+  b = __shfl(b, leader);
+  if (pred) {
+    uint64_t below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
+    q.buf[b + __popcll(below)] = val;
+  }
+}
+
+// every thread of the block calls; writes the staged entries to queue
+__device__ inline void bq_flush(BlockQueue& q, uint32_t* queue, uint32_t* count) {
+  __syncthreads();
+  uint32_t cnt = *q.n;
+  if (cnt) {
+    if (threadIdx.x == 0) *q.base = atomicAdd(count, cnt);
+    __syncthreads();
+    uint32_t b = *q.base;
+    for (uint32_t i = threadIdx.x; i < cnt; i += WG) queue[b + i] = q.buf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) *q.n = 0;
+  }
+  __syncthreads();
+}
+
+// flush when fewer than `room` free slots remain (block-uniform decision)
+__device__ inline void bq_maybe_flush(BlockQueue& q, uint32_t* queue, uint32_t* count, uint32_t room) {
+  __syncthreads();
+  if (*q.n + room > q.cap) bq_flush(q, queue, count);
+}
+
 __device__ inline uint32_t rotr32(uint32_t x, uint32_t r) { r &= 31; return r ? (x >> r) | (x << (32 - r)) : x; }
 __device__ inline uint32_t rotl32(uint32_t x, uint32_t r) { r &= 31; return r ? (x << r) | (x >> (32 - r)) : x; }
 
@@ -89,6 +187,9 @@ enum Counter : int {
   C_TRIM2,
   C_RES_N,
   C_FLAG,
+  C_MIN,   // minimum vertex of a decided set
+  C_FSEG,  // size of the current frontier segment
+  C_DTOT,  // running length of the decided list
   C_COUNT_ = 16
 };
 
@@ -111,9 +212,10 @@ struct Ctx {
   uint64_t cap_m = 0;
   // bitmaps (bm_words(n) words each)
   uint32_t* live = nullptr;  // 1 = undecided
-  uint32_t* fwb = nullptr;   // forward blocked: decided or reached
-  uint32_t* bwb = nullptr;   // backward blocked
+  uint32_t* fwv = nullptr;   // forward-reached from the pivot
+  uint32_t* bwv = nullptr;   // backward-reached (inside fwv)
   uint32_t* inq = nullptr;   // queue membership
+  uint32_t* fnew = nullptr;  // vertices reached by the last pull level
   // vertex arrays
   uint32_t* qa = nullptr;
   uint32_t* qb = nullptr;
@@ -124,7 +226,10 @@ struct Ctx {
   uint64_t* scan = nullptr;   // exclusive scan of deg (n+1)
   uint64_t* part = nullptr;   // per-workgroup partials
   uint32_t* cnt = nullptr;    // device counters
-  uint32_t* h_cnt = nullptr;  // pinned mirror
+  unsigned long long* cnt64 = nullptr;  // device 64-bit counters (edges scanned)
+  uint32_t* h_cnt = nullptr;  // host copy of the published counters
+  volatile uint32_t* h_map = nullptr;  // pinned, device-mapped: counters published by k_publish
+  uint32_t* d_map = nullptr;  // device address of h_map
   uint64_t* h_u64 = nullptr;  // pinned scratch for 64-bit reads
   void* cub_tmp = nullptr;
   size_t cub_bytes = 0;

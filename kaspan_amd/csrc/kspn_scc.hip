@@ -34,20 +34,34 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t count, uint32_t v) {
   for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i < count; i += (uint64_t)gridDim.x * WG) p[i] = v;
 }
 
-// words of `out` = op(a, b); op 0: ~a ; op 1: ~(a & b)
-__global__ void k_bm_not(uint32_t* out, const uint32_t* a, uint64_t words) {
-  for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i < words; i += (uint64_t)gridDim.x * WG) out[i] = ~a[i];
-}
-__global__ void k_bm_nand(uint32_t* out, const uint32_t* a, const uint32_t* b, uint64_t words) {
-  for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i < words; i += (uint64_t)gridDim.x * WG) out[i] = ~(a[i] & b[i]);
-}
-
 // *p = v (small device writes without pageable-memory copies)
 __global__ void k_set_u32(uint32_t* p, uint32_t v) {
   if (threadIdx.x == 0) *p = v;
 }
 __global__ void k_copy_u32(uint32_t* dst, const uint32_t* src) {
   if (threadIdx.x == 0) *dst = *src;
+}
+
+// counters -> host-mapped pinned memory (read by the host after a stream
+// sync): 16 u32 counters, then two u64 slots (*extra, cnt64[0])
+__global__ void k_publish(const uint32_t* cnt, const unsigned long long* cnt64, const uint64_t* extra, uint32_t* map) {
+  uint32_t t = threadIdx.x;
+  if (t < C_COUNT_) map[t] = cnt[t];
+  if (t == 32) {
+    uint64_t x = extra ? *extra : 0;
+    map[16] = (uint32_t)x;
+    map[17] = (uint32_t)(x >> 32);
+  }
+  if (t == 33) {
+    uint64_t y = cnt64[0];
+    map[18] = (uint32_t)y;
+    map[19] = (uint32_t)(y >> 32);
+  }
+  if (t == 34) {
+    uint64_t y = cnt64[1];
+    map[20] = (uint32_t)y;
+    map[21] = (uint32_t)(y >> 32);
+  }
 }
 
 // set bit v of bm
@@ -102,8 +116,8 @@ __global__ void k_frontier_deg(const uint32_t* __restrict__ F, const uint32_t* f
 // =====================================================================
 
 enum ExpandOp : int {
-  OP_FW = 0,        // claim fwb bit of the target (forward reachability)
-  OP_BW = 1,        // claim bwb bit of the target, label it with the pivot
+  OP_FW = 0,        // live target not yet reached forward: claim its fwv bit
+  OP_BW = 1,        // forward-reached target not yet reached backward: claim its bwv bit
   OP_NEIGH = 2,     // collect live targets once (trim candidates)
   OP_COLOR = 3,     // push colour max into live targets
   OP_COLOR_BW = 4,  // backward closure: decide live in-neighbours of the same colour
@@ -111,22 +125,23 @@ enum ExpandOp : int {
 
 struct OpArgs {
   uint32_t* live;
-  uint32_t* fwb;
-  uint32_t* bwb;
+  uint32_t* fwv;  // forward-reached
+  uint32_t* bwv;  // backward-reached (within fwv)
   uint32_t* inq;
   uint32_t* col;
   uint32_t* labels;
-  uint32_t value;  // pivot or rotation
+  uint32_t value;                // pivot or rotation
+  const uint64_t* noff;          // OP_FW/OP_BW: offsets whose degree the next level expands
+  unsigned long long* nsum;      // OP_FW/OP_BW: sum of those degrees over claimed targets
 };
 
 template <int OP>
-__device__ inline void expand_apply(uint32_t u, uint32_t w, const OpArgs& a, uint32_t* next, uint32_t* next_cnt) {
+__device__ inline void expand_apply(uint32_t u, uint32_t w, const OpArgs& a, BlockQueue& q, uint64_t& nsum) {
   bool push = false;
   if constexpr (OP == OP_FW) {
-    push = bm_claim(a.fwb, w);
+    push = bm_get(a.live, w) && bm_claim(a.fwv, w);
   } else if constexpr (OP == OP_BW) {
-    push = bm_claim(a.bwb, w);
-    if (push) a.labels[w] = a.value;
+    push = bm_get(a.fwv, w) && bm_claim(a.bwv, w);
   } else if constexpr (OP == OP_NEIGH) {
     push = bm_get(a.live, w) && bm_claim(a.inq, w);
   } else if constexpr (OP == OP_COLOR) {
@@ -143,7 +158,10 @@ __device__ inline void expand_apply(uint32_t u, uint32_t w, const OpArgs& a, uin
       push = true;
     }
   }
-  wave_append(push, w, next, next_cnt);
+  if constexpr (OP == OP_FW || OP == OP_BW) {
+    if (push) nsum += a.noff[w + 1] - a.noff[w];
+  }
+  bq_push(q, push, w);
 }
 
 constexpr int EXP_TILE = WG * 8;  // edges per workgroup tile
@@ -164,6 +182,13 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
   __shared__ uint64_t s_beg[EXP_WIN];
   __shared__ uint32_t s_v[EXP_WIN];
   __shared__ uint32_t s_i0, s_w;
+  __shared__ uint32_t s_out[2 * EXP_TILE];
+  __shared__ uint32_t s_on, s_ob;
+  __shared__ unsigned long long s_nsum;
+  BlockQueue q{s_out, &s_on, &s_ob, 2 * EXP_TILE};
+  if (threadIdx.x == 0) s_nsum = 0;
+  bq_init(q);
+  uint64_t nsum = 0;
   const uint32_t f = *fcount;
   if (f == 0) return;
   const uint64_t total = S[f];
@@ -205,7 +230,7 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
         }
         uint32_t u = s_v[lo];
         uint32_t x = adj[s_beg[lo] + (e - s_S[lo])];
-        expand_apply<OP>(u, x, a, next, next_cnt);
+        expand_apply<OP>(u, x, a, q, nsum);
       }
     } else {
       // window larger than LDS (many zero-degree frontier entries): search the scan in global memory
@@ -218,10 +243,16 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
         }
         uint32_t u = F[lo];
         uint32_t x = adj[off[u] + (e - S[lo])];
-        expand_apply<OP>(u, x, a, next, next_cnt);
+        expand_apply<OP>(u, x, a, q, nsum);
       }
     }
+    bq_maybe_flush(q, next, next_cnt, EXP_TILE);  // a tile appends at most EXP_TILE entries
+  }
+  bq_flush(q, next, next_cnt);
+  if constexpr (OP == OP_FW || OP == OP_BW) {
+    if (nsum) atomicAdd(&s_nsum, (unsigned long long)nsum);
     __syncthreads();
+    if (threadIdx.x == 0 && s_nsum) atomicAdd(a.nsum, s_nsum);
   }
 }
 
@@ -234,10 +265,11 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
 // ballot). Each workgroup reduces its (max in*out, min id) pair.
 __global__ __launch_bounds__(WG) void k_trim_first(uint32_t n, const uint64_t* __restrict__ fw_off,
                                                    const uint64_t* __restrict__ bw_off, uint32_t* __restrict__ live,
-                                                   uint32_t* __restrict__ labels, uint64_t* __restrict__ part,
-                                                   uint32_t* cnt) {
+                                                   uint32_t* __restrict__ labels, uint64_t* __restrict__ part) {
   __shared__ uint64_t s_p[WG];
   __shared__ uint32_t s_id[WG];
+  __shared__ uint32_t s_red[WG / WAVE];
+  uint32_t dead = 0;
   uint64_t best = 0;
   uint32_t best_id = 0xFFFFFFFFu;
   const uint64_t n_pad = ((uint64_t)n + WG - 1) / WG * WG;
@@ -260,8 +292,9 @@ __global__ __launch_bounds__(WG) void k_trim_first(uint32_t n, const uint64_t* _
     uint64_t word0 = (base + threadIdx.x - lane) >> 5;
     if (lane == 0) live[word0] = (uint32_t)mask;
     if (lane == 32) live[word0 + 1] = (uint32_t)(mask >> 32);
-    wave_count(v64 < n && !is_live, &cnt[C_DECIDED]);
+    dead += (v64 < n && !is_live);
   }
+  uint32_t dead_sum = block_sum(dead, s_red);
   s_p[threadIdx.x] = best;
   s_id[threadIdx.x] = best_id;
   __syncthreads();
@@ -277,22 +310,30 @@ __global__ __launch_bounds__(WG) void k_trim_first(uint32_t n, const uint64_t* _
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = s_p[0];
-    part[2 * blockIdx.x + 1] = s_id[0];
+    part[3 * blockIdx.x] = s_p[0];
+    part[3 * blockIdx.x + 1] = s_id[0];
+    part[3 * blockIdx.x + 2] = dead_sum;
   }
 }
 
-// (max product, smallest id) over workgroup partials; id 0xFFFFFFFF if none
-__global__ __launch_bounds__(WG) void k_pivot_reduce(const uint64_t* part, uint32_t nparts, uint32_t* out) {
+// (max product, smallest id) over workgroup partials -> out[0] (0xFFFFFFFF
+// if none); summed partial counts -> *count_out
+__global__ __launch_bounds__(WG) void k_pivot_reduce(const uint64_t* part, uint32_t nparts, uint32_t* out,
+                                                     uint32_t* count_out) {
   __shared__ uint64_t s_p[WG];
   __shared__ uint32_t s_id[WG];
+  __shared__ uint32_t s_red[WG / WAVE];
   uint64_t best = 0;
   uint32_t best_id = 0xFFFFFFFFu;
+  uint32_t total = 0;
   for (uint32_t i = threadIdx.x; i < nparts; i += WG) {
-    uint64_t p = part[2 * i];
-    uint32_t id = (uint32_t)part[2 * i + 1];
+    uint64_t p = part[3 * i];
+    uint32_t id = (uint32_t)part[3 * i + 1];
+    total += (uint32_t)part[3 * i + 2];
     if (id != 0xFFFFFFFFu && (p > best || (p == best && id < best_id))) { best = p; best_id = id; }
   }
+  total = block_sum(total, s_red);
+  if (threadIdx.x == 0 && count_out) *count_out = total;
   s_p[threadIdx.x] = best;
   s_id[threadIdx.x] = best_id;
   __syncthreads();
@@ -315,67 +356,154 @@ __global__ __launch_bounds__(WG) void k_pivot_reduce(const uint64_t* part, uint3
 // pull (bottom-up) reachability levels
 // =====================================================================
 
-// forward: a live unreached v is reached if any in-neighbour is reached
-// (reached = fwb & live); claims are visible within the sweep
-__global__ __launch_bounds__(WG) void k_pull_fw(uint32_t n, const uint32_t* __restrict__ live, uint32_t* fwb,
-                                                const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
-                                                uint32_t* __restrict__ next, uint32_t* next_cnt) {
+// Pull levels over all vertices (grid-stride, no barrier inside the loop).
+// Forward (DIR 0): a live v not yet reached is reached when any in-neighbour
+// is. Backward (DIR 1): a forward-reached v not yet reached backward is
+// reached when any out-neighbour is. One random bit per scanned edge; claims
+// made earlier in the same sweep are visible. Newly reached vertices are
+// written as a bitmap (each wavefront owns its two words, no atomics); per
+// workgroup partials: claimed count, edges scanned, next-level degree sum
+// (noff) of the claimed vertices.
+template <int DIR>
+__global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restrict__ cand, uint32_t* vis,
+                                             const uint64_t* __restrict__ roff, const uint32_t* __restrict__ radj,
+                                             const uint64_t* __restrict__ noff, uint32_t* __restrict__ fnew,
+                                             uint64_t* __restrict__ part) {
+  __shared__ uint64_t s_red[WG / WAVE];
+  uint64_t claimed = 0, edges = 0, nd = 0;
   const uint64_t n_pad = ((uint64_t)n + WG - 1) / WG * WG;
+  const uint32_t lane = lane_id();
   for (uint64_t base = (uint64_t)blockIdx.x * WG; base < n_pad; base += (uint64_t)gridDim.x * WG) {
-    uint64_t v64 = base + threadIdx.x;
+    const uint64_t v64 = base + threadIdx.x;
+    const uint64_t word = v64 >> 5;  // this lane's bitmap word: owned by this wavefront
+    const uint32_t vis_word = (v64 < n) ? vis[word] : 0xFFFFFFFFu;
     bool hit = false;
     if (v64 < n) {
-      uint32_t v = (uint32_t)v64;
-      if (!bm_get(fwb, v)) {
-        for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it) {
-          uint32_t u = bw_adj[it];
-          if (bm_get(fwb, u) && bm_get(live, u)) { hit = true; break; }
-        }
-        if (hit) hit = bm_claim(fwb, v);
-      }
-    }
-    wave_append(hit, (uint32_t)v64, next, next_cnt);
-  }
-}
-
-// backward: a reached, not yet backward-visited v (bwb bit 0) is visited if
-// any out-neighbour is visited (bwb & fwb & live); it joins the pivot's SCC
-__global__ __launch_bounds__(WG) void k_pull_bw(uint32_t n, const uint32_t* __restrict__ live,
-                                                const uint32_t* __restrict__ fwb, uint32_t* bwb,
-                                                const uint64_t* __restrict__ fw_off, const uint32_t* __restrict__ fw_adj,
-                                                uint32_t* __restrict__ labels, uint32_t pivot,
-                                                uint32_t* __restrict__ next, uint32_t* next_cnt) {
-  const uint64_t n_pad = ((uint64_t)n + WG - 1) / WG * WG;
-  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < n_pad; base += (uint64_t)gridDim.x * WG) {
-    uint64_t v64 = base + threadIdx.x;
-    bool hit = false;
-    if (v64 < n) {
-      uint32_t v = (uint32_t)v64;
-      if (!bm_get(bwb, v)) {
-        for (uint64_t it = fw_off[v], end = fw_off[v + 1]; it < end; ++it) {
-          uint32_t w = fw_adj[it];
-          if (bm_get(bwb, w) && bm_get(fwb, w) && bm_get(live, w)) { hit = true; break; }
+      const uint32_t v = (uint32_t)v64;
+      const uint32_t bit = 1u << (v & 31);
+      if ((cand[word] & bit) && !(vis_word & bit)) {
+        for (uint64_t it = roff[v], end = roff[v + 1]; it < end; ++it) {
+          ++edges;
+          if (bm_get(vis, radj[it])) { hit = true; break; }
         }
         if (hit) {
-          hit = bm_claim(bwb, v);
-          if (hit) labels[v] = pivot;
+          ++claimed;
+          nd += noff[v + 1] - noff[v];
         }
       }
     }
-    wave_append(hit, (uint32_t)v64, next, next_cnt);
+    // only this wavefront sets bits in its two words during a pull level, so
+    // the claim is a plain store of (old | new) per word, readable by others
+    const uint64_t mask = __ballot(hit);
+    const uint64_t word0 = (base + threadIdx.x - lane) >> 5;
+    if (lane == 0) {
+      fnew[word0] = (uint32_t)mask;
+      if ((uint32_t)mask) vis[word0] = vis_word | (uint32_t)mask;
+    }
+    if (lane == 32) {
+      fnew[word0 + 1] = (uint32_t)(mask >> 32);
+      if ((uint32_t)(mask >> 32)) vis[word0 + 1] = vis_word | (uint32_t)(mask >> 32);
+    }
+  }
+  claimed = block_sum64(claimed, s_red);
+  edges = block_sum64(edges, s_red);
+  nd = block_sum64(nd, s_red);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = claimed;
+    part[3 * blockIdx.x + 1] = edges;
+    part[3 * blockIdx.x + 2] = nd;
   }
 }
 
-// after FW-BW: scc = bwb & fwb & live; live &= ~scc; count
-__global__ void k_fwbw_finish(uint32_t* live, const uint32_t* fwb, const uint32_t* bwb, uint64_t words, uint32_t* cnt) {
-  uint32_t local = 0;
-  for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i < words; i += (uint64_t)gridDim.x * WG) {
-    uint32_t l = live[i];
-    uint32_t scc = bwb[i] & fwb[i] & l;
-    live[i] = l & ~scc;
-    local += __popc(scc);
+// sums of the three per-workgroup partials: [0] -> *count, [1] -> cnt64[0], [2] -> cnt64[1]
+__global__ __launch_bounds__(WG) void k_reduce3(const uint64_t* __restrict__ part, uint32_t nparts, uint32_t* count,
+                                                unsigned long long* cnt64) {
+  __shared__ uint64_t s_red[WG / WAVE];
+  uint64_t a = 0, b = 0, d = 0;
+  for (uint32_t i = threadIdx.x; i < nparts; i += WG) {
+    a += part[3 * i];
+    b += part[3 * i + 1];
+    d += part[3 * i + 2];
   }
-  if (local) atomicAdd(&cnt[C_DECIDED], local);
+  a = block_sum64(a, s_red);
+  b = block_sum64(b, s_red);
+  d = block_sum64(d, s_red);
+  if (threadIdx.x == 0) {
+    *count = (uint32_t)a;
+    cnt64[0] = b;
+    cnt64[1] = d;
+  }
+}
+
+// degrees of a frontier of at most 4096 vertices and their exclusive scan
+// (S[f] = total) in one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void k_deg_scan_small(const uint32_t* __restrict__ F, const uint32_t* fcount,
+                                                         const uint64_t* __restrict__ off, uint64_t* __restrict__ S) {
+  __shared__ uint64_t s[1024];
+  const uint32_t f = *fcount;
+  const uint32_t t = threadIdx.x;
+  uint64_t d[4];
+  uint64_t sum = 0;
+  for (int k = 0; k < 4; ++k) {
+    uint32_t i = 4 * t + k;
+    d[k] = (i < f) ? off[F[i] + 1] - off[F[i]] : 0;
+    sum += d[k];
+  }
+  s[t] = sum;
+  __syncthreads();
+  for (uint32_t o = 1; o < 1024; o <<= 1) {
+    uint64_t x = (t >= o) ? s[t - o] : 0;
+    __syncthreads();
+    s[t] += x;
+    __syncthreads();
+  }
+  uint64_t run = s[t] - sum;  // exclusive prefix of this thread's block
+  for (int k = 0; k < 4; ++k) {
+    uint32_t i = 4 * t + k;
+    if (i <= f) S[i] = run;
+    run += d[k];
+  }
+}
+
+// zero a queue counter and the 64-bit level counters
+__global__ void k_level_reset(uint32_t* cnt_slot, unsigned long long* cnt64) {
+  if (threadIdx.x == 0) {
+    *cnt_slot = 0;
+    cnt64[0] = 0;
+    cnt64[1] = 0;
+  }
+}
+
+// after FW-BW: scc = bwv (backward-reached within forward-reached);
+// live &= ~scc; per-workgroup partials of |scc| and of its smallest vertex
+__global__ __launch_bounds__(WG) void k_fwbw_finish(uint32_t* live, const uint32_t* bwv, uint64_t words, uint64_t* part) {
+  __shared__ uint32_t s_red[WG / WAVE];
+  uint32_t count = 0, mn = 0xFFFFFFFFu;
+  const uint64_t w_pad = (words + WG - 1) / WG * WG;
+  for (uint64_t base = blockIdx.x * (uint64_t)WG; base < w_pad; base += (uint64_t)gridDim.x * WG) {
+    uint64_t i = base + threadIdx.x;
+    if (i < words) {
+      uint32_t scc = bwv[i];
+      live[i] &= ~scc;
+      count += __popc(scc);
+      if (scc) mn = min(mn, (uint32_t)(i * 32 + __ffs(scc) - 1));
+    }
+  }
+  count = block_sum(count, s_red);
+  mn = block_min(mn, s_red);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = 0;
+    part[3 * blockIdx.x + 1] = mn;  // reduced as an id with product 0: smallest id wins
+    part[3 * blockIdx.x + 2] = count;
+  }
+}
+
+// labels[v] = *minv for every v whose bit is set in bm
+__global__ void k_label_bits(uint32_t n, const uint32_t* __restrict__ bm, const uint32_t* minv,
+                             uint32_t* __restrict__ labels) {
+  const uint32_t m = *minv;
+  for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG)
+    if (bm_get(bm, (uint32_t)v)) labels[v] = m;
 }
 
 // =====================================================================
@@ -394,13 +522,18 @@ __global__ __launch_bounds__(WG) void k_trim_sweep(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ fw_adj, const uint64_t* __restrict__ bw_off,
                                                    const uint32_t* __restrict__ bw_adj, uint32_t* __restrict__ labels,
                                                    uint32_t* __restrict__ out, uint32_t* out_cnt) {
-  const uint32_t q = *qcount;
-  const uint32_t q_pad = (q + WG - 1) / WG * WG;
+  __shared__ uint32_t s_out[8 * WG];
+  __shared__ uint32_t s_on, s_ob;
+  BlockQueue q{s_out, &s_on, &s_ob, 8 * WG};
+  uint32_t iter = 0;
+  bq_init(q);
+  const uint32_t qn = *qcount;
+  const uint32_t q_pad = (qn + WG - 1) / WG * WG;
   for (uint32_t base = blockIdx.x * WG; base < q_pad; base += gridDim.x * WG) {
     uint32_t i = base + threadIdx.x;
     bool dec = false;
     uint32_t v = 0;
-    if (i < q) {
+    if (i < qn) {
       v = Q[i];
       if (inq) atomicAnd(&inq[v >> 5], ~(1u << (v & 31)));
       if (bm_get(live, v)) {
@@ -418,8 +551,10 @@ __global__ __launch_bounds__(WG) void k_trim_sweep(const uint32_t* __restrict__ 
         }
       }
     }
-    wave_append(dec, v, out, out_cnt);
+    bq_push(q, dec, v);
+    if ((++iter & 3) == 0) bq_maybe_flush(q, out, out_cnt, 4 * WG);  // <= 4 appends per thread since the last check
   }
+  bq_flush(q, out, out_cnt);
 }
 
 // live neighbour (excluding v) count in one direction, up to 2; *only = the single one
@@ -449,13 +584,18 @@ __global__ __launch_bounds__(WG) void k_trim2(const uint32_t* __restrict__ Q, co
                                               const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj,
                                               uint32_t* __restrict__ labels, uint32_t* __restrict__ out,
                                               uint32_t* out_cnt) {
-  const uint32_t q = *qcount;
-  const uint32_t q_pad = (q + WG - 1) / WG * WG;
+  __shared__ uint32_t s_out[8 * WG];
+  __shared__ uint32_t s_on, s_ob;
+  BlockQueue q{s_out, &s_on, &s_ob, 8 * WG};
+  uint32_t iter = 0;
+  bq_init(q);
+  const uint32_t qn = *qcount;
+  const uint32_t q_pad = (qn + WG - 1) / WG * WG;
   for (uint32_t base = blockIdx.x * WG; base < q_pad; base += gridDim.x * WG) {
     uint32_t i = base + threadIdx.x;
     bool dec = false;
     uint32_t lo = 0, hi = 0;
-    if (i < q) {
+    if (i < qn) {
       uint32_t v = Q[i];
       uint32_t u = 0, back = 0;
       if (bm_get(live, v) && live_nbrs_upto2(v, off, adj, live, &u) == 1 &&
@@ -474,9 +614,11 @@ __global__ __launch_bounds__(WG) void k_trim2(const uint32_t* __restrict__ Q, co
         }
       }
     }
-    wave_append(dec, lo, out, out_cnt);
-    wave_append(dec, hi, out, out_cnt);
+    bq_push(q, dec, lo);
+    bq_push(q, dec, hi);
+    if ((++iter & 1) == 0) bq_maybe_flush(q, out, out_cnt, 4 * WG);  // <= 4 appends per thread since the last check
   }
+  bq_flush(q, out, out_cnt);
 }
 
 // =====================================================================
@@ -507,6 +649,11 @@ __global__ void k_color_init_list(const uint32_t* __restrict__ U, const uint32_t
 __global__ __launch_bounds__(WG) void k_color_pull(const uint32_t* __restrict__ U, const uint32_t* ucount,
                                                    const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
                                                    uint32_t* col, uint32_t* __restrict__ out, uint32_t* out_cnt) {
+  __shared__ uint32_t s_out[8 * WG];
+  __shared__ uint32_t s_on, s_ob;
+  BlockQueue q{s_out, &s_on, &s_ob, 8 * WG};
+  uint32_t iter = 0;
+  bq_init(q);
   const uint32_t u = *ucount;
   const uint32_t u_pad = (u + WG - 1) / WG * WG;
   for (uint32_t base = blockIdx.x * WG; base < u_pad; base += gridDim.x * WG) {
@@ -523,14 +670,21 @@ __global__ __launch_bounds__(WG) void k_color_pull(const uint32_t* __restrict__ 
         changed = true;
       }
     }
-    wave_append(changed, v, out, out_cnt);
+    bq_push(q, changed, v);
+    if ((++iter & 3) == 0) bq_maybe_flush(q, out, out_cnt, 4 * WG);  // <= 4 appends per thread since the last check
   }
+  bq_flush(q, out, out_cnt);
 }
 
 // roots: live v whose colour is its own key; decided with id v
 __global__ __launch_bounds__(WG) void k_color_roots(const uint32_t* __restrict__ U, const uint32_t* ucount, uint32_t* live,
                                                     const uint32_t* __restrict__ col, uint32_t* __restrict__ labels,
                                                     uint32_t r, uint32_t* __restrict__ out, uint32_t* out_cnt) {
+  __shared__ uint32_t s_out[8 * WG];
+  __shared__ uint32_t s_on, s_ob;
+  BlockQueue q{s_out, &s_on, &s_ob, 8 * WG};
+  uint32_t iter = 0;
+  bq_init(q);
   const uint32_t u = *ucount;
   const uint32_t u_pad = (u + WG - 1) / WG * WG;
   for (uint32_t base = blockIdx.x * WG; base < u_pad; base += gridDim.x * WG) {
@@ -544,42 +698,65 @@ __global__ __launch_bounds__(WG) void k_color_roots(const uint32_t* __restrict__
         root = true;
       }
     }
-    wave_append(root, v, out, out_cnt);
+    bq_push(q, root, v);
+    if ((++iter & 3) == 0) bq_maybe_flush(q, out, out_cnt, 4 * WG);  // <= 4 appends per thread since the last check
   }
+  bq_flush(q, out, out_cnt);
 }
 
 // =====================================================================
 // canonical labels
 // =====================================================================
 
-// minid[rep] = min { v : labels[v] == rep }, wavefront-aggregated when all
-// lanes share a representative (the common case inside large SCCs)
-__global__ __launch_bounds__(WG) void k_canon_min(uint32_t n, const uint32_t* __restrict__ labels, uint32_t* minid) {
-  const uint64_t n_pad = ((uint64_t)n + WG - 1) / WG * WG;
-  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < n_pad; base += (uint64_t)gridDim.x * WG) {
-    uint64_t v64 = base + threadIdx.x;
-    bool valid = v64 < n;
-    uint32_t rep = valid ? labels[v64] : 0xFFFFFFFFu;
-    if (rep >= n) valid = false;  // not a vertex id: left as is
-    uint32_t v = valid ? (uint32_t)v64 : 0xFFFFFFFFu;
-    uint32_t rep0 = __shfl(rep, 0);
-    bool same = __all(!valid || rep == rep0);
-    if (same) {
-      // lane 0 holds the smallest valid v of the wavefront
-      if (lane_id() == 0 && valid) atomicMin(&minid[rep0], v);
-    } else if (valid) {
-      atomicMin(&minid[rep], v);
-    }
+// atomicMin(minid[rep], v) with lanes grouped by rep: the wavefront peels
+// one representative per pass (leader = first pending lane), reduces the
+// group's minimum across lanes and issues one atomic per group. Large
+// components therefore cost one atomic per wavefront, not one per vertex.
+// All lanes of the wavefront call it (uniform control flow).
+__device__ inline void grouped_atomic_min(bool valid, uint32_t rep, uint32_t v, uint32_t* minid) {
+  uint64_t pending = __ballot(valid);
+  const uint32_t lane = lane_id();
+  while (pending) {
+    uint32_t leader = __ffsll((unsigned long long)pending) - 1;
+    uint32_t r = __shfl(rep, leader);
+    bool mine = valid && rep == r;
+    uint64_t grp = __ballot(mine);
+    uint32_t m = wave_min(mine ? v : 0xFFFFFFFFu);
+    if (lane == 0) atomicMin(&minid[r], m);
+    pending &= ~grp;
   }
 }
 
-// labels[v] <- minid[rep] (the smallest vertex carrying rep)
-__global__ void k_canon_apply(uint32_t n, uint32_t* __restrict__ labels, const uint32_t* __restrict__ minid) {
-  for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG) {
+// minid[rep] = min { v : labels[v] == rep } over v in [0,n) or over list[0..count)
+__global__ __launch_bounds__(WG) void k_canon_min(uint32_t n, const uint32_t* __restrict__ list, uint32_t count,
+                                                  const uint32_t* __restrict__ labels, uint32_t* minid) {
+  const uint64_t items = list ? count : n;
+  const uint64_t pad = (items + WG - 1) / WG * WG;
+  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < pad; base += (uint64_t)gridDim.x * WG) {
+    uint64_t i = base + threadIdx.x;
+    bool valid = i < items;
+    uint32_t v = valid ? (list ? list[i] : (uint32_t)i) : 0xFFFFFFFFu;
+    uint32_t rep = valid ? labels[v] : 0xFFFFFFFFu;
+    if (rep >= n) valid = false;  // not a vertex id: left as is
+    grouped_atomic_min(valid, rep, v, minid);
+  }
+}
+
+// labels[v] <- minid[labels[v]] over [0,n) or over list[0..count)
+__global__ void k_canon_apply(uint32_t n, const uint32_t* __restrict__ list, uint32_t count, uint32_t* __restrict__ labels,
+                              const uint32_t* __restrict__ minid) {
+  const uint64_t items = list ? count : n;
+  for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i < items; i += (uint64_t)gridDim.x * WG) {
+    uint32_t v = list ? list[i] : (uint32_t)i;
     uint32_t rep = labels[v];
     if (rep >= n) continue;
     labels[v] = minid[rep];
   }
+}
+
+// minid[list[i]] = 0xFFFFFFFF for i < count (the representatives)
+__global__ void k_fill_list(const uint32_t* __restrict__ list, uint32_t count, uint32_t* __restrict__ minid) {
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < count; i += gridDim.x * WG) minid[list[i]] = 0xFFFFFFFFu;
 }
 
 // =====================================================================
@@ -726,8 +903,9 @@ __global__ __launch_bounds__(WG) void k_pivot_live(const uint32_t* __restrict__ 
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = s_p[0];
-    part[2 * blockIdx.x + 1] = s_id[0];
+    part[3 * blockIdx.x] = s_p[0];
+    part[3 * blockIdx.x + 1] = s_id[0];
+    part[3 * blockIdx.x + 2] = 0;
   }
 }
 
@@ -767,12 +945,12 @@ int dalloc(T** p, uint64_t count) {
 enum KernId : int {
   K_TRIM_FIRST = 0, K_PIVOT, K_BITMAP, K_FRONTIER_DEG, K_SCAN, K_EXPAND_FW, K_EXPAND_BW, K_PULL_FW, K_PULL_BW,
   K_TRIM_SWEEP, K_TRIM2, K_EXPAND_NEIGH, K_COLOR_INIT, K_COLOR_PULL, K_EXPAND_COLOR, K_COLOR_ROOTS,
-  K_EXPAND_COLOR_BW, K_LIVE_LIST, K_CANON, K_RESIDUAL, K_TARJAN, K_COUNT
+  K_EXPAND_COLOR_BW, K_LIVE_LIST, K_CANON, K_RESIDUAL, K_TARJAN, K_LABEL, K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"trim_first", "pivot_reduce", "bitmap_ops", "frontier_deg", "scan", "expand_fw",
                                    "expand_bw", "pull_fw", "pull_bw", "trim_sweep", "trim2", "expand_neigh",
                                    "color_init", "color_pull", "expand_color", "color_roots", "expand_color_bw",
-                                   "live_list", "canonicalize", "residual_build", "tarjan"};
+                                   "live_list", "canonicalize", "residual_build", "tarjan", "label_scc"};
 
 struct Prof {
   Ctx* c;
@@ -840,9 +1018,10 @@ int ctx_reserve(Ctx* c, uint32_t n, uint64_t m) {
   if (nn < 64) nn = 64;
   uint64_t words = ((uint64_t)nn + 255) / 256 * 8 + 16;  // covers the 256-vertex padding of ballot writers
   KSPN_TRY(dalloc(&c->live, words));
-  KSPN_TRY(dalloc(&c->fwb, words));
-  KSPN_TRY(dalloc(&c->bwb, words));
+  KSPN_TRY(dalloc(&c->fwv, words));
+  KSPN_TRY(dalloc(&c->bwv, words));
   KSPN_TRY(dalloc(&c->inq, words));
+  KSPN_TRY(dalloc(&c->fnew, words));
   KSPN_TRY(dalloc(&c->qa, (uint64_t)nn + 64));
   KSPN_TRY(dalloc(&c->qb, (uint64_t)nn + 64));
   KSPN_TRY(dalloc(&c->qc, (uint64_t)nn + 64));
@@ -872,11 +1051,22 @@ int ctx_reserve(Ctx* c, uint32_t n, uint64_t m) {
   return KSPN_OK;
 }
 
-int read_counters(Ctx* c, hipStream_t s, int count) {
+// publish the device counters (and *extra) to host-mapped memory, wait,
+// and copy them into h_cnt / h_u64[0] (extra) / h_u64[1] (cnt64[0])
+int publish(Ctx* c, hipStream_t s, const uint64_t* extra) {
+  k_publish<<<1, 64, 0, s>>>(c->cnt, c->cnt64, extra, c->d_map);
   KSPN_HIP(hipGetLastError());
-  KSPN_HIP(hipMemcpyAsync(c->h_cnt, c->cnt, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, s));
   KSPN_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < C_COUNT_; ++i) c->h_cnt[i] = c->h_map[i];
+  c->h_u64[0] = (uint64_t)c->h_map[16] | ((uint64_t)c->h_map[17] << 32);
+  c->h_u64[1] = (uint64_t)c->h_map[18] | ((uint64_t)c->h_map[19] << 32);
+  c->h_u64[2] = (uint64_t)c->h_map[20] | ((uint64_t)c->h_map[21] << 32);
   return KSPN_OK;
+}
+
+int read_counters(Ctx* c, hipStream_t s, int count) {
+  (void)count;
+  return publish(c, s, nullptr);
 }
 
 int exclusive_scan_u64(Ctx* c, const uint64_t* in, uint64_t* out, uint64_t items, hipStream_t s) {
@@ -909,9 +1099,7 @@ int build_live_list(Ctx* c, uint32_t n, const uint32_t* bitmap, uint32_t* list, 
     k_bm_scatter<<<stream_grid(c, words), WG, 0, s>>>(bitmap, words, pos, list);
   }
   k_copy_u32<<<1, 64, 0, s>>>(&c->cnt[C_AUX], pos + words);
-  KSPN_HIP(hipMemcpyAsync(c->h_cnt + C_AUX, pos + words, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  KSPN_HIP(hipStreamSynchronize(s));
-  KSPN_HIP(hipGetLastError());
+  KSPN_TRY(publish(c, s, nullptr));
   *count_out = c->h_cnt[C_AUX];
   return KSPN_OK;
 }
@@ -954,9 +1142,8 @@ int expand(Run& R, const uint32_t* F, const uint32_t* fcount_dev, uint32_t f, co
   KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)f + 1, R.s));
   uint64_t total = total_hint;
   if (!known_total) {
-    KSPN_HIP(hipMemcpyAsync(c->h_u64, c->scan + f, sizeof(uint64_t), hipMemcpyDeviceToHost, R.s));
-    KSPN_HIP(hipStreamSynchronize(R.s));
-    total = *c->h_u64;
+    KSPN_TRY(publish(c, R.s, c->scan + f));
+    total = c->h_u64[0];
   }
   if (edges) *edges = total;
   if (total == 0) return KSPN_OK;
@@ -988,7 +1175,9 @@ int reach(Run& R, int dir, uint32_t pivot, uint32_t* fw_levels_push, uint32_t* f
   const uint32_t* adj = dir == 0 ? g->fw_adj : g->bw_adj;
   const uint64_t* roff = dir == 0 ? g->bw_off : g->fw_off;
   const uint32_t* radj = dir == 0 ? g->bw_adj : g->fw_adj;
-  OpArgs a{c->live, c->fwb, c->bwb, c->inq, c->col, R.labels, pivot};
+  uint32_t* vis = dir == 0 ? c->fwv : c->bwv;
+  const uint32_t* cand = dir == 0 ? c->live : c->fwv;
+  OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, pivot, off, c->cnt64 + 1};
   uint32_t* F = c->qa;
   uint32_t* N = c->qb;
   int fc = C_Q0, nc = C_Q1;
@@ -996,50 +1185,70 @@ int reach(Run& R, int dir, uint32_t pivot, uint32_t* fw_levels_push, uint32_t* f
   k_set_u32<<<1, 64, 0, R.s>>>(F, pivot);
   k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[fc], 1u);
   uint32_t f = 1;
+  uint64_t edges = 0;  // out-edges of the frontier in this direction (unknown for the pivot level)
+  bool known = false;
   bool pull = false;
-  double alpha = R.o.alpha > 0 ? R.o.alpha : 14.0;
-  double beta = R.o.beta > 0 ? R.o.beta : 24.0;
-  double m_rest = (double)g->m;
+  const double alpha = R.o.alpha > 0 ? R.o.alpha : 14.0;
+  const double beta = R.o.beta > 0 ? R.o.beta : 24.0;
+  double m_rest = (double)g->m;  // edges not yet expanded or reached
   while (f > 0) {
-    KSPN_HIP(hipMemsetAsync(&c->cnt[nc], 0, sizeof(uint32_t), R.s));
-    uint64_t edges = 0;
+    // direction: Beamer's rule on the frontier's edge count, which the
+    // previous level summed on the device for the vertices it claimed
+    if (!pull && known && (double)edges * alpha > m_rest && f > 64) pull = true;
+    k_level_reset<<<1, 64, 0, R.s>>>(&c->cnt[nc], c->cnt64);
     if (!pull) {
-      // degrees + scan first: their total decides the direction
       {
         Prof p(c, R.s, K_FRONTIER_DEG, 28.0 * f);
-        k_frontier_deg<<<stream_grid(c, (uint64_t)f + 1), WG, 0, R.s>>>(F, &c->cnt[fc], off, c->deg);
+        if (f < 4096) {  // S[0..f] fits the 4096 slots of one workgroup
+          k_deg_scan_small<<<1, 1024, 0, R.s>>>(F, &c->cnt[fc], off, c->scan);
+        } else {
+          k_frontier_deg<<<stream_grid(c, (uint64_t)f + 1), WG, 0, R.s>>>(F, &c->cnt[fc], off, c->deg);
+        }
       }
-      KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)f + 1, R.s));
-      KSPN_HIP(hipMemcpyAsync(c->h_u64, c->scan + f, sizeof(uint64_t), hipMemcpyDeviceToHost, R.s));
-      KSPN_HIP(hipStreamSynchronize(R.s));
-      edges = *c->h_u64;
-      if ((double)edges * alpha > m_rest && f > 64) {
-        pull = true;
-      } else if (edges > 0) {
-        int kern = dir == 0 ? K_EXPAND_FW : K_EXPAND_BW;
-        Prof p(c, R.s, kern, 20.0 * f + (4.0 + 0.125) * edges);
-        if (dir == 0)
-          k_expand<OP_FW><<<expand_grid(c, edges), WG, 0, R.s>>>(F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
-        else
-          k_expand<OP_BW><<<expand_grid(c, edges), WG, 0, R.s>>>(F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
-        m_rest -= (double)edges;
-        if (R.st) R.st->edges_scanned += edges;
-        if (fw_levels_push) (*fw_levels_push)++;
-      }
-    }
-    if (pull) {
-      int kern = dir == 0 ? K_PULL_FW : K_PULL_BW;
-      Prof p(c, R.s, kern, (double)R.n / 8.0 + 16.0 * R.live_n);
+      if (f >= 4096) KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)f + 1, R.s));
+      int kern = dir == 0 ? K_EXPAND_FW : K_EXPAND_BW;
+      uint32_t grid = known ? expand_grid(c, edges) : (uint32_t)c->num_cu * 8;
+      Prof p(c, R.s, kern, 20.0 * f + (4.0 + 0.125) * (double)edges);
       if (dir == 0)
-        k_pull_fw<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->live, c->fwb, roff, radj, N, &c->cnt[nc]);
+        k_expand<OP_FW><<<grid, WG, 0, R.s>>>(F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
       else
-        k_pull_bw<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->live, c->fwb, c->bwb, roff, radj, R.labels, pivot, N,
-                                                       &c->cnt[nc]);
+        k_expand<OP_BW><<<grid, WG, 0, R.s>>>(F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
+      if (fw_levels_push) (*fw_levels_push)++;
+    } else {
+      int kern = dir == 0 ? K_PULL_FW : K_PULL_BW;
+      uint32_t grid = stream_grid(c, R.n);
+      {
+        Prof p(c, R.s, kern, (double)R.n / 4.0 + 16.0 * R.live_n);
+        if (dir == 0)
+          k_pull<0><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, off, c->fnew, c->part);
+        else
+          k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, off, c->fnew, c->part);
+      }
+      k_reduce3<<<1, WG, 0, R.s>>>(c->part, grid, &c->cnt[nc], c->cnt64);
       if (fw_levels_pull) (*fw_levels_pull)++;
     }
-    KSPN_TRY(read_counters(c, R.s, 2));
+    KSPN_TRY(publish(c, R.s, pull ? nullptr : c->scan + f));
+    if (!pull) {
+      edges = c->h_u64[0];  // exact total of the level just expanded
+      m_rest -= (double)edges;
+      if (R.st) R.st->edges_scanned += edges;
+      if (c->profile && !c->pending.empty()) c->pending.back().bytes = 20.0 * f + 4.125 * (double)edges;
+    } else {
+      uint64_t scanned = c->h_u64[1];
+      if (R.st) R.st->edges_scanned += scanned;
+      if (c->profile && !c->pending.empty()) c->pending.back().bytes += 4.125 * (double)scanned;
+    }
     uint32_t nf = c->h_cnt[nc];
-    if (pull && (double)nf * beta < (double)R.live_n) pull = false;
+    edges = c->h_u64[2];
+    known = true;
+    if (pull) m_rest -= (double)edges;
+    if (pull && nf > 0 && (double)nf * beta < (double)R.live_n) {
+      // back to push: the frontier is the bitmap of the last pull level
+      pull = false;
+      uint32_t cnt_list = 0;
+      KSPN_TRY(build_live_list(c, R.n, c->fnew, N, &cnt_list, R.s));
+      k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[nc], &c->cnt[C_AUX]);
+    }
     std::swap(F, N);
     std::swap(fc, nc);
     f = nf;
@@ -1051,22 +1260,27 @@ int fwbw(Run& R, uint32_t pivot, uint32_t* decided) {
   Ctx* c = R.c;
   {
     Prof p(c, R.s, K_BITMAP, 8.0 * R.words);
-    k_bm_not<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->fwb, c->live, R.words);
-    k_bm_set_one<<<1, 64, 0, R.s>>>(c->fwb, pivot);
+    k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->fwv, R.words, 0u);
+    k_bm_set_one<<<1, 64, 0, R.s>>>(c->fwv, pivot);
   }
   uint32_t lp = 0, lq = 0;
   KSPN_TRY(reach(R, 0, pivot, &lp, &lq));
   {
     Prof p(c, R.s, K_BITMAP, 12.0 * R.words);
-    k_bm_nand<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwb, c->fwb, c->live, R.words);
-    k_bm_set_one<<<1, 64, 0, R.s>>>(c->bwb, pivot);
+    k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwv, R.words, 0u);
+    k_bm_set_one<<<1, 64, 0, R.s>>>(c->bwv, pivot);
   }
-  k_set_u32<<<1, 64, 0, R.s>>>(R.labels + pivot, pivot);
   KSPN_TRY(reach(R, 1, pivot, &lp, &lq));
-  KSPN_HIP(hipMemsetAsync(&c->cnt[C_DECIDED], 0, sizeof(uint32_t), R.s));
+  // the pivot's SCC: clear it from live, then label it with its smallest member
+  uint32_t nparts = stream_grid(c, R.words);
   {
     Prof p(c, R.s, K_BITMAP, 16.0 * R.words);
-    k_fwbw_finish<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->live, c->fwb, c->bwb, R.words, c->cnt);
+    k_fwbw_finish<<<nparts, WG, 0, R.s>>>(c->live, c->bwv, R.words, c->part);
+    k_pivot_reduce<<<1, WG, 0, R.s>>>(c->part, nparts, &c->cnt[C_MIN], &c->cnt[C_DECIDED]);
+  }
+  {
+    Prof p(c, R.s, K_LABEL, R.n / 8.0);
+    k_label_bits<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->bwv, &c->cnt[C_MIN], R.labels);
   }
   KSPN_TRY(read_counters(c, R.s, C_DECIDED + 1));
   *decided = c->h_cnt[C_DECIDED];
@@ -1092,7 +1306,7 @@ int trim_exhaustive(Run& R, uint32_t* decided) {
   uint32_t* NQ = c->qc;
   bool first = true;
   uint32_t total = 0;
-  OpArgs a{c->live, c->fwb, c->bwb, c->inq, c->col, R.labels, 0};
+  OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, 0};
   while (q > 0) {
     KSPN_HIP(hipMemsetAsync(&c->cnt[C_Q1], 0, sizeof(uint32_t), R.s));
     {
@@ -1193,7 +1407,7 @@ int color_round(Run& R, uint32_t* decided) {
   KSPN_TRY(build_live_list(c, R.n, c->live, c->ulist, &u, R.s));
   k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_RES_N], &c->cnt[C_AUX]);
   R.prev_u = u;
-  OpArgs a{c->live, c->fwb, c->bwb, c->inq, c->col, R.labels, r};
+  OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, r};
   // forward: pull sweep over all live vertices
   KSPN_HIP(hipMemsetAsync(&c->cnt[C_Q0], 0, sizeof(uint32_t), R.s));
   {
@@ -1228,29 +1442,36 @@ int color_round(Run& R, uint32_t* decided) {
   }
   // queue bits of the last (empty) frontier are clear; pull-sweep outputs
   // never set inq, push outputs were cleared before each expansion
-  // roots
-  KSPN_HIP(hipMemsetAsync(&c->cnt[C_Q0], 0, sizeof(uint32_t), R.s));
+  // roots start the decided list D (qc); each closure level appends to D
+  // and the next level expands the segment just appended
+  uint32_t* D = c->qc;
+  KSPN_HIP(hipMemsetAsync(&c->cnt[C_DTOT], 0, sizeof(uint32_t), R.s));
   {
     Prof p(c, R.s, K_COLOR_ROOTS, 12.0 * u);
-    k_color_roots<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], c->live, c->col, R.labels, r, c->qa,
-                                                     &c->cnt[C_Q0]);
+    k_color_roots<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], c->live, c->col, R.labels, r, D,
+                                                     &c->cnt[C_DTOT]);
   }
-  KSPN_TRY(read_counters(c, R.s, 1));
-  f = c->h_cnt[C_Q0];
-  uint32_t total = f;
-  F = c->qa;
-  N = c->qb;
-  fc = C_Q0;
-  nc = C_Q1;
-  while (f > 0) {
-    KSPN_HIP(hipMemsetAsync(&c->cnt[nc], 0, sizeof(uint32_t), R.s));
+  KSPN_TRY(read_counters(c, R.s, C_DTOT + 1));
+  const uint32_t roots = c->h_cnt[C_DTOT];
+  uint32_t seg_beg = 0, seg_end = roots;
+  while (seg_end > seg_beg) {
+    uint32_t fseg = seg_end - seg_beg;
+    k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_FSEG], fseg);
     uint64_t e = 0;
-    KSPN_TRY(expand<OP_COLOR_BW>(R, F, &c->cnt[fc], f, g->bw_off, g->bw_adj, a, N, &c->cnt[nc], K_EXPAND_COLOR_BW, 4.0, &e));
-    KSPN_TRY(read_counters(c, R.s, 2));
-    f = c->h_cnt[nc];
-    total += f;
-    std::swap(F, N);
-    std::swap(fc, nc);
+    KSPN_TRY(expand<OP_COLOR_BW>(R, D + seg_beg, &c->cnt[C_FSEG], fseg, g->bw_off, g->bw_adj, a, D, &c->cnt[C_DTOT],
+                                 K_EXPAND_COLOR_BW, 8.0, &e));
+    KSPN_TRY(read_counters(c, R.s, C_DTOT + 1));
+    seg_beg = seg_end;
+    seg_end = c->h_cnt[C_DTOT];
+  }
+  const uint32_t total = seg_end;
+  // canonical ids: every decision of this round carries its root's id;
+  // replace it by the smallest member (col[root] is the min table)
+  if (total) {
+    Prof p(c, R.s, K_CANON, 4.0 * roots + 16.0 * total);
+    k_fill_list<<<stream_grid(c, roots), WG, 0, R.s>>>(D, roots, c->col);
+    k_canon_min<<<stream_grid(c, total), WG, 0, R.s>>>(R.n, D, total, R.labels, c->col);
+    k_canon_apply<<<stream_grid(c, total), WG, 0, R.s>>>(R.n, D, total, R.labels, c->col);
   }
   *decided = total;
   R.live_n -= total;
@@ -1281,9 +1502,8 @@ int residual(Run& R, uint32_t* decided) {
                                                                nullptr, c->deg);
   }
   KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)u + 1, R.s));
-  KSPN_HIP(hipMemcpyAsync(c->h_u64, c->scan + u, sizeof(uint64_t), hipMemcpyDeviceToHost, R.s));
-  KSPN_HIP(hipStreamSynchronize(R.s));
-  uint64_t sm = *c->h_u64;
+  KSPN_TRY(publish(c, R.s, c->scan + u));
+  uint64_t sm = c->h_u64[0];
   if (sm + 1 > c->cap_sub_m) {
     KSPN_TRY(dalloc(&c->sub_adj, sm + 1));
     c->cap_sub_m = sm + 1;
@@ -1315,8 +1535,8 @@ int residual(Run& R, uint32_t* decided) {
 int canonicalize(Ctx* c, uint32_t n, uint32_t* labels, hipStream_t s) {
   Prof p(c, s, K_CANON, 16.0 * n);
   k_fill_u32<<<stream_grid(c, n), WG, 0, s>>>(c->col, n, 0xFFFFFFFFu);
-  k_canon_min<<<stream_grid(c, n), WG, 0, s>>>(n, labels, c->col);
-  k_canon_apply<<<stream_grid(c, n), WG, 0, s>>>(n, labels, c->col);
+  k_canon_min<<<stream_grid(c, n), WG, 0, s>>>(n, nullptr, 0, labels, c->col);
+  k_canon_apply<<<stream_grid(c, n), WG, 0, s>>>(n, nullptr, 0, labels, c->col);
   return KSPN_OK;
 }
 
@@ -1327,7 +1547,7 @@ int select_pivot_live(Run& R, uint32_t* pivot) {
   {
     Prof p(c, R.s, K_PIVOT, 20.0 * u);
     k_pivot_live<<<nparts, WG, 0, R.s>>>(c->ulist, &c->cnt[C_AUX], R.g->fw_off, R.g->bw_off, c->part);
-    k_pivot_reduce<<<1, WG, 0, R.s>>>(c->part, nparts, &c->cnt[C_FLAG]);
+    k_pivot_reduce<<<1, WG, 0, R.s>>>(c->part, nparts, &c->cnt[C_FLAG], nullptr);
   }
   KSPN_TRY(read_counters(c, R.s, C_FLAG + 1));
   *pivot = c->h_cnt[C_FLAG];
@@ -1349,16 +1569,18 @@ int run_pipeline(Run& R) {
   mark(0);
   // ---- trim-1 first pass + pivot
   KSPN_HIP(hipMemsetAsync(c->cnt, 0, sizeof(uint32_t) * C_COUNT_, R.s));
+  KSPN_HIP(hipMemsetAsync(c->cnt64, 0, sizeof(unsigned long long) * 4, R.s));
+  c->h_u64[1] = 0;
   KSPN_HIP(hipMemsetAsync(c->live, 0, R.words * sizeof(uint32_t), R.s));
   KSPN_HIP(hipMemsetAsync(c->inq, 0, R.words * sizeof(uint32_t), R.s));
   uint32_t nparts = std::min<uint32_t>(stream_grid(c, R.n), (uint32_t)c->num_cu * 16);
   {
     Prof p(c, R.s, K_TRIM_FIRST, 20.0 * R.n + R.n / 8.0);
-    k_trim_first<<<nparts, WG, 0, R.s>>>(R.n, g->fw_off, g->bw_off, c->live, R.labels, c->part, c->cnt);
+    k_trim_first<<<nparts, WG, 0, R.s>>>(R.n, g->fw_off, g->bw_off, c->live, R.labels, c->part);
   }
   {
-    Prof p(c, R.s, K_PIVOT, 16.0 * nparts);
-    k_pivot_reduce<<<1, WG, 0, R.s>>>(c->part, nparts, &c->cnt[C_FLAG]);
+    Prof p(c, R.s, K_PIVOT, 24.0 * nparts);
+    k_pivot_reduce<<<1, WG, 0, R.s>>>(c->part, nparts, &c->cnt[C_FLAG], &c->cnt[C_DECIDED]);
   }
   KSPN_TRY(read_counters(c, R.s, C_FLAG + 1));
   uint32_t decided = c->h_cnt[C_DECIDED];
@@ -1435,7 +1657,8 @@ int run_pipeline(Run& R) {
     if (st) st->decided_residual += d;
   }
   mark(4);
-  KSPN_TRY(canonicalize(c, R.n, R.labels, R.s));
+  // every stage writes canonical ids (own id, pair minimum, SCC minimum),
+  // so no pass over all labels is needed here
   mark(5);
   KSPN_HIP(hipStreamSynchronize(R.s));
   if (timing) {
@@ -1500,13 +1723,23 @@ int kspn_ctx_create(kspn_ctx** out, int device) {
   c->device = device;
   int cu = 0;
   if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0) c->num_cu = cu;
+  void* hm = nullptr;
   if (hipMalloc((void**)&c->cnt, sizeof(uint32_t) * C_COUNT_) != hipSuccess ||
+      hipMalloc((void**)&c->cnt64, sizeof(unsigned long long) * 4) != hipSuccess ||
       hipHostMalloc((void**)&c->h_cnt, sizeof(uint32_t) * C_COUNT_, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_u64, sizeof(uint64_t) * 4, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->h_u64, sizeof(uint64_t) * 4, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&hm, sizeof(uint32_t) * 64, hipHostMallocMapped) != hipSuccess) {
     kspn_ctx_destroy(k);
     return KSPN_E_OOM;
   }
+  c->h_map = (volatile uint32_t*)hm;
+  if (hipHostGetDevicePointer((void**)&c->d_map, hm, 0) != hipSuccess) {
+    kspn_ctx_destroy(k);
+    return KSPN_E_HIP;
+  }
   std::memset(c->h_cnt, 0, sizeof(uint32_t) * C_COUNT_);
+  std::memset(hm, 0, sizeof(uint32_t) * 64);
+  std::memset(c->h_u64, 0, sizeof(uint64_t) * 4);
   c->kt.resize(K_COUNT);
   for (int i = 0; i < K_COUNT; ++i) c->kt[i].name = kKernNames[i];
   *out = k;
@@ -1517,11 +1750,13 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   if (!k) return KSPN_OK;
   Ctx* c = &k->c;
   (void)hipSetDevice(c->device);
-  void* ptrs[] = {c->live, c->fwb, c->bwb, c->inq, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
+  void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
                   c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
+  if (c->h_map) (void)hipHostFree((void*)c->h_map);
+  if (c->cnt64) (void)hipFree(c->cnt64);
   if (c->h_u64) (void)hipHostFree(c->h_u64);
   for (auto& ep : c->pending) {
     (void)hipEventDestroy(ep.a);
@@ -1649,10 +1884,10 @@ int kspn_subgraph(kspn_ctx* k, uint32_t n, const uint64_t* off, const uint32_t* 
   hipStream_t s = (hipStream_t)stream;
   // byte mask -> bitmap -> ascending list of members
   uint64_t words = bm_words(n);
-  KSPN_HIP(hipMemsetAsync(c->fwb, 0, words * 4, s));
-  k_mask_to_bitmap<<<stream_grid(c, n), WG, 0, s>>>(n, mask, c->fwb);
+  KSPN_HIP(hipMemsetAsync(c->fwv, 0, words * 4, s));
+  k_mask_to_bitmap<<<stream_grid(c, n), WG, 0, s>>>(n, mask, c->fwv);
   uint32_t sn = 0;
-  KSPN_TRY(build_live_list(c, n, c->fwb, super_ids, &sn, s));
+  KSPN_TRY(build_live_list(c, n, c->fwv, super_ids, &sn, s));
   if (sn == 0) {
     KSPN_HIP(hipMemsetAsync(sub_off, 0, 8, s));
     KSPN_HIP(hipStreamSynchronize(s));
@@ -1664,10 +1899,9 @@ int kspn_subgraph(kspn_ctx* k, uint32_t n, const uint64_t* off, const uint32_t* 
   KSPN_TRY(exclusive_scan_u64(c, c->deg, sub_off, (uint64_t)sn + 1, s));
   k_sub_fill<<<stream_grid(c, sn), WG, 0, s>>>(super_ids, &c->cnt[C_RES_N], off, adj, nullptr, mask, c->col, sub_off,
                                               sub_adj);
-  KSPN_HIP(hipMemcpyAsync(c->h_u64, sub_off + sn, 8, hipMemcpyDeviceToHost, s));
-  KSPN_HIP(hipStreamSynchronize(s));
+  KSPN_TRY(publish(c, s, sub_off + sn));
   *sub_n = sn;
-  *sub_m = *c->h_u64;
+  *sub_m = c->h_u64[0];
   return KSPN_OK;
 }
 

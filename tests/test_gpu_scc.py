@@ -225,7 +225,7 @@ def test_profile_stats(ka, O):
     lab, st = ka.scc(g, profile=True)
     d = st.as_dict()
     names = {k["name"] for k in d["kernels"]}
-    assert "trim_first" in names and "canonicalize" in names
+    assert "trim_first" in names and "label_scc" in names
     assert all(k["ms"] >= 0 and k["launches"] > 0 for k in d["kernels"])
     assert st.ms_total > 0
 
