@@ -22,6 +22,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# library profile names -> kernel symbols as rocprofv3 reports them
+ROCPROF_NAMES = {"pull_fw": "kspn::k_pull<0>", "pull_bw": "kspn::k_pull<1>", "trim_first": "kspn::k_trim_first",
+                 "expand_fw": "kspn::k_expand<0>", "expand_bw": "kspn::k_expand<1>", "label_scc": "kspn::k_label_bits",
+                 "tail_solve": "kspn::k_tail_solve", "trim_sweep": "kspn::k_trim_sweep<1>"}
+
+
+def pmc_traffic(config, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"{config}_pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    data = json.load(open(path))
+    k = data.get("kernels", {}).get(ROCPROF_NAMES.get(kernel, ""))
+    return k["bytes_per_launch"] if k else None
 CONFIGS = {
     # name: (generator kind, kwargs, seed, description)
     "gnm64k": ("gnm", dict(n=65536, draws=524288), 0x6B5350AA00000000,
@@ -42,7 +57,19 @@ def main():
     ap.add_argument("--method", type=int, default=None, help="0 kaspan, 1 trim_ex (default: fastest known)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes 3 times (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.ones(args.calibrate_copy, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (multi-GPU sharding is a design, not built)"}))
         sys.exit(2)
@@ -101,7 +128,8 @@ def main():
         per_launch_s = dom["ms"] / dom["launches"] / 1e3
         achieved = per_launch_bytes / per_launch_s / 1e9
         roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": pmc_traffic(args.config, dom_name),
                     "bytes_per_launch": per_launch_bytes, "avg_launch_us": per_launch_s * 1e6}
 
     cpu = None

@@ -58,6 +58,7 @@ typedef struct kspn_graph {
 
 #define KSPN_FLAG_PROFILE 1u  /* time every kernel launch with HIP events (slower; fills kspn_stats.kern) */
 #define KSPN_FLAG_NO_TRIM2 2u /* KSPN_METHOD_TRIM_EX without the trim-2 pass */
+#define KSPN_FLAG_NO_TAIL 4u  /* never hand the rest to the single-workgroup tail solver */
 
 typedef struct kspn_opts {
   uint32_t method;             /* KSPN_METHOD_* */
@@ -65,6 +66,7 @@ typedef struct kspn_opts {
   uint32_t fwbw_rounds;        /* FW-BW rounds before colouring (TRIM_EX); 0 -> 1 */
   uint32_t residual_threshold; /* hand the rest to residual Tarjan once live <= this (0: never) */
   uint32_t max_rounds;         /* colour-round cap (0 -> unlimited); exceeding it -> KSPN_E_ITER_CAP */
+  uint32_t tail_threshold;     /* colour the rest in one workgroup once live <= this (0 -> 4096) */
   uint32_t flags;              /* KSPN_FLAG_* */
   float alpha;                 /* push->pull switch: edges(frontier) * alpha > edges(unvisited); 0 -> 14 */
   float beta;                  /* pull->push switch: frontier * beta < live; 0 -> 24 */

@@ -200,11 +200,12 @@ def generate(kind: str, seed: int, device="cuda", ctx: Context | None = None, st
 
 
 def _opts(method=METHOD_KASPAN, pivot=0, fwbw_rounds=1, residual_threshold=0, max_rounds=0, flags=0, alpha=14.0,
-          beta=24.0, mlp_params=None):
+          beta=24.0, mlp_params=None, tail_threshold=0):
     o = _lib.Opts()
     _lib.lib().kspn_opts_default(C.byref(o))
     o.method, o.pivot, o.fwbw_rounds = method, pivot, fwbw_rounds
     o.residual_threshold, o.max_rounds, o.flags = residual_threshold, max_rounds, flags
+    o.tail_threshold = tail_threshold
     o.alpha, o.beta = alpha, beta
     o.mlp_params = mlp_params.data_ptr() if mlp_params is not None else None
     return o

@@ -29,6 +29,7 @@ PIVOT_PLAIN = 0
 PIVOT_LEARNED = 1
 FLAG_PROFILE = 1
 FLAG_NO_TRIM2 = 2
+FLAG_NO_TAIL = 4
 GEN_GNM, GEN_RMAT, GEN_LATTICE = 0, 1, 2
 UNDECIDED = 0xFFFFFFFF
 
@@ -64,6 +65,7 @@ class Opts(C.Structure):
         ("fwbw_rounds", C.c_uint32),
         ("residual_threshold", C.c_uint32),
         ("max_rounds", C.c_uint32),
+        ("tail_threshold", C.c_uint32),
         ("flags", C.c_uint32),
         ("alpha", C.c_float),
         ("beta", C.c_float),

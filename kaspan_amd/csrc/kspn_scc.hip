@@ -865,6 +865,110 @@ __global__ void k_tarjan(uint32_t n, const uint64_t* __restrict__ off, const uin
   }
 }
 
+// Tail solver: colour rounds on a compacted subgraph inside one workgroup
+// of 1024 threads (block barriers, no host round trips). Vertex i of the
+// subgraph is super[i] (ascending); fo/fa and bo/ba are its out- and
+// in-edge CSRs. Round r keys live vertices with ~rotr(super[i], r), sweeps
+// max-colours along in-edges to a fixpoint, decides the roots and closes
+// backward over out-edges within each colour, then writes the minimum
+// member as the label of every vertex decided in the round. state: 0 live,
+// 2 decided this round, 1 decided. Rounds are capped by max_rounds (0: none);
+// *status = 1 if the cap was hit.
+__global__ __launch_bounds__(1024) void k_tail_solve(uint32_t sn, const uint32_t* __restrict__ super,
+                                                     const uint64_t* __restrict__ fo, const uint32_t* __restrict__ fa,
+                                                     const uint64_t* __restrict__ bo, const uint32_t* __restrict__ ba,
+                                                     uint32_t* col, uint32_t* state, uint32_t* minv,
+                                                     uint32_t* __restrict__ labels, uint32_t r0, uint32_t max_rounds,
+                                                     uint32_t* status) {
+  __shared__ uint32_t s_flag, s_live;
+  const uint32_t tid = threadIdx.x, T = blockDim.x;
+  for (uint32_t i = tid; i < sn; i += T) state[i] = 0;
+  uint32_t r = r0;
+  for (uint32_t round = 0;; ++round) {
+    __syncthreads();
+    if (tid == 0) s_live = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < sn; i += T)
+      if (state[i] == 0) { s_live = 1; break; }
+    __syncthreads();
+    if (!s_live) break;
+    if (max_rounds && round >= max_rounds) {
+      if (tid == 0) *status = 1;
+      break;
+    }
+    ++r;
+    for (uint32_t i = tid; i < sn; i += T) col[i] = state[i] == 0 ? ~rotr32(super[i], r) : 0u;
+    // forward: max colour over in-neighbours, in place, to a fixpoint
+    for (;;) {
+      __syncthreads();
+      if (tid == 0) s_flag = 0;
+      __syncthreads();
+      for (uint32_t i = tid; i < sn; i += T) {
+        if (state[i] != 0) continue;
+        uint32_t c0 = col[i], c = c0;
+        for (uint64_t it = bo[i], end = bo[i + 1]; it < end; ++it) c = max(c, col[ba[it]]);
+        if (c > c0) {
+          col[i] = c;
+          s_flag = 1;
+        }
+      }
+      __syncthreads();
+      if (!s_flag) break;
+    }
+    // roots, then backward closure within a colour over out-edges
+    for (uint32_t i = tid; i < sn; i += T)
+      if (state[i] == 0 && col[i] == ~rotr32(super[i], r)) {
+        state[i] = 2;
+        minv[i] = super[i];
+      }
+    for (;;) {
+      __syncthreads();
+      if (tid == 0) s_flag = 0;
+      __syncthreads();
+      for (uint32_t i = tid; i < sn; i += T) {
+        if (state[i] != 0) continue;
+        uint32_t ci = col[i];
+        for (uint64_t it = fo[i], end = fo[i + 1]; it < end; ++it) {
+          uint32_t j = fa[it];
+          if (state[j] == 2 && col[j] == ci) {
+            state[i] = 2;
+            s_flag = 1;
+            break;
+          }
+        }
+      }
+      __syncthreads();
+      if (!s_flag) break;
+    }
+    // minimum member per colour: the root of colour c is the vertex whose
+    // key is c; its compact index is found by binary search over super[]
+    for (uint32_t i = tid; i < sn; i += T) {
+      if (state[i] != 2) continue;
+      uint32_t g = rotl32(~col[i], r);
+      uint32_t lo = 0, hi = sn;
+      while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (super[mid] <= g) lo = mid; else hi = mid;
+      }
+      atomicMin(&minv[lo], super[i]);
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < sn; i += T) {
+      if (state[i] != 2) continue;
+      uint32_t g = rotl32(~col[i], r);
+      uint32_t lo = 0, hi = sn;
+      while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (super[mid] <= g) lo = mid; else hi = mid;
+      }
+      labels[super[i]] = minv[lo];
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < sn; i += T)
+      if (state[i] == 2) state[i] = 1;
+  }
+}
+
 // labels[U[i]] = U[sub_label[i]]
 __global__ void k_res_apply(const uint32_t* __restrict__ U, const uint32_t* ucount, const uint32_t* __restrict__ sublab,
                             uint32_t* __restrict__ labels) {
@@ -945,12 +1049,12 @@ int dalloc(T** p, uint64_t count) {
 enum KernId : int {
   K_TRIM_FIRST = 0, K_PIVOT, K_BITMAP, K_FRONTIER_DEG, K_SCAN, K_EXPAND_FW, K_EXPAND_BW, K_PULL_FW, K_PULL_BW,
   K_TRIM_SWEEP, K_TRIM2, K_EXPAND_NEIGH, K_COLOR_INIT, K_COLOR_PULL, K_EXPAND_COLOR, K_COLOR_ROOTS,
-  K_EXPAND_COLOR_BW, K_LIVE_LIST, K_CANON, K_RESIDUAL, K_TARJAN, K_LABEL, K_COUNT
+  K_EXPAND_COLOR_BW, K_LIVE_LIST, K_CANON, K_RESIDUAL, K_TARJAN, K_LABEL, K_TAIL, K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"trim_first", "pivot_reduce", "bitmap_ops", "frontier_deg", "scan", "expand_fw",
                                    "expand_bw", "pull_fw", "pull_bw", "trim_sweep", "trim2", "expand_neigh",
                                    "color_init", "color_pull", "expand_color", "color_roots", "expand_color_bw",
-                                   "live_list", "canonicalize", "residual_build", "tarjan", "label_scc"};
+                                   "live_list", "canonicalize", "residual_build", "tarjan", "label_scc", "tail_solve"};
 
 struct Prof {
   Ctx* c;
@@ -1119,6 +1223,7 @@ struct Run {
   uint32_t rotation = 0;
   bool col_valid = false;  // col is 0 for every vertex outside ulist (prev_u entries)
   uint32_t prev_u = 0;
+  uint32_t rounds = 0;  // colour rounds so far
 };
 
 int select_pivot_live(Run& R, uint32_t* pivot);
@@ -1531,6 +1636,68 @@ int residual(Run& R, uint32_t* decided) {
   return KSPN_OK;
 }
 
+// ---------------------------------------------------------------- tail
+// compact the live subgraph (both directions) and colour it in one workgroup
+int tail_solve(Run& R, uint32_t* decided) {
+  Ctx* c = R.c;
+  const kspn_graph* g = R.g;
+  uint32_t u = 0;
+  KSPN_TRY(build_live_list(c, R.n, c->live, c->ulist, &u, R.s));
+  if (u == 0) {
+    *decided = 0;
+    return KSPN_OK;
+  }
+  k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_RES_N], &c->cnt[C_AUX]);
+  uint32_t* subid = c->col;
+  {
+    Prof p(c, R.s, K_RESIDUAL, 8.0 * u);
+    k_sub_ids<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], subid);
+    k_sub_deg<<<stream_grid(c, (uint64_t)u + 1), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj,
+                                                               c->live, nullptr, c->deg);
+  }
+  KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)u + 1, R.s));
+  {
+    Prof p(c, R.s, K_RESIDUAL, 8.0 * u);
+    k_sub_deg<<<stream_grid(c, (uint64_t)u + 1), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj,
+                                                               c->live, nullptr, c->deg);
+  }
+  if (!c->sub_off) KSPN_TRY(dalloc(&c->sub_off, (uint64_t)c->cap_n + 64));
+  KSPN_TRY(exclusive_scan_u64(c, c->deg, c->sub_off, (uint64_t)u + 1, R.s));
+  KSPN_TRY(publish(c, R.s, c->scan + u));
+  uint64_t sm = c->h_u64[0];  // the two directions have the same edge count
+  if (2 * sm + 2 > c->cap_sub_m) {
+    KSPN_TRY(dalloc(&c->sub_adj, 2 * sm + 2));
+    c->cap_sub_m = 2 * sm + 2;
+  }
+  uint32_t* fa = c->sub_adj;
+  uint32_t* ba = c->sub_adj + sm + 1;
+  {
+    Prof p(c, R.s, K_RESIDUAL, 32.0 * u + 8.0 * sm);
+    k_sub_fill<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, c->live, nullptr,
+                                                  subid, c->scan, fa);
+    k_sub_fill<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->live, nullptr,
+                                                  subid, c->sub_off, ba);
+  }
+  k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_FLAG], 0u);
+  uint32_t cap = R.o.max_rounds ? R.o.max_rounds - R.rounds : 0;  // > 0: checked by the caller
+  {
+    Prof p(c, R.s, K_TAIL, 24.0 * u + 8.0 * sm);
+    k_tail_solve<<<1, 1024, 0, R.s>>>(u, c->ulist, c->scan, fa, c->sub_off, ba, c->qa, c->qb, c->qc, R.labels,
+                                      R.rotation, cap, &c->cnt[C_FLAG]);
+  }
+  KSPN_TRY(publish(c, R.s, nullptr));
+  if (c->h_cnt[C_FLAG]) return KSPN_E_ITER_CAP;
+  KSPN_HIP(hipMemsetAsync(c->live, 0, R.words * sizeof(uint32_t), R.s));
+  if (R.st) {
+    R.st->residual_n = u;
+    R.st->residual_m = sm;
+  }
+  *decided = u;
+  R.live_n -= u;
+  R.col_valid = false;  // col held sub ids
+  return KSPN_OK;
+}
+
 // ---------------------------------------------------------------- canonical ids
 int canonicalize(Ctx* c, uint32_t n, uint32_t* labels, hipStream_t s) {
   Prof p(c, s, K_CANON, 16.0 * n);
@@ -1635,13 +1802,19 @@ int run_pipeline(Run& R) {
   mark(2);
   // ---- colour rounds
   uint32_t cap = R.o.max_rounds;
-  uint32_t nrounds = 0;
+  const uint32_t tail = (R.o.flags & KSPN_FLAG_NO_TAIL) ? 0u : (R.o.tail_threshold ? R.o.tail_threshold : 4096u);
   while (R.live_n > 0) {
     if (R.o.residual_threshold && R.live_n <= R.o.residual_threshold) break;
-    if (cap && nrounds >= cap) return KSPN_E_ITER_CAP;
+    if (cap && R.rounds >= cap) return KSPN_E_ITER_CAP;
+    if (R.live_n <= tail) {
+      uint32_t d = 0;
+      KSPN_TRY(tail_solve(R, &d));
+      if (st) st->decided_color += d;
+      break;
+    }
     uint32_t d = 0;
     KSPN_TRY(color_round(R, &d));
-    ++nrounds;
+    ++R.rounds;
     if (st) st->decided_color += d;
     if (d == 0) return KSPN_E_ITER_CAP;  // cannot happen: the max key is always a root
     if (trim_ex && R.live_n > 0) {
