@@ -144,10 +144,31 @@ int kspn_subgraph(kspn_ctx* ctx, uint32_t n, const uint64_t* off, const uint32_t
 int kspn_tarjan(kspn_ctx* ctx, uint32_t n, uint64_t m, const uint64_t* off, const uint32_t* adj,
                 uint32_t* labels, void* stream);
 
+/* ---- learned pivot scorer (no reference counterpart; see KSPN_PIVOT_LEARNED) ----
+ * MLP 6 -> 32 -> 32 -> 1 (ReLU). Parameter vector of KSPN_MLP_PARAMS floats:
+ * W1[32][6], b1[32], W2[32][32], b2[32], w3[32], b3. Features per vertex:
+ * log1p(in), log1p(out), log1p(in)*log1p(out)/8, log1p(sum of out-degrees of
+ * out-neighbours), log1p(sum of in-degrees of in-neighbours), log1p(min(in,out)). */
+#define KSPN_MLP_PARAMS 1313
+
+/* feat[count][6] for vertices[0..count) (vertices NULL: 0..count-1); device pointers */
+int kspn_pivot_features(kspn_ctx* ctx, const kspn_graph* g, const uint32_t* vertices, uint32_t count, float* feat,
+                        void* stream);
+/* scores out[rows] = MLP(x[rows][6]); weights rounded to BF16, FP32 accumulation (MFMA) */
+int kspn_mlp_forward(kspn_ctx* ctx, const float* params, const float* x, uint32_t rows, float* out, void* stream);
+/* FP32 gradient of 0.5*mean((MLP(x) - target)^2): grad[KSPN_MLP_PARAMS], *loss (device) */
+int kspn_mlp_grad(kspn_ctx* ctx, const float* params, const float* x, const float* target, uint32_t rows, float* grad,
+                  float* loss, void* stream);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8); state[2*KSPN_MLP_PARAMS] starts at 0; step >= 1 */
+int kspn_mlp_adam(kspn_ctx* ctx, float* params, const float* grad, float* state, uint32_t step, float lr, void* stream);
+/* pivot outcome log: sizes[v] = size of v's SCC, from canonical labels (device) */
+int kspn_scc_sizes(kspn_ctx* ctx, uint32_t n, const uint32_t* labels, uint32_t* sizes, void* stream);
+
 /* ---- seeded graph construction (inputs for tests and benchmarks) --------- */
 #define KSPN_GEN_GNM 0     /* p0 = n, p1 = draws */
 #define KSPN_GEN_RMAT 1    /* p0 = scale, p1 = draws, a,b,c, permute */
 #define KSPN_GEN_LATTICE 2 /* p0 = side, a = p_bidirectional */
+#define KSPN_GEN_PLANTED 3 /* p0 = n, p1 = max component size, a = exponent, b = intra out-degree, c = inter out-degree */
 
 typedef struct kspn_gen_params {
   uint32_t kind;
@@ -167,6 +188,8 @@ int kspn_build_csr(kspn_ctx* ctx, uint32_t n, uint64_t count, const uint32_t* sr
                    uint64_t* m, uint64_t** fw_off, uint32_t** fw_adj, uint64_t** bw_off, uint32_t** bw_adj,
                    void* stream);
 int kspn_free(void* device_ptr);
+/* canonical labels of a KSPN_GEN_PLANTED graph, known by construction (device labels[n]) */
+int kspn_planted_labels(kspn_ctx* ctx, const kspn_gen_params* p, uint32_t* labels, void* stream);
 
 #ifdef __cplusplus
 }

@@ -35,7 +35,7 @@ from ._lib import KspnError, Stats
 __all__ = [
     "BidiGraph", "Context", "KspnError", "Stats", "scc", "scc_trim_ex", "scc_trim_ex_residual",
     "backward_complement_graph", "allgather_fw_sub_graph", "tarjan", "normalise_scc_id", "generate",
-    "METHOD_KASPAN", "METHOD_TRIM_EX", "UNDECIDED",
+    "METHOD_KASPAN", "METHOD_TRIM_EX", "UNDECIDED", "planted_labels", "scc_sizes",
 ]
 
 METHOD_KASPAN = _lib.METHOD_KASPAN
@@ -189,6 +189,8 @@ def generate(kind: str, seed: int, device="cuda", ctx: Context | None = None, st
         p.permute = int(kw.get("permute", True))
     elif kind == "lattice":
         p.kind, p.p0, p.a = _lib.GEN_LATTICE, kw["side"], kw.get("p_bidir", 0.1)
+    elif kind == "planted":
+        p = _planted_params(seed, **kw)
     else:
         raise KspnError(-1, f"unknown generator {kind!r}")
     n = C.c_uint32()
@@ -197,6 +199,23 @@ def generate(kind: str, seed: int, device="cuda", ctx: Context | None = None, st
     _lib.check(_lib.lib().kspn_generate(ctx.ptr, C.byref(p), C.byref(n), C.byref(m), *(C.byref(q) for q in ptrs),
                                         _stream_ptr(stream)), "kspn_generate")
     return BidiGraph._adopt(n.value, m.value, ptrs, device)
+
+
+def _planted_params(seed, n, max_size=10_000_000, expo=2.1, intra_deg=4.0, inter_deg=3.0):
+    p = _lib.GenParams()
+    p.seed, p.kind, p.p0, p.p1 = seed, _lib.GEN_PLANTED, n, max_size
+    p.a, p.b, p.c = expo, intra_deg, inter_deg
+    return p
+
+
+def planted_labels(seed, n, device="cuda", ctx: Context | None = None, stream=None, **kw) -> torch.Tensor:
+    """Canonical labels of generate("planted", seed, n=n, ...), known by construction."""
+    ctx = ctx or Context.default(torch.device(device).index or 0)
+    p = _planted_params(seed, n, **kw)
+    labels = torch.empty(n, dtype=torch.int32, device=device)
+    _lib.check(_lib.lib().kspn_planted_labels(ctx.ptr, C.byref(p), C.c_void_p(labels.data_ptr()), _stream_ptr(stream)),
+               "kspn_planted_labels")
+    return labels
 
 
 def _opts(method=METHOD_KASPAN, pivot=0, fwbw_rounds=1, residual_threshold=0, max_rounds=0, flags=0, alpha=14.0,
@@ -306,3 +325,12 @@ def allgather_fw_sub_graph(n, off: torch.Tensor, adj: torch.Tensor, mask: torch.
                                         C.c_void_p(sub_off.data_ptr()), C.c_void_p(sub_adj.data_ptr()), C.byref(sn),
                                         C.byref(sm), _stream_ptr(stream)), "kspn_subgraph")
     return super_ids[: sn.value], sub_off[: sn.value + 1], sub_adj[: sm.value]
+
+
+def scc_sizes(labels: torch.Tensor, ctx: Context | None = None, stream=None) -> torch.Tensor:
+    """sizes[v] = size of v's SCC, from canonical labels (the pivot outcome log)."""
+    ctx = ctx or Context.default(labels.device.index or 0)
+    sizes = torch.empty_like(labels)
+    _lib.check(_lib.lib().kspn_scc_sizes(ctx.ptr, labels.numel(), C.c_void_p(labels.data_ptr()),
+                                         C.c_void_p(sizes.data_ptr()), _stream_ptr(stream)), "kspn_scc_sizes")
+    return sizes

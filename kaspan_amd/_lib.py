@@ -30,14 +30,16 @@ PIVOT_LEARNED = 1
 FLAG_PROFILE = 1
 FLAG_NO_TRIM2 = 2
 FLAG_NO_TAIL = 4
-GEN_GNM, GEN_RMAT, GEN_LATTICE = 0, 1, 2
+GEN_GNM, GEN_RMAT, GEN_LATTICE, GEN_PLANTED = 0, 1, 2, 3
+MLP_PARAMS = 1313
 UNDECIDED = 0xFFFFFFFF
 
 # every symbol include/kspn/scc.h declares
 EXPORTS = (
     "kspn_opts_default", "kspn_strerror", "kspn_ctx_create", "kspn_ctx_destroy", "kspn_ctx_reserve",
     "kspn_scc", "kspn_scc_host", "kspn_transpose", "kspn_canonicalize", "kspn_subgraph", "kspn_tarjan",
-    "kspn_generate", "kspn_build_csr", "kspn_free",
+    "kspn_generate", "kspn_build_csr", "kspn_free", "kspn_planted_labels", "kspn_pivot_features",
+    "kspn_mlp_forward", "kspn_mlp_grad", "kspn_mlp_adam", "kspn_scc_sizes",
 )
 
 
@@ -146,6 +148,12 @@ def lib():
         "kspn_generate": (i32, [vp, P(GenParams), P(u32), P(u64), P(vp), P(vp), P(vp), P(vp), vp]),
         "kspn_build_csr": (i32, [vp, u32, u64, vp, vp, P(u64), P(vp), P(vp), P(vp), P(vp), vp]),
         "kspn_free": (i32, [vp]),
+        "kspn_planted_labels": (i32, [vp, P(GenParams), vp, vp]),
+        "kspn_pivot_features": (i32, [vp, P(Graph), vp, u32, vp, vp]),
+        "kspn_mlp_forward": (i32, [vp, vp, vp, u32, vp, vp]),
+        "kspn_mlp_grad": (i32, [vp, vp, vp, vp, u32, vp, vp, vp]),
+        "kspn_mlp_adam": (i32, [vp, vp, vp, vp, u32, C.c_float, vp]),
+        "kspn_scc_sizes": (i32, [vp, u32, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -12,6 +12,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "kspn_internal.h"
 
@@ -110,6 +112,74 @@ __global__ void k_gen_lattice(uint64_t seed, uint32_t side, uint32_t tb, uint64_
     }
     keys[2 * e] = k0;
     keys[2 * e + 1] = k1;
+  }
+}
+
+// cycle-walking bijection of [0, n) from the bijection of [0, 2^bits)
+__device__ inline uint32_t perm_n(uint64_t pseed, uint32_t bits, uint32_t n, uint32_t x) {
+  uint32_t y = perm_pow2(pseed, bits, x);
+  while (y >= n) y = perm_pow2(pseed, bits, y);
+  return y;
+}
+
+// Planted power-law SCC graph. Positions [start[c], start[c+1]) form
+// component c, components in topological order; vertex id of position p is
+// perm_n(p). Per position: a cycle edge to the next position of its
+// component (size > 1), `chords` edges to uniform positions of the
+// component (draw p*16+j), `inter` edges to uniform positions after the
+// component (draw p*16+8+j). Slots per position: 1 + chords + inter;
+// absent edges are the sentinel (n << 32).
+__global__ void k_gen_planted(uint32_t n, uint64_t ncomp, const uint32_t* __restrict__ start, uint64_t pseed,
+                              uint64_t eseed, uint32_t bits, uint32_t chords, uint32_t inter, uint64_t* __restrict__ keys) {
+  const uint32_t slots = 1 + chords + inter;
+  const uint64_t none = (uint64_t)n << 32;
+  for (uint64_t p = blockIdx.x * (uint64_t)WG + threadIdx.x; p < n; p += (uint64_t)gridDim.x * WG) {
+    uint64_t lo = 0, hi = ncomp;  // last c with start[c] <= p
+    while (hi - lo > 1) {
+      uint64_t mid = (lo + hi) >> 1;
+      if (start[mid] <= p) lo = mid; else hi = mid;
+    }
+    const uint32_t s0 = start[lo], len = start[lo + 1] - s0, q = (uint32_t)(p - s0);
+    const uint32_t u = perm_n(pseed, bits, n, (uint32_t)p);
+    uint64_t* out = keys + p * slots;
+    uint32_t k = 0;
+    if (len > 1) {
+      uint32_t nx = perm_n(pseed, bits, n, s0 + (q + 1) % len);
+      out[k++] = ((uint64_t)u << 32) | nx;
+      for (uint32_t j = 0; j < chords; ++j) {
+        uint64_t r = splitmix_at(eseed, p * 16 + j);
+        uint32_t t = perm_n(pseed, bits, n, s0 + mulhi32((uint32_t)(r >> 32), len));
+        out[k++] = t != u ? (((uint64_t)u << 32) | t) : none;
+      }
+    } else {
+      for (uint32_t j = 0; j <= chords; ++j) out[k++] = none;
+    }
+    const uint64_t rest = (uint64_t)n - (s0 + len);
+    for (uint32_t j = 0; j < inter; ++j) {
+      if (rest > 0) {
+        uint64_t r = splitmix_at(eseed, p * 16 + 8 + j);
+        uint64_t off = ((r >> 32) * rest) >> 32;
+        uint32_t t = perm_n(pseed, bits, n, (uint32_t)(s0 + len + off));
+        out[k++] = ((uint64_t)u << 32) | t;
+      } else {
+        out[k++] = none;
+      }
+    }
+  }
+}
+
+// planted canonical labels: minimum vertex id per component
+__global__ void k_planted_min(uint32_t n, uint64_t ncomp, const uint32_t* __restrict__ start, uint64_t pseed, uint32_t bits,
+                              uint32_t* __restrict__ cmin, int apply, uint32_t* __restrict__ labels) {
+  for (uint64_t p = blockIdx.x * (uint64_t)WG + threadIdx.x; p < n; p += (uint64_t)gridDim.x * WG) {
+    uint64_t lo = 0, hi = ncomp;
+    while (hi - lo > 1) {
+      uint64_t mid = (lo + hi) >> 1;
+      if (start[mid] <= p) lo = mid; else hi = mid;
+    }
+    uint32_t u = perm_n(pseed, bits, n, (uint32_t)p);
+    if (!apply) atomicMin(&cmin[lo], u);
+    else labels[u] = cmin[lo];
   }
 }
 
@@ -222,6 +292,31 @@ int transpose(uint32_t n, uint64_t m, const uint64_t* off, const uint32_t* adj, 
   return KSPN_OK;
 }
 
+// component starts of the planted graph (host; the same draws as the
+// oracle's orc_planted_sizes): size_i = floor((1-u)^(-1/(expo-1))) clamped
+// to [1, max_size], u from draw i of the size stream, until the sum is n
+int planted_starts(const kspn_gen_params* p, std::vector<uint32_t>* starts) {
+  const uint64_t n = p->p0;
+  const double expo = p->a;
+  const uint32_t max_size = (uint32_t)std::max<uint64_t>(1, p->p1);
+  if (!(expo > 1.0)) return KSPN_E_INVALID;
+  uint64_t sseed = mix64(p->seed ^ 0x53495A4553ull);
+  starts->clear();
+  starts->push_back(0);
+  uint64_t total = 0, k = 0;
+  while (total < n) {
+    uint64_t r = splitmix_at(sseed, k++);
+    double u = (double)(r >> 11) * (1.0 / 9007199254740992.0);
+    double x = std::pow(1.0 - u, -1.0 / (expo - 1.0));
+    uint32_t sz = (x >= (double)max_size) ? max_size : (uint32_t)x;
+    if (sz < 1) sz = 1;
+    if (total + sz > n) sz = (uint32_t)(n - total);
+    total += sz;
+    starts->push_back((uint32_t)total);
+  }
+  return KSPN_OK;
+}
+
 int build_both(uint32_t n, uint64_t count, uint64_t* keys, uint64_t* alt, uint64_t* m, uint64_t** fw_off, uint32_t** fw_adj,
                uint64_t** bw_off, uint32_t** bw_adj, hipStream_t s) {
   uint64_t *fo = nullptr, *bo = nullptr;
@@ -289,6 +384,11 @@ int kspn_generate(kspn_ctx* ctx, const kspn_gen_params* p, uint32_t* n_out, uint
     if (p->p0 < 2 || p->p0 > 1625) return KSPN_E_INVALID;
     n = p->p0 * p->p0 * p->p0;
     count = 2 * 3 * p->p0 * p->p0 * (p->p0 - 1);
+  } else if (p->kind == KSPN_GEN_PLANTED) {
+    n = p->p0;
+    uint32_t chords = (uint32_t)(p->b - 1.0 + 0.5), inter = (uint32_t)(p->c + 0.5);
+    if (p->b < 1.0 || p->c < 0.0 || chords > 8 || inter > 8) return KSPN_E_INVALID;
+    count = n * (1 + chords + inter);
   } else {
     return KSPN_E_INVALID;
   }
@@ -304,13 +404,46 @@ int kspn_generate(kspn_ctx* ctx, const kspn_gen_params* p, uint32_t* n_out, uint
       uint64_t pseed = mix64(p->seed ^ 0x5045524D55544531ull);
       k_gen_rmat<<<gen_grid(count), WG, 0, s>>>(p->seed, (uint32_t)p->p0, count, prob32(p->a), prob32(p->a + p->b),
                                                 prob32(p->a + p->b + p->c), (int)p->permute, pseed, keys);
-    } else {
+    } else if (p->kind == KSPN_GEN_LATTICE) {
       k_gen_lattice<<<gen_grid(count / 2), WG, 0, s>>>(p->seed, (uint32_t)p->p0, prob32(p->a), keys);
+    } else {
+      std::vector<uint32_t> starts;
+      KSPN_TRY(planted_starts(p, &starts));
+      DevBuf ds;
+      KSPN_TRY(ds.alloc(starts.size() * sizeof(uint32_t)));
+      KSPN_HIP(hipMemcpy(ds.p, starts.data(), starts.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      uint32_t chords = (uint32_t)(p->b - 1.0 + 0.5), inter = (uint32_t)(p->c + 0.5);
+      k_gen_planted<<<gen_grid(n), WG, 0, s>>>((uint32_t)n, starts.size() - 1, (const uint32_t*)ds.p,
+                                               mix64(p->seed ^ 0x544F504F4C4F4759ull), mix64(p->seed ^ 0x45444745ull),
+                                               bits_for(n - 1), chords, inter, keys);
+      KSPN_HIP(hipGetLastError());
+      KSPN_HIP(hipStreamSynchronize(s));  // ds is freed at scope exit
     }
     KSPN_HIP(hipGetLastError());
   }
   *n_out = (uint32_t)n;
   return build_both((uint32_t)n, count, keys, (uint64_t*)b.p, m, fw_off, fw_adj, bw_off, bw_adj, s);
+}
+
+int kspn_planted_labels(kspn_ctx* ctx, const kspn_gen_params* p, uint32_t* labels, void* stream) {
+  (void)ctx;
+  if (!p || p->kind != KSPN_GEN_PLANTED || !labels || p->p0 == 0 || p->p0 >= 0xFFFFFFFFull) return KSPN_E_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t n = (uint32_t)p->p0;
+  std::vector<uint32_t> starts;
+  KSPN_TRY(planted_starts(p, &starts));
+  uint64_t nc = starts.size() - 1;
+  DevBuf ds, dm;
+  KSPN_TRY(ds.alloc(starts.size() * sizeof(uint32_t)));
+  KSPN_TRY(dm.alloc(nc * sizeof(uint32_t)));
+  KSPN_HIP(hipMemcpy(ds.p, starts.data(), starts.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  KSPN_HIP(hipMemset(dm.p, 0xFF, nc * sizeof(uint32_t)));
+  uint64_t pseed = mix64(p->seed ^ 0x544F504F4C4F4759ull);
+  k_planted_min<<<gen_grid(n), WG, 0, s>>>(n, nc, (const uint32_t*)ds.p, pseed, bits_for(n - 1), (uint32_t*)dm.p, 0, labels);
+  k_planted_min<<<gen_grid(n), WG, 0, s>>>(n, nc, (const uint32_t*)ds.p, pseed, bits_for(n - 1), (uint32_t*)dm.p, 1, labels);
+  KSPN_HIP(hipGetLastError());
+  KSPN_HIP(hipStreamSynchronize(s));
+  return KSPN_OK;
 }
 
 int kspn_free(void* ptr) {

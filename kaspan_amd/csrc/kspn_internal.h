@@ -237,7 +237,8 @@ struct Ctx {
   uint64_t* sub_off = nullptr;
   uint32_t* sub_adj = nullptr;
   uint64_t cap_sub_m = 0;
-  uint32_t* tj = nullptr;  // Tarjan work arrays (5 * n words)
+  uint32_t* tj = nullptr;  // Tarjan work arrays (6 * n words)
+  uint32_t* mlp_buf = nullptr;  // learned pivot: candidates, histogram, features, scores
   // profiling
   bool profile = false;
   std::vector<KernTimer> kt;
@@ -261,4 +262,11 @@ inline uint32_t grid_for(uint64_t items, uint32_t per_block = WG, uint32_t cap =
   return (uint32_t)g;
 }
 
+int select_pivot_learned_impl(Ctx* c, const kspn_graph* g, const float* params, hipStream_t s, uint32_t* pivot);
+
 }  // namespace kspn
+
+// the opaque handle of the C ABI
+struct kspn_ctx {
+  kspn::Ctx c;
+};

@@ -1,0 +1,489 @@
+// kspn_mlp.hip -- learned pivot scorer: features, MLP inference (BF16
+// MFMA, FP32 accumulation), FP32 training kernels, Adam, outcome logging.
+//
+// No counterpart in the reference: KaSpan picks the pivot with the plain
+// degree-product rule (scc/trim_1_local.hpp:32-45, scc/pivot.hpp:53-74),
+// which stays the fallback. The scorer only changes which vertex seeds
+// FW-BW, never the labels.
+//
+// MLP: x[6] -> relu(W1 x + b1)[32] -> relu(W2 h1 + b2)[32] -> w3 . h2 + b3.
+// Parameter vector (float, KSPN_MLP_PARAMS = 1313): W1[32][6], b1[32],
+// W2[32][32], b2[32], w3[32], b3.
+#include <cmath>
+
+#include "kspn_internal.h"
+
+namespace kspn {
+
+constexpr int P_W1 = 0, P_B1 = 192, P_W2 = 224, P_B2 = 1248, P_W3 = 1280, P_B3 = 1312, P_COUNT = 1313;
+static_assert(P_COUNT == KSPN_MLP_PARAMS, "parameter layout");
+
+// ---------------------------------------------------------------- features
+// One wavefront per vertex: in/out degree, their logs, and the 2-hop sums
+// (out-degrees of out-neighbours, in-degrees of in-neighbours) reduced
+// across lanes, so hub rows are split over 64 lanes.
+__global__ __launch_bounds__(WG) void k_features(uint32_t count, const uint32_t* __restrict__ list,
+                                                 const uint64_t* __restrict__ fw_off, const uint32_t* __restrict__ fw_adj,
+                                                 const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
+                                                 float* __restrict__ feat) {
+  const uint32_t lane = lane_id();
+  const uint64_t wave = (blockIdx.x * (uint64_t)WG + threadIdx.x) / WAVE;
+  const uint64_t waves = (uint64_t)gridDim.x * WG / WAVE;
+  for (uint64_t i = wave; i < count; i += waves) {
+    const uint32_t v = list ? list[i] : (uint32_t)i;
+    const uint64_t fb = fw_off[v], fe = fw_off[v + 1], bb = bw_off[v], be = bw_off[v + 1];
+    uint64_t s2o = 0, s2i = 0;
+    for (uint64_t it = fb + lane; it < fe; it += WAVE) {
+      uint32_t w = fw_adj[it];
+      s2o += fw_off[w + 1] - fw_off[w];
+    }
+    for (uint64_t it = bb + lane; it < be; it += WAVE) {
+      uint32_t w = bw_adj[it];
+      s2i += bw_off[w + 1] - bw_off[w];
+    }
+    s2o = wave_sum64(s2o);
+    s2i = wave_sum64(s2i);
+    if (lane == 0) {
+      const uint64_t out = fe - fb, in = be - bb;
+      const float li = log1pf((float)in), lo = log1pf((float)out);
+      float* f = feat + i * 6;
+      f[0] = li;
+      f[1] = lo;
+      f[2] = li * lo * 0.125f;
+      f[3] = log1pf((float)s2o);
+      f[4] = log1pf((float)s2i);
+      f[5] = log1pf((float)(in < out ? in : out));
+    }
+  }
+}
+
+// ---------------------------------------------------------------- inference
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16 bits, round to nearest even (finite inputs)
+__device__ inline short to_bf16(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+// Each wavefront scores 16 rows per step with v_mfma_f32_16x16x16_bf16:
+// A (16x16, rows x k) holds in lane l row l%16, k = 4*(l/16)..+3; B (16x16,
+// k x n) holds in lane l column l%16, k = 4*(l/16)..+3; D holds in lane l
+// column l%16, rows 4*(l/16)..+3. Layer outputs go through LDS to be read
+// back in the A layout. Weights are rounded to BF16 once per workgroup.
+// A workgroup scores 64 rows per step (4 wavefronts x 16), so every
+// wavefront runs the same number of steps and barriers are uniform.
+__global__ __launch_bounds__(WG) void k_mlp_forward(uint32_t rows, const float* __restrict__ x,
+                                                    const float* __restrict__ params, float* __restrict__ out) {
+  __shared__ short s_w1[32 * 16];   // W1 as [n][k], k padded 6 -> 16
+  __shared__ short s_w2[32 * 32];   // W2 as [n][k]
+  __shared__ float s_b1[32], s_b2[32], s_w3[32];
+  __shared__ float s_h[WG / WAVE][16 * 33];  // per wavefront: 16 rows x 32 (+1 pad)
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < 32 * 16; i += WG) {
+    uint32_t n = i / 16, k = i % 16;
+    s_w1[i] = k < 6 ? to_bf16(params[P_W1 + n * 6 + k]) : (short)0;
+  }
+  for (uint32_t i = t; i < 32 * 32; i += WG) s_w2[i] = to_bf16(params[P_W2 + i]);
+  if (t < 32) {
+    s_b1[t] = params[P_B1 + t];
+    s_b2[t] = params[P_B2 + t];
+    s_w3[t] = params[P_W3 + t];
+  }
+  __syncthreads();
+  const float b3 = params[P_B3];
+  const uint32_t lane = lane_id(), wv = t / WAVE;
+  const uint32_t col = lane % 16, kq = lane / 16;  // column / k-quarter (row-quarter for D)
+  float* h = s_h[wv];
+  const uint64_t chunks = (rows + 63) / 64;
+  for (uint64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+    const uint64_t r0 = ch * 64 + wv * 16;
+    // layer 1: A = X rows (k < 6), two n-tiles of 16
+    v4s a;
+    for (int j = 0; j < 4; ++j) {
+      uint32_t k = 4 * kq + j;
+      uint64_t r = r0 + col;
+      a[j] = (k < 6 && r < rows) ? to_bf16(x[r * 6 + k]) : (short)0;
+    }
+    for (int nt = 0; nt < 2; ++nt) {
+      v4s b;
+      for (int j = 0; j < 4; ++j) b[j] = s_w1[(nt * 16 + col) * 16 + 4 * kq + j];
+      v4f d = {0.f, 0.f, 0.f, 0.f};
+      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, d, 0, 0, 0);
+      for (int j = 0; j < 4; ++j) {
+        uint32_t row = 4 * kq + j, n = nt * 16 + col;
+        h[row * 33 + n] = fmaxf(d[j] + s_b1[n], 0.f);
+      }
+    }
+    __syncthreads();
+    // layer 2: A = H1 (16 x 32), two k-steps, two n-tiles
+    v4s a2[2];
+    for (int ks = 0; ks < 2; ++ks)
+      for (int j = 0; j < 4; ++j) a2[ks][j] = to_bf16(h[col * 33 + ks * 16 + 4 * kq + j]);
+    __syncthreads();
+    for (int nt = 0; nt < 2; ++nt) {
+      v4f d = {0.f, 0.f, 0.f, 0.f};
+      for (int ks = 0; ks < 2; ++ks) {
+        v4s b;
+        for (int j = 0; j < 4; ++j) b[j] = s_w2[(nt * 16 + col) * 32 + ks * 16 + 4 * kq + j];
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[ks], b, d, 0, 0, 0);
+      }
+      for (int j = 0; j < 4; ++j) {
+        uint32_t row = 4 * kq + j, n = nt * 16 + col;
+        h[row * 33 + n] = fmaxf(d[j] + s_b2[n], 0.f);
+      }
+    }
+    __syncthreads();
+    // layer 3: 32 -> 1 dot product per row (lanes 0..15), FP32
+    if (lane < 16) {
+      uint64_t r = r0 + lane;
+      float acc = b3;
+      for (int k = 0; k < 32; ++k) acc += s_w3[k] * h[lane * 33 + k];
+      if (r < rows) out[r] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- training
+// FP32 forward + backward for loss = 0.5 * mean((y - t)^2) over `rows`.
+// Each workgroup takes batches of 128 rows: activations in LDS, weight
+// gradients reduced over the batch by thread-per-output loops, then one
+// atomicAdd per parameter per batch into grad[]. loss_sum += sum of 0.5 e^2.
+constexpr int TB = 128;
+__global__ __launch_bounds__(WG) void k_mlp_grad(uint32_t rows, const float* __restrict__ x,
+                                                 const float* __restrict__ target, const float* __restrict__ params,
+                                                 float inv_rows, float* __restrict__ grad, float* loss_sum) {
+  __shared__ float s_w1[192], s_w2[1024], s_b1[32], s_b2[32], s_w3[32];
+  __shared__ float s_x[TB * 6];
+  __shared__ float s_a1[TB * 33], s_a2[TB * 33];  // pre-activations (+pad)
+  __shared__ float s_g[TB * 33];                  // dH2 then dH1 (on pre-activations)
+  __shared__ float s_dy[TB];
+  __shared__ float s_loss[TB];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < 192; i += WG) s_w1[i] = params[P_W1 + i];
+  for (uint32_t i = t; i < 1024; i += WG) s_w2[i] = params[P_W2 + i];
+  if (t < 32) {
+    s_b1[t] = params[P_B1 + t];
+    s_b2[t] = params[P_B2 + t];
+    s_w3[t] = params[P_W3 + t];
+  }
+  const float b3 = params[P_B3];
+  __syncthreads();
+  const uint32_t batches = (rows + TB - 1) / TB;
+  for (uint32_t bi = blockIdx.x; bi < batches; bi += gridDim.x) {
+    const uint32_t r0 = bi * TB;
+    const uint32_t nb = min((uint32_t)TB, rows - r0);
+    for (uint32_t i = t; i < TB * 6; i += WG) s_x[i] = (i / 6 < nb) ? x[(uint64_t)r0 * 6 + i] : 0.f;
+    __syncthreads();
+    // layer 1 pre-activations
+    for (uint32_t i = t; i < TB * 32; i += WG) {
+      uint32_t r = i / 32, o = i % 32;
+      float acc = s_b1[o];
+      for (int k = 0; k < 6; ++k) acc += s_w1[o * 6 + k] * s_x[r * 6 + k];
+      s_a1[r * 33 + o] = acc;
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < TB * 32; i += WG) {
+      uint32_t r = i / 32, o = i % 32;
+      float acc = s_b2[o];
+      for (int k = 0; k < 32; ++k) acc += s_w2[o * 32 + k] * fmaxf(s_a1[r * 33 + k], 0.f);
+      s_a2[r * 33 + o] = acc;
+    }
+    __syncthreads();
+    float lpart = 0.f;
+    if (t < TB) {
+      float y = b3;
+      for (int k = 0; k < 32; ++k) y += s_w3[k] * fmaxf(s_a2[t * 33 + k], 0.f);
+      float e = (t < nb) ? y - target[r0 + t] : 0.f;
+      s_dy[t] = e * inv_rows;
+      lpart = 0.5f * e * e;
+    }
+    __syncthreads();
+    // dH2 (pre-activation) = dy * w3 * [a2 > 0]
+    for (uint32_t i = t; i < TB * 32; i += WG) {
+      uint32_t r = i / 32, o = i % 32;
+      s_g[r * 33 + o] = (s_a2[r * 33 + o] > 0.f) ? s_dy[r] * s_w3[o] : 0.f;
+    }
+    __syncthreads();
+    // dW3, db3, dW2, db2 (one parameter per thread, summed over the batch)
+    for (uint32_t p = t; p < 32 + 1 + 1024 + 32; p += WG) {
+      float acc = 0.f;
+      if (p < 32) {
+        for (uint32_t r = 0; r < nb; ++r) acc += s_dy[r] * fmaxf(s_a2[r * 33 + p], 0.f);
+        atomicAdd(&grad[P_W3 + p], acc);
+      } else if (p == 32) {
+        for (uint32_t r = 0; r < nb; ++r) acc += s_dy[r];
+        atomicAdd(&grad[P_B3], acc);
+      } else if (p < 33 + 1024) {
+        uint32_t q = p - 33, o = q / 32, k = q % 32;
+        for (uint32_t r = 0; r < nb; ++r) acc += s_g[r * 33 + o] * fmaxf(s_a1[r * 33 + k], 0.f);
+        atomicAdd(&grad[P_W2 + q], acc);
+      } else {
+        uint32_t o = p - 33 - 1024;
+        for (uint32_t r = 0; r < nb; ++r) acc += s_g[r * 33 + o];
+        atomicAdd(&grad[P_B2 + o], acc);
+      }
+    }
+    __syncthreads();
+    // dH1 (pre-activation) = (W2^T dH2) * [a1 > 0], written over s_a1's sign
+    for (uint32_t i = t; i < TB * 32; i += WG) {
+      uint32_t r = i / 32, k = i % 32;
+      float acc = 0.f;
+      for (int o = 0; o < 32; ++o) acc += s_w2[o * 32 + k] * s_g[r * 33 + o];
+      s_a2[r * 33 + k] = (s_a1[r * 33 + k] > 0.f) ? acc : 0.f;  // s_a2 now holds dH1
+    }
+    __syncthreads();
+    for (uint32_t p = t; p < 192 + 32; p += WG) {
+      float acc = 0.f;
+      if (p < 192) {
+        uint32_t o = p / 6, k = p % 6;
+        for (uint32_t r = 0; r < nb; ++r) acc += s_a2[r * 33 + o] * s_x[r * 6 + k];
+        atomicAdd(&grad[P_W1 + p], acc);
+      } else {
+        uint32_t o = p - 192;
+        for (uint32_t r = 0; r < nb; ++r) acc += s_a2[r * 33 + o];
+        atomicAdd(&grad[P_B1 + o], acc);
+      }
+    }
+    // batch loss
+    if (t < TB) s_loss[t] = lpart;
+    __syncthreads();
+    if (t == 0) {
+      float l = 0.f;
+      for (uint32_t r = 0; r < TB; ++r) l += s_loss[r];
+      atomicAdd(loss_sum, l);
+    }
+    __syncthreads();
+  }
+}
+
+// Adam (Kingma & Ba 2015) on the parameter vector; state = [m | v]
+__global__ void k_adam(float* __restrict__ params, const float* __restrict__ grad, float* __restrict__ state, uint32_t count,
+                       float lr, float b1, float b2, float eps, float c1, float c2) {
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < count; i += gridDim.x * WG) {
+    float g = grad[i];
+    float m = b1 * state[i] + (1.f - b1) * g;
+    float v = b2 * state[count + i] + (1.f - b2) * g * g;
+    state[i] = m;
+    state[count + i] = v;
+    params[i] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+  }
+}
+
+// ---------------------------------------------------------------- outcomes
+// SCC size of every vertex from canonical labels: count per representative
+// (wavefront-grouped atomics), then gather
+__global__ __launch_bounds__(WG) void k_count_labels(uint32_t n, const uint32_t* __restrict__ labels, uint32_t* cnt) {
+  const uint64_t pad = ((uint64_t)n + WG - 1) / WG * WG;
+  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < pad; base += (uint64_t)gridDim.x * WG) {
+    uint64_t v = base + threadIdx.x;
+    bool valid = v < n;
+    uint32_t rep = valid ? labels[v] : 0u;
+    if (rep >= n) valid = false;
+    uint64_t pending = __ballot(valid);
+    while (pending) {
+      uint32_t leader = __ffsll((unsigned long long)pending) - 1;
+      uint32_t r = __shfl(rep, leader);
+      uint64_t grp = __ballot(valid && rep == r);
+      if (lane_id() == leader) atomicAdd(&cnt[r], (uint32_t)__popcll(grp));
+      pending &= ~grp;
+    }
+  }
+}
+__global__ void k_gather_sizes(uint32_t n, const uint32_t* __restrict__ labels, const uint32_t* __restrict__ cnt,
+                               uint32_t* __restrict__ sizes) {
+  for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG) {
+    uint32_t r = labels[v];
+    sizes[v] = r < n ? cnt[r] : 0u;
+  }
+}
+
+// ---------------------------------------------------------------- candidates
+// log2 histogram of in*out (original degrees) over live vertices: bucket
+// b = 64 - clz(prod) (0 for prod 0)
+__global__ __launch_bounds__(WG) void k_prod_hist(uint32_t n, const uint32_t* __restrict__ live,
+                                                  const uint64_t* __restrict__ fw_off,
+                                                  const uint64_t* __restrict__ bw_off, uint32_t* hist) {
+  __shared__ uint32_t s_h[65];
+  for (uint32_t i = threadIdx.x; i < 65; i += WG) s_h[i] = 0;
+  __syncthreads();
+  for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG) {
+    if (!bm_get(live, (uint32_t)v)) continue;
+    uint64_t p = (fw_off[v + 1] - fw_off[v]) * (bw_off[v + 1] - bw_off[v]);
+    atomicAdd(&s_h[p ? 64 - __clzll(p) : 0], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < 65; i += WG)
+    if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+// live vertices with bucket >= b_min, at most cap of them
+__global__ __launch_bounds__(WG) void k_collect_cand(uint32_t n, const uint32_t* __restrict__ live,
+                                                     const uint64_t* __restrict__ fw_off,
+                                                     const uint64_t* __restrict__ bw_off, uint32_t b_min, uint32_t cap,
+                                                     uint32_t* __restrict__ list, uint32_t* count) {
+  for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG) {
+    if (!bm_get(live, (uint32_t)v)) continue;
+    uint64_t p = (fw_off[v + 1] - fw_off[v]) * (bw_off[v + 1] - bw_off[v]);
+    uint32_t b = p ? 64 - __clzll(p) : 0;
+    if (b < b_min) continue;
+    uint32_t slot = atomicAdd(count, 1u);
+    if (slot < cap) list[slot] = (uint32_t)v;
+  }
+}
+
+// argmax of score over list (ties: smallest vertex id) -> *out
+__global__ __launch_bounds__(1024) void k_argmax_score(uint32_t count, const uint32_t* __restrict__ list,
+                                                       const float* __restrict__ score, uint32_t* out) {
+  __shared__ float s_s[1024];
+  __shared__ uint32_t s_v[1024];
+  float best = -INFINITY;
+  uint32_t bv = 0xFFFFFFFFu;
+  for (uint32_t i = threadIdx.x; i < count; i += 1024) {
+    float s = score[i];
+    uint32_t v = list[i];
+    if (s > best || (s == best && v < bv)) { best = s; bv = v; }
+  }
+  s_s[threadIdx.x] = best;
+  s_v[threadIdx.x] = bv;
+  __syncthreads();
+  for (uint32_t st = 512; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      float s2 = s_s[threadIdx.x + st];
+      uint32_t v2 = s_v[threadIdx.x + st];
+      if (s2 > s_s[threadIdx.x] || (s2 == s_s[threadIdx.x] && v2 < s_v[threadIdx.x])) {
+        s_s[threadIdx.x] = s2;
+        s_v[threadIdx.x] = v2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s_v[0];
+}
+
+// host helpers used by the pipeline (kspn_scc.hip)
+int mlp_features(Ctx* c, const kspn_graph* g, const uint32_t* list, uint32_t count, float* feat, hipStream_t s) {
+  if (count == 0) return KSPN_OK;
+  uint64_t waves_needed = count;
+  uint32_t grid = (uint32_t)std::min<uint64_t>((waves_needed * WAVE + WG - 1) / WG, (uint64_t)c->num_cu * 16);
+  k_features<<<grid, WG, 0, s>>>(count, list, g->fw_off, g->fw_adj, g->bw_off, g->bw_adj, feat);
+  KSPN_HIP(hipGetLastError());
+  return KSPN_OK;
+}
+
+int mlp_forward(Ctx* c, const float* params, const float* x, uint32_t rows, float* out, hipStream_t s) {
+  if (rows == 0) return KSPN_OK;
+  uint64_t chunks = (rows + 63) / 64;
+  uint32_t grid = (uint32_t)std::min<uint64_t>(chunks, (uint64_t)c->num_cu * 8);
+  k_mlp_forward<<<grid, WG, 0, s>>>(rows, x, params, out);
+  KSPN_HIP(hipGetLastError());
+  return KSPN_OK;
+}
+
+// Learned pivot among live vertices: candidates = live vertices in the top
+// log2 buckets of in*out (at least `want`, at most `cap`), scored by the
+// MLP; the best score wins (smallest id on ties). Returns n if none.
+int select_pivot_learned_impl(Ctx* c, const kspn_graph* g, const float* params, hipStream_t s, uint32_t* pivot) {
+  const uint32_t want = 1024, cap = 4096;
+  if (!c->mlp_buf) {
+    // list[4096] | hist[65] | count + pivot | features[4096*6] | scores[4096]
+    if (hipMalloc((void**)&c->mlp_buf, sizeof(uint32_t) * (4096 + 128 + 4096 * 6 + 4096)) != hipSuccess) return KSPN_E_OOM;
+  }
+  uint32_t* list = c->mlp_buf;
+  uint32_t* hist = c->mlp_buf + 4096;
+  uint32_t* cnt = c->mlp_buf + 4096 + 96;
+  float* feat = reinterpret_cast<float*>(c->mlp_buf + 4096 + 128);
+  float* score = feat + 4096 * 6;
+  KSPN_HIP(hipMemsetAsync(hist, 0, 65 * sizeof(uint32_t), s));
+  uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)g->n + WG - 1) / WG, (uint64_t)c->num_cu * 16);
+  k_prod_hist<<<grid, WG, 0, s>>>(g->n, c->live, g->fw_off, g->bw_off, hist);
+  uint32_t h[65];
+  KSPN_HIP(hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, s));
+  KSPN_HIP(hipStreamSynchronize(s));
+  uint32_t b_min = 65, acc = 0;
+  while (b_min > 0 && acc < want) acc += h[--b_min];
+  if (acc == 0) {
+    *pivot = g->n;
+    return KSPN_OK;
+  }
+  KSPN_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), s));
+  k_collect_cand<<<grid, WG, 0, s>>>(g->n, c->live, g->fw_off, g->bw_off, b_min, cap, list, cnt);
+  uint32_t nc = 0;
+  KSPN_HIP(hipMemcpyAsync(&nc, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  KSPN_HIP(hipStreamSynchronize(s));
+  nc = std::min(nc, cap);
+  KSPN_TRY(mlp_features(c, g, list, nc, feat, s));
+  KSPN_TRY(mlp_forward(c, params, feat, nc, score, s));
+  k_argmax_score<<<1, 1024, 0, s>>>(nc, list, score, cnt + 1);
+  uint32_t pv = 0;
+  KSPN_HIP(hipMemcpyAsync(&pv, cnt + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  KSPN_HIP(hipStreamSynchronize(s));
+  *pivot = pv == 0xFFFFFFFFu ? g->n : pv;
+  return KSPN_OK;
+}
+
+}  // namespace kspn
+
+using namespace kspn;
+
+extern "C" {
+
+int kspn_pivot_features(kspn_ctx* k, const kspn_graph* g, const uint32_t* vertices, uint32_t count, float* feat,
+                        void* stream) {
+  if (!k || !g || (count && !feat) || (g->n > 0 && (!g->fw_off || !g->bw_off))) return KSPN_E_INVALID;
+  if (hipSetDevice(k->c.device) != hipSuccess) return KSPN_E_HIP;
+  return mlp_features(&k->c, g, vertices, count, feat, (hipStream_t)stream);
+}
+
+int kspn_mlp_forward(kspn_ctx* k, const float* params, const float* x, uint32_t rows, float* out, void* stream) {
+  if (!k || !params || (rows && (!x || !out))) return KSPN_E_INVALID;
+  if (hipSetDevice(k->c.device) != hipSuccess) return KSPN_E_HIP;
+  return mlp_forward(&k->c, params, x, rows, out, (hipStream_t)stream);
+}
+
+int kspn_mlp_grad(kspn_ctx* k, const float* params, const float* x, const float* target, uint32_t rows, float* grad,
+                  float* loss, void* stream) {
+  if (!k || !params || !grad || !loss || (rows && (!x || !target))) return KSPN_E_INVALID;
+  if (hipSetDevice(k->c.device) != hipSuccess) return KSPN_E_HIP;
+  hipStream_t s = (hipStream_t)stream;
+  KSPN_HIP(hipMemsetAsync(grad, 0, KSPN_MLP_PARAMS * sizeof(float), s));
+  KSPN_HIP(hipMemsetAsync(loss, 0, sizeof(float), s));
+  if (rows == 0) return KSPN_OK;
+  uint32_t batches = (rows + TB - 1) / TB;
+  uint32_t grid = std::min<uint32_t>(batches, (uint32_t)k->c.num_cu * 4);
+  k_mlp_grad<<<grid, WG, 0, s>>>(rows, x, target, params, 1.0f / rows, grad, loss);
+  KSPN_HIP(hipGetLastError());
+  return KSPN_OK;
+}
+
+int kspn_mlp_adam(kspn_ctx* k, float* params, const float* grad, float* state, uint32_t step, float lr, void* stream) {
+  if (!k || !params || !grad || !state || step == 0) return KSPN_E_INVALID;
+  if (hipSetDevice(k->c.device) != hipSuccess) return KSPN_E_HIP;
+  const float b1 = 0.9f, b2 = 0.999f;
+  float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
+  k_adam<<<(KSPN_MLP_PARAMS + WG - 1) / WG, WG, 0, (hipStream_t)stream>>>(params, grad, state, KSPN_MLP_PARAMS, lr, b1, b2,
+                                                                          1e-8f, c1, c2);
+  KSPN_HIP(hipGetLastError());
+  return KSPN_OK;
+}
+
+int kspn_scc_sizes(kspn_ctx* k, uint32_t n, const uint32_t* labels, uint32_t* sizes, void* stream) {
+  if (!k || (n && (!labels || !sizes))) return KSPN_E_INVALID;
+  if (n == 0) return KSPN_OK;
+  Ctx* c = &k->c;
+  if (hipSetDevice(c->device) != hipSuccess) return KSPN_E_HIP;
+  KSPN_TRY(ctx_reserve(c, n, 0));
+  hipStream_t s = (hipStream_t)stream;
+  KSPN_HIP(hipMemsetAsync(c->col, 0, (uint64_t)n * sizeof(uint32_t), s));
+  uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)n + WG - 1) / WG, (uint64_t)c->num_cu * 16);
+  k_count_labels<<<grid, WG, 0, s>>>(n, labels, c->col);
+  k_gather_sizes<<<grid, WG, 0, s>>>(n, labels, c->col, sizes);
+  KSPN_HIP(hipGetLastError());
+  KSPN_HIP(hipStreamSynchronize(s));
+  return KSPN_OK;
+}
+
+}  // extern "C"

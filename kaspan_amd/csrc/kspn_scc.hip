@@ -1776,7 +1776,11 @@ int run_pipeline(Run& R) {
   uint32_t rounds = R.o.fwbw_rounds ? R.o.fwbw_rounds : 1;
   for (uint32_t k = 0; k < rounds && R.live_n > 0; ++k) {
     if (R.o.residual_threshold && R.live_n <= R.o.residual_threshold) break;
-    if (k > 0 || trim_ex) {
+    if (R.o.pivot == KSPN_PIVOT_LEARNED) {
+      // MLP over the top degree-product candidates (the plain pivot is one of them)
+      KSPN_TRY(select_pivot_learned_impl(c, g, R.o.mlp_params, R.s, &pivot));
+      if (k == 0 && st) st->pivot = pivot;
+    } else if (k > 0 || trim_ex) {
       // pivot among live vertices by original degree product
       KSPN_HIP(hipMemsetAsync(c->cnt, 0, sizeof(uint32_t) * C_COUNT_, R.s));
       // reuse k_trim_first's reduction on a live-masked pass
@@ -1856,10 +1860,6 @@ int run_pipeline(Run& R) {
 
 using namespace kspn;
 
-struct kspn_ctx {
-  Ctx c;
-};
-
 extern "C" {
 
 void kspn_opts_default(kspn_opts* o) {
@@ -1924,7 +1924,7 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   Ctx* c = &k->c;
   (void)hipSetDevice(c->device);
   void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
-                  c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj};
+                  c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
@@ -1954,8 +1954,8 @@ int kspn_scc(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts
   if (opts) o = *opts;
   else kspn_opts_default(&o);
   if (o.method > KSPN_METHOD_TRIM_EX) return KSPN_E_INVALID;
-  if (o.pivot == KSPN_PIVOT_LEARNED) return KSPN_E_UNIMPLEMENTED;
   if (o.pivot > KSPN_PIVOT_LEARNED) return KSPN_E_INVALID;
+  if (o.pivot == KSPN_PIVOT_LEARNED && !o.mlp_params) return KSPN_E_INVALID;
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (g->n == 0) return KSPN_OK;
   Ctx* c = &k->c;

@@ -1,0 +1,115 @@
+"""Learned pivot proposal for FW-BW (no counterpart in the reference).
+
+KaSpan seeds forward-backward search with the vertex of maximum
+in-degree x out-degree (scc/trim_1_local.hpp:32-45, scc/pivot.hpp:53-74).
+Here a small MLP (6 -> 32 -> 32 -> 1) scores the top degree-product
+candidates and the best score seeds FW-BW; the plain rule's pivot is always
+among the candidates and remains the fallback. Pivot choice changes time,
+never labels.
+
+The scorer is trained on outcomes the method logs: solving a graph gives
+every vertex's SCC size (the SCC a FW-BW from that vertex would decide),
+and the target is log1p(size). Forward (BF16 MFMA), gradient (FP32) and
+Adam run as libkspn kernels on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+import torch
+
+from . import BidiGraph, Context, _lib, _stream_ptr, scc, scc_sizes
+
+PARAMS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pivot_mlp_params.npy")
+N_PARAMS = _lib.MLP_PARAMS
+
+
+def init_params(seed=0) -> torch.Tensor:
+    """He-initialised parameter vector (W1, b1, W2, b2, w3, b3)."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(N_PARAMS, np.float32)
+    p[0:192] = rng.normal(0, np.sqrt(2 / 6), 192)
+    p[224:1248] = rng.normal(0, np.sqrt(2 / 32), 1024)
+    p[1280:1312] = rng.normal(0, np.sqrt(1 / 32), 32)
+    return torch.from_numpy(p).cuda()
+
+
+def load_params(path=PARAMS_PATH) -> torch.Tensor | None:
+    if not os.path.exists(path):
+        return None
+    return torch.from_numpy(np.load(path, allow_pickle=False).astype(np.float32)).cuda()
+
+
+def features(g: BidiGraph, vertices: torch.Tensor | None = None, ctx: Context | None = None) -> torch.Tensor:
+    ctx = ctx or Context.default()
+    count = g.n if vertices is None else vertices.numel()
+    out = torch.empty((count, 6), dtype=torch.float32, device="cuda")
+    gv = g.view()
+    _lib.check(_lib.lib().kspn_pivot_features(ctx.ptr, C.byref(gv), None if vertices is None else C.c_void_p(vertices.data_ptr()),
+                                              count, C.c_void_p(out.data_ptr()), _stream_ptr()), "kspn_pivot_features")
+    return out
+
+
+def forward(params: torch.Tensor, x: torch.Tensor, ctx: Context | None = None) -> torch.Tensor:
+    ctx = ctx or Context.default()
+    x = x.contiguous()
+    out = torch.empty(x.shape[0], dtype=torch.float32, device="cuda")
+    _lib.check(_lib.lib().kspn_mlp_forward(ctx.ptr, C.c_void_p(params.data_ptr()), C.c_void_p(x.data_ptr()), x.shape[0],
+                                           C.c_void_p(out.data_ptr()), _stream_ptr()), "kspn_mlp_forward")
+    return out
+
+
+def grad(params: torch.Tensor, x: torch.Tensor, target: torch.Tensor, ctx: Context | None = None):
+    """(loss, gradient) of 0.5*mean((MLP(x) - target)^2), FP32 kernels."""
+    ctx = ctx or Context.default()
+    g = torch.empty(N_PARAMS, dtype=torch.float32, device="cuda")
+    loss = torch.empty(1, dtype=torch.float32, device="cuda")
+    x = x.contiguous()
+    target = target.contiguous()
+    _lib.check(_lib.lib().kspn_mlp_grad(ctx.ptr, C.c_void_p(params.data_ptr()), C.c_void_p(x.data_ptr()),
+                                        C.c_void_p(target.data_ptr()), x.shape[0], C.c_void_p(g.data_ptr()),
+                                        C.c_void_p(loss.data_ptr()), _stream_ptr()), "kspn_mlp_grad")
+    return loss, g
+
+
+def candidates(g: BidiGraph, k_top=4096, k_random=4096, seed=0) -> torch.Tensor:
+    """top-k_top vertices by in*out degree plus k_random uniform vertices with both degrees > 0"""
+    outd = (g.fw_off[1:] - g.fw_off[:-1])
+    ind = (g.bw_off[1:] - g.bw_off[:-1])
+    prod = outd * ind
+    alive = torch.nonzero(prod > 0).flatten()
+    if alive.numel() == 0:
+        return alive.to(torch.int32)
+    top = alive[torch.topk(prod[alive], min(k_top, alive.numel())).indices]
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    rnd = alive[torch.randint(0, alive.numel(), (min(k_random, alive.numel()),), device="cuda", generator=gen)]
+    return torch.unique(torch.cat([top, rnd])).to(torch.int32)
+
+
+def outcome_samples(g: BidiGraph, seed=0, **kw):
+    """(features, target) for candidate pivots of g: target = log1p(size of the SCC the pivot would decide)"""
+    labels, _ = scc(g)
+    sizes = scc_sizes(labels)
+    cand = candidates(g, seed=seed, **kw)
+    x = features(g, cand)
+    t = torch.log1p(sizes[cand.long()].float())
+    return x, t
+
+
+def train(x: torch.Tensor, t: torch.Tensor, params: torch.Tensor | None = None, epochs=400, lr=3e-3, seed=0,
+          ctx: Context | None = None):
+    """Full-batch Adam on the GPU kernels; returns (params, losses)."""
+    ctx = ctx or Context.default()
+    params = init_params(seed) if params is None else params.clone()
+    state = torch.zeros(2 * N_PARAMS, dtype=torch.float32, device="cuda")
+    losses = []
+    for step in range(1, epochs + 1):
+        loss, gr = grad(params, x, t, ctx)
+        _lib.check(_lib.lib().kspn_mlp_adam(ctx.ptr, C.c_void_p(params.data_ptr()), C.c_void_p(gr.data_ptr()),
+                                            C.c_void_p(state.data_ptr()), step, lr, _stream_ptr()), "kspn_mlp_adam")
+        if step == 1 or step % 50 == 0 or step == epochs:
+            losses.append(float(loss.item()))
+    return params, losses

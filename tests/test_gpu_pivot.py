@@ -1,0 +1,144 @@
+"""GPU tests of the learned pivot scorer and the planted generator.
+
+Floating point: features and gradients are FP32 on the GPU against the FP64
+oracle; the forward pass rounds weights, inputs and the first hidden layer
+to BF16 (FP32 accumulation) and is compared with a float64 model that rounds
+at the same points. Tolerances are written next to each comparison."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16_round(a):
+    """round float32 values to the nearest BF16 (ties to even), back to float64"""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def split(p):
+    p = np.asarray(p, np.float64)
+    return p[0:192], p[192:224], p[224:1248], p[1248:1280], p[1280:1312], float(p[1312])
+
+
+def forward_bf16_model(x, p):
+    w1, b1, w2, b2, w3, b3 = split(p)
+    xb = bf16_round(x)
+    h1 = np.maximum(xb @ bf16_round(w1).reshape(32, 6).T + b1, 0)
+    h2 = np.maximum(bf16_round(h1) @ bf16_round(w2).reshape(32, 32).T + b2, 0)
+    return h2 @ w3 + b3
+
+
+@pytest.fixture(scope="module")
+def piv(ka):
+    from kaspan_amd import pivot
+
+    return pivot
+
+
+def small_graph(ka):
+    return ka.generate("rmat", 21, scale=13, draws=8 << 13)
+
+
+def test_features_match_oracle(ka, O, piv):
+    import torch
+
+    g = small_graph(ka)
+    fo, fa, bo, ba = g.to_host()
+    hg = O.HostGraph(g.n, fo, fa, bo, ba)
+    ref = O.pivot_features(hg)
+    got = piv.features(g).cpu().numpy()
+    # FP32 log1p on both sides: a few ulp apart
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-5)
+    sub = torch.tensor([5, 77, 1000, 3], dtype=torch.int32, device="cuda")
+    np.testing.assert_allclose(piv.features(g, sub).cpu().numpy(), ref[[5, 77, 1000, 3]], rtol=1e-5, atol=1e-5)
+
+
+def test_forward_bf16_mfma(ka, O, piv):
+    import torch
+
+    rng = np.random.default_rng(1)
+    for rows in (1, 15, 16, 17, 64, 1000, 4099):
+        x = rng.normal(1.0, 1.0, size=(rows, 6)).astype(np.float32)
+        p = piv.init_params(seed=rows).cpu().numpy()
+        p[192:224] = rng.normal(0, 0.1, 32)
+        p[1248:1280] = rng.normal(0, 0.1, 32)
+        p[1312] = 0.3
+        out = piv.forward(torch.from_numpy(p).cuda(), torch.from_numpy(x).cuda()).cpu().numpy().astype(np.float64)
+        model = forward_bf16_model(x, p)
+        # same BF16 rounding points, FP32 vs FP64 accumulation: 1e-4 relative to the output scale
+        scale = max(1.0, float(np.abs(model).max()))
+        assert np.abs(out - model).max() <= 1e-4 * scale, rows
+        # against the unrounded FP64 oracle: BF16 weights/inputs give ~2^-8 relative error per term
+        ref = O.mlp_forward(x, *split(p))
+        assert np.abs(out - ref).max() <= 3e-2 * scale, rows
+
+
+def test_gradients_match_oracle(ka, O, piv):
+    import torch
+
+    rng = np.random.default_rng(2)
+    for rows in (1, 100, 129, 3000):
+        x = rng.normal(1.0, 1.0, size=(rows, 6)).astype(np.float32)
+        t = rng.normal(0.0, 1.0, size=rows).astype(np.float32)
+        p = piv.init_params(seed=7).cpu().numpy()
+        p[192:224] = rng.normal(0, 0.1, 32)
+        loss, g = piv.grad(torch.from_numpy(p).cuda(), torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda())
+        ref_loss, ref = O.mlp_loss_grad(x, t, *split(p))
+        g = g.cpu().numpy().astype(np.float64)
+        ref_flat = np.concatenate([ref[0], ref[1], ref[2], ref[3], ref[4], [ref[5]]])
+        # FP32 sums over the batch vs FP64: 1e-4 relative to the largest gradient entry
+        assert np.abs(g - ref_flat).max() <= 1e-4 * max(1.0, np.abs(ref_flat).max()), rows
+        assert abs(float(loss.item()) / rows - ref_loss) <= 1e-4 * max(1.0, ref_loss)
+
+
+def test_scc_sizes(ka, O):
+    g = ka.generate("gnm", 5, n=20000, draws=30000)
+    labels, _ = ka.scc(g)
+    sizes = ka.scc_sizes(labels).cpu().numpy()
+    lab = labels.cpu().numpy()
+    counts = np.bincount(lab, minlength=g.n)
+    assert np.array_equal(sizes, counts[lab])
+
+
+def test_planted_generator_matches_oracle(ka, O):
+    import helpers as H
+
+    for seed, n in [(3, 1), (4, 777), (5, 20000)]:
+        g = ka.generate("planted", seed, n=n, max_size=5000)
+        hg, planted = O.gen_planted(seed, n, max_size=5000)
+        fo, fa, bo, ba = g.to_host()
+        assert g.m == hg.m
+        assert H.csr_digest(fo, fa) == H.csr_digest(hg.fw_off, hg.fw_adj)
+        assert H.csr_digest(bo, ba) == H.csr_digest(hg.bw_off, hg.bw_adj)
+        lab = ka.planted_labels(seed, n, max_size=5000).cpu().numpy().view(np.uint32)
+        assert np.array_equal(lab, planted)
+        got, _ = ka.scc(g)
+        assert np.array_equal(got.cpu().numpy().view(np.uint32), planted)
+
+
+def test_training_and_learned_pivot(ka, O, piv):
+    import torch
+
+    xs, ts = [], []
+    for seed in range(3):
+        g = ka.generate("planted", 100 + seed, n=60000, max_size=20000)
+        x, t = piv.outcome_samples(g, seed=seed, k_top=1024, k_random=1024)
+        xs.append(x)
+        ts.append(t)
+    x = torch.cat(xs)
+    t = torch.cat(ts)
+    params, losses = piv.train(x, t, epochs=300, lr=3e-3)
+    assert losses[-1] < 0.5 * losses[0], losses
+    # learned pivots: identical labels, FW-BW seeded by the scorer
+    for seed in (200, 201):
+        g = ka.generate("planted", seed, n=60000, max_size=20000)
+        planted = ka.planted_labels(seed, 60000, max_size=20000).cpu().numpy()
+        plain, st_p = ka.scc(g, method=0, fwbw_rounds=4)
+        learned, st_l = ka.scc(g, method=0, fwbw_rounds=4, pivot=1, mlp_params=params)
+        assert np.array_equal(plain.cpu().numpy(), planted)
+        assert np.array_equal(learned.cpu().numpy(), planted)
+        print(f"seed {seed}: FW-BW decided plain={st_p.decided_fwbw} learned={st_l.decided_fwbw}")
+    with pytest.raises(ka.KspnError):
+        ka.scc(g, pivot=1)  # learned pivot without parameters
