@@ -43,8 +43,17 @@ CONFIGS = {
                "G(n,m) n=65536 m=524288 draws, self-loops/duplicates removed"),
     "rmat24": ("rmat", dict(scale=24, draws=16 << 24, a=0.57, b=0.19, c=0.19, permute=True), 0x6B5350AA00000001,
                "R-MAT scale 24, edge factor 16, a=.57 b=.19 c=.19 d=.05, permuted ids, self-loops/duplicates removed"),
+    "planted50m": ("planted", dict(n=50_000_000, max_size=10_000_000, expo=2.1, intra_deg=4.0, inter_deg=3.0),
+                   0x6B5350AA00000002,
+                   "planted SCCs, n=5e7, power-law sizes (exponent 2.1, 1..1e7), Hamiltonian cycle + chords to "
+                   "out-degree 4 inside, 3 forward inter-component edges per vertex"),
     "lattice256": ("lattice", dict(side=256, p_bidir=0.1), 0x6B5350AA00000003,
                    "3-D lattice 256^3, each edge one random direction, 10% bidirectional"),
+}
+# bounded CPU-baseline samples for workloads whose full size takes the serial restatement too long
+CPU_SAMPLE = {
+    "planted50m": ("planted", dict(n=2_000_000, max_size=400_000, expo=2.1, intra_deg=4.0, inter_deg=3.0)),
+    "lattice256": ("lattice", dict(side=96, p_bidir=0.1)),
 }
 
 
@@ -55,6 +64,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="rmat24", choices=sorted(CONFIGS))
     ap.add_argument("--method", type=int, default=None, help="0 kaspan, 1 trim_ex (default: fastest known)")
+    ap.add_argument("--pivot", choices=["plain", "learned"], default="plain")
+    ap.add_argument("--fwbw-rounds", type=int, default=1)
+    ap.add_argument("--ab", action="store_true", help="also time the other pivot rule; report both and the speedup")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -90,21 +102,50 @@ def main():
     method = args.method if args.method is not None else 0
     labels = torch.empty(g.n, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
+    mlp = None
+    if args.pivot == "learned" or args.ab:
+        from kaspan_amd import pivot as P
 
-    for _ in range(args.warmup):
-        ka.scc(g, labels, method=method)
-    torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    stats = None
-    for _ in range(args.steps):
-        _, stats = ka.scc(g, labels, method=method)
-    end.record(stream)
-    torch.cuda.synchronize()
-    ms_per_step = start.elapsed_time(end) / args.steps
+        mlp = P.load_params()
+        if mlp is None:
+            raise SystemExit("no trained scorer: run python -m kaspan_amd.train_pivot first")
+    opts = dict(method=method, fwbw_rounds=args.fwbw_rounds)
+    if args.pivot == "learned":
+        opts.update(pivot=1, mlp_params=mlp)
+
+    def timed(o):
+        for _ in range(args.warmup):
+            ka.scc(g, labels, **o)
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        st = None
+        for _ in range(args.steps):
+            _, st = ka.scc(g, labels, **o)
+        end.record(stream)
+        torch.cuda.synchronize()
+        return start.elapsed_time(end) / args.steps, st
+
+    ms_per_step, stats = timed(opts)
     value = g.m / (ms_per_step / 1e3)
     print(f"[bench] {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+    ab = None
+    if args.ab:
+        import numpy as np
+
+        first = labels.clone()
+        other = dict(method=method, fwbw_rounds=args.fwbw_rounds)
+        if args.pivot == "plain":
+            other.update(pivot=1, mlp_params=mlp)
+        ms_other, st_other = timed(other)
+        same = bool(torch.equal(first, labels))
+        plain_ms, learned_ms = (ms_per_step, ms_other) if args.pivot == "plain" else (ms_other, ms_per_step)
+        ab = {"plain_ms": plain_ms, "learned_ms": learned_ms, "speedup_learned": plain_ms / learned_ms,
+              "labels_identical": same, "fwbw_rounds": args.fwbw_rounds,
+              "fwbw_decided_plain": (stats if args.pivot == "plain" else st_other).decided_fwbw,
+              "fwbw_decided_learned": (st_other if args.pivot == "plain" else stats).decided_fwbw}
+        labels.copy_(first)
 
     # per-kernel profile over further steps (HIP events around every launch)
     roofline = None
@@ -112,7 +153,7 @@ def main():
     if not args.no_profile:
         agg = {}
         for _ in range(args.steps):
-            _, pst = ka.scc(g, labels, method=method, profile=True)
+            _, pst = ka.scc(g, labels, profile=True, **opts)
             for k in pst.as_dict()["kernels"]:
                 a = agg.setdefault(k["name"], {"launches": 0, "ms": 0.0, "bytes": 0.0})
                 a["launches"] += k["launches"]
@@ -136,15 +177,23 @@ def main():
     if not args.no_cpu_baseline:
         from oracle import oracle as O  # checker / CPU baseline only
 
-        fo, fa, bo, ba = g.to_host()
-        hg = O.HostGraph(g.n, fo, fa, bo, ba)
+        if args.config in CPU_SAMPLE:
+            skind, skw = CPU_SAMPLE[args.config]
+            sg = ka.generate(skind, seed, **skw)
+            slab, _ = ka.scc(sg)
+            what = f"{skind} sample {skw} (n={sg.n}, m={sg.m}) of the {args.config} workload"
+        else:
+            sg, slab = g, labels
+            what = f"whole {args.config} graph (n={g.n}, m={g.m})"
+        fo, fa, bo, ba = sg.to_host()
+        hg = O.HostGraph(sg.n, fo, fa, bo, ba)
         t = time.perf_counter()
         ref, _ = O.kaspan_scc(hg)
         cpu_s = time.perf_counter() - t
-        same = bool(np.array_equal(O.canonicalize(ref), labels.cpu().numpy().view(np.uint32)))
-        cpu = {"value": g.m / cpu_s, "unit": "edges/s", "cores": 1, "kind": "port",
-               "sample": f"whole {args.config} graph (n={g.n}, m={g.m}); single-thread C restatement of "
-                         f"kaspan::scc (scc/scc.hpp:21-87), lean form; {cpu_s:.2f}s; canonical labels equal: {same}"}
+        same = bool(np.array_equal(O.canonicalize(ref), slab.cpu().numpy().view(np.uint32)))
+        cpu = {"value": sg.m / cpu_s, "unit": "edges/s", "cores": 1, "kind": "port",
+               "sample": f"{what}; single-thread C restatement of kaspan::scc (scc/scc.hpp:21-87), lean form; "
+                         f"{cpu_s:.2f}s; canonical labels equal to the GPU's: {same}"}
 
     out = {
         "metric": "edges/sec to full SCC labels on 1 MI355X",
@@ -160,7 +209,8 @@ def main():
         "dtype": "uint32",
         "data": "synthetic",
         "config": {"workload": desc, "name": args.config, "n": g.n, "m": g.m, "method": ["kaspan", "trim_ex"][method],
-                   "pivot": "plain", "seed": seed},
+                   "pivot": args.pivot, "fwbw_rounds": args.fwbw_rounds, "seed": seed},
+        "ab": ab,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "phases_ms": {k: getattr(stats, k) for k in ("ms_trim", "ms_fwbw", "ms_color", "ms_residual", "ms_canon")},

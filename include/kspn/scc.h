@@ -55,10 +55,12 @@ typedef struct kspn_graph {
 
 #define KSPN_PIVOT_PLAIN 0   /* max in*out degree product, smallest id on ties (trim_1_local.hpp:32-45) */
 #define KSPN_PIVOT_LEARNED 1 /* MLP score (kspn_mlp_*), plain rule as fallback */
+#define KSPN_PIVOT_RANDOM 2  /* seeded uniform choice per partition (for collecting pivot outcomes) */
 
 #define KSPN_FLAG_PROFILE 1u  /* time every kernel launch with HIP events (slower; fills kspn_stats.kern) */
 #define KSPN_FLAG_NO_TRIM2 2u /* KSPN_METHOD_TRIM_EX without the trim-2 pass */
 #define KSPN_FLAG_NO_TAIL 4u  /* never hand the rest to the single-workgroup tail solver */
+#define KSPN_FLAG_NO_REFINE 8u /* colour rounds over the whole live graph, without partition refinement */
 
 typedef struct kspn_opts {
   uint32_t method;             /* KSPN_METHOD_* */
@@ -71,7 +73,14 @@ typedef struct kspn_opts {
   float alpha;                 /* push->pull switch: edges(frontier) * alpha > edges(unvisited); 0 -> 14 */
   float beta;                  /* pull->push switch: frontier * beta < live; 0 -> 24 */
   const float* mlp_params;     /* device, KSPN_MLP_PARAMS floats, for KSPN_PIVOT_LEARNED */
+  uint32_t* pivot_log;         /* device, KSPN_LOG_WORDS u32 per record, or NULL: outcome of every multi-pivot FW-BW pivot */
+  uint32_t pivot_log_cap;      /* records that fit in pivot_log */
+  uint32_t seed;               /* KSPN_PIVOT_RANDOM */
 } kspn_opts;
+
+/* pivot outcome record: pivot, round, partition size, SCC size (decided),
+ * forward-only, backward-only and unreached vertices of the partition, 0 */
+#define KSPN_LOG_WORDS 8
 
 /* per-kernel profile line, filled with KSPN_FLAG_PROFILE */
 #define KSPN_KERN_MAX 24
@@ -89,6 +98,10 @@ typedef struct kspn_stats {
   uint32_t fwbw_levels_push, fwbw_levels_pull;
   uint32_t color_rounds, color_iters;
   uint32_t residual_n;
+  uint32_t fwbw_rounds_done;  /* multi-pivot FW-BW rounds run */
+  uint32_t partitions_max;    /* largest partition count of those rounds */
+  uint32_t pivot_log_count;   /* records written to opts.pivot_log */
+  uint32_t reserved0;
   uint64_t residual_m;
   uint64_t edges_scanned;  /* adjacency entries read, all phases */
   double bytes_trim, bytes_fwbw, bytes_color, bytes_residual, bytes_canon; /* algorithmic bytes */

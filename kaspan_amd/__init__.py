@@ -219,12 +219,15 @@ def planted_labels(seed, n, device="cuda", ctx: Context | None = None, stream=No
 
 
 def _opts(method=METHOD_KASPAN, pivot=0, fwbw_rounds=1, residual_threshold=0, max_rounds=0, flags=0, alpha=14.0,
-          beta=24.0, mlp_params=None, tail_threshold=0):
+          beta=24.0, mlp_params=None, tail_threshold=0, pivot_log=None, seed=0):
     o = _lib.Opts()
     _lib.lib().kspn_opts_default(C.byref(o))
     o.method, o.pivot, o.fwbw_rounds = method, pivot, fwbw_rounds
     o.residual_threshold, o.max_rounds, o.flags = residual_threshold, max_rounds, flags
-    o.tail_threshold = tail_threshold
+    o.tail_threshold, o.seed = tail_threshold, seed
+    if pivot_log is not None:
+        o.pivot_log = pivot_log.data_ptr()
+        o.pivot_log_cap = pivot_log.numel() // _lib.LOG_WORDS
     o.alpha, o.beta = alpha, beta
     o.mlp_params = mlp_params.data_ptr() if mlp_params is not None else None
     return o

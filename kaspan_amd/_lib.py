@@ -27,9 +27,12 @@ METHOD_KASPAN = 0
 METHOD_TRIM_EX = 1
 PIVOT_PLAIN = 0
 PIVOT_LEARNED = 1
+PIVOT_RANDOM = 2
+LOG_WORDS = 8
 FLAG_PROFILE = 1
 FLAG_NO_TRIM2 = 2
 FLAG_NO_TAIL = 4
+FLAG_NO_REFINE = 8
 GEN_GNM, GEN_RMAT, GEN_LATTICE, GEN_PLANTED = 0, 1, 2, 3
 MLP_PARAMS = 1313
 UNDECIDED = 0xFFFFFFFF
@@ -72,6 +75,9 @@ class Opts(C.Structure):
         ("alpha", C.c_float),
         ("beta", C.c_float),
         ("mlp_params", C.c_void_p),
+        ("pivot_log", C.c_void_p),
+        ("pivot_log_cap", C.c_uint32),
+        ("seed", C.c_uint32),
     ]
 
 
@@ -89,6 +95,8 @@ class Stats(C.Structure):
         ("fwbw_levels_push", C.c_uint32), ("fwbw_levels_pull", C.c_uint32),
         ("color_rounds", C.c_uint32), ("color_iters", C.c_uint32),
         ("residual_n", C.c_uint32),
+        ("fwbw_rounds_done", C.c_uint32), ("partitions_max", C.c_uint32),
+        ("pivot_log_count", C.c_uint32), ("reserved0", C.c_uint32),
         ("residual_m", C.c_uint64),
         ("edges_scanned", C.c_uint64),
         ("bytes_trim", C.c_double), ("bytes_fwbw", C.c_double), ("bytes_color", C.c_double),

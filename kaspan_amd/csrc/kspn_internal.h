@@ -239,6 +239,14 @@ struct Ctx {
   uint64_t cap_sub_m = 0;
   uint32_t* tj = nullptr;  // Tarjan work arrays (6 * n words)
   uint32_t* mlp_buf = nullptr;  // learned pivot: candidates, histogram, features, scores
+  // partitioned FW-BW
+  uint32_t* pid = nullptr;     // partition of every vertex
+  uint64_t* pbest = nullptr;   // per partition: best pivot key
+  uint32_t* pmin = nullptr;    // per partition: smallest member of the pivot's SCC
+  uint32_t* pflags = nullptr;  // per (partition, class): used
+  uint32_t* pnew = nullptr;    // per (partition, class): new partition id (scan of pflags)
+  float* vscore = nullptr;     // per vertex: MLP score
+  float* vfeat = nullptr;      // features of live vertices (6 per vertex) + scores
   // profiling
   bool profile = false;
   std::vector<KernTimer> kt;
@@ -263,6 +271,8 @@ inline uint32_t grid_for(uint64_t items, uint32_t per_block = WG, uint32_t cap =
 }
 
 int select_pivot_learned_impl(Ctx* c, const kspn_graph* g, const float* params, hipStream_t s, uint32_t* pivot);
+int mlp_features(Ctx* c, const kspn_graph* g, const uint32_t* list, uint32_t count, float* feat, hipStream_t s);
+int mlp_forward(Ctx* c, const float* params, const float* x, uint32_t rows, float* out, hipStream_t s);
 
 }  // namespace kspn
 

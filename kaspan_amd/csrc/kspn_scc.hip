@@ -121,6 +121,7 @@ enum ExpandOp : int {
   OP_NEIGH = 2,     // collect live targets once (trim candidates)
   OP_COLOR = 3,     // push colour max into live targets
   OP_COLOR_BW = 4,  // backward closure: decide live in-neighbours of the same colour
+  OP_PBW = 5,       // live target in the source's partition not yet reached backward: claim its bwv bit
 };
 
 struct OpArgs {
@@ -133,21 +134,24 @@ struct OpArgs {
   uint32_t value;                // pivot or rotation
   const uint64_t* noff;          // OP_FW/OP_BW: offsets whose degree the next level expands
   unsigned long long* nsum;      // OP_FW/OP_BW: sum of those degrees over claimed targets
+  const uint32_t* part;          // OP_FW/OP_PBW: partition of every vertex (nullptr: one partition)
 };
 
 template <int OP>
 __device__ inline void expand_apply(uint32_t u, uint32_t w, const OpArgs& a, BlockQueue& q, uint64_t& nsum) {
   bool push = false;
   if constexpr (OP == OP_FW) {
-    push = bm_get(a.live, w) && bm_claim(a.fwv, w);
+    push = bm_get(a.live, w) && (!a.part || a.part[w] == a.part[u]) && bm_claim(a.fwv, w);
   } else if constexpr (OP == OP_BW) {
     push = bm_get(a.fwv, w) && bm_claim(a.bwv, w);
+  } else if constexpr (OP == OP_PBW) {
+    push = bm_get(a.live, w) && a.part[w] == a.part[u] && bm_claim(a.bwv, w);
   } else if constexpr (OP == OP_NEIGH) {
     push = bm_get(a.live, w) && bm_claim(a.inq, w);
   } else if constexpr (OP == OP_COLOR) {
     uint32_t c = a.col[u];
     uint32_t cw = a.col[w];
-    if (cw != 0u && cw < c) {
+    if (cw != 0u && cw < c && (!a.part || a.part[w] == a.part[u])) {
       uint32_t old = atomicMax(&a.col[w], c);
       push = (old < c) && bm_claim(a.inq, w);
     }
@@ -158,7 +162,7 @@ __device__ inline void expand_apply(uint32_t u, uint32_t w, const OpArgs& a, Blo
       push = true;
     }
   }
-  if constexpr (OP == OP_FW || OP == OP_BW) {
+  if constexpr (OP == OP_FW || OP == OP_BW || OP == OP_PBW) {
     if (push) nsum += a.noff[w + 1] - a.noff[w];
   }
   bq_push(q, push, w);
@@ -249,7 +253,7 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
     bq_maybe_flush(q, next, next_cnt, EXP_TILE);  // a tile appends at most EXP_TILE entries
   }
   bq_flush(q, next, next_cnt);
-  if constexpr (OP == OP_FW || OP == OP_BW) {
+  if constexpr (OP == OP_FW || OP == OP_BW || OP == OP_PBW) {
     if (nsum) atomicAdd(&s_nsum, (unsigned long long)nsum);
     __syncthreads();
     if (threadIdx.x == 0 && s_nsum) atomicAdd(a.nsum, s_nsum);
@@ -368,7 +372,7 @@ template <int DIR>
 __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restrict__ cand, uint32_t* vis,
                                              const uint64_t* __restrict__ roff, const uint32_t* __restrict__ radj,
                                              const uint64_t* __restrict__ noff, uint32_t* __restrict__ fnew,
-                                             uint64_t* __restrict__ part) {
+                                             uint64_t* __restrict__ part, const uint32_t* __restrict__ pid) {
   __shared__ uint64_t s_red[WG / WAVE];
   uint64_t claimed = 0, edges = 0, nd = 0;
   const uint64_t n_pad = ((uint64_t)n + WG - 1) / WG * WG;
@@ -382,9 +386,11 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
       const uint32_t v = (uint32_t)v64;
       const uint32_t bit = 1u << (v & 31);
       if ((cand[word] & bit) && !(vis_word & bit)) {
+        const uint32_t pv = pid ? pid[v] : 0u;
         for (uint64_t it = roff[v], end = roff[v + 1]; it < end; ++it) {
           ++edges;
-          if (bm_get(vis, radj[it])) { hit = true; break; }
+          const uint32_t u = radj[it];
+          if (bm_get(vis, u) && (!pid || pid[u] == pv)) { hit = true; break; }
         }
         if (hit) {
           ++claimed;
@@ -648,7 +654,8 @@ __global__ void k_color_init_list(const uint32_t* __restrict__ U, const uint32_t
 // 0); only v's thread writes col[v]; changed vertices go to `out`
 __global__ __launch_bounds__(WG) void k_color_pull(const uint32_t* __restrict__ U, const uint32_t* ucount,
                                                    const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
-                                                   uint32_t* col, uint32_t* __restrict__ out, uint32_t* out_cnt) {
+                                                   uint32_t* col, uint32_t* __restrict__ out, uint32_t* out_cnt,
+                                                   const uint32_t* __restrict__ pid) {
   __shared__ uint32_t s_out[8 * WG];
   __shared__ uint32_t s_on, s_ob;
   BlockQueue q{s_out, &s_on, &s_ob, 8 * WG};
@@ -664,7 +671,15 @@ __global__ __launch_bounds__(WG) void k_color_pull(const uint32_t* __restrict__ 
       v = U[i];
       uint32_t c0 = col[v];
       uint32_t c = c0;
-      for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it) c = max(c, col[bw_adj[it]]);
+      if (pid) {
+        const uint32_t pv = pid[v];
+        for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it) {
+          const uint32_t u = bw_adj[it];
+          if (pid[u] == pv) c = max(c, col[u]);
+        }
+      } else {
+        for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it) c = max(c, col[bw_adj[it]]);
+      }
       if (c > c0 && c0 != 0u) {
         atomicMax(&col[v], c);
         changed = true;
@@ -674,6 +689,18 @@ __global__ __launch_bounds__(WG) void k_color_pull(const uint32_t* __restrict__ 
     if ((++iter & 3) == 0) bq_maybe_flush(q, out, out_cnt, 4 * WG);  // <= 4 appends per thread since the last check
   }
   bq_flush(q, out, out_cnt);
+}
+
+// partition refinement after a colour round: the vertices of one SCC share
+// their ancestors and hence their colour, so (partition, colour) refines the
+// partition; the colour's root id names the new partition
+__global__ void k_refine_pid(const uint32_t* __restrict__ U, const uint32_t* ucount, const uint32_t* __restrict__ live,
+                             const uint32_t* __restrict__ col, uint32_t r, uint32_t* __restrict__ pid) {
+  uint32_t u = *ucount;
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < u; i += gridDim.x * WG) {
+    uint32_t v = U[i];
+    if (bm_get(live, v)) pid[v] = rotl32(~col[v], r);
+  }
 }
 
 // roots: live v whose colour is its own key; decided with id v
@@ -713,18 +740,23 @@ __global__ __launch_bounds__(WG) void k_color_roots(const uint32_t* __restrict__
 // group's minimum across lanes and issues one atomic per group. Large
 // components therefore cost one atomic per wavefront, not one per vertex.
 // All lanes of the wavefront call it (uniform control flow).
+// At most two peeling passes: groups of at least 8 lanes are common inside
+// large components; the remaining lanes issue their own atomics.
 __device__ inline void grouped_atomic_min(bool valid, uint32_t rep, uint32_t v, uint32_t* minid) {
   uint64_t pending = __ballot(valid);
   const uint32_t lane = lane_id();
-  while (pending) {
+  for (int pass = 0; pass < 2 && pending; ++pass) {
     uint32_t leader = __ffsll((unsigned long long)pending) - 1;
     uint32_t r = __shfl(rep, leader);
     bool mine = valid && rep == r;
     uint64_t grp = __ballot(mine);
+    if (__popcll(grp) < 8) break;
     uint32_t m = wave_min(mine ? v : 0xFFFFFFFFu);
     if (lane == 0) atomicMin(&minid[r], m);
     pending &= ~grp;
+    if (mine) valid = false;
   }
+  if (valid && ((pending >> lane) & 1ull)) atomicMin(&minid[rep], v);
 }
 
 // minid[rep] = min { v : labels[v] == rep } over v in [0,n) or over list[0..count)
@@ -1027,6 +1059,178 @@ __global__ void k_mask_to_bitmap(uint32_t n, const uint8_t* mask, uint32_t* bm) 
 }
 
 // =====================================================================
+// partitioned multi-pivot FW-BW
+// =====================================================================
+// Every live vertex carries a partition id; SCCs never cross partitions.
+// A round picks one pivot per partition, runs forward and backward
+// reachability from all pivots at once (claims stay inside the source's
+// partition), decides FW & BW (the pivots' SCCs) and splits every partition
+// into FW-only, BW-only and neither (Fleischer, Hendrickson, Pinar 2000).
+
+__device__ inline uint32_t float_order(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ inline unsigned long long wave_max64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_down(v, off);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// pbest[p] = max over live v in p of (priority << 32 | ~v): priority is
+// min(in*out, 2^32-1) (plain rule, smallest id on ties) or the ordered MLP
+// score; lanes are grouped by partition before the 64-bit atomicMax
+__global__ __launch_bounds__(WG) void k_part_key(uint32_t n, const uint32_t* __restrict__ live,
+                                                 const uint32_t* __restrict__ pid, const uint64_t* __restrict__ fw_off,
+                                                 const uint64_t* __restrict__ bw_off, const float* __restrict__ score,
+                                                 uint64_t rnd_seed, unsigned long long* pbest) {
+  const uint64_t pad = ((uint64_t)n + WG - 1) / WG * WG;
+  const uint32_t lane = lane_id();
+  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < pad; base += (uint64_t)gridDim.x * WG) {
+    const uint64_t v = base + threadIdx.x;
+    const bool valid = v < n && bm_get(live, (uint32_t)v);
+    uint32_t p = 0;
+    unsigned long long key = 0;
+    if (valid) {
+      p = pid[v];
+      uint32_t pri;
+      if (score) {
+        pri = float_order(score[v]);
+      } else if (rnd_seed) {
+        uint64_t z = ((uint64_t)rnd_seed << 32 | (uint32_t)v) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 31)) * 0xBF58476D1CE4E5B9ull;
+        pri = (uint32_t)(z >> 32) | 1u;
+      } else {
+        uint64_t prod = (fw_off[v + 1] - fw_off[v]) * (bw_off[v + 1] - bw_off[v]);
+        pri = prod > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)prod;
+      }
+      key = ((unsigned long long)pri << 32) | (uint32_t)~(uint32_t)v;
+    }
+    // one peeling pass for the most common partition, then per-lane atomics
+    uint64_t pending = __ballot(valid);
+    bool left = valid;
+    if (pending) {
+      uint32_t leader = __ffsll((unsigned long long)pending) - 1;
+      uint32_t r = __shfl(p, leader);
+      bool mine = valid && p == r;
+      uint64_t grp = __ballot(mine);
+      if (__popcll(grp) >= 8) {
+        unsigned long long m = wave_max64(mine ? key : 0ull);
+        if (lane == 0) atomicMax(&pbest[r], m);
+        if (mine) left = false;
+      }
+    }
+    if (left && pbest[p] < key) atomicMax(&pbest[p], key);
+  }
+}
+
+// pivot of every partition into F; its forward and backward bits set
+__global__ void k_part_pivots(uint32_t np, const unsigned long long* __restrict__ pbest, uint32_t* __restrict__ F,
+                              uint32_t* fwv, uint32_t* bwv) {
+  for (uint32_t p = blockIdx.x * WG + threadIdx.x; p < np; p += gridDim.x * WG) {
+    uint32_t v = ~(uint32_t)pbest[p];
+    F[p] = v;
+    atomicOr(&fwv[v >> 5], 1u << (v & 31));
+    atomicOr(&bwv[v >> 5], 1u << (v & 31));
+  }
+}
+
+// after a round: pivot SCCs (forward and backward reached) contribute their
+// smallest member to pmin[p]; every other live vertex flags its class
+// (0 neither, 1 forward only, 2 backward only) in flags[3p + class]
+__global__ __launch_bounds__(WG) void k_part_classify(uint32_t n, const uint32_t* __restrict__ live,
+                                                      const uint32_t* __restrict__ fwv, const uint32_t* __restrict__ bwv,
+                                                      const uint32_t* __restrict__ pid, uint32_t* pmin, uint32_t* flags,
+                                                      uint32_t* pcount) {
+  const uint64_t pad = ((uint64_t)n + WG - 1) / WG * WG;
+  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < pad; base += (uint64_t)gridDim.x * WG) {
+    const uint64_t v = base + threadIdx.x;
+    const bool is_live = v < n && bm_get(live, (uint32_t)v);
+    bool dec = false;
+    uint32_t p = 0;
+    if (is_live) {
+      p = pid[v];
+      bool f = bm_get(fwv, (uint32_t)v), b = bm_get(bwv, (uint32_t)v);
+      dec = f && b;
+      if (!dec) flags[3ull * p + (f ? 1 : (b ? 2 : 0))] = 1u;
+      if (pcount) atomicAdd(&pcount[4ull * p + (dec ? 0 : (f ? 1 : (b ? 2 : 3)))], 1u);
+    }
+    grouped_atomic_min(dec, p, (uint32_t)v, pmin);
+  }
+}
+
+// decided: label = pmin[p], removed from live (each wavefront owns its two
+// words); others: pid = newid[3p + class]. Per-workgroup decided counts.
+__global__ __launch_bounds__(WG) void k_part_apply(uint32_t n, uint32_t* live, const uint32_t* __restrict__ fwv,
+                                                   const uint32_t* __restrict__ bwv, uint32_t* __restrict__ pid,
+                                                   const uint32_t* __restrict__ pmin, const uint32_t* __restrict__ newid,
+                                                   uint32_t* __restrict__ labels, uint64_t* __restrict__ partials) {
+  __shared__ uint64_t s_red[WG / WAVE];
+  const uint64_t pad = ((uint64_t)n + WG - 1) / WG * WG;
+  const uint32_t lane = lane_id();
+  uint64_t count = 0;
+  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < pad; base += (uint64_t)gridDim.x * WG) {
+    const uint64_t v = base + threadIdx.x;
+    const uint32_t lw = v < n ? live[v >> 5] : 0u;
+    const bool is_live = v < n && ((lw >> (v & 31)) & 1u);
+    bool dec = false;
+    if (is_live) {
+      const uint32_t p = pid[v];
+      bool f = bm_get(fwv, (uint32_t)v), b = bm_get(bwv, (uint32_t)v);
+      if (f && b) {
+        dec = true;
+        labels[v] = pmin[p];
+        ++count;
+      } else {
+        pid[v] = newid[3ull * p + (f ? 1 : (b ? 2 : 0))];
+      }
+    }
+    const uint64_t mask = __ballot(dec);
+    const uint64_t word0 = (base + threadIdx.x - lane) >> 5;
+    if (lane == 0 && (uint32_t)mask) live[word0] = lw & ~(uint32_t)mask;
+    if (lane == 32 && (uint32_t)(mask >> 32)) live[word0 + 1] = lw & ~(uint32_t)(mask >> 32);
+  }
+  count = block_sum64(count, s_red);
+  if (threadIdx.x == 0) {
+    partials[3 * blockIdx.x] = count;
+    partials[3 * blockIdx.x + 1] = 0;
+    partials[3 * blockIdx.x + 2] = 0;
+  }
+}
+
+// pivot outcome records of one round (KSPN_LOG_WORDS u32 each)
+__global__ void k_part_log(uint32_t np, const unsigned long long* __restrict__ pbest, const uint32_t* __restrict__ pcount,
+                           uint32_t round, uint32_t* __restrict__ log, uint32_t first, uint32_t cap) {
+  for (uint32_t p = blockIdx.x * WG + threadIdx.x; p < np; p += gridDim.x * WG) {
+    if (first + p >= cap) break;
+    uint32_t* r = log + (uint64_t)(first + p) * KSPN_LOG_WORDS;
+    const uint32_t* c = pcount + 4ull * p;
+    r[0] = ~(uint32_t)pbest[p];
+    r[1] = round;
+    r[2] = c[0] + c[1] + c[2] + c[3];
+    r[3] = c[0];
+    r[4] = c[1];
+    r[5] = c[2];
+    r[6] = c[3];
+    r[7] = 0;
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long* p, uint64_t count, unsigned long long v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i < count; i += (uint64_t)gridDim.x * WG) p[i] = v;
+}
+
+// out[list[i]] = vals[i]
+__global__ void k_scatter_f32(const uint32_t* __restrict__ list, const uint32_t* count, const float* __restrict__ vals,
+                              float* __restrict__ out) {
+  uint32_t cnt = *count;
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < cnt; i += gridDim.x * WG) out[list[i]] = vals[i];
+}
+
+// =====================================================================
 // host side
 // =====================================================================
 
@@ -1146,10 +1350,17 @@ int ctx_reserve(Ctx* c, uint32_t n, uint64_t m) {
     if (hipMalloc(&c->cub_tmp, need) != hipSuccess) return KSPN_E_OOM;
     c->cub_bytes = need;
   }
-  if (c->tj) {  // Tarjan work area is sized by cap_n; reallocated on demand
-    (void)hipFree(c->tj);
-    c->tj = nullptr;
-  }
+  // buffers sized by cap_n and allocated on first use: drop them so they are reallocated
+  for (uint32_t** p : {&c->tj, &c->pid, &c->pmin, &c->pflags, &c->pnew})
+    if (*p) {
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
+  for (void** p : {(void**)&c->pbest, (void**)&c->vscore, (void**)&c->vfeat, (void**)&c->sub_off})
+    if (*p) {
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
   c->cap_n = nn;
   c->cap_m = mm;
   return KSPN_OK;
@@ -1224,6 +1435,10 @@ struct Run {
   bool col_valid = false;  // col is 0 for every vertex outside ulist (prev_u entries)
   uint32_t prev_u = 0;
   uint32_t rounds = 0;  // colour rounds so far
+  bool pid_preset = false;  // partition ids already set (batched graphs)
+  uint32_t log_count = 0;   // pivot outcome records written
+  bool pid_active = false;  // pid partitions the live vertices (after multi-pivot FW-BW)
+  uint32_t np0 = 1;         // number of preset partitions
 };
 
 int select_pivot_live(Run& R, uint32_t* pivot);
@@ -1273,7 +1488,12 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b) {
 // expansion of the frontier) or pull (sweep over all vertices), switched
 // by Beamer's rule: pull when frontier edges * alpha > remaining edges,
 // back to push when frontier * beta < live.
-int reach(Run& R, int dir, uint32_t pivot, uint32_t* fw_levels_push, uint32_t* fw_levels_pull) {
+// Reachability in direction dir (0 forward, 1 backward) from the frontier
+// the caller put in qa (count f0, also in cnt[C_Q0]; their vis bits set).
+// Single partition: forward claims live vertices, backward claims
+// forward-reached ones. With partitions (pid != nullptr) both directions
+// claim live vertices of the source's partition.
+int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels_push, uint32_t* fw_levels_pull) {
   Ctx* c = R.c;
   const kspn_graph* g = R.g;
   const uint64_t* off = dir == 0 ? g->fw_off : g->bw_off;
@@ -1281,15 +1501,12 @@ int reach(Run& R, int dir, uint32_t pivot, uint32_t* fw_levels_push, uint32_t* f
   const uint64_t* roff = dir == 0 ? g->bw_off : g->fw_off;
   const uint32_t* radj = dir == 0 ? g->bw_adj : g->fw_adj;
   uint32_t* vis = dir == 0 ? c->fwv : c->bwv;
-  const uint32_t* cand = dir == 0 ? c->live : c->fwv;
-  OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, pivot, off, c->cnt64 + 1};
+  const uint32_t* cand = (dir == 0 || pid) ? c->live : c->fwv;
+  OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, 0, off, c->cnt64 + 1, pid};
   uint32_t* F = c->qa;
   uint32_t* N = c->qb;
   int fc = C_Q0, nc = C_Q1;
-  // frontier = { pivot }
-  k_set_u32<<<1, 64, 0, R.s>>>(F, pivot);
-  k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[fc], 1u);
-  uint32_t f = 1;
+  uint32_t f = f0;
   uint64_t edges = 0;  // out-edges of the frontier in this direction (unknown for the pivot level)
   bool known = false;
   bool pull = false;
@@ -1316,6 +1533,8 @@ int reach(Run& R, int dir, uint32_t pivot, uint32_t* fw_levels_push, uint32_t* f
       Prof p(c, R.s, kern, 20.0 * f + (4.0 + 0.125) * (double)edges);
       if (dir == 0)
         k_expand<OP_FW><<<grid, WG, 0, R.s>>>(F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
+      else if (pid)
+        k_expand<OP_PBW><<<grid, WG, 0, R.s>>>(F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
       else
         k_expand<OP_BW><<<grid, WG, 0, R.s>>>(F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
       if (fw_levels_push) (*fw_levels_push)++;
@@ -1325,9 +1544,9 @@ int reach(Run& R, int dir, uint32_t pivot, uint32_t* fw_levels_push, uint32_t* f
       {
         Prof p(c, R.s, kern, (double)R.n / 4.0 + 16.0 * R.live_n);
         if (dir == 0)
-          k_pull<0><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, off, c->fnew, c->part);
+          k_pull<0><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, off, c->fnew, c->part, pid);
         else
-          k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, off, c->fnew, c->part);
+          k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, off, c->fnew, c->part, pid);
       }
       k_reduce3<<<1, WG, 0, R.s>>>(c->part, grid, &c->cnt[nc], c->cnt64);
       if (fw_levels_pull) (*fw_levels_pull)++;
@@ -1369,13 +1588,17 @@ int fwbw(Run& R, uint32_t pivot, uint32_t* decided) {
     k_bm_set_one<<<1, 64, 0, R.s>>>(c->fwv, pivot);
   }
   uint32_t lp = 0, lq = 0;
-  KSPN_TRY(reach(R, 0, pivot, &lp, &lq));
+  k_set_u32<<<1, 64, 0, R.s>>>(c->qa, pivot);
+  k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], 1u);
+  KSPN_TRY(reach(R, 0, 1, nullptr, &lp, &lq));
   {
     Prof p(c, R.s, K_BITMAP, 12.0 * R.words);
     k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwv, R.words, 0u);
     k_bm_set_one<<<1, 64, 0, R.s>>>(c->bwv, pivot);
   }
-  KSPN_TRY(reach(R, 1, pivot, &lp, &lq));
+  k_set_u32<<<1, 64, 0, R.s>>>(c->qa, pivot);
+  k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], 1u);
+  KSPN_TRY(reach(R, 1, 1, nullptr, &lp, &lq));
   // the pivot's SCC: clear it from live, then label it with its smallest member
   uint32_t nparts = stream_grid(c, R.words);
   {
@@ -1394,6 +1617,103 @@ int fwbw(Run& R, uint32_t pivot, uint32_t* decided) {
     R.st->fwbw_levels_push += lp;
     R.st->fwbw_levels_pull += lq;
   }
+  return KSPN_OK;
+}
+
+// ---------------------------------------------------------------- partitioned FW-BW rounds
+
+int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
+  Ctx* c = R.c;
+  const kspn_graph* g = R.g;
+  const uint64_t n = R.n;
+  if (!c->pid) KSPN_TRY(dalloc(&c->pid, (uint64_t)c->cap_n + 64));
+  if (!c->pbest) KSPN_TRY(dalloc(&c->pbest, (uint64_t)c->cap_n + 64));
+  if (!c->pmin) KSPN_TRY(dalloc(&c->pmin, (uint64_t)c->cap_n + 64));
+  if (!c->pflags) KSPN_TRY(dalloc(&c->pflags, 3ull * c->cap_n + 64));
+  if (!c->pnew) KSPN_TRY(dalloc(&c->pnew, 3ull * c->cap_n + 64));
+  if (!R.pid_preset) {
+    k_fill_u32<<<stream_grid(c, n), WG, 0, R.s>>>(c->pid, n, 0u);
+    np = 1;
+  }
+  const float* score = nullptr;
+  if (R.o.pivot == KSPN_PIVOT_LEARNED) {
+    // one MLP score per live vertex for the whole solve
+    uint32_t u = 0;
+    KSPN_TRY(build_live_list(c, R.n, c->live, c->ulist, &u, R.s));
+    if (!c->vscore) KSPN_TRY(dalloc(&c->vscore, (uint64_t)c->cap_n + 64));
+    if (!c->vfeat) KSPN_TRY(dalloc(&c->vfeat, 7ull * c->cap_n + 64));
+    float* feat = c->vfeat;
+    float* sc = c->vfeat + 6ull * c->cap_n;
+    {
+      Prof p(c, R.s, K_PIVOT, 24.0 * u);
+      KSPN_TRY(mlp_features(c, g, c->ulist, u, feat, R.s));
+      KSPN_TRY(mlp_forward(c, R.o.mlp_params, feat, u, sc, R.s));
+      k_scatter_f32<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_AUX], sc, c->vscore);
+    }
+    score = c->vscore;
+  }
+  uint32_t total = 0;
+  for (uint32_t round = 0; round < K && R.live_n > 0 && np > 0; ++round) {
+    if (R.o.residual_threshold && R.live_n <= R.o.residual_threshold) break;
+    const uint32_t gridn = stream_grid(c, n);
+    {
+      Prof p(c, R.s, K_PIVOT, 20.0 * R.live_n + 8.0 * np);
+      k_fill_u64<<<stream_grid(c, np), WG, 0, R.s>>>((unsigned long long*)c->pbest, np, 0ull);
+      const uint64_t rs = R.o.pivot == KSPN_PIVOT_RANDOM ? ((uint64_t)R.o.seed * 0x10001ull + round + 1) : 0;
+      k_part_key<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->pid, g->fw_off, g->bw_off, score, rs,
+                                         (unsigned long long*)c->pbest);
+      k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->fwv, R.words, 0u);
+      k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwv, R.words, 0u);
+      k_part_pivots<<<stream_grid(c, np), WG, 0, R.s>>>(np, (unsigned long long*)c->pbest, c->qa, c->fwv, c->bwv);
+      k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], np);
+    }
+    uint32_t lp = 0, lq = 0;
+    KSPN_TRY(reach(R, 0, np, c->pid, &lp, &lq));
+    {
+      Prof p(c, R.s, K_PIVOT, 8.0 * np);
+      k_part_pivots<<<stream_grid(c, np), WG, 0, R.s>>>(np, (unsigned long long*)c->pbest, c->qa, c->fwv, c->bwv);
+      k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], np);
+    }
+    KSPN_TRY(reach(R, 1, np, c->pid, &lp, &lq));
+    {
+      Prof p(c, R.s, K_BITMAP, 12.0 * R.live_n + 16.0 * np);
+      k_fill_u32<<<stream_grid(c, np), WG, 0, R.s>>>(c->pmin, np, 0xFFFFFFFFu);
+      k_fill_u32<<<stream_grid(c, 3ull * np + 1), WG, 0, R.s>>>(c->pflags, 3ull * np + 1, 0u);
+      uint32_t* pcount = nullptr;
+      if (R.o.pivot_log) {
+        pcount = reinterpret_cast<uint32_t*>(c->pbest) + 2ull * np;  // 4 counters per partition after the keys
+        if (4ull * np > 2ull * (c->cap_n + 64) - 2ull * np) pcount = nullptr;
+      }
+      if (pcount) k_fill_u32<<<stream_grid(c, 4ull * np), WG, 0, R.s>>>(pcount, 4ull * np, 0u);
+      k_part_classify<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->fwv, c->bwv, c->pid, c->pmin, c->pflags, pcount);
+      if (pcount && R.log_count < R.o.pivot_log_cap) {
+        k_part_log<<<stream_grid(c, np), WG, 0, R.s>>>(np, (unsigned long long*)c->pbest, pcount, round,
+                                                      R.o.pivot_log, R.log_count, R.o.pivot_log_cap);
+        R.log_count = std::min<uint64_t>((uint64_t)R.log_count + np, R.o.pivot_log_cap);
+        if (R.st) R.st->pivot_log_count = R.log_count;
+      }
+    }
+    KSPN_TRY(exclusive_scan_u32(c, c->pflags, c->pnew, 3ull * np + 1, R.s));
+    {
+      Prof p(c, R.s, K_LABEL, 16.0 * R.live_n);
+      k_part_apply<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->fwv, c->bwv, c->pid, c->pmin, c->pnew, R.labels, c->part);
+      k_reduce3<<<1, WG, 0, R.s>>>(c->part, gridn, &c->cnt[C_DECIDED], c->cnt64);
+    }
+    k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_AUX], c->pnew + 3ull * np);  // scan total = new partition count
+    KSPN_TRY(publish(c, R.s, nullptr));
+    const uint32_t d = c->h_cnt[C_DECIDED];
+    np = c->h_cnt[C_AUX];
+    R.live_n -= d;
+    total += d;
+    if (R.st) {
+      R.st->fwbw_levels_push += lp;
+      R.st->fwbw_levels_pull += lq;
+      R.st->fwbw_rounds_done++;
+      if (np > R.st->partitions_max) R.st->partitions_max = np;
+    }
+    if (d == 0) break;
+  }
+  *decided_total = total;
   return KSPN_OK;
 }
 
@@ -1512,13 +1832,14 @@ int color_round(Run& R, uint32_t* decided) {
   KSPN_TRY(build_live_list(c, R.n, c->live, c->ulist, &u, R.s));
   k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_RES_N], &c->cnt[C_AUX]);
   R.prev_u = u;
-  OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, r};
+  const uint32_t* pid = R.pid_active ? c->pid : nullptr;
+  OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, r, nullptr, nullptr, pid};
   // forward: pull sweep over all live vertices
   KSPN_HIP(hipMemsetAsync(&c->cnt[C_Q0], 0, sizeof(uint32_t), R.s));
   {
     Prof p(c, R.s, K_COLOR_PULL, 24.0 * u);
     k_color_pull<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->col, c->qa,
-                                                    &c->cnt[C_Q0]);
+                                                    &c->cnt[C_Q0], pid);
   }
   KSPN_TRY(read_counters(c, R.s, 1));
   uint32_t f = c->h_cnt[C_Q0];
@@ -1534,7 +1855,7 @@ int color_round(Run& R, uint32_t* decided) {
       // large frontier: another pull sweep over every live vertex
       Prof p(c, R.s, K_COLOR_PULL, 24.0 * u);
       k_color_pull<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->col, N,
-                                                      &c->cnt[nc]);
+                                                      &c->cnt[nc], pid);
     } else {
       uint64_t e = 0;
       KSPN_TRY(expand<OP_COLOR>(R, F, &c->cnt[fc], f, g->fw_off, g->fw_adj, a, N, &c->cnt[nc], K_EXPAND_COLOR, 8.0, &e));
@@ -1570,6 +1891,10 @@ int color_round(Run& R, uint32_t* decided) {
     seg_end = c->h_cnt[C_DTOT];
   }
   const uint32_t total = seg_end;
+  if (pid) {
+    Prof p(c, R.s, K_COLOR_INIT, 12.0 * u);
+    k_refine_pid<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], c->live, c->col, r, c->pid);
+  }
   // canonical ids: every decision of this round carries its root's id;
   // replace it by the smallest member (col[root] is the min table)
   if (total) {
@@ -1774,6 +2099,19 @@ int run_pipeline(Run& R) {
   mark(1);
   // ---- FW-BW rounds
   uint32_t rounds = R.o.fwbw_rounds ? R.o.fwbw_rounds : 1;
+  if ((rounds > 1 || R.pid_preset) && R.live_n > 0) {
+    // multi-pivot: one pivot per partition per round
+    uint32_t d = 0;
+    KSPN_TRY(pfwbw_rounds(R, rounds, R.np0, &d));
+    if (st) st->decided_fwbw += d;
+    R.pid_active = true;
+    if (R.live_n > 0) {
+      if (trim_ex) KSPN_TRY(trim_exhaustive(R, &d));
+      else KSPN_TRY(trim_normal(R, &d));
+      if (st) st->decided_trim += d;
+    }
+    rounds = 0;  // the single-pivot loop below does nothing
+  }
   for (uint32_t k = 0; k < rounds && R.live_n > 0; ++k) {
     if (R.o.residual_threshold && R.live_n <= R.o.residual_threshold) break;
     if (R.o.pivot == KSPN_PIVOT_LEARNED) {
@@ -1807,6 +2145,13 @@ int run_pipeline(Run& R) {
   // ---- colour rounds
   uint32_t cap = R.o.max_rounds;
   const uint32_t tail = (R.o.flags & KSPN_FLAG_NO_TAIL) ? 0u : (R.o.tail_threshold ? R.o.tail_threshold : 4096u);
+  if (!(R.o.flags & KSPN_FLAG_NO_REFINE) && !R.pid_active && R.live_n > tail) {
+    // colour rounds refine one partition of all live vertices
+    if (!c->pid) KSPN_TRY(dalloc(&c->pid, (uint64_t)c->cap_n + 64));
+    k_fill_u32<<<stream_grid(c, R.n), WG, 0, R.s>>>(c->pid, R.n, 0u);
+    R.pid_active = true;
+  }
+  if (R.o.flags & KSPN_FLAG_NO_REFINE) R.pid_active = false;
   while (R.live_n > 0) {
     if (R.o.residual_threshold && R.live_n <= R.o.residual_threshold) break;
     if (cap && R.rounds >= cap) return KSPN_E_ITER_CAP;
@@ -1924,7 +2269,8 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   Ctx* c = &k->c;
   (void)hipSetDevice(c->device);
   void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
-                  c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf};
+                  c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf, c->pid, c->pbest,
+                  c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
@@ -1954,7 +2300,7 @@ int kspn_scc(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts
   if (opts) o = *opts;
   else kspn_opts_default(&o);
   if (o.method > KSPN_METHOD_TRIM_EX) return KSPN_E_INVALID;
-  if (o.pivot > KSPN_PIVOT_LEARNED) return KSPN_E_INVALID;
+  if (o.pivot > KSPN_PIVOT_RANDOM) return KSPN_E_INVALID;
   if (o.pivot == KSPN_PIVOT_LEARNED && !o.mlp_params) return KSPN_E_INVALID;
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (g->n == 0) return KSPN_OK;

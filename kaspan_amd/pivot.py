@@ -7,10 +7,13 @@ candidates and the best score seeds FW-BW; the plain rule's pivot is always
 among the candidates and remains the fallback. Pivot choice changes time,
 never labels.
 
-The scorer is trained on outcomes the method logs: solving a graph gives
-every vertex's SCC size (the SCC a FW-BW from that vertex would decide),
-and the target is log1p(size). Forward (BF16 MFMA), gradient (FP32) and
-Adam run as libkspn kernels on the GPU.
+The scorer is trained on outcomes the method logs: with multi-pivot FW-BW
+every round records, per partition, the pivot, the partition size, the SCC
+it decided and the forward-only / backward-only / unreached parts it left
+(opts.pivot_log). The target is the residual shrinkage
+1 - max(parts) / size: a pivot that splits its partition evenly (or decides
+much of it) scores high. Data comes from seeded random pivots; forward
+(BF16 MFMA), gradient (FP32) and Adam run as libkspn kernels on the GPU.
 """
 from __future__ import annotations
 
@@ -89,13 +92,24 @@ def candidates(g: BidiGraph, k_top=4096, k_random=4096, seed=0) -> torch.Tensor:
     return torch.unique(torch.cat([top, rnd])).to(torch.int32)
 
 
-def outcome_samples(g: BidiGraph, seed=0, **kw):
-    """(features, target) for candidate pivots of g: target = log1p(size of the SCC the pivot would decide)"""
-    labels, _ = scc(g)
-    sizes = scc_sizes(labels)
-    cand = candidates(g, seed=seed, **kw)
-    x = features(g, cand)
-    t = torch.log1p(sizes[cand.long()].float())
+def pivot_log(g: BidiGraph, rounds=24, pivot=_lib.PIVOT_RANDOM, seed=0, cap=1 << 20, **opts) -> torch.Tensor:
+    """Solve g with multi-pivot FW-BW and return the logged pivot outcomes as an int64 [records, 8] tensor:
+    pivot, round, partition size, SCC size, forward-only, backward-only, unreached, 0."""
+    log = torch.zeros(cap * _lib.LOG_WORDS, dtype=torch.int32, device="cuda")
+    _, st = scc(g, fwbw_rounds=rounds, pivot=pivot, seed=seed, pivot_log=log, **opts)
+    rec = log[: st.pivot_log_count * _lib.LOG_WORDS].view(-1, _lib.LOG_WORDS)
+    return rec.to(torch.int64) & 0xFFFFFFFF
+
+
+def outcome_samples(g: BidiGraph, rounds=24, seed=0, min_size=32):
+    """(features, target) from the method's own pivot log on g: random pivots per partition, target =
+    residual shrinkage 1 - max(forward-only, backward-only, unreached) / partition size"""
+    rec = pivot_log(g, rounds=rounds, pivot=_lib.PIVOT_RANDOM, seed=seed)
+    rec = rec[rec[:, 2] >= min_size]
+    size = rec[:, 2].float()
+    largest = torch.maximum(torch.maximum(rec[:, 4], rec[:, 5]), rec[:, 6]).float()
+    t = 1.0 - largest / size
+    x = features(g, rec[:, 0].to(torch.int32).contiguous())
     return x, t
 
 

@@ -26,9 +26,12 @@ METHODS = [
     dict(method=0, flags=4),               # colour rounds only, no tail solver
     dict(method=1, flags=4),
     dict(method=0, tail_threshold=1 << 30),  # tail solver takes everything after FW-BW
+    dict(method=0, fwbw_rounds=64, flags=4),   # partitioned multi-pivot FW-BW, then colour rounds
+    dict(method=0, fwbw_rounds=3, alpha=1e9, beta=1.0),  # partitioned, pull levels
+    dict(method=0, flags=4 | 8),               # the reference's colour rounds: no tail, no refinement
 ]
 MIDS = ["kaspan", "trim_ex", "trim_ex_no_trim2", "fwbw4", "residual300", "kaspan_residual_all", "pull", "push",
-        "kaspan_no_tail", "trim_ex_no_tail", "tail_all"]
+        "kaspan_no_tail", "trim_ex_no_tail", "tail_all", "multi_pivot64", "multi_pivot3_pull", "kaspan_plain_colour"]
 
 
 def host_graph(O, g):
@@ -217,7 +220,7 @@ def test_iteration_cap(ka, O):
     # a forward DAG with every vertex live after trim-1's one pass needs many colour rounds
     hg0, _ = O.gen_planted(77, 5000)
     g = ka.BidiGraph.from_host(hg0.n, hg0.fw_off, hg0.fw_adj, hg0.bw_off, hg0.bw_adj)
-    for flags in (0, 4):  # cap inside the tail solver and in the host colour loop
+    for flags in (0, 4, 12):  # cap inside the tail solver and in the host colour loop
         with pytest.raises(ka.KspnError) as ei:
             ka.scc(g, method=0, max_rounds=1, flags=flags)
         assert ei.value.code == -4
