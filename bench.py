@@ -6,6 +6,12 @@ duplicates removed, uint32 ids / uint64 offsets, CSR + reverse CSR resident
 in HBM. A step is one full solve (CSR in HBM -> canonical labels in HBM).
 value = m_final / seconds per step.
 
+The batch workload (--config batch4096) packs 4096 independent graphs (G(n,m)
+or R-MAT with probability 1/2, n log-uniform in [1e3, 1e6], average
+out-degree log-uniform in [2, 16]) into one union graph solved by one
+kspn_scc_batch call per step; it also reports graphs/sec and the p50/p99
+latency of solving sampled graphs one at a time.
+
 Prints one JSON line. `roofline` is measured by timing every kernel launch
 of the solve with HIP events recorded on the solve's stream (the library's
 profile mode) over --steps further solves; `cpu_baseline` is the oracle's
@@ -49,7 +55,13 @@ CONFIGS = {
                    "out-degree 4 inside, 3 forward inter-component edges per vertex"),
     "lattice256": ("lattice", dict(side=256, p_bidir=0.1), 0x6B5350AA00000003,
                    "3-D lattice 256^3, each edge one random direction, 10% bidirectional"),
+    "batch4096": ("batch", dict(count=4096, n_min=1000, n_max=1_000_000, deg_min=2.0, deg_max=16.0, p_rmat=0.5),
+                  0x6B5350AA00000004,
+                  "4096 independent graphs in one batch: G(n,m) or R-MAT(.57/.19/.19/.05) with p=.5, n log-uniform "
+                  "in [1e3, 1e6], average out-degree log-uniform in [2, 16], deduplicated, concatenated CSR with "
+                  "per-graph vertex/edge offsets"),
 }
+LATENCY_SAMPLE = 128  # graphs of the batch solved one at a time for p50/p99
 # bounded CPU-baseline samples for workloads whose full size takes the serial restatement too long
 CPU_SAMPLE = {
     "planted50m": ("planted", dict(n=2_000_000, max_size=400_000, expo=2.1, intra_deg=4.0, inter_deg=3.0)),
@@ -95,10 +107,22 @@ def main():
     kind, kw, seed, desc = CONFIGS[args.config]
     print(f"[bench] generating {args.config}", file=sys.stderr, flush=True)
     t0 = time.time()
-    g = ka.generate(kind, seed, **kw)
+    batch = None
+    if kind == "batch":
+        bkw = dict(kw)
+        count = bkw.pop("count")
+        batch = ka.generate_batch(seed, count, **bkw)
+        g = batch.graph
+    else:
+        g = ka.generate(kind, seed, **kw)
     torch.cuda.synchronize()
     gen_s = time.time() - t0
     print(f"[bench] n={g.n} m={g.m} generated in {gen_s:.1f}s", file=sys.stderr, flush=True)
+
+    def solve(lab, **o):
+        if batch is not None:
+            return ka.scc_batch(batch, lab, **o)
+        return ka.scc(g, lab, **o)
     method = args.method if args.method is not None else 0
     labels = torch.empty(g.n, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
@@ -115,14 +139,14 @@ def main():
 
     def timed(o):
         for _ in range(args.warmup):
-            ka.scc(g, labels, **o)
+            solve(labels, **o)
         torch.cuda.synchronize()
         start = torch.cuda.Event(enable_timing=True)
         end = torch.cuda.Event(enable_timing=True)
         start.record(stream)
         st = None
         for _ in range(args.steps):
-            _, st = ka.scc(g, labels, **o)
+            _, st = solve(labels, **o)
         end.record(stream)
         torch.cuda.synchronize()
         return start.elapsed_time(end) / args.steps, st
@@ -153,7 +177,7 @@ def main():
     if not args.no_profile:
         agg = {}
         for _ in range(args.steps):
-            _, pst = ka.scc(g, labels, profile=True, **opts)
+            _, pst = solve(labels, profile=True, **opts)
             for k in pst.as_dict()["kernels"]:
                 a = agg.setdefault(k["name"], {"launches": 0, "ms": 0.0, "bytes": 0.0})
                 a["launches"] += k["launches"]
@@ -173,8 +197,54 @@ def main():
                     "traffic": pmc_traffic(args.config, dom_name),
                     "bytes_per_launch": per_launch_bytes, "avg_launch_us": per_launch_s * 1e6}
 
+    batch_info = None
+    if batch is not None:
+        # serving one graph at a time: latency of each sampled graph alone (same options)
+        idx = sorted(set(np.linspace(0, batch.count - 1, LATENCY_SAMPLE).astype(int).tolist()))
+        lat = []
+        for i in idx:
+            sg = batch.subgraph(i)
+            slab = torch.empty(sg.n, dtype=torch.int32, device="cuda")
+            ka.scc(sg, slab, **opts)  # warm-up of this size
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            ka.scc(sg, slab, **opts)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            lat.append(e0.elapsed_time(e1))
+        lat = np.array(lat)
+        batch_info = {"graphs": batch.count, "graphs_per_s": batch.count / (ms_per_step / 1e3),
+                      "batch_latency_ms": ms_per_step,
+                      "solo_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                          "mean": float(lat.mean()), "sample_graphs": len(idx)},
+                      "solo_graphs_per_s_est": 1e3 / float(lat.mean()), "generate_s": gen_s}
+
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and batch is not None:
+        from oracle import oracle as O  # checker / CPU baseline only
+
+        # graphs of the batch in a fixed order until about 15 s of CPU time
+        order = np.random.default_rng(0).permutation(batch.count)
+        cpu_s, cpu_m, done, same = 0.0, 0, 0, True
+        lab_h = labels.cpu().numpy().view(np.uint32)
+        vo = batch.vtx_off.cpu().numpy().view(np.uint32)
+        for i in order:
+            fo, fa, bo, ba = batch.subgraph(int(i)).to_host()
+            hg = O.HostGraph(batch.n_of(int(i)), fo, fa, bo, ba)
+            t = time.perf_counter()
+            ref, _ = O.kaspan_scc(hg)
+            cpu_s += time.perf_counter() - t
+            cpu_m += hg.m
+            done += 1
+            same &= bool(np.array_equal(O.canonicalize(ref), lab_h[vo[i]: vo[i + 1]]))
+            if cpu_s > 15.0:
+                break
+        cpu = {"value": cpu_m / cpu_s, "unit": "edges/s", "cores": 1, "kind": "port",
+               "sample": f"{done} random graphs of the batch ({cpu_m} edges) solved one by one; single-thread C "
+                         f"restatement of kaspan::scc (scc/scc.hpp:21-87), lean form; {cpu_s:.2f}s; canonical "
+                         f"labels equal to the GPU batch's: {same}"}
+    elif not args.no_cpu_baseline:
         from oracle import oracle as O  # checker / CPU baseline only
 
         if args.config in CPU_SAMPLE:
@@ -220,6 +290,8 @@ def main():
                                                  "color_iters", "edges_scanned", "residual_n")},
         "kernels": kernels[:8],
     }
+    if batch_info is not None:
+        out["batch"] = batch_info
     print(json.dumps(out))
 
 

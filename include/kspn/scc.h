@@ -133,6 +133,34 @@ int kspn_scc_host(kspn_ctx* ctx, uint32_t n, uint64_t m, const uint64_t* fw_off,
                   const uint64_t* bw_off, const uint32_t* bw_adj, uint32_t* labels, const kspn_opts* opts,
                   kspn_stats* stats);
 
+/* ---- batches of independent graphs ------------------------------------- */
+/* A batch is the union of `count` graphs with batch-global ids: graph i owns
+ * vertices [vtx_off[i], vtx_off[i+1]) and the edge slots [edge_off[i],
+ * edge_off[i+1]) of fw_adj and bw_adj; its rows point only inside its own
+ * vertex range (fw_off[vtx_off[i]] = edge_off[i]). Offsets are uint64, so
+ * the union may hold more than 2^32 edges. There is no batch entry point in
+ * the reference: it solves one graph per scc() call (scc/scc.hpp:21-25);
+ * this is that call for every graph of the batch at once. */
+typedef struct kspn_batch {
+  uint32_t count;
+  uint32_t n;               /* vtx_off[count] */
+  uint64_t m;               /* edge_off[count] */
+  const uint32_t* vtx_off;  /* device, count+1 */
+  const uint64_t* edge_off; /* device, count+1 */
+  const uint64_t* fw_off;   /* device, n+1 */
+  const uint32_t* fw_adj;   /* device, m */
+  const uint64_t* bw_off;   /* device, n+1 */
+  const uint32_t* bw_adj;   /* device, m */
+} kspn_batch;
+
+/* kaspan::scc for every graph of the batch: labels[vtx_off[i] + v] is the
+ * canonical id of v inside graph i (its SCC's smallest local id), the value
+ * kspn_scc gives for graph i alone. Graph i is partition i of the
+ * multi-pivot FW-BW rounds (one pivot per graph and round). Requires
+ * count <= n. Stream-ordered; returns after the labels are complete. */
+int kspn_scc_batch(kspn_ctx* ctx, const kspn_batch* b, uint32_t* labels, const kspn_opts* opts, kspn_stats* stats,
+                   void* stream);
+
 /* ---- stages callable on their own ---------------------------------------- */
 /* backward_complement_graph (kaspan/include/kaspan/scc/backward_complement.hpp:22-68):
  * reverse CSR with ascending sources inside each row. Device pointers. */
@@ -201,6 +229,27 @@ int kspn_build_csr(kspn_ctx* ctx, uint32_t n, uint64_t count, const uint32_t* sr
                    uint64_t* m, uint64_t** fw_off, uint32_t** fw_adj, uint64_t** bw_off, uint32_t** bw_adj,
                    void* stream);
 int kspn_free(void* device_ptr);
+
+/* seeded batch of independent graphs: graph i is G(n,m) or R-MAT with
+ * probability p_rmat, n log-uniform in [n_min, n_max], draws = n * degree
+ * with the degree log-uniform in [deg_min, deg_max] */
+typedef struct kspn_batch_gen {
+  uint64_t seed;
+  uint32_t count;
+  uint32_t n_min, n_max;
+  double deg_min, deg_max;
+  double p_rmat;
+  double a, b, c; /* R-MAT quadrant probabilities */
+} kspn_batch_gen;
+
+/* per-graph plan (host arrays of count entries): kind (KSPN_GEN_GNM or
+ * KSPN_GEN_RMAT), vertices, edge draws before deduplication, seed */
+int kspn_batch_plan(const kspn_batch_gen* p, uint32_t* kind, uint32_t* n, uint64_t* draws, uint64_t* seed);
+/* generates and packs every graph of the plan (self-loops and duplicates
+ * removed, reverse CSR built per graph). The library allocates the six
+ * arrays of *out with hipMalloc; free each with kspn_free. fw_adj and
+ * bw_adj hold room for the draws of the plan (at least out->m). */
+int kspn_generate_batch(kspn_ctx* ctx, const kspn_batch_gen* p, kspn_batch* out, void* stream);
 /* canonical labels of a KSPN_GEN_PLANTED graph, known by construction (device labels[n]) */
 int kspn_planted_labels(kspn_ctx* ctx, const kspn_gen_params* p, uint32_t* labels, void* stream);
 

@@ -14,6 +14,7 @@ kaspan_amd                             reference
 ``allgather_fw_sub_graph``             ``allgather_fw_sub_graph`` at one rank (scc/allgather_sub_graph.hpp:253-301)
 ``tarjan``                             ``tarjan`` with min-id callback (scc/tarjan.hpp:124-136)
 ``normalise_scc_id``                   ``normalise_scc_id`` (test/normalise_scc.hpp:10-25)
+``Batch`` / ``scc_batch``              one ``scc`` call per graph, for many independent graphs at once
 =====================================  ===========================================================
 
 Every call goes through libkspn's C ABI (``include/kspn/scc.h``); device
@@ -35,7 +36,8 @@ from ._lib import KspnError, Stats
 __all__ = [
     "BidiGraph", "Context", "KspnError", "Stats", "scc", "scc_trim_ex", "scc_trim_ex_residual",
     "backward_complement_graph", "allgather_fw_sub_graph", "tarjan", "normalise_scc_id", "generate",
-    "METHOD_KASPAN", "METHOD_TRIM_EX", "UNDECIDED", "planted_labels", "scc_sizes",
+    "METHOD_KASPAN", "METHOD_TRIM_EX", "UNDECIDED", "planted_labels", "scc_sizes", "Batch", "generate_batch",
+    "batch_plan", "scc_batch",
 ]
 
 METHOD_KASPAN = _lib.METHOD_KASPAN
@@ -337,3 +339,131 @@ def scc_sizes(labels: torch.Tensor, ctx: Context | None = None, stream=None) -> 
     _lib.check(_lib.lib().kspn_scc_sizes(ctx.ptr, labels.numel(), C.c_void_p(labels.data_ptr()),
                                          C.c_void_p(sizes.data_ptr()), _stream_ptr(stream)), "kspn_scc_sizes")
     return sizes
+
+
+# --------------------------------------------------------------------------- batches
+class Batch:
+    """Many independent graphs as one union graph with batch-global ids.
+
+    ``graph`` is the union (a BidiGraph); graph i owns vertices
+    ``vtx_off[i]:vtx_off[i+1]`` and edge slots ``edge_off[i]:edge_off[i+1]``.
+    """
+
+    def __init__(self, graph: BidiGraph, vtx_off: torch.Tensor, edge_off: torch.Tensor):
+        self.graph = graph
+        self.vtx_off = _as_u32_tensor(vtx_off, graph.device)
+        self.edge_off = _as_u64_tensor(edge_off, graph.device)
+        self.count = self.vtx_off.numel() - 1
+        self._h_vo = self.vtx_off.cpu().numpy().view(np.uint32).astype(np.int64)
+        self._h_eo = self.edge_off.cpu().numpy().view(np.uint64).astype(np.int64)
+        if self.count < 1 or self._h_vo[-1] != graph.n or self._h_eo[-1] != graph.m:
+            raise KspnError(-1, "vtx_off/edge_off do not match the union graph")
+
+    @classmethod
+    def from_graphs(cls, graphs) -> "Batch":
+        """Pack BidiGraphs (local ids) into one union graph."""
+        if not graphs:
+            raise KspnError(-1, "empty batch")
+        dev = graphs[0].device
+        vo, eo = [0], [0]
+        fo, fa, bo, ba = [torch.zeros(1, dtype=torch.int64, device=dev)], [], [torch.zeros(1, dtype=torch.int64, device=dev)], []
+        for g in graphs:
+            fo.append(g.fw_off[1:] + eo[-1])
+            bo.append(g.bw_off[1:] + eo[-1])
+            fa.append(g.fw_adj + vo[-1])
+            ba.append(g.bw_adj + vo[-1])
+            vo.append(vo[-1] + g.n)
+            eo.append(eo[-1] + g.m)
+        u = BidiGraph.__new__(BidiGraph)
+        u.n, u.m, u.device = vo[-1], eo[-1], dev
+        u.fw_off, u.bw_off = torch.cat(fo), torch.cat(bo)
+        u.fw_adj = torch.cat(fa) if fa else torch.zeros(0, dtype=torch.int32, device=dev)
+        u.bw_adj = torch.cat(ba) if ba else torch.zeros(0, dtype=torch.int32, device=dev)
+        return cls(u, torch.tensor(np.array(vo, np.uint32).view(np.int32)), torch.tensor(np.array(eo, np.int64)))
+
+    def n_of(self, i: int) -> int:
+        return int(self._h_vo[i + 1] - self._h_vo[i])
+
+    def m_of(self, i: int) -> int:
+        return int(self._h_eo[i + 1] - self._h_eo[i])
+
+    def subgraph(self, i: int) -> BidiGraph:
+        """graph i alone, with local ids (copies)"""
+        v0, v1 = int(self._h_vo[i]), int(self._h_vo[i + 1])
+        e0, e1 = int(self._h_eo[i]), int(self._h_eo[i + 1])
+        u = self.graph
+        g = BidiGraph.__new__(BidiGraph)
+        g.n, g.m, g.device = v1 - v0, e1 - e0, u.device
+        g.fw_off = u.fw_off[v0 : v1 + 1] - e0
+        g.bw_off = u.bw_off[v0 : v1 + 1] - e0
+        g.fw_adj = u.fw_adj[e0:e1] - v0
+        g.bw_adj = u.bw_adj[e0:e1] - v0
+        return g
+
+    def view(self) -> _lib.Batch:
+        u = self.graph
+        return _lib.Batch(self.count, u.n, u.m, self.vtx_off.data_ptr(), self.edge_off.data_ptr(), u.fw_off.data_ptr(),
+                          u.fw_adj.data_ptr() if u.m else None, u.bw_off.data_ptr(), u.bw_adj.data_ptr() if u.m else None)
+
+
+def _batch_gen(seed, count, n_min=1000, n_max=1_000_000, deg_min=2.0, deg_max=16.0, p_rmat=0.5, a=0.57, b=0.19, c=0.19):
+    p = _lib.BatchGen()
+    p.seed, p.count, p.n_min, p.n_max = seed, count, n_min, n_max
+    p.deg_min, p.deg_max, p.p_rmat, p.a, p.b, p.c = deg_min, deg_max, p_rmat, a, b, c
+    return p
+
+
+def batch_plan(seed, count, **kw):
+    """(kind, n, draws, seed) per graph of a generated batch (numpy arrays)."""
+    p = _batch_gen(seed, count, **kw)
+    kind = np.zeros(count, np.uint32)
+    n = np.zeros(count, np.uint32)
+    draws = np.zeros(count, np.uint64)
+    gseed = np.zeros(count, np.uint64)
+    ptr = lambda a: C.c_void_p(a.ctypes.data)
+    _lib.check(_lib.lib().kspn_batch_plan(C.byref(p), ptr(kind), ptr(n), ptr(draws), ptr(gseed)), "kspn_batch_plan")
+    return kind, n, draws, gseed
+
+
+def generate_batch(seed, count, device="cuda", ctx: Context | None = None, stream=None, **kw) -> Batch:
+    """Seeded batch: graph i is G(n,m) or R-MAT (probability p_rmat) with n log-uniform in
+    [n_min, n_max] and average out-degree log-uniform in [deg_min, deg_max]."""
+    ctx = ctx or Context.default(torch.device(device).index or 0)
+    p = _batch_gen(seed, count, **kw)
+    b = _lib.Batch()
+    _lib.check(_lib.lib().kspn_generate_batch(ctx.ptr, C.byref(p), C.byref(b), _stream_ptr(stream)),
+               "kspn_generate_batch")
+    union = BidiGraph._adopt(b.n, b.m, [C.c_void_p(b.fw_off), C.c_void_p(b.fw_adj), C.c_void_p(b.bw_off),
+                                        C.c_void_p(b.bw_adj)], device)
+    vo = torch.empty(count + 1, dtype=torch.int32, device=device)
+    eo = torch.empty(count + 1, dtype=torch.int64, device=device)
+    try:
+        _lib.memcpy_d2d(vo.data_ptr(), b.vtx_off, 4 * (count + 1))
+        _lib.memcpy_d2d(eo.data_ptr(), b.edge_off, 8 * (count + 1))
+    finally:
+        _lib.lib().kspn_free(C.c_void_p(b.vtx_off))
+        _lib.lib().kspn_free(C.c_void_p(b.edge_off))
+    return Batch(union, vo, eo)
+
+
+def scc_batch(batch: Batch, scc_id: torch.Tensor | None = None, *, ctx: Context | None = None, stream=None,
+              profile=False, **opts) -> tuple[torch.Tensor, Stats]:
+    """scc() for every graph of the batch in one solve.
+
+    ``scc_id[vtx_off[i] + v]`` is v's canonical id inside graph i (local ids),
+    what ``scc(batch.subgraph(i))`` returns.
+    """
+    u = batch.graph
+    ctx = ctx or Context.default(u.device.index or 0)
+    if scc_id is None:
+        scc_id = torch.empty(u.n, dtype=torch.int32, device=u.device)
+    if scc_id.numel() != u.n or scc_id.dtype != torch.int32 or not scc_id.is_contiguous():
+        raise KspnError(-1, "scc_id must be a contiguous int32 tensor with one entry per batch vertex")
+    o = _opts(**opts)
+    if profile:
+        o.flags |= _lib.FLAG_PROFILE
+    st = Stats()
+    b = batch.view()
+    _lib.check(_lib.lib().kspn_scc_batch(ctx.ptr, C.byref(b), C.c_void_p(scc_id.data_ptr()), C.byref(o), C.byref(st),
+                                         _stream_ptr(stream)), "kspn_scc_batch")
+    return scc_id, st

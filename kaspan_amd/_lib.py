@@ -42,7 +42,8 @@ EXPORTS = (
     "kspn_opts_default", "kspn_strerror", "kspn_ctx_create", "kspn_ctx_destroy", "kspn_ctx_reserve",
     "kspn_scc", "kspn_scc_host", "kspn_transpose", "kspn_canonicalize", "kspn_subgraph", "kspn_tarjan",
     "kspn_generate", "kspn_build_csr", "kspn_free", "kspn_planted_labels", "kspn_pivot_features",
-    "kspn_mlp_forward", "kspn_mlp_grad", "kspn_mlp_adam", "kspn_scc_sizes",
+    "kspn_mlp_forward", "kspn_mlp_grad", "kspn_mlp_adam", "kspn_scc_sizes", "kspn_scc_batch", "kspn_batch_plan",
+    "kspn_generate_batch",
 )
 
 
@@ -60,6 +61,35 @@ class Graph(C.Structure):
         ("fw_adj", C.c_void_p),
         ("bw_off", C.c_void_p),
         ("bw_adj", C.c_void_p),
+    ]
+
+
+class Batch(C.Structure):
+    _fields_ = [
+        ("count", C.c_uint32),
+        ("n", C.c_uint32),
+        ("m", C.c_uint64),
+        ("vtx_off", C.c_void_p),
+        ("edge_off", C.c_void_p),
+        ("fw_off", C.c_void_p),
+        ("fw_adj", C.c_void_p),
+        ("bw_off", C.c_void_p),
+        ("bw_adj", C.c_void_p),
+    ]
+
+
+class BatchGen(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("count", C.c_uint32),
+        ("n_min", C.c_uint32),
+        ("n_max", C.c_uint32),
+        ("deg_min", C.c_double),
+        ("deg_max", C.c_double),
+        ("p_rmat", C.c_double),
+        ("a", C.c_double),
+        ("b", C.c_double),
+        ("c", C.c_double),
     ]
 
 
@@ -162,6 +192,9 @@ def lib():
         "kspn_mlp_grad": (i32, [vp, vp, vp, vp, u32, vp, vp, vp]),
         "kspn_mlp_adam": (i32, [vp, vp, vp, vp, u32, C.c_float, vp]),
         "kspn_scc_sizes": (i32, [vp, u32, vp, vp, vp]),
+        "kspn_scc_batch": (i32, [vp, P(Batch), vp, P(Opts), P(Stats), vp]),
+        "kspn_batch_plan": (i32, [P(BatchGen), vp, vp, vp, vp]),
+        "kspn_generate_batch": (i32, [vp, P(BatchGen), P(Batch), vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -122,6 +122,30 @@ __device__ inline uint32_t perm_n(uint64_t pseed, uint32_t bits, uint32_t n, uin
   return y;
 }
 
+// R-MAT on any n >= 1 (graphs of a batch): `bits` = ceil(log2 n) levels of
+// k_gen_rmat's quadrant choice, an id x >= n folds to x - n (2^bits < 2n),
+// then perm_n scatters the ids
+__global__ void k_gen_rmat_n(uint64_t seed, uint32_t n, uint32_t bits, uint64_t draws, uint32_t ta, uint32_t tab,
+                             uint32_t tabc, uint64_t pseed, uint64_t* keys) {
+  for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i < draws; i += (uint64_t)gridDim.x * WG) {
+    uint32_t u = 0, v = 0;
+    uint64_t w = 0;
+    for (uint32_t l = 0; l < bits; ++l) {
+      if ((l & 1) == 0) w = splitmix_at(seed, i * 16 + (l >> 1));
+      uint32_t x = (l & 1) ? (uint32_t)w : (uint32_t)(w >> 32);
+      uint32_t bu = x >= tab;
+      uint32_t bv = (x >= ta && x < tab) || x >= tabc;
+      u = (u << 1) | bu;
+      v = (v << 1) | bv;
+    }
+    if (u >= n) u -= n;
+    if (v >= n) v -= n;
+    u = perm_n(pseed, bits, n, u);
+    v = perm_n(pseed, bits, n, v);
+    keys[i] = (u != v) ? (((uint64_t)u << 32) | v) : ((uint64_t)n << 32);
+  }
+}
+
 // Planted power-law SCC graph. Positions [start[c], start[c+1]) form
 // component c, components in topological order; vertex id of position p is
 // perm_n(p). Per position: a cycle edge to the next position of its
@@ -190,27 +214,30 @@ __global__ void k_keys_from_pairs(const uint32_t* src, const uint32_t* dst, uint
   }
 }
 
-// keys (dst << 32 | src) for every CSR edge; the source is found by binary search over off
+// keys (dst << 32 | src) for every CSR edge; the source is found by binary
+// search over off. Offsets may carry a base (ebase, a batch's edge slot) and
+// ids a vertex base (vbase); keys are local to the graph.
 __global__ void k_keys_transposed(uint32_t n, uint64_t m, const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj,
-                                  uint64_t* __restrict__ keys) {
+                                  uint32_t vbase, uint64_t ebase, uint64_t* __restrict__ keys) {
   for (uint64_t e = blockIdx.x * (uint64_t)WG + threadIdx.x; e < m; e += (uint64_t)gridDim.x * WG) {
     uint32_t lo = 0, hi = n;  // last row with off[row] <= e
     while (hi - lo > 1) {
       uint32_t mid = (lo + hi) >> 1;
-      if (off[mid] <= e) lo = mid; else hi = mid;
+      if (off[mid] - ebase <= e) lo = mid; else hi = mid;
     }
-    keys[e] = ((uint64_t)adj[e] << 32) | lo;
+    keys[e] = ((uint64_t)(adj[e] - vbase) << 32) | lo;
   }
 }
 
-// sorted unique keys -> adj (low words) and row offsets from row boundaries
-__global__ void k_csr_fill(const uint64_t* __restrict__ keys, uint64_t m, uint32_t n, uint64_t* __restrict__ off,
-                           uint32_t* __restrict__ adj) {
+// sorted unique keys -> adj (low words) and row offsets from row
+// boundaries; ids shift by vbase and offsets by ebase (0 for a lone graph)
+__global__ void k_csr_fill(const uint64_t* __restrict__ keys, uint64_t m, uint32_t n, uint32_t vbase, uint64_t ebase,
+                           uint64_t* __restrict__ off, uint32_t* __restrict__ adj) {
   for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i <= m; i += (uint64_t)gridDim.x * WG) {
     int64_t prev = (i == 0) ? -1 : (int64_t)(keys[i - 1] >> 32);
     int64_t cur = (i == m) ? (int64_t)n : (int64_t)(keys[i] >> 32);
-    for (int64_t r = prev + 1; r <= cur; ++r) off[r] = i;
-    if (i < m) adj[i] = (uint32_t)keys[i];
+    for (int64_t r = prev + 1; r <= cur; ++r) off[r] = ebase + i;
+    if (i < m) adj[i] = vbase + (uint32_t)keys[i];
   }
 }
 
@@ -239,7 +266,8 @@ uint32_t gen_grid(uint64_t items) {
 // row, and write CSR (off[n+1], adj[m]). Returns m. `keys` and `alt` are
 // both clobbered.
 int keys_to_csr(uint32_t n, uint64_t count, uint64_t* keys, uint64_t* alt, uint64_t* off, uint32_t** adj_out,
-                uint64_t* m_out, bool alloc_adj, uint32_t* adj_given, hipStream_t s) {
+                uint64_t* m_out, bool alloc_adj, uint32_t* adj_given, hipStream_t s, uint32_t vbase = 0,
+                uint64_t ebase = 0, uint64_t adj_cap = ~0ull) {
   uint32_t end_bit = 32 + bits_for(n);
   if (end_bit > 64) end_bit = 64;
   hipcub::DoubleBuffer<uint64_t> db(keys, alt);
@@ -267,10 +295,11 @@ int keys_to_csr(uint32_t n, uint64_t count, uint64_t* keys, uint64_t* alt, uint6
     if ((last >> 32) >= n) m = nu - 1;  // the sentinel row
   }
   uint32_t* adj = adj_given;
+  if (!alloc_adj && m > adj_cap) return KSPN_E_INVALID;
   if (alloc_adj) {
     if (hipMalloc((void**)&adj, (m ? m : 1) * sizeof(uint32_t)) != hipSuccess) return KSPN_E_OOM;
   }
-  k_csr_fill<<<gen_grid(m + 1), WG, 0, s>>>(uniq, m, n, off, adj);
+  k_csr_fill<<<gen_grid(m + 1), WG, 0, s>>>(uniq, m, n, vbase, ebase, off, adj);
   KSPN_HIP(hipGetLastError());
   KSPN_HIP(hipStreamSynchronize(s));
   if (adj_out) *adj_out = adj;
@@ -278,16 +307,18 @@ int keys_to_csr(uint32_t n, uint64_t count, uint64_t* keys, uint64_t* alt, uint6
   return KSPN_OK;
 }
 
-// reverse CSR of (n, m, off, adj) into bw_off / bw_adj (caller buffers)
+// reverse CSR of (n, m, off, adj) into bw_off / bw_adj (caller buffers);
+// with bases, (off, adj) is graph slice of a batch and the result is
+// written with the same bases
 int transpose(uint32_t n, uint64_t m, const uint64_t* off, const uint32_t* adj, uint64_t* bw_off, uint32_t* bw_adj,
-              hipStream_t s) {
+              hipStream_t s, uint32_t vbase = 0, uint64_t ebase = 0) {
   DevBuf a, b;
   KSPN_TRY(a.alloc((m ? m : 1) * sizeof(uint64_t)));
   KSPN_TRY(b.alloc((m ? m : 1) * sizeof(uint64_t)));
-  if (m) k_keys_transposed<<<gen_grid(m), WG, 0, s>>>(n, m, off, adj, (uint64_t*)a.p);
+  if (m) k_keys_transposed<<<gen_grid(m), WG, 0, s>>>(n, m, off, adj, vbase, ebase, (uint64_t*)a.p);
   KSPN_HIP(hipGetLastError());
   uint64_t mm = 0;
-  KSPN_TRY(keys_to_csr(n, m, (uint64_t*)a.p, (uint64_t*)b.p, bw_off, nullptr, &mm, false, bw_adj, s));
+  KSPN_TRY(keys_to_csr(n, m, (uint64_t*)a.p, (uint64_t*)b.p, bw_off, nullptr, &mm, false, bw_adj, s, vbase, ebase, m));
   if (mm != m) return KSPN_E_INVALID;  // duplicate or out-of-range edges in the input
   return KSPN_OK;
 }
@@ -337,6 +368,39 @@ int build_both(uint32_t n, uint64_t count, uint64_t* keys, uint64_t* alt, uint64
   *fw_adj = fa;
   *bw_off = bo;
   *bw_adj = ba;
+  return KSPN_OK;
+}
+
+// per-graph plan of a batch: draws 4i..4i+3 of the batch stream give graph
+// i's size, average out-degree, kind and seed; log-uniform values are
+// exp(log lo + u (log hi - log lo)), u = top 53 bits / 2^53, truncated and
+// clamped to [lo, hi]; draws = floor(n * degree). The oracle's
+// orc_batch_plan evaluates the same expressions in the same order.
+int batch_plan(const kspn_batch_gen* p, uint32_t* kind, uint32_t* n, uint64_t* draws, uint64_t* gseed) {
+#pragma clang fp contract(off)
+  if (!p || p->count == 0 || p->n_min < 1 || p->n_min > p->n_max || p->n_max >= 0xFFFFFFFFu) return KSPN_E_INVALID;
+  if (!(p->deg_min > 0.0) || !(p->deg_min <= p->deg_max) || !(p->deg_max < 1e6)) return KSPN_E_INVALID;
+  const double inv53 = 1.0 / 9007199254740992.0;
+  const double ln0 = std::log((double)p->n_min), ln1 = std::log((double)p->n_max);
+  const double ld0 = std::log(p->deg_min), ld1 = std::log(p->deg_max);
+  for (uint32_t i = 0; i < p->count; ++i) {
+    double u0 = (double)(splitmix_at(p->seed, 4ull * i) >> 11) * inv53;
+    double u1 = (double)(splitmix_at(p->seed, 4ull * i + 1) >> 11) * inv53;
+    double u2 = (double)(splitmix_at(p->seed, 4ull * i + 2) >> 11) * inv53;
+    double t0 = u0 * (ln1 - ln0);
+    double x = std::exp(ln0 + t0);
+    uint32_t ni = x >= (double)p->n_max ? p->n_max : (uint32_t)x;
+    if (ni < p->n_min) ni = p->n_min;
+    double t1 = u1 * (ld1 - ld0);
+    double d = std::exp(ld0 + t1);
+    if (d > p->deg_max) d = p->deg_max;
+    if (d < p->deg_min) d = p->deg_min;
+    n[i] = ni;
+    draws[i] = (uint64_t)((double)ni * d);
+    kind[i] = u2 < p->p_rmat ? (uint32_t)KSPN_GEN_RMAT : (uint32_t)KSPN_GEN_GNM;
+    gseed[i] = splitmix_at(p->seed, 4ull * i + 3);
+    if (draws[i] >= (1ull << 31)) return KSPN_E_INVALID;  // one device sort per graph
+  }
   return KSPN_OK;
 }
 
@@ -443,6 +507,81 @@ int kspn_planted_labels(kspn_ctx* ctx, const kspn_gen_params* p, uint32_t* label
   k_planted_min<<<gen_grid(n), WG, 0, s>>>(n, nc, (const uint32_t*)ds.p, pseed, bits_for(n - 1), (uint32_t*)dm.p, 1, labels);
   KSPN_HIP(hipGetLastError());
   KSPN_HIP(hipStreamSynchronize(s));
+  return KSPN_OK;
+}
+
+int kspn_batch_plan(const kspn_batch_gen* p, uint32_t* kind, uint32_t* n, uint64_t* draws, uint64_t* seed) {
+  if (!kind || !n || !draws || !seed) return KSPN_E_INVALID;
+  return batch_plan(p, kind, n, draws, seed);
+}
+
+int kspn_generate_batch(kspn_ctx* ctx, const kspn_batch_gen* p, kspn_batch* out, void* stream) {
+  (void)ctx;
+  if (!p || !out) return KSPN_E_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t count = p->count;
+  std::vector<uint32_t> kind(count), ns(count);
+  std::vector<uint64_t> draws(count), seeds(count);
+  KSPN_TRY(batch_plan(p, kind.data(), ns.data(), draws.data(), seeds.data()));
+  uint64_t total_n = 0, total_draws = 0, max_draws = 1;
+  for (uint32_t i = 0; i < count; ++i) {
+    total_n += ns[i];
+    total_draws += draws[i];
+    max_draws = std::max<uint64_t>(max_draws, draws[i]);
+  }
+  if (total_n >= 0xFFFFFFFFull) return KSPN_E_INVALID;
+  const uint32_t N = (uint32_t)total_n;
+  // results (owned by the caller on success) and per-graph scratch
+  DevBuf fo, bo, fa, ba, vo, eo, keys, alt;
+  KSPN_TRY(fo.alloc((N + 1ull) * 8));
+  KSPN_TRY(bo.alloc((N + 1ull) * 8));
+  KSPN_TRY(fa.alloc(std::max<uint64_t>(total_draws, 1) * 4));
+  KSPN_TRY(ba.alloc(std::max<uint64_t>(total_draws, 1) * 4));
+  KSPN_TRY(vo.alloc((count + 1ull) * 4));
+  KSPN_TRY(eo.alloc((count + 1ull) * 8));
+  KSPN_TRY(keys.alloc(max_draws * 8));
+  KSPN_TRY(alt.alloc(max_draws * 8));
+  uint64_t* fw_off = (uint64_t*)fo.p;
+  uint64_t* bw_off = (uint64_t*)bo.p;
+  uint32_t* fw_adj = (uint32_t*)fa.p;
+  uint32_t* bw_adj = (uint32_t*)ba.p;
+  std::vector<uint32_t> h_vo(count + 1, 0);
+  std::vector<uint64_t> h_eo(count + 1, 0);
+  const uint32_t ta = prob32(p->a), tab = prob32(p->a + p->b), tabc = prob32(p->a + p->b + p->c);
+  for (uint32_t i = 0; i < count; ++i) {
+    const uint32_t vb = h_vo[i], n = ns[i];
+    const uint64_t eb = h_eo[i], cnt = draws[i];
+    uint64_t* k = (uint64_t*)keys.p;
+    if (cnt) {
+      if (kind[i] == KSPN_GEN_RMAT) {
+        uint32_t bits = 0;
+        while (bits < 32 && (1ull << bits) < n) ++bits;
+        k_gen_rmat_n<<<gen_grid(cnt), WG, 0, s>>>(seeds[i], n, bits, cnt, ta, tab, tabc,
+                                                  mix64(seeds[i] ^ 0x5045524D55544531ull), k);
+      } else {
+        k_gen_gnm<<<gen_grid(cnt), WG, 0, s>>>(seeds[i], n, cnt, k);
+      }
+      KSPN_HIP(hipGetLastError());
+    }
+    uint64_t m = 0;
+    KSPN_TRY(keys_to_csr(n, cnt, k, (uint64_t*)alt.p, fw_off + vb, nullptr, &m, false, fw_adj + eb, s, vb, eb,
+                         total_draws - eb));
+    KSPN_TRY(transpose(n, m, fw_off + vb, fw_adj + eb, bw_off + vb, bw_adj + eb, s, vb, eb));
+    h_vo[i + 1] = vb + n;
+    h_eo[i + 1] = eb + m;
+  }
+  KSPN_HIP(hipMemcpy(vo.p, h_vo.data(), (count + 1ull) * 4, hipMemcpyHostToDevice));
+  KSPN_HIP(hipMemcpy(eo.p, h_eo.data(), (count + 1ull) * 8, hipMemcpyHostToDevice));
+  out->count = count;
+  out->n = N;
+  out->m = h_eo[count];
+  out->vtx_off = (const uint32_t*)vo.p;
+  out->edge_off = (const uint64_t*)eo.p;
+  out->fw_off = fw_off;
+  out->fw_adj = fw_adj;
+  out->bw_off = bw_off;
+  out->bw_adj = bw_adj;
+  for (DevBuf* d : {&fo, &bo, &fa, &ba, &vo, &eo}) d->p = nullptr;  // handed to the caller
   return KSPN_OK;
 }
 

@@ -190,6 +190,7 @@ enum Counter : int {
   C_MIN,   // minimum vertex of a decided set
   C_FSEG,  // size of the current frontier segment
   C_DTOT,  // running length of the decided list
+  C_ERR,   // set by a kernel that met a broken internal invariant
   C_COUNT_ = 16
 };
 

@@ -1089,6 +1089,8 @@ __global__ __launch_bounds__(WG) void k_part_key(uint32_t n, const uint32_t* __r
                                                  uint64_t rnd_seed, unsigned long long* pbest) {
   const uint64_t pad = ((uint64_t)n + WG - 1) / WG * WG;
   const uint32_t lane = lane_id();
+  uint32_t wr = 0xFFFFFFFFu;  // partition of the wavefront's running maximum (uniform)
+  unsigned long long wm = 0;   // the running maximum (lane 0)
   for (uint64_t base = (uint64_t)blockIdx.x * WG; base < pad; base += (uint64_t)gridDim.x * WG) {
     const uint64_t v = base + threadIdx.x;
     const bool valid = v < n && bm_get(live, (uint32_t)v);
@@ -1109,7 +1111,10 @@ __global__ __launch_bounds__(WG) void k_part_key(uint32_t n, const uint32_t* __r
       }
       key = ((unsigned long long)pri << 32) | (uint32_t)~(uint32_t)v;
     }
-    // one peeling pass for the most common partition, then per-lane atomics
+    // one peeling pass for the most common partition; the wavefront keeps a
+    // running maximum for the last partition it peeled (grid-stride steps of
+    // one wavefront mostly see the same large partition), so one atomic per
+    // wavefront and partition change, not per step; then per-lane atomics
     uint64_t pending = __ballot(valid);
     bool left = valid;
     if (pending) {
@@ -1118,23 +1123,39 @@ __global__ __launch_bounds__(WG) void k_part_key(uint32_t n, const uint32_t* __r
       bool mine = valid && p == r;
       uint64_t grp = __ballot(mine);
       if (__popcll(grp) >= 8) {
-        unsigned long long m = wave_max64(mine ? key : 0ull);
-        if (lane == 0) atomicMax(&pbest[r], m);
+        unsigned long long m = wave_max64(mine ? key : 0ull);  // valid in lane 0
+        if (r != wr) {
+          if (lane == 0 && wr != 0xFFFFFFFFu && pbest[wr] < wm) atomicMax(&pbest[wr], wm);
+          wr = r;
+          wm = m;
+        } else if (m > wm) {
+          wm = m;
+        }
         if (mine) left = false;
       }
     }
     if (left && pbest[p] < key) atomicMax(&pbest[p], key);
   }
+  if (lane == 0 && wr != 0xFFFFFFFFu && pbest[wr] < wm) atomicMax(&pbest[wr], wm);
 }
 
-// pivot of every partition into F; its forward and backward bits set
-__global__ void k_part_pivots(uint32_t np, const unsigned long long* __restrict__ pbest, uint32_t* __restrict__ F,
-                              uint32_t* fwv, uint32_t* bwv) {
+// pivot of every partition into F; its forward and backward bits set.
+// Every partition holds a live vertex (pfwbw_rounds keeps it so); a key of
+// 0 or an id >= n would break that and is reported through *err instead of
+// being used as an index.
+__global__ void k_part_pivots(uint32_t np, uint32_t n, const unsigned long long* __restrict__ pbest,
+                              uint32_t* __restrict__ F, uint32_t* fwv, uint32_t* bwv, uint32_t* err) {
   for (uint32_t p = blockIdx.x * WG + threadIdx.x; p < np; p += gridDim.x * WG) {
-    uint32_t v = ~(uint32_t)pbest[p];
+    const unsigned long long k = pbest[p];
+    uint32_t v = ~(uint32_t)k;
+    if (k == 0ull || v >= n) {
+      *err = 1u;
+      v = 0;
+    } else {
+      atomicOr(&fwv[v >> 5], 1u << (v & 31));
+      atomicOr(&bwv[v >> 5], 1u << (v & 31));
+    }
     F[p] = v;
-    atomicOr(&fwv[v >> 5], 1u << (v & 31));
-    atomicOr(&bwv[v >> 5], 1u << (v & 31));
   }
 }
 
@@ -1146,19 +1167,49 @@ __global__ __launch_bounds__(WG) void k_part_classify(uint32_t n, const uint32_t
                                                       const uint32_t* __restrict__ pid, uint32_t* pmin, uint32_t* flags,
                                                       uint32_t* pcount) {
   const uint64_t pad = ((uint64_t)n + WG - 1) / WG * WG;
+  const uint32_t lane = lane_id();
+  // per wavefront (uniform): the partition whose pmin it last lowered and the
+  // flag it last set. Vertex ids grow along a lane's grid-stride steps and
+  // along the lanes, so the first decided vertex a wavefront meets in a
+  // partition is its smallest there: later ones of that partition skip the
+  // atomic, and a flag already set needs no second store.
+  uint32_t last_min = 0xFFFFFFFFu, last_flag = 0xFFFFFFFFu;
   for (uint64_t base = (uint64_t)blockIdx.x * WG; base < pad; base += (uint64_t)gridDim.x * WG) {
     const uint64_t v = base + threadIdx.x;
     const bool is_live = v < n && bm_get(live, (uint32_t)v);
     bool dec = false;
-    uint32_t p = 0;
+    uint32_t p = 0, fl = 0xFFFFFFFFu;
     if (is_live) {
       p = pid[v];
       bool f = bm_get(fwv, (uint32_t)v), b = bm_get(bwv, (uint32_t)v);
       dec = f && b;
-      if (!dec) flags[3ull * p + (f ? 1 : (b ? 2 : 0))] = 1u;
+      if (!dec) fl = 3u * p + (f ? 1u : (b ? 2u : 0u));
       if (pcount) atomicAdd(&pcount[4ull * p + (dec ? 0 : (f ? 1 : (b ? 2 : 3)))], 1u);
     }
-    grouped_atomic_min(dec, p, (uint32_t)v, pmin);
+    // pmin: the first decided lane's partition is peeled (its lowest lane holds its minimum)
+    uint64_t dm = __ballot(dec);
+    if (dm) {
+      const uint32_t leader = __ffsll((unsigned long long)dm) - 1;
+      const uint32_t r = __shfl(p, leader);
+      if (dec && p == r) {
+        if (lane == leader && r != last_min) atomicMin(&pmin[r], (uint32_t)v);
+        dec = false;
+      }
+      last_min = r;
+      if (dec && pmin[p] > (uint32_t)v) atomicMin(&pmin[p], (uint32_t)v);
+    }
+    // flags: the first flagging lane's flag is peeled, the rest store if unset
+    uint64_t fm = __ballot(fl != 0xFFFFFFFFu);
+    if (fm) {
+      const uint32_t leader = __ffsll((unsigned long long)fm) - 1;
+      const uint32_t r = __shfl(fl, leader);
+      if (fl == r) {
+        if (lane == leader && r != last_flag) flags[r] = 1u;
+        fl = 0xFFFFFFFFu;
+      }
+      last_flag = r;
+      if (fl != 0xFFFFFFFFu && flags[fl] == 0u) flags[fl] = 1u;
+    }
   }
 }
 
@@ -1228,6 +1279,23 @@ __global__ void k_scatter_f32(const uint32_t* __restrict__ list, const uint32_t*
                               float* __restrict__ out) {
   uint32_t cnt = *count;
   for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < cnt; i += gridDim.x * WG) out[list[i]] = vals[i];
+}
+
+// batches: pid[v] = i for the vertices of graph i (one workgroup per graph)
+__global__ void k_batch_pid(uint32_t count, const uint32_t* __restrict__ vtx_off, uint32_t* __restrict__ pid) {
+  for (uint32_t g = blockIdx.x; g < count; g += gridDim.x) {
+    const uint32_t b = vtx_off[g], e = vtx_off[g + 1];
+    for (uint32_t v = b + threadIdx.x; v < e; v += WG) pid[v] = g;
+  }
+}
+
+// batch labels hold the SCC's smallest batch-global id, a vertex of the same
+// graph: subtracting the graph's first id gives the local canonical id
+__global__ void k_batch_localize(uint32_t count, const uint32_t* __restrict__ vtx_off, uint32_t* __restrict__ labels) {
+  for (uint32_t g = blockIdx.x; g < count; g += gridDim.x) {
+    const uint32_t b = vtx_off[g], e = vtx_off[g + 1];
+    for (uint32_t v = b + threadIdx.x; v < e; v += WG) labels[v] -= b;
+  }
 }
 
 // =====================================================================
@@ -1343,7 +1411,12 @@ int ctx_reserve(Ctx* c, uint32_t n, uint64_t m) {
     return KSPN_E_HIP;
   if (hipcub::DeviceScan::ExclusiveSum(nullptr, b2, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)(words + 1)) != hipSuccess)
     return KSPN_E_HIP;
-  size_t need = std::max(b1, b2) + 256;
+  // partition flag scans: 3 flags per partition, partitions <= vertices
+  size_t b3 = 0;
+  const uint64_t flags_max = std::min<uint64_t>(3ull * nn + 1, 0x7FFFFFFFull);
+  if (hipcub::DeviceScan::ExclusiveSum(nullptr, b3, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)flags_max) != hipSuccess)
+    return KSPN_E_HIP;
+  size_t need = std::max(std::max(b1, b2), b3) + 256;
   if (need > c->cub_bytes) {
     if (c->cub_tmp) (void)hipFree(c->cub_tmp);
     c->cub_tmp = nullptr;
@@ -1439,6 +1512,7 @@ struct Run {
   uint32_t log_count = 0;   // pivot outcome records written
   bool pid_active = false;  // pid partitions the live vertices (after multi-pivot FW-BW)
   uint32_t np0 = 1;         // number of preset partitions
+  const uint32_t* vtx_off = nullptr;  // batch: graph i owns [vtx_off[i], vtx_off[i+1]); labels are made local
 };
 
 int select_pivot_live(Run& R, uint32_t* pivot);
@@ -1634,6 +1708,22 @@ int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
   if (!R.pid_preset) {
     k_fill_u32<<<stream_grid(c, n), WG, 0, R.s>>>(c->pid, n, 0u);
     np = 1;
+  } else {
+    // preset partitions (the graphs of a batch) may have lost every vertex
+    // to the first trim pass: renumber those that hold live vertices. With
+    // nothing reached, classify flags class 0 of each such partition and
+    // apply maps pid to the scanned flag (no vertex is decided).
+    const uint32_t gridn = stream_grid(c, n);
+    Prof p(c, R.s, K_BITMAP, 12.0 * R.live_n + 12.0 * np);
+    k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->fwv, R.words, 0u);
+    k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwv, R.words, 0u);
+    k_fill_u32<<<stream_grid(c, 3ull * np + 1), WG, 0, R.s>>>(c->pflags, 3ull * np + 1, 0u);
+    k_part_classify<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->fwv, c->bwv, c->pid, c->pmin, c->pflags, nullptr);
+    KSPN_TRY(exclusive_scan_u32(c, c->pflags, c->pnew, 3ull * np + 1, R.s));
+    k_part_apply<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->fwv, c->bwv, c->pid, c->pmin, c->pnew, R.labels, c->part);
+    k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_AUX], c->pnew + 3ull * np);
+    KSPN_TRY(publish(c, R.s, nullptr));
+    np = c->h_cnt[C_AUX];
   }
   const float* score = nullptr;
   if (R.o.pivot == KSPN_PIVOT_LEARNED) {
@@ -1664,14 +1754,16 @@ int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
                                          (unsigned long long*)c->pbest);
       k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->fwv, R.words, 0u);
       k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwv, R.words, 0u);
-      k_part_pivots<<<stream_grid(c, np), WG, 0, R.s>>>(np, (unsigned long long*)c->pbest, c->qa, c->fwv, c->bwv);
+      k_part_pivots<<<stream_grid(c, np), WG, 0, R.s>>>(np, R.n, (unsigned long long*)c->pbest, c->qa, c->fwv, c->bwv,
+                                                      &c->cnt[C_ERR]);
       k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], np);
     }
     uint32_t lp = 0, lq = 0;
     KSPN_TRY(reach(R, 0, np, c->pid, &lp, &lq));
     {
       Prof p(c, R.s, K_PIVOT, 8.0 * np);
-      k_part_pivots<<<stream_grid(c, np), WG, 0, R.s>>>(np, (unsigned long long*)c->pbest, c->qa, c->fwv, c->bwv);
+      k_part_pivots<<<stream_grid(c, np), WG, 0, R.s>>>(np, R.n, (unsigned long long*)c->pbest, c->qa, c->fwv, c->bwv,
+                                                      &c->cnt[C_ERR]);
       k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], np);
     }
     KSPN_TRY(reach(R, 1, np, c->pid, &lp, &lq));
@@ -1701,6 +1793,7 @@ int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
     }
     k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_AUX], c->pnew + 3ull * np);  // scan total = new partition count
     KSPN_TRY(publish(c, R.s, nullptr));
+    if (c->h_cnt[C_ERR]) return KSPN_E_INVALID;  // a partition without a live vertex
     const uint32_t d = c->h_cnt[C_DECIDED];
     np = c->h_cnt[C_AUX];
     R.live_n -= d;
@@ -2180,7 +2273,12 @@ int run_pipeline(Run& R) {
   }
   mark(4);
   // every stage writes canonical ids (own id, pair minimum, SCC minimum),
-  // so no pass over all labels is needed here
+  // so no pass over all labels is needed here; a batch's labels become
+  // local to their graph
+  if (R.vtx_off) {
+    Prof p(c, R.s, K_LABEL, 8.0 * R.n);
+    k_batch_localize<<<std::min<uint32_t>(R.np0, 16384u), WG, 0, R.s>>>(R.np0, R.vtx_off, R.labels);
+  }
   mark(5);
   KSPN_HIP(hipStreamSynchronize(R.s));
   if (timing) {
@@ -2292,7 +2390,13 @@ int kspn_ctx_reserve(kspn_ctx* k, uint32_t n, uint64_t m) {
   return ctx_reserve(&k->c, n, m);
 }
 
-int kspn_scc(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* opts, kspn_stats* stats, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// one solve: a graph, or the union graph of a batch (vtx_off != nullptr)
+int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* opts, kspn_stats* stats, void* stream,
+          const uint32_t* vtx_off, uint32_t count) {
   if (!k || !g || (g->n > 0 && (!labels || !g->fw_off || !g->bw_off))) return KSPN_E_INVALID;
   if (g->m > 0 && (!g->fw_adj || !g->bw_adj)) return KSPN_E_INVALID;
   if (g->n == 0xFFFFFFFFu) return KSPN_E_INVALID;  // 0xFFFFFFFF is the undecided sentinel
@@ -2304,6 +2408,7 @@ int kspn_scc(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts
   if (o.pivot == KSPN_PIVOT_LEARNED && !o.mlp_params) return KSPN_E_INVALID;
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (g->n == 0) return KSPN_OK;
+  if (vtx_off && (count == 0 || count > g->n)) return KSPN_E_INVALID;
   Ctx* c = &k->c;
   if (hipSetDevice(c->device) != hipSuccess) return KSPN_E_HIP;
   KSPN_TRY(ctx_reserve(c, g->n, g->m));
@@ -2312,6 +2417,15 @@ int kspn_scc(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts
   if (c->profile)
     for (auto& t : c->kt) { t.launches = 0; t.ms = 0; t.bytes = 0; }
   Run R{c, g, labels, o, stats, s, g->n, bm_words(g->n)};
+  if (vtx_off) {
+    // graph i is preset partition i of the multi-pivot FW-BW rounds
+    if (!c->pid) KSPN_TRY(dalloc(&c->pid, (uint64_t)c->cap_n + 64));
+    k_batch_pid<<<std::min<uint32_t>(count, 16384u), WG, 0, s>>>(count, vtx_off, c->pid);
+    KSPN_HIP(hipGetLastError());
+    R.pid_preset = true;
+    R.np0 = count;
+    R.vtx_off = vtx_off;
+  }
   int rc = run_pipeline(R);
   if (c->profile) {
     (void)hipStreamSynchronize(s);
@@ -2329,6 +2443,21 @@ int kspn_scc(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts
     c->profile = false;
   }
   return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kspn_scc(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* opts, kspn_stats* stats, void* stream) {
+  return solve(k, g, labels, opts, stats, stream, nullptr, 0);
+}
+
+int kspn_scc_batch(kspn_ctx* k, const kspn_batch* b, uint32_t* labels, const kspn_opts* opts, kspn_stats* stats,
+                   void* stream) {
+  if (!b || (b->n > 0 && !b->vtx_off)) return KSPN_E_INVALID;
+  kspn_graph g{b->n, b->m, b->fw_off, b->fw_adj, b->bw_off, b->bw_adj};
+  return solve(k, &g, labels, opts, stats, stream, b->vtx_off, b->count);
 }
 
 int kspn_scc_host(kspn_ctx* k, uint32_t n, uint64_t m, const uint64_t* fw_off, const uint32_t* fw_adj,

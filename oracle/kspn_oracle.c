@@ -115,6 +115,65 @@ uint64_t orc_gen_rmat(uint64_t seed, uint32_t scale, uint64_t draws, double a, d
   return k;
 }
 
+/* R-MAT on an arbitrary vertex count (the graphs of a batch): the levels of
+ * orc_gen_rmat over [0, 2^bits) with bits = ceil(log2 n); since 2^bits < 2n
+ * an id x >= n folds to x - n; ids are then scattered by perm_n below. */
+static uint32_t perm_n(uint64_t pseed, uint32_t bits, uint32_t n, uint32_t x);
+uint64_t orc_gen_rmat_n(uint64_t seed, uint32_t n, uint64_t draws, double a, double b, double c, uint32_t* src,
+                        uint32_t* dst) {
+  uint32_t ta = prob32(a), tab = prob32(a + b), tabc = prob32(a + b + c);
+  uint64_t pseed = mix64(seed ^ 0x5045524D55544531ull);
+  uint32_t bits = 0;
+  while (bits < 32 && (1ull << bits) < n) ++bits;
+  uint64_t k = 0;
+  for (uint64_t i = 0; i < draws; ++i) {
+    uint32_t u = 0, v = 0;
+    uint64_t w = 0;
+    for (uint32_t l = 0; l < bits; ++l) {
+      if ((l & 1) == 0) w = orc_splitmix64_at(seed, i * 16 + (l >> 1));
+      uint32_t x = (l & 1) ? (uint32_t)w : (uint32_t)(w >> 32);
+      uint32_t bu = x >= tab;
+      uint32_t bv = (x >= ta && x < tab) || x >= tabc;
+      u = (u << 1) | bu;
+      v = (v << 1) | bv;
+    }
+    if (u >= n) u -= n;
+    if (v >= n) v -= n;
+    u = perm_n(pseed, bits, n, u);
+    v = perm_n(pseed, bits, n, v);
+    if (u != v) { src[k] = u; dst[k] = v; ++k; }
+  }
+  return k;
+}
+
+/* Per-graph plan of a batch: draws 4i..4i+3 of the batch stream give graph
+ * i's size, average out-degree, kind and seed. Log-uniform values are
+ * exp(log lo + u (log hi - log lo)) with u = top 53 bits / 2^53, truncated
+ * and clamped to [lo, hi]; draws = floor(n * degree). */
+void orc_batch_plan(uint64_t seed, uint32_t count, uint32_t n_min, uint32_t n_max, double deg_min, double deg_max,
+                    double p_rmat, uint32_t* kind, uint32_t* n, uint64_t* draws, uint64_t* gseed) {
+  const double inv53 = 1.0 / 9007199254740992.0;
+  const double ln0 = log((double)n_min), ln1 = log((double)n_max);
+  const double ld0 = log(deg_min), ld1 = log(deg_max);
+  for (uint32_t i = 0; i < count; ++i) {
+    double u0 = (double)(orc_splitmix64_at(seed, 4ull * i) >> 11) * inv53;
+    double u1 = (double)(orc_splitmix64_at(seed, 4ull * i + 1) >> 11) * inv53;
+    double u2 = (double)(orc_splitmix64_at(seed, 4ull * i + 2) >> 11) * inv53;
+    double t0 = u0 * (ln1 - ln0);
+    double x = exp(ln0 + t0);
+    uint32_t ni = x >= (double)n_max ? n_max : (uint32_t)x;
+    if (ni < n_min) ni = n_min;
+    double t1 = u1 * (ld1 - ld0);
+    double d = exp(ld0 + t1);
+    if (d > deg_max) d = deg_max;
+    if (d < deg_min) d = deg_min;
+    n[i] = ni;
+    draws[i] = (uint64_t)((double)ni * d);
+    kind[i] = u2 < p_rmat ? 1u : 0u;
+    gseed[i] = orc_splitmix64_at(seed, 4ull * i + 3);
+  }
+}
+
 uint64_t orc_lattice_max_edges(uint32_t side) {
   uint64_t s = side;
   return s < 2 ? 0 : 2 * 3 * s * s * (s - 1);

@@ -37,6 +37,14 @@ uint64_t orc_gen_fuzzy(uint64_t seed, uint32_t n, double d, uint32_t* src, uint3
                        uint64_t cap, uint32_t* labels);
 /* power-law planted SCC graph (component sizes, Hamiltonian cycles, chords,
  * forward-only inter-component edges); labels = planted canonical labels */
+/* R-MAT on any n >= 1: ceil(log2 n) levels, an id x >= n folds to x - n,
+ * then the cycle-walking bijection of [0, n) scatters the ids */
+uint64_t orc_gen_rmat_n(uint64_t seed, uint32_t n, uint64_t draws, double a, double b, double c, uint32_t* src,
+                        uint32_t* dst);
+/* batch of independent graphs: per graph its kind (0 G(n,m), 1 R-MAT), n,
+ * draws and seed; n and the average out-degree are log-uniform */
+void orc_batch_plan(uint64_t seed, uint32_t count, uint32_t n_min, uint32_t n_max, double deg_min, double deg_max,
+                    double p_rmat, uint32_t* kind, uint32_t* n, uint64_t* draws, uint64_t* gseed);
 uint64_t orc_gen_planted(uint64_t seed, uint32_t n, double expo, uint32_t max_size, double intra_deg,
                          double inter_deg, uint32_t* src, uint32_t* dst, uint64_t cap,
                          uint32_t* labels);

@@ -60,6 +60,8 @@ def lib():
             "orc_gen_gnm": (u64, [u64, u32, u64, _u32p, _u32p]),
             "orc_gen_rmat": (u64, [u64, u32, u64, dbl, dbl, dbl, i32, _u32p, _u32p]),
             "orc_gen_lattice": (u64, [u64, u32, dbl, _u32p, _u32p]),
+            "orc_gen_rmat_n": (u64, [u64, u32, u64, dbl, dbl, dbl, _u32p, _u32p]),
+            "orc_batch_plan": (None, [u64, u32, u32, u32, dbl, dbl, dbl, _u32p, _u32p, _u64p, _u64p]),
             "orc_lattice_max_edges": (u64, [u32]),
             "orc_gen_fuzzy": (u64, [u64, u32, dbl, _u32p, _u32p, u64, _u32p]),
             "orc_gen_planted": (u64, [u64, u32, dbl, u32, dbl, dbl, _u32p, _u32p, u64, _u32p]),
@@ -141,6 +143,57 @@ def gen_rmat(seed, scale, draws, a=0.57, b=0.19, c=0.19, permute=True):
     dst = np.zeros(max(draws, 1), np.uint32)
     k = lib().orc_gen_rmat(seed, scale, draws, a, b, c, int(permute), src, dst)
     return src[:k], dst[:k]
+
+
+def gen_rmat_n(seed, n, draws, a=0.57, b=0.19, c=0.19):
+    src = np.zeros(max(draws, 1), np.uint32)
+    dst = np.zeros(max(draws, 1), np.uint32)
+    k = lib().orc_gen_rmat_n(seed, n, draws, a, b, c, src, dst)
+    return src[:k], dst[:k]
+
+
+def batch_plan(seed, count, n_min=1000, n_max=1_000_000, deg_min=2.0, deg_max=16.0, p_rmat=0.5):
+    """Per-graph (kind, n, draws, seed) arrays of a batch (kind 0 G(n,m), 1 R-MAT)."""
+    kind = np.zeros(max(count, 1), np.uint32)
+    n = np.zeros(max(count, 1), np.uint32)
+    draws = np.zeros(max(count, 1), np.uint64)
+    gseed = np.zeros(max(count, 1), np.uint64)
+    lib().orc_batch_plan(seed, count, n_min, n_max, deg_min, deg_max, p_rmat, kind, n, draws, gseed)
+    return kind[:count].copy(), n[:count].copy(), draws[:count].copy(), gseed[:count].copy()
+
+
+def gen_batch(seed, count, n_min=1000, n_max=1_000_000, deg_min=2.0, deg_max=16.0, p_rmat=0.5, a=0.57, b=0.19, c=0.19):
+    """The graphs of a batch plan, each deduplicated, and their union with
+    batch-global ids: (union HostGraph, vtx_off[count+1], edge_off[count+1], graphs)."""
+    kind, ns, draws, gseed = batch_plan(seed, count, n_min, n_max, deg_min, deg_max, p_rmat)
+    graphs = []
+    for i in range(count):
+        if kind[i]:
+            src, dst = gen_rmat_n(int(gseed[i]), int(ns[i]), int(draws[i]), a, b, c)
+        else:
+            src, dst = gen_gnm(int(gseed[i]), int(ns[i]), int(draws[i]))
+        graphs.append(csr_from_edges(int(ns[i]), src, dst))
+    vtx_off = np.zeros(count + 1, np.uint32)
+    edge_off = np.zeros(count + 1, np.uint64)
+    for i, g in enumerate(graphs):
+        vtx_off[i + 1] = vtx_off[i] + g.n
+        edge_off[i + 1] = edge_off[i] + g.m
+    return pack_batch(graphs), vtx_off, edge_off, graphs
+
+
+def pack_batch(graphs) -> "HostGraph":
+    """Union of independent graphs: graph i's ids shift by the vertices before it."""
+    fo, fa, bo, ba = [np.zeros(1, np.uint64)], [], [np.zeros(1, np.uint64)], []
+    vb, eb = 0, 0
+    for g in graphs:
+        fo.append(g.fw_off[1:] + np.uint64(eb))
+        bo.append(g.bw_off[1:] + np.uint64(eb))
+        fa.append(g.fw_adj[: g.m] + np.uint32(vb))
+        ba.append(g.bw_adj[: g.m] + np.uint32(vb))
+        vb += g.n
+        eb += g.m
+    cat = lambda xs, dt: np.concatenate(xs).astype(dt) if xs else np.zeros(0, dt)
+    return HostGraph(vb, cat(fo, np.uint64), cat(fa, np.uint32), cat(bo, np.uint64), cat(ba, np.uint32))
 
 
 def gen_lattice(seed, side, p_bidir=0.1):

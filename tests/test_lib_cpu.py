@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -46,6 +47,8 @@ int main(void) {
          sizeof(kspn_gen_params), sizeof(kspn_kern_stat));
   printf("%zu %zu %zu\n", offsetof(kspn_stats, edges_scanned), offsetof(kspn_stats, n_kern), offsetof(kspn_stats, kern));
   printf("%zu %zu\n", offsetof(kspn_opts, mlp_params), offsetof(kspn_gen_params, permute));
+  printf("%zu %zu %zu %zu\n", sizeof(kspn_batch), offsetof(kspn_batch, bw_adj), sizeof(kspn_batch_gen),
+         offsetof(kspn_batch_gen, c));
   return 0;
 }
 ''')
@@ -54,7 +57,8 @@ int main(void) {
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
     got = [C.sizeof(_lib.Graph), C.sizeof(_lib.Opts), C.sizeof(_lib.Stats), C.sizeof(_lib.GenParams),
            C.sizeof(_lib.KernStat), _lib.Stats.edges_scanned.offset, _lib.Stats.n_kern.offset, _lib.Stats.kern.offset,
-           _lib.Opts.mlp_params.offset, _lib.GenParams.permute.offset]
+           _lib.Opts.mlp_params.offset, _lib.GenParams.permute.offset, C.sizeof(_lib.Batch), _lib.Batch.bw_adj.offset,
+           C.sizeof(_lib.BatchGen), _lib.BatchGen.c.offset]
     assert [int(x) for x in out] == got
 
 
@@ -82,3 +86,23 @@ def test_ctx_create_fails_cleanly_without_device():
     p = C.c_void_p()
     assert _lib.lib().kspn_ctx_create(C.byref(p), 0) == -1
     assert not p.value
+
+
+def test_batch_plan_matches_oracle(O):
+    """The library's host-side batch plan and the oracle's are the same numbers."""
+    import kaspan_amd as ka
+
+    for seed, count, kw in [(0x6B5350AA00000004, 4096, {}), (7, 50, dict(n_min=1, n_max=5, deg_min=0.5, deg_max=3.0)),
+                            (8, 300, dict(n_min=100, n_max=100, p_rmat=1.0)), (9, 10, dict(p_rmat=0.0))]:
+        got = ka.batch_plan(seed, count, **kw)
+        ref = O.batch_plan(seed, count, **kw)
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b)
+    from kaspan_amd import _lib
+
+    bad = _lib.BatchGen()
+    bad.count, bad.n_min, bad.n_max, bad.deg_min, bad.deg_max = 4, 10, 5, 1.0, 2.0  # n_min > n_max
+    z = np.zeros(4, np.uint64)
+    ptr = lambda a: C.c_void_p(a.ctypes.data)
+    assert _lib.lib().kspn_batch_plan(C.byref(bad), ptr(z), ptr(z), ptr(z), ptr(z)) == -1
+    assert _lib.lib().kspn_scc_batch(None, None, None, None, None, None) == -1

@@ -247,6 +247,47 @@ def test_generator_shapes(O):
     assert s.max() < 1000 and d.max() < 1000 and np.all(s != d)
 
 
+def test_rmat_n_reduces_to_rmat_on_powers_of_two(O):
+    # for n = 2^s nothing folds and the cycle-walking bijection is perm_pow2:
+    # the batch's R-MAT stream is then exactly the pinned scale-s stream
+    for seed, s in [(3, 6), (9, 10), (21, 13)]:
+        a, b = O.gen_rmat_n(seed, 1 << s, 5000), O.gen_rmat(seed, s, 5000, permute=True)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    # any n: ids stay in range, no self-loops, the degree stays heavy-tailed
+    s, d = O.gen_rmat_n(5, 3000, 60000)
+    assert s.max() < 3000 and d.max() < 3000 and np.all(s != d)
+    deg = np.bincount(s, minlength=3000)
+    assert deg.max() > 10 * deg.mean()
+    assert O.gen_rmat_n(5, 1, 10)[0].size == 0  # one vertex: every draw is a self-loop
+
+
+def test_batch_plan_distribution(O):
+    kind, n, draws, seeds = O.batch_plan(0x6B5350AA00000004, 4096)
+    assert n.min() >= 1000 and n.max() <= 1_000_000
+    deg = draws / n
+    assert deg.min() >= 2.0 - 1e-3 and deg.max() <= 16.0
+    # log-uniform: log n has mean (ln 1e3 + ln 1e6) / 2 and the kinds split evenly
+    assert abs(np.log(n.astype(np.float64)).mean() - np.log(1e3 * 1e6) / 2) < 0.15
+    assert 0.45 < kind.mean() < 0.55
+    assert len(np.unique(seeds)) == 4096
+    # the whole plan of the 4096-graph workload: about 5.9e8 vertices, 4.0e9 draws
+    assert 5.0e8 < int(n.sum()) < 7.0e8 and 3.4e9 < int(draws.sum()) < 4.8e9
+
+
+def test_batch_union_labels(O):
+    u, vo, eo, graphs = O.gen_batch(11, 20, 40, 2000, 1.0, 6.0)
+    assert u.n == int(vo[-1]) and u.m == int(eo[-1])
+    lab = O.tarjan(u)
+    for i, g in enumerate(graphs):
+        local = O.tarjan(g)
+        # the union's components never cross graphs, so its min ids are local ids + the graph's base
+        assert np.array_equal(lab[vo[i]: vo[i + 1]] - vo[i], local)
+        # rows point inside the graph's own ranges
+        assert int(u.fw_off[vo[i]]) == int(eo[i])
+        seg = u.fw_adj[eo[i]: eo[i + 1]]
+        assert seg.size == 0 or (seg.min() >= vo[i] and seg.max() < vo[i + 1])
+
+
 # ---------------------------------------------------------------- MLP
 def test_mlp_gradients_finite_differences(O):
     rng = np.random.default_rng(0)
