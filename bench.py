@@ -78,6 +78,8 @@ def main():
     ap.add_argument("--method", type=int, default=None, help="0 kaspan, 1 trim_ex (default: fastest known)")
     ap.add_argument("--pivot", choices=["plain", "learned"], default="plain")
     ap.add_argument("--fwbw-rounds", type=int, default=1)
+    ap.add_argument("--coop-edges", type=int, default=0, help="persistent-kernel frontier edge bound (0: library default)")
+    ap.add_argument("--coop-workgroups", type=int, default=0, help="persistent-kernel workgroups (0: library default)")
     ap.add_argument("--ab", action="store_true", help="also time the other pivot rule; report both and the speedup")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -133,7 +135,8 @@ def main():
         mlp = P.load_params()
         if mlp is None:
             raise SystemExit("no trained scorer: run python -m kaspan_amd.train_pivot first")
-    opts = dict(method=method, fwbw_rounds=args.fwbw_rounds)
+    opts = dict(method=method, fwbw_rounds=args.fwbw_rounds, coop_edges=args.coop_edges,
+                coop_workgroups=args.coop_workgroups)
     if args.pivot == "learned":
         opts.update(pivot=1, mlp_params=mlp)
 
@@ -159,7 +162,8 @@ def main():
         import numpy as np
 
         first = labels.clone()
-        other = dict(method=method, fwbw_rounds=args.fwbw_rounds)
+        other = dict(method=method, fwbw_rounds=args.fwbw_rounds, coop_edges=args.coop_edges,
+                coop_workgroups=args.coop_workgroups)
         if args.pivot == "plain":
             other.update(pivot=1, mlp_params=mlp)
         ms_other, st_other = timed(other)

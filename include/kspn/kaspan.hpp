@@ -27,7 +27,10 @@ using index_t = uint64_t;
 
 struct error : std::runtime_error {
   int code;
-  error(int c, const char* where) : std::runtime_error(std::string(where) + ": " + kspn_strerror(c)), code(c) {}
+  error(int c, const char* where)
+      : std::runtime_error(std::string(where) + ": " + kspn_strerror(c) +
+                           (c == KSPN_E_HIP ? std::string(" [") + kspn_last_error() + "]" : std::string())),
+        code(c) {}
 };
 
 inline void check(int rc, const char* where) {

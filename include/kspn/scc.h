@@ -61,6 +61,7 @@ typedef struct kspn_graph {
 #define KSPN_FLAG_NO_TRIM2 2u /* KSPN_METHOD_TRIM_EX without the trim-2 pass */
 #define KSPN_FLAG_NO_TAIL 4u  /* never hand the rest to the single-workgroup tail solver */
 #define KSPN_FLAG_NO_REFINE 8u /* colour rounds over the whole live graph, without partition refinement */
+#define KSPN_FLAG_NO_COOP 16u  /* every FW/BW level as separate launches (no persistent multi-level kernel) */
 
 typedef struct kspn_opts {
   uint32_t method;             /* KSPN_METHOD_* */
@@ -76,6 +77,9 @@ typedef struct kspn_opts {
   uint32_t* pivot_log;         /* device, KSPN_LOG_WORDS u32 per record, or NULL: outcome of every multi-pivot FW-BW pivot */
   uint32_t pivot_log_cap;      /* records that fit in pivot_log */
   uint32_t seed;               /* KSPN_PIVOT_RANDOM */
+  uint32_t coop_edges;         /* persistent multi-level kernel takes frontiers with at most this many out-edges (0 -> 2^20) */
+  uint32_t coop_degree;        /* ... and no vertex of out-degree above this (0 -> 2048) */
+  uint32_t coop_workgroups;    /* its workgroups, at most one per CU (0 -> 128) */
 } kspn_opts;
 
 /* pivot outcome record: pivot, round, partition size, SCC size (decided),
@@ -101,7 +105,7 @@ typedef struct kspn_stats {
   uint32_t fwbw_rounds_done;  /* multi-pivot FW-BW rounds run */
   uint32_t partitions_max;    /* largest partition count of those rounds */
   uint32_t pivot_log_count;   /* records written to opts.pivot_log */
-  uint32_t reserved0;
+  uint32_t fwbw_levels_coop;  /* FW/BW push levels run inside the persistent multi-level kernel */
   uint64_t residual_m;
   uint64_t edges_scanned;  /* adjacency entries read, all phases */
   double bytes_trim, bytes_fwbw, bytes_color, bytes_residual, bytes_canon; /* algorithmic bytes */
@@ -112,6 +116,9 @@ typedef struct kspn_stats {
 /* defaults: KASPAN method, plain pivot, 1 FW-BW round, no residual, no cap */
 void kspn_opts_default(kspn_opts* o);
 const char* kspn_strerror(int code);
+/* the calling thread's last failing HIP call inside the library (call, HIP
+ * error name, source location); "" if none */
+const char* kspn_last_error(void);
 
 /* ---- context ------------------------------------------------------------ */
 /* Creates a context on `device`; scratch grows on demand to the largest graph seen. */

@@ -33,13 +33,14 @@ FLAG_PROFILE = 1
 FLAG_NO_TRIM2 = 2
 FLAG_NO_TAIL = 4
 FLAG_NO_REFINE = 8
+FLAG_NO_COOP = 16
 GEN_GNM, GEN_RMAT, GEN_LATTICE, GEN_PLANTED = 0, 1, 2, 3
 MLP_PARAMS = 1313
 UNDECIDED = 0xFFFFFFFF
 
 # every symbol include/kspn/scc.h declares
 EXPORTS = (
-    "kspn_opts_default", "kspn_strerror", "kspn_ctx_create", "kspn_ctx_destroy", "kspn_ctx_reserve",
+    "kspn_opts_default", "kspn_strerror", "kspn_last_error", "kspn_ctx_create", "kspn_ctx_destroy", "kspn_ctx_reserve",
     "kspn_scc", "kspn_scc_host", "kspn_transpose", "kspn_canonicalize", "kspn_subgraph", "kspn_tarjan",
     "kspn_generate", "kspn_build_csr", "kspn_free", "kspn_planted_labels", "kspn_pivot_features",
     "kspn_mlp_forward", "kspn_mlp_grad", "kspn_mlp_adam", "kspn_scc_sizes", "kspn_scc_batch", "kspn_batch_plan",
@@ -108,6 +109,9 @@ class Opts(C.Structure):
         ("pivot_log", C.c_void_p),
         ("pivot_log_cap", C.c_uint32),
         ("seed", C.c_uint32),
+        ("coop_edges", C.c_uint32),
+        ("coop_degree", C.c_uint32),
+        ("coop_workgroups", C.c_uint32),
     ]
 
 
@@ -126,7 +130,7 @@ class Stats(C.Structure):
         ("color_rounds", C.c_uint32), ("color_iters", C.c_uint32),
         ("residual_n", C.c_uint32),
         ("fwbw_rounds_done", C.c_uint32), ("partitions_max", C.c_uint32),
-        ("pivot_log_count", C.c_uint32), ("reserved0", C.c_uint32),
+        ("pivot_log_count", C.c_uint32), ("fwbw_levels_coop", C.c_uint32),
         ("residual_m", C.c_uint64),
         ("edges_scanned", C.c_uint64),
         ("bytes_trim", C.c_double), ("bytes_fwbw", C.c_double), ("bytes_color", C.c_double),
@@ -174,6 +178,7 @@ def lib():
     sig = {
         "kspn_opts_default": (None, [P(Opts)]),
         "kspn_strerror": (C.c_char_p, [i32]),
+        "kspn_last_error": (C.c_char_p, []),
         "kspn_ctx_create": (i32, [P(vp), i32]),
         "kspn_ctx_destroy": (i32, [vp]),
         "kspn_ctx_reserve": (i32, [vp, u32, u64]),
@@ -206,6 +211,10 @@ def lib():
 
 def check(rc, where):
     if rc != KSPN_OK:
+        if rc == -3:
+            detail = lib().kspn_last_error()
+            if detail:
+                where = f"{where} [{detail.decode(errors='replace')}]"
         raise KspnError(rc, where)
 
 

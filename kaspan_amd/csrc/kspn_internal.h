@@ -22,10 +22,16 @@ constexpr uint32_t UNDEC = KSPN_UNDECIDED;
     if (_rc != KSPN_OK) return _rc;    \
   } while (0)
 
-#define KSPN_HIP(expr)                        \
-  do {                                        \
-    hipError_t _e = (expr);                   \
-    if (_e != hipSuccess) return KSPN_E_HIP;  \
+// records the failing call for kspn_last_error (thread-local)
+void note_error(const char* what, const char* detail, const char* file, int line);
+
+#define KSPN_HIP(expr)                                                         \
+  do {                                                                         \
+    hipError_t _e = (expr);                                                    \
+    if (_e != hipSuccess) {                                                    \
+      ::kspn::note_error(#expr, hipGetErrorName(_e), __FILE__, __LINE__);      \
+      return KSPN_E_HIP;                                                       \
+    }                                                                          \
   } while (0)
 
 // ---------------------------------------------------------------- bitmaps
@@ -255,6 +261,23 @@ struct Ctx {
   std::vector<hipEvent_t> free_events;
   hipStream_t stream = nullptr;
   int num_cu = 256;
+  // persistent multi-level reachability (cooperative launch)
+  struct CoopCtl* coop = nullptr;  // device control block
+  uint32_t coop_grid = 0;          // co-resident workgroups (0: cooperative launch unavailable)
+};
+
+// Control block of the persistent reachability kernel: per-level values
+// live in rings of three indexed by level % 3 (frontier being expanded,
+// next frontier being filled, the one after is cleared for reuse).
+struct CoopCtl {
+  uint32_t cnt[4];              // frontier sizes
+  uint32_t work[4];             // 64-vertex chunks handed out
+  unsigned long long nsum[4];   // out-degree sums of the frontiers
+  uint32_t maxd[4];             // largest out-degree in the frontiers
+  unsigned long long expanded;  // edges expanded by all levels of the launch
+  unsigned long long verts;     // frontier vertices expanded by all levels of the launch
+  uint32_t bar_count, bar_pad;  // grid barrier: arrivals so far in the launch
+  uint32_t levels, reason, err, pad;
 };
 
 // host helpers implemented in kspn_scc.hip

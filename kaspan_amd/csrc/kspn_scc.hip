@@ -62,6 +62,11 @@ __global__ void k_publish(const uint32_t* cnt, const unsigned long long* cnt64, 
     map[20] = (uint32_t)y;
     map[21] = (uint32_t)(y >> 32);
   }
+  if (t == 35) {
+    uint64_t y = cnt64[2];
+    map[22] = (uint32_t)y;
+    map[23] = (uint32_t)(y >> 32);
+  }
 }
 
 // set bit v of bm
@@ -133,19 +138,28 @@ struct OpArgs {
   uint32_t* labels;
   uint32_t value;                // pivot or rotation
   const uint64_t* noff;          // OP_FW/OP_BW: offsets whose degree the next level expands
-  unsigned long long* nsum;      // OP_FW/OP_BW: sum of those degrees over claimed targets
+  unsigned long long* nsum;      // OP_FW/OP_BW: sum of those degrees over claimed targets; nsum[1]: their maximum
   const uint32_t* part;          // OP_FW/OP_PBW: partition of every vertex (nullptr: one partition)
 };
 
+// reachability claim of edge u -> w: true for the one thread that claims w
 template <int OP>
-__device__ inline void expand_apply(uint32_t u, uint32_t w, const OpArgs& a, BlockQueue& q, uint64_t& nsum) {
-  bool push = false;
+__device__ inline bool reach_claim(uint32_t u, uint32_t w, const OpArgs& a) {
   if constexpr (OP == OP_FW) {
-    push = bm_get(a.live, w) && (!a.part || a.part[w] == a.part[u]) && bm_claim(a.fwv, w);
+    return bm_get(a.live, w) && (!a.part || a.part[w] == a.part[u]) && bm_claim(a.fwv, w);
   } else if constexpr (OP == OP_BW) {
-    push = bm_get(a.fwv, w) && bm_claim(a.bwv, w);
-  } else if constexpr (OP == OP_PBW) {
-    push = bm_get(a.live, w) && a.part[w] == a.part[u] && bm_claim(a.bwv, w);
+    return bm_get(a.fwv, w) && bm_claim(a.bwv, w);
+  } else {
+    return bm_get(a.live, w) && a.part[w] == a.part[u] && bm_claim(a.bwv, w);
+  }
+}
+
+template <int OP>
+__device__ inline void expand_apply(uint32_t u, uint32_t w, const OpArgs& a, BlockQueue& q, uint64_t& nsum,
+                                    uint32_t& nmax) {
+  bool push = false;
+  if constexpr (OP == OP_FW || OP == OP_BW || OP == OP_PBW) {
+    push = reach_claim<OP>(u, w, a);
   } else if constexpr (OP == OP_NEIGH) {
     push = bm_get(a.live, w) && bm_claim(a.inq, w);
   } else if constexpr (OP == OP_COLOR) {
@@ -163,7 +177,11 @@ __device__ inline void expand_apply(uint32_t u, uint32_t w, const OpArgs& a, Blo
     }
   }
   if constexpr (OP == OP_FW || OP == OP_BW || OP == OP_PBW) {
-    if (push) nsum += a.noff[w + 1] - a.noff[w];
+    if (push) {
+      const uint64_t d = a.noff[w + 1] - a.noff[w];
+      nsum += d;
+      nmax = max(nmax, (uint32_t)min(d, (uint64_t)0xFFFFFFFFu));
+    }
   }
   bq_push(q, push, w);
 }
@@ -189,10 +207,15 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
   __shared__ uint32_t s_out[2 * EXP_TILE];
   __shared__ uint32_t s_on, s_ob;
   __shared__ unsigned long long s_nsum;
+  __shared__ uint32_t s_nmax;
   BlockQueue q{s_out, &s_on, &s_ob, 2 * EXP_TILE};
-  if (threadIdx.x == 0) s_nsum = 0;
+  if (threadIdx.x == 0) {
+    s_nsum = 0;
+    s_nmax = 0;
+  }
   bq_init(q);
   uint64_t nsum = 0;
+  uint32_t nmax = 0;
   const uint32_t f = *fcount;
   if (f == 0) return;
   const uint64_t total = S[f];
@@ -234,7 +257,7 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
         }
         uint32_t u = s_v[lo];
         uint32_t x = adj[s_beg[lo] + (e - s_S[lo])];
-        expand_apply<OP>(u, x, a, q, nsum);
+        expand_apply<OP>(u, x, a, q, nsum, nmax);
       }
     } else {
       // window larger than LDS (many zero-degree frontier entries): search the scan in global memory
@@ -247,7 +270,7 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
         }
         uint32_t u = F[lo];
         uint32_t x = adj[off[u] + (e - S[lo])];
-        expand_apply<OP>(u, x, a, q, nsum);
+        expand_apply<OP>(u, x, a, q, nsum, nmax);
       }
     }
     bq_maybe_flush(q, next, next_cnt, EXP_TILE);  // a tile appends at most EXP_TILE entries
@@ -255,8 +278,303 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
   bq_flush(q, next, next_cnt);
   if constexpr (OP == OP_FW || OP == OP_BW || OP == OP_PBW) {
     if (nsum) atomicAdd(&s_nsum, (unsigned long long)nsum);
+    if (nmax) atomicMax(&s_nmax, nmax);
     __syncthreads();
     if (threadIdx.x == 0 && s_nsum) atomicAdd(a.nsum, s_nsum);
+    if (threadIdx.x == 0 && s_nmax) atomicMax(&a.nsum[1], (unsigned long long)s_nmax);
+  }
+}
+
+// =====================================================================
+// persistent multi-level reachability
+// =====================================================================
+// One persistent launch runs push levels back to back with a grid barrier
+// between them, so a level costs a barrier instead of three launches and a
+// host round trip. Wavefronts take 64-vertex chunks of the frontier from a
+// counter (dynamic balance), map the chunk's edges to their sources with a
+// wavefront scan of the degrees staged in LDS, claim targets with the same
+// bitmap atomics as k_expand and stage claimed vertices in a
+// wavefront-private LDS queue (one global atomic per flush). Frontiers
+// whose degree sum exceeds e_max or that hold a vertex of out-degree above
+// d_max, and the pull switch (Beamer: degree sum * alpha > unexpanded
+// edges), end the launch: the host continues with k_expand / k_pull.
+
+constexpr int COOP_WG = 1024;
+constexpr int COOP_WAVES = COOP_WG / WAVE;
+constexpr uint32_t COOP_QCAP = 256;           // staged entries per wavefront
+constexpr uint32_t COOP_SPIN_LIMIT = 1u << 21;  // barrier polls before giving up (about a second)
+enum CoopReason : uint32_t { COOP_EMPTY = 1, COOP_PULL = 2, COOP_BIG = 3, COOP_ERR = 4 };
+
+// relaxed device-scope loads: coherent across XCDs without a cache
+// invalidation (ordering comes from the one fence pair per barrier)
+__device__ inline uint32_t ld_relaxed(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned long long ld_relaxed64(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The next level's frontier size, degree sum, maximum degree and the error
+// flag, read by thread 0 and shared with the workgroup.
+struct CoopLevel {
+  uint32_t f, fmax, err;
+  unsigned long long fe;
+};
+
+__device__ inline void coop_read_level(const CoopCtl* ctl, uint32_t slot, CoopLevel* lv) {
+  lv->f = ld_relaxed(&ctl->cnt[slot]);
+  lv->fe = ld_relaxed64(&ctl->nsum[slot]);
+  lv->fmax = ld_relaxed(&ctl->maxd[slot]);
+  lv->err = ld_relaxed(&ctl->err);
+}
+
+// Barrier over the workgroups of a persistent launch (resident together on
+// an otherwise idle device). Everything one workgroup writes and another
+// reads inside the launch goes through device-scope atomics (counters,
+// bitmap claims) or device-scope relaxed stores and loads (the frontier
+// queues), which are coherent across XCDs without cache maintenance; the
+// workgroup barrier before arrival waits for the workgroup's stores to
+// complete. So the barrier needs no L2 writeback or invalidation, and the
+// read-only graph arrays stay cached from level to level. The arrival
+// counter only grows: barrier k of a launch waits for k * workgroups
+// arrivals. Thread 0 arrives, polls, then reads the next level's values
+// into *lv. Returns false if the others did not arrive within the spin limit
+// or one of them gave up: a launch that is not fully resident ends with an
+// error instead of waiting forever.
+__device__ bool coop_barrier(CoopCtl* ctl, uint32_t k, uint32_t next_slot, CoopLevel* lv) {
+  __shared__ uint32_t s_ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t ok = 1;
+    const uint32_t target = k * gridDim.x;
+    const uint32_t arrived = __hip_atomic_fetch_add(&ctl->bar_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived + 1 < target) {
+      uint32_t spins = 0;
+      while (ld_relaxed(&ctl->bar_count) < target) {
+        if (++spins > COOP_SPIN_LIMIT || ld_relaxed(&ctl->err)) {
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    coop_read_level(ctl, next_slot, lv);
+    s_ok = ok;
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+__device__ inline uint64_t wave_incl_scan64(uint64_t v) {
+  const uint32_t lane = lane_id();
+  for (uint32_t o = 1; o < WAVE; o <<= 1) {
+    const uint64_t t = __shfl_up(v, o);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// a full wavefront queue goes out on its own (one atomic per flush)
+__device__ inline void coop_flush(uint32_t* sq, uint32_t& qn, uint32_t* q, uint32_t* qcount) {
+  const uint32_t lane = lane_id();
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(qcount, qn);
+  base = __shfl(base, 0);
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t k = lane; k < qn; k += WAVE) __hip_atomic_store(&q[base + k], sq[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __builtin_amdgcn_wave_barrier();
+  qn = 0;
+}
+
+template <int OP>
+__global__ __launch_bounds__(COOP_WG) void k_reach_coop(uint32_t n, uint32_t* F0, uint32_t* F1, const uint64_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ adj, OpArgs a, CoopCtl* ctl,
+                                                        double m_rest, float alpha, unsigned long long e_max,
+                                                        uint32_t d_max) {
+  __shared__ uint64_t s_ex[COOP_WAVES][WAVE];
+  __shared__ uint64_t s_beg[COOP_WAVES][WAVE];
+  __shared__ uint32_t s_v[COOP_WAVES][WAVE];
+  __shared__ uint32_t s_q[COOP_WAVES][COOP_QCAP];
+  __shared__ uint32_t s_qn[COOP_WAVES];
+  __shared__ unsigned long long s_sum[COOP_WAVES];
+  __shared__ uint32_t s_max[COOP_WAVES];
+  __shared__ uint32_t s_base;
+  __shared__ CoopLevel s_lv;
+  const uint32_t wid = threadIdx.x / WAVE, lane = lane_id();
+  const uint64_t below = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+  const uint32_t waves = gridDim.x * COOP_WAVES;          // wavefronts of the launch
+  const uint32_t gw = blockIdx.x * COOP_WAVES + wid;      // this wavefront's index
+  unsigned long long expanded = 0, verts = 0;
+  uint32_t L = 0;
+  if (threadIdx.x == 0) coop_read_level(ctl, 0, &s_lv);  // written by the init kernel before the launch
+  __syncthreads();
+  for (;;) {
+    const uint32_t s0 = L % 3, s1 = (L + 1) % 3, s2 = (L + 2) % 3;
+    const uint32_t f = s_lv.f, fmax = s_lv.fmax;
+    const unsigned long long fe = s_lv.fe;
+    if (s_lv.err) return;  // a workgroup timed out at a barrier
+    uint32_t reason = 0;   // identical in every workgroup: read after the same barrier
+    if (f == 0) reason = COOP_EMPTY;
+    else if (f > 64 && (double)fe * alpha > m_rest) reason = COOP_PULL;
+    else if (fe > e_max || fmax > d_max) reason = COOP_BIG;
+    if (reason) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->levels = L;
+        ctl->reason = reason;
+        ctl->expanded = expanded;
+        ctl->verts = verts;
+      }
+      return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // the ring slot after next: last read before the previous barrier
+      __hip_atomic_store(&ctl->cnt[s2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl->work[s2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl->nsum[s2], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl->maxd[s2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    uint32_t* F = (L & 1) ? F1 : F0;
+    uint32_t* N = (L & 1) ? F0 : F1;
+    uint64_t my_sum = 0;
+    uint32_t my_max = 0, qn = 0;
+    // chunk gw first (no atomic); beyond the first pass chunks come from a counter
+    const uint64_t nchunks = ((uint64_t)f + WAVE - 1) / WAVE;
+    uint64_t chunk = gw;
+    while (chunk < nchunks) {
+      const uint64_t i = chunk * WAVE + lane;
+      uint32_t v = 0;
+      uint64_t b = 0, d = 0;
+      if (i < f) {
+        v = __hip_atomic_load(&F[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v < n) {  // always true; an id out of range would be skipped, not used as an index
+          b = off[v];
+          d = off[v + 1] - b;
+        }
+      }
+      const uint64_t incl = wave_incl_scan64(d);
+      const uint64_t total = __shfl(incl, WAVE - 1);
+      s_ex[wid][lane] = incl - d;
+      s_beg[wid][lane] = b;
+      s_v[wid][lane] = v;
+      __builtin_amdgcn_wave_barrier();
+      uint32_t lo = 0;
+      for (uint64_t e0 = 0; e0 < total; e0 += WAVE) {
+        const uint64_t e = e0 + lane;
+        bool push = false;
+        uint32_t w = 0;
+        if (e < total) {
+          uint32_t hi = WAVE;  // last source whose edges start at or before e
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_ex[wid][mid] <= e) lo = mid; else hi = mid;
+          }
+          w = adj[s_beg[wid][lo] + (e - s_ex[wid][lo])];
+          push = reach_claim<OP>(s_v[wid][lo], w, a);
+          if (push) {
+            const uint64_t dd = a.noff[w + 1] - a.noff[w];
+            my_sum += dd;
+            my_max = max(my_max, (uint32_t)min(dd, (uint64_t)0xFFFFFFFFu));
+          }
+        }
+        const uint64_t mask = __ballot(push);
+        if (mask) {
+          if (push) s_q[wid][qn + __popcll(mask & below)] = w;
+          qn += (uint32_t)__popcll(mask);
+          if (qn > COOP_QCAP - WAVE) coop_flush(s_q[wid], qn, N, &ctl->cnt[s1]);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (nchunks <= waves) break;
+      uint32_t k = 0;
+      if (lane == 0) k = atomicAdd(&ctl->work[s0], 1u);
+      chunk = (uint64_t)waves + __shfl(k, 0);
+    }
+    // the workgroup's staged entries, degree sum and maximum: one atomic each
+    my_sum = wave_sum64(my_sum);
+    for (int o = 32; o > 0; o >>= 1) my_max = max(my_max, __shfl_down(my_max, o));
+    if (lane == 0) {
+      s_sum[wid] = my_sum;
+      s_max[wid] = my_max;
+      s_qn[wid] = qn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long bs = 0;
+      uint32_t bm = 0, bq = 0;
+      for (int k = 0; k < COOP_WAVES; ++k) {
+        bs += s_sum[k];
+        bm = max(bm, s_max[k]);
+        const uint32_t q = s_qn[k];
+        s_qn[k] = bq;  // exclusive prefix
+        bq += q;
+      }
+      if (bs) atomicAdd(&ctl->nsum[s1], bs);
+      if (bm) atomicMax(&ctl->maxd[s1], bm);
+      s_base = bq ? atomicAdd(&ctl->cnt[s1], bq) : 0u;
+    }
+    __syncthreads();
+    {
+      const uint32_t dst = s_base + s_qn[wid];
+      for (uint32_t k = lane; k < qn; k += WAVE)
+        __hip_atomic_store(&N[dst + k], s_q[wid][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    expanded += fe;
+    verts += f;
+    m_rest -= (double)fe;
+    ++L;
+    if (!coop_barrier(ctl, L, L % 3, &s_lv)) {
+      if (threadIdx.x == 0) {
+        atomicOr(&ctl->err, 1u);
+        if (blockIdx.x == 0) {
+          ctl->levels = L;
+          ctl->reason = COOP_ERR;
+          ctl->expanded = expanded;
+          ctl->verts = verts;
+        }
+      }
+      return;
+    }
+  }
+}
+
+// control block for a launch: level 0 is the host's frontier (count at
+// *fcount, degree sum and maximum known)
+__global__ void k_coop_init(CoopCtl* ctl, const uint32_t* fcount, unsigned long long edges, uint32_t maxd) {
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 4; ++k) {
+      ctl->cnt[k] = 0;
+      ctl->work[k] = 0;
+      ctl->nsum[k] = 0;
+      ctl->maxd[k] = 0;
+    }
+    ctl->cnt[0] = *fcount;
+    ctl->nsum[0] = edges;
+    ctl->maxd[0] = maxd;
+    ctl->expanded = 0;
+    ctl->verts = 0;
+    ctl->bar_count = 0;
+    ctl->levels = ctl->reason = ctl->err = 0;
+  }
+}
+
+// after a launch: the last frontier's size into the host's counter slot of
+// the buffer that holds it; results into the mapped host counters
+__global__ void k_coop_publish(const CoopCtl* ctl, uint32_t* cnt_even, uint32_t* cnt_odd, uint32_t* map) {
+  if (threadIdx.x == 0) {
+    const uint32_t L = ctl->levels;
+    const uint32_t f = ctl->cnt[L % 3];
+    *((L & 1) ? cnt_odd : cnt_even) = f;
+    const unsigned long long fe = ctl->nsum[L % 3], ex = ctl->expanded;
+    map[32] = L;
+    map[33] = ctl->reason;
+    map[34] = ctl->err;
+    map[35] = f;
+    map[36] = (uint32_t)fe;
+    map[37] = (uint32_t)(fe >> 32);
+    map[38] = ctl->maxd[L % 3];
+    map[39] = (uint32_t)ex;
+    map[40] = (uint32_t)(ex >> 32);
+    map[41] = (uint32_t)ctl->verts;
+    map[42] = (uint32_t)(ctl->verts >> 32);
   }
 }
 
@@ -477,6 +795,7 @@ __global__ void k_level_reset(uint32_t* cnt_slot, unsigned long long* cnt64) {
     *cnt_slot = 0;
     cnt64[0] = 0;
     cnt64[1] = 0;
+    cnt64[2] = 0;
   }
 }
 
@@ -1318,15 +1637,27 @@ int dalloc(T** p, uint64_t count) {
   return KSPN_OK;
 }
 
+}  // namespace
+
+thread_local char g_last_error[256] = "";
+
+void note_error(const char* what, const char* detail, const char* file, int line) {
+  const char* base = std::strrchr(file, '/');
+  std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%s:%d)", what, detail, base ? base + 1 : file, line);
+}
+
+namespace {
+
 enum KernId : int {
   K_TRIM_FIRST = 0, K_PIVOT, K_BITMAP, K_FRONTIER_DEG, K_SCAN, K_EXPAND_FW, K_EXPAND_BW, K_PULL_FW, K_PULL_BW,
   K_TRIM_SWEEP, K_TRIM2, K_EXPAND_NEIGH, K_COLOR_INIT, K_COLOR_PULL, K_EXPAND_COLOR, K_COLOR_ROOTS,
-  K_EXPAND_COLOR_BW, K_LIVE_LIST, K_CANON, K_RESIDUAL, K_TARJAN, K_LABEL, K_TAIL, K_COUNT
+  K_EXPAND_COLOR_BW, K_LIVE_LIST, K_CANON, K_RESIDUAL, K_TARJAN, K_LABEL, K_TAIL, K_COOP, K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"trim_first", "pivot_reduce", "bitmap_ops", "frontier_deg", "scan", "expand_fw",
                                    "expand_bw", "pull_fw", "pull_bw", "trim_sweep", "trim2", "expand_neigh",
                                    "color_init", "color_pull", "expand_color", "color_roots", "expand_color_bw",
-                                   "live_list", "canonicalize", "residual_build", "tarjan", "label_scc", "tail_solve"};
+                                   "live_list", "canonicalize", "residual_build", "tarjan", "label_scc", "tail_solve",
+                                   "reach_coop"};
 
 struct Prof {
   Ctx* c;
@@ -1449,6 +1780,7 @@ int publish(Ctx* c, hipStream_t s, const uint64_t* extra) {
   c->h_u64[0] = (uint64_t)c->h_map[16] | ((uint64_t)c->h_map[17] << 32);
   c->h_u64[1] = (uint64_t)c->h_map[18] | ((uint64_t)c->h_map[19] << 32);
   c->h_u64[2] = (uint64_t)c->h_map[20] | ((uint64_t)c->h_map[21] << 32);
+  c->h_u64[3] = (uint64_t)c->h_map[22] | ((uint64_t)c->h_map[23] << 32);
   return KSPN_OK;
 }
 
@@ -1513,6 +1845,9 @@ struct Run {
   bool pid_active = false;  // pid partitions the live vertices (after multi-pivot FW-BW)
   uint32_t np0 = 1;         // number of preset partitions
   const uint32_t* vtx_off = nullptr;  // batch: graph i owns [vtx_off[i], vtx_off[i+1]); labels are made local
+  unsigned long long coop_edges = 0;  // frontiers the persistent kernel takes (degree sum, largest degree)
+  uint32_t coop_deg = 0;
+  uint32_t coop_wg = 0;               // its workgroups
 };
 
 int select_pivot_live(Run& R, uint32_t* pivot);
@@ -1567,6 +1902,54 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b) {
 // Single partition: forward claims live vertices, backward claims
 // forward-reached ones. With partitions (pid != nullptr) both directions
 // claim live vertices of the source's partition.
+// frontiers the persistent kernel takes by default: degree sum and largest degree bounds
+constexpr uint64_t kCoopEdges = 1ull << 20;
+constexpr uint32_t kCoopDeg = 2048;
+constexpr uint32_t kCoopWorkgroups = 128;
+
+struct CoopOut {
+  uint32_t levels, reason, f, maxd;
+  uint64_t edges, expanded, verts;
+};
+
+// push levels of one reachability in one persistent launch, starting from
+// the frontier F (count at cnt[fc]) whose degree sum and maximum are known
+int run_coop(Run& R, int op, const OpArgs& a, const uint64_t* off, const uint32_t* adj, uint32_t* F, uint32_t* N,
+             int fc, int nc, uint64_t edges, uint32_t maxd, double m_rest, double alpha, CoopOut* out) {
+  Ctx* c = R.c;
+  k_coop_init<<<1, 64, 0, R.s>>>(c->coop, &c->cnt[fc], edges, maxd);
+  KSPN_HIP(hipGetLastError());
+  const float al = (float)alpha;
+  {
+    Prof p(c, R.s, K_COOP, 0.0);
+    const dim3 grid(R.coop_wg), block(COOP_WG);
+    if (op == OP_FW)
+      k_reach_coop<OP_FW><<<grid, block, 0, R.s>>>(R.n, F, N, off, adj, a, c->coop, m_rest, al, R.coop_edges, R.coop_deg);
+    else if (op == OP_BW)
+      k_reach_coop<OP_BW><<<grid, block, 0, R.s>>>(R.n, F, N, off, adj, a, c->coop, m_rest, al, R.coop_edges, R.coop_deg);
+    else
+      k_reach_coop<OP_PBW><<<grid, block, 0, R.s>>>(R.n, F, N, off, adj, a, c->coop, m_rest, al, R.coop_edges, R.coop_deg);
+    KSPN_HIP(hipGetLastError());
+  }
+  k_coop_publish<<<1, 64, 0, R.s>>>(c->coop, &c->cnt[fc], &c->cnt[nc], c->d_map);
+  KSPN_HIP(hipGetLastError());
+  KSPN_HIP(hipStreamSynchronize(R.s));
+  const volatile uint32_t* m = c->h_map;
+  out->levels = m[32];
+  out->reason = m[33];
+  out->f = m[35];
+  out->edges = (uint64_t)m[36] | ((uint64_t)m[37] << 32);
+  out->maxd = m[38];
+  out->expanded = (uint64_t)m[39] | ((uint64_t)m[40] << 32);
+  out->verts = (uint64_t)m[41] | ((uint64_t)m[42] << 32);
+  if (m[34] || out->reason == COOP_ERR) {
+    note_error("k_reach_coop", "grid barrier timed out (workgroups not resident together)", __FILE__, __LINE__);
+    return KSPN_E_HIP;
+  }
+  if (c->profile && !c->pending.empty()) c->pending.back().bytes = 20.0 * out->verts + 4.125 * (double)out->expanded;
+  return KSPN_OK;
+}
+
 int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels_push, uint32_t* fw_levels_pull) {
   Ctx* c = R.c;
   const kspn_graph* g = R.g;
@@ -1587,10 +1970,32 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   const double alpha = R.o.alpha > 0 ? R.o.alpha : 14.0;
   const double beta = R.o.beta > 0 ? R.o.beta : 24.0;
   double m_rest = (double)g->m;  // edges not yet expanded or reached
+  uint32_t maxd = 0xFFFFFFFFu;   // largest out-degree in the frontier (unknown)
+  const bool coop = R.coop_wg > 0 && !(R.o.flags & KSPN_FLAG_NO_COOP);
+  const int op = dir == 0 ? OP_FW : (pid ? OP_PBW : OP_BW);
   while (f > 0) {
     // direction: Beamer's rule on the frontier's edge count, which the
     // previous level summed on the device for the vertices it claimed
     if (!pull && known && (double)edges * alpha > m_rest && f > 64) pull = true;
+    if (!pull && known && coop && edges <= R.coop_edges && maxd <= R.coop_deg) {
+      // the following push levels in one persistent launch
+      CoopOut co{};
+      KSPN_TRY(run_coop(R, op, a, off, adj, F, N, fc, nc, edges, maxd, m_rest, alpha, &co));
+      if (co.levels & 1) {
+        std::swap(F, N);
+        std::swap(fc, nc);
+      }
+      f = co.f;
+      edges = co.edges;
+      maxd = co.maxd;
+      m_rest -= (double)co.expanded;
+      if (R.st) {
+        R.st->edges_scanned += co.expanded;
+        R.st->fwbw_levels_coop += co.levels;
+      }
+      if (co.levels > 0) continue;
+      // nothing expanded (not expected: the bounds were checked): one level on the host
+    }
     k_level_reset<<<1, 64, 0, R.s>>>(&c->cnt[nc], c->cnt64);
     if (!pull) {
       {
@@ -1638,6 +2043,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
     }
     uint32_t nf = c->h_cnt[nc];
     edges = c->h_u64[2];
+    maxd = pull ? 0xFFFFFFFFu : (uint32_t)std::min<uint64_t>(c->h_u64[3], 0xFFFFFFFFull);
     known = true;
     if (pull) m_rest -= (double)edges;
     if (pull && nf > 0 && (double)nf * beta < (double)R.live_n) {
@@ -2315,6 +2721,8 @@ void kspn_opts_default(kspn_opts* o) {
   o->beta = 24.0f;
 }
 
+const char* kspn_last_error(void) { return kspn::g_last_error; }
+
 const char* kspn_strerror(int code) {
   switch (code) {
     case KSPN_OK: return "ok";
@@ -2358,6 +2766,29 @@ int kspn_ctx_create(kspn_ctx** out, int device) {
   std::memset(c->h_u64, 0, sizeof(uint64_t) * 4);
   c->kt.resize(K_COUNT);
   for (int i = 0; i < K_COUNT; ++i) c->kt[i].name = kKernNames[i];
+  // persistent reachability: one workgroup of COOP_WG threads per CU, all
+  // resident at once (the kernel's barrier gives up rather than wait forever)
+  {
+    int per_cu = 1 << 30;
+    for (const void* fn : {reinterpret_cast<const void*>(&k_reach_coop<OP_FW>),
+                           reinterpret_cast<const void*>(&k_reach_coop<OP_BW>),
+                           reinterpret_cast<const void*>(&k_reach_coop<OP_PBW>)}) {
+      int b = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, COOP_WG, 0) != hipSuccess) b = 0;
+      per_cu = std::min(per_cu, b);
+    }
+    if (per_cu >= 1) {
+      if (hipMalloc((void**)&c->coop, sizeof(CoopCtl)) != hipSuccess) {
+        kspn_ctx_destroy(k);
+        return KSPN_E_OOM;
+      }
+      if (hipMemset(c->coop, 0, sizeof(CoopCtl)) != hipSuccess) {
+        kspn_ctx_destroy(k);
+        return KSPN_E_HIP;
+      }
+      c->coop_grid = (uint32_t)c->num_cu;
+    }
+  }
   *out = k;
   return KSPN_OK;
 }
@@ -2368,7 +2799,7 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   (void)hipSetDevice(c->device);
   void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
                   c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf, c->pid, c->pbest,
-                  c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat};
+                  c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
@@ -2417,6 +2848,9 @@ int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* o
   if (c->profile)
     for (auto& t : c->kt) { t.launches = 0; t.ms = 0; t.bytes = 0; }
   Run R{c, g, labels, o, stats, s, g->n, bm_words(g->n)};
+  R.coop_edges = o.coop_edges ? o.coop_edges : kCoopEdges;
+  R.coop_deg = o.coop_degree ? o.coop_degree : kCoopDeg;
+  R.coop_wg = std::min<uint32_t>(o.coop_workgroups ? o.coop_workgroups : kCoopWorkgroups, c->coop_grid);
   if (vtx_off) {
     // graph i is preset partition i of the multi-pivot FW-BW rounds
     if (!c->pid) KSPN_TRY(dalloc(&c->pid, (uint64_t)c->cap_n + 64));

@@ -29,9 +29,12 @@ METHODS = [
     dict(method=0, fwbw_rounds=64, flags=4),   # partitioned multi-pivot FW-BW, then colour rounds
     dict(method=0, fwbw_rounds=3, alpha=1e9, beta=1.0),  # partitioned, pull levels
     dict(method=0, flags=4 | 8),               # the reference's colour rounds: no tail, no refinement
+    dict(method=0, flags=16),                  # every FW/BW level as separate launches
+    dict(method=0, fwbw_rounds=8, flags=16),   # partitioned rounds, separate launches
 ]
 MIDS = ["kaspan", "trim_ex", "trim_ex_no_trim2", "fwbw4", "residual300", "kaspan_residual_all", "pull", "push",
-        "kaspan_no_tail", "trim_ex_no_tail", "tail_all", "multi_pivot64", "multi_pivot3_pull", "kaspan_plain_colour"]
+        "kaspan_no_tail", "trim_ex_no_tail", "tail_all", "multi_pivot64", "multi_pivot3_pull", "kaspan_plain_colour",
+        "no_coop", "multi_pivot8_no_coop"]
 
 
 def host_graph(O, g):
@@ -157,6 +160,20 @@ def test_random_graphs(ka, O, kind, opts):
     assert np.array_equal(lab, O.tarjan(hg))
     total = st.decided_trim + st.decided_trim2 + st.decided_fwbw + st.decided_color + st.decided_residual
     assert total == g.n
+
+
+def test_persistent_levels(ka, O):
+    """long-diameter lattice: the persistent kernel runs the push levels; the
+    same labels and the same edge count as separate launches per level"""
+    g = ka.generate("lattice", 31, side=40, p_bidir=0.05)
+    ref = O.tarjan(host_graph(O, g))
+    for rounds in (1, 16):
+        lab, st = gpu_labels(ka, g, fwbw_rounds=rounds)
+        lab2, st2 = gpu_labels(ka, g, fwbw_rounds=rounds, flags=16)
+        assert np.array_equal(lab, ref) and np.array_equal(lab2, ref)
+        assert st.fwbw_levels_coop > 0 and st2.fwbw_levels_coop == 0
+        assert st.fwbw_levels_coop + st.fwbw_levels_push == st2.fwbw_levels_push
+        assert st.decided_fwbw == st2.decided_fwbw
 
 
 def test_c1_gnm_full(ka, O):
