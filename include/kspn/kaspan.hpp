@@ -6,6 +6,7 @@
 //   kaspan::scc_trim_ex_residual(graph, ...)   scc/variant/scc_trim_ex_residual.hpp:17-128
 //   kaspan::backward_complement_graph(g)       scc/backward_complement.hpp:22-68
 //   kaspan::normalise_scc_id(n, scc_id)        test/normalise_scc.hpp:10-25
+//   kaspan::scc_batch(batch, scc_id)           one scc() per graph of a packed batch
 //
 // The graph is a device-resident bidi_graph_view (graph/bidi_graph.hpp:250):
 // n, m, fw.head/fw.csr, bw.head/bw.csr with uint64 heads and uint32 ids
@@ -92,6 +93,14 @@ inline void scc_trim_ex_residual(bidi_graph_view graph, vertex_t* scc_id, kspn_s
   o.method = KSPN_METHOD_TRIM_EX;
   o.residual_threshold = 0xFFFFFFFFu;  // one rank: everything after the trim goes to Tarjan
   scc(graph, scc_id, o, stats, stream);
+}
+
+// every graph of a packed batch (kspn_batch: union graph, batch-global ids);
+// scc_id[vtx_off[i] + v] receives v's canonical id inside graph i
+inline void scc_batch(const kspn_batch& batch, vertex_t* scc_id, const kspn_opts& opts = default_opts(),
+                      kspn_stats* stats = nullptr, void* stream = nullptr,
+                      context& ctx = context::thread_default()) {
+  check(kspn_scc_batch(ctx.get(), &batch, scc_id, &opts, stats, stream), "kaspan::scc_batch");
 }
 
 // fills g.bw from g.fw (both device)
