@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--coop-workgroups", type=int, default=0, help="persistent-kernel workgroups (0: library default)")
     ap.add_argument("--alpha", type=float, default=14.0, help="push->pull switch (Beamer alpha)")
     ap.add_argument("--beta", type=float, default=24.0, help="pull->push switch (Beamer beta)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune alpha/beta/persistent-kernel/round/tail/method options on this workload first")
     ap.add_argument("--ab", action="store_true", help="also time the other pivot rule; report both and the speedup")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -142,6 +144,21 @@ def main():
     if args.pivot == "learned":
         opts.update(pivot=1, mlp_params=mlp)
 
+    tuned = None
+    if args.autotune:
+        from kaspan_amd import autotune
+
+        space = dict(autotune.SPACE)
+        if args.pivot == "learned":
+            space.pop("method")
+        t0 = time.time()
+        best, best_ms, trials = autotune.tune(batch if batch is not None else g, space=space,
+                                              start={**autotune.DEFAULTS, **{k: v for k, v in opts.items()
+                                                                             if k in autotune.DEFAULTS}},
+                                              log=lambda m: print(m, file=sys.stderr, flush=True))
+        opts.update(best)
+        tuned = {"opts": best, "ms": best_ms, "trials": len(trials), "seconds": time.time() - t0}
+
     def timed(o):
         for _ in range(args.warmup):
             solve(labels, **o)
@@ -164,8 +181,7 @@ def main():
         import numpy as np
 
         first = labels.clone()
-        other = dict(method=method, fwbw_rounds=args.fwbw_rounds, coop_edges=args.coop_edges,
-                coop_workgroups=args.coop_workgroups, alpha=args.alpha, beta=args.beta)
+        other = {k: v for k, v in opts.items() if k not in ("pivot", "mlp_params")}
         if args.pivot == "plain":
             other.update(pivot=1, mlp_params=mlp)
         ms_other, st_other = timed(other)
@@ -288,9 +304,12 @@ def main():
         "vs_baseline": None,
         "dtype": "uint32",
         "data": "synthetic",
-        "config": {"workload": desc, "name": args.config, "n": g.n, "m": g.m, "method": ["kaspan", "trim_ex"][method],
-                   "pivot": args.pivot, "fwbw_rounds": args.fwbw_rounds, "seed": seed},
+        "config": {"workload": desc, "name": args.config, "n": g.n, "m": g.m,
+                   "method": ["kaspan", "trim_ex"][opts.get("method", method)], "pivot": args.pivot,
+                   "fwbw_rounds": opts.get("fwbw_rounds", args.fwbw_rounds), "seed": seed,
+                   "options": {k: v for k, v in opts.items() if k != "mlp_params"}},
         "ab": ab,
+        "autotune": tuned,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "phases_ms": {k: getattr(stats, k) for k in ("ms_trim", "ms_fwbw", "ms_color", "ms_residual", "ms_canon")},

@@ -686,47 +686,96 @@ __global__ __launch_bounds__(WG) void k_pivot_reduce(const uint64_t* part, uint3
 // written as a bitmap (each wavefront owns its two words, no atomics); per
 // workgroup partials: claimed count, edges scanned, next-level degree sum
 // (noff) of the claimed vertices.
+constexpr int PULL_ILP = 4;  // vertices in flight per thread
+
 template <int DIR>
 __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restrict__ cand, uint32_t* vis,
                                              const uint64_t* __restrict__ roff, const uint32_t* __restrict__ radj,
                                              const uint64_t* __restrict__ noff, uint32_t* __restrict__ fnew,
                                              uint64_t* __restrict__ part, const uint32_t* __restrict__ pid) {
+  // A candidate costs a chain of dependent loads (bitmap word, row offsets,
+  // in-neighbour, its visited bit, its partition). Each thread carries
+  // PULL_ILP vertices (one per 256-vertex tile of its workgroup's span)
+  // through every step together, so their loads overlap.
   __shared__ uint64_t s_red[WG / WAVE];
   uint64_t claimed = 0, edges = 0, nd = 0;
-  const uint64_t n_pad = ((uint64_t)n + WG - 1) / WG * WG;
+  constexpr uint64_t SPAN = (uint64_t)WG * PULL_ILP;
+  const uint64_t n_pad = ((uint64_t)n + SPAN - 1) / SPAN * SPAN;
   const uint32_t lane = lane_id();
-  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < n_pad; base += (uint64_t)gridDim.x * WG) {
-    const uint64_t v64 = base + threadIdx.x;
-    const uint64_t word = v64 >> 5;  // this lane's bitmap word: owned by this wavefront
-    const uint32_t vis_word = (v64 < n) ? vis[word] : 0xFFFFFFFFu;
-    bool hit = false;
-    if (v64 < n) {
-      const uint32_t v = (uint32_t)v64;
+  for (uint64_t base = (uint64_t)blockIdx.x * SPAN; base < n_pad; base += (uint64_t)gridDim.x * SPAN) {
+    uint32_t vw[PULL_ILP], pv[PULL_ILP];
+    uint64_t it[PULL_ILP], end[PULL_ILP];
+    bool act[PULL_ILP], hit[PULL_ILP];
+#pragma unroll
+    for (int k = 0; k < PULL_ILP; ++k) {
+      const uint64_t v = base + (uint64_t)k * WG + threadIdx.x;
+      const uint64_t word = v >> 5;
+      vw[k] = (v < n) ? vis[word] : 0xFFFFFFFFu;
+      const uint32_t cw = (v < n) ? cand[word] : 0u;
       const uint32_t bit = 1u << (v & 31);
-      if ((cand[word] & bit) && !(vis_word & bit)) {
-        const uint32_t pv = pid ? pid[v] : 0u;
-        for (uint64_t it = roff[v], end = roff[v + 1]; it < end; ++it) {
-          ++edges;
-          const uint32_t u = radj[it];
-          if (bm_get(vis, u) && (!pid || pid[u] == pv)) { hit = true; break; }
-        }
-        if (hit) {
-          ++claimed;
-          nd += noff[v + 1] - noff[v];
+      act[k] = (cw & bit) && !(vw[k] & bit);
+      hit[k] = false;
+      it[k] = end[k] = 0;
+      pv[k] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < PULL_ILP; ++k) {
+      if (act[k]) {
+        const uint64_t v = base + (uint64_t)k * WG + threadIdx.x;
+        it[k] = roff[v];
+        end[k] = roff[v + 1];
+        if (pid) pv[k] = pid[v];
+      }
+    }
+    // scan the rows together, one in-neighbour of every unfinished row per step
+    for (;;) {
+      bool more = false;
+      uint32_t u[PULL_ILP];
+#pragma unroll
+      for (int k = 0; k < PULL_ILP; ++k) {
+        act[k] = act[k] && it[k] < end[k];
+        more |= act[k];
+        u[k] = act[k] ? radj[it[k]] : 0u;
+      }
+      if (!more) break;
+      uint32_t uw[PULL_ILP];
+#pragma unroll
+      for (int k = 0; k < PULL_ILP; ++k) uw[k] = act[k] ? vis[u[k] >> 5] : 0u;
+#pragma unroll
+      for (int k = 0; k < PULL_ILP; ++k) {
+        if (!act[k]) continue;
+        ++edges;
+        ++it[k];
+        if ((uw[k] >> (u[k] & 31)) & 1u) {
+          if (!pid || pid[u[k]] == pv[k]) {
+            hit[k] = true;
+            act[k] = false;
+          }
         }
       }
     }
-    // only this wavefront sets bits in its two words during a pull level, so
-    // the claim is a plain store of (old | new) per word, readable by others
-    const uint64_t mask = __ballot(hit);
-    const uint64_t word0 = (base + threadIdx.x - lane) >> 5;
-    if (lane == 0) {
-      fnew[word0] = (uint32_t)mask;
-      if ((uint32_t)mask) vis[word0] = vis_word | (uint32_t)mask;
-    }
-    if (lane == 32) {
-      fnew[word0 + 1] = (uint32_t)(mask >> 32);
-      if ((uint32_t)(mask >> 32)) vis[word0 + 1] = vis_word | (uint32_t)(mask >> 32);
+#pragma unroll
+    for (int k = 0; k < PULL_ILP; ++k) {
+      const uint64_t v = base + (uint64_t)k * WG + threadIdx.x;
+      if (hit[k]) {
+        ++claimed;
+        nd += noff[v + 1] - noff[v];
+      }
+      // only this wavefront sets bits in its two words during a pull level,
+      // so the claim is a plain store of (old | new) per word
+      const uint64_t mask = __ballot(hit[k]);
+      const uint64_t first = v - lane;  // the wavefront's first vertex of tile k
+      const uint64_t word0 = first >> 5;
+      if (first < n) {  // the padding past n owns no bitmap words
+        if (lane == 0) {
+          fnew[word0] = (uint32_t)mask;
+          if ((uint32_t)mask) vis[word0] = vw[k] | (uint32_t)mask;
+        }
+        if (lane == 32) {
+          fnew[word0 + 1] = (uint32_t)(mask >> 32);
+          if ((uint32_t)(mask >> 32)) vis[word0 + 1] = vw[k] | (uint32_t)(mask >> 32);
+        }
+      }
     }
   }
   claimed = block_sum64(claimed, s_red);

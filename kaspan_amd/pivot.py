@@ -7,13 +7,13 @@ candidates and the best score seeds FW-BW; the plain rule's pivot is always
 among the candidates and remains the fallback. Pivot choice changes time,
 never labels.
 
-The scorer is trained on outcomes the method logs: with multi-pivot FW-BW
-every round records, per partition, the pivot, the partition size, the SCC
-it decided and the forward-only / backward-only / unreached parts it left
-(opts.pivot_log). The target is the residual shrinkage
-1 - max(parts) / size: a pivot that splits its partition evenly (or decides
-much of it) scores high. Data comes from seeded random pivots; forward
-(BF16 MFMA), gradient (FP32) and Adam run as libkspn kernels on the GPU.
+In the multi-pivot rounds the score replaces in*out as each partition's
+pivot priority. The scorer is trained on pivot outcomes: by default the SCC
+a FW-BW from the vertex decides (log1p of its size; one solve gives it for
+every vertex), or, from the rounds' own outcome log (opts.pivot_log), the
+partition's residual shrinkage. Forward (BF16 MFMA), gradient (FP32) and
+Adam run as libkspn kernels on the GPU; inputs and target are standardised
+during training and the scaling is folded back into W1, b1, w3 and b3.
 """
 from __future__ import annotations
 
@@ -101,9 +101,27 @@ def pivot_log(g: BidiGraph, rounds=24, pivot=_lib.PIVOT_RANDOM, seed=0, cap=1 <<
     return rec.to(torch.int64) & 0xFFFFFFFF
 
 
-def outcome_samples(g: BidiGraph, rounds=24, seed=0, min_size=32):
-    """(features, target) from the method's own pivot log on g: random pivots per partition, target =
-    residual shrinkage 1 - max(forward-only, backward-only, unreached) / partition size"""
+def outcome_samples(g: BidiGraph, rounds=24, seed=0, min_size=32, target="size", sample=1 << 19):
+    """(features, target) pairs from pivot outcomes on g.
+
+    target="size": every live vertex is a candidate pivot whose outcome is the
+    SCC a FW-BW from it decides; one solve gives all of them (kspn_scc_sizes)
+    and the target is log1p(SCC size), for a seeded sample of ``sample`` live
+    vertices. target="shrinkage": the multi-pivot rounds' own log with random
+    pivots; per partition of at least ``min_size`` vertices the target is
+    1 - max(forward-only, backward-only, unreached) / partition size.
+    """
+    if target == "size":
+        labels, _ = scc(g)
+        sizes = scc_sizes(labels)
+        ind = (g.bw_off[1:] - g.bw_off[:-1])
+        outd = (g.fw_off[1:] - g.fw_off[:-1])
+        live = torch.nonzero((ind > 0) & (outd > 0)).flatten()
+        gen = torch.Generator(device="cpu").manual_seed(seed)
+        if live.numel() > sample:
+            live = live[torch.randperm(live.numel(), generator=gen)[:sample].to(live.device)]
+        v = live.to(torch.int32).contiguous()
+        return features(g, v), torch.log1p(sizes[v.long()].float())
     rec = pivot_log(g, rounds=rounds, pivot=_lib.PIVOT_RANDOM, seed=seed)
     rec = rec[rec[:, 2] >= min_size]
     size = rec[:, 2].float()
@@ -113,17 +131,40 @@ def outcome_samples(g: BidiGraph, rounds=24, seed=0, min_size=32):
     return x, t
 
 
+def fold_scaling(params: torch.Tensor, mu: torch.Tensor, sd: torch.Tensor, tm: float, ts: float) -> torch.Tensor:
+    """Parameters of a model trained on (x - mu) / sd -> (t - tm) / ts, as a model of x -> t."""
+    p = params.clone()
+    w1 = p[0:192].view(32, 6)
+    b1 = p[192:224]
+    b1 -= (w1 * (mu / sd)).sum(dim=1)
+    w1 /= sd
+    p[1280:1312] *= ts
+    p[1312] = p[1312] * ts + tm
+    return p
+
+
 def train(x: torch.Tensor, t: torch.Tensor, params: torch.Tensor | None = None, epochs=400, lr=3e-3, seed=0,
-          ctx: Context | None = None):
-    """Full-batch Adam on the GPU kernels; returns (params, losses)."""
+          ctx: Context | None = None, lr_final=None):
+    """Full-batch Adam on the GPU kernels with standardised inputs and target
+    and a cosine learning-rate decay to ``lr_final`` (lr / 20); returns the
+    folded parameters and the losses on the standardised problem."""
     ctx = ctx or Context.default()
+    mu = x.mean(dim=0)
+    sd = x.std(dim=0).clamp_min(1e-3)
+    tm = float(t.mean())
+    ts = max(float(t.std()), 1e-3)
+    xn = ((x - mu) / sd).contiguous()
+    tn = ((t - tm) / ts).contiguous()
     params = init_params(seed) if params is None else params.clone()
     state = torch.zeros(2 * N_PARAMS, dtype=torch.float32, device="cuda")
+    lr_final = lr / 20 if lr_final is None else lr_final
     losses = []
     for step in range(1, epochs + 1):
-        loss, gr = grad(params, x, t, ctx)
+        loss, gr = grad(params, xn, tn, ctx)
+        lr_t = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * (step - 1) / max(epochs - 1, 1)))
         _lib.check(_lib.lib().kspn_mlp_adam(ctx.ptr, C.c_void_p(params.data_ptr()), C.c_void_p(gr.data_ptr()),
-                                            C.c_void_p(state.data_ptr()), step, lr, _stream_ptr()), "kspn_mlp_adam")
+                                            C.c_void_p(state.data_ptr()), step, float(lr_t), _stream_ptr()),
+                   "kspn_mlp_adam")
         if step == 1 or step % 50 == 0 or step == epochs:
-            losses.append(float(loss.item()))
-    return params, losses
+            losses.append(float(loss.item()) / max(x.shape[0], 1))
+    return fold_scaling(params, mu, sd, tm, ts), losses

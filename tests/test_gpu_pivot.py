@@ -93,6 +93,25 @@ def test_gradients_match_oracle(ka, O, piv):
         assert abs(float(loss.item()) / rows - ref_loss) <= 1e-4 * max(1.0, ref_loss)
 
 
+def test_fold_scaling_is_exact(ka, O, piv):
+    """a model of standardised data folded into raw-feature parameters computes the same function (FP64 oracle)"""
+    import torch
+
+    rng = np.random.default_rng(5)
+    x = rng.normal(2.0, 1.5, size=(300, 6)).astype(np.float32)
+    p = piv.init_params(seed=3)
+    p[192:224] = torch.from_numpy(rng.normal(0, 0.2, 32).astype(np.float32)).cuda()
+    p[1312] = 0.4
+    mu = torch.from_numpy(x.mean(0)).cuda()
+    sd = torch.from_numpy(x.std(0)).cuda()
+    folded = piv.fold_scaling(p, mu, sd, 3.0, 2.5).cpu().numpy()
+    xn = (x - mu.cpu().numpy()) / sd.cpu().numpy()
+    a = O.mlp_forward(xn.astype(np.float32), *split(p.cpu().numpy())) * 2.5 + 3.0
+    b = O.mlp_forward(x, *split(folded))
+    # FP32 parameters and standardised inputs on one side: agreement to FP32 rounding
+    assert np.abs(a - b).max() <= 1e-4 * max(1.0, np.abs(a).max())
+
+
 def test_scc_sizes(ka, O):
     g = ka.generate("gnm", 5, n=20000, draws=30000)
     labels, _ = ka.scc(g)
@@ -124,9 +143,13 @@ def test_training_and_learned_pivot(ka, O, piv):
     xs, ts = [], []
     for seed in range(3):
         g = ka.generate("planted", 100 + seed, n=60000, max_size=20000)
-        x, t = piv.outcome_samples(g, rounds=12, seed=seed + 1)
+        x, t = piv.outcome_samples(g, seed=seed + 1)  # target log1p(SCC size)
         assert x.shape[0] == t.shape[0] > 50
-        assert float(t.min()) >= 0.0 and float(t.max()) <= 1.0
+        assert float(t.min()) >= float(np.log1p(1.0)) - 1e-6
+        # shrinkage samples from the rounds' own log lie in [0, 1]
+        xs2, ts2 = piv.outcome_samples(g, rounds=12, seed=seed + 1, target="shrinkage")
+        assert xs2.shape[0] == ts2.shape[0] > 10
+        assert float(ts2.min()) >= 0.0 and float(ts2.max()) <= 1.0
         xs.append(x)
         ts.append(t)
     x = torch.cat(xs)
