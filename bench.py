@@ -29,20 +29,29 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # library profile names -> kernel symbols as rocprofv3 reports them
-ROCPROF_NAMES = {"pull_fw": "kspn::k_pull<0>", "pull_bw": "kspn::k_pull<1>", "trim_first": "kspn::k_trim_first",
-                 "expand_fw": "kspn::k_expand<0>", "expand_bw": "kspn::k_expand<1>", "label_scc": "kspn::k_label_bits",
-                 "tail_solve": "kspn::k_tail_solve", "trim_sweep": "kspn::k_trim_sweep<1>"}
+ROCPROF_NAMES = {"pull_fw": ["kspn::k_pull<0>"], "pull_bw": ["kspn::k_pull<1>"], "trim_first": ["kspn::k_trim_first"],
+                 "expand_fw": ["kspn::k_expand<0>"], "expand_bw": ["kspn::k_expand<1>", "kspn::k_expand<5>"],
+                 "label_scc": ["kspn::k_label_bits", "kspn::k_part_apply"], "tail_solve": ["kspn::k_tail_solve"],
+                 "trim_sweep": ["kspn::k_trim_sweep<1>"],
+                 "reach_coop": ["kspn::k_reach_coop<0>", "kspn::k_reach_coop<1>", "kspn::k_reach_coop<5>"],
+                 "bitmap_ops": ["kspn::k_part_classify"], "pivot_reduce": ["kspn::k_part_key"]}
 
 
 def pmc_traffic(config, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary of this workload (copy-calibrated), or None."""
+    summary of this workload (copy-calibrated, launch-weighted over the kernel's
+    instantiations), or None."""
     path = os.path.join(ROOT, "profiles", f"{config}_pmc_traffic.json")
     if not os.path.exists(path):
         return None
-    data = json.load(open(path))
-    k = data.get("kernels", {}).get(ROCPROF_NAMES.get(kernel, ""))
-    return k["bytes_per_launch"] if k else None
+    data = json.load(open(path)).get("kernels", {})
+    found = [data[k] for k in ROCPROF_NAMES.get(kernel, []) if k in data]
+    launches = sum(k["launches"] for k in found)
+    if not launches:
+        return None
+    return sum(k["bytes_per_launch"] * k["launches"] for k in found) / launches
+
+
 CONFIGS = {
     # name: (generator kind, kwargs, seed, description)
     "gnm64k": ("gnm", dict(n=65536, draws=524288), 0x6B5350AA00000000,
@@ -62,6 +71,9 @@ CONFIGS = {
                   "per-graph vertex/edge offsets"),
 }
 LATENCY_SAMPLE = 128  # graphs of the batch solved one at a time for p50/p99
+# FW-BW rounds when --fwbw-rounds is not given: the planted workload's many large SCCs want the
+# partitioned multi-pivot rounds (32 is the plain rule's best in a sweep of 1..64)
+DEFAULT_ROUNDS = {"planted50m": 32}
 # bounded CPU-baseline samples for workloads whose full size takes the serial restatement too long
 CPU_SAMPLE = {
     "planted50m": ("planted", dict(n=500_000, max_size=100_000, expo=2.1, intra_deg=4.0, inter_deg=3.0)),
@@ -77,7 +89,7 @@ def main():
     ap.add_argument("--config", default="rmat24", choices=sorted(CONFIGS))
     ap.add_argument("--method", type=int, default=None, help="0 kaspan, 1 trim_ex (default: fastest known)")
     ap.add_argument("--pivot", choices=["plain", "learned"], default="plain")
-    ap.add_argument("--fwbw-rounds", type=int, default=1)
+    ap.add_argument("--fwbw-rounds", type=int, default=None)
     ap.add_argument("--coop-edges", type=int, default=0, help="persistent-kernel frontier edge bound (0: library default)")
     ap.add_argument("--coop-workgroups", type=int, default=0, help="persistent-kernel workgroups (0: library default)")
     ap.add_argument("--alpha", type=float, default=14.0, help="push->pull switch (Beamer alpha)")
@@ -100,6 +112,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy}))
         return
+    if args.fwbw_rounds is None:
+        args.fwbw_rounds = DEFAULT_ROUNDS.get(args.config, 1)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (multi-GPU sharding is a design, not built)"}))
         sys.exit(2)

@@ -261,6 +261,7 @@ struct Ctx {
   std::vector<hipEvent_t> free_events;
   hipStream_t stream = nullptr;
   int num_cu = 256;
+  uint32_t pub_seq = 0;  // sequence number of the last counter publication (h_map[63])
   // persistent multi-level reachability (cooperative launch)
   struct CoopCtl* coop = nullptr;  // device control block
   uint32_t coop_grid = 0;          // co-resident workgroups (0: cooperative launch unavailable)
