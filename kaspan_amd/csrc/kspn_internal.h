@@ -197,6 +197,7 @@ enum Counter : int {
   C_FSEG,  // size of the current frontier segment
   C_DTOT,  // running length of the decided list
   C_ERR,   // set by a kernel that met a broken internal invariant
+  C_CURSOR,  // write cursor of an unordered bitmap compaction
   C_COUNT_ = 16
 };
 

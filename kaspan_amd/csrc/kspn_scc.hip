@@ -55,8 +55,31 @@ __device__ inline void publish_seq(uint32_t* map, uint32_t seq) {
   }
 }
 
-__global__ void k_publish(const uint32_t* cnt, const unsigned long long* cnt64, const uint64_t* extra, uint32_t* map,
-                          uint32_t seq) {
+// Optionally first reduces per-workgroup partials (claimed count -> *pcount,
+// edges scanned -> cnt64[0], next-level degree sum -> cnt64[1]); after the
+// copy it can zero *reset and cnt64[0..2] for the next level, so a level
+// needs no separate reset or reduction launch. One workgroup of WG threads.
+__global__ __launch_bounds__(WG) void k_publish(uint32_t* cnt, unsigned long long* cnt64, const uint64_t* extra,
+                                                uint32_t* map, uint32_t seq, const uint64_t* __restrict__ part,
+                                                uint32_t nparts, uint32_t* pcount, uint32_t* reset) {
+  __shared__ uint64_t s_red[WG / WAVE];
+  if (part) {
+    uint64_t a = 0, b = 0, d = 0;
+    for (uint32_t i = threadIdx.x; i < nparts; i += WG) {
+      a += part[3 * i];
+      b += part[3 * i + 1];
+      d += part[3 * i + 2];
+    }
+    a = block_sum64(a, s_red);
+    b = block_sum64(b, s_red);
+    d = block_sum64(d, s_red);
+    if (threadIdx.x == 0) {
+      *pcount = (uint32_t)a;
+      cnt64[0] = b;
+      cnt64[1] = d;
+    }
+    __syncthreads();
+  }
   uint32_t t = threadIdx.x;
   if (t < C_COUNT_) map[t] = cnt[t];
   if (t == 32) {
@@ -80,6 +103,75 @@ __global__ void k_publish(const uint32_t* cnt, const unsigned long long* cnt64, 
     map[23] = (uint32_t)(y >> 32);
   }
   publish_seq(map, seq);
+  if (reset) {
+    __syncthreads();  // every copy above is done
+    if (threadIdx.x == 0) {
+      *reset = 0;
+      cnt64[0] = 0;
+      cnt64[1] = 0;
+      cnt64[2] = 0;
+    }
+  }
+}
+
+__global__ void k_zero2(uint32_t* a, uint32_t* b) {
+  if (threadIdx.x == 0) {
+    *a = 0;
+    *b = 0;
+  }
+}
+
+// FW-BW seed: bm = {v} (the word holding v gets its bit, all others 0),
+// queue = [v], *count = 1; one launch for the four steps
+__global__ void k_seed(uint32_t* bm, uint64_t words, uint32_t v, uint32_t* queue, uint32_t* count) {
+  for (uint64_t w = blockIdx.x * (uint64_t)WG + threadIdx.x; w < words; w += (uint64_t)gridDim.x * WG)
+    bm[w] = (w == (v >> 5)) ? (1u << (v & 31)) : 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    queue[0] = v;
+    *count = 1;
+  }
+}
+
+// Unordered bitmap -> list: each workgroup step takes 4*WG words, scans
+// their popcounts in LDS and reserves its range with one atomicAdd on
+// *cursor (zero before the launch). For lists whose order does not matter.
+__global__ __launch_bounds__(WG) void k_bitmap_to_queue(const uint32_t* __restrict__ bm, uint64_t words,
+                                                        uint32_t* __restrict__ list, uint32_t* cursor) {
+  __shared__ uint32_t s_scan[WG];
+  __shared__ uint32_t s_base;
+  constexpr uint32_t PER = 4;
+  for (uint64_t base = (uint64_t)blockIdx.x * WG * PER; base < words; base += (uint64_t)gridDim.x * WG * PER) {
+    uint32_t w[PER];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+      const uint64_t i = base + (uint64_t)threadIdx.x * PER + k;
+      w[k] = i < words ? bm[i] : 0u;
+      cnt += __popc(w[k]);
+    }
+    s_scan[threadIdx.x] = cnt;
+    __syncthreads();
+    for (uint32_t o = 1; o < WG; o <<= 1) {
+      const uint32_t x = threadIdx.x >= o ? s_scan[threadIdx.x - o] : 0u;
+      __syncthreads();
+      s_scan[threadIdx.x] += x;
+      __syncthreads();
+    }
+    if (threadIdx.x == WG - 1) s_base = s_scan[WG - 1] ? atomicAdd(cursor, s_scan[WG - 1]) : 0u;
+    __syncthreads();
+    uint32_t pos = s_base + s_scan[threadIdx.x] - cnt;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+      const uint64_t i = base + (uint64_t)threadIdx.x * PER + k;
+      uint32_t x = w[k];
+      while (x) {
+        const uint32_t b = __ffs(x) - 1;
+        list[pos++] = (uint32_t)(i * 32 + b);
+        x &= x - 1;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // set bit v of bm
@@ -570,12 +662,16 @@ __global__ void k_coop_init(CoopCtl* ctl, const uint32_t* fcount, unsigned long 
 }
 
 // after a launch: the last frontier's size into the host's counter slot of
-// the buffer that holds it; results into the mapped host counters
-__global__ void k_coop_publish(const CoopCtl* ctl, uint32_t* cnt_even, uint32_t* cnt_odd, uint32_t* map, uint32_t seq) {
+// the buffer that holds it, the other slot and the level counters zeroed
+// for a following host level; results into the mapped host counters
+__global__ void k_coop_publish(const CoopCtl* ctl, uint32_t* cnt_even, uint32_t* cnt_odd, unsigned long long* cnt64,
+                               uint32_t* map, uint32_t seq) {
   if (threadIdx.x == 0) {
     const uint32_t L = ctl->levels;
     const uint32_t f = ctl->cnt[L % 3];
     *((L & 1) ? cnt_odd : cnt_even) = f;
+    *((L & 1) ? cnt_even : cnt_odd) = 0;  // the host's next level appends there
+    cnt64[0] = cnt64[1] = cnt64[2] = 0;
     const unsigned long long fe = ctl->nsum[L % 3], ex = ctl->expanded;
     map[32] = L;
     map[33] = ctl->reason;
@@ -706,7 +802,8 @@ template <int DIR>
 __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restrict__ cand, uint32_t* vis,
                                              const uint64_t* __restrict__ roff, const uint32_t* __restrict__ radj,
                                              const uint64_t* __restrict__ noff, uint32_t* __restrict__ fnew,
-                                             uint64_t* __restrict__ part, const uint32_t* __restrict__ pid) {
+                                             uint64_t* __restrict__ part, const uint32_t* __restrict__ pid,
+                                             uint32_t* cursor) {
   // A candidate costs a chain of dependent loads (bitmap word, row offsets,
   // in-neighbour, its visited bit, its partition). Each thread carries
   // PULL_ILP vertices (one per 256-vertex tile of its workgroup's span)
@@ -799,6 +896,7 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
     part[3 * blockIdx.x] = claimed;
     part[3 * blockIdx.x + 1] = edges;
     part[3 * blockIdx.x + 2] = nd;
+    if (blockIdx.x == 0) *cursor = 0;  // for a compaction of this level's bitmap
   }
 }
 
@@ -1857,10 +1955,13 @@ int wait_published(Ctx* c, hipStream_t s, uint32_t seq) {
 }
 
 // publish the device counters (and *extra) to host-mapped memory, wait,
-// and copy them into h_cnt / h_u64[0] (extra) / h_u64[1] (cnt64[0])
-int publish(Ctx* c, hipStream_t s, const uint64_t* extra) {
+// and copy them into h_cnt / h_u64[0] (extra) / h_u64[1] (cnt64[0]).
+// With part: reduce nparts per-workgroup partials first (into *pcount and
+// cnt64[0..1]); with reset: zero *reset and cnt64[0..2] afterwards.
+int publish(Ctx* c, hipStream_t s, const uint64_t* extra, const uint64_t* part = nullptr, uint32_t nparts = 0,
+            uint32_t* pcount = nullptr, uint32_t* reset = nullptr) {
   const uint32_t seq = ++c->pub_seq;
-  k_publish<<<1, 64, 0, s>>>(c->cnt, c->cnt64, extra, c->d_map, seq);
+  k_publish<<<1, WG, 0, s>>>(c->cnt, c->cnt64, extra, c->d_map, seq, part, nparts, pcount, reset);
   KSPN_HIP(hipGetLastError());
   KSPN_TRY(wait_published(c, s, seq));
   for (int i = 0; i < C_COUNT_; ++i) c->h_cnt[i] = c->h_map[i];
@@ -2019,7 +2120,7 @@ int run_coop(Run& R, int op, const OpArgs& a, const uint64_t* off, const uint32_
     KSPN_HIP(hipGetLastError());
   }
   const uint32_t seq = ++c->pub_seq;
-  k_coop_publish<<<1, 64, 0, R.s>>>(c->coop, &c->cnt[fc], &c->cnt[nc], c->d_map, seq);
+  k_coop_publish<<<1, 64, 0, R.s>>>(c->coop, &c->cnt[fc], &c->cnt[nc], c->cnt64, c->d_map, seq);
   KSPN_HIP(hipGetLastError());
   KSPN_TRY(wait_published(c, R.s, seq));
   const volatile uint32_t* m = c->h_map;
@@ -2061,6 +2162,8 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   uint32_t maxd = 0xFFFFFFFFu;   // largest out-degree in the frontier (unknown)
   const bool coop = R.coop_wg > 0 && !(R.o.flags & KSPN_FLAG_NO_COOP);
   const int op = dir == 0 ? OP_FW : (pid ? OP_PBW : OP_BW);
+  // each level's publication zeroes the counters of the level after it
+  k_level_reset<<<1, 64, 0, R.s>>>(&c->cnt[nc], c->cnt64);
   while (f > 0) {
     // direction: Beamer's rule on the frontier's edge count, which the
     // previous level summed on the device for the vertices it claimed
@@ -2084,7 +2187,6 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       if (co.levels > 0) continue;
       // nothing expanded (not expected: the bounds were checked): one level on the host
     }
-    k_level_reset<<<1, 64, 0, R.s>>>(&c->cnt[nc], c->cnt64);
     if (!pull) {
       {
         Prof p(c, R.s, K_FRONTIER_DEG, 28.0 * f);
@@ -2111,14 +2213,15 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       {
         Prof p(c, R.s, kern, (double)R.n / 4.0 + 16.0 * R.live_n);
         if (dir == 0)
-          k_pull<0><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, off, c->fnew, c->part, pid);
+          k_pull<0><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, off, c->fnew, c->part, pid, &c->cnt[C_CURSOR]);
         else
-          k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, off, c->fnew, c->part, pid);
+          k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, off, c->fnew, c->part, pid, &c->cnt[C_CURSOR]);
       }
-      k_reduce3<<<1, WG, 0, R.s>>>(c->part, grid, &c->cnt[nc], c->cnt64);
       if (fw_levels_pull) (*fw_levels_pull)++;
+      // the workgroup partials are reduced by the publication
+      KSPN_TRY(publish(c, R.s, nullptr, c->part, grid, &c->cnt[nc], &c->cnt[fc]));
     }
-    KSPN_TRY(publish(c, R.s, pull ? nullptr : c->scan + f));
+    if (!pull) KSPN_TRY(publish(c, R.s, c->scan + f, nullptr, 0, nullptr, &c->cnt[fc]));
     if (!pull) {
       edges = c->h_u64[0];  // exact total of the level just expanded
       m_rest -= (double)edges;
@@ -2135,11 +2238,11 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
     known = true;
     if (pull) m_rest -= (double)edges;
     if (pull && nf > 0 && (double)nf * beta < (double)R.live_n) {
-      // back to push: the frontier is the bitmap of the last pull level
+      // back to push: the frontier is the bitmap of the last pull level (its
+      // size nf is already in cnt[nc]; the pull zeroed the cursor)
       pull = false;
-      uint32_t cnt_list = 0;
-      KSPN_TRY(build_live_list(c, R.n, c->fnew, N, &cnt_list, R.s));
-      k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[nc], &c->cnt[C_AUX]);
+      Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words + 4.0 * nf);
+      k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->fnew, R.words, N, &c->cnt[C_CURSOR]);
     }
     std::swap(F, N);
     std::swap(fc, nc);
@@ -2151,21 +2254,15 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
 int fwbw(Run& R, uint32_t pivot, uint32_t* decided) {
   Ctx* c = R.c;
   {
-    Prof p(c, R.s, K_BITMAP, 8.0 * R.words);
-    k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->fwv, R.words, 0u);
-    k_bm_set_one<<<1, 64, 0, R.s>>>(c->fwv, pivot);
+    Prof p(c, R.s, K_BITMAP, 4.0 * R.words);
+    k_seed<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->fwv, R.words, pivot, c->qa, &c->cnt[C_Q0]);
   }
   uint32_t lp = 0, lq = 0;
-  k_set_u32<<<1, 64, 0, R.s>>>(c->qa, pivot);
-  k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], 1u);
   KSPN_TRY(reach(R, 0, 1, nullptr, &lp, &lq));
   {
-    Prof p(c, R.s, K_BITMAP, 12.0 * R.words);
-    k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwv, R.words, 0u);
-    k_bm_set_one<<<1, 64, 0, R.s>>>(c->bwv, pivot);
+    Prof p(c, R.s, K_BITMAP, 4.0 * R.words);
+    k_seed<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwv, R.words, pivot, c->qa, &c->cnt[C_Q0]);
   }
-  k_set_u32<<<1, 64, 0, R.s>>>(c->qa, pivot);
-  k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], 1u);
   KSPN_TRY(reach(R, 1, 1, nullptr, &lp, &lq));
   // the pivot's SCC: clear it from live, then label it with its smallest member
   uint32_t nparts = stream_grid(c, R.words);
@@ -2354,10 +2451,13 @@ int trim_exhaustive(Run& R, uint32_t* decided) {
 int trim_normal(Run& R, uint32_t* decided) {
   Ctx* c = R.c;
   const kspn_graph* g = R.g;
-  uint32_t q = 0;
-  KSPN_TRY(build_live_list(c, R.n, c->live, c->qa, &q, R.s));
-  k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], &c->cnt[C_AUX]);
-  KSPN_HIP(hipMemsetAsync(&c->cnt[C_Q1], 0, sizeof(uint32_t), R.s));
+  // candidates: every live vertex, in any order (count known on the host)
+  const uint32_t q = R.live_n;
+  k_zero2<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], &c->cnt[C_Q1]);
+  {
+    Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words + 4.0 * q);
+    k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->live, R.words, c->qa, &c->cnt[C_Q0]);
+  }
   if (q) {
     Prof p(c, R.s, K_TRIM_SWEEP, 20.0 * q);
     k_trim_sweep<TRIM_NORMAL><<<stream_grid(c, q), WG, 0, R.s>>>(c->qa, &c->cnt[C_Q0], c->live, nullptr, g->fw_off,
