@@ -1379,7 +1379,7 @@ __global__ void k_tarjan(uint32_t n, const uint64_t* __restrict__ off, const uin
 
 // Tail solver: colour rounds on a compacted subgraph inside one workgroup
 // of 1024 threads (block barriers, no host round trips). Vertex i of the
-// subgraph is super[i] (ascending); fo/fa and bo/ba are its out- and
+// subgraph is super[i] (any order; subid maps back); fo/fa and bo/ba are its out- and
 // in-edge CSRs. Round r keys live vertices with ~rotr(super[i], r), sweeps
 // max-colours along in-edges to a fixpoint, decides the roots and closes
 // backward over out-edges within each colour, then writes the minimum
@@ -1387,11 +1387,12 @@ __global__ void k_tarjan(uint32_t n, const uint64_t* __restrict__ off, const uin
 // 2 decided this round, 1 decided. Rounds are capped by max_rounds (0: none);
 // *status = 1 if the cap was hit.
 __global__ __launch_bounds__(1024) void k_tail_solve(uint32_t sn, const uint32_t* __restrict__ super,
+                                                     const uint32_t* __restrict__ subid,
                                                      const uint64_t* __restrict__ fo, const uint32_t* __restrict__ fa,
                                                      const uint64_t* __restrict__ bo, const uint32_t* __restrict__ ba,
                                                      uint32_t* col, uint32_t* state, uint32_t* minv,
                                                      uint32_t* __restrict__ labels, uint32_t r0, uint32_t max_rounds,
-                                                     uint32_t* status) {
+                                                     uint32_t* status, uint32_t* live) {
   __shared__ uint32_t s_flag, s_live;
   const uint32_t tid = threadIdx.x, T = blockDim.x;
   for (uint32_t i = tid; i < sn; i += T) state[i] = 0;
@@ -1453,31 +1454,112 @@ __global__ __launch_bounds__(1024) void k_tail_solve(uint32_t sn, const uint32_t
       if (!s_flag) break;
     }
     // minimum member per colour: the root of colour c is the vertex whose
-    // key is c; its compact index is found by binary search over super[]
+    // key is c, at compact index subid[root]
     for (uint32_t i = tid; i < sn; i += T) {
       if (state[i] != 2) continue;
-      uint32_t g = rotl32(~col[i], r);
-      uint32_t lo = 0, hi = sn;
-      while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (super[mid] <= g) lo = mid; else hi = mid;
-      }
-      atomicMin(&minv[lo], super[i]);
+      atomicMin(&minv[subid[rotl32(~col[i], r)]], super[i]);
     }
     __syncthreads();
     for (uint32_t i = tid; i < sn; i += T) {
       if (state[i] != 2) continue;
-      uint32_t g = rotl32(~col[i], r);
-      uint32_t lo = 0, hi = sn;
-      while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (super[mid] <= g) lo = mid; else hi = mid;
-      }
-      labels[super[i]] = minv[lo];
+      labels[super[i]] = minv[subid[rotl32(~col[i], r)]];
     }
     __syncthreads();
     for (uint32_t i = tid; i < sn; i += T)
       if (state[i] == 2) state[i] = 1;
+  }
+  // every decided vertex leaves the live set
+  __syncthreads();
+  for (uint32_t i = tid; i < sn; i += T)
+    if (state[i] == 1) atomicAnd(&live[super[i] >> 5], ~(1u << (super[i] & 31)));
+}
+
+// tail subgraph, one pass: compact ids, live out- and in-degrees (entry u = 0)
+__global__ void k_tail_prep(const uint32_t* __restrict__ U, const uint32_t* ucount, const uint64_t* __restrict__ fw_off,
+                            const uint32_t* __restrict__ fw_adj, const uint64_t* __restrict__ bw_off,
+                            const uint32_t* __restrict__ bw_adj, const uint32_t* __restrict__ live,
+                            uint32_t* __restrict__ subid, uint64_t* __restrict__ degf, uint64_t* __restrict__ degb) {
+  const uint32_t u = *ucount;
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i <= u; i += gridDim.x * WG) {
+    if (i == u) {
+      degf[i] = 0;
+      degb[i] = 0;
+      continue;
+    }
+    const uint32_t v = U[i];
+    subid[v] = i;
+    uint64_t df = 0, db = 0;
+    for (uint64_t it = fw_off[v], end = fw_off[v + 1]; it < end; ++it) df += bm_get(live, fw_adj[it]);
+    for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it) db += bm_get(live, bw_adj[it]);
+    degf[i] = df;
+    degb[i] = db;
+  }
+}
+
+// in-place exclusive scans of two arrays of u+1 <= 4096 entries in one
+// workgroup (entry u becomes the total); *flag = 0 for the tail solver. A
+// larger u (*ucount) is reported through *err and nothing is written.
+__global__ __launch_bounds__(1024) void k_tail_scan(const uint32_t* ucount, uint64_t* __restrict__ a,
+                                                    uint64_t* __restrict__ b, uint32_t* flag, uint32_t* err) {
+  __shared__ uint64_t sa[1024], sb[1024];
+  const uint32_t t = threadIdx.x;
+  const uint32_t u = *ucount;
+  if (t == 0) *flag = 0;
+  if (u >= 4096) {
+    if (t == 0) *err = 1;
+    return;
+  }
+  uint64_t da[4], db[4], suma = 0, sumb = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t i = 4 * t + k;
+    da[k] = i <= u ? a[i] : 0;
+    db[k] = i <= u ? b[i] : 0;
+    suma += da[k];
+    sumb += db[k];
+  }
+  sa[t] = suma;
+  sb[t] = sumb;
+  __syncthreads();
+  for (uint32_t o = 1; o < 1024; o <<= 1) {
+    const uint64_t xa = t >= o ? sa[t - o] : 0, xb = t >= o ? sb[t - o] : 0;
+    __syncthreads();
+    sa[t] += xa;
+    sb[t] += xb;
+    __syncthreads();
+  }
+  uint64_t ra = sa[t] - suma, rb = sb[t] - sumb;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t i = 4 * t + k;
+    if (i <= u) {
+      a[i] = ra;
+      b[i] = rb;
+    }
+    ra += da[k];
+    rb += db[k];
+  }
+}
+
+// both directions' rows of the tail subgraph in one launch
+__global__ void k_tail_fill(const uint32_t* __restrict__ U, const uint32_t* ucount, const uint64_t* __restrict__ fw_off,
+                            const uint32_t* __restrict__ fw_adj, const uint64_t* __restrict__ bw_off,
+                            const uint32_t* __restrict__ bw_adj, const uint32_t* __restrict__ live,
+                            const uint32_t* __restrict__ subid, const uint64_t* __restrict__ fo, uint32_t* __restrict__ fa,
+                            const uint64_t* __restrict__ bo, uint32_t* __restrict__ ba) {
+  const uint32_t u = *ucount;
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < u; i += gridDim.x * WG) {
+    const uint32_t v = U[i];
+    uint64_t p = fo[i];
+    for (uint64_t it = fw_off[v], end = fw_off[v + 1]; it < end; ++it) {
+      const uint32_t w = fw_adj[it];
+      if (bm_get(live, w)) fa[p++] = subid[w];
+    }
+    p = bo[i];
+    for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it) {
+      const uint32_t w = bw_adj[it];
+      if (bm_get(live, w)) ba[p++] = subid[w];
+    }
   }
 }
 
@@ -2653,29 +2735,47 @@ int residual(Run& R, uint32_t* decided) {
 int tail_solve(Run& R, uint32_t* decided) {
   Ctx* c = R.c;
   const kspn_graph* g = R.g;
-  uint32_t u = 0;
-  KSPN_TRY(build_live_list(c, R.n, c->live, c->ulist, &u, R.s));
+  const uint32_t u = R.live_n;  // the live vertices, in any order
   if (u == 0) {
     *decided = 0;
     return KSPN_OK;
   }
-  k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_RES_N], &c->cnt[C_AUX]);
   uint32_t* subid = c->col;
-  {
-    Prof p(c, R.s, K_RESIDUAL, 8.0 * u);
-    k_sub_ids<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], subid);
-    k_sub_deg<<<stream_grid(c, (uint64_t)u + 1), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj,
-                                                               c->live, nullptr, c->deg);
-  }
-  KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)u + 1, R.s));
-  {
-    Prof p(c, R.s, K_RESIDUAL, 8.0 * u);
-    k_sub_deg<<<stream_grid(c, (uint64_t)u + 1), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj,
-                                                               c->live, nullptr, c->deg);
-  }
   if (!c->sub_off) KSPN_TRY(dalloc(&c->sub_off, (uint64_t)c->cap_n + 64));
-  KSPN_TRY(exclusive_scan_u64(c, c->deg, c->sub_off, (uint64_t)u + 1, R.s));
+  k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_RES_N], 0u);
+  {
+    Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words + 4.0 * u);
+    k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->live, R.words, c->ulist, &c->cnt[C_RES_N]);
+  }
+  if (u < 4096) {
+    // compact ids, degrees, both scans and the solver's flag: two launches
+    Prof p(c, R.s, K_RESIDUAL, 24.0 * u);
+    k_tail_prep<<<stream_grid(c, (uint64_t)u + 1), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj,
+                                                                g->bw_off, g->bw_adj, c->live, subid, c->scan,
+                                                                c->sub_off);
+    k_tail_scan<<<1, 1024, 0, R.s>>>(&c->cnt[C_RES_N], c->scan, c->sub_off, &c->cnt[C_FLAG], &c->cnt[C_ERR]);
+  } else {
+    {
+      Prof p(c, R.s, K_RESIDUAL, 8.0 * u);
+      k_sub_ids<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], subid);
+      k_sub_deg<<<stream_grid(c, (uint64_t)u + 1), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj,
+                                                                 c->live, nullptr, c->deg);
+    }
+    KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)u + 1, R.s));
+    {
+      Prof p(c, R.s, K_RESIDUAL, 8.0 * u);
+      k_sub_deg<<<stream_grid(c, (uint64_t)u + 1), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj,
+                                                                 c->live, nullptr, c->deg);
+    }
+    KSPN_TRY(exclusive_scan_u64(c, c->deg, c->sub_off, (uint64_t)u + 1, R.s));
+    k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_FLAG], 0u);
+  }
   KSPN_TRY(publish(c, R.s, c->scan + u));
+  // the compaction's own count (the host's live count should agree)
+  if (c->h_cnt[C_ERR] || c->h_cnt[C_RES_N] != u) {
+    note_error("tail_solve", "live count differs from the live bitmap", __FILE__, __LINE__);
+    return KSPN_E_INVALID;
+  }
   uint64_t sm = c->h_u64[0];  // the two directions have the same edge count
   if (2 * sm + 2 > c->cap_sub_m) {
     KSPN_TRY(dalloc(&c->sub_adj, 2 * sm + 2));
@@ -2685,21 +2785,17 @@ int tail_solve(Run& R, uint32_t* decided) {
   uint32_t* ba = c->sub_adj + sm + 1;
   {
     Prof p(c, R.s, K_RESIDUAL, 32.0 * u + 8.0 * sm);
-    k_sub_fill<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, c->live, nullptr,
-                                                  subid, c->scan, fa);
-    k_sub_fill<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->live, nullptr,
-                                                  subid, c->sub_off, ba);
+    k_tail_fill<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, g->bw_off, g->bw_adj,
+                                                   c->live, subid, c->scan, fa, c->sub_off, ba);
   }
-  k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_FLAG], 0u);
   uint32_t cap = R.o.max_rounds ? R.o.max_rounds - R.rounds : 0;  // > 0: checked by the caller
   {
     Prof p(c, R.s, K_TAIL, 24.0 * u + 8.0 * sm);
-    k_tail_solve<<<1, 1024, 0, R.s>>>(u, c->ulist, c->scan, fa, c->sub_off, ba, c->qa, c->qb, c->qc, R.labels,
-                                      R.rotation, cap, &c->cnt[C_FLAG]);
+    k_tail_solve<<<1, 1024, 0, R.s>>>(u, c->ulist, subid, c->scan, fa, c->sub_off, ba, c->qa, c->qb, c->qc, R.labels,
+                                      R.rotation, cap, &c->cnt[C_FLAG], c->live);
   }
   KSPN_TRY(publish(c, R.s, nullptr));
   if (c->h_cnt[C_FLAG]) return KSPN_E_ITER_CAP;
-  KSPN_HIP(hipMemsetAsync(c->live, 0, R.words * sizeof(uint32_t), R.s));
   if (R.st) {
     R.st->residual_n = u;
     R.st->residual_m = sm;
