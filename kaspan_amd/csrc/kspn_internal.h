@@ -249,10 +249,11 @@ struct Ctx {
   uint32_t* mlp_buf = nullptr;  // learned pivot: candidates, histogram, features, scores
   // partitioned FW-BW
   uint32_t* pid = nullptr;     // partition of every vertex
-  uint64_t* pbest = nullptr;   // per partition: best pivot key
+  uint64_t* pbest = nullptr;   // per sparse partition id: best pivot key (3n slots)
   uint32_t* pmin = nullptr;    // per partition: smallest member of the pivot's SCC
-  uint32_t* pflags = nullptr;  // per (partition, class): used
-  uint32_t* pnew = nullptr;    // per (partition, class): new partition id (scan of pflags)
+  uint32_t* pflags = nullptr;  // per sparse partition id: used (pbest != 0)
+  uint32_t* pnew = nullptr;    // per sparse partition id: compact index (scan of pflags)
+  uint32_t* pcnt = nullptr;    // per partition: outcome counters for the pivot log
   float* vscore = nullptr;     // per vertex: MLP score
   float* vfeat = nullptr;      // features of live vertices (6 per vertex) + scores
   // profiling

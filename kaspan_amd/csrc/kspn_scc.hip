@@ -1648,11 +1648,65 @@ __device__ inline unsigned long long wave_max64(unsigned long long v) {
 __global__ __launch_bounds__(WG) void k_part_key(uint32_t n, const uint32_t* __restrict__ live,
                                                  const uint32_t* __restrict__ pid, const uint64_t* __restrict__ fw_off,
                                                  const uint64_t* __restrict__ bw_off, const float* __restrict__ score,
+                                                 uint64_t rnd_seed, unsigned long long* pbest);
+
+// Per-workgroup aggregation of (slot, value) updates in an LDS hash table
+// (linear probing over AGG_SLOTS entries); a full table falls back to the
+// global atomic. Many vertices update the same slot (a partition's pivot
+// key, an SCC's smallest member), so this turns one global atomic per
+// vertex into one per workgroup and slot.
+constexpr uint32_t AGG_SLOTS = 512;
+constexpr uint32_t AGG_EMPTY = 0xFFFFFFFFu;
+
+template <typename T>
+__device__ inline void agg_init(uint32_t* key, T* val, T none) {
+  for (uint32_t i = threadIdx.x; i < AGG_SLOTS; i += blockDim.x) {
+    key[i] = AGG_EMPTY;
+    val[i] = none;
+  }
+  __syncthreads();
+}
+
+// true if the update landed in the table
+template <typename T, bool MAX>
+__device__ inline bool agg_put(uint32_t* key, T* val, uint32_t k, T v) {
+  uint32_t h = (k * 2654435761u) >> 23;  // 9 bits: AGG_SLOTS
+  for (int probe = 0; probe < 8; ++probe) {
+    uint32_t old = key[h];
+    if (old == AGG_EMPTY) old = atomicCAS(&key[h], AGG_EMPTY, k);
+    if (old == AGG_EMPTY || old == k) {
+      if constexpr (MAX) atomicMax(&val[h], v);
+      else atomicMin(&val[h], v);
+      return true;
+    }
+    h = (h + 1) & (AGG_SLOTS - 1);
+  }
+  return false;
+}
+
+template <typename T, bool MAX>
+__device__ inline void agg_flush(const uint32_t* key, const T* val, T* global) {
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < AGG_SLOTS; i += blockDim.x) {
+    if (key[i] == AGG_EMPTY) continue;
+    if constexpr (MAX) atomicMax(&global[key[i]], val[i]);
+    else atomicMin(&global[key[i]], val[i]);
+  }
+}
+
+// pbest[p] = max over live v in p of (priority << 32 | ~v): priority is
+// min(in*out, 2^32-1) (plain rule, smallest id on ties), the ordered MLP
+// score, or a hash (random pivots). The wavefront's largest group makes one
+// update; updates are aggregated per workgroup in LDS.
+__global__ __launch_bounds__(WG) void k_part_key(uint32_t n, const uint32_t* __restrict__ live,
+                                                 const uint32_t* __restrict__ pid, const uint64_t* __restrict__ fw_off,
+                                                 const uint64_t* __restrict__ bw_off, const float* __restrict__ score,
                                                  uint64_t rnd_seed, unsigned long long* pbest) {
+  __shared__ uint32_t s_key[AGG_SLOTS];
+  __shared__ unsigned long long s_val[AGG_SLOTS];
+  agg_init<unsigned long long>(s_key, s_val, 0ull);
   const uint64_t pad = ((uint64_t)n + WG - 1) / WG * WG;
   const uint32_t lane = lane_id();
-  uint32_t wr = 0xFFFFFFFFu;  // partition of the wavefront's running maximum (uniform)
-  unsigned long long wm = 0;   // the running maximum (lane 0)
   for (uint64_t base = (uint64_t)blockIdx.x * WG; base < pad; base += (uint64_t)gridDim.x * WG) {
     const uint64_t v = base + threadIdx.x;
     const bool valid = v < n && bm_get(live, (uint32_t)v);
@@ -1673,110 +1727,99 @@ __global__ __launch_bounds__(WG) void k_part_key(uint32_t n, const uint32_t* __r
       }
       key = ((unsigned long long)pri << 32) | (uint32_t)~(uint32_t)v;
     }
-    // one peeling pass for the most common partition; the wavefront keeps a
-    // running maximum for the last partition it peeled (grid-stride steps of
-    // one wavefront mostly see the same large partition), so one atomic per
-    // wavefront and partition change, not per step; then per-lane atomics
-    uint64_t pending = __ballot(valid);
+    const uint64_t pending = __ballot(valid);
     bool left = valid;
     if (pending) {
-      uint32_t leader = __ffsll((unsigned long long)pending) - 1;
-      uint32_t r = __shfl(p, leader);
-      bool mine = valid && p == r;
-      uint64_t grp = __ballot(mine);
-      if (__popcll(grp) >= 8) {
-        unsigned long long m = wave_max64(mine ? key : 0ull);  // valid in lane 0
-        if (r != wr) {
-          if (lane == 0 && wr != 0xFFFFFFFFu && pbest[wr] < wm) atomicMax(&pbest[wr], wm);
-          wr = r;
-          wm = m;
-        } else if (m > wm) {
-          wm = m;
-        }
+      const uint32_t leader = __ffsll((unsigned long long)pending) - 1;
+      const uint32_t r = __shfl(p, leader);
+      const bool mine = valid && p == r;
+      if (__popcll(__ballot(mine)) >= 8) {
+        const unsigned long long m = wave_max64(mine ? key : 0ull);  // valid in lane 0
+        if (lane == 0 && !agg_put<unsigned long long, true>(s_key, s_val, r, m)) atomicMax(&pbest[r], m);
         if (mine) left = false;
       }
     }
-    if (left && pbest[p] < key) atomicMax(&pbest[p], key);
+    if (left && !agg_put<unsigned long long, true>(s_key, s_val, p, key)) atomicMax(&pbest[p], key);
   }
-  if (lane == 0 && wr != 0xFFFFFFFFu && pbest[wr] < wm) atomicMax(&pbest[wr], wm);
+  agg_flush<unsigned long long, true>(s_key, s_val, pbest);
 }
 
-// pivot of every partition into F; its forward and backward bits set.
-// Every partition holds a live vertex (pfwbw_rounds keeps it so); a key of
-// 0 or an id >= n would break that and is reported through *err instead of
-// being used as an index.
-__global__ void k_part_pivots(uint32_t np, uint32_t n, const unsigned long long* __restrict__ pbest,
-                              uint32_t* __restrict__ F, uint32_t* fwv, uint32_t* bwv, uint32_t* err) {
-  for (uint32_t p = blockIdx.x * WG + threadIdx.x; p < np; p += gridDim.x * WG) {
-    const unsigned long long k = pbest[p];
-    uint32_t v = ~(uint32_t)k;
-    if (k == 0ull || v >= n) {
+// Partition ids are sparse inside a round: after a round, the part of
+// partition q in class c (0 unreached, 1 forward only, 2 backward only) gets
+// id 3 * newid[q] + c. The next round's pivot keys (pbest, one slot per
+// sparse id) show which ids are in use; an exclusive scan of "slot used"
+// gives newid, the compact index of each used partition. So no per-vertex
+// pass writes per-partition flags.
+
+// flags[q] = (pbest[q] != 0) for q < S, flags[S] = 0
+__global__ void k_part_used(uint32_t S, const unsigned long long* __restrict__ pbest, uint32_t* __restrict__ flags) {
+  for (uint32_t q = blockIdx.x * WG + threadIdx.x; q <= S; q += gridDim.x * WG) flags[q] = (q < S && pbest[q] != 0ull);
+}
+
+// pivot of every used partition into F[newid[q]], its forward and backward
+// bits set. A key naming no vertex (id >= n) is reported through *err
+// instead of being used as an index.
+__global__ void k_part_emit(uint32_t S, uint32_t n, const unsigned long long* __restrict__ pbest,
+                            const uint32_t* __restrict__ newid, uint32_t* __restrict__ F, uint32_t* fwv, uint32_t* bwv,
+                            uint32_t* err) {
+  for (uint32_t q = blockIdx.x * WG + threadIdx.x; q < S; q += gridDim.x * WG) {
+    const unsigned long long k = pbest[q];
+    if (k == 0ull) continue;
+    const uint32_t v = ~(uint32_t)k;
+    if (v >= n) {
       *err = 1u;
-      v = 0;
-    } else {
-      atomicOr(&fwv[v >> 5], 1u << (v & 31));
-      atomicOr(&bwv[v >> 5], 1u << (v & 31));
+      continue;
     }
-    F[p] = v;
+    F[newid[q]] = v;
+    atomicOr(&fwv[v >> 5], 1u << (v & 31));
+    atomicOr(&bwv[v >> 5], 1u << (v & 31));
   }
 }
 
 // after a round: pivot SCCs (forward and backward reached) contribute their
-// smallest member to pmin[p]; every other live vertex flags its class
-// (0 neither, 1 forward only, 2 backward only) in flags[3p + class]
+// smallest member to pmin[newid[pid]] (aggregated per workgroup in LDS);
+// with pcount, every live vertex counts itself in its partition's
+// (decided, forward-only, backward-only, unreached) counters. Only decided
+// vertices read pid: otherwise a pass over bitmaps.
 __global__ __launch_bounds__(WG) void k_part_classify(uint32_t n, const uint32_t* __restrict__ live,
                                                       const uint32_t* __restrict__ fwv, const uint32_t* __restrict__ bwv,
-                                                      const uint32_t* __restrict__ pid, uint32_t* pmin, uint32_t* flags,
-                                                      uint32_t* pcount) {
+                                                      const uint32_t* __restrict__ pid, const uint32_t* __restrict__ newid,
+                                                      uint32_t* pmin, uint32_t* pcount) {
+  __shared__ uint32_t s_key[AGG_SLOTS];
+  __shared__ uint32_t s_val[AGG_SLOTS];
+  agg_init<uint32_t>(s_key, s_val, 0xFFFFFFFFu);
   const uint64_t pad = ((uint64_t)n + WG - 1) / WG * WG;
   const uint32_t lane = lane_id();
-  // per wavefront (uniform): the partition whose pmin it last lowered and the
-  // flag it last set. Vertex ids grow along a lane's grid-stride steps and
-  // along the lanes, so the first decided vertex a wavefront meets in a
-  // partition is its smallest there: later ones of that partition skip the
-  // atomic, and a flag already set needs no second store.
-  uint32_t last_min = 0xFFFFFFFFu, last_flag = 0xFFFFFFFFu;
   for (uint64_t base = (uint64_t)blockIdx.x * WG; base < pad; base += (uint64_t)gridDim.x * WG) {
     const uint64_t v = base + threadIdx.x;
     const bool is_live = v < n && bm_get(live, (uint32_t)v);
     bool dec = false;
-    uint32_t p = 0, fl = 0xFFFFFFFFu;
+    uint32_t k = 0;
     if (is_live) {
-      p = pid[v];
-      bool f = bm_get(fwv, (uint32_t)v), b = bm_get(bwv, (uint32_t)v);
+      const bool f = bm_get(fwv, (uint32_t)v), b = bm_get(bwv, (uint32_t)v);
       dec = f && b;
-      if (!dec) fl = 3u * p + (f ? 1u : (b ? 2u : 0u));
-      if (pcount) atomicAdd(&pcount[4ull * p + (dec ? 0 : (f ? 1 : (b ? 2 : 3)))], 1u);
+      if (dec || pcount) k = newid[pid[v]];
+      if (pcount) atomicAdd(&pcount[4ull * k + (dec ? 0 : (f ? 1 : (b ? 2 : 3)))], 1u);
     }
-    // pmin: the first decided lane's partition is peeled (its lowest lane holds its minimum)
-    uint64_t dm = __ballot(dec);
+    // the wavefront's largest group (leader's partition): one update with
+    // its first (smallest) member; the other decided lanes update alone
+    const uint64_t dm = __ballot(dec);
     if (dm) {
       const uint32_t leader = __ffsll((unsigned long long)dm) - 1;
-      const uint32_t r = __shfl(p, leader);
-      if (dec && p == r) {
-        if (lane == leader && r != last_min) atomicMin(&pmin[r], (uint32_t)v);
+      const uint32_t r = __shfl(k, leader);
+      if (dec && k == r) {
+        if (lane == leader && !agg_put<uint32_t, false>(s_key, s_val, r, (uint32_t)v)) atomicMin(&pmin[r], (uint32_t)v);
         dec = false;
       }
-      last_min = r;
-      if (dec && pmin[p] > (uint32_t)v) atomicMin(&pmin[p], (uint32_t)v);
-    }
-    // flags: the first flagging lane's flag is peeled, the rest store if unset
-    uint64_t fm = __ballot(fl != 0xFFFFFFFFu);
-    if (fm) {
-      const uint32_t leader = __ffsll((unsigned long long)fm) - 1;
-      const uint32_t r = __shfl(fl, leader);
-      if (fl == r) {
-        if (lane == leader && r != last_flag) flags[r] = 1u;
-        fl = 0xFFFFFFFFu;
-      }
-      last_flag = r;
-      if (fl != 0xFFFFFFFFu && flags[fl] == 0u) flags[fl] = 1u;
+      if (dec && !agg_put<uint32_t, false>(s_key, s_val, k, (uint32_t)v)) atomicMin(&pmin[k], (uint32_t)v);
     }
   }
+  agg_flush<uint32_t, false>(s_key, s_val, pmin);
 }
 
-// decided: label = pmin[p], removed from live (each wavefront owns its two
-// words); others: pid = newid[3p + class]. Per-workgroup decided counts.
+// decided: label = pmin[newid[pid]], removed from live (each wavefront owns
+// its two words); others: pid = 3 * newid[pid] + class. Per-workgroup
+// decided counts.
 __global__ __launch_bounds__(WG) void k_part_apply(uint32_t n, uint32_t* live, const uint32_t* __restrict__ fwv,
                                                    const uint32_t* __restrict__ bwv, uint32_t* __restrict__ pid,
                                                    const uint32_t* __restrict__ pmin, const uint32_t* __restrict__ newid,
@@ -1791,14 +1834,14 @@ __global__ __launch_bounds__(WG) void k_part_apply(uint32_t n, uint32_t* live, c
     const bool is_live = v < n && ((lw >> (v & 31)) & 1u);
     bool dec = false;
     if (is_live) {
-      const uint32_t p = pid[v];
+      const uint32_t k = newid[pid[v]];
       bool f = bm_get(fwv, (uint32_t)v), b = bm_get(bwv, (uint32_t)v);
       if (f && b) {
         dec = true;
-        labels[v] = pmin[p];
+        labels[v] = pmin[k];
         ++count;
       } else {
-        pid[v] = newid[3ull * p + (f ? 1 : (b ? 2 : 0))];
+        pid[v] = 3u * k + (f ? 1u : (b ? 2u : 0u));
       }
     }
     const uint64_t mask = __ballot(dec);
@@ -1814,14 +1857,17 @@ __global__ __launch_bounds__(WG) void k_part_apply(uint32_t n, uint32_t* live, c
   }
 }
 
-// pivot outcome records of one round (KSPN_LOG_WORDS u32 each)
-__global__ void k_part_log(uint32_t np, const unsigned long long* __restrict__ pbest, const uint32_t* __restrict__ pcount,
-                           uint32_t round, uint32_t* __restrict__ log, uint32_t first, uint32_t cap) {
-  for (uint32_t p = blockIdx.x * WG + threadIdx.x; p < np; p += gridDim.x * WG) {
-    if (first + p >= cap) break;
-    uint32_t* r = log + (uint64_t)(first + p) * KSPN_LOG_WORDS;
-    const uint32_t* c = pcount + 4ull * p;
-    r[0] = ~(uint32_t)pbest[p];
+// pivot outcome records of one round (KSPN_LOG_WORDS u32 each), one per used partition
+__global__ void k_part_log(uint32_t S, const unsigned long long* __restrict__ pbest, const uint32_t* __restrict__ newid,
+                           const uint32_t* __restrict__ pcount, uint32_t round, uint32_t* __restrict__ log, uint32_t first,
+                           uint32_t cap) {
+  for (uint32_t q = blockIdx.x * WG + threadIdx.x; q < S; q += gridDim.x * WG) {
+    if (pbest[q] == 0ull) continue;
+    const uint32_t k = newid[q];
+    if (first + k >= cap) continue;
+    uint32_t* r = log + (uint64_t)(first + k) * KSPN_LOG_WORDS;
+    const uint32_t* c = pcount + 4ull * k;
+    r[0] = ~(uint32_t)pbest[q];
     r[1] = round;
     r[2] = c[0] + c[1] + c[2] + c[3];
     r[3] = c[0];
@@ -1998,7 +2044,7 @@ int ctx_reserve(Ctx* c, uint32_t n, uint64_t m) {
     c->cub_bytes = need;
   }
   // buffers sized by cap_n and allocated on first use: drop them so they are reallocated
-  for (uint32_t** p : {&c->tj, &c->pid, &c->pmin, &c->pflags, &c->pnew})
+  for (uint32_t** p : {&c->tj, &c->pid, &c->pmin, &c->pflags, &c->pnew, &c->pcnt})
     if (*p) {
       (void)hipFree(*p);
       *p = nullptr;
@@ -2369,34 +2415,42 @@ int fwbw(Run& R, uint32_t pivot, uint32_t* decided) {
 
 // ---------------------------------------------------------------- partitioned FW-BW rounds
 
+// Pivot keys of the next round over the S sparse partition ids and the
+// "used" flag of every id (pflags)
+void part_keys(Run& R, uint32_t S, const float* score, uint32_t round) {
+  Ctx* c = R.c;
+  const kspn_graph* g = R.g;
+  Prof p(c, R.s, K_PIVOT, 20.0 * R.live_n + 12.0 * S);
+  k_fill_u64<<<stream_grid(c, S), WG, 0, R.s>>>((unsigned long long*)c->pbest, S, 0ull);
+  const uint64_t rs = R.o.pivot == KSPN_PIVOT_RANDOM ? ((uint64_t)R.o.seed * 0x10001ull + round + 1) : 0;
+  k_part_key<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->live, c->pid, g->fw_off, g->bw_off, score, rs,
+                                                  (unsigned long long*)c->pbest);
+  k_part_used<<<stream_grid(c, (uint64_t)S + 1), WG, 0, R.s>>>(S, (unsigned long long*)c->pbest, c->pflags);
+}
+
+// compact index of every used id (newid = c->pnew), their count into cnt[C_AUX]
+int part_compact(Run& R, uint32_t S) {
+  Ctx* c = R.c;
+  KSPN_TRY(exclusive_scan_u32(c, c->pflags, c->pnew, (uint64_t)S + 1, R.s));
+  k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_AUX], c->pnew + S);
+  return KSPN_OK;
+}
+
 int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
   Ctx* c = R.c;
   const kspn_graph* g = R.g;
   const uint64_t n = R.n;
   if (!c->pid) KSPN_TRY(dalloc(&c->pid, (uint64_t)c->cap_n + 64));
-  if (!c->pbest) KSPN_TRY(dalloc(&c->pbest, (uint64_t)c->cap_n + 64));
+  if (!c->pbest) KSPN_TRY(dalloc(&c->pbest, 3ull * c->cap_n + 64));
   if (!c->pmin) KSPN_TRY(dalloc(&c->pmin, (uint64_t)c->cap_n + 64));
   if (!c->pflags) KSPN_TRY(dalloc(&c->pflags, 3ull * c->cap_n + 64));
   if (!c->pnew) KSPN_TRY(dalloc(&c->pnew, 3ull * c->cap_n + 64));
+  // sparse ids of the first round: 0 for every vertex, or the preset ids
+  // (the graphs of a batch; those the first trim emptied are not used)
+  uint32_t S = np;
   if (!R.pid_preset) {
     k_fill_u32<<<stream_grid(c, n), WG, 0, R.s>>>(c->pid, n, 0u);
-    np = 1;
-  } else {
-    // preset partitions (the graphs of a batch) may have lost every vertex
-    // to the first trim pass: renumber those that hold live vertices. With
-    // nothing reached, classify flags class 0 of each such partition and
-    // apply maps pid to the scanned flag (no vertex is decided).
-    const uint32_t gridn = stream_grid(c, n);
-    Prof p(c, R.s, K_BITMAP, 12.0 * R.live_n + 12.0 * np);
-    k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->fwv, R.words, 0u);
-    k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwv, R.words, 0u);
-    k_fill_u32<<<stream_grid(c, 3ull * np + 1), WG, 0, R.s>>>(c->pflags, 3ull * np + 1, 0u);
-    k_part_classify<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->fwv, c->bwv, c->pid, c->pmin, c->pflags, nullptr);
-    KSPN_TRY(exclusive_scan_u32(c, c->pflags, c->pnew, 3ull * np + 1, R.s));
-    k_part_apply<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->fwv, c->bwv, c->pid, c->pmin, c->pnew, R.labels, c->part);
-    k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_AUX], c->pnew + 3ull * np);
-    KSPN_TRY(publish(c, R.s, nullptr));
-    np = c->h_cnt[C_AUX];
+    S = 1;
   }
   const float* score = nullptr;
   if (R.o.pivot == KSPN_PIVOT_LEARNED) {
@@ -2415,60 +2469,62 @@ int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
     }
     score = c->vscore;
   }
+  uint32_t* pcount = nullptr;
+  if (R.o.pivot_log) {
+    if (!c->pcnt) KSPN_TRY(dalloc(&c->pcnt, 4ull * c->cap_n + 64));
+    pcount = c->pcnt;
+  }
+  // the first round's keys and compaction
+  part_keys(R, S, score, 0);
+  KSPN_TRY(part_compact(R, S));
+  KSPN_TRY(publish(c, R.s, nullptr));
+  np = c->h_cnt[C_AUX];
   uint32_t total = 0;
   for (uint32_t round = 0; round < K && R.live_n > 0 && np > 0; ++round) {
     if (R.o.residual_threshold && R.live_n <= R.o.residual_threshold) break;
     const uint32_t gridn = stream_grid(c, n);
     {
-      Prof p(c, R.s, K_PIVOT, 20.0 * R.live_n + 8.0 * np);
-      k_fill_u64<<<stream_grid(c, np), WG, 0, R.s>>>((unsigned long long*)c->pbest, np, 0ull);
-      const uint64_t rs = R.o.pivot == KSPN_PIVOT_RANDOM ? ((uint64_t)R.o.seed * 0x10001ull + round + 1) : 0;
-      k_part_key<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->pid, g->fw_off, g->bw_off, score, rs,
-                                         (unsigned long long*)c->pbest);
+      Prof p(c, R.s, K_PIVOT, 8.0 * np);
       k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->fwv, R.words, 0u);
       k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwv, R.words, 0u);
-      k_part_pivots<<<stream_grid(c, np), WG, 0, R.s>>>(np, R.n, (unsigned long long*)c->pbest, c->qa, c->fwv, c->bwv,
-                                                      &c->cnt[C_ERR]);
+      k_part_emit<<<stream_grid(c, S), WG, 0, R.s>>>(S, R.n, (unsigned long long*)c->pbest, c->pnew, c->qa, c->fwv,
+                                                     c->bwv, &c->cnt[C_ERR]);
       k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], np);
     }
     uint32_t lp = 0, lq = 0;
     KSPN_TRY(reach(R, 0, np, c->pid, &lp, &lq));
     {
       Prof p(c, R.s, K_PIVOT, 8.0 * np);
-      k_part_pivots<<<stream_grid(c, np), WG, 0, R.s>>>(np, R.n, (unsigned long long*)c->pbest, c->qa, c->fwv, c->bwv,
-                                                      &c->cnt[C_ERR]);
+      k_part_emit<<<stream_grid(c, S), WG, 0, R.s>>>(S, R.n, (unsigned long long*)c->pbest, c->pnew, c->qa, c->fwv,
+                                                     c->bwv, &c->cnt[C_ERR]);
       k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], np);
     }
     KSPN_TRY(reach(R, 1, np, c->pid, &lp, &lq));
     {
-      Prof p(c, R.s, K_BITMAP, 12.0 * R.live_n + 16.0 * np);
+      Prof p(c, R.s, K_BITMAP, 0.375 * R.n + 12.0 * np);
       k_fill_u32<<<stream_grid(c, np), WG, 0, R.s>>>(c->pmin, np, 0xFFFFFFFFu);
-      k_fill_u32<<<stream_grid(c, 3ull * np + 1), WG, 0, R.s>>>(c->pflags, 3ull * np + 1, 0u);
-      uint32_t* pcount = nullptr;
-      if (R.o.pivot_log) {
-        pcount = reinterpret_cast<uint32_t*>(c->pbest) + 2ull * np;  // 4 counters per partition after the keys
-        if (4ull * np > 2ull * (c->cap_n + 64) - 2ull * np) pcount = nullptr;
-      }
       if (pcount) k_fill_u32<<<stream_grid(c, 4ull * np), WG, 0, R.s>>>(pcount, 4ull * np, 0u);
-      k_part_classify<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->fwv, c->bwv, c->pid, c->pmin, c->pflags, pcount);
+      k_part_classify<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->fwv, c->bwv, c->pid, c->pnew, c->pmin, pcount);
       if (pcount && R.log_count < R.o.pivot_log_cap) {
-        k_part_log<<<stream_grid(c, np), WG, 0, R.s>>>(np, (unsigned long long*)c->pbest, pcount, round,
+        k_part_log<<<stream_grid(c, S), WG, 0, R.s>>>(S, (unsigned long long*)c->pbest, c->pnew, pcount, round,
                                                       R.o.pivot_log, R.log_count, R.o.pivot_log_cap);
         R.log_count = std::min<uint64_t>((uint64_t)R.log_count + np, R.o.pivot_log_cap);
         if (R.st) R.st->pivot_log_count = R.log_count;
       }
     }
-    KSPN_TRY(exclusive_scan_u32(c, c->pflags, c->pnew, 3ull * np + 1, R.s));
     {
       Prof p(c, R.s, K_LABEL, 16.0 * R.live_n);
       k_part_apply<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->fwv, c->bwv, c->pid, c->pmin, c->pnew, R.labels, c->part);
-      k_reduce3<<<1, WG, 0, R.s>>>(c->part, gridn, &c->cnt[C_DECIDED], c->cnt64);
     }
-    k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_AUX], c->pnew + 3ull * np);  // scan total = new partition count
+    // the next round's keys and compaction ride on this round's publication
+    const uint32_t S_next = 3u * np;
+    const bool more = round + 1 < K;
+    if (more) part_keys(R, S_next, score, round + 1);
+    k_reduce3<<<1, WG, 0, R.s>>>(c->part, gridn, &c->cnt[C_DECIDED], c->cnt64);
+    if (more) KSPN_TRY(part_compact(R, S_next));
     KSPN_TRY(publish(c, R.s, nullptr));
-    if (c->h_cnt[C_ERR]) return KSPN_E_INVALID;  // a partition without a live vertex
+    if (c->h_cnt[C_ERR]) return KSPN_E_INVALID;  // a pivot key that names no vertex
     const uint32_t d = c->h_cnt[C_DECIDED];
-    np = c->h_cnt[C_AUX];
     R.live_n -= d;
     total += d;
     if (R.st) {
@@ -2477,7 +2533,9 @@ int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
       R.st->fwbw_rounds_done++;
       if (np > R.st->partitions_max) R.st->partitions_max = np;
     }
-    if (d == 0) break;
+    if (!more || d == 0) break;
+    S = S_next;
+    np = c->h_cnt[C_AUX];
   }
   *decided_total = total;
   return KSPN_OK;
@@ -3083,7 +3141,7 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   (void)hipSetDevice(c->device);
   void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
                   c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf, c->pid, c->pbest,
-                  c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop};
+                  c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
