@@ -34,6 +34,7 @@ extern "C" {
 #define KSPN_E_HIP (-3)           /* a HIP runtime call failed */
 #define KSPN_E_ITER_CAP (-4)      /* max_rounds reached before every vertex was decided */
 #define KSPN_E_UNIMPLEMENTED (-5) /* option combination not built */
+#define KSPN_E_IO (-6)            /* a file could not be opened or read, or has the wrong size */
 
 #define KSPN_UNDECIDED 0xFFFFFFFFu
 
@@ -49,9 +50,16 @@ typedef struct kspn_graph {
   const uint32_t* bw_adj; /* device, m */
 } kspn_graph;
 
-/* pipeline shapes; all give identical canonical labels */
-#define KSPN_METHOD_KASPAN 0  /* scc.hpp:21-87: trim-1 first pass + pivot, FW-BW, trim-1 normal pass, colour rounds */
-#define KSPN_METHOD_TRIM_EX 1 /* variant/scc_trim_ex*.hpp: exhaustive trim-1 (+trim-2), FW-BW rounds, colour rounds with trims, residual Tarjan */
+/* pipeline shapes, one per variant of the reference's driver
+ * (tool/bench_kaspan/src/main.cpp:63-71), at one rank; all give identical
+ * canonical labels. "residual" stages decide everything left. */
+#define KSPN_METHOD_KASPAN 0  /* scc.hpp:21-87 (also async/scc.hpp): trim-1 first pass + pivot, FW-BW, trim-1 normal pass, colour rounds */
+#define KSPN_METHOD_TRIM_EX 1 /* variant/scc_trim_ex.hpp: exhaustive trim-1 (+trim-2), FW-BW rounds, colour rounds with trims */
+#define KSPN_METHOD_HPC_LIKE 2               /* variant/scc_hpc_like.hpp: trim-1 first pass, FW-BW, colour rounds */
+#define KSPN_METHOD_ISPAN_LIKE 3             /* variant/scc_ispan_like.hpp: trim-1 first pass, FW-BW, two trim-1 normal passes, residual */
+#define KSPN_METHOD_HPC_TRIM_EX 4            /* variant/scc_hpc_trim_ex.hpp: exhaustive trim-1, pivot, FW-BW, colour rounds */
+#define KSPN_METHOD_TRIM_EX_RESIDUAL 5       /* variant/scc_trim_ex_residual.hpp: exhaustive trim-1, residual */
+#define KSPN_METHOD_TRIM_EX_LIGHT_RESIDUAL 6 /* variant/scc_trim_ex_light_residual.hpp: exhaustive trim-1, trim after decisions, residual */
 
 #define KSPN_PIVOT_PLAIN 0   /* max in*out degree product, smallest id on ties (trim_1_local.hpp:32-45) */
 #define KSPN_PIVOT_LEARNED 1 /* MLP score (kspn_mlp_*), plain rule as fallback */
@@ -62,6 +70,7 @@ typedef struct kspn_graph {
 #define KSPN_FLAG_NO_TAIL 4u  /* never hand the rest to the single-workgroup tail solver */
 #define KSPN_FLAG_NO_REFINE 8u /* colour rounds over the whole live graph, without partition refinement */
 #define KSPN_FLAG_NO_COOP 16u  /* every FW/BW level as separate launches (no persistent multi-level kernel) */
+#define KSPN_FLAG_STATS_MEMORY 32u /* fill kspn_phase.memory (one hipMemGetInfo per scope that carries it) */
 
 typedef struct kspn_opts {
   uint32_t method;             /* KSPN_METHOD_* */
@@ -95,6 +104,21 @@ typedef struct kspn_kern_stat {
   double bytes; /* summed algorithmic bytes (each array element touched once, stored width) */
 } kspn_kern_stat;
 
+/* one statistics scope of the reference (debug/statistic.hpp:124-205) as the
+ * variant's code names it (scc.hpp:33-86, variant/scc_*.hpp): "trim_1_first",
+ * "trim_ex_first", "trim_2", "select_pivot", "forward_backward_search",
+ * "trim_1_normal", "trim_ex", "color", "color_trim_ex", "residual". A scope
+ * entered more than once (trims between FW-BW rounds) accumulates. */
+#define KSPN_PHASE_MAX 12
+typedef struct kspn_phase {
+  char name[32];
+  double ms;           /* GPU time (HIP events on the solve's stream), summed over entries */
+  uint64_t decided;    /* vertices the scope decided ("decided_count") */
+  uint64_t memory;     /* device bytes in use when the scope ended (hipMemGetInfo), if has_memory */
+  uint32_t entries;    /* times the scope was entered */
+  uint32_t has_memory; /* the reference records "memory" for this scope */
+} kspn_phase;
+
 typedef struct kspn_stats {
   double ms_total, ms_trim, ms_fwbw, ms_color, ms_residual, ms_canon;
   uint32_t decided_trim, decided_trim2, decided_fwbw, decided_color, decided_residual;
@@ -111,6 +135,8 @@ typedef struct kspn_stats {
   double bytes_trim, bytes_fwbw, bytes_color, bytes_residual, bytes_canon; /* algorithmic bytes */
   uint32_t n_kern;
   kspn_kern_stat kern[KSPN_KERN_MAX];
+  uint32_t n_phase;                   /* scopes in order of first entry */
+  kspn_phase phase[KSPN_PHASE_MAX];
 } kspn_stats;
 
 /* defaults: KASPAN method, plain pivot, 1 FW-BW round, no residual, no cap */
@@ -259,6 +285,57 @@ int kspn_batch_plan(const kspn_batch_gen* p, uint32_t* kind, uint32_t* n, uint64
 int kspn_generate_batch(kspn_ctx* ctx, const kspn_batch_gen* p, kspn_batch* out, void* stream);
 /* canonical labels of a KSPN_GEN_PLANTED graph, known by construction (device labels[n]) */
 int kspn_planted_labels(kspn_ctx* ctx, const kspn_gen_params* p, uint32_t* labels, void* stream);
+
+/* ---- graphs stored as a manifest (scc/adapter/manifest.hpp) -------------
+ * A text file of "key value" lines ('%' starts a comment line) naming four
+ * binary files of dense unsigned integers: fw/bw row offsets (n+1 elements
+ * of graph.head.bytes each) and fw/bw adjacency (m elements of
+ * graph.csr.bytes each), in graph.endian byte order. Written by the
+ * reference's convert_graph (scc/adapter/external_edgelist.hpp:348-367). */
+#define KSPN_MANIFEST_TEXT 256
+#define KSPN_MANIFEST_PATH 4096
+typedef struct kspn_manifest {
+  uint32_t schema_version;           /* schema.version */
+  uint32_t big_endian;               /* graph.endian: 0 little, 1 big */
+  uint64_t node_count;               /* graph.node_count */
+  uint64_t edge_count;               /* graph.edge_count */
+  uint32_t contains_self_loops;      /* graph.contains_self_loops */
+  uint32_t contains_duplicate_edges; /* graph.contains_duplicate_edges */
+  uint32_t head_bytes;               /* graph.head.bytes */
+  uint32_t csr_bytes;                /* graph.csr.bytes */
+  char code[KSPN_MANIFEST_TEXT];     /* graph.code */
+  char name[KSPN_MANIFEST_TEXT];     /* graph.name */
+  /* {fw,bw}.{head,csr}.path resolved against the manifest's directory */
+  char fw_head_path[KSPN_MANIFEST_PATH];
+  char fw_csr_path[KSPN_MANIFEST_PATH];
+  char bw_head_path[KSPN_MANIFEST_PATH];
+  char bw_csr_path[KSPN_MANIFEST_PATH];
+} kspn_manifest;
+
+/* manifest::load (scc/adapter/manifest.hpp:136-197): every key present,
+ * numbers as whole unsigned decimals, booleans "true"/"false", endian
+ * "little"/"big", each path a regular file. KSPN_E_INVALID for a malformed
+ * manifest, KSPN_E_IO for a missing file; kspn_last_error names the key. */
+int kspn_manifest_parse(const char* path, kspn_manifest* out);
+/* load_graph_from_manifest (scc/adapter/manifest.hpp:200-250) into caller
+ * device buffers: fw_off/bw_off hold node_count+1 entries, fw_adj/bw_adj
+ * edge_count. Each file must hold exactly its elements' bytes (KSPN_E_IO
+ * otherwise). Every offset must be <= edge_count and every id <
+ * node_count; offsets must start at 0, end at edge_count and not decrease
+ * (KSPN_E_INVALID otherwise). Synchronises the stream before returning. */
+int kspn_manifest_load(kspn_ctx* ctx, const kspn_manifest* mf, uint64_t* fw_off, uint32_t* fw_adj,
+                       uint64_t* bw_off, uint32_t* bw_adj, void* stream);
+/* kspn_manifest_load into arrays the library allocates with hipMalloc
+ * (free each with kspn_free); *n and *m receive the graph's sizes */
+int kspn_manifest_graph(kspn_ctx* ctx, const kspn_manifest* mf, uint32_t* n, uint64_t* m, uint64_t** fw_off,
+                        uint32_t** fw_adj, uint64_t** bw_off, uint32_t** bw_adj, void* stream);
+/* dense_unsigned_accessor::get over a device array
+ * (memory/dense_unsigned_accessor.hpp:15-110, dense_unsigned_ops.hpp:14-44):
+ * out[i] (out_bytes 4 or 8) = element i of `packed` (elem_bytes 1..8, byte
+ * order big_endian ? big : little). KSPN_E_INVALID if any value exceeds
+ * `limit`. Synchronises the stream before returning. */
+int kspn_unpack_dense(kspn_ctx* ctx, const void* packed, uint64_t count, uint32_t elem_bytes, uint32_t big_endian,
+                      uint64_t limit, void* out, uint32_t out_bytes, void* stream);
 
 #ifdef __cplusplus
 }

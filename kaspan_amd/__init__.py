@@ -10,6 +10,12 @@ kaspan_amd                             reference
 ``scc(graph, scc_id)``                 ``scc(bidi_graph_part_view, vertex_t* scc_id)`` (scc/scc.hpp:21-87)
 ``scc_trim_ex(graph, scc_id)``         ``scc_trim_ex`` (scc/variant/scc_trim_ex.hpp:16-100)
 ``scc_trim_ex_residual(graph, ...)``   ``scc_trim_ex_residual`` (scc/variant/scc_trim_ex_residual.hpp:17-128)
+``scc_trim_ex_light_residual``         ``scc_trim_ex_light_residual`` (scc/variant/scc_trim_ex_light_residual.hpp:16-131)
+``scc_hpc_like`` / ``scc_hpc_trim_ex``  ``scc_hpc_like`` / ``scc_hpc_trim_ex`` (scc/variant/scc_hpc_*.hpp)
+``scc_ispan_like``                     ``scc_ispan_like`` (scc/variant/scc_ispan_like.hpp:18-87)
+``scc_variant(name, graph)``           the driver's ``--variant`` switch (tool/bench_kaspan/src/main.cpp:63-71)
+``manifest.load_manifest(path)``       ``manifest::load`` + ``load_graph_from_manifest`` (scc/adapter/manifest.hpp)
+``statistics.statistics_json``         ``KASPAN_STATISTIC_*`` tree written as JSON (debug/statistic.hpp:124-205)
 ``backward_complement_graph``          ``backward_complement_graph`` (scc/backward_complement.hpp:22-68)
 ``allgather_fw_sub_graph``             ``allgather_fw_sub_graph`` at one rank (scc/allgather_sub_graph.hpp:253-301)
 ``tarjan``                             ``tarjan`` with min-id callback (scc/tarjan.hpp:124-136)
@@ -37,11 +43,23 @@ __all__ = [
     "BidiGraph", "Context", "KspnError", "Stats", "scc", "scc_trim_ex", "scc_trim_ex_residual",
     "backward_complement_graph", "allgather_fw_sub_graph", "tarjan", "normalise_scc_id", "generate",
     "METHOD_KASPAN", "METHOD_TRIM_EX", "UNDECIDED", "planted_labels", "scc_sizes", "Batch", "generate_batch",
-    "batch_plan", "scc_batch",
+    "batch_plan", "scc_batch", "scc_hpc_like", "scc_hpc_trim_ex", "scc_ispan_like", "scc_trim_ex_light_residual",
+    "scc_variant", "VARIANTS",
 ]
 
 METHOD_KASPAN = _lib.METHOD_KASPAN
 METHOD_TRIM_EX = _lib.METHOD_TRIM_EX
+# the driver's --variant names (tool/bench_kaspan/src/main.cpp:63-71); None is kaspan::scc
+VARIANTS = {
+    None: _lib.METHOD_KASPAN,
+    "async": _lib.METHOD_KASPAN,
+    "hpc_like": _lib.METHOD_HPC_LIKE,
+    "hpc_trim_ex": _lib.METHOD_HPC_TRIM_EX,
+    "ispan_like": _lib.METHOD_ISPAN_LIKE,
+    "trim_ex": _lib.METHOD_TRIM_EX,
+    "trim_ex_light_residual": _lib.METHOD_TRIM_EX_LIGHT_RESIDUAL,
+    "trim_ex_residual": _lib.METHOD_TRIM_EX_RESIDUAL,
+}
 UNDECIDED = _lib.UNDECIDED
 
 
@@ -269,9 +287,39 @@ def scc_trim_ex(graph: BidiGraph, scc_id=None, **kw):
 
 def scc_trim_ex_residual(graph: BidiGraph, scc_id=None, **kw):
     """scc_trim_ex_residual at one rank (scc/variant/scc_trim_ex_residual.hpp:17-128): exhaustive
-    trim, then Tarjan on everything left (the one-rank decided threshold is 0, :47)."""
-    kw.setdefault("residual_threshold", 0xFFFFFFFF)
-    return scc(graph, scc_id, method=METHOD_TRIM_EX, **kw)
+    trim, then everything left is the residual (the one-rank decided threshold is 0, :47)."""
+    return scc(graph, scc_id, method=_lib.METHOD_TRIM_EX_RESIDUAL, **kw)
+
+
+def scc_trim_ex_light_residual(graph: BidiGraph, scc_id=None, **kw):
+    """scc_trim_ex_light_residual at one rank (scc/variant/scc_trim_ex_light_residual.hpp:16-131):
+    exhaustive trim, the trim over decisions since (none at one rank), residual."""
+    return scc(graph, scc_id, method=_lib.METHOD_TRIM_EX_LIGHT_RESIDUAL, **kw)
+
+
+def scc_hpc_like(graph: BidiGraph, scc_id=None, **kw):
+    """scc_hpc_like (scc/variant/scc_hpc_like.hpp:16-69): trim_1_first, FW-BW, colour rounds."""
+    return scc(graph, scc_id, method=_lib.METHOD_HPC_LIKE, **kw)
+
+
+def scc_hpc_trim_ex(graph: BidiGraph, scc_id=None, **kw):
+    """scc_hpc_trim_ex (scc/variant/scc_hpc_trim_ex.hpp:16-76): exhaustive trim, pivot, FW-BW, colour."""
+    return scc(graph, scc_id, method=_lib.METHOD_HPC_TRIM_EX, **kw)
+
+
+def scc_ispan_like(graph: BidiGraph, scc_id=None, **kw):
+    """scc_ispan_like (scc/variant/scc_ispan_like.hpp:18-87): trim_1_first, FW-BW, two trim_1_normal
+    passes, residual."""
+    return scc(graph, scc_id, method=_lib.METHOD_ISPAN_LIKE, **kw)
+
+
+def scc_variant(variant: str | None, graph: BidiGraph, scc_id=None, **kw):
+    """the driver's variant switch (tool/bench_kaspan/src/main.cpp:63-71): ``variant`` is one of
+    ``VARIANTS`` (None: kaspan::scc). ``scc_trim_ex`` runs the trim-2 pass unless
+    ``flags`` has ``FLAG_NO_TRIM2``; the other variants never do."""
+    if variant not in VARIANTS:
+        raise KspnError(-1, f"unknown variant {variant!r}; one of {sorted(k for k in VARIANTS if k)}")
+    return scc(graph, scc_id, method=VARIANTS[variant], **kw)
 
 
 def backward_complement_graph(n, fw_off: torch.Tensor, fw_adj: torch.Tensor, ctx: Context | None = None, stream=None):

@@ -22,9 +22,15 @@ ERRORS = {
     -3: "KSPN_E_HIP",
     -4: "KSPN_E_ITER_CAP",
     -5: "KSPN_E_UNIMPLEMENTED",
+    -6: "KSPN_E_IO",
 }
 METHOD_KASPAN = 0
 METHOD_TRIM_EX = 1
+METHOD_HPC_LIKE = 2
+METHOD_ISPAN_LIKE = 3
+METHOD_HPC_TRIM_EX = 4
+METHOD_TRIM_EX_RESIDUAL = 5
+METHOD_TRIM_EX_LIGHT_RESIDUAL = 6
 PIVOT_PLAIN = 0
 PIVOT_LEARNED = 1
 PIVOT_RANDOM = 2
@@ -34,6 +40,10 @@ FLAG_NO_TRIM2 = 2
 FLAG_NO_TAIL = 4
 FLAG_NO_REFINE = 8
 FLAG_NO_COOP = 16
+FLAG_STATS_MEMORY = 32
+PHASE_MAX = 12
+MANIFEST_TEXT = 256
+MANIFEST_PATH = 4096
 GEN_GNM, GEN_RMAT, GEN_LATTICE, GEN_PLANTED = 0, 1, 2, 3
 MLP_PARAMS = 1313
 UNDECIDED = 0xFFFFFFFF
@@ -44,7 +54,7 @@ EXPORTS = (
     "kspn_scc", "kspn_scc_host", "kspn_transpose", "kspn_canonicalize", "kspn_subgraph", "kspn_tarjan",
     "kspn_generate", "kspn_build_csr", "kspn_free", "kspn_planted_labels", "kspn_pivot_features",
     "kspn_mlp_forward", "kspn_mlp_grad", "kspn_mlp_adam", "kspn_scc_sizes", "kspn_scc_batch", "kspn_batch_plan",
-    "kspn_generate_batch",
+    "kspn_generate_batch", "kspn_manifest_parse", "kspn_manifest_load", "kspn_manifest_graph", "kspn_unpack_dense",
 )
 
 
@@ -119,6 +129,11 @@ class KernStat(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint32), ("ms", C.c_double), ("bytes", C.c_double)]
 
 
+class Phase(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("ms", C.c_double), ("decided", C.c_uint64), ("memory", C.c_uint64),
+                ("entries", C.c_uint32), ("has_memory", C.c_uint32)]
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("ms_total", C.c_double), ("ms_trim", C.c_double), ("ms_fwbw", C.c_double), ("ms_color", C.c_double),
@@ -137,16 +152,44 @@ class Stats(C.Structure):
         ("bytes_residual", C.c_double), ("bytes_canon", C.c_double),
         ("n_kern", C.c_uint32),
         ("kern", KernStat * 24),
+        ("n_phase", C.c_uint32),
+        ("phase", Phase * PHASE_MAX),
     ]
 
+    def phases(self):
+        """the statistics scopes in order of first entry"""
+        return [{"name": p.name.decode(), "ms": p.ms, "decided": p.decided, "memory": p.memory,
+                 "entries": p.entries, "has_memory": bool(p.has_memory)}
+                for p in self.phase[: self.n_phase]]
+
     def as_dict(self):
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("kern", "n_kern")}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("kern", "n_kern", "phase", "n_phase")}
+        d["phases"] = self.phases()
         d["kernels"] = [
             {"name": self.kern[i].name.decode(), "launches": self.kern[i].launches, "ms": self.kern[i].ms,
              "bytes": self.kern[i].bytes}
             for i in range(self.n_kern)
         ]
         return d
+
+
+class Manifest(C.Structure):
+    _fields_ = [
+        ("schema_version", C.c_uint32),
+        ("big_endian", C.c_uint32),
+        ("node_count", C.c_uint64),
+        ("edge_count", C.c_uint64),
+        ("contains_self_loops", C.c_uint32),
+        ("contains_duplicate_edges", C.c_uint32),
+        ("head_bytes", C.c_uint32),
+        ("csr_bytes", C.c_uint32),
+        ("code", C.c_char * MANIFEST_TEXT),
+        ("name", C.c_char * MANIFEST_TEXT),
+        ("fw_head_path", C.c_char * MANIFEST_PATH),
+        ("fw_csr_path", C.c_char * MANIFEST_PATH),
+        ("bw_head_path", C.c_char * MANIFEST_PATH),
+        ("bw_csr_path", C.c_char * MANIFEST_PATH),
+    ]
 
 
 class GenParams(C.Structure):
@@ -200,6 +243,10 @@ def lib():
         "kspn_scc_batch": (i32, [vp, P(Batch), vp, P(Opts), P(Stats), vp]),
         "kspn_batch_plan": (i32, [P(BatchGen), vp, vp, vp, vp]),
         "kspn_generate_batch": (i32, [vp, P(BatchGen), P(Batch), vp]),
+        "kspn_manifest_parse": (i32, [C.c_char_p, P(Manifest)]),
+        "kspn_manifest_load": (i32, [vp, P(Manifest), vp, vp, vp, vp, vp]),
+        "kspn_manifest_graph": (i32, [vp, P(Manifest), P(u32), P(u64), P(vp), P(vp), P(vp), P(vp), vp]),
+        "kspn_unpack_dense": (i32, [vp, vp, u64, u32, u32, u64, vp, u32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -209,9 +256,11 @@ def lib():
     return L
 
 
-def check(rc, where):
+def check(rc, where, detail=False):
+    """raise KspnError for a failing call; the library's last error message is
+    appended for HIP and file errors, and for every error when `detail`"""
     if rc != KSPN_OK:
-        if rc == -3:
+        if rc in (-3, -6) or detail:
             detail = lib().kspn_last_error()
             if detail:
                 where = f"{where} [{detail.decode(errors='replace')}]"

@@ -24,6 +24,7 @@ constexpr uint32_t UNDEC = KSPN_UNDECIDED;
 
 // records the failing call for kspn_last_error (thread-local)
 void note_error(const char* what, const char* detail, const char* file, int line);
+void clear_error();
 
 #define KSPN_HIP(expr)                                                         \
   do {                                                                         \

@@ -1570,11 +1570,16 @@ __global__ void k_res_apply(const uint32_t* __restrict__ U, const uint32_t* ucou
   for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < u; i += gridDim.x * WG) labels[U[i]] = U[sublab[i]];
 }
 
-// pivot among live vertices: max original in*out, smallest id on ties
-// (select_pivot_from_head, scc/pivot.hpp:28-47, restricted to one rank)
+// pivot among live vertices: max in*out, smallest id on ties. LIVE = false:
+// original degrees (select_pivot_from_head, scc/pivot.hpp:28-47); LIVE =
+// true: live neighbours only (select_pivot_from_degree, scc/pivot.hpp:53-74,
+// whose degree arrays count exactly the live neighbours after an exhaustive
+// trim). Restricted to one rank.
+template <bool LIVE>
 __global__ __launch_bounds__(WG) void k_pivot_live(const uint32_t* __restrict__ U, const uint32_t* ucount,
-                                                   const uint64_t* __restrict__ fw_off, const uint64_t* __restrict__ bw_off,
-                                                   uint64_t* __restrict__ part) {
+                                                   const uint64_t* __restrict__ fw_off, const uint32_t* __restrict__ fw_adj,
+                                                   const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
+                                                   const uint32_t* __restrict__ live, uint64_t* __restrict__ part) {
   __shared__ uint64_t s_p[WG];
   __shared__ uint32_t s_id[WG];
   uint64_t best = 0;
@@ -1582,7 +1587,15 @@ __global__ __launch_bounds__(WG) void k_pivot_live(const uint32_t* __restrict__ 
   uint32_t u = *ucount;
   for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < u; i += gridDim.x * WG) {
     uint32_t v = U[i];
-    uint64_t p = (fw_off[v + 1] - fw_off[v]) * (bw_off[v + 1] - bw_off[v]);
+    uint64_t p;
+    if (LIVE) {
+      uint64_t dout = 0, din = 0;
+      for (uint64_t it = fw_off[v], e = fw_off[v + 1]; it < e; ++it) dout += bm_get(live, fw_adj[it]);
+      for (uint64_t it = bw_off[v], e = bw_off[v + 1]; it < e; ++it) din += bm_get(live, bw_adj[it]);
+      p = dout * din;
+    } else {
+      p = (fw_off[v + 1] - fw_off[v]) * (bw_off[v + 1] - bw_off[v]);
+    }
     if (best_id == 0xFFFFFFFFu || p > best || (p == best && v < best_id)) { best = p; best_id = v; }
   }
   s_p[threadIdx.x] = best;
@@ -1928,12 +1941,14 @@ int dalloc(T** p, uint64_t count) {
 
 }  // namespace
 
-thread_local char g_last_error[256] = "";
+thread_local char g_last_error[1024] = "";
 
 void note_error(const char* what, const char* detail, const char* file, int line) {
   const char* base = std::strrchr(file, '/');
   std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%s:%d)", what, detail, base ? base + 1 : file, line);
 }
+
+void clear_error() { g_last_error[0] = 0; }
 
 namespace {
 
@@ -2164,6 +2179,7 @@ struct Run {
   unsigned long long coop_edges = 0;  // frontiers the persistent kernel takes (degree sum, largest degree)
   uint32_t coop_deg = 0;
   uint32_t coop_wg = 0;               // its workgroups
+  bool pivot_live_deg = false;        // pivots by live degree product (after an exhaustive trim)
 };
 
 int select_pivot_live(Run& R, uint32_t* pivot);
@@ -2879,7 +2895,12 @@ int select_pivot_live(Run& R, uint32_t* pivot) {
   uint32_t nparts = std::min<uint32_t>(stream_grid(c, u), (uint32_t)c->num_cu * 16);
   {
     Prof p(c, R.s, K_PIVOT, 20.0 * u);
-    k_pivot_live<<<nparts, WG, 0, R.s>>>(c->ulist, &c->cnt[C_AUX], R.g->fw_off, R.g->bw_off, c->part);
+    if (R.pivot_live_deg)
+      k_pivot_live<true><<<nparts, WG, 0, R.s>>>(c->ulist, &c->cnt[C_AUX], R.g->fw_off, R.g->fw_adj, R.g->bw_off,
+                                                 R.g->bw_adj, c->live, c->part);
+    else
+      k_pivot_live<false><<<nparts, WG, 0, R.s>>>(c->ulist, &c->cnt[C_AUX], R.g->fw_off, R.g->fw_adj, R.g->bw_off,
+                                                  R.g->bw_adj, c->live, c->part);
     k_pivot_reduce<<<1, WG, 0, R.s>>>(c->part, nparts, &c->cnt[C_FLAG], nullptr);
   }
   KSPN_TRY(read_counters(c, R.s, C_FLAG + 1));
@@ -2888,19 +2909,132 @@ int select_pivot_live(Run& R, uint32_t* pivot) {
   return KSPN_OK;
 }
 
+// Statistics scopes of the reference (debug/statistic.hpp:124-205), timed
+// with HIP events on the solve's stream: begin() closes the scope before it,
+// end() closes the last one, finish() (after the stream is synchronised)
+// turns consecutive marks into scope times. Events come from the context's
+// pool and go back to it.
+struct Phases {
+  Ctx* c;
+  kspn_stats* st;
+  hipStream_t s;
+  bool memory;  // record device memory in use at the end of scopes that carry it
+  std::vector<std::pair<int, hipEvent_t>> marks;
+  int cur = -1;
+  Phases(Ctx* c_, kspn_stats* st_, hipStream_t s_, bool mem) : c(c_), st(st_), s(s_), memory(mem) {}
+  ~Phases() {
+    for (auto& m : marks) c->free_events.push_back(m.second);
+  }
+  int mark(int idx) {
+    hipEvent_t e = nullptr;
+    if (!c->free_events.empty()) {
+      e = c->free_events.back();
+      c->free_events.pop_back();
+    } else {
+      KSPN_HIP(hipEventCreate(&e));
+    }
+    marks.push_back({idx, e});
+    KSPN_HIP(hipEventRecord(e, s));
+    return KSPN_OK;
+  }
+  void close_scope() {
+    if (cur >= 0 && memory && st->phase[cur].has_memory) {
+      size_t fr = 0, tot = 0;
+      if (hipMemGetInfo(&fr, &tot) == hipSuccess) st->phase[cur].memory = (uint64_t)(tot - fr);
+    }
+  }
+  int begin(const char* name, bool has_memory) {
+    if (!st) return KSPN_OK;
+    close_scope();
+    int idx = -1;
+    for (uint32_t i = 0; i < st->n_phase; ++i)
+      if (std::strcmp(st->phase[i].name, name) == 0) idx = (int)i;
+    if (idx < 0 && st->n_phase < KSPN_PHASE_MAX) {
+      idx = (int)st->n_phase++;
+      kspn_phase& ph = st->phase[idx];
+      std::memset(&ph, 0, sizeof(ph));
+      std::snprintf(ph.name, sizeof(ph.name), "%s", name);
+      ph.has_memory = has_memory ? 1u : 0u;
+    }
+    if (idx >= 0) st->phase[idx].entries++;
+    cur = idx;
+    return mark(idx);
+  }
+  void decided(uint64_t d) {
+    if (st && cur >= 0) st->phase[cur].decided += d;
+  }
+  int end() {
+    if (!st) return KSPN_OK;
+    close_scope();
+    cur = -1;
+    return mark(-1);
+  }
+  void finish() {
+    if (!st) return;
+    for (size_t i = 0; i + 1 < marks.size(); ++i)
+      if (marks[i].first >= 0) st->phase[marks[i].first].ms += elapsed_ms(marks[i].second, marks[i + 1].second);
+  }
+};
+
+// The stage sequence of each variant at one rank. Scope names follow the
+// variant's own statistics (scc.hpp:33-86, variant/scc_*.hpp).
+struct Plan {
+  bool ex_first;       // exhaustive trim-1 first (trim_1_exhaustive_first) instead of one trim_1_first pass
+  bool trim_ex;        // scc_trim_ex: trim-2, and exhaustive trims after FW-BW and after every colour round
+  bool fwbw;           // pivot + FW-BW; the residual variants skip it at one rank, where their
+                       // decided threshold n - n/1 is 0 (scc_trim_ex_residual.hpp:47, light :36)
+  int after;           // after FW-BW: 0 nothing, 1 trim_1_normal, 2 two trim_1_normal passes, 3 exhaustive
+  bool residual_rest;  // everything left after the trims is the "residual" scope
+  bool light;          // scc_trim_ex_light_residual: trim over the decided stack (empty here) first
+};
+
+Plan plan_for(uint32_t method) {
+  switch (method) {
+    case KSPN_METHOD_TRIM_EX: return {true, true, true, 3, false, false};
+    case KSPN_METHOD_HPC_LIKE: return {false, false, true, 0, false, false};
+    case KSPN_METHOD_ISPAN_LIKE: return {false, false, true, 2, true, false};
+    case KSPN_METHOD_HPC_TRIM_EX: return {true, false, true, 0, false, false};
+    case KSPN_METHOD_TRIM_EX_RESIDUAL: return {true, false, false, 0, true, false};
+    case KSPN_METHOD_TRIM_EX_LIGHT_RESIDUAL: return {true, false, false, 0, true, true};
+    default: return {false, false, true, 1, false, false};  // KSPN_METHOD_KASPAN
+  }
+}
+
 int run_pipeline(Run& R) {
   Ctx* c = R.c;
   const kspn_graph* g = R.g;
   kspn_stats* st = R.st;
-  hipEvent_t ev[8];
-  bool timing = st != nullptr;
-  if (timing)
-    for (auto& e : ev) KSPN_HIP(hipEventCreate(&e));
-  auto mark = [&](int i) {
-    if (timing) (void)hipEventRecord(ev[i], R.s);
+  const Plan plan = plan_for(R.o.method);
+  R.pivot_live_deg = plan.ex_first;
+  Phases P(c, st, R.s, (R.o.flags & KSPN_FLAG_STATS_MEMORY) != 0);
+  // coarse stage boundaries for the ms_* fields (marks 0..5)
+  std::vector<hipEvent_t> ev;
+  auto mark = [&](int) -> int {
+    if (!st) return KSPN_OK;
+    hipEvent_t e = nullptr;
+    if (!c->free_events.empty()) {
+      e = c->free_events.back();
+      c->free_events.pop_back();
+    } else {
+      KSPN_HIP(hipEventCreate(&e));
+    }
+    ev.push_back(e);
+    KSPN_HIP(hipEventRecord(e, R.s));
+    return KSPN_OK;
   };
-  mark(0);
+  struct Give {
+    Ctx* c;
+    std::vector<hipEvent_t>& ev;
+    ~Give() {
+      for (auto e : ev) c->free_events.push_back(e);
+    }
+  } give{c, ev};
+  const char* first_name = plan.ex_first ? "trim_ex_first" : "trim_1_first";
+  const char* after_name = plan.after == 3 ? "trim_ex" : "trim_1_normal";
+  const char* rest_name = plan.residual_rest ? "residual" : (plan.trim_ex ? "color_trim_ex" : "color");
+  KSPN_TRY(mark(0));
   // ---- trim-1 first pass + pivot
+  KSPN_TRY(P.begin(first_name, plan.ex_first));
   KSPN_HIP(hipMemsetAsync(c->cnt, 0, sizeof(uint32_t) * C_COUNT_, R.s));
   KSPN_HIP(hipMemsetAsync(c->cnt64, 0, sizeof(unsigned long long) * 4, R.s));
   c->h_u64[1] = 0;
@@ -2919,48 +3053,71 @@ int run_pipeline(Run& R) {
   uint32_t decided = c->h_cnt[C_DECIDED];
   uint32_t pivot = c->h_cnt[C_FLAG];
   R.live_n = R.n - decided;
+  P.decided(decided);
   if (st) {
     st->decided_trim = decided;
     st->pivot = pivot;
   }
-  const bool trim_ex = R.o.method == KSPN_METHOD_TRIM_EX;
-  if (trim_ex && R.live_n > 0) {
+  if (plan.ex_first && R.live_n > 0) {
     uint32_t d = 0;
     KSPN_TRY(trim_exhaustive(R, &d));
     if (st) st->decided_trim += d;
-    if (!(R.o.flags & KSPN_FLAG_NO_TRIM2) && R.live_n > 0) {
+    P.decided(d);
+    if (plan.trim_ex && !(R.o.flags & KSPN_FLAG_NO_TRIM2) && R.live_n > 0) {
+      KSPN_TRY(P.begin("trim_2", false));
       KSPN_TRY(trim_two(R, &d));
       if (st) st->decided_trim2 += d;
+      P.decided(d);
       if (d && R.live_n > 0) {
         KSPN_TRY(trim_exhaustive(R, &d));
         if (st) st->decided_trim += d;
+        P.decided(d);
       }
     }
   }
-  mark(1);
+  KSPN_TRY(mark(1));
+  // trims after a FW-BW round
+  auto after_trim = [&]() -> int {
+    if (plan.after == 0 || R.live_n == 0) return KSPN_OK;
+    KSPN_TRY(P.begin(after_name, false));
+    uint32_t d = 0;
+    if (plan.after == 3) {
+      KSPN_TRY(trim_exhaustive(R, &d));
+    } else {
+      KSPN_TRY(trim_normal(R, &d));
+      if (plan.after == 2 && R.live_n > 0) {
+        uint32_t d2 = 0;
+        KSPN_TRY(trim_normal(R, &d2));
+        d += d2;
+      }
+    }
+    if (st) st->decided_trim += d;
+    P.decided(d);
+    return KSPN_OK;
+  };
   // ---- FW-BW rounds
-  uint32_t rounds = R.o.fwbw_rounds ? R.o.fwbw_rounds : 1;
-  if ((rounds > 1 || R.pid_preset) && R.live_n > 0) {
+  uint32_t rounds = plan.fwbw ? (R.o.fwbw_rounds ? R.o.fwbw_rounds : 1) : 0;
+  if (plan.fwbw && (rounds > 1 || R.pid_preset) && R.live_n > 0) {
     // multi-pivot: one pivot per partition per round
     uint32_t d = 0;
+    KSPN_TRY(P.begin("forward_backward_search", true));
     KSPN_TRY(pfwbw_rounds(R, rounds, R.np0, &d));
     if (st) st->decided_fwbw += d;
+    P.decided(d);
     R.pid_active = true;
-    if (R.live_n > 0) {
-      if (trim_ex) KSPN_TRY(trim_exhaustive(R, &d));
-      else KSPN_TRY(trim_normal(R, &d));
-      if (st) st->decided_trim += d;
-    }
+    KSPN_TRY(after_trim());
     rounds = 0;  // the single-pivot loop below does nothing
   }
   for (uint32_t k = 0; k < rounds && R.live_n > 0; ++k) {
     if (R.o.residual_threshold && R.live_n <= R.o.residual_threshold) break;
     if (R.o.pivot == KSPN_PIVOT_LEARNED) {
       // MLP over the top degree-product candidates (the plain pivot is one of them)
+      KSPN_TRY(P.begin("select_pivot", false));
       KSPN_TRY(select_pivot_learned_impl(c, g, R.o.mlp_params, R.s, &pivot));
       if (k == 0 && st) st->pivot = pivot;
-    } else if (k > 0 || trim_ex) {
+    } else if (k > 0 || plan.ex_first) {
       // pivot among live vertices by original degree product
+      KSPN_TRY(P.begin("select_pivot", false));
       KSPN_HIP(hipMemsetAsync(c->cnt, 0, sizeof(uint32_t) * C_COUNT_, R.s));
       // reuse k_trim_first's reduction on a live-masked pass
       uint32_t u = 0;
@@ -2971,21 +3128,24 @@ int run_pipeline(Run& R) {
     }
     if (pivot >= R.n) break;
     uint32_t d = 0;
+    KSPN_TRY(P.begin("forward_backward_search", true));
     KSPN_TRY(fwbw(R, pivot, &d));
     if (st) st->decided_fwbw += d;
+    P.decided(d);
     if (R.live_n == 0) break;
-    if (trim_ex) {
-      KSPN_TRY(trim_exhaustive(R, &d));
-      if (st) st->decided_trim += d;
-    } else {
-      KSPN_TRY(trim_normal(R, &d));
-      if (st) st->decided_trim += d;
-    }
+    KSPN_TRY(after_trim());
   }
-  mark(2);
-  // ---- colour rounds
+  KSPN_TRY(mark(2));
+  // ---- colour rounds (the residual variants' "residual" scope)
+  if (plan.light && R.live_n > 0) {
+    // trim_1_exhaustive over the decisions since the first trim: at one rank
+    // FW-BW and colouring were skipped, the decided stack is empty
+    KSPN_TRY(P.begin("trim_ex", false));
+  }
   uint32_t cap = R.o.max_rounds;
   const uint32_t tail = (R.o.flags & KSPN_FLAG_NO_TAIL) ? 0u : (R.o.tail_threshold ? R.o.tail_threshold : 4096u);
+  const uint32_t stop = plan.residual_rest ? 0u : R.o.residual_threshold;
+  if (R.live_n > 0 && !(stop && R.live_n <= stop)) KSPN_TRY(P.begin(rest_name, true));
   if (!(R.o.flags & KSPN_FLAG_NO_REFINE) && !R.pid_active && R.live_n > tail) {
     // colour rounds refine one partition of all live vertices
     if (!c->pid) KSPN_TRY(dalloc(&c->pid, (uint64_t)c->cap_n + 64));
@@ -2993,33 +3153,40 @@ int run_pipeline(Run& R) {
     R.pid_active = true;
   }
   if (R.o.flags & KSPN_FLAG_NO_REFINE) R.pid_active = false;
+  uint32_t* rest_field = st ? (plan.residual_rest ? &st->decided_residual : &st->decided_color) : nullptr;
   while (R.live_n > 0) {
-    if (R.o.residual_threshold && R.live_n <= R.o.residual_threshold) break;
+    if (stop && R.live_n <= stop) break;
     if (cap && R.rounds >= cap) return KSPN_E_ITER_CAP;
     if (R.live_n <= tail) {
       uint32_t d = 0;
       KSPN_TRY(tail_solve(R, &d));
-      if (st) st->decided_color += d;
+      if (rest_field) *rest_field += d;
+      P.decided(d);
       break;
     }
     uint32_t d = 0;
     KSPN_TRY(color_round(R, &d));
     ++R.rounds;
-    if (st) st->decided_color += d;
+    if (rest_field) *rest_field += d;
+    P.decided(d);
     if (d == 0) return KSPN_E_ITER_CAP;  // cannot happen: the max key is always a root
-    if (trim_ex && R.live_n > 0) {
+    if (plan.trim_ex && R.live_n > 0) {
       KSPN_TRY(trim_exhaustive(R, &d));
       if (st) st->decided_trim += d;
+      P.decided(d);
     }
   }
-  mark(3);
-  // ---- residual Tarjan
+  KSPN_TRY(mark(3));
+  // ---- residual Tarjan (opts.residual_threshold)
   if (R.live_n > 0) {
     uint32_t d = 0;
+    KSPN_TRY(P.begin("residual", true));
     KSPN_TRY(residual(R, &d));
     if (st) st->decided_residual += d;
+    P.decided(d);
   }
-  mark(4);
+  KSPN_TRY(P.end());
+  KSPN_TRY(mark(4));
   // every stage writes canonical ids (own id, pair minimum, SCC minimum),
   // so no pass over all labels is needed here; a batch's labels become
   // local to their graph
@@ -3027,16 +3194,16 @@ int run_pipeline(Run& R) {
     Prof p(c, R.s, K_LABEL, 8.0 * R.n);
     k_batch_localize<<<std::min<uint32_t>(R.np0, 16384u), WG, 0, R.s>>>(R.np0, R.vtx_off, R.labels);
   }
-  mark(5);
+  KSPN_TRY(mark(5));
   KSPN_HIP(hipStreamSynchronize(R.s));
-  if (timing) {
+  if (st) {
     st->ms_trim = elapsed_ms(ev[0], ev[1]);
     st->ms_fwbw = elapsed_ms(ev[1], ev[2]);
     st->ms_color = elapsed_ms(ev[2], ev[3]);
     st->ms_residual = elapsed_ms(ev[3], ev[4]);
     st->ms_canon = elapsed_ms(ev[4], ev[5]);
     st->ms_total = elapsed_ms(ev[0], ev[5]);
-    for (auto& e : ev) (void)hipEventDestroy(e);
+    P.finish();
   }
   return KSPN_OK;
 }
@@ -3073,6 +3240,7 @@ const char* kspn_strerror(int code) {
     case KSPN_E_HIP: return "HIP runtime error";
     case KSPN_E_ITER_CAP: return "iteration cap reached";
     case KSPN_E_UNIMPLEMENTED: return "not implemented";
+    case KSPN_E_IO: return "file error";
     default: return "unknown error";
   }
 }
@@ -3176,7 +3344,7 @@ int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* o
   kspn_opts o;
   if (opts) o = *opts;
   else kspn_opts_default(&o);
-  if (o.method > KSPN_METHOD_TRIM_EX) return KSPN_E_INVALID;
+  if (o.method > KSPN_METHOD_TRIM_EX_LIGHT_RESIDUAL) return KSPN_E_INVALID;
   if (o.pivot > KSPN_PIVOT_RANDOM) return KSPN_E_INVALID;
   if (o.pivot == KSPN_PIVOT_LEARNED && !o.mlp_params) return KSPN_E_INVALID;
   if (stats) std::memset(stats, 0, sizeof(*stats));

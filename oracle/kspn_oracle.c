@@ -1023,6 +1023,114 @@ int orc_scc_trim_ex(uint32_t n, const uint64_t* fw_off, const uint32_t* fw_adj, 
   return decided == n ? 0 : -4;
 }
 
+/* Tarjan on the live vertices' forward subgraph, min-id labels
+ * (allgather_fw_sub_graph + tarjan, scc_ispan_like.hpp:69-84) */
+static uint32_t residual_tarjan(uint32_t n, const uint64_t* fw_off, const uint32_t* fw_adj, const uint8_t* live,
+                                uint32_t* scc_id, orc_scc_stats* s) {
+  uint64_t m = fw_off[n];
+  uint32_t* super_ids = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint64_t* sub_off = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)n + 1));
+  uint32_t* sub_adj = (uint32_t*)malloc(sizeof(uint32_t) * (m ? m : 1));
+  uint32_t* sub_lab = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t sn = 0;
+  uint64_t sm = orc_subgraph(n, fw_off, fw_adj, live, super_ids, &sn, sub_off, sub_adj);
+  orc_tarjan(sn, sub_off, sub_adj, sub_lab);
+  for (uint32_t i = 0; i < sn; ++i) scc_id[super_ids[i]] = super_ids[sub_lab[i]];
+  s->residual_n = sn;
+  s->residual_m = sm;
+  s->decided_residual += sn;
+  free(super_ids); free(sub_off); free(sub_adj); free(sub_lab);
+  return sn;
+}
+
+/* The other variants of the driver (tool/bench_kaspan/src/main.cpp:63-71)
+ * at one rank, by the method numbers of include/kspn/scc.h:
+ *   2 scc_hpc_like (variant/scc_hpc_like.hpp:16-69): trim_1_first, FW-BW,
+ *     push colouring (rot_label_search) until done;
+ *   3 scc_ispan_like (variant/scc_ispan_like.hpp:18-87): trim_1_first,
+ *     FW-BW, trim_1_normal twice, Tarjan on the rest;
+ *   4 scc_hpc_trim_ex (variant/scc_hpc_trim_ex.hpp:16-76):
+ *     trim_1_exhaustive_first, select_pivot_from_degree, FW-BW, push
+ *     colouring until done;
+ *   6 scc_trim_ex_light_residual (variant/scc_trim_ex_light_residual.hpp:
+ *     16-131): trim_1_exhaustive_first; the decided threshold n - n/1 is 0,
+ *     so FW-BW and colouring are skipped and the decided stack stays empty;
+ *     Tarjan on the rest.
+ * Stage counts: decided_trim_first (first trim), decided_fwbw,
+ * decided_trim_normal (trims after FW-BW), decided_color,
+ * decided_residual. Raw (non-canonical) ids. */
+int orc_scc_variant(uint32_t n, const uint64_t* fw_off, const uint32_t* fw_adj, const uint64_t* bw_off,
+                    const uint32_t* bw_adj, int variant, uint32_t* scc_id, orc_scc_stats* st) {
+  if (variant != 2 && variant != 3 && variant != 4 && variant != 6) return -1;
+  orc_scc_stats s;
+  memset(&s, 0, sizeof(s));
+  uint64_t m = fw_off[n];
+  uint8_t* live = (uint8_t*)malloc(n ? n : 1);
+  memset(live, 1, n);
+  uint32_t* outdeg = (uint32_t*)calloc(n ? n : 1, sizeof(uint32_t));
+  uint32_t* indeg = (uint32_t*)calloc(n ? n : 1, sizeof(uint32_t));
+  uint32_t* queue = (uint32_t*)malloc(sizeof(uint32_t) * (m + n + 1));
+  uint32_t* label = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t* dl = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t decided = 0, pivot = n;
+  int rc = 0;
+  if (variant == 2 || variant == 3) {
+    pivot = orc_trim_1_first(n, fw_off, bw_off, live, scc_id);
+    for (uint32_t k = 0; k < n; ++k) decided += !live[k];
+  } else {
+    decided = trim_ex_first(n, fw_off, fw_adj, bw_off, bw_adj, live, outdeg, indeg, queue, scc_id);
+    if (variant == 4 && decided < n) pivot = select_pivot_from_degree(n, live, outdeg, indeg);
+  }
+  s.decided_trim_first = decided;
+  s.pivot = pivot;
+  if (decided < n && variant != 6) {
+    s.decided_fwbw = orc_fwbw(n, fw_off, fw_adj, bw_off, bw_adj, live, pivot, scc_id);
+    decided += s.decided_fwbw;
+  }
+  if (decided < n && variant == 3) {
+    s.decided_trim_normal = trim_1_normal(n, fw_off, fw_adj, bw_off, bw_adj, live, scc_id);
+    s.decided_trim_normal += trim_1_normal(n, fw_off, fw_adj, bw_off, bw_adj, live, scc_id);
+    decided += s.decided_trim_normal;
+  }
+  if (decided < n && (variant == 3 || variant == 6)) {
+    decided += residual_tarjan(n, fw_off, fw_adj, live, scc_id, &s);
+  }
+  uint32_t rotation = 0;
+  while (decided < n) {
+    uint32_t ndl = 0;
+    ++rotation;
+    uint32_t c = rot_label_search_push(n, fw_off, fw_adj, bw_off, bw_adj, live, rotation, label, scc_id, dl, &ndl);
+    s.decided_color += c;
+    s.color_rounds++;
+    decided += c;
+    if (c == 0) { rc = -4; break; }
+  }
+  if (st) *st = s;
+  free(live); free(outdeg); free(indeg); free(queue); free(label); free(dl);
+  return rc;
+}
+
+/* The fixpoint of trim-1 from the live set live[]: vertices removed by
+ * repeatedly deciding any live vertex without a live in-neighbour or without
+ * a live out-neighbour (self loops count as neighbours). Unique, whatever
+ * the order; live[] is updated. */
+uint32_t orc_trim_fixpoint(uint32_t n, const uint64_t* fw_off, const uint32_t* fw_adj, const uint64_t* bw_off,
+                           const uint32_t* bw_adj, uint8_t* live) {
+  uint32_t decided = 0;
+  int changed = 1;
+  while (changed) {
+    changed = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+      if (!live[k]) continue;
+      int has_in = 0, has_out = 0;
+      for (uint64_t it = bw_off[k]; it < bw_off[k + 1] && !has_in; ++it) has_in = live[bw_adj[it]];
+      for (uint64_t it = fw_off[k]; it < fw_off[k + 1] && !has_out; ++it) has_out = live[fw_adj[it]];
+      if (!has_in || !has_out) { live[k] = 0; ++decided; changed = 1; }
+    }
+  }
+  return decided;
+}
+
 /* ======================================================================== */
 /* learned pivot scorer: features and an FP64 MLP                            */
 /* ======================================================================== */

@@ -105,6 +105,14 @@ int orc_scc_trim_ex_residual(uint32_t n, const uint64_t* fw_off, const uint32_t*
 int orc_scc_trim_ex(uint32_t n, const uint64_t* fw_off, const uint32_t* fw_adj, const uint64_t* bw_off,
                     const uint32_t* bw_adj, uint32_t* scc_id, orc_scc_stats* st);
 
+/* scc_hpc_like (2), scc_ispan_like (3), scc_hpc_trim_ex (4),
+ * scc_trim_ex_light_residual (6) at one rank; raw ids; -1 for another number */
+int orc_scc_variant(uint32_t n, const uint64_t* fw_off, const uint32_t* fw_adj, const uint64_t* bw_off,
+                    const uint32_t* bw_adj, int variant, uint32_t* scc_id, orc_scc_stats* st);
+/* fixpoint of trim-1 (no live in- or no live out-neighbour) from live[]; returns the count removed */
+uint32_t orc_trim_fixpoint(uint32_t n, const uint64_t* fw_off, const uint32_t* fw_adj, const uint64_t* bw_off,
+                           const uint32_t* bw_adj, uint8_t* live);
+
 /* ---- learned pivot scorer: FP64 reference of the MLP ---------------------- */
 void orc_pivot_features(uint32_t n, const uint64_t* fw_off, const uint32_t* fw_adj,
                         const uint64_t* bw_off, const uint32_t* bw_adj, float* feat);

@@ -78,6 +78,8 @@ def lib():
             "orc_kaspan_scc": (i32, [u32, _u64p, _u32p, _u64p, _u32p, _u32p, i32, C.POINTER(SccStats)]),
             "orc_scc_trim_ex_residual": (i32, [u32, _u64p, _u32p, _u64p, _u32p, _u32p, C.POINTER(SccStats)]),
             "orc_scc_trim_ex": (i32, [u32, _u64p, _u32p, _u64p, _u32p, _u32p, C.POINTER(SccStats)]),
+            "orc_scc_variant": (i32, [u32, _u64p, _u32p, _u64p, _u32p, i32, _u32p, C.POINTER(SccStats)]),
+            "orc_trim_fixpoint": (u32, [u32, _u64p, _u32p, _u64p, _u32p, _u8p]),
             "orc_pivot_features": (None, [u32, _u64p, _u32p, _u64p, _u32p, _f32p]),
             "orc_mlp_forward": (None, [u32, _f32p, _f64p, _f64p, _f64p, _f64p, _f64p, dbl, _f64p]),
             "orc_mlp_loss_grad": (dbl, [u32, _f32p, _f32p, _f64p, _f64p, _f64p, _f64p, _f64p, dbl,
@@ -272,6 +274,89 @@ def scc_trim_ex_residual(g: HostGraph):
 
 def scc_trim_ex(g: HostGraph):
     return _pipeline(lib().orc_scc_trim_ex, g)
+
+
+def scc_variant(g: HostGraph, method: int):
+    """scc_hpc_like (2), scc_ispan_like (3), scc_hpc_trim_ex (4), scc_trim_ex_light_residual (6)
+    at one rank (scc/variant/*.hpp): raw ids, stats"""
+    labels = np.full(max(g.n, 1), UNDECIDED, np.uint32)
+    st = SccStats()
+    rc = lib().orc_scc_variant(g.n, g.fw_off, _nonempty(g.fw_adj, np.uint32), g.bw_off,
+                               _nonempty(g.bw_adj, np.uint32), int(method), labels, C.byref(st))
+    if rc:
+        raise RuntimeError(f"oracle variant {method} failed with {rc}")
+    return labels[: g.n].copy(), st
+
+
+def trim_fixpoint(g: HostGraph, live=None):
+    """(count, live) of the trim-1 fixpoint from `live` (all vertices if None)"""
+    live = np.ones(max(g.n, 1), np.uint8) if live is None else np.array(live, np.uint8, copy=True)
+    if live.size == 0:
+        live = np.ones(1, np.uint8)
+    d = lib().orc_trim_fixpoint(g.n, g.fw_off, _nonempty(g.fw_adj, np.uint32), g.bw_off,
+                                _nonempty(g.bw_adj, np.uint32), live)
+    return int(d), live[: g.n].copy()
+
+
+# --------------------------------------------------------------------------- manifests
+def decode_dense_unsigned(data: bytes, count: int, width: int, endian: str) -> np.ndarray:
+    """dense_unsigned_ops::get for every element (memory/dense_unsigned_ops.hpp:14-44):
+    element i is bytes [i*width, (i+1)*width) read in `endian` byte order"""
+    b = np.frombuffer(data, dtype=np.uint8, count=count * width).reshape(count, width).astype(np.uint64)
+    if endian == "big":
+        b = b[:, ::-1]
+    out = np.zeros(count, np.uint64)
+    for j in range(width):
+        out |= b[:, j] << np.uint64(8 * j)
+    return out
+
+
+def read_manifest(path: str) -> dict:
+    """manifest::load (scc/adapter/manifest.hpp:20-51,136-197): key/value lines, '%' comments,
+    key up to the first space, value after the spaces; paths relative to the manifest"""
+    kv = {}
+    with open(path, "r", newline="\n") as f:
+        for line in f.read().split("\n"):
+            if not line or line[0] == "%":
+                continue
+            if " " not in line:
+                raise ValueError(f"line without a value: {line!r}")
+            k, v = line.split(" ", 1)
+            if k in kv:
+                raise ValueError(f"repeated key {k}")
+            kv[k] = v.lstrip(" ")
+    base = os.path.dirname(os.path.abspath(path))
+    out = {
+        "schema_version": int(kv["schema.version"]), "code": kv["graph.code"], "name": kv["graph.name"],
+        "endian": kv["graph.endian"], "node_count": int(kv["graph.node_count"]),
+        "edge_count": int(kv["graph.edge_count"]),
+        "contains_self_loops": {"true": True, "false": False}[kv["graph.contains_self_loops"]],
+        "contains_duplicate_edges": {"true": True, "false": False}[kv["graph.contains_duplicate_edges"]],
+        "head_bytes": int(kv["graph.head.bytes"]), "csr_bytes": int(kv["graph.csr.bytes"]),
+    }
+    if out["endian"] not in ("little", "big"):
+        raise ValueError("graph.endian")
+    for key in ("fw.head.path", "fw.csr.path", "bw.head.path", "bw.csr.path"):
+        out[key.replace(".", "_")] = os.path.realpath(os.path.join(base, kv[key]))
+    return out
+
+
+def load_manifest(path: str) -> HostGraph:
+    """load_graph_from_manifest (scc/adapter/manifest.hpp:200-250) on the host"""
+    mf = read_manifest(path)
+    n, m = mf["node_count"], mf["edge_count"]
+    arrays = []
+    for key, count, width in (("fw_head_path", n + 1, mf["head_bytes"]), ("fw_csr_path", m, mf["csr_bytes"]),
+                              ("bw_head_path", n + 1, mf["head_bytes"]), ("bw_csr_path", m, mf["csr_bytes"])):
+        with open(mf[key], "rb") as f:
+            data = f.read()
+        if len(data) != count * width:
+            raise ValueError(f"{mf[key]}: {len(data)} bytes, expected {count * width}")
+        arrays.append(decode_dense_unsigned(data, count, width, mf["endian"]))
+    fo, fa, bo, ba = arrays
+    if np.any(fo > m) or np.any(bo > m) or (m and (fa.max() >= n or ba.max() >= n)):
+        raise ValueError("value out of range")
+    return HostGraph(n, fo, fa.astype(np.uint32), bo, ba.astype(np.uint32))
 
 
 def trim_1_first(g: HostGraph):

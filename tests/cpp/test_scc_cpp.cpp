@@ -1,5 +1,7 @@
 // C++ caller of the kaspan:: mirror over the C ABI: the iSpan tiny graph
-// (extern/ispan_test/test_ispan_tiny.cpp:21-90) and a directed cycle.
+// (extern/ispan_test/test_ispan_tiny.cpp:21-90) and a directed cycle under
+// every variant, and (with a manifest path as argument) the tiny graph
+// loaded through kaspan::manifest::load / load_graph_from_manifest.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -38,9 +40,42 @@ static int run(std::vector<uint64_t> fo, std::vector<uint32_t> fa, const std::ve
   return 0;
 }
 
-int main() {
+// argv[1]: manifest of the iSpan tiny graph
+static int run_manifest(const char* path) {
+  const std::vector<uint32_t> expect{0, 1, 0, 1, 1, 5, 5};
+  auto mf = kaspan::manifest::load(path);
+  auto g = kaspan::load_graph_from_manifest(mf);
+  if (g.n() != 7 || g.m() != 11) {
+    std::printf("manifest graph has n=%u m=%llu\n", g.n(), (unsigned long long)g.m());
+    return 1;
+  }
+  uint32_t* d_ids = nullptr;
+  HIP_OK(hipMalloc(&d_ids, 4 * 7));
+  kspn_stats st;
+  kaspan::scc_ispan_like(g.view(), d_ids, &st);
+  std::vector<uint32_t> got(7);
+  HIP_OK(hipMemcpy(got.data(), d_ids, 4 * 7, hipMemcpyDeviceToHost));
+  (void)hipFree(d_ids);
+  if (got != expect) {
+    std::printf("manifest graph: wrong labels\n");
+    return 1;
+  }
+  // a manifest naming a file that does not exist
   try {
-    for (int method : {KSPN_METHOD_KASPAN, KSPN_METHOD_TRIM_EX}) {
+    auto bad = kaspan::manifest::load(std::string(path) + ".missing");
+    std::printf("expected an exception\n");
+    return 1;
+  } catch (const kaspan::error& e) {
+    if (e.code != KSPN_E_IO) return 1;
+  }
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  try {
+    if (argc > 1 && run_manifest(argv[1])) return 1;
+    for (int method : {KSPN_METHOD_KASPAN, KSPN_METHOD_TRIM_EX, KSPN_METHOD_HPC_LIKE, KSPN_METHOD_ISPAN_LIKE,
+                       KSPN_METHOD_HPC_TRIM_EX, KSPN_METHOD_TRIM_EX_RESIDUAL, KSPN_METHOD_TRIM_EX_LIGHT_RESIDUAL}) {
       if (run({0, 1, 4, 5, 7, 9, 10, 11}, {2, 0, 2, 3, 0, 2, 4, 1, 3, 6, 5}, {0, 1, 0, 1, 1, 5, 5}, method)) return 1;
       std::vector<uint64_t> off(1001);
       std::vector<uint32_t> adj(1000), zero(1000, 0);

@@ -267,11 +267,16 @@ def test_c2_rmat24_full(ka, O):
     assert np.array_equal(lab2, tar)
 
 
-def test_cpp_mirror(ka):
-    """include/kspn/kaspan.hpp from C++: golden labels and error -> exception"""
+def test_cpp_mirror(ka, tmp_path):
+    """include/kspn/kaspan.hpp from C++: golden labels under every variant, a
+    manifest loaded through kaspan::manifest::load, error -> exception"""
     import subprocess
 
+    from kaspan_amd import manifest as M
+
+    mf = M.write_manifest(str(tmp_path / "tiny"), 7, H.ISPAN_TINY_FW_OFF, H.ISPAN_TINY_FW_ADJ, H.ISPAN_TINY_BW_OFF,
+                          H.ISPAN_TINY_BW_ADJ, endian="big", head_bytes=3, csr_bytes=2)
     exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "test_scc_cpp")
-    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    out = subprocess.run([exe, mf], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "cpp ok" in out.stdout

@@ -49,6 +49,10 @@ int main(void) {
   printf("%zu %zu\n", offsetof(kspn_opts, mlp_params), offsetof(kspn_gen_params, permute));
   printf("%zu %zu %zu %zu\n", sizeof(kspn_batch), offsetof(kspn_batch, bw_adj), sizeof(kspn_batch_gen),
          offsetof(kspn_batch_gen, c));
+  printf("%zu %zu %zu %zu\n", sizeof(kspn_phase), offsetof(kspn_phase, entries), offsetof(kspn_stats, n_phase),
+         offsetof(kspn_stats, phase));
+  printf("%zu %zu %zu %zu\n", sizeof(kspn_manifest), offsetof(kspn_manifest, csr_bytes), offsetof(kspn_manifest, name),
+         offsetof(kspn_manifest, bw_csr_path));
   return 0;
 }
 ''')
@@ -58,7 +62,10 @@ int main(void) {
     got = [C.sizeof(_lib.Graph), C.sizeof(_lib.Opts), C.sizeof(_lib.Stats), C.sizeof(_lib.GenParams),
            C.sizeof(_lib.KernStat), _lib.Stats.edges_scanned.offset, _lib.Stats.n_kern.offset, _lib.Stats.kern.offset,
            _lib.Opts.mlp_params.offset, _lib.GenParams.permute.offset, C.sizeof(_lib.Batch), _lib.Batch.bw_adj.offset,
-           C.sizeof(_lib.BatchGen), _lib.BatchGen.c.offset]
+           C.sizeof(_lib.BatchGen), _lib.BatchGen.c.offset,
+           C.sizeof(_lib.Phase), _lib.Phase.entries.offset, _lib.Stats.n_phase.offset, _lib.Stats.phase.offset,
+           C.sizeof(_lib.Manifest), _lib.Manifest.csr_bytes.offset, _lib.Manifest.name.offset,
+           _lib.Manifest.bw_csr_path.offset]
     assert [int(x) for x in out] == got
 
 
