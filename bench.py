@@ -70,6 +70,8 @@ CONFIGS = {
                   "in [1e3, 1e6], average out-degree log-uniform in [2, 16], deduplicated, concatenated CSR with "
                   "per-graph vertex/edge offsets"),
 }
+METHOD_NAMES = ["kaspan", "trim_ex", "hpc_like", "ispan_like", "hpc_trim_ex", "trim_ex_residual",
+                "trim_ex_light_residual"]
 LATENCY_SAMPLE = 128  # graphs of the batch solved one at a time for p50/p99
 # FW-BW rounds when --fwbw-rounds is not given: the planted workload's many large SCCs want the
 # partitioned multi-pivot rounds (32 is the plain rule's best in a sweep of 1..64)
@@ -87,7 +89,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="rmat24", choices=sorted(CONFIGS))
+    ap.add_argument("--manifest", default=None, metavar="FILE",
+                    help="solve the graph of a manifest file (convert_graph's format) instead of --config")
     ap.add_argument("--method", type=int, default=None, help="0 kaspan, 1 trim_ex (default: fastest known)")
+    ap.add_argument("--variant", default=None,
+                    help="one of the reference driver's variant names (sets --method): async, hpc_like, "
+                         "hpc_trim_ex, ispan_like, trim_ex, trim_ex_light_residual, trim_ex_residual")
     ap.add_argument("--pivot", choices=["plain", "learned"], default="plain")
     ap.add_argument("--fwbw-rounds", type=int, default=None)
     ap.add_argument("--coop-edges", type=int, default=0, help="persistent-kernel frontier edge bound (0: library default)")
@@ -124,11 +131,27 @@ def main():
     import kaspan_amd as ka
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    kind, kw, seed, desc = CONFIGS[args.config]
+    if args.variant is not None:
+        if args.variant not in ka.VARIANTS:
+            raise SystemExit(f"unknown variant {args.variant!r}")
+        args.method = ka.VARIANTS[args.variant]
+    if args.manifest:
+        from kaspan_amd import manifest as M
+
+        mf = M.parse_manifest(args.manifest)
+        args.config = f"manifest:{mf.name}"
+        kind, kw, seed = "manifest", {}, None
+        desc = (f"manifest graph {mf.name} (n={mf.node_count}, m={mf.edge_count}, {mf.head_bytes}/{mf.csr_bytes}-byte "
+                f"{mf.endian}-endian arrays, self-loops: {mf.contains_self_loops}, duplicate edges: "
+                f"{mf.contains_duplicate_edges})")
+    else:
+        kind, kw, seed, desc = CONFIGS[args.config]
     print(f"[bench] generating {args.config}", file=sys.stderr, flush=True)
     t0 = time.time()
     batch = None
-    if kind == "batch":
+    if kind == "manifest":
+        g = M.load_manifest(mf)
+    elif kind == "batch":
         bkw = dict(kw)
         count = bkw.pop("count")
         batch = ka.generate_batch(seed, count, **bkw)
@@ -319,7 +342,7 @@ def main():
         "dtype": "uint32",
         "data": "synthetic",
         "config": {"workload": desc, "name": args.config, "n": g.n, "m": g.m,
-                   "method": ["kaspan", "trim_ex"][opts.get("method", method)], "pivot": args.pivot,
+                   "method": METHOD_NAMES[opts.get("method", method)], "pivot": args.pivot,
                    "fwbw_rounds": opts.get("fwbw_rounds", args.fwbw_rounds), "seed": seed,
                    "options": {k: v for k, v in opts.items() if k != "mlp_params"}},
         "ab": ab,
@@ -332,6 +355,7 @@ def main():
         "solve": {k: getattr(stats, k) for k in ("pivot", "fwbw_levels_push", "fwbw_levels_pull", "color_rounds",
                                                  "color_iters", "edges_scanned", "residual_n")},
         "kernels": kernels[:8],
+        "scopes": stats.phases(),
     }
     if batch_info is not None:
         out["batch"] = batch_info
