@@ -256,13 +256,14 @@ def _opts(method=METHOD_KASPAN, pivot=0, fwbw_rounds=1, residual_threshold=0, ma
 
 
 def scc(graph: BidiGraph, scc_id: torch.Tensor | None = None, *, ctx: Context | None = None, stream=None,
-        profile=False, **opts) -> tuple[torch.Tensor, Stats]:
+        profile=False, stats=True, **opts) -> tuple[torch.Tensor, Stats | None]:
     """kaspan::scc(graph, scc_id) (scc/scc.hpp:21-87) on the GPU.
 
     Fills ``scc_id`` (int32 device tensor of n entries, allocated if None)
     with canonical ids and returns ``(scc_id, stats)``. Keyword options map
     to ``kspn_opts`` (method, pivot, fwbw_rounds, residual_threshold,
-    max_rounds, flags, alpha, beta, mlp_params).
+    max_rounds, flags, alpha, beta, mlp_params). ``stats=False`` skips the
+    statistics (no per-scope HIP events) and returns None for them.
     """
     ctx = ctx or Context.default(graph.device.index or 0)
     if scc_id is None:
@@ -272,10 +273,10 @@ def scc(graph: BidiGraph, scc_id: torch.Tensor | None = None, *, ctx: Context | 
     o = _opts(**opts)
     if profile:
         o.flags |= _lib.FLAG_PROFILE
-    st = Stats()
+    st = Stats() if (stats or profile) else None
     g = graph.view()
-    rc = _lib.lib().kspn_scc(ctx.ptr, C.byref(g), C.c_void_p(scc_id.data_ptr()), C.byref(o), C.byref(st),
-                             _stream_ptr(stream))
+    rc = _lib.lib().kspn_scc(ctx.ptr, C.byref(g), C.c_void_p(scc_id.data_ptr()), C.byref(o),
+                             C.byref(st) if st is not None else None, _stream_ptr(stream))
     _lib.check(rc, "kspn_scc")
     return scc_id, st
 
@@ -497,7 +498,7 @@ def generate_batch(seed, count, device="cuda", ctx: Context | None = None, strea
 
 
 def scc_batch(batch: Batch, scc_id: torch.Tensor | None = None, *, ctx: Context | None = None, stream=None,
-              profile=False, **opts) -> tuple[torch.Tensor, Stats]:
+              profile=False, stats=True, **opts) -> tuple[torch.Tensor, Stats | None]:
     """scc() for every graph of the batch in one solve.
 
     ``scc_id[vtx_off[i] + v]`` is v's canonical id inside graph i (local ids),
@@ -512,8 +513,9 @@ def scc_batch(batch: Batch, scc_id: torch.Tensor | None = None, *, ctx: Context 
     o = _opts(**opts)
     if profile:
         o.flags |= _lib.FLAG_PROFILE
-    st = Stats()
+    st = Stats() if (stats or profile) else None
     b = batch.view()
-    _lib.check(_lib.lib().kspn_scc_batch(ctx.ptr, C.byref(b), C.c_void_p(scc_id.data_ptr()), C.byref(o), C.byref(st),
-                                         _stream_ptr(stream)), "kspn_scc_batch")
+    _lib.check(_lib.lib().kspn_scc_batch(ctx.ptr, C.byref(b), C.c_void_p(scc_id.data_ptr()), C.byref(o),
+                                         C.byref(st) if st is not None else None, _stream_ptr(stream)),
+               "kspn_scc_batch")
     return scc_id, st

@@ -36,8 +36,8 @@ DEFAULTS = {"alpha": 14.0, "beta": 24.0, "coop_edges": 1 << 20, "coop_workgroups
 
 def _solver(target):
     if isinstance(target, ka.Batch):
-        return lambda lab, **o: ka.scc_batch(target, lab, **o), target.graph.n
-    return lambda lab, **o: ka.scc(target, lab, **o), target.n
+        return lambda lab, **o: ka.scc_batch(target, lab, stats=False, **o), target.graph.n
+    return lambda lab, **o: ka.scc(target, lab, stats=False, **o), target.n
 
 
 def time_opts(target, opts, reps=3, labels=None):

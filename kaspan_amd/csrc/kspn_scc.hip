@@ -870,7 +870,7 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
       const uint64_t v = base + (uint64_t)k * WG + threadIdx.x;
       if (hit[k]) {
         ++claimed;
-        nd += noff[v + 1] - noff[v];
+        if (noff) nd += noff[v + 1] - noff[v];
       }
       // only this wavefront sets bits in its two words during a pull level,
       // so the claim is a plain store of (old | new) per word
@@ -2356,10 +2356,15 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       uint32_t grid = stream_grid(c, R.n);
       {
         Prof p(c, R.s, kern, (double)R.n / 4.0 + 16.0 * R.live_n);
+        // no out-degree sum of the claimed vertices: it would read a third
+        // offset array at scattered rows, and the first push level after the
+        // pull levels sums its frontier's degrees itself
         if (dir == 0)
-          k_pull<0><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, off, c->fnew, c->part, pid, &c->cnt[C_CURSOR]);
+          k_pull<0><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
+                                          &c->cnt[C_CURSOR]);
         else
-          k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, off, c->fnew, c->part, pid, &c->cnt[C_CURSOR]);
+          k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
+                                          &c->cnt[C_CURSOR]);
       }
       if (fw_levels_pull) (*fw_levels_pull)++;
       // the workgroup partials are reduced by the publication
@@ -2379,8 +2384,9 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
     uint32_t nf = c->h_cnt[nc];
     edges = c->h_u64[2];
     maxd = pull ? 0xFFFFFFFFu : (uint32_t)std::min<uint64_t>(c->h_u64[3], 0xFFFFFFFFull);
-    known = true;
-    if (pull) m_rest -= (double)edges;
+    // after a pull level the frontier's out-edges are unknown until a push
+    // level sums them; the remaining-edge estimate keeps them until expanded
+    known = !pull;
     if (pull && nf > 0 && (double)nf * beta < (double)R.live_n) {
       // back to push: the frontier is the bitmap of the last pull level (its
       // size nf is already in cnt[nc]; the pull zeroed the cursor)
@@ -2963,16 +2969,37 @@ struct Phases {
   void decided(uint64_t d) {
     if (st && cur >= 0) st->phase[cur].decided += d;
   }
-  int end() {
+  // closes the open scope; `hidden` starts an interval outside every scope
+  // (the batch's label localisation), timed into ms_canon
+  int end(bool hidden = false) {
     if (!st) return KSPN_OK;
     close_scope();
     cur = -1;
-    return mark(-1);
+    return mark(hidden ? -2 : -1);
   }
+  // scope times, and the coarse ms_* fields as sums of scopes
   void finish() {
     if (!st) return;
-    for (size_t i = 0; i + 1 < marks.size(); ++i)
-      if (marks[i].first >= 0) st->phase[marks[i].first].ms += elapsed_ms(marks[i].second, marks[i + 1].second);
+    double canon = 0;
+    for (size_t i = 0; i + 1 < marks.size(); ++i) {
+      const double ms = elapsed_ms(marks[i].second, marks[i + 1].second);
+      if (marks[i].first >= 0) st->phase[marks[i].first].ms += ms;
+      else if (marks[i].first == -2) canon += ms;
+    }
+    st->ms_canon = canon;
+    st->ms_total = marks.size() > 1 ? elapsed_ms(marks.front().second, marks.back().second) : 0.0;
+    for (uint32_t i = 0; i < st->n_phase; ++i) {
+      const kspn_phase& ph = st->phase[i];
+      const char* nm = ph.name;
+      if (!std::strcmp(nm, "trim_1_first") || !std::strcmp(nm, "trim_ex_first") || !std::strcmp(nm, "trim_2"))
+        st->ms_trim += ph.ms;
+      else if (!std::strcmp(nm, "color") || !std::strcmp(nm, "color_trim_ex"))
+        st->ms_color += ph.ms;
+      else if (!std::strcmp(nm, "residual"))
+        st->ms_residual += ph.ms;
+      else
+        st->ms_fwbw += ph.ms;  // select_pivot, forward_backward_search, trim_1_normal, trim_ex
+    }
   }
 };
 
@@ -3007,32 +3034,9 @@ int run_pipeline(Run& R) {
   const Plan plan = plan_for(R.o.method);
   R.pivot_live_deg = plan.ex_first;
   Phases P(c, st, R.s, (R.o.flags & KSPN_FLAG_STATS_MEMORY) != 0);
-  // coarse stage boundaries for the ms_* fields (marks 0..5)
-  std::vector<hipEvent_t> ev;
-  auto mark = [&](int) -> int {
-    if (!st) return KSPN_OK;
-    hipEvent_t e = nullptr;
-    if (!c->free_events.empty()) {
-      e = c->free_events.back();
-      c->free_events.pop_back();
-    } else {
-      KSPN_HIP(hipEventCreate(&e));
-    }
-    ev.push_back(e);
-    KSPN_HIP(hipEventRecord(e, R.s));
-    return KSPN_OK;
-  };
-  struct Give {
-    Ctx* c;
-    std::vector<hipEvent_t>& ev;
-    ~Give() {
-      for (auto e : ev) c->free_events.push_back(e);
-    }
-  } give{c, ev};
   const char* first_name = plan.ex_first ? "trim_ex_first" : "trim_1_first";
   const char* after_name = plan.after == 3 ? "trim_ex" : "trim_1_normal";
   const char* rest_name = plan.residual_rest ? "residual" : (plan.trim_ex ? "color_trim_ex" : "color");
-  KSPN_TRY(mark(0));
   // ---- trim-1 first pass + pivot
   KSPN_TRY(P.begin(first_name, plan.ex_first));
   KSPN_HIP(hipMemsetAsync(c->cnt, 0, sizeof(uint32_t) * C_COUNT_, R.s));
@@ -3075,7 +3079,6 @@ int run_pipeline(Run& R) {
       }
     }
   }
-  KSPN_TRY(mark(1));
   // trims after a FW-BW round
   auto after_trim = [&]() -> int {
     if (plan.after == 0 || R.live_n == 0) return KSPN_OK;
@@ -3135,7 +3138,6 @@ int run_pipeline(Run& R) {
     if (R.live_n == 0) break;
     KSPN_TRY(after_trim());
   }
-  KSPN_TRY(mark(2));
   // ---- colour rounds (the residual variants' "residual" scope)
   if (plan.light && R.live_n > 0) {
     // trim_1_exhaustive over the decisions since the first trim: at one rank
@@ -3176,7 +3178,6 @@ int run_pipeline(Run& R) {
       P.decided(d);
     }
   }
-  KSPN_TRY(mark(3));
   // ---- residual Tarjan (opts.residual_threshold)
   if (R.live_n > 0) {
     uint32_t d = 0;
@@ -3185,26 +3186,17 @@ int run_pipeline(Run& R) {
     if (st) st->decided_residual += d;
     P.decided(d);
   }
-  KSPN_TRY(P.end());
-  KSPN_TRY(mark(4));
   // every stage writes canonical ids (own id, pair minimum, SCC minimum),
   // so no pass over all labels is needed here; a batch's labels become
   // local to their graph
   if (R.vtx_off) {
+    KSPN_TRY(P.end(true));
     Prof p(c, R.s, K_LABEL, 8.0 * R.n);
     k_batch_localize<<<std::min<uint32_t>(R.np0, 16384u), WG, 0, R.s>>>(R.np0, R.vtx_off, R.labels);
   }
-  KSPN_TRY(mark(5));
+  KSPN_TRY(P.end());
   KSPN_HIP(hipStreamSynchronize(R.s));
-  if (st) {
-    st->ms_trim = elapsed_ms(ev[0], ev[1]);
-    st->ms_fwbw = elapsed_ms(ev[1], ev[2]);
-    st->ms_color = elapsed_ms(ev[2], ev[3]);
-    st->ms_residual = elapsed_ms(ev[3], ev[4]);
-    st->ms_canon = elapsed_ms(ev[4], ev[5]);
-    st->ms_total = elapsed_ms(ev[0], ev[5]);
-    P.finish();
-  }
+  P.finish();
   return KSPN_OK;
 }
 
