@@ -175,10 +175,33 @@ __device__ inline void bq_flush(BlockQueue& q, uint32_t* queue, uint32_t* count)
   __syncthreads();
 }
 
+// bq_flush for a queue of `cap` entries: the count always grows, the
+// entries are written only while they fit
+__device__ inline void bq_flush_capped(BlockQueue& q, uint32_t* queue, uint32_t* count, uint32_t cap) {
+  __syncthreads();
+  uint32_t cnt = *q.n;
+  if (cnt) {
+    if (threadIdx.x == 0) *q.base = atomicAdd(count, cnt);
+    __syncthreads();
+    uint32_t b = *q.base;
+    if (b + cnt <= cap)
+      for (uint32_t i = threadIdx.x; i < cnt; i += WG) queue[b + i] = q.buf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) *q.n = 0;
+  }
+  __syncthreads();
+}
+
 // flush when fewer than `room` free slots remain (block-uniform decision)
 __device__ inline void bq_maybe_flush(BlockQueue& q, uint32_t* queue, uint32_t* count, uint32_t room) {
   __syncthreads();
   if (*q.n + room > q.cap) bq_flush(q, queue, count);
+}
+
+__device__ inline void bq_maybe_flush_capped(BlockQueue& q, uint32_t* queue, uint32_t* count, uint32_t cap,
+                                             uint32_t room) {
+  __syncthreads();
+  if (*q.n + room > q.cap) bq_flush_capped(q, queue, count, cap);
 }
 
 __device__ inline uint32_t rotr32(uint32_t x, uint32_t r) { r &= 31; return r ? (x >> r) | (x << (32 - r)) : x; }
@@ -199,6 +222,8 @@ enum Counter : int {
   C_DTOT,  // running length of the decided list
   C_ERR,   // set by a kernel that met a broken internal invariant
   C_CURSOR,  // write cursor of an unordered bitmap compaction
+  C_SURV0,   // sizes of the two lists of vertices a pull level left unreached
+  C_SURV1,
   C_COUNT_ = 16
 };
 
@@ -225,6 +250,7 @@ struct Ctx {
   uint32_t* bwv = nullptr;   // backward-reached (inside fwv)
   uint32_t* inq = nullptr;   // queue membership
   uint32_t* fnew = nullptr;  // vertices reached by the last pull level
+  uint32_t* sbm = nullptr;   // candidates the last pull sweep left unreached
   // vertex arrays
   uint32_t* qa = nullptr;
   uint32_t* qb = nullptr;

@@ -61,7 +61,8 @@ __device__ inline void publish_seq(uint32_t* map, uint32_t seq) {
 // needs no separate reset or reduction launch. One workgroup of WG threads.
 __global__ __launch_bounds__(WG) void k_publish(uint32_t* cnt, unsigned long long* cnt64, const uint64_t* extra,
                                                 uint32_t* map, uint32_t seq, const uint64_t* __restrict__ part,
-                                                uint32_t nparts, uint32_t* pcount, uint32_t* reset) {
+                                                uint32_t nparts, uint32_t* pcount, uint32_t* reset,
+                                                uint32_t* reset2) {
   __shared__ uint64_t s_red[WG / WAVE];
   if (part) {
     uint64_t a = 0, b = 0, d = 0;
@@ -107,6 +108,7 @@ __global__ __launch_bounds__(WG) void k_publish(uint32_t* cnt, unsigned long lon
     __syncthreads();  // every copy above is done
     if (threadIdx.x == 0) {
       *reset = 0;
+      if (reset2) *reset2 = 0;
       cnt64[0] = 0;
       cnt64[1] = 0;
       cnt64[2] = 0;
@@ -795,15 +797,20 @@ __global__ __launch_bounds__(WG) void k_pivot_reduce(const uint64_t* part, uint3
 // made earlier in the same sweep are visible. Newly reached vertices are
 // written as a bitmap (each wavefront owns its two words, no atomics); per
 // workgroup partials: claimed count, edges scanned, next-level degree sum
-// (noff) of the claimed vertices.
+// (noff, optional) of the claimed vertices, or with `sbm` the number of
+// candidates left unreached, whose bits it receives (written like fnew) so
+// that a short remainder can be compacted into a list for k_pull_list.
+// Block 0 zeroes the two list counters `lcnt`, which the sweep does not use.
 constexpr int PULL_ILP = 2;  // vertices in flight per thread (2 beat 1 and 4 on R-MAT 24 and the 4096-graph batch)
+constexpr uint32_t PULL_FLUSH = 4;                     // k_pull_list: iterations between checks of its LDS queues
+constexpr uint32_t PULL_QCAP = PULL_FLUSH * WG * PULL_ILP;  // entries of each LDS queue (8 KiB)
 
 template <int DIR>
 __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restrict__ cand, uint32_t* vis,
                                              const uint64_t* __restrict__ roff, const uint32_t* __restrict__ radj,
                                              const uint64_t* __restrict__ noff, uint32_t* __restrict__ fnew,
                                              uint64_t* __restrict__ part, const uint32_t* __restrict__ pid,
-                                             uint32_t* cursor) {
+                                             uint32_t* cursor, uint32_t* __restrict__ sbm, uint32_t* lcnt) {
   // A candidate costs a chain of dependent loads (bitmap word, row offsets,
   // in-neighbour, its visited bit, its partition). Each thread carries
   // PULL_ILP vertices (one per 256-vertex tile of its workgroup's span)
@@ -816,7 +823,7 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
   for (uint64_t base = (uint64_t)blockIdx.x * SPAN; base < n_pad; base += (uint64_t)gridDim.x * SPAN) {
     uint32_t vw[PULL_ILP], pv[PULL_ILP];
     uint64_t it[PULL_ILP], end[PULL_ILP];
-    bool act[PULL_ILP], hit[PULL_ILP];
+    bool act[PULL_ILP], hit[PULL_ILP], cnd[PULL_ILP];
 #pragma unroll
     for (int k = 0; k < PULL_ILP; ++k) {
       const uint64_t v = base + (uint64_t)k * WG + threadIdx.x;
@@ -825,6 +832,7 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
       const uint32_t cw = (v < n) ? cand[word] : 0u;
       const uint32_t bit = 1u << (v & 31);
       act[k] = (cw & bit) && !(vw[k] & bit);
+      cnd[k] = act[k];
       hit[k] = false;
       it[k] = end[k] = 0;
       pv[k] = 0;
@@ -872,19 +880,24 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
         ++claimed;
         if (noff) nd += noff[v + 1] - noff[v];
       }
+      const bool left = cnd[k] && !hit[k];
+      if (sbm && left) ++nd;
       // only this wavefront sets bits in its two words during a pull level,
       // so the claim is a plain store of (old | new) per word
       const uint64_t mask = __ballot(hit[k]);
+      const uint64_t lmask = __ballot(left);
       const uint64_t first = v - lane;  // the wavefront's first vertex of tile k
       const uint64_t word0 = first >> 5;
       if (first < n) {  // the padding past n owns no bitmap words
         if (lane == 0) {
           fnew[word0] = (uint32_t)mask;
           if ((uint32_t)mask) vis[word0] = vw[k] | (uint32_t)mask;
+          if (sbm) sbm[word0] = (uint32_t)lmask;
         }
         if (lane == 32) {
           fnew[word0 + 1] = (uint32_t)(mask >> 32);
           if ((uint32_t)(mask >> 32)) vis[word0 + 1] = vw[k] | (uint32_t)(mask >> 32);
+          if (sbm) sbm[word0 + 1] = (uint32_t)(lmask >> 32);
         }
       }
     }
@@ -896,7 +909,106 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
     part[3 * blockIdx.x] = claimed;
     part[3 * blockIdx.x + 1] = edges;
     part[3 * blockIdx.x + 2] = nd;
-    if (blockIdx.x == 0) *cursor = 0;  // for a compaction of this level's bitmap
+    if (blockIdx.x == 0) {
+      *cursor = 0;  // for a compaction of this level's bitmap
+      if (lcnt) lcnt[0] = lcnt[1] = 0;
+    }
+  }
+}
+
+// A pull level over a list of candidates: the vertices the previous pull
+// level left unreached (each once, none reached yet). The scan is k_pull's;
+// a claim sets the vertex's bit with an atomic OR (list neighbours need not
+// share a bitmap word) and appends it to `claimed` (the next push level's
+// frontier). Candidates still unreached go to `surv`. Partials as k_pull's,
+// without a degree sum.
+template <int DIR>
+__global__ __launch_bounds__(WG) void k_pull_list(const uint32_t* __restrict__ list, const uint32_t* list_cnt,
+                                                  uint32_t* vis, const uint64_t* __restrict__ roff,
+                                                  const uint32_t* __restrict__ radj, const uint32_t* __restrict__ pid,
+                                                  uint32_t* __restrict__ claimed_q, uint32_t* claimed_cnt,
+                                                  uint32_t* __restrict__ surv, uint32_t* surv_cnt,
+                                                  uint64_t* __restrict__ part) {
+  __shared__ uint64_t s_red[WG / WAVE];
+  __shared__ uint32_t s_cq[PULL_QCAP], s_sq[PULL_QCAP];
+  __shared__ uint32_t s_cn, s_cb, s_sn, s_sb;
+  BlockQueue cq{s_cq, &s_cn, &s_cb, PULL_QCAP}, sq{s_sq, &s_sn, &s_sb, PULL_QCAP};
+  bq_init(cq);
+  bq_init(sq);
+  uint32_t iter = 0;
+  uint64_t claimed = 0, edges = 0;
+  constexpr uint32_t SPAN = (uint32_t)WG * PULL_ILP;
+  const uint32_t cnt = *list_cnt;
+  const uint32_t cnt_pad = (cnt + SPAN - 1) / SPAN * SPAN;
+  for (uint32_t base = blockIdx.x * SPAN; base < cnt_pad; base += gridDim.x * SPAN) {
+    uint32_t vv[PULL_ILP], pv[PULL_ILP];
+    uint64_t it[PULL_ILP], end[PULL_ILP];
+    bool act[PULL_ILP], hit[PULL_ILP], cnd[PULL_ILP];
+#pragma unroll
+    for (int k = 0; k < PULL_ILP; ++k) {
+      const uint32_t i = base + (uint32_t)k * WG + threadIdx.x;
+      cnd[k] = act[k] = i < cnt;
+      vv[k] = act[k] ? list[i] : 0u;
+      hit[k] = false;
+      it[k] = end[k] = 0;
+      pv[k] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < PULL_ILP; ++k) {
+      if (act[k]) {
+        it[k] = roff[vv[k]];
+        end[k] = roff[vv[k] + 1];
+        if (pid) pv[k] = pid[vv[k]];
+      }
+    }
+    for (;;) {
+      bool more = false;
+      uint32_t u[PULL_ILP];
+#pragma unroll
+      for (int k = 0; k < PULL_ILP; ++k) {
+        act[k] = act[k] && it[k] < end[k];
+        more |= act[k];
+        u[k] = act[k] ? radj[it[k]] : 0u;
+      }
+      if (!more) break;
+      uint32_t uw[PULL_ILP];
+#pragma unroll
+      for (int k = 0; k < PULL_ILP; ++k) uw[k] = act[k] ? vis[u[k] >> 5] : 0u;
+#pragma unroll
+      for (int k = 0; k < PULL_ILP; ++k) {
+        if (!act[k]) continue;
+        ++edges;
+        ++it[k];
+        if ((uw[k] >> (u[k] & 31)) & 1u) {
+          if (!pid || pid[u[k]] == pv[k]) {
+            hit[k] = true;
+            act[k] = false;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PULL_ILP; ++k) {
+      if (hit[k]) {
+        ++claimed;
+        atomicOr(&vis[vv[k] >> 5], 1u << (vv[k] & 31));
+      }
+      bq_push(cq, hit[k], vv[k]);
+      bq_push(sq, cnd[k] && !hit[k], vv[k]);
+    }
+    if ((++iter % PULL_FLUSH) == 0) {
+      bq_maybe_flush(cq, claimed_q, claimed_cnt, PULL_FLUSH * SPAN);
+      bq_maybe_flush(sq, surv, surv_cnt, PULL_FLUSH * SPAN);
+    }
+  }
+  bq_flush(cq, claimed_q, claimed_cnt);
+  bq_flush(sq, surv, surv_cnt);
+  claimed = block_sum64(claimed, s_red);
+  edges = block_sum64(edges, s_red);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = claimed;
+    part[3 * blockIdx.x + 1] = edges;
+    part[3 * blockIdx.x + 2] = 0;
   }
 }
 
@@ -951,12 +1063,15 @@ __global__ __launch_bounds__(1024) void k_deg_scan_small(const uint32_t* __restr
 }
 
 // zero a queue counter and the 64-bit level counters
-__global__ void k_level_reset(uint32_t* cnt_slot, unsigned long long* cnt64) {
+__global__ void k_level_reset(uint32_t* cnt_slot, unsigned long long* cnt64, uint32_t* s0 = nullptr,
+                              uint32_t* s1 = nullptr) {
   if (threadIdx.x == 0) {
     *cnt_slot = 0;
     cnt64[0] = 0;
     cnt64[1] = 0;
     cnt64[2] = 0;
+    if (s0) *s0 = 0;
+    if (s1) *s1 = 0;
   }
 }
 
@@ -2033,6 +2148,7 @@ int ctx_reserve(Ctx* c, uint32_t n, uint64_t m) {
   KSPN_TRY(dalloc(&c->bwv, words));
   KSPN_TRY(dalloc(&c->inq, words));
   KSPN_TRY(dalloc(&c->fnew, words));
+  KSPN_TRY(dalloc(&c->sbm, words));
   KSPN_TRY(dalloc(&c->qa, (uint64_t)nn + 64));
   KSPN_TRY(dalloc(&c->qb, (uint64_t)nn + 64));
   KSPN_TRY(dalloc(&c->qc, (uint64_t)nn + 64));
@@ -2102,9 +2218,9 @@ int wait_published(Ctx* c, hipStream_t s, uint32_t seq) {
 // With part: reduce nparts per-workgroup partials first (into *pcount and
 // cnt64[0..1]); with reset: zero *reset and cnt64[0..2] afterwards.
 int publish(Ctx* c, hipStream_t s, const uint64_t* extra, const uint64_t* part = nullptr, uint32_t nparts = 0,
-            uint32_t* pcount = nullptr, uint32_t* reset = nullptr) {
+            uint32_t* pcount = nullptr, uint32_t* reset = nullptr, uint32_t* reset2 = nullptr) {
   const uint32_t seq = ++c->pub_seq;
-  k_publish<<<1, WG, 0, s>>>(c->cnt, c->cnt64, extra, c->d_map, seq, part, nparts, pcount, reset);
+  k_publish<<<1, WG, 0, s>>>(c->cnt, c->cnt64, extra, c->d_map, seq, part, nparts, pcount, reset, reset2);
   KSPN_HIP(hipGetLastError());
   KSPN_TRY(wait_published(c, s, seq));
   for (int i = 0; i < C_COUNT_; ++i) c->h_cnt[i] = c->h_map[i];
@@ -2297,6 +2413,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   uint32_t* N = c->qb;
   int fc = C_Q0, nc = C_Q1;
   uint32_t f = f0;
+  uint32_t surv_n = 0;  // candidates the last pull level left unreached
   uint64_t edges = 0;  // out-edges of the frontier in this direction (unknown for the pivot level)
   bool known = false;
   bool pull = false;
@@ -2306,8 +2423,18 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   uint32_t maxd = 0xFFFFFFFFu;   // largest out-degree in the frontier (unknown)
   const bool coop = R.coop_wg > 0 && !(R.o.flags & KSPN_FLAG_NO_COOP);
   const int op = dir == 0 ? OP_FW : (pid ? OP_PBW : OP_BW);
+  // A pull sweep leaves its unreached candidates in the bitmap sbm. When
+  // they are few, the next pull level visits only them: they are compacted
+  // into list 0 (qc; qc and ulist are free while reaching), and each list
+  // level writes the ones it leaves into the other list. A list level's
+  // publication zeroes its input list's counter; a sweep zeroes both.
+  uint32_t* lists[2] = {c->qc, c->ulist};
+  const int lcnt[2] = {C_SURV0, C_SURV1};
+  const uint32_t list_max = std::max<uint32_t>(R.n / 16, 1u);  // longest remainder worth a list
+  int cur = -1;             // the list holding the next pull level's candidates; -1: sweep
+  bool list_level = false;  // the level just run visited a list
   // each level's publication zeroes the counters of the level after it
-  k_level_reset<<<1, 64, 0, R.s>>>(&c->cnt[nc], c->cnt64);
+  k_level_reset<<<1, 64, 0, R.s>>>(&c->cnt[nc], c->cnt64, &c->cnt[C_SURV0], &c->cnt[C_SURV1]);
   while (f > 0) {
     // direction: Beamer's rule on the frontier's edge count, which the
     // previous level summed on the device for the vertices it claimed
@@ -2353,22 +2480,42 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       if (fw_levels_push) (*fw_levels_push)++;
     } else {
       int kern = dir == 0 ? K_PULL_FW : K_PULL_BW;
-      uint32_t grid = stream_grid(c, R.n);
-      {
+      uint32_t grid;
+      list_level = cur >= 0;
+      if (list_level) {
+        // the previous pull level's unreached candidates, `surv_n` of them
+        grid = stream_grid(c, surv_n / PULL_ILP + 1);
+        Prof p(c, R.s, kern, 24.0 * surv_n);
+        if (dir == 0)
+          k_pull_list<0><<<grid, WG, 0, R.s>>>(lists[cur], &c->cnt[lcnt[cur]], vis, roff, radj, pid, N,
+                                               &c->cnt[nc], lists[1 - cur], &c->cnt[lcnt[1 - cur]], c->part);
+        else
+          k_pull_list<1><<<grid, WG, 0, R.s>>>(lists[cur], &c->cnt[lcnt[cur]], vis, roff, radj, pid, N,
+                                               &c->cnt[nc], lists[1 - cur], &c->cnt[lcnt[1 - cur]], c->part);
+      } else {
+        grid = stream_grid(c, R.n);
         Prof p(c, R.s, kern, (double)R.n / 4.0 + 16.0 * R.live_n);
         // no out-degree sum of the claimed vertices: it would read a third
         // offset array at scattered rows, and the first push level after the
         // pull levels sums its frontier's degrees itself
         if (dir == 0)
           k_pull<0><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
-                                          &c->cnt[C_CURSOR]);
+                                          &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0]);
         else
           k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
-                                          &c->cnt[C_CURSOR]);
+                                          &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0]);
       }
       if (fw_levels_pull) (*fw_levels_pull)++;
-      // the workgroup partials are reduced by the publication
-      KSPN_TRY(publish(c, R.s, nullptr, c->part, grid, &c->cnt[nc], &c->cnt[fc]));
+      // the workgroup partials are reduced by the publication, which also
+      // zeroes a list level's input counter
+      KSPN_TRY(publish(c, R.s, nullptr, c->part, grid, &c->cnt[nc], &c->cnt[fc],
+                       list_level ? &c->cnt[lcnt[cur]] : nullptr));
+      if (list_level) {
+        surv_n = c->h_cnt[lcnt[1 - cur]];
+        cur = 1 - cur;
+      } else {
+        surv_n = (uint32_t)c->h_u64[2];  // the sweep's third partial: candidates left
+      }
     }
     if (!pull) KSPN_TRY(publish(c, R.s, c->scan + f, nullptr, 0, nullptr, &c->cnt[fc]));
     if (!pull) {
@@ -2388,11 +2535,22 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
     // level sums them; the remaining-edge estimate keeps them until expanded
     known = !pull;
     if (pull && nf > 0 && (double)nf * beta < (double)R.live_n) {
-      // back to push: the frontier is the bitmap of the last pull level (its
-      // size nf is already in cnt[nc]; the pull zeroed the cursor)
+      // back to push: the frontier is what the last pull level claimed. A
+      // list level appended it to N already; a sweep left it in the bitmap
+      // fnew (its size nf is already in cnt[nc]; the pull zeroed the cursor)
       pull = false;
-      Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words + 4.0 * nf);
-      k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->fnew, R.words, N, &c->cnt[C_CURSOR]);
+      cur = -1;
+      if (!list_level) {
+        Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words + 4.0 * nf);
+        k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->fnew, R.words, N, &c->cnt[C_CURSOR]);
+      }
+    } else if (pull && nf > 0 && !list_level && surv_n <= list_max) {
+      // few candidates left: the next pull level visits a list of them
+      // (the sweep zeroed both list counters)
+      Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words + 4.0 * surv_n);
+      k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->sbm, R.words, lists[0],
+                                                                         &c->cnt[lcnt[0]]);
+      cur = 0;
     }
     std::swap(F, N);
     std::swap(fc, nc);
@@ -3299,7 +3457,7 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   if (!k) return KSPN_OK;
   Ctx* c = &k->c;
   (void)hipSetDevice(c->device);
-  void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
+  void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->sbm, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
                   c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf, c->pid, c->pbest,
                   c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt};
   for (void* p : ptrs)
