@@ -222,8 +222,9 @@ enum Counter : int {
   C_DTOT,  // running length of the decided list
   C_ERR,   // set by a kernel that met a broken internal invariant
   C_CURSOR,  // write cursor of an unordered bitmap compaction
-  C_SURV0,   // sizes of the two lists of vertices a pull level left unreached
+  C_SURV0,   // sizes of the candidate lists of pull list levels (three, in rotation)
   C_SURV1,
+  C_SURV2,
   C_COUNT_ = 16
 };
 

@@ -327,6 +327,8 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
   if (f == 0) return;
   const uint64_t total = S[f];
   const uint64_t ntiles = (total + EXP_TILE - 1) / EXP_TILE;
+  // a grid sized before the edge total was known leaves blocks without a tile
+  if (blockIdx.x >= ntiles) return;
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const uint64_t e0 = t * EXP_TILE;
     const uint64_t e1 = min(e0 + (uint64_t)EXP_TILE, total);
@@ -800,7 +802,7 @@ __global__ __launch_bounds__(WG) void k_pivot_reduce(const uint64_t* part, uint3
 // (noff, optional) of the claimed vertices, or with `sbm` the number of
 // candidates left unreached, whose bits it receives (written like fnew) so
 // that a short remainder can be compacted into a list for k_pull_list.
-// Block 0 zeroes the two list counters `lcnt`, which the sweep does not use.
+// Block 0 zeroes the three list counters `lcnt`, which the sweep does not use.
 constexpr int PULL_ILP = 2;  // vertices in flight per thread (2 beat 1 and 4 on R-MAT 24 and the 4096-graph batch)
 constexpr uint32_t PULL_FLUSH = 4;                     // k_pull_list: iterations between checks of its LDS queues
 constexpr uint32_t PULL_QCAP = PULL_FLUSH * WG * PULL_ILP;  // entries of each LDS queue (8 KiB)
@@ -911,7 +913,7 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
     part[3 * blockIdx.x + 2] = nd;
     if (blockIdx.x == 0) {
       *cursor = 0;  // for a compaction of this level's bitmap
-      if (lcnt) lcnt[0] = lcnt[1] = 0;
+      if (lcnt) lcnt[0] = lcnt[1] = lcnt[2] = 0;
     }
   }
 }
@@ -919,24 +921,28 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
 // A pull level over a list of candidates: the vertices the previous pull
 // level left unreached (each once, none reached yet). The scan is k_pull's;
 // a claim sets the vertex's bit with an atomic OR (list neighbours need not
-// share a bitmap word) and appends it to `claimed` (the next push level's
-// frontier). Candidates still unreached go to `surv`. Partials as k_pull's,
-// without a degree sum.
+// share a bitmap word) and, when `claimed_q` is given, appends it there
+// (the frontier of a push level that may follow). Candidates still
+// unreached go to `out`. Several list levels run back to back without the
+// host: each reads its candidate count from the device, block 0 zeroes the
+// counter `zero` (neither read nor written here: the next level's output),
+// and every block adds its scanned edges to *edges.
 template <int DIR>
 __global__ __launch_bounds__(WG) void k_pull_list(const uint32_t* __restrict__ list, const uint32_t* list_cnt,
                                                   uint32_t* vis, const uint64_t* __restrict__ roff,
                                                   const uint32_t* __restrict__ radj, const uint32_t* __restrict__ pid,
                                                   uint32_t* __restrict__ claimed_q, uint32_t* claimed_cnt,
-                                                  uint32_t* __restrict__ surv, uint32_t* surv_cnt,
-                                                  uint64_t* __restrict__ part) {
+                                                  uint32_t* __restrict__ out, uint32_t* out_cnt, uint32_t* zero,
+                                                  unsigned long long* edges_sum) {
   __shared__ uint64_t s_red[WG / WAVE];
   __shared__ uint32_t s_cq[PULL_QCAP], s_sq[PULL_QCAP];
   __shared__ uint32_t s_cn, s_cb, s_sn, s_sb;
   BlockQueue cq{s_cq, &s_cn, &s_cb, PULL_QCAP}, sq{s_sq, &s_sn, &s_sb, PULL_QCAP};
   bq_init(cq);
   bq_init(sq);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
   uint32_t iter = 0;
-  uint64_t claimed = 0, edges = 0;
+  uint64_t edges = 0;
   constexpr uint32_t SPAN = (uint32_t)WG * PULL_ILP;
   const uint32_t cnt = *list_cnt;
   const uint32_t cnt_pad = (cnt + SPAN - 1) / SPAN * SPAN;
@@ -989,27 +995,19 @@ __global__ __launch_bounds__(WG) void k_pull_list(const uint32_t* __restrict__ l
     }
 #pragma unroll
     for (int k = 0; k < PULL_ILP; ++k) {
-      if (hit[k]) {
-        ++claimed;
-        atomicOr(&vis[vv[k] >> 5], 1u << (vv[k] & 31));
-      }
-      bq_push(cq, hit[k], vv[k]);
+      if (hit[k]) atomicOr(&vis[vv[k] >> 5], 1u << (vv[k] & 31));
+      if (claimed_q) bq_push(cq, hit[k], vv[k]);
       bq_push(sq, cnd[k] && !hit[k], vv[k]);
     }
     if ((++iter % PULL_FLUSH) == 0) {
-      bq_maybe_flush(cq, claimed_q, claimed_cnt, PULL_FLUSH * SPAN);
-      bq_maybe_flush(sq, surv, surv_cnt, PULL_FLUSH * SPAN);
+      if (claimed_q) bq_maybe_flush(cq, claimed_q, claimed_cnt, PULL_FLUSH * SPAN);
+      bq_maybe_flush(sq, out, out_cnt, PULL_FLUSH * SPAN);
     }
   }
-  bq_flush(cq, claimed_q, claimed_cnt);
-  bq_flush(sq, surv, surv_cnt);
-  claimed = block_sum64(claimed, s_red);
+  if (claimed_q) bq_flush(cq, claimed_q, claimed_cnt);
+  bq_flush(sq, out, out_cnt);
   edges = block_sum64(edges, s_red);
-  if (threadIdx.x == 0) {
-    part[3 * blockIdx.x] = claimed;
-    part[3 * blockIdx.x + 1] = edges;
-    part[3 * blockIdx.x + 2] = 0;
-  }
+  if (threadIdx.x == 0 && edges) atomicAdd(edges_sum, (unsigned long long)edges);
 }
 
 // sums of the three per-workgroup partials: [0] -> *count, [1] -> cnt64[0], [2] -> cnt64[1]
@@ -2424,17 +2422,23 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   const bool coop = R.coop_wg > 0 && !(R.o.flags & KSPN_FLAG_NO_COOP);
   const int op = dir == 0 ? OP_FW : (pid ? OP_PBW : OP_BW);
   // A pull sweep leaves its unreached candidates in the bitmap sbm. When
-  // they are few, the next pull level visits only them: they are compacted
-  // into list 0 (qc; qc and ulist are free while reaching), and each list
-  // level writes the ones it leaves into the other list. A list level's
-  // publication zeroes its input list's counter; a sweep zeroes both.
+  // they are few, the following pull levels visit only them: they are
+  // compacted into a list, and list levels run in batches of LIST_BATCH
+  // launches without a host round trip, each passing the candidates it
+  // leaves to the next (lists alternate between qc and ulist, both free
+  // while reaching; their counters rotate over three slots, which the sweep
+  // zeroes). The last level of a batch also records what it claimed, the
+  // frontier if push levels follow; a batch whose last level claims nothing
+  // has reached everything.
+  constexpr uint32_t LIST_BATCH = 4;
   uint32_t* lists[2] = {c->qc, c->ulist};
-  const int lcnt[2] = {C_SURV0, C_SURV1};
+  const int lcnt[3] = {C_SURV0, C_SURV1, C_SURV2};
   const uint32_t list_max = std::max<uint32_t>(R.n / 16, 1u);  // longest remainder worth a list
-  int cur = -1;             // the list holding the next pull level's candidates; -1: sweep
-  bool list_level = false;  // the level just run visited a list
+  bool list_mode = false;   // the next pull level visits a list
+  bool list_level = false;  // the level just run visited lists
+  uint32_t L = 0;           // list levels since the last sweep
   // each level's publication zeroes the counters of the level after it
-  k_level_reset<<<1, 64, 0, R.s>>>(&c->cnt[nc], c->cnt64, &c->cnt[C_SURV0], &c->cnt[C_SURV1]);
+  k_level_reset<<<1, 64, 0, R.s>>>(&c->cnt[nc], c->cnt64);
   while (f > 0) {
     // direction: Beamer's rule on the frontier's edge count, which the
     // previous level summed on the device for the vertices it claimed
@@ -2480,18 +2484,27 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       if (fw_levels_push) (*fw_levels_push)++;
     } else {
       int kern = dir == 0 ? K_PULL_FW : K_PULL_BW;
-      uint32_t grid;
-      list_level = cur >= 0;
+      uint32_t grid = 0;
+      list_level = list_mode;
       if (list_level) {
-        // the previous pull level's unreached candidates, `surv_n` of them
-        grid = stream_grid(c, surv_n / PULL_ILP + 1);
-        Prof p(c, R.s, kern, 24.0 * surv_n);
-        if (dir == 0)
-          k_pull_list<0><<<grid, WG, 0, R.s>>>(lists[cur], &c->cnt[lcnt[cur]], vis, roff, radj, pid, N,
-                                               &c->cnt[nc], lists[1 - cur], &c->cnt[lcnt[1 - cur]], c->part);
-        else
-          k_pull_list<1><<<grid, WG, 0, R.s>>>(lists[cur], &c->cnt[lcnt[cur]], vis, roff, radj, pid, N,
-                                               &c->cnt[nc], lists[1 - cur], &c->cnt[lcnt[1 - cur]], c->part);
+        // at most `surv_n` candidates in each level of the batch
+        const uint32_t lgrid = stream_grid(c, surv_n / PULL_ILP + 1);
+        for (uint32_t j = 0; j < LIST_BATCH; ++j, ++L) {
+          uint32_t* rec = j + 1 == LIST_BATCH ? N : nullptr;
+          Prof p(c, R.s, kern, 24.0 * surv_n);
+          if (dir == 0)
+            k_pull_list<0><<<lgrid, WG, 0, R.s>>>(lists[L % 2], &c->cnt[lcnt[L % 3]], vis, roff, radj, pid, rec,
+                                                  &c->cnt[nc], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 3]],
+                                                  &c->cnt[lcnt[(L + 2) % 3]], c->cnt64);
+          else
+            k_pull_list<1><<<lgrid, WG, 0, R.s>>>(lists[L % 2], &c->cnt[lcnt[L % 3]], vis, roff, radj, pid, rec,
+                                                  &c->cnt[nc], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 3]],
+                                                  &c->cnt[lcnt[(L + 2) % 3]], c->cnt64);
+          if (fw_levels_pull) (*fw_levels_pull)++;
+        }
+        // cnt[nc] counts the last level's claims; cnt64[0] the edges scanned
+        KSPN_TRY(publish(c, R.s, nullptr, nullptr, 0, nullptr, &c->cnt[fc]));
+        surv_n = c->h_cnt[lcnt[L % 3]];
       } else {
         grid = stream_grid(c, R.n);
         Prof p(c, R.s, kern, (double)R.n / 4.0 + 16.0 * R.live_n);
@@ -2505,15 +2518,10 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
           k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
                                           &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0]);
       }
-      if (fw_levels_pull) (*fw_levels_pull)++;
-      // the workgroup partials are reduced by the publication, which also
-      // zeroes a list level's input counter
-      KSPN_TRY(publish(c, R.s, nullptr, c->part, grid, &c->cnt[nc], &c->cnt[fc],
-                       list_level ? &c->cnt[lcnt[cur]] : nullptr));
-      if (list_level) {
-        surv_n = c->h_cnt[lcnt[1 - cur]];
-        cur = 1 - cur;
-      } else {
+      if (!list_level) {
+        if (fw_levels_pull) (*fw_levels_pull)++;
+        // the workgroup partials are reduced by the publication
+        KSPN_TRY(publish(c, R.s, nullptr, c->part, grid, &c->cnt[nc], &c->cnt[fc]));
         surv_n = (uint32_t)c->h_u64[2];  // the sweep's third partial: candidates left
       }
     }
@@ -2539,18 +2547,19 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       // list level appended it to N already; a sweep left it in the bitmap
       // fnew (its size nf is already in cnt[nc]; the pull zeroed the cursor)
       pull = false;
-      cur = -1;
+      list_mode = false;
       if (!list_level) {
         Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words + 4.0 * nf);
         k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->fnew, R.words, N, &c->cnt[C_CURSOR]);
       }
     } else if (pull && nf > 0 && !list_level && surv_n <= list_max) {
-      // few candidates left: the next pull level visits a list of them
-      // (the sweep zeroed both list counters)
+      // few candidates left: the next pull levels visit a list of them (the
+      // sweep zeroed the three list counters)
       Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words + 4.0 * surv_n);
       k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->sbm, R.words, lists[0],
                                                                          &c->cnt[lcnt[0]]);
-      cur = 0;
+      L = 0;
+      list_mode = true;
     }
     std::swap(F, N);
     std::swap(fc, nc);
