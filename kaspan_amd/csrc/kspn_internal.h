@@ -214,7 +214,7 @@ enum Counter : int {
   C_Q1,       // pong
   C_DECIDED,  // decisions made by a launch
   C_AUX,
-  C_TRIM2,
+  C_ROOM,    // the tail's sub-CSR did not fit its buffer
   C_RES_N,
   C_FLAG,
   C_MIN,   // minimum vertex of a decided set
@@ -225,6 +225,7 @@ enum Counter : int {
   C_SURV0,   // sizes of the candidate lists of pull list levels (three, in rotation)
   C_SURV1,
   C_SURV2,
+  C_ISO,     // live vertices without a live neighbour, counted by the tail preparation
   C_COUNT_ = 16
 };
 

@@ -116,6 +116,10 @@ __global__ __launch_bounds__(WG) void k_publish(uint32_t* cnt, unsigned long lon
   }
 }
 
+__global__ void k_zero3(uint32_t* a, uint32_t* b, uint32_t* c) {
+  if (threadIdx.x == 0) *a = *b = *c = 0;
+}
+
 __global__ void k_zero2(uint32_t* a, uint32_t* b) {
   if (threadIdx.x == 0) {
     *a = 0;
@@ -1499,15 +1503,23 @@ __global__ void k_tarjan(uint32_t n, const uint64_t* __restrict__ off, const uin
 // member as the label of every vertex decided in the round. state: 0 live,
 // 2 decided this round, 1 decided. Rounds are capped by max_rounds (0: none);
 // *status = 1 if the cap was hit.
-__global__ __launch_bounds__(1024) void k_tail_solve(uint32_t sn, const uint32_t* __restrict__ super,
+__global__ __launch_bounds__(1024) void k_tail_solve(const uint32_t* sn_ptr, const uint32_t* __restrict__ super,
                                                      const uint32_t* __restrict__ subid,
-                                                     const uint64_t* __restrict__ fo, const uint32_t* __restrict__ fa,
-                                                     const uint64_t* __restrict__ bo, const uint32_t* __restrict__ ba,
+                                                     const uint64_t* __restrict__ fo, const uint64_t* __restrict__ bo,
+                                                     const uint32_t* __restrict__ sub, uint64_t cap,
                                                      uint32_t* col, uint32_t* state, uint32_t* minv,
                                                      uint32_t* __restrict__ labels, uint32_t r0, uint32_t max_rounds,
-                                                     uint32_t* status, uint32_t* live) {
+                                                     uint32_t* status, uint32_t* live, const uint32_t* skip,
+                                                     const uint32_t* room) {
   __shared__ uint32_t s_flag, s_live;
   const uint32_t tid = threadIdx.x, T = blockDim.x;
+  // the sizes and the sub-CSR come from the launches before (k_tail_fill)
+  const uint32_t sn = *sn_ptr;
+  if (*skip || *room) return;
+  const uint64_t sm = fo[sn];
+  if (2 * sm + 2 > cap) return;
+  const uint32_t* __restrict__ fa = sub;
+  const uint32_t* __restrict__ ba = sub + sm + 1;
   for (uint32_t i = tid; i < sn; i += T) state[i] = 0;
   uint32_t r = r0;
   for (uint32_t round = 0;; ++round) {
@@ -1588,37 +1600,89 @@ __global__ __launch_bounds__(1024) void k_tail_solve(uint32_t sn, const uint32_t
 }
 
 // tail subgraph, one pass: compact ids, live out- and in-degrees (entry u = 0)
-__global__ void k_tail_prep(const uint32_t* __restrict__ U, const uint32_t* ucount, const uint64_t* __restrict__ fw_off,
-                            const uint32_t* __restrict__ fw_adj, const uint64_t* __restrict__ bw_off,
-                            const uint32_t* __restrict__ bw_adj, const uint32_t* __restrict__ live,
-                            uint32_t* __restrict__ subid, uint64_t* __restrict__ degf, uint64_t* __restrict__ degb) {
+// Nothing is done when there are more than `limit` live vertices (the
+// tail scan reports that case). *iso counts live vertices without a live
+// neighbour in either direction: trim_1_normal's decisions.
+__global__ void k_tail_prep(const uint32_t* __restrict__ U, const uint32_t* ucount, uint32_t limit,
+                            const uint64_t* __restrict__ fw_off, const uint32_t* __restrict__ fw_adj,
+                            const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
+                            const uint32_t* __restrict__ live, uint32_t* __restrict__ subid,
+                            uint64_t* __restrict__ degf, uint64_t* __restrict__ degb, uint32_t* iso) {
   const uint32_t u = *ucount;
-  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i <= u; i += gridDim.x * WG) {
+  if (u > limit) return;
+  const uint32_t u_pad = (u + 1 + WG - 1) / WG * WG;
+  for (uint32_t base = blockIdx.x * WG; base < u_pad; base += gridDim.x * WG) {
+    const uint32_t i = base + threadIdx.x;
+    bool alone = false;
     if (i == u) {
       degf[i] = 0;
       degb[i] = 0;
-      continue;
+    } else if (i < u) {
+      const uint32_t v = U[i];
+      subid[v] = i;
+      uint64_t df = 0, db = 0;
+      for (uint64_t it = fw_off[v], end = fw_off[v + 1]; it < end; ++it) df += bm_get(live, fw_adj[it]);
+      for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it) db += bm_get(live, bw_adj[it]);
+      degf[i] = df;
+      degb[i] = db;
+      alone = df == 0 && db == 0;
     }
-    const uint32_t v = U[i];
-    subid[v] = i;
-    uint64_t df = 0, db = 0;
-    for (uint64_t it = fw_off[v], end = fw_off[v + 1]; it < end; ++it) df += bm_get(live, fw_adj[it]);
-    for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it) db += bm_get(live, bw_adj[it]);
-    degf[i] = df;
-    degb[i] = db;
+    if (iso) wave_count(alone, iso);
   }
+}
+
+// trim_1_normal over a list of live vertices, for the tail: a vertex with
+// no live neighbour in either direction is decided (own id, live bit
+// cleared); the others go to `rest`, staged per workgroup in LDS. Removing
+// such a vertex changes no other vertex's test: it is nobody's live
+// neighbour.
+__global__ __launch_bounds__(WG) void k_tail_split(const uint32_t* __restrict__ L, const uint32_t* lcount,
+                                                   const uint64_t* __restrict__ fw_off,
+                                                   const uint32_t* __restrict__ fw_adj,
+                                                   const uint64_t* __restrict__ bw_off,
+                                                   const uint32_t* __restrict__ bw_adj, uint32_t* live,
+                                                   uint32_t* __restrict__ labels, uint32_t* __restrict__ rest,
+                                                   uint32_t* rest_cnt, uint32_t* iso_cnt) {
+  __shared__ uint32_t s_q[2 * WG];
+  __shared__ uint32_t s_n, s_b;
+  BlockQueue q{s_q, &s_n, &s_b, 2 * WG};
+  bq_init(q);
+  const uint32_t cnt = *lcount;
+  const uint32_t cnt_pad = (cnt + WG - 1) / WG * WG;
+  for (uint32_t base = blockIdx.x * WG; base < cnt_pad; base += gridDim.x * WG) {
+    const uint32_t i = base + threadIdx.x;
+    bool alone = false, keep = false;
+    uint32_t v = 0;
+    if (i < cnt) {
+      v = L[i];
+      bool any = false;
+      for (uint64_t it = fw_off[v], end = fw_off[v + 1]; it < end && !any; ++it) any = bm_get(live, fw_adj[it]);
+      for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end && !any; ++it) any = bm_get(live, bw_adj[it]);
+      alone = !any;
+      keep = any;
+    }
+    if (alone) {
+      labels[v] = v;
+      atomicAnd(&live[v >> 5], ~(1u << (v & 31)));
+    }
+    wave_count(alone, iso_cnt);
+    bq_push(q, keep, v);
+    bq_maybe_flush(q, rest, rest_cnt, WG);  // at most WG appends per iteration
+  }
+  bq_flush(q, rest, rest_cnt);
 }
 
 // in-place exclusive scans of two arrays of u+1 <= 4096 entries in one
 // workgroup (entry u becomes the total); *flag = 0 for the tail solver. A
-// larger u (*ucount) is reported through *err and nothing is written.
-__global__ __launch_bounds__(1024) void k_tail_scan(const uint32_t* ucount, uint64_t* __restrict__ a,
+// u (*ucount) above `limit` (at most 4095) is reported through *err and
+// nothing is written.
+__global__ __launch_bounds__(1024) void k_tail_scan(const uint32_t* ucount, uint32_t limit, uint64_t* __restrict__ a,
                                                     uint64_t* __restrict__ b, uint32_t* flag, uint32_t* err) {
   __shared__ uint64_t sa[1024], sb[1024];
   const uint32_t t = threadIdx.x;
   const uint32_t u = *ucount;
   if (t == 0) *flag = 0;
-  if (u >= 4096) {
+  if (u > limit || u >= 4096) {
     if (t == 0) *err = 1;
     return;
   }
@@ -1655,12 +1719,24 @@ __global__ __launch_bounds__(1024) void k_tail_scan(const uint32_t* ucount, uint
 }
 
 // both directions' rows of the tail subgraph in one launch
+// The sub-CSR's two adjacency arrays share `sub` (cap entries): forward
+// at 0, backward after the forward total fo[u] and one spare entry. Nothing
+// is done when *skip is set; when they do not fit, *room is set instead.
 __global__ void k_tail_fill(const uint32_t* __restrict__ U, const uint32_t* ucount, const uint64_t* __restrict__ fw_off,
                             const uint32_t* __restrict__ fw_adj, const uint64_t* __restrict__ bw_off,
                             const uint32_t* __restrict__ bw_adj, const uint32_t* __restrict__ live,
-                            const uint32_t* __restrict__ subid, const uint64_t* __restrict__ fo, uint32_t* __restrict__ fa,
-                            const uint64_t* __restrict__ bo, uint32_t* __restrict__ ba) {
+                            const uint32_t* __restrict__ subid, const uint64_t* __restrict__ fo,
+                            const uint64_t* __restrict__ bo, uint32_t* __restrict__ sub, uint64_t cap,
+                            const uint32_t* skip, uint32_t* room) {
   const uint32_t u = *ucount;
+  if (*skip) return;
+  const uint64_t sm = fo[u];
+  if (2 * sm + 2 > cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *room = 1;
+    return;
+  }
+  uint32_t* __restrict__ fa = sub;
+  uint32_t* __restrict__ ba = sub + sm + 1;
   for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < u; i += gridDim.x * WG) {
     const uint32_t v = U[i];
     uint64_t p = fo[i];
@@ -2568,7 +2644,9 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   return KSPN_OK;
 }
 
-int fwbw(Run& R, uint32_t pivot, uint32_t* decided) {
+// With publish = false the decided count stays on the device
+// (cnt[C_DECIDED]) for the caller's next publication.
+int fwbw(Run& R, uint32_t pivot, uint32_t* decided, bool publish = true) {
   Ctx* c = R.c;
   {
     Prof p(c, R.s, K_BITMAP, 4.0 * R.words);
@@ -2592,13 +2670,15 @@ int fwbw(Run& R, uint32_t pivot, uint32_t* decided) {
     Prof p(c, R.s, K_LABEL, R.n / 8.0);
     k_label_bits<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->bwv, &c->cnt[C_MIN], R.labels);
   }
-  KSPN_TRY(read_counters(c, R.s, C_DECIDED + 1));
-  *decided = c->h_cnt[C_DECIDED];
-  R.live_n -= *decided;
   if (R.st) {
     R.st->fwbw_levels_push += lp;
     R.st->fwbw_levels_pull += lq;
   }
+  *decided = 0;
+  if (!publish) return KSPN_OK;
+  KSPN_TRY(read_counters(c, R.s, C_DECIDED + 1));
+  *decided = c->h_cnt[C_DECIDED];
+  R.live_n -= *decided;
   return KSPN_OK;
 }
 
@@ -2979,6 +3059,28 @@ int residual(Run& R, uint32_t* decided) {
 
 // ---------------------------------------------------------------- tail
 // compact the live subgraph (both directions) and colour it in one workgroup
+// the sub-CSR's adjacency and the solver, for u live vertices whose compact
+// ids, offsets (scan, sub_off) and live list (ulist) are built; publishes
+int tail_finish(Run& R, uint32_t u) {
+  Ctx* c = R.c;
+  const kspn_graph* g = R.g;
+  uint32_t* subid = c->col;
+  {
+    Prof p(c, R.s, K_RESIDUAL, 32.0 * u);
+    k_tail_fill<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, g->bw_off,
+                                                   g->bw_adj, c->live, subid, c->scan, c->sub_off, c->sub_adj,
+                                                   c->cap_sub_m, &c->cnt[C_ERR], &c->cnt[C_ROOM]);
+  }
+  uint32_t cap = R.o.max_rounds ? R.o.max_rounds - R.rounds : 0;  // > 0: checked by the caller
+  {
+    Prof p(c, R.s, K_TAIL, 24.0 * u);
+    k_tail_solve<<<1, 1024, 0, R.s>>>(&c->cnt[C_RES_N], c->ulist, subid, c->scan, c->sub_off, c->sub_adj, c->cap_sub_m,
+                                      c->qa, c->qb, c->qc, R.labels, R.rotation, cap, &c->cnt[C_FLAG], c->live,
+                                      &c->cnt[C_ERR], &c->cnt[C_ROOM]);
+  }
+  return publish(c, R.s, nullptr);
+}
+
 int tail_solve(Run& R, uint32_t* decided) {
   Ctx* c = R.c;
   const kspn_graph* g = R.g;
@@ -2989,7 +3091,7 @@ int tail_solve(Run& R, uint32_t* decided) {
   }
   uint32_t* subid = c->col;
   if (!c->sub_off) KSPN_TRY(dalloc(&c->sub_off, (uint64_t)c->cap_n + 64));
-  k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_RES_N], 0u);
+  k_zero3<<<1, 64, 0, R.s>>>(&c->cnt[C_RES_N], &c->cnt[C_ISO], &c->cnt[C_ROOM]);
   {
     Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words + 4.0 * u);
     k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->live, R.words, c->ulist, &c->cnt[C_RES_N]);
@@ -2997,10 +3099,10 @@ int tail_solve(Run& R, uint32_t* decided) {
   if (u < 4096) {
     // compact ids, degrees, both scans and the solver's flag: two launches
     Prof p(c, R.s, K_RESIDUAL, 24.0 * u);
-    k_tail_prep<<<stream_grid(c, (uint64_t)u + 1), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj,
-                                                                g->bw_off, g->bw_adj, c->live, subid, c->scan,
-                                                                c->sub_off);
-    k_tail_scan<<<1, 1024, 0, R.s>>>(&c->cnt[C_RES_N], c->scan, c->sub_off, &c->cnt[C_FLAG], &c->cnt[C_ERR]);
+    k_tail_prep<<<stream_grid(c, (uint64_t)u + 1), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], 4095u, g->fw_off,
+                                                                g->fw_adj, g->bw_off, g->bw_adj, c->live, subid,
+                                                                c->scan, c->sub_off, &c->cnt[C_ISO]);
+    k_tail_scan<<<1, 1024, 0, R.s>>>(&c->cnt[C_RES_N], 4095u, c->scan, c->sub_off, &c->cnt[C_FLAG], &c->cnt[C_ERR]);
   } else {
     {
       Prof p(c, R.s, K_RESIDUAL, 8.0 * u);
@@ -3028,20 +3130,7 @@ int tail_solve(Run& R, uint32_t* decided) {
     KSPN_TRY(dalloc(&c->sub_adj, 2 * sm + 2));
     c->cap_sub_m = 2 * sm + 2;
   }
-  uint32_t* fa = c->sub_adj;
-  uint32_t* ba = c->sub_adj + sm + 1;
-  {
-    Prof p(c, R.s, K_RESIDUAL, 32.0 * u + 8.0 * sm);
-    k_tail_fill<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, g->bw_off, g->bw_adj,
-                                                   c->live, subid, c->scan, fa, c->sub_off, ba);
-  }
-  uint32_t cap = R.o.max_rounds ? R.o.max_rounds - R.rounds : 0;  // > 0: checked by the caller
-  {
-    Prof p(c, R.s, K_TAIL, 24.0 * u + 8.0 * sm);
-    k_tail_solve<<<1, 1024, 0, R.s>>>(u, c->ulist, subid, c->scan, fa, c->sub_off, ba, c->qa, c->qb, c->qc, R.labels,
-                                      R.rotation, cap, &c->cnt[C_FLAG], c->live);
-  }
-  KSPN_TRY(publish(c, R.s, nullptr));
+  KSPN_TRY(tail_finish(R, u));
   if (c->h_cnt[C_FLAG]) return KSPN_E_ITER_CAP;
   if (R.st) {
     R.st->residual_n = u;
@@ -3050,6 +3139,89 @@ int tail_solve(Run& R, uint32_t* decided) {
   *decided = u;
   R.live_n -= u;
   R.col_valid = false;  // col held sub ids
+  return KSPN_OK;
+}
+
+// trim_1_normal and a tail right after the last FW-BW round, launched
+// before the host knows how many vertices are left: the live lists, the
+// sub-CSR and the solver run with every size read on the device, and one
+// publication reports the FW-BW's decisions (cnt[C_DECIDED]), the trim's
+// (*iso) and the tail's (*decided). When more than `limit` vertices are
+// left after the trim the tail kernels do nothing (*done = false) and the
+// caller goes on with colour rounds; a sub-CSR larger than its buffer is
+// built again once the buffer has grown.
+int tail_spec(Run& R, uint32_t limit, uint32_t* fwbw_decided, uint32_t* decided, uint32_t* iso, bool* done) {
+  Ctx* c = R.c;
+  const kspn_graph* g = R.g;
+  *done = false;
+  *decided = *iso = 0;
+  uint32_t* subid = c->col;
+  if (!c->sub_off) KSPN_TRY(dalloc(&c->sub_off, (uint64_t)c->cap_n + 64));
+  if (c->cap_sub_m < 2) {
+    KSPN_TRY(dalloc(&c->sub_adj, 1u << 16));
+    c->cap_sub_m = 1u << 16;
+  }
+  k_zero3<<<1, 64, 0, R.s>>>(&c->cnt[C_RES_N], &c->cnt[C_ISO], &c->cnt[C_ROOM]);
+  k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_AUX], 0u);
+  {
+    // every live vertex (qa), then trim_1_normal keeps those with a live
+    // neighbour (ulist)
+    Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words);
+    k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->live, R.words, c->qa, &c->cnt[C_AUX]);
+  }
+  {
+    Prof p(c, R.s, K_TRIM_SWEEP, 20.0 * limit);
+    k_tail_split<<<(uint32_t)c->num_cu * 4, WG, 0, R.s>>>(c->qa, &c->cnt[C_AUX], g->fw_off, g->fw_adj, g->bw_off,
+                                                          g->bw_adj, c->live, R.labels, c->ulist, &c->cnt[C_RES_N],
+                                                          &c->cnt[C_ISO]);
+  }
+  {
+    Prof p(c, R.s, K_RESIDUAL, 24.0 * limit);
+    k_tail_prep<<<stream_grid(c, (uint64_t)limit + 1), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], limit, g->fw_off,
+                                                                    g->fw_adj, g->bw_off, g->bw_adj, c->live, subid,
+                                                                    c->scan, c->sub_off, nullptr);
+    k_tail_scan<<<1, 1024, 0, R.s>>>(&c->cnt[C_RES_N], limit, c->scan, c->sub_off, &c->cnt[C_FLAG], &c->cnt[C_ERR]);
+    k_tail_fill<<<stream_grid(c, limit), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, g->bw_off,
+                                                       g->bw_adj, c->live, subid, c->scan, c->sub_off, c->sub_adj,
+                                                       c->cap_sub_m, &c->cnt[C_ERR], &c->cnt[C_ROOM]);
+  }
+  const uint32_t cap = R.o.max_rounds ? R.o.max_rounds - R.rounds : 0;
+  {
+    Prof p(c, R.s, K_TAIL, 24.0 * limit);
+    k_tail_solve<<<1, 1024, 0, R.s>>>(&c->cnt[C_RES_N], c->ulist, subid, c->scan, c->sub_off, c->sub_adj, c->cap_sub_m,
+                                      c->qa, c->qb, c->qc, R.labels, R.rotation, cap, &c->cnt[C_FLAG], c->live,
+                                      &c->cnt[C_ERR], &c->cnt[C_ROOM]);
+  }
+  KSPN_TRY(read_counters(c, R.s, C_COUNT_));
+  *fwbw_decided = c->h_cnt[C_DECIDED];
+  R.live_n -= *fwbw_decided;
+  *iso = c->h_cnt[C_ISO];
+  R.live_n -= *iso;
+  const uint32_t u = c->h_cnt[C_RES_N];
+  if (c->h_cnt[C_ERR]) {
+    // more than `limit` left after trim_1_normal: nothing else was solved
+    k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_ERR], 0u);
+    return KSPN_OK;
+  }
+  if (u != R.live_n) {
+    note_error("tail_spec", "live count differs from the live bitmap", __FILE__, __LINE__);
+    return KSPN_E_INVALID;
+  }
+  if (c->h_cnt[C_ROOM]) {
+    // grow the sub-CSR's buffer, then build and solve it again
+    KSPN_TRY(publish(c, R.s, c->scan + u));
+    const uint64_t sm = c->h_u64[0];
+    KSPN_TRY(dalloc(&c->sub_adj, 2 * sm + 2));
+    c->cap_sub_m = 2 * sm + 2;
+    k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_ROOM], 0u);
+    KSPN_TRY(tail_finish(R, u));
+  }
+  if (c->h_cnt[C_FLAG]) return KSPN_E_ITER_CAP;
+  if (R.st) R.st->residual_n = u;
+  *decided = u;
+  R.live_n -= u;
+  R.col_valid = false;  // col held sub ids
+  *done = true;
   return KSPN_OK;
 }
 
@@ -3116,12 +3288,17 @@ struct Phases {
       if (hipMemGetInfo(&fr, &tot) == hipSuccess) st->phase[cur].memory = (uint64_t)(tot - fr);
     }
   }
+  int find(const char* name) const {
+    for (uint32_t i = 0; i < st->n_phase; ++i)
+      if (std::strcmp(st->phase[i].name, name) == 0) return (int)i;
+    return -1;
+  }
+  // enters scope `name` (nothing happens when it is the open one)
   int begin(const char* name, bool has_memory) {
     if (!st) return KSPN_OK;
+    if (cur >= 0 && std::strcmp(st->phase[cur].name, name) == 0) return KSPN_OK;
     close_scope();
-    int idx = -1;
-    for (uint32_t i = 0; i < st->n_phase; ++i)
-      if (std::strcmp(st->phase[i].name, name) == 0) idx = (int)i;
+    int idx = find(name);
     if (idx < 0 && st->n_phase < KSPN_PHASE_MAX) {
       idx = (int)st->n_phase++;
       kspn_phase& ph = st->phase[idx];
@@ -3135,6 +3312,25 @@ struct Phases {
   }
   void decided(uint64_t d) {
     if (st && cur >= 0) st->phase[cur].decided += d;
+  }
+  // decisions of scope `name`, made while another scope was open
+  void credit(const char* name, uint64_t d) {
+    if (!st) return;
+    const int i = find(name);
+    if (i >= 0) st->phase[i].decided += d;
+  }
+  // forgets the open scope when it was entered once, decided nothing and
+  // is the last one (a stage the variant would not have entered); its time
+  // goes to the scope before it
+  void retract_empty() {
+    if (!st || cur < 0 || cur + 1 != (int)st->n_phase) return;
+    kspn_phase& ph = st->phase[cur];
+    if (ph.entries != 1 || ph.decided != 0) return;
+    const int prev = marks.size() >= 2 ? marks[marks.size() - 2].first : -1;
+    for (auto& m : marks)
+      if (m.first == cur) m.first = prev;
+    --st->n_phase;
+    cur = prev;
   }
   // closes the open scope; `hidden` starts an interval outside every scope
   // (the batch's label localisation), timed into ms_canon
@@ -3267,6 +3463,13 @@ int run_pipeline(Run& R) {
   };
   // ---- FW-BW rounds
   uint32_t rounds = plan.fwbw ? (R.o.fwbw_rounds ? R.o.fwbw_rounds : 1) : 0;
+  const uint32_t tail = (R.o.flags & KSPN_FLAG_NO_TAIL) ? 0u : (R.o.tail_threshold ? R.o.tail_threshold : 4096u);
+  uint32_t* rest_field = st ? (plan.residual_rest ? &st->decided_residual : &st->decided_color) : nullptr;
+  // the single-pivot path may hand what FW-BW leaves straight to the tail
+  // solver (trim_1_normal's decisions are the solver's isolated vertices)
+  const uint32_t fold_limit = std::min<uint32_t>(tail, 4095u);
+  const bool fold_tail = fold_limit > 0 && plan.after != 3 && !R.o.residual_threshold && !R.o.max_rounds &&
+                         !R.pid_preset && !(R.o.flags & KSPN_FLAG_NO_REFINE);
   if (plan.fwbw && (rounds > 1 || R.pid_preset) && R.live_n > 0) {
     // multi-pivot: one pivot per partition per round
     uint32_t d = 0;
@@ -3299,6 +3502,34 @@ int run_pipeline(Run& R) {
     if (pivot >= R.n) break;
     uint32_t d = 0;
     KSPN_TRY(P.begin("forward_backward_search", true));
+    if (k + 1 == rounds && fold_tail) {
+      // the last round: what is left may be small enough for the tail
+      // solver, which then runs before the host has seen the FW-BW result
+      KSPN_TRY(fwbw(R, pivot, &d, false));
+      KSPN_TRY(P.begin(plan.after ? after_name : rest_name, !plan.after));
+      uint32_t u = 0, iso = 0;
+      bool done = false;
+      KSPN_TRY(tail_spec(R, fold_limit, &d, &u, &iso, &done));
+      if (st) st->decided_fwbw += d;
+      P.credit("forward_backward_search", d);
+      // trim_1_normal's decisions (live vertices without a live neighbour)
+      // belong to that scope; variants without it colour them
+      P.decided(iso);
+      if (st) {
+        if (plan.after) st->decided_trim += iso;
+        else if (rest_field) *rest_field += iso;
+      }
+      if ((uint64_t)R.live_n + iso + (done ? u : 0) == 0) {
+        P.retract_empty();  // FW-BW decided everything
+        break;
+      }
+      if (done && u > 0) {
+        if (plan.after) KSPN_TRY(P.begin(rest_name, true));
+        P.decided(u);
+        if (rest_field) *rest_field += u;
+      }
+      break;  // on to the colour rounds when the tail did not run
+    }
     KSPN_TRY(fwbw(R, pivot, &d));
     if (st) st->decided_fwbw += d;
     P.decided(d);
@@ -3312,7 +3543,6 @@ int run_pipeline(Run& R) {
     KSPN_TRY(P.begin("trim_ex", false));
   }
   uint32_t cap = R.o.max_rounds;
-  const uint32_t tail = (R.o.flags & KSPN_FLAG_NO_TAIL) ? 0u : (R.o.tail_threshold ? R.o.tail_threshold : 4096u);
   const uint32_t stop = plan.residual_rest ? 0u : R.o.residual_threshold;
   if (R.live_n > 0 && !(stop && R.live_n <= stop)) KSPN_TRY(P.begin(rest_name, true));
   if (!(R.o.flags & KSPN_FLAG_NO_REFINE) && !R.pid_active && R.live_n > tail) {
@@ -3322,7 +3552,6 @@ int run_pipeline(Run& R) {
     R.pid_active = true;
   }
   if (R.o.flags & KSPN_FLAG_NO_REFINE) R.pid_active = false;
-  uint32_t* rest_field = st ? (plan.residual_rest ? &st->decided_residual : &st->decided_color) : nullptr;
   while (R.live_n > 0) {
     if (stop && R.live_n <= stop) break;
     if (cap && R.rounds >= cap) return KSPN_E_ITER_CAP;
