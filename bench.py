@@ -34,7 +34,8 @@ ROCPROF_NAMES = {"pull_fw": ["kspn::k_pull<0>"], "pull_bw": ["kspn::k_pull<1>"],
                  "label_scc": ["kspn::k_label_bits", "kspn::k_part_apply"], "tail_solve": ["kspn::k_tail_solve"],
                  "trim_sweep": ["kspn::k_trim_sweep<1>"],
                  "reach_coop": ["kspn::k_reach_coop<0>", "kspn::k_reach_coop<1>", "kspn::k_reach_coop<5>"],
-                 "bitmap_ops": ["kspn::k_part_classify"], "pivot_reduce": ["kspn::k_part_key"]}
+                 "bitmap_ops": ["kspn::k_part_classify"], "pivot_reduce": ["kspn::k_part_key"],
+                 "pull_list": ["kspn::k_pull_list<0>", "kspn::k_pull_list<1>"]}
 
 
 def pmc_traffic(config, kernel):

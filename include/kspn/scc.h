@@ -96,7 +96,7 @@ typedef struct kspn_opts {
 #define KSPN_LOG_WORDS 8
 
 /* per-kernel profile line, filled with KSPN_FLAG_PROFILE */
-#define KSPN_KERN_MAX 24
+#define KSPN_KERN_MAX 32
 typedef struct kspn_kern_stat {
   char name[32];
   uint32_t launches;

@@ -151,7 +151,7 @@ class Stats(C.Structure):
         ("bytes_trim", C.c_double), ("bytes_fwbw", C.c_double), ("bytes_color", C.c_double),
         ("bytes_residual", C.c_double), ("bytes_canon", C.c_double),
         ("n_kern", C.c_uint32),
-        ("kern", KernStat * 24),
+        ("kern", KernStat * 32),
         ("n_phase", C.c_uint32),
         ("phase", Phase * PHASE_MAX),
     ]

@@ -226,7 +226,8 @@ enum Counter : int {
   C_SURV1,
   C_SURV2,
   C_ISO,     // live vertices without a live neighbour, counted by the tail preparation
-  C_COUNT_ = 16
+  C_LREC,    // claims of the last pull list level of a batch
+  C_COUNT_ = 24
 };
 
 struct KernTimer {

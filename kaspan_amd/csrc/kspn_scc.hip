@@ -43,8 +43,8 @@ __global__ void k_copy_u32(uint32_t* dst, const uint32_t* src) {
   if (threadIdx.x == 0) *dst = *src;
 }
 
-// counters -> host-mapped pinned memory: 16 u32 counters, then u64 slots
-// (*extra, cnt64[0..2]); the sequence number goes last (map[63]) after a
+// counters -> host-mapped pinned memory: C_COUNT_ u32 counters at 0, u64
+// slots (*extra, cnt64[0..2]) at 48; the sequence number goes last (map[63]) after a
 // system-scope fence, so a host that sees it sees the counters
 __device__ inline void publish_seq(uint32_t* map, uint32_t seq) {
   __threadfence_system();
@@ -85,23 +85,23 @@ __global__ __launch_bounds__(WG) void k_publish(uint32_t* cnt, unsigned long lon
   if (t < C_COUNT_) map[t] = cnt[t];
   if (t == 32) {
     uint64_t x = extra ? *extra : 0;
-    map[16] = (uint32_t)x;
-    map[17] = (uint32_t)(x >> 32);
+    map[48] = (uint32_t)x;
+    map[49] = (uint32_t)(x >> 32);
   }
   if (t == 33) {
     uint64_t y = cnt64[0];
-    map[18] = (uint32_t)y;
-    map[19] = (uint32_t)(y >> 32);
+    map[50] = (uint32_t)y;
+    map[51] = (uint32_t)(y >> 32);
   }
   if (t == 34) {
     uint64_t y = cnt64[1];
-    map[20] = (uint32_t)y;
-    map[21] = (uint32_t)(y >> 32);
+    map[52] = (uint32_t)y;
+    map[53] = (uint32_t)(y >> 32);
   }
   if (t == 35) {
     uint64_t y = cnt64[2];
-    map[22] = (uint32_t)y;
-    map[23] = (uint32_t)(y >> 32);
+    map[54] = (uint32_t)y;
+    map[55] = (uint32_t)(y >> 32);
   }
   publish_seq(map, seq);
   if (reset) {
@@ -142,9 +142,12 @@ __global__ void k_seed(uint32_t* bm, uint64_t words, uint32_t v, uint32_t* queue
 // their popcounts in LDS and reserves its range with one atomicAdd on
 // *cursor (zero before the launch). For lists whose order does not matter.
 __global__ __launch_bounds__(WG) void k_bitmap_to_queue(const uint32_t* __restrict__ bm, uint64_t words,
-                                                        uint32_t* __restrict__ list, uint32_t* cursor) {
+                                                        uint32_t* __restrict__ list, uint32_t* cursor,
+                                                        const unsigned long long* guard = nullptr,
+                                                        unsigned long long guard_max = 0) {
   __shared__ uint32_t s_scan[WG];
   __shared__ uint32_t s_base;
+  if (guard && *guard > guard_max) return;  // the set bits are more than the caller wants listed
   constexpr uint32_t PER = 4;
   for (uint64_t base = (uint64_t)blockIdx.x * WG * PER; base < words; base += (uint64_t)gridDim.x * WG * PER) {
     uint32_t w[PER];
@@ -937,14 +940,17 @@ __global__ __launch_bounds__(WG) void k_pull_list(const uint32_t* __restrict__ l
                                                   const uint32_t* __restrict__ radj, const uint32_t* __restrict__ pid,
                                                   uint32_t* __restrict__ claimed_q, uint32_t* claimed_cnt,
                                                   uint32_t* __restrict__ out, uint32_t* out_cnt, uint32_t* zero,
-                                                  unsigned long long* edges_sum) {
+                                                  uint32_t* zero2, unsigned long long* edges_sum) {
   __shared__ uint64_t s_red[WG / WAVE];
   __shared__ uint32_t s_cq[PULL_QCAP], s_sq[PULL_QCAP];
   __shared__ uint32_t s_cn, s_cb, s_sn, s_sb;
   BlockQueue cq{s_cq, &s_cn, &s_cb, PULL_QCAP}, sq{s_sq, &s_sn, &s_sb, PULL_QCAP};
   bq_init(cq);
   bq_init(sq);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *zero = 0;
+    if (zero2) *zero2 = 0;
+  }
   uint32_t iter = 0;
   uint64_t edges = 0;
   constexpr uint32_t SPAN = (uint32_t)WG * PULL_ILP;
@@ -2144,13 +2150,13 @@ namespace {
 enum KernId : int {
   K_TRIM_FIRST = 0, K_PIVOT, K_BITMAP, K_FRONTIER_DEG, K_SCAN, K_EXPAND_FW, K_EXPAND_BW, K_PULL_FW, K_PULL_BW,
   K_TRIM_SWEEP, K_TRIM2, K_EXPAND_NEIGH, K_COLOR_INIT, K_COLOR_PULL, K_EXPAND_COLOR, K_COLOR_ROOTS,
-  K_EXPAND_COLOR_BW, K_LIVE_LIST, K_CANON, K_RESIDUAL, K_TARJAN, K_LABEL, K_TAIL, K_COOP, K_COUNT
+  K_EXPAND_COLOR_BW, K_LIVE_LIST, K_CANON, K_RESIDUAL, K_TARJAN, K_LABEL, K_TAIL, K_COOP, K_PULL_LIST, K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"trim_first", "pivot_reduce", "bitmap_ops", "frontier_deg", "scan", "expand_fw",
                                    "expand_bw", "pull_fw", "pull_bw", "trim_sweep", "trim2", "expand_neigh",
                                    "color_init", "color_pull", "expand_color", "color_roots", "expand_color_bw",
                                    "live_list", "canonicalize", "residual_build", "tarjan", "label_scc", "tail_solve",
-                                   "reach_coop"};
+                                   "reach_coop", "pull_list"};
 
 struct Prof {
   Ctx* c;
@@ -2298,10 +2304,10 @@ int publish(Ctx* c, hipStream_t s, const uint64_t* extra, const uint64_t* part =
   KSPN_HIP(hipGetLastError());
   KSPN_TRY(wait_published(c, s, seq));
   for (int i = 0; i < C_COUNT_; ++i) c->h_cnt[i] = c->h_map[i];
-  c->h_u64[0] = (uint64_t)c->h_map[16] | ((uint64_t)c->h_map[17] << 32);
-  c->h_u64[1] = (uint64_t)c->h_map[18] | ((uint64_t)c->h_map[19] << 32);
-  c->h_u64[2] = (uint64_t)c->h_map[20] | ((uint64_t)c->h_map[21] << 32);
-  c->h_u64[3] = (uint64_t)c->h_map[22] | ((uint64_t)c->h_map[23] << 32);
+  c->h_u64[0] = (uint64_t)c->h_map[48] | ((uint64_t)c->h_map[49] << 32);
+  c->h_u64[1] = (uint64_t)c->h_map[50] | ((uint64_t)c->h_map[51] << 32);
+  c->h_u64[2] = (uint64_t)c->h_map[52] | ((uint64_t)c->h_map[53] << 32);
+  c->h_u64[3] = (uint64_t)c->h_map[54] | ((uint64_t)c->h_map[55] << 32);
   return KSPN_OK;
 }
 
@@ -2560,45 +2566,76 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       if (fw_levels_push) (*fw_levels_push)++;
     } else {
       int kern = dir == 0 ? K_PULL_FW : K_PULL_BW;
-      uint32_t grid = 0;
-      list_level = list_mode;
-      if (list_level) {
-        // at most `surv_n` candidates in each level of the batch
-        const uint32_t lgrid = stream_grid(c, surv_n / PULL_ILP + 1);
+      bool batch = list_mode;  // a batch of list levels follows (after a sweep: when the remainder is short)
+      bool swept = false;
+      int sweep_prof = -1, list_prof = -1;  // their profile entries, completed after the publication
+      if (!list_mode) {
+        const uint32_t grid = stream_grid(c, R.n);
+        {
+          Prof p(c, R.s, kern, (double)R.n / 4.0 + 16.0 * R.live_n);
+          sweep_prof = p.idx;
+          // no out-degree sum of the claimed vertices: it would read a third
+          // offset array at scattered rows, and the first push level after the
+          // pull levels sums its frontier's degrees itself
+          if (dir == 0)
+            k_pull<0><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
+                                            &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0]);
+          else
+            k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
+                                            &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0]);
+        }
+        if (fw_levels_pull) (*fw_levels_pull)++;
+        // claims -> cnt[nc], edges -> cnt64[0], candidates left -> cnt64[1];
+        // the list of those is built only when it is short (list_max), so
+        // the batch below does nothing otherwise
+        k_reduce3<<<1, WG, 0, R.s>>>(c->part, grid, &c->cnt[nc], c->cnt64);
+        {
+          Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words);
+          k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->sbm, R.words, lists[0],
+                                                                             &c->cnt[lcnt[0]], c->cnt64 + 1,
+                                                                             list_max);
+        }
+        L = 0;
+        batch = true;
+        swept = true;
+        surv_n = list_max;  // at most
+      }
+      const uint32_t list_in = surv_n;
+      if (batch) {
+        const uint32_t lgrid = std::min<uint32_t>(stream_grid(c, surv_n / PULL_ILP + 1), (uint32_t)c->num_cu * 4);
         for (uint32_t j = 0; j < LIST_BATCH; ++j, ++L) {
+          // the first level zeroes the claim count the last one records
           uint32_t* rec = j + 1 == LIST_BATCH ? N : nullptr;
-          Prof p(c, R.s, kern, 24.0 * surv_n);
+          uint32_t* zero2 = j == 0 ? &c->cnt[C_LREC] : nullptr;
+          Prof p(c, R.s, K_PULL_LIST, 0.0);
+          list_prof = p.idx;
           if (dir == 0)
             k_pull_list<0><<<lgrid, WG, 0, R.s>>>(lists[L % 2], &c->cnt[lcnt[L % 3]], vis, roff, radj, pid, rec,
-                                                  &c->cnt[nc], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 3]],
-                                                  &c->cnt[lcnt[(L + 2) % 3]], c->cnt64);
+                                                  &c->cnt[C_LREC], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 3]],
+                                                  &c->cnt[lcnt[(L + 2) % 3]], zero2, c->cnt64 + 2);
           else
             k_pull_list<1><<<lgrid, WG, 0, R.s>>>(lists[L % 2], &c->cnt[lcnt[L % 3]], vis, roff, radj, pid, rec,
-                                                  &c->cnt[nc], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 3]],
-                                                  &c->cnt[lcnt[(L + 2) % 3]], c->cnt64);
-          if (fw_levels_pull) (*fw_levels_pull)++;
+                                                  &c->cnt[C_LREC], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 3]],
+                                                  &c->cnt[lcnt[(L + 2) % 3]], zero2, c->cnt64 + 2);
         }
-        // cnt[nc] counts the last level's claims; cnt64[0] the edges scanned
-        KSPN_TRY(publish(c, R.s, nullptr, nullptr, 0, nullptr, &c->cnt[fc]));
-        surv_n = c->h_cnt[lcnt[L % 3]];
-      } else {
-        grid = stream_grid(c, R.n);
-        Prof p(c, R.s, kern, (double)R.n / 4.0 + 16.0 * R.live_n);
-        // no out-degree sum of the claimed vertices: it would read a third
-        // offset array at scattered rows, and the first push level after the
-        // pull levels sums its frontier's degrees itself
-        if (dir == 0)
-          k_pull<0><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
-                                          &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0]);
-        else
-          k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
-                                          &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0]);
       }
-      if (!list_level) {
-        if (fw_levels_pull) (*fw_levels_pull)++;
-        // the workgroup partials are reduced by the publication
-        KSPN_TRY(publish(c, R.s, nullptr, c->part, grid, &c->cnt[nc], &c->cnt[fc]));
-        surv_n = (uint32_t)c->h_u64[2];  // the sweep's third partial: candidates left
+      KSPN_TRY(publish(c, R.s, nullptr, nullptr, 0, nullptr, &c->cnt[fc]));
+      // a sweep's list batch ran iff its remainder was short
+      const uint64_t left = c->h_u64[2];
+      if (swept && left > list_max) batch = false;
+      list_level = batch;
+      if (c->profile) {
+        // algorithmic bytes: per scanned edge an id and a bitmap bit; per list entry its id, offsets, out entry
+        if (sweep_prof >= 0) c->pending[sweep_prof].bytes += 4.125 * (double)c->h_u64[1];
+        if (list_prof >= 0)
+          c->pending[list_prof].bytes += 4.125 * (double)c->h_u64[3] + (batch ? 24.0 * (swept ? left : list_in) : 0.0);
+      }
+      if (batch) {
+        if (fw_levels_pull) *fw_levels_pull += LIST_BATCH;
+        surv_n = c->h_cnt[lcnt[L % 3]];
+        list_mode = true;
+      } else {
+        surv_n = (uint32_t)left;
       }
     }
     if (!pull) KSPN_TRY(publish(c, R.s, c->scan + f, nullptr, 0, nullptr, &c->cnt[fc]));
@@ -2608,11 +2645,10 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       if (R.st) R.st->edges_scanned += edges;
       if (c->profile && !c->pending.empty()) c->pending.back().bytes = 20.0 * f + 4.125 * (double)edges;
     } else {
-      uint64_t scanned = c->h_u64[1];
-      if (R.st) R.st->edges_scanned += scanned;
-      if (c->profile && !c->pending.empty()) c->pending.back().bytes += 4.125 * (double)scanned;
+      if (R.st) R.st->edges_scanned += c->h_u64[1] + c->h_u64[3];  // the sweep's and the list levels'
     }
-    uint32_t nf = c->h_cnt[nc];
+    // a pull ending in list levels: what the last one claimed (in N)
+    uint32_t nf = pull && list_level ? c->h_cnt[C_LREC] : c->h_cnt[nc];
     edges = c->h_u64[2];
     maxd = pull ? 0xFFFFFFFFu : (uint32_t)std::min<uint64_t>(c->h_u64[3], 0xFFFFFFFFull);
     // after a pull level the frontier's out-edges are unknown until a push
@@ -2620,22 +2656,17 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
     known = !pull;
     if (pull && nf > 0 && (double)nf * beta < (double)R.live_n) {
       // back to push: the frontier is what the last pull level claimed. A
-      // list level appended it to N already; a sweep left it in the bitmap
-      // fnew (its size nf is already in cnt[nc]; the pull zeroed the cursor)
+      // list level appended it to N (its count goes to cnt[nc]); a sweep left
+      // it in the bitmap fnew (its size nf is in cnt[nc]; the sweep zeroed
+      // the cursor)
       pull = false;
       list_mode = false;
-      if (!list_level) {
+      if (list_level) {
+        k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[nc], &c->cnt[C_LREC]);
+      } else {
         Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words + 4.0 * nf);
         k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->fnew, R.words, N, &c->cnt[C_CURSOR]);
       }
-    } else if (pull && nf > 0 && !list_level && surv_n <= list_max) {
-      // few candidates left: the next pull levels visit a list of them (the
-      // sweep zeroed the three list counters)
-      Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words + 4.0 * surv_n);
-      k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->sbm, R.words, lists[0],
-                                                                         &c->cnt[lcnt[0]]);
-      L = 0;
-      list_mode = true;
     }
     std::swap(F, N);
     std::swap(fc, nc);
