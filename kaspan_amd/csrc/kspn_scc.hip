@@ -129,12 +129,17 @@ __global__ void k_zero2(uint32_t* a, uint32_t* b) {
 
 // FW-BW seed: bm = {v} (the word holding v gets its bit, all others 0),
 // queue = [v], *count = 1; one launch for the four steps
-__global__ void k_seed(uint32_t* bm, uint64_t words, uint32_t v, uint32_t* queue, uint32_t* count) {
+// bitmap = {v}, queue = [v]; with vdev, v is read there and 0xFFFFFFFF
+// (no pivot) seeds nothing
+__global__ void k_seed(uint32_t* bm, uint64_t words, uint32_t v, const uint32_t* vdev, uint32_t* queue,
+                       uint32_t* count) {
+  if (vdev) v = *vdev;
+  const bool none = v == 0xFFFFFFFFu;
   for (uint64_t w = blockIdx.x * (uint64_t)WG + threadIdx.x; w < words; w += (uint64_t)gridDim.x * WG)
-    bm[w] = (w == (v >> 5)) ? (1u << (v & 31)) : 0u;
+    bm[w] = (!none && w == (v >> 5)) ? (1u << (v & 31)) : 0u;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     queue[0] = v;
-    *count = 1;
+    *count = none ? 0u : 1u;
   }
 }
 
@@ -2376,6 +2381,11 @@ struct Run {
   uint32_t coop_deg = 0;
   uint32_t coop_wg = 0;               // its workgroups
   bool pivot_live_deg = false;        // pivots by live degree product (after an exhaustive trim)
+  // the first trim's decided count and pivot are still on the device
+  // (cnt[C_DECIDED], cnt[C_FLAG]); the first publication of the forward
+  // reach brings them to the host
+  bool live_pending = false;
+  uint32_t trim_decided = 0, first_pivot = 0;
 };
 
 int select_pivot_live(Run& R, uint32_t* pivot);
@@ -2647,6 +2657,12 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
     } else {
       if (R.st) R.st->edges_scanned += c->h_u64[1] + c->h_u64[3];  // the sweep's and the list levels'
     }
+    if (R.live_pending) {
+      R.trim_decided = c->h_cnt[C_DECIDED];
+      R.first_pivot = c->h_cnt[C_FLAG];
+      R.live_n = R.n - R.trim_decided;
+      R.live_pending = false;
+    }
     // a pull ending in list levels: what the last one claimed (in N)
     uint32_t nf = pull && list_level ? c->h_cnt[C_LREC] : c->h_cnt[nc];
     edges = c->h_u64[2];
@@ -2681,13 +2697,16 @@ int fwbw(Run& R, uint32_t pivot, uint32_t* decided, bool publish = true) {
   Ctx* c = R.c;
   {
     Prof p(c, R.s, K_BITMAP, 4.0 * R.words);
-    k_seed<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->fwv, R.words, pivot, c->qa, &c->cnt[C_Q0]);
+    k_seed<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->fwv, R.words, pivot, R.live_pending ? &c->cnt[C_FLAG] : nullptr,
+                                                    c->qa, &c->cnt[C_Q0]);
   }
+  // the backward seed reads the pivot on the device too when the forward one did
+  const uint32_t* seed_dev = R.live_pending ? &c->cnt[C_FLAG] : nullptr;
   uint32_t lp = 0, lq = 0;
   KSPN_TRY(reach(R, 0, 1, nullptr, &lp, &lq));
   {
     Prof p(c, R.s, K_BITMAP, 4.0 * R.words);
-    k_seed<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwv, R.words, pivot, c->qa, &c->cnt[C_Q0]);
+    k_seed<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwv, R.words, pivot, seed_dev, c->qa, &c->cnt[C_Q0]);
   }
   KSPN_TRY(reach(R, 1, 1, nullptr, &lp, &lq));
   // the pivot's SCC: clear it from live, then label it with its smallest member
@@ -3357,7 +3376,12 @@ struct Phases {
     if (!st || cur < 0 || cur + 1 != (int)st->n_phase) return;
     kspn_phase& ph = st->phase[cur];
     if (ph.entries != 1 || ph.decided != 0) return;
-    const int prev = marks.size() >= 2 ? marks[marks.size() - 2].first : -1;
+    int prev = -1;  // the scope open before it
+    for (size_t i = marks.size(); i-- > 0;)
+      if (marks[i].first != cur) {
+        prev = marks[i].first;
+        break;
+      }
     for (auto& m : marks)
       if (m.first == cur) m.first = prev;
     --st->n_phase;
@@ -3447,14 +3471,26 @@ int run_pipeline(Run& R) {
     Prof p(c, R.s, K_PIVOT, 24.0 * nparts);
     k_pivot_reduce<<<1, WG, 0, R.s>>>(c->part, nparts, &c->cnt[C_FLAG], &c->cnt[C_DECIDED]);
   }
-  KSPN_TRY(read_counters(c, R.s, C_FLAG + 1));
-  uint32_t decided = c->h_cnt[C_DECIDED];
-  uint32_t pivot = c->h_cnt[C_FLAG];
-  R.live_n = R.n - decided;
-  P.decided(decided);
-  if (st) {
-    st->decided_trim = decided;
-    st->pivot = pivot;
+  // The plain single-pivot path seeds FW-BW with the pivot on the device and
+  // learns the trim's decisions from the first publication of the forward
+  // reach, one host round trip fewer
+  const uint32_t rounds_cfg = plan.fwbw ? (R.o.fwbw_rounds ? R.o.fwbw_rounds : 1) : 0;
+  const bool defer = rounds_cfg == 1 && !plan.ex_first && !R.pid_preset && R.o.pivot == KSPN_PIVOT_PLAIN &&
+                     !R.o.residual_threshold && R.n > 0;
+  uint32_t decided = 0, pivot = 0;
+  if (defer) {
+    R.live_pending = true;
+    R.live_n = R.n;  // an upper bound until then
+  } else {
+    KSPN_TRY(read_counters(c, R.s, C_FLAG + 1));
+    decided = c->h_cnt[C_DECIDED];
+    pivot = c->h_cnt[C_FLAG];
+    R.live_n = R.n - decided;
+    P.decided(decided);
+    if (st) {
+      st->decided_trim = decided;
+      st->pivot = pivot;
+    }
   }
   if (plan.ex_first && R.live_n > 0) {
     uint32_t d = 0;
@@ -3530,13 +3566,22 @@ int run_pipeline(Run& R) {
       KSPN_TRY(select_pivot_live(R, &pivot));
       if (k == 0 && st) st->pivot = pivot;
     }
-    if (pivot >= R.n) break;
+    if (pivot >= R.n && !R.live_pending) break;
     uint32_t d = 0;
     KSPN_TRY(P.begin("forward_backward_search", true));
     if (k + 1 == rounds && fold_tail) {
       // the last round: what is left may be small enough for the tail
       // solver, which then runs before the host has seen the FW-BW result
+      const bool deferred = R.live_pending;
       KSPN_TRY(fwbw(R, pivot, &d, false));
+      if (deferred) {
+        // the trim's counts arrived with the forward reach
+        P.credit(first_name, R.trim_decided);
+        if (st) {
+          st->decided_trim = R.trim_decided;
+          st->pivot = R.first_pivot == 0xFFFFFFFFu ? R.n : R.first_pivot;
+        }
+      }
       KSPN_TRY(P.begin(plan.after ? after_name : rest_name, !plan.after));
       uint32_t u = 0, iso = 0;
       bool done = false;
@@ -3552,6 +3597,7 @@ int run_pipeline(Run& R) {
       }
       if ((uint64_t)R.live_n + iso + (done ? u : 0) == 0) {
         P.retract_empty();  // FW-BW decided everything
+        if (R.trim_decided == R.n && deferred) P.retract_empty();  // so did the trim: no FW-BW scope either
         break;
       }
       if (done && u > 0) {
@@ -3561,7 +3607,16 @@ int run_pipeline(Run& R) {
       }
       break;  // on to the colour rounds when the tail did not run
     }
+    const bool deferred = R.live_pending;
     KSPN_TRY(fwbw(R, pivot, &d));
+    if (deferred) {
+      P.credit(first_name, R.trim_decided);
+      if (st) {
+        st->decided_trim = R.trim_decided;
+        st->pivot = R.first_pivot == 0xFFFFFFFFu ? R.n : R.first_pivot;
+      }
+      if (R.trim_decided == R.n) P.retract_empty();  // the trim decided everything: no FW-BW scope
+    }
     if (st) st->decided_fwbw += d;
     P.decided(d);
     if (R.live_n == 0) break;

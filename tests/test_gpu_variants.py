@@ -156,3 +156,35 @@ def test_statistics_tree(ka, O):
     assert sum(v["decided_count"] for v in stages.values()) == g.n
     tar = O.tarjan(host_graph(O, g))
     assert b["global_component_count"] == b["local_component_count"] == int(np.sum(tar == np.arange(g.n)))
+
+
+@pytest.mark.parametrize("variant", [None, "hpc_like", "ispan_like", "trim_ex"])
+def test_scopes_when_an_early_stage_decides_everything(ka, O, variant):
+    """the reference returns after the stage that decides the last vertex
+    (scc.hpp:44,60,70): a bipartite DAG ends after the first trim, a cycle
+    after FW-BW"""
+    import numpy as np
+
+    n = 1000
+    # sources -> sinks: every vertex has in- or out-degree 0
+    rng = np.random.default_rng(3)
+    src = rng.integers(0, n // 2, 4000)
+    dst = rng.integers(n // 2, n, 4000)
+    g = ka.BidiGraph.from_edges(n, src, dst)
+    lab, st = ka.scc_variant(variant, g, flags=NO_TRIM2)
+    assert lab.cpu().tolist() == list(range(n))
+    first = "trim_ex_first" if variant == "trim_ex" else "trim_1_first"
+    assert [p["name"] for p in st.phases()] == [first]
+    assert st.phases()[0]["decided"] == n
+    # one directed cycle with chords: FW-BW decides everything
+    src = np.concatenate([np.arange(n), np.arange(0, n, 3)])
+    dst = np.concatenate([(np.arange(n) + 1) % n, (np.arange(0, n, 3) + 5) % n])
+    g = ka.BidiGraph.from_edges(n, src, dst)
+    lab, st = ka.scc_variant(variant, g, flags=NO_TRIM2)
+    assert lab.cpu().tolist() == [0] * n
+    names = [p["name"] for p in st.phases()]
+    expect = [first, "select_pivot", "forward_backward_search"] if variant == "trim_ex" else \
+        [first, "forward_backward_search"]
+    assert names == expect, names
+    assert st.phases()[-1]["decided"] == n
+    assert st.pivot < n
