@@ -1613,40 +1613,50 @@ __global__ __launch_bounds__(1024) void k_tail_solve(const uint32_t* sn_ptr, con
 // tail subgraph, one pass: compact ids, live out- and in-degrees (entry u = 0)
 // Nothing is done when there are more than `limit` live vertices (the
 // tail scan reports that case). *iso counts live vertices without a live
-// neighbour in either direction: trim_1_normal's decisions.
-__global__ void k_tail_prep(const uint32_t* __restrict__ U, const uint32_t* ucount, uint32_t limit,
-                            const uint64_t* __restrict__ fw_off, const uint32_t* __restrict__ fw_adj,
-                            const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
-                            const uint32_t* __restrict__ live, uint32_t* __restrict__ subid,
-                            uint64_t* __restrict__ degf, uint64_t* __restrict__ degb, uint32_t* iso) {
+// neighbour in either direction: trim_1_normal's decisions. One wavefront
+// per vertex: tail vertices can have long rows.
+__global__ __launch_bounds__(WG) void k_tail_prep(const uint32_t* __restrict__ U, const uint32_t* ucount,
+                                                  uint32_t limit, const uint64_t* __restrict__ fw_off,
+                                                  const uint32_t* __restrict__ fw_adj,
+                                                  const uint64_t* __restrict__ bw_off,
+                                                  const uint32_t* __restrict__ bw_adj,
+                                                  const uint32_t* __restrict__ live, uint32_t* __restrict__ subid,
+                                                  uint64_t* __restrict__ degf, uint64_t* __restrict__ degb,
+                                                  uint32_t* iso) {
+  __shared__ uint32_t s_iso;
   const uint32_t u = *ucount;
   if (u > limit) return;
-  const uint32_t u_pad = (u + 1 + WG - 1) / WG * WG;
-  for (uint32_t base = blockIdx.x * WG; base < u_pad; base += gridDim.x * WG) {
-    const uint32_t i = base + threadIdx.x;
-    bool alone = false;
-    if (i == u) {
-      degf[i] = 0;
-      degb[i] = 0;
-    } else if (i < u) {
-      const uint32_t v = U[i];
+  const uint32_t lane = lane_id();
+  const uint32_t wpb = WG / WAVE;
+  if (threadIdx.x == 0) s_iso = 0;
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    degf[u] = 0;
+    degb[u] = 0;
+  }
+  for (uint32_t i = blockIdx.x * wpb + threadIdx.x / WAVE; i < u; i += gridDim.x * wpb) {
+    const uint32_t v = U[i];
+    uint32_t df = 0, db = 0;
+    for (uint64_t it = fw_off[v] + lane, end = fw_off[v + 1]; it < end; it += WAVE) df += bm_get(live, fw_adj[it]);
+    for (uint64_t it = bw_off[v] + lane, end = bw_off[v + 1]; it < end; it += WAVE) db += bm_get(live, bw_adj[it]);
+    df = wave_sum(df);
+    db = wave_sum(db);
+    if (lane == 0) {
       subid[v] = i;
-      uint64_t df = 0, db = 0;
-      for (uint64_t it = fw_off[v], end = fw_off[v + 1]; it < end; ++it) df += bm_get(live, fw_adj[it]);
-      for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it) db += bm_get(live, bw_adj[it]);
       degf[i] = df;
       degb[i] = db;
-      alone = df == 0 && db == 0;
+      if (df == 0 && db == 0) atomicAdd(&s_iso, 1u);
     }
-    if (iso) wave_count(alone, iso);
   }
+  __syncthreads();
+  if (iso && threadIdx.x == 0 && s_iso) atomicAdd(iso, s_iso);  // one global add per workgroup
 }
 
 // trim_1_normal over a list of live vertices, for the tail: a vertex with
 // no live neighbour in either direction is decided (own id, live bit
 // cleared); the others go to `rest`, staged per workgroup in LDS. Removing
 // such a vertex changes no other vertex's test: it is nobody's live
-// neighbour.
+// neighbour. One wavefront per vertex, stopping at the first live neighbour.
 __global__ __launch_bounds__(WG) void k_tail_split(const uint32_t* __restrict__ L, const uint32_t* lcount,
                                                    const uint64_t* __restrict__ fw_off,
                                                    const uint32_t* __restrict__ fw_adj,
@@ -1655,32 +1665,38 @@ __global__ __launch_bounds__(WG) void k_tail_split(const uint32_t* __restrict__ 
                                                    uint32_t* __restrict__ labels, uint32_t* __restrict__ rest,
                                                    uint32_t* rest_cnt, uint32_t* iso_cnt) {
   __shared__ uint32_t s_q[2 * WG];
-  __shared__ uint32_t s_n, s_b;
+  __shared__ uint32_t s_n, s_b, s_iso;
   BlockQueue q{s_q, &s_n, &s_b, 2 * WG};
+  if (threadIdx.x == 0) s_iso = 0;
   bq_init(q);
+  const uint32_t lane = lane_id();
+  const uint32_t wpb = WG / WAVE;
   const uint32_t cnt = *lcount;
-  const uint32_t cnt_pad = (cnt + WG - 1) / WG * WG;
-  for (uint32_t base = blockIdx.x * WG; base < cnt_pad; base += gridDim.x * WG) {
-    const uint32_t i = base + threadIdx.x;
-    bool alone = false, keep = false;
+  const uint32_t cnt_pad = (cnt + wpb - 1) / wpb * wpb;
+  for (uint32_t base = blockIdx.x * wpb; base < cnt_pad; base += gridDim.x * wpb) {
+    const uint32_t i = base + threadIdx.x / WAVE;
+    bool keep = false, alone = false;
     uint32_t v = 0;
     if (i < cnt) {
       v = L[i];
       bool any = false;
-      for (uint64_t it = fw_off[v], end = fw_off[v + 1]; it < end && !any; ++it) any = bm_get(live, fw_adj[it]);
-      for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end && !any; ++it) any = bm_get(live, bw_adj[it]);
-      alone = !any;
-      keep = any;
+      for (uint64_t b = fw_off[v], end = fw_off[v + 1]; b < end && !any; b += WAVE)
+        any = __any(b + lane < end && bm_get(live, fw_adj[b + lane]));
+      for (uint64_t b = bw_off[v], end = bw_off[v + 1]; b < end && !any; b += WAVE)
+        any = __any(b + lane < end && bm_get(live, bw_adj[b + lane]));
+      alone = lane == 0 && !any;
+      keep = lane == 0 && any;
     }
     if (alone) {
       labels[v] = v;
       atomicAnd(&live[v >> 5], ~(1u << (v & 31)));
+      atomicAdd(&s_iso, 1u);  // one global add per workgroup, below
     }
-    wave_count(alone, iso_cnt);
     bq_push(q, keep, v);
-    bq_maybe_flush(q, rest, rest_cnt, WG);  // at most WG appends per iteration
+    bq_maybe_flush(q, rest, rest_cnt, wpb);  // at most one append per wavefront per iteration
   }
   bq_flush(q, rest, rest_cnt);
+  if (threadIdx.x == 0 && s_iso) atomicAdd(iso_cnt, s_iso);
 }
 
 // in-place exclusive scans of two arrays of u+1 <= 4096 entries in one
@@ -1733,12 +1749,15 @@ __global__ __launch_bounds__(1024) void k_tail_scan(const uint32_t* ucount, uint
 // The sub-CSR's two adjacency arrays share `sub` (cap entries): forward
 // at 0, backward after the forward total fo[u] and one spare entry. Nothing
 // is done when *skip is set; when they do not fit, *room is set instead.
-__global__ void k_tail_fill(const uint32_t* __restrict__ U, const uint32_t* ucount, const uint64_t* __restrict__ fw_off,
-                            const uint32_t* __restrict__ fw_adj, const uint64_t* __restrict__ bw_off,
-                            const uint32_t* __restrict__ bw_adj, const uint32_t* __restrict__ live,
-                            const uint32_t* __restrict__ subid, const uint64_t* __restrict__ fo,
-                            const uint64_t* __restrict__ bo, uint32_t* __restrict__ sub, uint64_t cap,
-                            const uint32_t* skip, uint32_t* room) {
+__global__ __launch_bounds__(WG) void k_tail_fill(const uint32_t* __restrict__ U, const uint32_t* ucount,
+                                                  const uint64_t* __restrict__ fw_off,
+                                                  const uint32_t* __restrict__ fw_adj,
+                                                  const uint64_t* __restrict__ bw_off,
+                                                  const uint32_t* __restrict__ bw_adj,
+                                                  const uint32_t* __restrict__ live,
+                                                  const uint32_t* __restrict__ subid, const uint64_t* __restrict__ fo,
+                                                  const uint64_t* __restrict__ bo, uint32_t* __restrict__ sub,
+                                                  uint64_t cap, const uint32_t* skip, uint32_t* room) {
   const uint32_t u = *ucount;
   if (*skip) return;
   const uint64_t sm = fo[u];
@@ -1748,17 +1767,29 @@ __global__ void k_tail_fill(const uint32_t* __restrict__ U, const uint32_t* ucou
   }
   uint32_t* __restrict__ fa = sub;
   uint32_t* __restrict__ ba = sub + sm + 1;
-  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < u; i += gridDim.x * WG) {
+  const uint32_t lane = lane_id();
+  const uint32_t wpb = WG / WAVE;
+  const uint64_t below = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+  // one wavefront per vertex; live neighbours in row order
+  for (uint32_t i = blockIdx.x * wpb + threadIdx.x / WAVE; i < u; i += gridDim.x * wpb) {
     const uint32_t v = U[i];
     uint64_t p = fo[i];
-    for (uint64_t it = fw_off[v], end = fw_off[v + 1]; it < end; ++it) {
-      const uint32_t w = fw_adj[it];
-      if (bm_get(live, w)) fa[p++] = subid[w];
+    for (uint64_t b = fw_off[v], end = fw_off[v + 1]; b < end; b += WAVE) {
+      const uint64_t it = b + lane;
+      const uint32_t w = it < end ? fw_adj[it] : 0u;
+      const bool keep = it < end && bm_get(live, w);
+      const uint64_t m = __ballot(keep);
+      if (keep) fa[p + __popcll(m & below)] = subid[w];
+      p += __popcll(m);
     }
     p = bo[i];
-    for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it) {
-      const uint32_t w = bw_adj[it];
-      if (bm_get(live, w)) ba[p++] = subid[w];
+    for (uint64_t b = bw_off[v], end = bw_off[v + 1]; b < end; b += WAVE) {
+      const uint64_t it = b + lane;
+      const uint32_t w = it < end ? bw_adj[it] : 0u;
+      const bool keep = it < end && bm_get(live, w);
+      const uint64_t m = __ballot(keep);
+      if (keep) ba[p + __popcll(m & below)] = subid[w];
+      p += __popcll(m);
     }
   }
 }
@@ -3117,7 +3148,7 @@ int tail_finish(Run& R, uint32_t u) {
   uint32_t* subid = c->col;
   {
     Prof p(c, R.s, K_RESIDUAL, 32.0 * u);
-    k_tail_fill<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, g->bw_off,
+    k_tail_fill<<<stream_grid(c, (uint64_t)u * WAVE), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, g->bw_off,
                                                    g->bw_adj, c->live, subid, c->scan, c->sub_off, c->sub_adj,
                                                    c->cap_sub_m, &c->cnt[C_ERR], &c->cnt[C_ROOM]);
   }
@@ -3149,7 +3180,7 @@ int tail_solve(Run& R, uint32_t* decided) {
   if (u < 4096) {
     // compact ids, degrees, both scans and the solver's flag: two launches
     Prof p(c, R.s, K_RESIDUAL, 24.0 * u);
-    k_tail_prep<<<stream_grid(c, (uint64_t)u + 1), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], 4095u, g->fw_off,
+    k_tail_prep<<<stream_grid(c, (uint64_t)u * WAVE), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], 4095u, g->fw_off,
                                                                 g->fw_adj, g->bw_off, g->bw_adj, c->live, subid,
                                                                 c->scan, c->sub_off, &c->cnt[C_ISO]);
     k_tail_scan<<<1, 1024, 0, R.s>>>(&c->cnt[C_RES_N], 4095u, c->scan, c->sub_off, &c->cnt[C_FLAG], &c->cnt[C_ERR]);
@@ -3227,11 +3258,11 @@ int tail_spec(Run& R, uint32_t limit, uint32_t* fwbw_decided, uint32_t* decided,
   }
   {
     Prof p(c, R.s, K_RESIDUAL, 24.0 * limit);
-    k_tail_prep<<<stream_grid(c, (uint64_t)limit + 1), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], limit, g->fw_off,
+    k_tail_prep<<<stream_grid(c, (uint64_t)limit * WAVE), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], limit, g->fw_off,
                                                                     g->fw_adj, g->bw_off, g->bw_adj, c->live, subid,
                                                                     c->scan, c->sub_off, nullptr);
     k_tail_scan<<<1, 1024, 0, R.s>>>(&c->cnt[C_RES_N], limit, c->scan, c->sub_off, &c->cnt[C_FLAG], &c->cnt[C_ERR]);
-    k_tail_fill<<<stream_grid(c, limit), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, g->bw_off,
+    k_tail_fill<<<stream_grid(c, (uint64_t)limit * WAVE), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, g->bw_off,
                                                        g->bw_adj, c->live, subid, c->scan, c->sub_off, c->sub_adj,
                                                        c->cap_sub_m, &c->cnt[C_ERR], &c->cnt[C_ROOM]);
   }
