@@ -311,6 +311,8 @@ struct CoopCtl {
   unsigned long long verts;     // frontier vertices expanded by all levels of the launch
   uint32_t bar_count, bar_pad;  // grid barrier: arrivals so far in the launch
   uint32_t levels, reason, err, pad;
+  unsigned long long lexp[4];   // edges expanded by the wavefronts' local rounds of a level
+  unsigned long long lvert[4];  // vertices expanded by them
 };
 
 // host helpers implemented in kspn_scc.hip

@@ -424,6 +424,7 @@ constexpr int COOP_WG = 1024;
 constexpr int COOP_WAVES = COOP_WG / WAVE;
 constexpr uint32_t COOP_QCAP = 256;           // staged entries per wavefront
 constexpr uint32_t COOP_SPIN_LIMIT = 1u << 21;  // barrier polls before giving up (about a second)
+constexpr uint32_t COOP_LOCAL = 8;              // rounds a wavefront expands its own claims within a level
 enum CoopReason : uint32_t { COOP_EMPTY = 1, COOP_PULL = 2, COOP_BIG = 3, COOP_ERR = 4 };
 
 // relaxed device-scope loads: coherent across XCDs without a cache
@@ -439,14 +440,17 @@ __device__ inline unsigned long long ld_relaxed64(const unsigned long long* p) {
 // flag, read by thread 0 and shared with the workgroup.
 struct CoopLevel {
   uint32_t f, fmax, err;
-  unsigned long long fe;
+  unsigned long long fe, lexp, lvert;
 };
 
+// slot of the level just finished (lexp, lvert) is the one before `slot`
 __device__ inline void coop_read_level(const CoopCtl* ctl, uint32_t slot, CoopLevel* lv) {
   lv->f = ld_relaxed(&ctl->cnt[slot]);
   lv->fe = ld_relaxed64(&ctl->nsum[slot]);
   lv->fmax = ld_relaxed(&ctl->maxd[slot]);
   lv->err = ld_relaxed(&ctl->err);
+  lv->lexp = ld_relaxed64(&ctl->lexp[slot]);
+  lv->lvert = ld_relaxed64(&ctl->lvert[slot]);
 }
 
 // Barrier over the workgroups of a persistent launch (resident together on
@@ -518,6 +522,7 @@ __global__ __launch_bounds__(COOP_WG) void k_reach_coop(uint32_t n, uint32_t* F0
   __shared__ uint32_t s_q[COOP_WAVES][COOP_QCAP];
   __shared__ uint32_t s_qn[COOP_WAVES];
   __shared__ unsigned long long s_sum[COOP_WAVES];
+  __shared__ unsigned long long s_lexp[COOP_WAVES], s_lvert[COOP_WAVES];
   __shared__ uint32_t s_max[COOP_WAVES];
   __shared__ uint32_t s_base;
   __shared__ CoopLevel s_lv;
@@ -533,6 +538,10 @@ __global__ __launch_bounds__(COOP_WG) void k_reach_coop(uint32_t n, uint32_t* F0
     const uint32_t s0 = L % 3, s1 = (L + 1) % 3, s2 = (L + 2) % 3;
     const uint32_t f = s_lv.f, fmax = s_lv.fmax;
     const unsigned long long fe = s_lv.fe;
+    // the local rounds of the level before (slot s0 holds what they added)
+    expanded += s_lv.lexp;
+    verts += s_lv.lvert;
+    m_rest -= (double)s_lv.lexp;
     if (s_lv.err) return;  // a workgroup timed out at a barrier
     uint32_t reason = 0;   // identical in every workgroup: read after the same barrier
     if (f == 0) reason = COOP_EMPTY;
@@ -552,25 +561,17 @@ __global__ __launch_bounds__(COOP_WG) void k_reach_coop(uint32_t n, uint32_t* F0
       __hip_atomic_store(&ctl->work[s2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&ctl->nsum[s2], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&ctl->maxd[s2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl->lexp[s2], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl->lvert[s2], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     uint32_t* F = (L & 1) ? F1 : F0;
     uint32_t* N = (L & 1) ? F0 : F1;
     uint64_t my_sum = 0;
     uint32_t my_max = 0, qn = 0;
-    // chunk gw first (no atomic); beyond the first pass chunks come from a counter
-    const uint64_t nchunks = ((uint64_t)f + WAVE - 1) / WAVE;
-    uint64_t chunk = gw;
-    while (chunk < nchunks) {
-      const uint64_t i = chunk * WAVE + lane;
-      uint32_t v = 0;
-      uint64_t b = 0, d = 0;
-      if (i < f) {
-        v = __hip_atomic_load(&F[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (v < n) {  // always true; an id out of range would be skipped, not used as an index
-          b = off[v];
-          d = off[v + 1] - b;
-        }
-      }
+    // edge-balanced expansion of up to 64 sources (one per lane: v, row
+    // start b, degree d); claims go to the wavefront's LDS queue, a full
+    // queue out to the next frontier
+    auto expand = [&](uint32_t v, uint64_t b, uint64_t d) {
       const uint64_t incl = wave_incl_scan64(d);
       const uint64_t total = __shfl(incl, WAVE - 1);
       s_ex[wid][lane] = incl - d;
@@ -604,10 +605,60 @@ __global__ __launch_bounds__(COOP_WG) void k_reach_coop(uint32_t n, uint32_t* F0
         }
       }
       __builtin_amdgcn_wave_barrier();
+      return total;
+    };
+    // chunk gw first (no atomic); beyond the first pass chunks come from a counter
+    const uint64_t nchunks = ((uint64_t)f + WAVE - 1) / WAVE;
+    uint64_t chunk = gw;
+    while (chunk < nchunks) {
+      const uint64_t i = chunk * WAVE + lane;
+      uint32_t v = 0;
+      uint64_t b = 0, d = 0;
+      if (i < f) {
+        v = __hip_atomic_load(&F[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v < n) {  // always true; an id out of range would be skipped, not used as an index
+          b = off[v];
+          d = off[v + 1] - b;
+        }
+      }
+      expand(v, b, d);
       if (nchunks <= waves) break;
       uint32_t k = 0;
       if (lane == 0) k = atomicAdd(&ctl->work[s0], 1u);
       chunk = (uint64_t)waves + __shfl(k, 0);
+    }
+    // Reachability needs each claimed vertex expanded once, not level by
+    // level. On a thin frontier of low degree (at most 16 vertices per
+    // wavefront, at most 4 edges per vertex: long-diameter graphs) a
+    // wavefront goes on with its own claims for a few rounds while they fit
+    // one round, so such a frontier takes fewer barriers; other frontiers
+    // wait for the next level, where all wavefronts share them. A claim of
+    // degree above d_max stays for the next level (its degree was counted
+    // for it).
+    uint64_t loc_e = 0, loc_v = 0;
+    const bool thin = f <= 16ull * waves && fe <= 4ull * f;
+    for (uint32_t r = 0; thin && r < COOP_LOCAL && qn > 0 && qn <= WAVE; ++r) {
+      const uint32_t take = min(qn, (uint32_t)WAVE);
+      qn -= take;
+      const bool have = lane < take;
+      uint32_t v = have ? s_q[wid][qn + lane] : 0u;
+      uint64_t b = 0, d = 0;
+      if (have) {
+        b = off[v];
+        d = off[v + 1] - b;
+      }
+      __builtin_amdgcn_wave_barrier();
+      const bool big = have && d > d_max;
+      const uint64_t bm = __ballot(big);
+      if (bm) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&ctl->cnt[s1], (uint32_t)__popcll(bm));
+        base = __shfl(base, 0);
+        if (big) __hip_atomic_store(&N[base + __popcll(bm & below)], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (have && !big) my_sum -= d;  // not in the next frontier after all (a.noff is off)
+      loc_v += (uint32_t)__popcll(__ballot(have && !big));
+      loc_e += expand(v, b, big ? 0 : d);
     }
     // the workgroup's staged entries, degree sum and maximum: one atomic each
     my_sum = wave_sum64(my_sum);
@@ -616,20 +667,26 @@ __global__ __launch_bounds__(COOP_WG) void k_reach_coop(uint32_t n, uint32_t* F0
       s_sum[wid] = my_sum;
       s_max[wid] = my_max;
       s_qn[wid] = qn;
+      s_lexp[wid] = loc_e;
+      s_lvert[wid] = loc_v;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      unsigned long long bs = 0;
+      unsigned long long bs = 0, le = 0, lv = 0;
       uint32_t bm = 0, bq = 0;
       for (int k = 0; k < COOP_WAVES; ++k) {
         bs += s_sum[k];
         bm = max(bm, s_max[k]);
+        le += s_lexp[k];
+        lv += s_lvert[k];
         const uint32_t q = s_qn[k];
         s_qn[k] = bq;  // exclusive prefix
         bq += q;
       }
       if (bs) atomicAdd(&ctl->nsum[s1], bs);
       if (bm) atomicMax(&ctl->maxd[s1], bm);
+      if (le) atomicAdd(&ctl->lexp[s1], le);
+      if (lv) atomicAdd(&ctl->lvert[s1], lv);
       s_base = bq ? atomicAdd(&ctl->cnt[s1], bq) : 0u;
     }
     __syncthreads();
@@ -666,6 +723,8 @@ __global__ void k_coop_init(CoopCtl* ctl, const uint32_t* fcount, unsigned long 
       ctl->work[k] = 0;
       ctl->nsum[k] = 0;
       ctl->maxd[k] = 0;
+      ctl->lexp[k] = 0;
+      ctl->lvert[k] = 0;
     }
     ctl->cnt[0] = *fcount;
     ctl->nsum[0] = edges;

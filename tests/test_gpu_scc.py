@@ -163,8 +163,10 @@ def test_random_graphs(ka, O, kind, opts):
 
 
 def test_persistent_levels(ka, O):
-    """long-diameter lattice: the persistent kernel runs the push levels; the
-    same labels and the same edge count as separate launches per level"""
+    """long-diameter lattice: the persistent kernel runs the push levels (and
+    expands its own claims for a few rounds per level, so it needs fewer
+    levels); the same labels and the same FW-BW decisions as separate
+    launches per level"""
     g = ka.generate("lattice", 31, side=40, p_bidir=0.05)
     ref = O.tarjan(host_graph(O, g))
     for rounds in (1, 16):
@@ -172,7 +174,7 @@ def test_persistent_levels(ka, O):
         lab2, st2 = gpu_labels(ka, g, fwbw_rounds=rounds, flags=16)
         assert np.array_equal(lab, ref) and np.array_equal(lab2, ref)
         assert st.fwbw_levels_coop > 0 and st2.fwbw_levels_coop == 0
-        assert st.fwbw_levels_coop + st.fwbw_levels_push == st2.fwbw_levels_push
+        assert st.fwbw_levels_coop + st.fwbw_levels_push <= st2.fwbw_levels_push
         assert st.decided_fwbw == st2.decided_fwbw
 
 
