@@ -770,9 +770,19 @@ __global__ void k_coop_publish(const CoopCtl* ctl, uint32_t* cnt_even, uint32_t*
 // One pass over all vertices: in- or out-degree 0 -> decided with its own
 // id; the rest form the live bitmap (written as two words per wavefront
 // ballot). Each workgroup reduces its (max in*out, min id) pair.
+// Besides the live bitmap it writes zeros to the words of inq, fwv and bwv
+// it owns, and block 0 zeroes the ncnt counters cnt and the four cnt64:
+// the state a solve starts from, without fill launches of its own.
 __global__ __launch_bounds__(WG) void k_trim_first(uint32_t n, const uint64_t* __restrict__ fw_off,
                                                    const uint64_t* __restrict__ bw_off, uint32_t* __restrict__ live,
-                                                   uint32_t* __restrict__ labels, uint64_t* __restrict__ part) {
+                                                   uint32_t* __restrict__ labels, uint64_t* __restrict__ part,
+                                                   uint32_t* __restrict__ inq, uint32_t* __restrict__ fwv,
+                                                   uint32_t* __restrict__ bwv, uint32_t* __restrict__ cnt,
+                                                   uint32_t ncnt, unsigned long long* __restrict__ cnt64) {
+  if (blockIdx.x == 0) {
+    for (uint32_t i = threadIdx.x; i < ncnt; i += WG) cnt[i] = 0;
+    if (threadIdx.x < 4) cnt64[threadIdx.x] = 0;
+  }
   __shared__ uint64_t s_p[WG];
   __shared__ uint32_t s_id[WG];
   __shared__ uint32_t s_red[WG / WAVE];
@@ -797,8 +807,13 @@ __global__ __launch_bounds__(WG) void k_trim_first(uint32_t n, const uint64_t* _
     uint64_t mask = __ballot(is_live);
     uint32_t lane = lane_id();
     uint64_t word0 = (base + threadIdx.x - lane) >> 5;
-    if (lane == 0) live[word0] = (uint32_t)mask;
-    if (lane == 32) live[word0 + 1] = (uint32_t)(mask >> 32);
+    if (lane == 0 || lane == 32) {
+      const uint64_t w = word0 + (lane >> 5);
+      live[w] = (uint32_t)(mask >> lane);
+      inq[w] = 0;
+      fwv[w] = 0;
+      bwv[w] = 0;
+    }
     dead += (v64 < n && !is_live);
   }
   uint32_t dead_sum = block_sum(dead, s_red);
@@ -823,10 +838,24 @@ __global__ __launch_bounds__(WG) void k_trim_first(uint32_t n, const uint64_t* _
   }
 }
 
+// What k_pivot_reduce seeds for FW-BW from the pivot p it finds, into
+// bitmaps and counters k_trim_first zeroed: bit p of fwv and bwv, the
+// forward frontier queue = [p] with *qcount = 1, and the exclusive scan of
+// its out-degree S = [0, deg(p)] (the first level's, so the forward reach
+// starts with its expansion). No pivot: *qcount = 0, S = [0].
+struct FwbwSeed {
+  uint32_t* fwv;
+  uint32_t* bwv;
+  uint32_t* queue;
+  uint32_t* qcount;
+  const uint64_t* off;
+  uint64_t* S;
+};
+
 // (max product, smallest id) over workgroup partials -> out[0] (0xFFFFFFFF
 // if none); summed partial counts -> *count_out
 __global__ __launch_bounds__(WG) void k_pivot_reduce(const uint64_t* part, uint32_t nparts, uint32_t* out,
-                                                     uint32_t* count_out) {
+                                                     uint32_t* count_out, FwbwSeed seed = {}) {
   __shared__ uint64_t s_p[WG];
   __shared__ uint32_t s_id[WG];
   __shared__ uint32_t s_red[WG / WAVE];
@@ -856,7 +885,21 @@ __global__ __launch_bounds__(WG) void k_pivot_reduce(const uint64_t* part, uint3
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = s_id[0];
+  if (threadIdx.x == 0) {
+    const uint32_t p = s_id[0];
+    *out = p;
+    if (seed.queue) {
+      const bool any = p != 0xFFFFFFFFu;
+      if (any) {
+        seed.fwv[p >> 5] = 1u << (p & 31);
+        seed.bwv[p >> 5] = 1u << (p & 31);
+        seed.queue[0] = p;
+        seed.S[1] = seed.off[p + 1] - seed.off[p];
+      }
+      *seed.qcount = any ? 1u : 0u;
+      seed.S[0] = 0;
+    }
+  }
 }
 
 // =====================================================================
@@ -1144,6 +1187,26 @@ __global__ void k_level_reset(uint32_t* cnt_slot, unsigned long long* cnt64, uin
     cnt64[2] = 0;
     if (s0) *s0 = 0;
     if (s1) *s1 = 0;
+  }
+}
+
+// The start of a reach from one vertex whose bit is already set: queue =
+// [*vdev] with *qcount = 1 (0 and nothing else when *vdev is 0xFFFFFFFF),
+// the level counters k_level_reset zeroes, and the first level's degree
+// scan S = [0, deg]
+__global__ void k_reach_start(const uint32_t* vdev, uint32_t* queue, uint32_t* qcount, uint32_t* cnt_slot,
+                              unsigned long long* cnt64, const uint64_t* __restrict__ off, uint64_t* S) {
+  if (threadIdx.x == 0) {
+    const uint32_t v = *vdev;
+    const bool any = v != 0xFFFFFFFFu;
+    if (any) {
+      queue[0] = v;
+      S[1] = off[v + 1] - off[v];
+    }
+    *qcount = any ? 1u : 0u;
+    S[0] = 0;
+    *cnt_slot = 0;
+    cnt64[0] = cnt64[1] = cnt64[2] = 0;
   }
 }
 
@@ -2579,7 +2642,11 @@ int run_coop(Run& R, int op, const OpArgs& a, const uint64_t* off, const uint32_
   return KSPN_OK;
 }
 
-int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels_push, uint32_t* fw_levels_pull) {
+// With `seeded`, the frontier queue, the level counters and the first
+// level's degree scan are already on the device (k_pivot_reduce's FwbwSeed,
+// k_reach_start).
+int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels_push, uint32_t* fw_levels_pull,
+          bool seeded = false) {
   Ctx* c = R.c;
   const kspn_graph* g = R.g;
   const uint64_t* off = dir == 0 ? g->fw_off : g->bw_off;
@@ -2620,7 +2687,8 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   bool list_level = false;  // the level just run visited lists
   uint32_t L = 0;           // list levels since the last sweep
   // each level's publication zeroes the counters of the level after it
-  k_level_reset<<<1, 64, 0, R.s>>>(&c->cnt[nc], c->cnt64);
+  if (!seeded) k_level_reset<<<1, 64, 0, R.s>>>(&c->cnt[nc], c->cnt64);
+  bool scanned = seeded;  // the first level's degree scan is done
   while (f > 0) {
     // direction: Beamer's rule on the frontier's edge count, which the
     // previous level summed on the device for the vertices it claimed
@@ -2645,7 +2713,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       // nothing expanded (not expected: the bounds were checked): one level on the host
     }
     if (!pull) {
-      {
+      if (!scanned) {
         Prof p(c, R.s, K_FRONTIER_DEG, 28.0 * f);
         if (f < 4096) {  // S[0..f] fits the 4096 slots of one workgroup
           k_deg_scan_small<<<1, 1024, 0, R.s>>>(F, &c->cnt[fc], off, c->scan);
@@ -2653,7 +2721,8 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
           k_frontier_deg<<<stream_grid(c, (uint64_t)f + 1), WG, 0, R.s>>>(F, &c->cnt[fc], off, c->deg);
         }
       }
-      if (f >= 4096) KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)f + 1, R.s));
+      if (f >= 4096 && !scanned) KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)f + 1, R.s));
+      scanned = false;
       int kern = dir == 0 ? K_EXPAND_FW : K_EXPAND_BW;
       uint32_t grid = known ? expand_grid(c, edges) : (uint32_t)c->num_cu * 8;
       Prof p(c, R.s, kern, 20.0 * f + (4.0 + 0.125) * (double)edges);
@@ -2785,20 +2854,23 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
 // (cnt[C_DECIDED]) for the caller's next publication.
 int fwbw(Run& R, uint32_t pivot, uint32_t* decided, bool publish = true) {
   Ctx* c = R.c;
-  {
-    Prof p(c, R.s, K_BITMAP, 4.0 * R.words);
-    k_seed<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->fwv, R.words, pivot, R.live_pending ? &c->cnt[C_FLAG] : nullptr,
-                                                    c->qa, &c->cnt[C_Q0]);
-  }
-  // the backward seed reads the pivot on the device too when the forward one did
-  const uint32_t* seed_dev = R.live_pending ? &c->cnt[C_FLAG] : nullptr;
+  // After the first trim of the plain single-pivot path, k_pivot_reduce
+  // seeded both directions on the device (R.live_pending)
+  const bool seeded = R.live_pending;
   uint32_t lp = 0, lq = 0;
-  KSPN_TRY(reach(R, 0, 1, nullptr, &lp, &lq));
-  {
+  if (!seeded) {
     Prof p(c, R.s, K_BITMAP, 4.0 * R.words);
-    k_seed<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwv, R.words, pivot, seed_dev, c->qa, &c->cnt[C_Q0]);
+    k_seed<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->fwv, R.words, pivot, nullptr, c->qa, &c->cnt[C_Q0]);
   }
-  KSPN_TRY(reach(R, 1, 1, nullptr, &lp, &lq));
+  KSPN_TRY(reach(R, 0, 1, nullptr, &lp, &lq, seeded));
+  if (seeded) {
+    k_reach_start<<<1, 64, 0, R.s>>>(&c->cnt[C_FLAG], c->qa, &c->cnt[C_Q0], &c->cnt[C_Q1], c->cnt64, R.g->bw_off,
+                                     c->scan);
+  } else {
+    Prof p(c, R.s, K_BITMAP, 4.0 * R.words);
+    k_seed<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwv, R.words, pivot, nullptr, c->qa, &c->cnt[C_Q0]);
+  }
+  KSPN_TRY(reach(R, 1, 1, nullptr, &lp, &lq, seeded));
   // the pivot's SCC: clear it from live, then label it with its smallest member
   uint32_t nparts = stream_grid(c, R.words);
   {
@@ -3547,26 +3619,26 @@ int run_pipeline(Run& R) {
   const char* rest_name = plan.residual_rest ? "residual" : (plan.trim_ex ? "color_trim_ex" : "color");
   // ---- trim-1 first pass + pivot
   KSPN_TRY(P.begin(first_name, plan.ex_first));
-  KSPN_HIP(hipMemsetAsync(c->cnt, 0, sizeof(uint32_t) * C_COUNT_, R.s));
-  KSPN_HIP(hipMemsetAsync(c->cnt64, 0, sizeof(unsigned long long) * 4, R.s));
   c->h_u64[1] = 0;
-  KSPN_HIP(hipMemsetAsync(c->live, 0, R.words * sizeof(uint32_t), R.s));
-  KSPN_HIP(hipMemsetAsync(c->inq, 0, R.words * sizeof(uint32_t), R.s));
-  uint32_t nparts = std::min<uint32_t>(stream_grid(c, R.n), (uint32_t)c->num_cu * 16);
-  {
-    Prof p(c, R.s, K_TRIM_FIRST, 20.0 * R.n + R.n / 8.0);
-    k_trim_first<<<nparts, WG, 0, R.s>>>(R.n, g->fw_off, g->bw_off, c->live, R.labels, c->part);
-  }
-  {
-    Prof p(c, R.s, K_PIVOT, 24.0 * nparts);
-    k_pivot_reduce<<<1, WG, 0, R.s>>>(c->part, nparts, &c->cnt[C_FLAG], &c->cnt[C_DECIDED]);
-  }
   // The plain single-pivot path seeds FW-BW with the pivot on the device and
   // learns the trim's decisions from the first publication of the forward
   // reach, one host round trip fewer
   const uint32_t rounds_cfg = plan.fwbw ? (R.o.fwbw_rounds ? R.o.fwbw_rounds : 1) : 0;
   const bool defer = rounds_cfg == 1 && !plan.ex_first && !R.pid_preset && R.o.pivot == KSPN_PIVOT_PLAIN &&
                      !R.o.residual_threshold && R.n > 0;
+  // the trim also zeroes the counters and the inq, fwv and bwv bitmaps
+  uint32_t nparts = std::min<uint32_t>(stream_grid(c, R.n), (uint32_t)c->num_cu * 16);
+  {
+    Prof p(c, R.s, K_TRIM_FIRST, 20.0 * R.n + R.n / 8.0);
+    k_trim_first<<<nparts, WG, 0, R.s>>>(R.n, g->fw_off, g->bw_off, c->live, R.labels, c->part, c->inq, c->fwv,
+                                         c->bwv, c->cnt, C_COUNT_, c->cnt64);
+  }
+  {
+    Prof p(c, R.s, K_PIVOT, 24.0 * nparts);
+    FwbwSeed seed{};
+    if (defer) seed = FwbwSeed{c->fwv, c->bwv, c->qa, &c->cnt[C_Q0], g->fw_off, c->scan};
+    k_pivot_reduce<<<1, WG, 0, R.s>>>(c->part, nparts, &c->cnt[C_FLAG], &c->cnt[C_DECIDED], seed);
+  }
   uint32_t decided = 0, pivot = 0;
   if (defer) {
     R.live_pending = true;
