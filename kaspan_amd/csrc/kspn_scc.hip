@@ -770,7 +770,8 @@ __global__ void k_coop_publish(const CoopCtl* ctl, uint32_t* cnt_even, uint32_t*
 // One pass over all vertices: in- or out-degree 0 -> decided with its own
 // id; the rest form the live bitmap (written as two words per wavefront
 // ballot). Each workgroup reduces its (max in*out, min id) pair.
-// Besides the live bitmap it writes zeros to the words of inq, fwv and bwv
+// Without `labels` the labels stay for k_label_bits's dense pass. Besides
+// the live bitmap it writes zeros to the words of inq, fwv and bwv
 // it owns, and block 0 zeroes the ncnt counters cnt and the four cnt64:
 // the state a solve starts from, without fill launches of its own.
 __global__ __launch_bounds__(WG) void k_trim_first(uint32_t n, const uint64_t* __restrict__ fw_off,
@@ -789,32 +790,54 @@ __global__ __launch_bounds__(WG) void k_trim_first(uint32_t n, const uint64_t* _
   uint32_t dead = 0;
   uint64_t best = 0;
   uint32_t best_id = 0xFFFFFFFFu;
-  const uint64_t n_pad = ((uint64_t)n + WG - 1) / WG * WG;
-  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < n_pad; base += (uint64_t)gridDim.x * WG) {
-    uint64_t v64 = base + threadIdx.x;
-    bool is_live = false;
-    if (v64 < n) {
-      uint32_t v = (uint32_t)v64;
-      uint64_t out = fw_off[v + 1] - fw_off[v];
-      uint64_t in = bw_off[v + 1] - bw_off[v];
-      is_live = in != 0 && out != 0;
-      labels[v] = is_live ? UNDEC : v;
-      if (is_live) {
-        uint64_t p = in * out;
-        if (p > best || (p == best && v < best_id)) { best = p; best_id = v; }
+  // TRIM_U vertices per thread and step (one per 256-vertex tile), their
+  // offset loads issued together
+  constexpr int TRIM_U = 4;
+  constexpr uint64_t SPAN = (uint64_t)WG * TRIM_U;
+  const uint64_t n_pad = ((uint64_t)n + SPAN - 1) / SPAN * SPAN;
+  const uint64_t n_pad256 = ((uint64_t)n + WG - 1) / WG * WG;
+  const uint32_t lane = lane_id();
+  for (uint64_t base = (uint64_t)blockIdx.x * SPAN; base < n_pad; base += (uint64_t)gridDim.x * SPAN) {
+    uint64_t fo[TRIM_U], fo1[TRIM_U], bo[TRIM_U], bo1[TRIM_U];
+#pragma unroll
+    for (int k = 0; k < TRIM_U; ++k) {
+      const uint64_t v = base + (uint64_t)k * WG + threadIdx.x;
+      const bool ok = v < n;
+      fo[k] = ok ? fw_off[v] : 0;
+      fo1[k] = ok ? fw_off[v + 1] : 0;
+      bo[k] = ok ? bw_off[v] : 0;
+      bo1[k] = ok ? bw_off[v + 1] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < TRIM_U; ++k) {
+      const uint64_t v64 = base + (uint64_t)k * WG + threadIdx.x;
+      const uint64_t out = fo1[k] - fo[k];
+      const uint64_t in = bo1[k] - bo[k];
+      const bool is_live = v64 < n && in != 0 && out != 0;
+      if (v64 < n) {
+        const uint32_t v = (uint32_t)v64;
+        if (labels) labels[v] = is_live ? UNDEC : v;
+        if (is_live) {
+          const uint64_t p = in * out;
+          if (p > best || (p == best && v < best_id)) {
+            best = p;
+            best_id = v;
+          }
+        }
       }
+      const uint64_t mask = __ballot(is_live);
+      const uint64_t first = v64 - lane;  // the wavefront's first vertex of tile k
+      // words up to the 256-vertex padding get written (zero past n), as the
+      // bitmaps are allocated; the steps' 1024-vertex padding owns none
+      if (first < n_pad256 && (lane == 0 || lane == 32)) {
+        const uint64_t w = (first >> 5) + (lane >> 5);
+        live[w] = (uint32_t)(mask >> lane);
+        inq[w] = 0;
+        fwv[w] = 0;
+        bwv[w] = 0;
+      }
+      dead += (v64 < n && !is_live);
     }
-    uint64_t mask = __ballot(is_live);
-    uint32_t lane = lane_id();
-    uint64_t word0 = (base + threadIdx.x - lane) >> 5;
-    if (lane == 0 || lane == 32) {
-      const uint64_t w = word0 + (lane >> 5);
-      live[w] = (uint32_t)(mask >> lane);
-      inq[w] = 0;
-      fwv[w] = 0;
-      bwv[w] = 0;
-    }
-    dead += (v64 < n && !is_live);
   }
   uint32_t dead_sum = block_sum(dead, s_red);
   s_p[threadIdx.x] = best;
@@ -1235,11 +1258,33 @@ __global__ __launch_bounds__(WG) void k_fwbw_finish(uint32_t* live, const uint32
 }
 
 // labels[v] = *minv for every v whose bit is set in bm
+// With `live` (the live bitmap the SCC has left, after a first trim that
+// wrote no labels) every label is written, in full lines: the SCC's
+// *minv, UNDEC for live vertices, and their own id for the vertices the
+// first trim decided (neither live nor in the SCC).
 __global__ void k_label_bits(uint32_t n, const uint32_t* __restrict__ bm, const uint32_t* minv,
-                             uint32_t* __restrict__ labels) {
+                             uint32_t* __restrict__ labels, const uint32_t* __restrict__ live = nullptr) {
+  constexpr int LU = 8;  // vertices per thread and step (one per 256-vertex tile), word loads issued together
   const uint32_t m = *minv;
-  for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG)
-    if (bm_get(bm, (uint32_t)v)) labels[v] = m;
+  for (uint64_t base = blockIdx.x * (uint64_t)WG * LU; base < n; base += (uint64_t)gridDim.x * WG * LU) {
+    uint32_t bw[LU], lw[LU];
+#pragma unroll
+    for (int k = 0; k < LU; ++k) {
+      const uint64_t v = base + (uint64_t)k * WG + threadIdx.x;
+      bw[k] = v < n ? bm[v >> 5] : 0u;
+      lw[k] = (live && v < n) ? live[v >> 5] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < LU; ++k) {
+      const uint64_t v = base + (uint64_t)k * WG + threadIdx.x;
+      if (v >= n) break;
+      const bool in = (bw[k] >> (v & 31)) & 1u;
+      if (live)
+        labels[v] = in ? m : (((lw[k] >> (v & 31)) & 1u) ? UNDEC : (uint32_t)v);
+      else if (in)
+        labels[v] = m;
+    }
+  }
 }
 
 // =====================================================================
@@ -2879,8 +2924,9 @@ int fwbw(Run& R, uint32_t pivot, uint32_t* decided, bool publish = true) {
     k_pivot_reduce<<<1, WG, 0, R.s>>>(c->part, nparts, &c->cnt[C_MIN], &c->cnt[C_DECIDED]);
   }
   {
-    Prof p(c, R.s, K_LABEL, R.n / 8.0);
-    k_label_bits<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->bwv, &c->cnt[C_MIN], R.labels);
+    Prof p(c, R.s, K_LABEL, seeded ? R.n / 4.0 + 4.0 * R.n : R.n / 8.0);
+    k_label_bits<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->bwv, &c->cnt[C_MIN], R.labels,
+                                                      seeded ? c->live : nullptr);
   }
   if (R.st) {
     R.st->fwbw_levels_push += lp;
@@ -3629,9 +3675,10 @@ int run_pipeline(Run& R) {
   // the trim also zeroes the counters and the inq, fwv and bwv bitmaps
   uint32_t nparts = std::min<uint32_t>(stream_grid(c, R.n), (uint32_t)c->num_cu * 16);
   {
-    Prof p(c, R.s, K_TRIM_FIRST, 20.0 * R.n + R.n / 8.0);
-    k_trim_first<<<nparts, WG, 0, R.s>>>(R.n, g->fw_off, g->bw_off, c->live, R.labels, c->part, c->inq, c->fwv,
-                                         c->bwv, c->cnt, C_COUNT_, c->cnt64);
+    Prof p(c, R.s, K_TRIM_FIRST, (defer ? 16.0 : 20.0) * R.n + R.n / 2.0);
+    // the seeded FW-BW writes every label after it (k_label_bits)
+    k_trim_first<<<nparts, WG, 0, R.s>>>(R.n, g->fw_off, g->bw_off, c->live, defer ? nullptr : R.labels, c->part,
+                                         c->inq, c->fwv, c->bwv, c->cnt, C_COUNT_, c->cnt64);
   }
   {
     Prof p(c, R.s, K_PIVOT, 24.0 * nparts);
