@@ -222,11 +222,13 @@ enum Counter : int {
   C_DTOT,  // running length of the decided list
   C_ERR,   // set by a kernel that met a broken internal invariant
   C_CURSOR,  // write cursor of an unordered bitmap compaction
-  C_SURV0,   // sizes of the candidate lists of pull list levels (three, in rotation)
+  C_SURV0,   // sizes of the candidate lists of pull list levels (four, in rotation)
   C_SURV1,
   C_SURV2,
+  C_SURV3,
   C_ISO,     // live vertices without a live neighbour, counted by the tail preparation
   C_LREC,    // claims of the last pull list level of a batch
+  C_LIVE,    // vertices left live after the last FW-BW, listed by k_fwbw_finish for the tail solver
   C_COUNT_ = 24
 };
 

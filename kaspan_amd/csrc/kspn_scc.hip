@@ -143,48 +143,96 @@ __global__ void k_seed(uint32_t* bm, uint64_t words, uint32_t v, const uint32_t*
   }
 }
 
-// Unordered bitmap -> list: each workgroup step takes 4*WG words, scans
-// their popcounts in LDS and reserves its range with one atomicAdd on
-// *cursor (zero before the launch). For lists whose order does not matter.
+constexpr uint32_t B2Q_STEPS = 8;
+constexpr uint64_t B2Q_CHUNK = (uint64_t)WG * 4 * B2Q_STEPS;
+
+// inclusive prefix sum over the wavefront
+__device__ inline uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (uint32_t o = 1; o < WAVE; o <<= 1) {
+    const uint32_t x = __shfl_up(v, o);
+    if (lane >= o) v += x;
+  }
+  return v;
+}
+
+// Appends the set bits of a workgroup's chunk of B2Q_CHUNK words (w[t][k]
+// is word base + t*4*WG + 4*threadIdx.x + k) to list, reserving the chunk's
+// range with one atomicAdd on *cursor; each thread writes its bits to a run
+// of its own. Every thread of the workgroup calls.
+__device__ inline void chunk_to_list(const uint32_t (&w)[B2Q_STEPS][4], uint64_t base, uint32_t* __restrict__ list,
+                                     uint32_t* cursor, uint32_t* s_wave, uint32_t* s_base) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / WAVE;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (uint32_t t = 0; t < B2Q_STEPS; ++t) cnt += __popc(w[t][0]) + __popc(w[t][1]) + __popc(w[t][2]) + __popc(w[t][3]);
+  const uint32_t inc = wave_incl_scan(cnt);
+  if (lane == WAVE - 1) s_wave[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (uint32_t k = 0; k < WG / WAVE; ++k) {
+      const uint32_t t = s_wave[k];
+      s_wave[k] = tot;
+      tot += t;
+    }
+    *s_base = tot ? atomicAdd(cursor, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t pos = *s_base + s_wave[wave] + inc - cnt;
+#pragma unroll
+  for (uint32_t t = 0; t < B2Q_STEPS; ++t) {
+    const uint64_t i0 = base + (uint64_t)t * WG * 4 + (uint64_t)threadIdx.x * 4;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+      uint32_t x = w[t][k];
+      while (x) {
+        const uint32_t b = __ffs(x) - 1;
+        list[pos++] = (uint32_t)((i0 + k) * 32 + b);
+        x &= x - 1;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Unordered bitmap -> list. Each workgroup takes chunks of B2Q_CHUNK words
+// (B2Q_STEPS 16-byte loads per thread, all in flight together) and appends
+// their set bits with chunk_to_list: few, large chunks keep the atomics on
+// *cursor (zero before the launch) few. For lists whose order does not
+// matter. With `guard`, nothing is written when *guard exceeds guard_max
+// (read after the words, so the two loads overlap). Launch b2q_grid(words)
+// workgroups.
 __global__ __launch_bounds__(WG) void k_bitmap_to_queue(const uint32_t* __restrict__ bm, uint64_t words,
                                                         uint32_t* __restrict__ list, uint32_t* cursor,
                                                         const unsigned long long* guard = nullptr,
                                                         unsigned long long guard_max = 0) {
-  __shared__ uint32_t s_scan[WG];
+  __shared__ uint32_t s_wave[WG / WAVE];
   __shared__ uint32_t s_base;
-  if (guard && *guard > guard_max) return;  // the set bits are more than the caller wants listed
-  constexpr uint32_t PER = 4;
-  for (uint64_t base = (uint64_t)blockIdx.x * WG * PER; base < words; base += (uint64_t)gridDim.x * WG * PER) {
-    uint32_t w[PER];
-    uint32_t cnt = 0;
+  bool first = true;
+  for (uint64_t base = (uint64_t)blockIdx.x * B2Q_CHUNK; base < words; base += (uint64_t)gridDim.x * B2Q_CHUNK) {
+    uint32_t w[B2Q_STEPS][4];
 #pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) {
-      const uint64_t i = base + (uint64_t)threadIdx.x * PER + k;
-      w[k] = i < words ? bm[i] : 0u;
-      cnt += __popc(w[k]);
-    }
-    s_scan[threadIdx.x] = cnt;
-    __syncthreads();
-    for (uint32_t o = 1; o < WG; o <<= 1) {
-      const uint32_t x = threadIdx.x >= o ? s_scan[threadIdx.x - o] : 0u;
-      __syncthreads();
-      s_scan[threadIdx.x] += x;
-      __syncthreads();
-    }
-    if (threadIdx.x == WG - 1) s_base = s_scan[WG - 1] ? atomicAdd(cursor, s_scan[WG - 1]) : 0u;
-    __syncthreads();
-    uint32_t pos = s_base + s_scan[threadIdx.x] - cnt;
+    for (uint32_t t = 0; t < B2Q_STEPS; ++t) {
+      const uint64_t i0 = base + (uint64_t)t * WG * 4 + (uint64_t)threadIdx.x * 4;
+      if (i0 + 4 <= words) {
+        const uint4 q = *reinterpret_cast<const uint4*>(bm + i0);  // i0 is a multiple of 4 words
+        w[t][0] = q.x;
+        w[t][1] = q.y;
+        w[t][2] = q.z;
+        w[t][3] = q.w;
+      } else {
 #pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) {
-      const uint64_t i = base + (uint64_t)threadIdx.x * PER + k;
-      uint32_t x = w[k];
-      while (x) {
-        const uint32_t b = __ffs(x) - 1;
-        list[pos++] = (uint32_t)(i * 32 + b);
-        x &= x - 1;
+        for (uint32_t k = 0; k < 4; ++k) w[t][k] = i0 + k < words ? bm[i0 + k] : 0u;
       }
     }
-    __syncthreads();
+    if (first) {
+      first = false;
+      if (guard && *guard > guard_max) return;  // the set bits are more than the caller wants listed
+    }
+    chunk_to_list(w, base, list, cursor, s_wave, &s_base);
   }
 }
 
@@ -861,6 +909,8 @@ __global__ __launch_bounds__(WG) void k_trim_first(uint32_t n, const uint64_t* _
   }
 }
 
+constexpr uint32_t RED_U = 16;  // single-workgroup reductions: partials per thread loaded together
+
 // What k_pivot_reduce seeds for FW-BW from the pivot p it finds, into
 // bitmaps and counters k_trim_first zeroed: bit p of fwv and bwv, the
 // forward frontier queue = [p] with *qcount = 1, and the exclusive scan of
@@ -885,11 +935,26 @@ __global__ __launch_bounds__(WG) void k_pivot_reduce(const uint64_t* part, uint3
   uint64_t best = 0;
   uint32_t best_id = 0xFFFFFFFFu;
   uint32_t total = 0;
-  for (uint32_t i = threadIdx.x; i < nparts; i += WG) {
-    uint64_t p = part[3 * i];
-    uint32_t id = (uint32_t)part[3 * i + 1];
-    total += (uint32_t)part[3 * i + 2];
-    if (id != 0xFFFFFFFFu && (p > best || (p == best && id < best_id))) { best = p; best_id = id; }
+  // RED_U partials per thread and step, loaded together
+  for (uint32_t i0 = threadIdx.x; i0 < nparts; i0 += WG * RED_U) {
+    uint64_t x[RED_U][3];
+#pragma unroll
+    for (uint32_t r = 0; r < RED_U; ++r) {
+      const uint32_t i = i0 + r * WG;
+      x[r][0] = i < nparts ? part[3 * i] : 0;
+      x[r][1] = i < nparts ? part[3 * i + 1] : 0xFFFFFFFFull;
+      x[r][2] = i < nparts ? part[3 * i + 2] : 0;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < RED_U; ++r) {
+      const uint64_t p = x[r][0];
+      const uint32_t id = (uint32_t)x[r][1];
+      total += (uint32_t)x[r][2];
+      if (id != 0xFFFFFFFFu && (p > best || (p == best && id < best_id))) {
+        best = p;
+        best_id = id;
+      }
+    }
   }
   total = block_sum(total, s_red);
   if (threadIdx.x == 0 && count_out) *count_out = total;
@@ -939,7 +1004,7 @@ __global__ __launch_bounds__(WG) void k_pivot_reduce(const uint64_t* part, uint3
 // (noff, optional) of the claimed vertices, or with `sbm` the number of
 // candidates left unreached, whose bits it receives (written like fnew) so
 // that a short remainder can be compacted into a list for k_pull_list.
-// Block 0 zeroes the three list counters `lcnt`, which the sweep does not use.
+// Block 0 zeroes the four list counters `lcnt`, which the sweep does not use.
 constexpr int PULL_ILP = 2;  // vertices in flight per thread (2 beat 1 and 4 on R-MAT 24 and the 4096-graph batch)
 constexpr uint32_t PULL_FLUSH = 4;                     // k_pull_list: iterations between checks of its LDS queues
 constexpr uint32_t PULL_QCAP = PULL_FLUSH * WG * PULL_ILP;  // entries of each LDS queue (8 KiB)
@@ -1050,7 +1115,7 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
     part[3 * blockIdx.x + 2] = nd;
     if (blockIdx.x == 0) {
       *cursor = 0;  // for a compaction of this level's bitmap
-      if (lcnt) lcnt[0] = lcnt[1] = lcnt[2] = 0;
+      if (lcnt) lcnt[0] = lcnt[1] = lcnt[2] = lcnt[3] = 0;
     }
   }
 }
@@ -1070,7 +1135,8 @@ __global__ __launch_bounds__(WG) void k_pull_list(const uint32_t* __restrict__ l
                                                   const uint32_t* __restrict__ radj, const uint32_t* __restrict__ pid,
                                                   uint32_t* __restrict__ claimed_q, uint32_t* claimed_cnt,
                                                   uint32_t* __restrict__ out, uint32_t* out_cnt, uint32_t* zero,
-                                                  uint32_t* zero2, unsigned long long* edges_sum) {
+                                                  uint32_t* zero2, unsigned long long* edges_sum,
+                                                  const uint32_t* prev_cnt) {
   __shared__ uint64_t s_red[WG / WAVE];
   __shared__ uint32_t s_cq[PULL_QCAP], s_sq[PULL_QCAP];
   __shared__ uint32_t s_cn, s_cb, s_sn, s_sb;
@@ -1081,10 +1147,14 @@ __global__ __launch_bounds__(WG) void k_pull_list(const uint32_t* __restrict__ l
     *zero = 0;
     if (zero2) *zero2 = 0;
   }
+  const uint32_t cnt = *list_cnt;
+  // every candidate of the previous level stayed unreached: it claimed
+  // nothing, so nothing is left to reach. This level returns (leaving its
+  // output count zero), and the batch's later levels find no candidates.
+  if (prev_cnt && *prev_cnt == cnt) return;
   uint32_t iter = 0;
   uint64_t edges = 0;
   constexpr uint32_t SPAN = (uint32_t)WG * PULL_ILP;
-  const uint32_t cnt = *list_cnt;
   const uint32_t cnt_pad = (cnt + SPAN - 1) / SPAN * SPAN;
   for (uint32_t base = blockIdx.x * SPAN; base < cnt_pad; base += gridDim.x * SPAN) {
     uint32_t vv[PULL_ILP], pv[PULL_ILP];
@@ -1155,10 +1225,21 @@ __global__ __launch_bounds__(WG) void k_reduce3(const uint64_t* __restrict__ par
                                                 unsigned long long* cnt64) {
   __shared__ uint64_t s_red[WG / WAVE];
   uint64_t a = 0, b = 0, d = 0;
-  for (uint32_t i = threadIdx.x; i < nparts; i += WG) {
-    a += part[3 * i];
-    b += part[3 * i + 1];
-    d += part[3 * i + 2];
+  // RED_U partials per thread and step, loaded together
+  for (uint32_t i0 = threadIdx.x; i0 < nparts; i0 += WG * RED_U) {
+    uint64_t x[RED_U][3];
+#pragma unroll
+    for (uint32_t r = 0; r < RED_U; ++r) {
+      const uint32_t i = i0 + r * WG;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) x[r][j] = i < nparts ? part[3 * i + j] : 0;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < RED_U; ++r) {
+      a += x[r][0];
+      b += x[r][1];
+      d += x[r][2];
+    }
   }
   a = block_sum64(a, s_red);
   b = block_sum64(b, s_red);
@@ -1233,39 +1314,93 @@ __global__ void k_reach_start(const uint32_t* vdev, uint32_t* queue, uint32_t* q
   }
 }
 
-// after FW-BW: scc = bwv (backward-reached within forward-reached);
-// live &= ~scc; per-workgroup partials of |scc| and of its smallest vertex
-__global__ __launch_bounds__(WG) void k_fwbw_finish(uint32_t* live, const uint32_t* bwv, uint64_t words, uint64_t* part) {
+// After FW-BW: scc = bwv (backward-reached within forward-reached), live &=
+// ~scc. Each workgroup takes chunks of B2Q_CHUNK words (launch
+// finish_grid(words) workgroups, at most FINISH_PARTS) and leaves the
+// partials of |scc| (part[2b]) and of its smallest vertex (part[2b + 1])
+// that k_label_bits reduces. With `list`, the vertices left live are
+// appended to it as k_bitmap_to_queue appends (*cursor zero before the
+// launch). Block 0 zeroes the given counters `zero` (of the kernels after it).
+constexpr uint32_t FINISH_PARTS = WG;
+struct Zero3 {
+  uint32_t* p[3];
+};
+__global__ __launch_bounds__(WG) void k_fwbw_finish(uint32_t* live, const uint32_t* __restrict__ bwv, uint64_t words,
+                                                    uint64_t* __restrict__ part, uint32_t* __restrict__ list,
+                                                    uint32_t* cursor, Zero3 zero) {
+  __shared__ uint32_t s_wave[WG / WAVE];
+  __shared__ uint32_t s_base;
   __shared__ uint32_t s_red[WG / WAVE];
+  if (blockIdx.x == 0 && threadIdx.x < 3 && zero.p[threadIdx.x]) *zero.p[threadIdx.x] = 0;
   uint32_t count = 0, mn = 0xFFFFFFFFu;
-  const uint64_t w_pad = (words + WG - 1) / WG * WG;
-  for (uint64_t base = blockIdx.x * (uint64_t)WG; base < w_pad; base += (uint64_t)gridDim.x * WG) {
-    uint64_t i = base + threadIdx.x;
-    if (i < words) {
-      uint32_t scc = bwv[i];
-      live[i] &= ~scc;
-      count += __popc(scc);
-      if (scc) mn = min(mn, (uint32_t)(i * 32 + __ffs(scc) - 1));
+  for (uint64_t base = (uint64_t)blockIdx.x * B2Q_CHUNK; base < words; base += (uint64_t)gridDim.x * B2Q_CHUNK) {
+    uint32_t w[B2Q_STEPS][4];  // live words after the SCC left
+#pragma unroll
+    for (uint32_t t = 0; t < B2Q_STEPS; ++t) {
+      const uint64_t i0 = base + (uint64_t)t * WG * 4 + (uint64_t)threadIdx.x * 4;
+      uint32_t sc[4];
+      if (i0 + 4 <= words) {  // i0 is a multiple of 4 words: 16-byte accesses
+        const uint4 a = *reinterpret_cast<const uint4*>(bwv + i0);
+        const uint4 l = *reinterpret_cast<const uint4*>(live + i0);
+        sc[0] = a.x, sc[1] = a.y, sc[2] = a.z, sc[3] = a.w;
+        w[t][0] = l.x & ~a.x, w[t][1] = l.y & ~a.y, w[t][2] = l.z & ~a.z, w[t][3] = l.w & ~a.w;
+        *reinterpret_cast<uint4*>(live + i0) = make_uint4(w[t][0], w[t][1], w[t][2], w[t][3]);
+      } else {
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+          const bool in = i0 + k < words;
+          sc[k] = in ? bwv[i0 + k] : 0u;
+          w[t][k] = in ? live[i0 + k] & ~sc[k] : 0u;
+          if (in) live[i0 + k] = w[t][k];
+        }
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) {
+        count += __popc(sc[k]);
+        if (sc[k]) mn = min(mn, (uint32_t)((i0 + k) * 32 + __ffs(sc[k]) - 1));
+      }
     }
+    if (list) chunk_to_list(w, base, list, cursor, s_wave, &s_base);
   }
   count = block_sum(count, s_red);
   mn = block_min(mn, s_red);
   if (threadIdx.x == 0) {
-    part[3 * blockIdx.x] = 0;
-    part[3 * blockIdx.x + 1] = mn;  // reduced as an id with product 0: smallest id wins
-    part[3 * blockIdx.x + 2] = count;
+    part[2 * blockIdx.x] = count;
+    part[2 * blockIdx.x + 1] = mn;
   }
 }
 
-// labels[v] = *minv for every v whose bit is set in bm
-// With `live` (the live bitmap the SCC has left, after a first trim that
-// wrote no labels) every label is written, in full lines: the SCC's
-// *minv, UNDEC for live vertices, and their own id for the vertices the
-// first trim decided (neither live nor in the SCC).
-__global__ void k_label_bits(uint32_t n, const uint32_t* __restrict__ bm, const uint32_t* minv,
-                             uint32_t* __restrict__ labels, const uint32_t* __restrict__ live = nullptr) {
+// Labels of the pivot's SCC (the bits of bm): every workgroup reduces
+// k_fwbw_finish's nparts partials (at most WG) to the SCC's smallest vertex
+// m and its size, which block 0 writes to *minv_out and *count_out; then
+// labels[v] = m for the SCC. With `live` (the live bitmap the SCC has left,
+// after a first trim that wrote no labels) every label is written, in full
+// lines: m in the SCC, UNDEC for live vertices, and their own id for the
+// vertices the first trim decided (neither live nor in the SCC).
+__global__ __launch_bounds__(WG) void k_label_bits(uint32_t n, const uint32_t* __restrict__ bm,
+                                                   const uint64_t* __restrict__ part, uint32_t nparts,
+                                                   uint32_t* minv_out, uint32_t* count_out,
+                                                   uint32_t* __restrict__ labels,
+                                                   const uint32_t* __restrict__ live = nullptr) {
+  __shared__ uint32_t s_red[WG / WAVE];
+  __shared__ uint32_t s_m;
+  {
+    const bool have = threadIdx.x < nparts;
+    uint32_t c = have ? (uint32_t)part[2 * threadIdx.x] : 0u;
+    uint32_t x = have ? (uint32_t)part[2 * threadIdx.x + 1] : 0xFFFFFFFFu;
+    c = block_sum(c, s_red);
+    x = block_min(x, s_red);
+    if (threadIdx.x == 0) {
+      s_m = x;
+      if (blockIdx.x == 0) {
+        *minv_out = x;
+        *count_out = c;
+      }
+    }
+    __syncthreads();
+  }
   constexpr int LU = 8;  // vertices per thread and step (one per 256-vertex tile), word loads issued together
-  const uint32_t m = *minv;
+  const uint32_t m = s_m;
   for (uint64_t base = blockIdx.x * (uint64_t)WG * LU; base < n; base += (uint64_t)gridDim.x * WG * LU) {
     uint32_t bw[LU], lw[LU];
 #pragma unroll
@@ -2411,6 +2546,16 @@ inline uint32_t expand_grid(Ctx* c, uint64_t total) {
   return (uint32_t)std::min<uint64_t>(tiles, cap);
 }
 
+// workgroups of a k_fwbw_finish launch over `words` words (its partials)
+inline uint32_t finish_grid(uint64_t words) {
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((words + B2Q_CHUNK - 1) / B2Q_CHUNK, FINISH_PARTS));
+}
+
+// workgroups of a k_bitmap_to_queue launch over `words` words: one chunk each
+inline uint32_t b2q_grid(uint64_t words) {
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((words + B2Q_CHUNK - 1) / B2Q_CHUNK, 1u << 20));
+}
+
 inline uint32_t stream_grid(Ctx* c, uint64_t items) {
   uint64_t g = (items + WG - 1) / WG;
   uint64_t cap = (uint64_t)c->num_cu * 16;
@@ -2726,7 +2871,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   // has reached everything.
   constexpr uint32_t LIST_BATCH = 4;
   uint32_t* lists[2] = {c->qc, c->ulist};
-  const int lcnt[3] = {C_SURV0, C_SURV1, C_SURV2};
+  const int lcnt[4] = {C_SURV0, C_SURV1, C_SURV2, C_SURV3};
   const uint32_t list_max = std::max<uint32_t>(R.n / 16, 1u);  // longest remainder worth a list
   bool list_mode = false;   // the next pull level visits a list
   bool list_level = false;  // the level just run visited lists
@@ -2805,7 +2950,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
         k_reduce3<<<1, WG, 0, R.s>>>(c->part, grid, &c->cnt[nc], c->cnt64);
         {
           Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words);
-          k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->sbm, R.words, lists[0],
+          k_bitmap_to_queue<<<b2q_grid(R.words), WG, 0, R.s>>>(c->sbm, R.words, lists[0],
                                                                              &c->cnt[lcnt[0]], c->cnt64 + 1,
                                                                              list_max);
         }
@@ -2821,16 +2966,18 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
           // the first level zeroes the claim count the last one records
           uint32_t* rec = j + 1 == LIST_BATCH ? N : nullptr;
           uint32_t* zero2 = j == 0 ? &c->cnt[C_LREC] : nullptr;
+          // the previous level's candidate count (its slot is untouched until the next level)
+          const uint32_t* prev = j == 0 ? nullptr : &c->cnt[lcnt[(L + 3) % 4]];
           Prof p(c, R.s, K_PULL_LIST, 0.0);
           list_prof = p.idx;
           if (dir == 0)
-            k_pull_list<0><<<lgrid, WG, 0, R.s>>>(lists[L % 2], &c->cnt[lcnt[L % 3]], vis, roff, radj, pid, rec,
-                                                  &c->cnt[C_LREC], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 3]],
-                                                  &c->cnt[lcnt[(L + 2) % 3]], zero2, c->cnt64 + 2);
+            k_pull_list<0><<<lgrid, WG, 0, R.s>>>(lists[L % 2], &c->cnt[lcnt[L % 4]], vis, roff, radj, pid, rec,
+                                                  &c->cnt[C_LREC], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 4]],
+                                                  &c->cnt[lcnt[(L + 2) % 4]], zero2, c->cnt64 + 2, prev);
           else
-            k_pull_list<1><<<lgrid, WG, 0, R.s>>>(lists[L % 2], &c->cnt[lcnt[L % 3]], vis, roff, radj, pid, rec,
-                                                  &c->cnt[C_LREC], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 3]],
-                                                  &c->cnt[lcnt[(L + 2) % 3]], zero2, c->cnt64 + 2);
+            k_pull_list<1><<<lgrid, WG, 0, R.s>>>(lists[L % 2], &c->cnt[lcnt[L % 4]], vis, roff, radj, pid, rec,
+                                                  &c->cnt[C_LREC], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 4]],
+                                                  &c->cnt[lcnt[(L + 2) % 4]], zero2, c->cnt64 + 2, prev);
         }
       }
       KSPN_TRY(publish(c, R.s, nullptr, nullptr, 0, nullptr, &c->cnt[fc]));
@@ -2846,7 +2993,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       }
       if (batch) {
         if (fw_levels_pull) *fw_levels_pull += LIST_BATCH;
-        surv_n = c->h_cnt[lcnt[L % 3]];
+        surv_n = c->h_cnt[lcnt[L % 4]];
         list_mode = true;
       } else {
         surv_n = (uint32_t)left;
@@ -2885,7 +3032,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
         k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[nc], &c->cnt[C_LREC]);
       } else {
         Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words + 4.0 * nf);
-        k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->fnew, R.words, N, &c->cnt[C_CURSOR]);
+        k_bitmap_to_queue<<<b2q_grid(R.words), WG, 0, R.s>>>(c->fnew, R.words, N, &c->cnt[C_CURSOR]);
       }
     }
     std::swap(F, N);
@@ -2897,7 +3044,10 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
 
 // With publish = false the decided count stays on the device
 // (cnt[C_DECIDED]) for the caller's next publication.
-int fwbw(Run& R, uint32_t pivot, uint32_t* decided, bool publish = true) {
+// With `tail`, the vertices left live are listed in qa (count cnt[C_LIVE],
+// zero since the first trim: one such call per solve) and the tail solver's
+// counters are zeroed, for tail_spec.
+int fwbw(Run& R, uint32_t pivot, uint32_t* decided, bool publish = true, bool tail = false) {
   Ctx* c = R.c;
   // After the first trim of the plain single-pivot path, k_pivot_reduce
   // seeded both directions on the device (R.live_pending)
@@ -2916,17 +3066,20 @@ int fwbw(Run& R, uint32_t pivot, uint32_t* decided, bool publish = true) {
     k_seed<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->bwv, R.words, pivot, nullptr, c->qa, &c->cnt[C_Q0]);
   }
   KSPN_TRY(reach(R, 1, 1, nullptr, &lp, &lq, seeded));
-  // the pivot's SCC: clear it from live, then label it with its smallest member
-  uint32_t nparts = stream_grid(c, R.words);
+  // the pivot's SCC: clear it from live (listing the rest for the tail
+  // solver when it follows), then label it with its smallest member
+  const uint32_t nparts = finish_grid(R.words);
   {
-    Prof p(c, R.s, K_BITMAP, 16.0 * R.words);
-    k_fwbw_finish<<<nparts, WG, 0, R.s>>>(c->live, c->bwv, R.words, c->part);
-    k_pivot_reduce<<<1, WG, 0, R.s>>>(c->part, nparts, &c->cnt[C_MIN], &c->cnt[C_DECIDED]);
+    Prof p(c, R.s, K_BITMAP, 12.0 * R.words);
+    Zero3 z{};
+    if (tail) z = Zero3{{&c->cnt[C_RES_N], &c->cnt[C_ISO], &c->cnt[C_ROOM]}};
+    k_fwbw_finish<<<nparts, WG, 0, R.s>>>(c->live, c->bwv, R.words, c->part, tail ? c->qa : nullptr,
+                                          &c->cnt[C_LIVE], z);
   }
   {
     Prof p(c, R.s, K_LABEL, seeded ? R.n / 4.0 + 4.0 * R.n : R.n / 8.0);
-    k_label_bits<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->bwv, &c->cnt[C_MIN], R.labels,
-                                                      seeded ? c->live : nullptr);
+    k_label_bits<<<stream_grid(c, R.n / 8 + 1), WG, 0, R.s>>>(R.n, c->bwv, c->part, nparts, &c->cnt[C_MIN],
+                                                              &c->cnt[C_DECIDED], R.labels, seeded ? c->live : nullptr);
   }
   if (R.st) {
     R.st->fwbw_levels_push += lp;
@@ -3123,7 +3276,7 @@ int trim_normal(Run& R, uint32_t* decided) {
   k_zero2<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], &c->cnt[C_Q1]);
   {
     Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words + 4.0 * q);
-    k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->live, R.words, c->qa, &c->cnt[C_Q0]);
+    k_bitmap_to_queue<<<b2q_grid(R.words), WG, 0, R.s>>>(c->live, R.words, c->qa, &c->cnt[C_Q0]);
   }
   if (q) {
     Prof p(c, R.s, K_TRIM_SWEEP, 20.0 * q);
@@ -3352,7 +3505,7 @@ int tail_solve(Run& R, uint32_t* decided) {
   k_zero3<<<1, 64, 0, R.s>>>(&c->cnt[C_RES_N], &c->cnt[C_ISO], &c->cnt[C_ROOM]);
   {
     Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words + 4.0 * u);
-    k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->live, R.words, c->ulist, &c->cnt[C_RES_N]);
+    k_bitmap_to_queue<<<b2q_grid(R.words), WG, 0, R.s>>>(c->live, R.words, c->ulist, &c->cnt[C_RES_N]);
   }
   if (u < 4096) {
     // compact ids, degrees, both scans and the solver's flag: two launches
@@ -3419,17 +3572,12 @@ int tail_spec(Run& R, uint32_t limit, uint32_t* fwbw_decided, uint32_t* decided,
     KSPN_TRY(dalloc(&c->sub_adj, 1u << 16));
     c->cap_sub_m = 1u << 16;
   }
-  k_zero3<<<1, 64, 0, R.s>>>(&c->cnt[C_RES_N], &c->cnt[C_ISO], &c->cnt[C_ROOM]);
-  k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_AUX], 0u);
   {
-    // every live vertex (qa), then trim_1_normal keeps those with a live
+    // every live vertex (qa, listed by fwbw's k_fwbw_finish, which also
+    // zeroed the counters below), then trim_1_normal keeps those with a live
     // neighbour (ulist)
-    Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words);
-    k_bitmap_to_queue<<<stream_grid(c, R.words / 4 + 1), WG, 0, R.s>>>(c->live, R.words, c->qa, &c->cnt[C_AUX]);
-  }
-  {
     Prof p(c, R.s, K_TRIM_SWEEP, 20.0 * limit);
-    k_tail_split<<<(uint32_t)c->num_cu * 4, WG, 0, R.s>>>(c->qa, &c->cnt[C_AUX], g->fw_off, g->fw_adj, g->bw_off,
+    k_tail_split<<<(uint32_t)c->num_cu * 4, WG, 0, R.s>>>(c->qa, &c->cnt[C_LIVE], g->fw_off, g->fw_adj, g->bw_off,
                                                           g->bw_adj, c->live, R.labels, c->ulist, &c->cnt[C_RES_N],
                                                           &c->cnt[C_ISO]);
   }
@@ -3782,7 +3930,7 @@ int run_pipeline(Run& R) {
       // the last round: what is left may be small enough for the tail
       // solver, which then runs before the host has seen the FW-BW result
       const bool deferred = R.live_pending;
-      KSPN_TRY(fwbw(R, pivot, &d, false));
+      KSPN_TRY(fwbw(R, pivot, &d, false, true));
       if (deferred) {
         // the trim's counts arrived with the forward reach
         P.credit(first_name, R.trim_decided);
