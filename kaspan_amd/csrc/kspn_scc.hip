@@ -353,6 +353,63 @@ __device__ inline void expand_apply(uint32_t u, uint32_t w, const OpArgs& a, Blo
   bq_push(q, push, w);
 }
 
+// The reach claims of a thread's K edges u[r] -> x[r] (ok[r]: the edge
+// exists), with the loads of each step issued for all K together: candidate
+// and visited words, partitions, the claiming atomics, the claimed
+// vertices' degrees. Same claims as expand_apply on each edge in turn.
+template <int OP, int K>
+__device__ inline void reach_batch(const uint32_t (&u)[K], const uint32_t (&x)[K], const bool (&ok)[K],
+                                   const OpArgs& a, BlockQueue& q, uint64_t& nsum, uint32_t& nmax) {
+  const uint32_t* cand = OP == OP_BW ? a.fwv : a.live;
+  uint32_t* vis = OP == OP_FW ? a.fwv : a.bwv;
+  const bool part = OP == OP_PBW || (OP == OP_FW && a.part);
+  uint32_t cw[K], vw[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    cw[r] = ok[r] ? cand[x[r] >> 5] : 0u;
+    vw[r] = ok[r] ? vis[x[r] >> 5] : 0xFFFFFFFFu;
+  }
+  bool c[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    const uint32_t bit = 1u << (x[r] & 31);
+    c[r] = (cw[r] & bit) && !(vw[r] & bit);
+  }
+  if (part) {
+    uint32_t pu[K], px[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      pu[r] = c[r] ? a.part[u[r]] : 0u;
+      px[r] = c[r] ? a.part[x[r]] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r) c[r] = c[r] && pu[r] == px[r];
+  }
+  uint32_t old[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    const uint32_t bit = 1u << (x[r] & 31);
+    old[r] = c[r] ? atomicOr(&vis[x[r] >> 5], bit) : bit;
+  }
+  bool cl[K];
+  uint64_t d0[K], d1[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    cl[r] = !(old[r] & (1u << (x[r] & 31)));
+    d0[r] = cl[r] ? a.noff[x[r]] : 0;
+    d1[r] = cl[r] ? a.noff[x[r] + 1] : 0;
+  }
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    if (cl[r]) {
+      const uint64_t d = d1[r] - d0[r];
+      nsum += d;
+      nmax = max(nmax, (uint32_t)min(d, (uint64_t)0xFFFFFFFFu));
+    }
+    bq_push(q, cl[r], x[r]);
+  }
+}
+
 constexpr int EXP_TILE = WG * 8;  // edges per workgroup tile
 constexpr int EXP_WIN = 1024;     // frontier entries staged in LDS per tile
 
@@ -418,15 +475,38 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
       }
       __syncthreads();
       uint32_t lo = 0;
-      for (uint64_t e = e0 + threadIdx.x; e < e1; e += WG) {
-        uint32_t hi = w;
-        while (hi - lo > 1) {
-          uint32_t mid = (lo + hi) >> 1;
-          if (s_S[mid] <= e) lo = mid; else hi = mid;
+      if constexpr (OP == OP_FW || OP == OP_BW || OP == OP_PBW) {
+        // the thread's edges of the tile, their claims in flight together
+        constexpr int EPT = EXP_TILE / WG;
+        uint32_t uu[EPT], xx[EPT];
+        bool ok[EPT];
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) {
+          const uint64_t e = e0 + threadIdx.x + (uint64_t)r * WG;
+          ok[r] = e < e1;
+          uu[r] = xx[r] = 0;
+          if (ok[r]) {
+            uint32_t hi = w;
+            while (hi - lo > 1) {
+              uint32_t mid = (lo + hi) >> 1;
+              if (s_S[mid] <= e) lo = mid; else hi = mid;
+            }
+            uu[r] = s_v[lo];
+            xx[r] = adj[s_beg[lo] + (e - s_S[lo])];
+          }
         }
-        uint32_t u = s_v[lo];
-        uint32_t x = adj[s_beg[lo] + (e - s_S[lo])];
-        expand_apply<OP>(u, x, a, q, nsum, nmax);
+        reach_batch<OP, EPT>(uu, xx, ok, a, q, nsum, nmax);
+      } else {
+        for (uint64_t e = e0 + threadIdx.x; e < e1; e += WG) {
+          uint32_t hi = w;
+          while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (s_S[mid] <= e) lo = mid; else hi = mid;
+          }
+          uint32_t u = s_v[lo];
+          uint32_t x = adj[s_beg[lo] + (e - s_S[lo])];
+          expand_apply<OP>(u, x, a, q, nsum, nmax);
+        }
       }
     } else {
       // window larger than LDS (many zero-degree frontier entries): search the scan in global memory
