@@ -2038,7 +2038,9 @@ __global__ __launch_bounds__(WG) void k_tail_prep(const uint32_t* __restrict__ U
 // no live neighbour in either direction is decided (own id, live bit
 // cleared); the others go to `rest`, staged per workgroup in LDS. Removing
 // such a vertex changes no other vertex's test: it is nobody's live
-// neighbour. One wavefront per vertex, stopping at the first live neighbour.
+// neighbour. A group of 16 lanes per vertex (four per wavefront) reads 16
+// entries of its out-row and of its in-row per step, stopping at the first
+// live neighbour.
 __global__ __launch_bounds__(WG) void k_tail_split(const uint32_t* __restrict__ L, const uint32_t* lcount,
                                                    const uint64_t* __restrict__ fw_off,
                                                    const uint32_t* __restrict__ fw_adj,
@@ -2046,36 +2048,51 @@ __global__ __launch_bounds__(WG) void k_tail_split(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ bw_adj, uint32_t* live,
                                                    uint32_t* __restrict__ labels, uint32_t* __restrict__ rest,
                                                    uint32_t* rest_cnt, uint32_t* iso_cnt) {
-  __shared__ uint32_t s_q[2 * WG];
+  constexpr uint32_t GL = 16;              // lanes per vertex
+  constexpr uint32_t VPW = WAVE / GL;      // vertices per wavefront
+  constexpr uint32_t VPB = VPW * (WG / WAVE);  // vertices per workgroup and step
+  __shared__ uint32_t s_q[2 * VPB];
   __shared__ uint32_t s_n, s_b, s_iso;
-  BlockQueue q{s_q, &s_n, &s_b, 2 * WG};
+  BlockQueue q{s_q, &s_n, &s_b, 2 * VPB};
   if (threadIdx.x == 0) s_iso = 0;
   bq_init(q);
   const uint32_t lane = lane_id();
-  const uint32_t wpb = WG / WAVE;
+  const uint32_t gl = lane % GL;
+  const uint64_t gmask = ((1ull << GL) - 1) << (lane - gl);
   const uint32_t cnt = *lcount;
-  const uint32_t cnt_pad = (cnt + wpb - 1) / wpb * wpb;
-  for (uint32_t base = blockIdx.x * wpb; base < cnt_pad; base += gridDim.x * wpb) {
-    const uint32_t i = base + threadIdx.x / WAVE;
-    bool keep = false, alone = false;
-    uint32_t v = 0;
-    if (i < cnt) {
-      v = L[i];
-      bool any = false;
-      for (uint64_t b = fw_off[v], end = fw_off[v + 1]; b < end && !any; b += WAVE)
-        any = __any(b + lane < end && bm_get(live, fw_adj[b + lane]));
-      for (uint64_t b = bw_off[v], end = bw_off[v + 1]; b < end && !any; b += WAVE)
-        any = __any(b + lane < end && bm_get(live, bw_adj[b + lane]));
-      alone = lane == 0 && !any;
-      keep = lane == 0 && any;
+  const uint32_t cnt_pad = (cnt + VPB - 1) / VPB * VPB;
+  for (uint32_t base = blockIdx.x * VPB; base < cnt_pad; base += gridDim.x * VPB) {
+    const uint32_t i = base + threadIdx.x / GL;
+    const bool valid = i < cnt;
+    const uint32_t v = valid ? L[i] : 0u;
+    uint64_t fb = 0, fe = 0, bb = 0, be = 0;
+    if (valid) {
+      fb = fw_off[v];
+      fe = fw_off[v + 1];
+      bb = bw_off[v];
+      be = bw_off[v + 1];
     }
-    if (alone) {
+    bool found = false;
+    for (uint64_t c = gl;; c += GL) {
+      const bool act = valid && !found && (fb + c - gl < fe || bb + c - gl < be);
+      if (!__any(act)) break;
+      bool hit = false;
+      if (act) {
+        const bool fx = fb + c < fe, bx = bb + c < be;
+        const uint32_t x = fx ? fw_adj[fb + c] : 0u;
+        const uint32_t y = bx ? bw_adj[bb + c] : 0u;
+        hit = (fx && bm_get(live, x)) || (bx && bm_get(live, y));
+      }
+      found = found || (__ballot(hit) & gmask) != 0;
+    }
+    const bool keep = gl == 0 && valid && found;
+    if (gl == 0 && valid && !found) {
       labels[v] = v;
       atomicAnd(&live[v >> 5], ~(1u << (v & 31)));
       atomicAdd(&s_iso, 1u);  // one global add per workgroup, below
     }
     bq_push(q, keep, v);
-    bq_maybe_flush(q, rest, rest_cnt, wpb);  // at most one append per wavefront per iteration
+    bq_maybe_flush(q, rest, rest_cnt, VPB);  // at most one append per vertex of the step
   }
   bq_flush(q, rest, rest_cnt);
   if (threadIdx.x == 0 && s_iso) atomicAdd(iso_cnt, s_iso);
@@ -3657,7 +3674,7 @@ int tail_spec(Run& R, uint32_t limit, uint32_t* fwbw_decided, uint32_t* decided,
     // zeroed the counters below), then trim_1_normal keeps those with a live
     // neighbour (ulist)
     Prof p(c, R.s, K_TRIM_SWEEP, 20.0 * limit);
-    k_tail_split<<<(uint32_t)c->num_cu * 4, WG, 0, R.s>>>(c->qa, &c->cnt[C_LIVE], g->fw_off, g->fw_adj, g->bw_off,
+    k_tail_split<<<(uint32_t)c->num_cu * 8, WG, 0, R.s>>>(c->qa, &c->cnt[C_LIVE], g->fw_off, g->fw_adj, g->bw_off,
                                                           g->bw_adj, c->live, R.labels, c->ulist, &c->cnt[C_RES_N],
                                                           &c->cnt[C_ISO]);
   }
