@@ -157,17 +157,18 @@ __device__ inline uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
-// Appends the set bits of a workgroup's chunk of B2Q_CHUNK words (w[t][k]
-// is word base + t*4*WG + 4*threadIdx.x + k) to list, reserving the chunk's
+// Appends the set bits of a workgroup's chunk of STEPS * 4 * WG words
+// (w[t][k] is word base + t*4*WG + 4*threadIdx.x + k) to list, reserving the chunk's
 // range with one atomicAdd on *cursor; each thread writes its bits to a run
 // of its own. Every thread of the workgroup calls.
-__device__ inline void chunk_to_list(const uint32_t (&w)[B2Q_STEPS][4], uint64_t base, uint32_t* __restrict__ list,
+template <uint32_t STEPS>
+__device__ inline void chunk_to_list(const uint32_t (&w)[STEPS][4], uint64_t base, uint32_t* __restrict__ list,
                                      uint32_t* cursor, uint32_t* s_wave, uint32_t* s_base) {
   const uint32_t lane = lane_id();
   const uint32_t wave = threadIdx.x / WAVE;
   uint32_t cnt = 0;
 #pragma unroll
-  for (uint32_t t = 0; t < B2Q_STEPS; ++t) cnt += __popc(w[t][0]) + __popc(w[t][1]) + __popc(w[t][2]) + __popc(w[t][3]);
+  for (uint32_t t = 0; t < STEPS; ++t) cnt += __popc(w[t][0]) + __popc(w[t][1]) + __popc(w[t][2]) + __popc(w[t][3]);
   const uint32_t inc = wave_incl_scan(cnt);
   if (lane == WAVE - 1) s_wave[wave] = inc;
   __syncthreads();
@@ -183,7 +184,7 @@ __device__ inline void chunk_to_list(const uint32_t (&w)[B2Q_STEPS][4], uint64_t
   __syncthreads();
   uint32_t pos = *s_base + s_wave[wave] + inc - cnt;
 #pragma unroll
-  for (uint32_t t = 0; t < B2Q_STEPS; ++t) {
+  for (uint32_t t = 0; t < STEPS; ++t) {
     const uint64_t i0 = base + (uint64_t)t * WG * 4 + (uint64_t)threadIdx.x * 4;
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) {
@@ -1395,13 +1396,15 @@ __global__ void k_reach_start(const uint32_t* vdev, uint32_t* queue, uint32_t* q
 }
 
 // After FW-BW: scc = bwv (backward-reached within forward-reached), live &=
-// ~scc. Each workgroup takes chunks of B2Q_CHUNK words (launch
+// ~scc. Each workgroup takes chunks of FIN_CHUNK words (launch
 // finish_grid(words) workgroups, at most FINISH_PARTS) and leaves the
 // partials of |scc| (part[2b]) and of its smallest vertex (part[2b + 1])
 // that k_label_bits reduces. With `list`, the vertices left live are
 // appended to it as k_bitmap_to_queue appends (*cursor zero before the
 // launch). Block 0 zeroes the given counters `zero` (of the kernels after it).
 constexpr uint32_t FINISH_PARTS = WG;
+constexpr uint32_t FIN_STEPS = 2;  // 16-byte words per thread and chunk
+constexpr uint64_t FIN_CHUNK = (uint64_t)WG * 4 * FIN_STEPS;
 struct Zero3 {
   uint32_t* p[3];
 };
@@ -1413,10 +1416,10 @@ __global__ __launch_bounds__(WG) void k_fwbw_finish(uint32_t* live, const uint32
   __shared__ uint32_t s_red[WG / WAVE];
   if (blockIdx.x == 0 && threadIdx.x < 3 && zero.p[threadIdx.x]) *zero.p[threadIdx.x] = 0;
   uint32_t count = 0, mn = 0xFFFFFFFFu;
-  for (uint64_t base = (uint64_t)blockIdx.x * B2Q_CHUNK; base < words; base += (uint64_t)gridDim.x * B2Q_CHUNK) {
-    uint32_t w[B2Q_STEPS][4];  // live words after the SCC left
+  for (uint64_t base = (uint64_t)blockIdx.x * FIN_CHUNK; base < words; base += (uint64_t)gridDim.x * FIN_CHUNK) {
+    uint32_t w[FIN_STEPS][4];  // live words after the SCC left
 #pragma unroll
-    for (uint32_t t = 0; t < B2Q_STEPS; ++t) {
+    for (uint32_t t = 0; t < FIN_STEPS; ++t) {
       const uint64_t i0 = base + (uint64_t)t * WG * 4 + (uint64_t)threadIdx.x * 4;
       uint32_t sc[4];
       if (i0 + 4 <= words) {  // i0 is a multiple of 4 words: 16-byte accesses
@@ -2645,7 +2648,7 @@ inline uint32_t expand_grid(Ctx* c, uint64_t total) {
 
 // workgroups of a k_fwbw_finish launch over `words` words (its partials)
 inline uint32_t finish_grid(uint64_t words) {
-  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((words + B2Q_CHUNK - 1) / B2Q_CHUNK, FINISH_PARTS));
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((words + FIN_CHUNK - 1) / FIN_CHUNK, FINISH_PARTS));
 }
 
 // workgroups of a k_bitmap_to_queue launch over `words` words: one chunk each
