@@ -209,6 +209,8 @@ __device__ inline uint32_t rotl32(uint32_t x, uint32_t r) { r &= 31; return r ? 
 
 // ---------------------------------------------------------------- counters
 // Device-side counters, read back through a pinned mirror.
+constexpr uint32_t MAP_BANK = 64;  // words per bank of the host-mapped publication page (sequence word at 63)
+
 enum Counter : int {
   C_Q0 = 0,   // size of queue written by a launch (ping)
   C_Q1,       // pong
@@ -229,6 +231,7 @@ enum Counter : int {
   C_ISO,     // live vertices without a live neighbour, counted by the tail preparation
   C_LREC,    // claims of the last pull list level of a batch
   C_LIVE,    // vertices left live after the last FW-BW, listed by k_fwbw_finish for the tail solver
+  C_SPEC,    // k_spec_gate: a speculatively launched pull sweep is to run
   C_COUNT_ = 24
 };
 
@@ -268,7 +271,7 @@ struct Ctx {
   uint32_t* cnt = nullptr;    // device counters
   unsigned long long* cnt64 = nullptr;  // device 64-bit counters (edges scanned)
   uint32_t* h_cnt = nullptr;  // host copy of the published counters
-  volatile uint32_t* h_map = nullptr;  // pinned, device-mapped: counters published by k_publish
+  volatile uint32_t* h_map = nullptr;  // pinned, device-mapped: counters published by k_publish (two banks of MAP_BANK)
   uint32_t* d_map = nullptr;  // device address of h_map
   uint64_t* h_u64 = nullptr;  // pinned scratch for 64-bit reads
   void* cub_tmp = nullptr;

@@ -58,12 +58,14 @@ __device__ inline void publish_seq(uint32_t* map, uint32_t seq) {
 // Optionally first reduces per-workgroup partials (claimed count -> *pcount,
 // edges scanned -> cnt64[0], next-level degree sum -> cnt64[1]); after the
 // copy it can zero *reset and cnt64[0..2] for the next level, so a level
-// needs no separate reset or reduction launch. One workgroup of WG threads.
+// needs no separate reset or reduction launch. With `gate` (k_spec_gate),
+// it does nothing unless *gate is set. One workgroup of WG threads.
 __global__ __launch_bounds__(WG) void k_publish(uint32_t* cnt, unsigned long long* cnt64, const uint64_t* extra,
                                                 uint32_t* map, uint32_t seq, const uint64_t* __restrict__ part,
                                                 uint32_t nparts, uint32_t* pcount, uint32_t* reset,
-                                                uint32_t* reset2) {
+                                                uint32_t* reset2, const uint32_t* gate) {
   __shared__ uint64_t s_red[WG / WAVE];
+  if (gate && *gate == 0) return;  // a speculative launch that lost: nothing is published
   if (part) {
     uint64_t a = 0, b = 0, d = 0;
     for (uint32_t i = threadIdx.x; i < nparts; i += WG) {
@@ -113,6 +115,24 @@ __global__ __launch_bounds__(WG) void k_publish(uint32_t* cnt, unsigned long lon
       cnt64[1] = 0;
       cnt64[2] = 0;
     }
+  }
+}
+
+// The host's direction rule for the level after a push level, evaluated on
+// the device so that a pull sweep can be launched before the host has read
+// the push level's counters: pull when (degree sum of the claimed vertices)
+// * alpha > m_rest - (edges the push level expanded) and more than 64
+// vertices were claimed, with the host's operands and double arithmetic.
+// *gate = 1 for pull. Otherwise the four list counters `lcnt` are zeroed
+// (a sweep would have zeroed them), so the list levels launched behind the
+// gated sweep find no candidates.
+__global__ void k_spec_gate(uint32_t* gate, const uint32_t* claimed, const unsigned long long* nsum,
+                            const uint64_t* expanded, double m_rest, double alpha, uint32_t* lcnt) {
+  if (threadIdx.x == 0) {
+    const double rest = m_rest - (double)*expanded;
+    const bool pull = (double)*nsum * alpha > rest && *claimed > 64;
+    *gate = pull ? 1u : 0u;
+    if (!pull) lcnt[0] = lcnt[1] = lcnt[2] = lcnt[3] = 0;
   }
 }
 
@@ -204,14 +224,16 @@ __device__ inline void chunk_to_list(const uint32_t (&w)[STEPS][4], uint64_t bas
 // their set bits with chunk_to_list: few, large chunks keep the atomics on
 // *cursor (zero before the launch) few. For lists whose order does not
 // matter. With `guard`, nothing is written when *guard exceeds guard_max
-// (read after the words, so the two loads overlap). Launch b2q_grid(words)
-// workgroups.
+// (read after the words, so the two loads overlap); with `gate`, nothing
+// unless *gate is set (k_spec_gate). Launch b2q_grid(words) workgroups.
 __global__ __launch_bounds__(WG) void k_bitmap_to_queue(const uint32_t* __restrict__ bm, uint64_t words,
                                                         uint32_t* __restrict__ list, uint32_t* cursor,
                                                         const unsigned long long* guard = nullptr,
-                                                        unsigned long long guard_max = 0) {
+                                                        unsigned long long guard_max = 0,
+                                                        const uint32_t* gate = nullptr) {
   __shared__ uint32_t s_wave[WG / WAVE];
   __shared__ uint32_t s_base;
+  if (gate && *gate == 0) return;  // a speculative launch that lost
   bool first = true;
   for (uint64_t base = (uint64_t)blockIdx.x * B2Q_CHUNK; base < words; base += (uint64_t)gridDim.x * B2Q_CHUNK) {
     uint32_t w[B2Q_STEPS][4];
@@ -1086,6 +1108,7 @@ __global__ __launch_bounds__(WG) void k_pivot_reduce(const uint64_t* part, uint3
 // candidates left unreached, whose bits it receives (written like fnew) so
 // that a short remainder can be compacted into a list for k_pull_list.
 // Block 0 zeroes the four list counters `lcnt`, which the sweep does not use.
+// With `gate`, the sweep does nothing unless *gate is set.
 constexpr int PULL_ILP = 2;  // vertices in flight per thread (2 beat 1 and 4 on R-MAT 24 and the 4096-graph batch)
 constexpr uint32_t PULL_FLUSH = 4;                     // k_pull_list: iterations between checks of its LDS queues
 constexpr uint32_t PULL_QCAP = PULL_FLUSH * WG * PULL_ILP;  // entries of each LDS queue (8 KiB)
@@ -1095,7 +1118,9 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
                                              const uint64_t* __restrict__ roff, const uint32_t* __restrict__ radj,
                                              const uint64_t* __restrict__ noff, uint32_t* __restrict__ fnew,
                                              uint64_t* __restrict__ part, const uint32_t* __restrict__ pid,
-                                             uint32_t* cursor, uint32_t* __restrict__ sbm, uint32_t* lcnt) {
+                                             uint32_t* cursor, uint32_t* __restrict__ sbm, uint32_t* lcnt,
+                                             const uint32_t* gate) {
+  if (gate && *gate == 0) return;  // a speculative sweep that lost (k_spec_gate)
   // A candidate costs a chain of dependent loads (bitmap word, row offsets,
   // in-neighbour, its visited bit, its partition). Each thread carries
   // PULL_ILP vertices (one per 256-vertex tile of its workgroup's span)
@@ -1303,7 +1328,8 @@ __global__ __launch_bounds__(WG) void k_pull_list(const uint32_t* __restrict__ l
 
 // sums of the three per-workgroup partials: [0] -> *count, [1] -> cnt64[0], [2] -> cnt64[1]
 __global__ __launch_bounds__(WG) void k_reduce3(const uint64_t* __restrict__ part, uint32_t nparts, uint32_t* count,
-                                                unsigned long long* cnt64) {
+                                                unsigned long long* cnt64, const uint32_t* gate = nullptr) {
+  if (gate && *gate == 0) return;  // a speculative launch that lost
   __shared__ uint64_t s_red[WG / WAVE];
   uint64_t a = 0, b = 0, d = 0;
   // RED_U partials per thread and step, loaded together
@@ -2722,10 +2748,11 @@ int ctx_reserve(Ctx* c, uint32_t n, uint64_t m) {
 // mapped host buffer. Polling the sequence word returns about 5 us sooner
 // than a stream synchronisation; after 20 ms of polling the stream is
 // synchronised instead, which also reports a failed launch.
-int wait_published(Ctx* c, hipStream_t s, uint32_t seq) {
+int wait_published(Ctx* c, hipStream_t s, uint32_t seq, uint32_t bank = 0) {
+  const volatile uint32_t* m = c->h_map + MAP_BANK * bank;
   const auto t0 = std::chrono::steady_clock::now();
   for (uint32_t spins = 0;; ++spins) {
-    if (c->h_map[63] == seq) {
+    if (m[63] == seq) {
       std::atomic_thread_fence(std::memory_order_acquire);
       return KSPN_OK;
     }
@@ -2733,7 +2760,7 @@ int wait_published(Ctx* c, hipStream_t s, uint32_t seq) {
     __builtin_ia32_pause();
   }
   KSPN_HIP(hipStreamSynchronize(s));
-  if (c->h_map[63] != seq) {
+  if (m[63] != seq) {
     note_error("k_publish", "counters not published after the stream completed", __FILE__, __LINE__);
     return KSPN_E_HIP;
   }
@@ -2745,18 +2772,33 @@ int wait_published(Ctx* c, hipStream_t s, uint32_t seq) {
 // and copy them into h_cnt / h_u64[0] (extra) / h_u64[1] (cnt64[0]).
 // With part: reduce nparts per-workgroup partials first (into *pcount and
 // cnt64[0..1]); with reset: zero *reset and cnt64[0..2] afterwards.
+// The two halves of publish: the launch (into map bank `bank`; with `gate`
+// the kernel publishes only if *gate is set, and the caller takes the
+// sequence number back when it learns that it did not) and the wait + copy.
+uint32_t publish_launch(Ctx* c, hipStream_t s, const uint64_t* extra, const uint64_t* part, uint32_t nparts,
+                        uint32_t* pcount, uint32_t* reset, uint32_t* reset2, uint32_t bank = 0,
+                        const uint32_t* gate = nullptr) {
+  const uint32_t seq = ++c->pub_seq;
+  k_publish<<<1, WG, 0, s>>>(c->cnt, c->cnt64, extra, c->d_map + MAP_BANK * bank, seq, part, nparts, pcount, reset,
+                             reset2, gate);
+  return seq;
+}
+
+int publish_collect(Ctx* c, hipStream_t s, uint32_t seq, uint32_t bank = 0) {
+  KSPN_HIP(hipGetLastError());
+  KSPN_TRY(wait_published(c, s, seq, bank));
+  const volatile uint32_t* m = c->h_map + MAP_BANK * bank;
+  for (int i = 0; i < C_COUNT_; ++i) c->h_cnt[i] = m[i];
+  c->h_u64[0] = (uint64_t)m[48] | ((uint64_t)m[49] << 32);
+  c->h_u64[1] = (uint64_t)m[50] | ((uint64_t)m[51] << 32);
+  c->h_u64[2] = (uint64_t)m[52] | ((uint64_t)m[53] << 32);
+  c->h_u64[3] = (uint64_t)m[54] | ((uint64_t)m[55] << 32);
+  return KSPN_OK;
+}
+
 int publish(Ctx* c, hipStream_t s, const uint64_t* extra, const uint64_t* part = nullptr, uint32_t nparts = 0,
             uint32_t* pcount = nullptr, uint32_t* reset = nullptr, uint32_t* reset2 = nullptr) {
-  const uint32_t seq = ++c->pub_seq;
-  k_publish<<<1, WG, 0, s>>>(c->cnt, c->cnt64, extra, c->d_map, seq, part, nparts, pcount, reset, reset2);
-  KSPN_HIP(hipGetLastError());
-  KSPN_TRY(wait_published(c, s, seq));
-  for (int i = 0; i < C_COUNT_; ++i) c->h_cnt[i] = c->h_map[i];
-  c->h_u64[0] = (uint64_t)c->h_map[48] | ((uint64_t)c->h_map[49] << 32);
-  c->h_u64[1] = (uint64_t)c->h_map[50] | ((uint64_t)c->h_map[51] << 32);
-  c->h_u64[2] = (uint64_t)c->h_map[52] | ((uint64_t)c->h_map[53] << 32);
-  c->h_u64[3] = (uint64_t)c->h_map[54] | ((uint64_t)c->h_map[55] << 32);
-  return KSPN_OK;
+  return publish_collect(c, s, publish_launch(c, s, extra, part, nparts, pcount, reset, reset2));
 }
 
 int read_counters(Ctx* c, hipStream_t s, int count) {
@@ -2976,13 +3018,74 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   bool list_mode = false;   // the next pull level visits a list
   bool list_level = false;  // the level just run visited lists
   uint32_t L = 0;           // list levels since the last sweep
+  // A sweep over all vertices: claims -> cnt[claim_slot], edges ->
+  // cnt64[0], candidates left -> cnt64[1], and their list (lists[0]) when
+  // there are at most list_max of them, so a batch after it does nothing
+  // otherwise. With `gate`, every launch is conditional on k_spec_gate.
+  auto launch_sweep = [&](int claim_slot, const uint32_t* gate, int* prof) {
+    const uint32_t grid = stream_grid(c, R.n);
+    {
+      Prof p(c, R.s, dir == 0 ? K_PULL_FW : K_PULL_BW, (double)R.n / 4.0 + 16.0 * R.live_n);
+      if (prof) *prof = p.idx;
+      // no out-degree sum of the claimed vertices: it would read a third
+      // offset array at scattered rows, and the first push level after the
+      // pull levels sums its frontier's degrees itself
+      if (dir == 0)
+        k_pull<0><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
+                                        &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0], gate);
+      else
+        k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
+                                        &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0], gate);
+    }
+    k_reduce3<<<1, WG, 0, R.s>>>(c->part, grid, &c->cnt[claim_slot], c->cnt64, gate);
+    Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words);
+    k_bitmap_to_queue<<<b2q_grid(R.words), WG, 0, R.s>>>(c->sbm, R.words, lists[0], &c->cnt[lcnt[0]], c->cnt64 + 1,
+                                                         list_max, gate);
+  };
+  // LIST_BATCH list levels from list level L on; the last records its
+  // claims in `rec` (count cnt[C_LREC])
+  auto launch_batch = [&](uint32_t* rec, uint32_t n_in, int* prof) {
+    const uint32_t lgrid = std::min<uint32_t>(stream_grid(c, n_in / PULL_ILP + 1), (uint32_t)c->num_cu * 4);
+    for (uint32_t j = 0; j < LIST_BATCH; ++j, ++L) {
+      // the first level zeroes the claim count the last one records
+      uint32_t* q = j + 1 == LIST_BATCH ? rec : nullptr;
+      uint32_t* zero2 = j == 0 ? &c->cnt[C_LREC] : nullptr;
+      // the previous level's candidate count (its slot is untouched until the next level)
+      const uint32_t* prev = j == 0 ? nullptr : &c->cnt[lcnt[(L + 3) % 4]];
+      Prof p(c, R.s, K_PULL_LIST, 0.0);
+      if (prof) *prof = p.idx;
+      if (dir == 0)
+        k_pull_list<0><<<lgrid, WG, 0, R.s>>>(lists[L % 2], &c->cnt[lcnt[L % 4]], vis, roff, radj, pid, q,
+                                              &c->cnt[C_LREC], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 4]],
+                                              &c->cnt[lcnt[(L + 2) % 4]], zero2, c->cnt64 + 2, prev);
+      else
+        k_pull_list<1><<<lgrid, WG, 0, R.s>>>(lists[L % 2], &c->cnt[lcnt[L % 4]], vis, roff, radj, pid, q,
+                                              &c->cnt[C_LREC], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 4]],
+                                              &c->cnt[lcnt[(L + 2) % 4]], zero2, c->cnt64 + 2, prev);
+    }
+  };
+  // Speculation: on a large graph, the first push level of a direction
+  // (from the pivot) is followed at once by a sweep and its list batch,
+  // conditional on the device evaluating the host's direction rule
+  // (k_spec_gate), and by their publication into map bank 1. The host reads
+  // the push level's publication meanwhile; if its rule then chooses pull,
+  // the sweep is already running, else the gated launches did nothing and
+  // the host takes the sequence number of their publication back.
+  const bool speculate = !pid && !c->profile && R.n >= (1u << 20);
+  uint32_t spec_seq = 0;  // the speculative publication's sequence number, while unresolved
   // each level's publication zeroes the counters of the level after it
   if (!seeded) k_level_reset<<<1, 64, 0, R.s>>>(&c->cnt[nc], c->cnt64);
   bool scanned = seeded;  // the first level's degree scan is done
+  bool first_level = true;
   while (f > 0) {
     // direction: Beamer's rule on the frontier's edge count, which the
     // previous level summed on the device for the vertices it claimed
     if (!pull && known && (double)edges * alpha > m_rest && f > 64) pull = true;
+    const bool spec_taken = spec_seq != 0 && pull;
+    if (spec_seq != 0 && !pull) {
+      c->pub_seq = spec_seq - 1;  // the gated publication did not happen
+      spec_seq = 0;
+    }
     if (!pull && known && coop && edges <= R.coop_edges && maxd <= R.coop_deg) {
       // the following push levels in one persistent launch
       CoopOut co{};
@@ -3024,63 +3127,27 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
         k_expand<OP_BW><<<grid, WG, 0, R.s>>>(F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
       if (fw_levels_push) (*fw_levels_push)++;
     } else {
-      int kern = dir == 0 ? K_PULL_FW : K_PULL_BW;
       bool batch = list_mode;  // a batch of list levels follows (after a sweep: when the remainder is short)
       bool swept = false;
       int sweep_prof = -1, list_prof = -1;  // their profile entries, completed after the publication
       if (!list_mode) {
-        const uint32_t grid = stream_grid(c, R.n);
-        {
-          Prof p(c, R.s, kern, (double)R.n / 4.0 + 16.0 * R.live_n);
-          sweep_prof = p.idx;
-          // no out-degree sum of the claimed vertices: it would read a third
-          // offset array at scattered rows, and the first push level after the
-          // pull levels sums its frontier's degrees itself
-          if (dir == 0)
-            k_pull<0><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
-                                            &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0]);
-          else
-            k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
-                                            &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0]);
-        }
+        // the sweep's claims go to cnt[nc]
+        if (!spec_taken) launch_sweep(nc, nullptr, &sweep_prof);
         if (fw_levels_pull) (*fw_levels_pull)++;
-        // claims -> cnt[nc], edges -> cnt64[0], candidates left -> cnt64[1];
-        // the list of those is built only when it is short (list_max), so
-        // the batch below does nothing otherwise
-        k_reduce3<<<1, WG, 0, R.s>>>(c->part, grid, &c->cnt[nc], c->cnt64);
-        {
-          Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words);
-          k_bitmap_to_queue<<<b2q_grid(R.words), WG, 0, R.s>>>(c->sbm, R.words, lists[0],
-                                                                             &c->cnt[lcnt[0]], c->cnt64 + 1,
-                                                                             list_max);
-        }
         L = 0;
         batch = true;
         swept = true;
         surv_n = list_max;  // at most
       }
       const uint32_t list_in = surv_n;
-      if (batch) {
-        const uint32_t lgrid = std::min<uint32_t>(stream_grid(c, surv_n / PULL_ILP + 1), (uint32_t)c->num_cu * 4);
-        for (uint32_t j = 0; j < LIST_BATCH; ++j, ++L) {
-          // the first level zeroes the claim count the last one records
-          uint32_t* rec = j + 1 == LIST_BATCH ? N : nullptr;
-          uint32_t* zero2 = j == 0 ? &c->cnt[C_LREC] : nullptr;
-          // the previous level's candidate count (its slot is untouched until the next level)
-          const uint32_t* prev = j == 0 ? nullptr : &c->cnt[lcnt[(L + 3) % 4]];
-          Prof p(c, R.s, K_PULL_LIST, 0.0);
-          list_prof = p.idx;
-          if (dir == 0)
-            k_pull_list<0><<<lgrid, WG, 0, R.s>>>(lists[L % 2], &c->cnt[lcnt[L % 4]], vis, roff, radj, pid, rec,
-                                                  &c->cnt[C_LREC], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 4]],
-                                                  &c->cnt[lcnt[(L + 2) % 4]], zero2, c->cnt64 + 2, prev);
-          else
-            k_pull_list<1><<<lgrid, WG, 0, R.s>>>(lists[L % 2], &c->cnt[lcnt[L % 4]], vis, roff, radj, pid, rec,
-                                                  &c->cnt[C_LREC], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 4]],
-                                                  &c->cnt[lcnt[(L + 2) % 4]], zero2, c->cnt64 + 2, prev);
-        }
+      if (spec_taken) {
+        L += LIST_BATCH;  // launched with the sweep
+        KSPN_TRY(publish_collect(c, R.s, spec_seq, 1));
+        spec_seq = 0;
+      } else {
+        if (batch) launch_batch(N, surv_n, &list_prof);
+        KSPN_TRY(publish(c, R.s, nullptr, nullptr, 0, nullptr, &c->cnt[fc]));
       }
-      KSPN_TRY(publish(c, R.s, nullptr, nullptr, 0, nullptr, &c->cnt[fc]));
       // a sweep's list batch ran iff its remainder was short
       const uint64_t left = c->h_u64[2];
       if (swept && left > list_max) batch = false;
@@ -3099,7 +3166,25 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
         surv_n = (uint32_t)left;
       }
     }
-    if (!pull) KSPN_TRY(publish(c, R.s, c->scan + f, nullptr, 0, nullptr, &c->cnt[fc]));
+    if (!pull) {
+      // the device evaluates the rule before the publication resets the
+      // level's counters; the sweep and batch run behind it, with the
+      // frontier buffers and counter slots as they will be after the swap
+      // below (claims to cnt[fc], claimed list to F, reset of cnt[nc])
+      const bool spec = speculate && first_level && !list_mode;
+      if (spec)
+        k_spec_gate<<<1, 64, 0, R.s>>>(&c->cnt[C_SPEC], &c->cnt[nc], c->cnt64 + 1, c->scan + f, m_rest, alpha,
+                                       &c->cnt[C_SURV0]);
+      const uint32_t seq0 = publish_launch(c, R.s, c->scan + f, nullptr, 0, nullptr, &c->cnt[fc], nullptr);
+      if (spec) {
+        launch_sweep(fc, &c->cnt[C_SPEC], nullptr);
+        L = 0;
+        launch_batch(F, list_max, nullptr);
+        spec_seq = publish_launch(c, R.s, nullptr, nullptr, 0, nullptr, &c->cnt[nc], nullptr, 1, &c->cnt[C_SPEC]);
+      }
+      KSPN_TRY(publish_collect(c, R.s, seq0));
+    }
+    first_level = false;
     if (!pull) {
       edges = c->h_u64[0];  // exact total of the level just expanded
       m_rest -= (double)edges;
@@ -3139,6 +3224,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
     std::swap(fc, nc);
     f = nf;
   }
+  if (spec_seq != 0) c->pub_seq = spec_seq - 1;  // nothing claimed: the gated publication did not happen
   return KSPN_OK;
 }
 
@@ -4193,7 +4279,7 @@ int kspn_ctx_create(kspn_ctx** out, int device) {
       hipMalloc((void**)&c->cnt64, sizeof(unsigned long long) * 4) != hipSuccess ||
       hipHostMalloc((void**)&c->h_cnt, sizeof(uint32_t) * C_COUNT_, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->h_u64, sizeof(uint64_t) * 4, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(&hm, sizeof(uint32_t) * 64, hipHostMallocMapped) != hipSuccess) {
+      hipHostMalloc(&hm, sizeof(uint32_t) * MAP_BANK * 2, hipHostMallocMapped) != hipSuccess) {
     kspn_ctx_destroy(k);
     return KSPN_E_OOM;
   }
@@ -4203,7 +4289,7 @@ int kspn_ctx_create(kspn_ctx** out, int device) {
     return KSPN_E_HIP;
   }
   std::memset(c->h_cnt, 0, sizeof(uint32_t) * C_COUNT_);
-  std::memset(hm, 0, sizeof(uint32_t) * 64);
+  std::memset(hm, 0, sizeof(uint32_t) * MAP_BANK * 2);
   std::memset(c->h_u64, 0, sizeof(uint64_t) * 4);
   c->kt.resize(K_COUNT);
   for (int i = 0; i < K_COUNT; ++i) c->kt[i].name = kKernNames[i];

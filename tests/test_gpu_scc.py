@@ -257,6 +257,30 @@ def test_profile_stats(ka, O):
     assert st.ms_total > 0
 
 
+@pytest.mark.parametrize("kind", ["rmat20", "gnm20"])
+def test_speculative_sweep(ka, O, kind):
+    """On graphs of at least 2^20 vertices the sweep after a direction's
+    first push level is launched before the host has decided (R-MAT: the
+    device's rule chooses pull; sparse G(n,m): it does not, and the gated
+    launches and their publication must leave no trace). Labels equal
+    Tarjan's, and the level counts equal a profiled solve's, which never
+    speculates; three solves in a row on one context."""
+    if kind == "rmat20":
+        g = ka.generate("rmat", 91, scale=20, draws=16 << 20)
+    else:
+        g = ka.generate("gnm", 92, n=1 << 20, draws=3 << 20)
+    tar = O.tarjan(host_graph(O, g))
+    ref, st_ref = ka.scc(g, profile=True)
+    assert np.array_equal(ref.cpu().numpy().view(np.uint32), tar)
+    for _ in range(3):
+        lab, st = ka.scc(g)
+        assert np.array_equal(lab.cpu().numpy().view(np.uint32), tar)
+        assert (st.fwbw_levels_push, st.fwbw_levels_pull, st.fwbw_levels_coop) == \
+            (st_ref.fwbw_levels_push, st_ref.fwbw_levels_pull, st_ref.fwbw_levels_coop)
+    if kind == "rmat20":
+        assert st.fwbw_levels_pull > 0
+
+
 def test_c2_rmat24_full(ka, O):
     """config 2 at full size: R-MAT scale 24, edge factor 16, permuted ids;
     labels equal the oracle's Tarjan labels bit for bit."""
