@@ -163,7 +163,7 @@ __global__ void k_seed(uint32_t* bm, uint64_t words, uint32_t v, const uint32_t*
   }
 }
 
-constexpr uint32_t B2Q_STEPS = 8;
+constexpr uint32_t B2Q_STEPS = 4;  // 4 beat 8 and 2 on R-MAT 24's survivor lists
 constexpr uint64_t B2Q_CHUNK = (uint64_t)WG * 4 * B2Q_STEPS;
 
 // inclusive prefix sum over the wavefront
@@ -1508,8 +1508,43 @@ __global__ __launch_bounds__(WG) void k_label_bits(uint32_t n, const uint32_t* _
     }
     __syncthreads();
   }
-  constexpr int LU = 8;  // vertices per thread and step (one per 256-vertex tile), word loads issued together
   const uint32_t m = s_m;
+  if (live && ((uintptr_t)labels & 15) == 0) {
+    // dense pass into 16-byte aligned labels: four consecutive vertices per
+    // thread and store (one bitmap word holds them), four stores in flight
+    constexpr int LQ = 4;
+    const uint64_t quads = (uint64_t)n / 4;
+    uint4* lab4 = reinterpret_cast<uint4*>(labels);
+    for (uint64_t base = blockIdx.x * (uint64_t)WG * LQ; base < quads; base += (uint64_t)gridDim.x * WG * LQ) {
+      uint32_t bw[LQ], lw[LQ];
+#pragma unroll
+      for (int k = 0; k < LQ; ++k) {
+        const uint64_t qd = base + (uint64_t)k * WG + threadIdx.x;
+        bw[k] = qd < quads ? bm[qd >> 3] : 0u;
+        lw[k] = qd < quads ? live[qd >> 3] : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < LQ; ++k) {
+        const uint64_t qd = base + (uint64_t)k * WG + threadIdx.x;
+        if (qd >= quads) break;
+        const uint32_t sh = (uint32_t)(qd & 7) * 4;
+        uint32_t o[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+          const uint32_t v = (uint32_t)(qd * 4 + j);
+          o[j] = ((bw[k] >> (sh + j)) & 1u) ? m : (((lw[k] >> (sh + j)) & 1u) ? UNDEC : v);
+        }
+        lab4[qd] = make_uint4(o[0], o[1], o[2], o[3]);
+      }
+    }
+    // the last n % 4 vertices
+    if (blockIdx.x == 0 && threadIdx.x < (n & 3u)) {
+      const uint32_t v = (uint32_t)(quads * 4 + threadIdx.x);
+      labels[v] = bm_get(bm, v) ? m : (bm_get(live, v) ? UNDEC : v);
+    }
+    return;
+  }
+  constexpr int LU = 8;  // vertices per thread and step (one per 256-vertex tile), word loads issued together
   for (uint64_t base = blockIdx.x * (uint64_t)WG * LU; base < n; base += (uint64_t)gridDim.x * WG * LU) {
     uint32_t bw[LU], lw[LU];
 #pragma unroll
