@@ -3058,7 +3058,11 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   // there are at most list_max of them, so a batch after it does nothing
   // otherwise. With `gate`, every launch is conditional on k_spec_gate.
   auto launch_sweep = [&](int claim_slot, const uint32_t* gate, int* prof) {
-    const uint32_t grid = stream_grid(c, R.n);
+    // up to 2^25 vertices one resident round of workgroups (2048 beat 1024,
+    // 4096 and 16384 on R-MAT 24), above it two (the 4096-graph batch, n =
+    // 6e8, lost 2.4 % with one)
+    const uint32_t grid =
+        std::min<uint32_t>(stream_grid(c, R.n), (uint32_t)c->num_cu * (R.n <= (1u << 25) ? 8 : 16));
     {
       Prof p(c, R.s, dir == 0 ? K_PULL_FW : K_PULL_BW, (double)R.n / 4.0 + 16.0 * R.live_n);
       if (prof) *prof = p.idx;
