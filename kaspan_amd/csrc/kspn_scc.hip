@@ -3042,10 +3042,11 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   // compacted into a list, and list levels run in batches of LIST_BATCH
   // launches without a host round trip, each passing the candidates it
   // leaves to the next (lists alternate between qc and ulist, both free
-  // while reaching; their counters rotate over three slots, which the sweep
-  // zeroes). The last level of a batch also records what it claimed, the
-  // frontier if push levels follow; a batch whose last level claims nothing
-  // has reached everything.
+  // while reaching; their counters rotate over four slots, which the sweep
+  // zeroes, so a level can compare its count with its predecessor's). The
+  // last level of a batch also records what it claimed, the frontier if
+  // push levels follow; a batch whose last level claims nothing has reached
+  // everything.
   constexpr uint32_t LIST_BATCH = 4;
   uint32_t* lists[2] = {c->qc, c->ulist};
   const int lcnt[4] = {C_SURV0, C_SURV1, C_SURV2, C_SURV3};
