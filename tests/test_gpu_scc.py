@@ -257,18 +257,31 @@ def test_profile_stats(ka, O):
     assert st.ms_total > 0
 
 
-@pytest.mark.parametrize("kind", ["rmat20", "gnm20"])
+@pytest.mark.parametrize("kind", ["rmat20", "gnm20", "rmat20_cycle"])
 def test_speculative_sweep(ka, O, kind):
     """On graphs of at least 2^20 vertices the sweep after a direction's
     first push level is launched before the host has decided (R-MAT: the
     device's rule chooses pull; sparse G(n,m): it does not, and the gated
-    launches and their publication must leave no trace). Labels equal
-    Tarjan's, and the level counts equal a profiled solve's, which never
-    speculates; three solves in a row on one context."""
-    if kind == "rmat20":
-        g = ka.generate("rmat", 91, scale=20, draws=16 << 20)
-    else:
+    launches and their publication must leave no trace; R-MAT with a
+    24-vertex cycle through its largest hub: a list batch that does not end
+    the reach, one cycle vertex per level, then push levels again). Labels
+    equal Tarjan's, and the level counts equal a profiled solve's, which
+    never speculates; three solves in a row on one context."""
+    if kind == "gnm20":
         g = ka.generate("gnm", 92, n=1 << 20, draws=3 << 20)
+    else:
+        g = ka.generate("rmat", 91, scale=20, draws=16 << 20)
+    if kind == "rmat20_cycle":
+        fo, fa, _, _ = g.to_host()
+        deg = np.diff(fo.astype(np.int64))
+        hub = int(np.argmax(deg))
+        n0 = g.n
+        src = np.repeat(np.arange(n0, dtype=np.int64), deg)
+        dst = fa.astype(np.int64)
+        cyc = np.arange(n0, n0 + 24, dtype=np.int64)
+        src = np.concatenate([src, [hub], cyc[:-1], [cyc[-1]]])
+        dst = np.concatenate([dst, [cyc[0]], cyc[1:], [hub]])
+        g = ka.BidiGraph.from_edges(n0 + 24, src, dst)
     tar = O.tarjan(host_graph(O, g))
     ref, st_ref = ka.scc(g, profile=True)
     assert np.array_equal(ref.cpu().numpy().view(np.uint32), tar)
