@@ -58,14 +58,39 @@ __device__ inline void publish_seq(uint32_t* map, uint32_t seq) {
 // Optionally first reduces per-workgroup partials (claimed count -> *pcount,
 // edges scanned -> cnt64[0], next-level degree sum -> cnt64[1]); after the
 // copy it can zero *reset and cnt64[0..2] for the next level, so a level
-// needs no separate reset or reduction launch. With `gate` (k_spec_gate),
-// it does nothing unless *gate is set. One workgroup of WG threads.
+// needs no separate reset or reduction launch. With `gate` (SpecGate),
+// it does nothing unless *gate is set; with sg.gate it evaluates SpecGate
+// first. One workgroup of WG threads.
+// The host's direction rule for the level after a push level, evaluated by
+// that level's publication before it resets the counters, so that a pull
+// sweep can be launched before the host has read them: pull when (degree
+// sum of the claimed vertices, cnt64[1]) * alpha > m_rest - *expanded (the
+// edges the push level expanded) and more than 64 vertices were claimed,
+// with the host's operands and double arithmetic. *gate = 1 for pull;
+// otherwise the four list counters `lcnt` are zeroed (a sweep would have
+// zeroed them), so the list levels launched behind the gated sweep find no
+// candidates.
+struct SpecGate {
+  uint32_t* gate;
+  const uint32_t* claimed;
+  const uint64_t* expanded;
+  double m_rest;
+  double alpha;
+  uint32_t* lcnt;
+};
+
 __global__ __launch_bounds__(WG) void k_publish(uint32_t* cnt, unsigned long long* cnt64, const uint64_t* extra,
                                                 uint32_t* map, uint32_t seq, const uint64_t* __restrict__ part,
                                                 uint32_t nparts, uint32_t* pcount, uint32_t* reset,
-                                                uint32_t* reset2, const uint32_t* gate) {
+                                                uint32_t* reset2, const uint32_t* gate, SpecGate sg) {
   __shared__ uint64_t s_red[WG / WAVE];
   if (gate && *gate == 0) return;  // a speculative launch that lost: nothing is published
+  if (sg.gate && threadIdx.x == 0) {
+    const double rest = sg.m_rest - (double)*sg.expanded;
+    const bool pull = (double)cnt64[1] * sg.alpha > rest && *sg.claimed > 64;
+    *sg.gate = pull ? 1u : 0u;
+    if (!pull) sg.lcnt[0] = sg.lcnt[1] = sg.lcnt[2] = sg.lcnt[3] = 0;
+  }
   if (part) {
     uint64_t a = 0, b = 0, d = 0;
     for (uint32_t i = threadIdx.x; i < nparts; i += WG) {
@@ -115,24 +140,6 @@ __global__ __launch_bounds__(WG) void k_publish(uint32_t* cnt, unsigned long lon
       cnt64[1] = 0;
       cnt64[2] = 0;
     }
-  }
-}
-
-// The host's direction rule for the level after a push level, evaluated on
-// the device so that a pull sweep can be launched before the host has read
-// the push level's counters: pull when (degree sum of the claimed vertices)
-// * alpha > m_rest - (edges the push level expanded) and more than 64
-// vertices were claimed, with the host's operands and double arithmetic.
-// *gate = 1 for pull. Otherwise the four list counters `lcnt` are zeroed
-// (a sweep would have zeroed them), so the list levels launched behind the
-// gated sweep find no candidates.
-__global__ void k_spec_gate(uint32_t* gate, const uint32_t* claimed, const unsigned long long* nsum,
-                            const uint64_t* expanded, double m_rest, double alpha, uint32_t* lcnt) {
-  if (threadIdx.x == 0) {
-    const double rest = m_rest - (double)*expanded;
-    const bool pull = (double)*nsum * alpha > rest && *claimed > 64;
-    *gate = pull ? 1u : 0u;
-    if (!pull) lcnt[0] = lcnt[1] = lcnt[2] = lcnt[3] = 0;
   }
 }
 
@@ -225,7 +232,7 @@ __device__ inline void chunk_to_list(const uint32_t (&w)[STEPS][4], uint64_t bas
 // *cursor (zero before the launch) few. For lists whose order does not
 // matter. With `guard`, nothing is written when *guard exceeds guard_max
 // (read after the words, so the two loads overlap); with `gate`, nothing
-// unless *gate is set (k_spec_gate). Launch b2q_grid(words) workgroups.
+// unless *gate is set (the SpecGate). Launch b2q_grid(words) workgroups.
 __global__ __launch_bounds__(WG) void k_bitmap_to_queue(const uint32_t* __restrict__ bm, uint64_t words,
                                                         uint32_t* __restrict__ list, uint32_t* cursor,
                                                         const unsigned long long* guard = nullptr,
@@ -1120,7 +1127,7 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
                                              uint64_t* __restrict__ part, const uint32_t* __restrict__ pid,
                                              uint32_t* cursor, uint32_t* __restrict__ sbm, uint32_t* lcnt,
                                              const uint32_t* gate) {
-  if (gate && *gate == 0) return;  // a speculative sweep that lost (k_spec_gate)
+  if (gate && *gate == 0) return;  // a speculative sweep that lost (the SpecGate)
   // A candidate costs a chain of dependent loads (bitmap word, row offsets,
   // in-neighbour, its visited bit, its partition). Each thread carries
   // PULL_ILP vertices (one per 256-vertex tile of its workgroup's span)
@@ -1355,6 +1362,71 @@ __global__ __launch_bounds__(WG) void k_reduce3(const uint64_t* __restrict__ par
     *count = (uint32_t)a;
     cnt64[0] = b;
     cnt64[1] = d;
+  }
+}
+
+// After a pull sweep (its reduction and survivor listing in one launch):
+// every workgroup sums the sweep's per-workgroup partials (claims, scanned
+// edges, candidates left; at most RED_U * WG of them), block 0 writes the
+// sums to *claims, cnt64[0] and cnt64[1], and when at most list_max
+// candidates are left the workgroups list their bits in sbm as
+// k_bitmap_to_queue does (*cursor zero before the launch). With `gate`,
+// nothing unless *gate is set. Launch b2q_grid(words) workgroups.
+__global__ __launch_bounds__(WG) void k_sweep_list(const uint64_t* __restrict__ part, uint32_t nparts, uint32_t* claims,
+                                                   unsigned long long* cnt64, const uint32_t* __restrict__ sbm,
+                                                   uint64_t words, uint32_t* __restrict__ list, uint32_t* cursor,
+                                                   unsigned long long list_max, const uint32_t* gate) {
+  __shared__ uint64_t s_red[WG / WAVE];
+  __shared__ unsigned long long s_left;
+  __shared__ uint32_t s_wave[WG / WAVE];
+  __shared__ uint32_t s_base;
+  if (gate && *gate == 0) return;  // a speculative sweep that lost (SpecGate)
+  uint64_t a = 0, b = 0, d = 0;
+  for (uint32_t i0 = threadIdx.x; i0 < nparts; i0 += WG * RED_U) {
+    uint64_t x[RED_U][3];
+#pragma unroll
+    for (uint32_t r = 0; r < RED_U; ++r) {
+      const uint32_t i = i0 + r * WG;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) x[r][j] = i < nparts ? part[3 * i + j] : 0;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < RED_U; ++r) {
+      a += x[r][0];
+      b += x[r][1];
+      d += x[r][2];
+    }
+  }
+  a = block_sum64(a, s_red);
+  b = block_sum64(b, s_red);
+  d = block_sum64(d, s_red);
+  if (threadIdx.x == 0) {
+    s_left = d;
+    if (blockIdx.x == 0) {
+      *claims = (uint32_t)a;
+      cnt64[0] = b;
+      cnt64[1] = d;
+    }
+  }
+  __syncthreads();
+  if (s_left > list_max) return;  // too many for list levels: the host sweeps again
+  for (uint64_t base = (uint64_t)blockIdx.x * B2Q_CHUNK; base < words; base += (uint64_t)gridDim.x * B2Q_CHUNK) {
+    uint32_t w[B2Q_STEPS][4];
+#pragma unroll
+    for (uint32_t t = 0; t < B2Q_STEPS; ++t) {
+      const uint64_t i0 = base + (uint64_t)t * WG * 4 + (uint64_t)threadIdx.x * 4;
+      if (i0 + 4 <= words) {
+        const uint4 q = *reinterpret_cast<const uint4*>(sbm + i0);  // i0 is a multiple of 4 words
+        w[t][0] = q.x;
+        w[t][1] = q.y;
+        w[t][2] = q.z;
+        w[t][3] = q.w;
+      } else {
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) w[t][k] = i0 + k < words ? sbm[i0 + k] : 0u;
+      }
+    }
+    chunk_to_list(w, base, list, cursor, s_wave, &s_base);
   }
 }
 
@@ -2812,10 +2884,10 @@ int wait_published(Ctx* c, hipStream_t s, uint32_t seq, uint32_t bank = 0) {
 // sequence number back when it learns that it did not) and the wait + copy.
 uint32_t publish_launch(Ctx* c, hipStream_t s, const uint64_t* extra, const uint64_t* part, uint32_t nparts,
                         uint32_t* pcount, uint32_t* reset, uint32_t* reset2, uint32_t bank = 0,
-                        const uint32_t* gate = nullptr) {
+                        const uint32_t* gate = nullptr, SpecGate sg = {}) {
   const uint32_t seq = ++c->pub_seq;
   k_publish<<<1, WG, 0, s>>>(c->cnt, c->cnt64, extra, c->d_map + MAP_BANK * bank, seq, part, nparts, pcount, reset,
-                             reset2, gate);
+                             reset2, gate, sg);
   return seq;
 }
 
@@ -3057,7 +3129,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   // A sweep over all vertices: claims -> cnt[claim_slot], edges ->
   // cnt64[0], candidates left -> cnt64[1], and their list (lists[0]) when
   // there are at most list_max of them, so a batch after it does nothing
-  // otherwise. With `gate`, every launch is conditional on k_spec_gate.
+  // otherwise. With `gate`, every launch is conditional on the SpecGate.
   auto launch_sweep = [&](int claim_slot, const uint32_t* gate, int* prof) {
     // up to 2^25 vertices one resident round of workgroups (2048 beat 1024,
     // 4096 and 16384 on R-MAT 24), above it two (the 4096-graph batch, n =
@@ -3077,10 +3149,9 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
         k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
                                         &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0], gate);
     }
-    k_reduce3<<<1, WG, 0, R.s>>>(c->part, grid, &c->cnt[claim_slot], c->cnt64, gate);
     Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words);
-    k_bitmap_to_queue<<<b2q_grid(R.words), WG, 0, R.s>>>(c->sbm, R.words, lists[0], &c->cnt[lcnt[0]], c->cnt64 + 1,
-                                                         list_max, gate);
+    k_sweep_list<<<b2q_grid(R.words), WG, 0, R.s>>>(c->part, grid, &c->cnt[claim_slot], c->cnt64, c->sbm, R.words,
+                                                    lists[0], &c->cnt[lcnt[0]], list_max, gate);
   };
   // LIST_BATCH list levels from list level L on; the last records its
   // claims in `rec` (count cnt[C_LREC])
@@ -3107,7 +3178,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   // Speculation: on a large graph, the first push level of a direction
   // (from the pivot) is followed at once by a sweep and its list batch,
   // conditional on the device evaluating the host's direction rule
-  // (k_spec_gate), and by their publication into map bank 1. The host reads
+  // (the SpecGate), and by their publication into map bank 1. The host reads
   // the push level's publication meanwhile; if its rule then chooses pull,
   // the sweep is already running, else the gated launches did nothing and
   // the host takes the sequence number of their publication back.
@@ -3212,10 +3283,9 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       // frontier buffers and counter slots as they will be after the swap
       // below (claims to cnt[fc], claimed list to F, reset of cnt[nc])
       const bool spec = speculate && first_level && !list_mode;
-      if (spec)
-        k_spec_gate<<<1, 64, 0, R.s>>>(&c->cnt[C_SPEC], &c->cnt[nc], c->cnt64 + 1, c->scan + f, m_rest, alpha,
-                                       &c->cnt[C_SURV0]);
-      const uint32_t seq0 = publish_launch(c, R.s, c->scan + f, nullptr, 0, nullptr, &c->cnt[fc], nullptr);
+      SpecGate sg{};
+      if (spec) sg = SpecGate{&c->cnt[C_SPEC], &c->cnt[nc], c->scan + f, m_rest, alpha, &c->cnt[C_SURV0]};
+      const uint32_t seq0 = publish_launch(c, R.s, c->scan + f, nullptr, 0, nullptr, &c->cnt[fc], nullptr, 0, nullptr, sg);
       if (spec) {
         launch_sweep(fc, &c->cnt[C_SPEC], nullptr);
         L = 0;
