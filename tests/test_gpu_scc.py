@@ -257,6 +257,29 @@ def test_profile_stats(ka, O):
     assert st.ms_total > 0
 
 
+@pytest.mark.parametrize("offset", [0, 1, 2, 3])
+def test_labels_any_alignment(ka, O, offset):
+    """The dense label pass after the seeded first FW-BW stores four labels
+    per 16-byte write when the labels start on a 16-byte boundary, one label
+    per write otherwise: a caller's buffer at every 4-byte offset, and n not
+    a multiple of 4, give Tarjan's labels; the words around it stay as they
+    were."""
+    s, d = O.gen_rmat(61, 14, 16 << 14)
+    n = (1 << 14) - 3
+    keep = (s < n) & (d < n)
+    hg = O.csr_from_edges(n, s[keep], d[keep])
+    g = ka.BidiGraph.from_host(hg.n, hg.fw_off, hg.fw_adj, hg.bw_off, hg.bw_adj)
+    import torch
+
+    buf = torch.full((n + 8,), -7, dtype=torch.int32, device="cuda")
+    assert buf.data_ptr() % 16 == 0
+    view = buf[offset:offset + n]
+    lab, _ = ka.scc(g, view)
+    assert np.array_equal(lab.cpu().numpy().view(np.uint32), O.tarjan(hg))
+    rest = buf.cpu().numpy()
+    assert (rest[:offset] == -7).all() and (rest[offset + n:] == -7).all()
+
+
 @pytest.mark.parametrize("kind", ["rmat20", "gnm20", "rmat20_cycle"])
 def test_speculative_sweep(ka, O, kind):
     """On graphs of at least 2^20 vertices the sweep after a direction's
