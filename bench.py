@@ -87,8 +87,8 @@ CPU_MIN_S = 10.0  # the CPU baseline repeats a solve until this much time has pa
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="rmat24", choices=sorted(CONFIGS))
     ap.add_argument("--manifest", default=None, metavar="FILE",
                     help="solve the graph of a manifest file (convert_graph's format) instead of --config")
