@@ -231,16 +231,14 @@ __device__ inline void chunk_to_list(const uint32_t (&w)[STEPS][4], uint64_t bas
 // their set bits with chunk_to_list: few, large chunks keep the atomics on
 // *cursor (zero before the launch) few. For lists whose order does not
 // matter. With `guard`, nothing is written when *guard exceeds guard_max
-// (read after the words, so the two loads overlap); with `gate`, nothing
-// unless *gate is set (the SpecGate). Launch b2q_grid(words) workgroups.
+// (read after the words, so the two loads overlap). Launch b2q_grid(words)
+// workgroups.
 __global__ __launch_bounds__(WG) void k_bitmap_to_queue(const uint32_t* __restrict__ bm, uint64_t words,
                                                         uint32_t* __restrict__ list, uint32_t* cursor,
                                                         const unsigned long long* guard = nullptr,
-                                                        unsigned long long guard_max = 0,
-                                                        const uint32_t* gate = nullptr) {
+                                                        unsigned long long guard_max = 0) {
   __shared__ uint32_t s_wave[WG / WAVE];
   __shared__ uint32_t s_base;
-  if (gate && *gate == 0) return;  // a speculative launch that lost
   bool first = true;
   for (uint64_t base = (uint64_t)blockIdx.x * B2Q_CHUNK; base < words; base += (uint64_t)gridDim.x * B2Q_CHUNK) {
     uint32_t w[B2Q_STEPS][4];
@@ -1127,7 +1125,7 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
                                              uint64_t* __restrict__ part, const uint32_t* __restrict__ pid,
                                              uint32_t* cursor, uint32_t* __restrict__ sbm, uint32_t* lcnt,
                                              const uint32_t* gate) {
-  if (gate && *gate == 0) return;  // a speculative sweep that lost (the SpecGate)
+  if (gate && *gate == 0) return;  // a speculative sweep that lost (SpecGate)
   // A candidate costs a chain of dependent loads (bitmap word, row offsets,
   // in-neighbour, its visited bit, its partition). Each thread carries
   // PULL_ILP vertices (one per 256-vertex tile of its workgroup's span)
@@ -1335,8 +1333,7 @@ __global__ __launch_bounds__(WG) void k_pull_list(const uint32_t* __restrict__ l
 
 // sums of the three per-workgroup partials: [0] -> *count, [1] -> cnt64[0], [2] -> cnt64[1]
 __global__ __launch_bounds__(WG) void k_reduce3(const uint64_t* __restrict__ part, uint32_t nparts, uint32_t* count,
-                                                unsigned long long* cnt64, const uint32_t* gate = nullptr) {
-  if (gate && *gate == 0) return;  // a speculative launch that lost
+                                                unsigned long long* cnt64) {
   __shared__ uint64_t s_red[WG / WAVE];
   uint64_t a = 0, b = 0, d = 0;
   // RED_U partials per thread and step, loaded together
@@ -3178,10 +3175,11 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   // Speculation: on a large graph, the first push level of a direction
   // (from the pivot) is followed at once by a sweep and its list batch,
   // conditional on the device evaluating the host's direction rule
-  // (the SpecGate), and by their publication into map bank 1. The host reads
-  // the push level's publication meanwhile; if its rule then chooses pull,
-  // the sweep is already running, else the gated launches did nothing and
-  // the host takes the sequence number of their publication back.
+  // (SpecGate, in the level's publication), and by their publication into
+  // map bank 1. The host reads the push level's publication meanwhile; if
+  // its rule then chooses pull, the sweep is already running, else the
+  // gated launches did nothing and the host takes the sequence number of
+  // their publication back.
   const bool speculate = !pid && !c->profile && R.n >= (1u << 20);
   uint32_t spec_seq = 0;  // the speculative publication's sequence number, while unresolved
   // each level's publication zeroes the counters of the level after it
