@@ -58,3 +58,23 @@ def forward_dag(n, m, rng):
         if u != v:
             pairs.add((int(u), int(v)))
     return csr_from_pairs(n, sorted(pairs))
+
+
+def layered(seed, layers, width, fan=8, back=1):
+    """`layers` layers of `width` vertices; each vertex has `fan` edges to
+    random vertices of the next layer and `back` edges to the previous one.
+    Layer j holds ids [(layers-1-j)*width, (layers-j)*width), so forward
+    edges run from high ids to low: a pull sweep, which sees claims made
+    earlier in the same sweep only by chance, reaches about one layer per
+    sweep and leaves more than n/16 candidates for the next. Returns
+    (n, src, dst) as int64 arrays."""
+    rng = np.random.default_rng(seed)
+    n = layers * width
+    src, dst = [], []
+    for j in range(layers - 1):
+        hi, lo = (layers - 1 - j) * width, (layers - 2 - j) * width
+        src.append(np.repeat(np.arange(width, dtype=np.int64) + hi, fan))
+        dst.append(rng.integers(0, width, width * fan) + lo)
+        src.append(np.repeat(np.arange(width, dtype=np.int64) + lo, back))
+        dst.append(rng.integers(0, width, width * back) + hi)
+    return n, np.concatenate(src), np.concatenate(dst)

@@ -317,6 +317,30 @@ def test_speculative_sweep(ka, O, kind):
         assert st.fwbw_levels_pull > 0
 
 
+@pytest.mark.parametrize("shape", [(16, 4096), (32, 32768)], ids=["l16x4096", "l32x32768"])
+@pytest.mark.parametrize("orient", ["down", "up"])
+def test_repeated_sweeps(ka, O, shape, orient):
+    """Layered graphs whose reach advances about one layer per pull sweep, so
+    a sweep leaves more than n/16 candidates, lists none, and the host sweeps
+    again (at 2^20 vertices after a speculative first sweep). Labels equal
+    Tarjan's in a profiled solve and three plain ones; the profiled solve
+    sweeps at least three times per direction. Level counts are not compared:
+    claims made earlier in the same sweep are seen or not by timing."""
+    layers, width = shape
+    n, src, dst = H.layered(7, layers, width)
+    if orient == "up":
+        src, dst = dst, src
+    g = ka.BidiGraph.from_edges(n, src, dst)
+    tar = O.tarjan(host_graph(O, g))
+    lab, st = ka.scc(g, profile=True)
+    assert np.array_equal(lab.cpu().numpy().view(np.uint32), tar)
+    sweeps = {k["name"]: k["launches"] for k in st.as_dict()["kernels"]}
+    assert sweeps.get("pull_fw", 0) >= 3 and sweeps.get("pull_bw", 0) >= 3, sweeps
+    for _ in range(3):
+        lab, _ = ka.scc(g)
+        assert np.array_equal(lab.cpu().numpy().view(np.uint32), tar)
+
+
 def test_c2_rmat24_full(ka, O):
     """config 2 at full size: R-MAT scale 24, edge factor 16, permuted ids;
     labels equal the oracle's Tarjan labels bit for bit."""
