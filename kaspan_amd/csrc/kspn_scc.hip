@@ -1239,7 +1239,8 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
 // unreached go to `out`. Several list levels run back to back without the
 // host: each reads its candidate count from the device, block 0 zeroes the
 // counter `zero` (neither read nor written here: the next level's output),
-// and every block adds its scanned edges to *edges.
+// and every block adds its scanned edges to *edges_sum when it is given
+// (statistics only: one add per workgroup on one address is not free).
 template <int DIR>
 __global__ __launch_bounds__(WG) void k_pull_list(const uint32_t* __restrict__ list, const uint32_t* list_cnt,
                                                   uint32_t* vis, const uint64_t* __restrict__ roff,
@@ -1327,8 +1328,10 @@ __global__ __launch_bounds__(WG) void k_pull_list(const uint32_t* __restrict__ l
   }
   if (claimed_q) bq_flush(cq, claimed_q, claimed_cnt);
   bq_flush(sq, out, out_cnt);
-  edges = block_sum64(edges, s_red);
-  if (threadIdx.x == 0 && edges) atomicAdd(edges_sum, (unsigned long long)edges);
+  if (edges_sum) {
+    edges = block_sum64(edges, s_red);
+    if (threadIdx.x == 0 && edges) atomicAdd(edges_sum, (unsigned long long)edges);
+  }
 }
 
 // sums of the three per-workgroup partials: [0] -> *count, [1] -> cnt64[0], [2] -> cnt64[1]
@@ -2169,7 +2172,9 @@ __global__ __launch_bounds__(WG) void k_tail_prep(const uint32_t* __restrict__ U
 
 // trim_1_normal over a list of live vertices, for the tail: a vertex with
 // no live neighbour in either direction is decided (own id, live bit
-// cleared); the others go to `rest`, staged per workgroup in LDS. Removing
+// cleared); the others go to `rest`, staged per workgroup in LDS, so the
+// decided ones number *lcount - *rest_cnt (not counted here: a count per
+// workgroup on one address costs more than the scans). Removing
 // such a vertex changes no other vertex's test: it is nobody's live
 // neighbour. A group of 16 lanes per vertex (four per wavefront) reads 16
 // entries of its out-row and of its in-row per step, stopping at the first
@@ -2180,14 +2185,13 @@ __global__ __launch_bounds__(WG) void k_tail_split(const uint32_t* __restrict__ 
                                                    const uint64_t* __restrict__ bw_off,
                                                    const uint32_t* __restrict__ bw_adj, uint32_t* live,
                                                    uint32_t* __restrict__ labels, uint32_t* __restrict__ rest,
-                                                   uint32_t* rest_cnt, uint32_t* iso_cnt) {
+                                                   uint32_t* rest_cnt) {
   constexpr uint32_t GL = 16;              // lanes per vertex
   constexpr uint32_t VPW = WAVE / GL;      // vertices per wavefront
   constexpr uint32_t VPB = VPW * (WG / WAVE);  // vertices per workgroup and step
   __shared__ uint32_t s_q[2 * VPB];
-  __shared__ uint32_t s_n, s_b, s_iso;
+  __shared__ uint32_t s_n, s_b;
   BlockQueue q{s_q, &s_n, &s_b, 2 * VPB};
-  if (threadIdx.x == 0) s_iso = 0;
   bq_init(q);
   const uint32_t lane = lane_id();
   const uint32_t gl = lane % GL;
@@ -2222,13 +2226,11 @@ __global__ __launch_bounds__(WG) void k_tail_split(const uint32_t* __restrict__ 
     if (gl == 0 && valid && !found) {
       labels[v] = v;
       atomicAnd(&live[v >> 5], ~(1u << (v & 31)));
-      atomicAdd(&s_iso, 1u);  // one global add per workgroup, below
     }
     bq_push(q, keep, v);
     bq_maybe_flush(q, rest, rest_cnt, VPB);  // at most one append per vertex of the step
   }
   bq_flush(q, rest, rest_cnt);
-  if (threadIdx.x == 0 && s_iso) atomicAdd(iso_cnt, s_iso);
 }
 
 // in-place exclusive scans of two arrays of u+1 <= 4096 entries in one
@@ -3153,6 +3155,8 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   // LIST_BATCH list levels from list level L on; the last records its
   // claims in `rec` (count cnt[C_LREC])
   auto launch_batch = [&](uint32_t* rec, uint32_t n_in, int* prof) {
+    // the levels' scanned edges, for statistics and profiles only
+    unsigned long long* esum = R.st || c->profile ? c->cnt64 + 2 : nullptr;
     const uint32_t lgrid = std::min<uint32_t>(stream_grid(c, n_in / PULL_ILP + 1), (uint32_t)c->num_cu * 4);
     for (uint32_t j = 0; j < LIST_BATCH; ++j, ++L) {
       // the first level zeroes the claim count the last one records
@@ -3165,11 +3169,11 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       if (dir == 0)
         k_pull_list<0><<<lgrid, WG, 0, R.s>>>(lists[L % 2], &c->cnt[lcnt[L % 4]], vis, roff, radj, pid, q,
                                               &c->cnt[C_LREC], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 4]],
-                                              &c->cnt[lcnt[(L + 2) % 4]], zero2, c->cnt64 + 2, prev);
+                                              &c->cnt[lcnt[(L + 2) % 4]], zero2, esum, prev);
       else
         k_pull_list<1><<<lgrid, WG, 0, R.s>>>(lists[L % 2], &c->cnt[lcnt[L % 4]], vis, roff, radj, pid, q,
                                               &c->cnt[C_LREC], lists[(L + 1) % 2], &c->cnt[lcnt[(L + 1) % 4]],
-                                              &c->cnt[lcnt[(L + 2) % 4]], zero2, c->cnt64 + 2, prev);
+                                              &c->cnt[lcnt[(L + 2) % 4]], zero2, esum, prev);
     }
   };
   // Speculation: on a large graph, the first push level of a direction
@@ -3872,8 +3876,7 @@ int tail_spec(Run& R, uint32_t limit, uint32_t* fwbw_decided, uint32_t* decided,
     // neighbour (ulist)
     Prof p(c, R.s, K_TRIM_SWEEP, 20.0 * limit);
     k_tail_split<<<(uint32_t)c->num_cu * 8, WG, 0, R.s>>>(c->qa, &c->cnt[C_LIVE], g->fw_off, g->fw_adj, g->bw_off,
-                                                          g->bw_adj, c->live, R.labels, c->ulist, &c->cnt[C_RES_N],
-                                                          &c->cnt[C_ISO]);
+                                                          g->bw_adj, c->live, R.labels, c->ulist, &c->cnt[C_RES_N]);
   }
   {
     Prof p(c, R.s, K_RESIDUAL, 24.0 * limit);
@@ -3895,7 +3898,7 @@ int tail_spec(Run& R, uint32_t limit, uint32_t* fwbw_decided, uint32_t* decided,
   KSPN_TRY(read_counters(c, R.s, C_COUNT_));
   *fwbw_decided = c->h_cnt[C_DECIDED];
   R.live_n -= *fwbw_decided;
-  *iso = c->h_cnt[C_ISO];
+  *iso = c->h_cnt[C_LIVE] - c->h_cnt[C_RES_N];  // the split decides every listed vertex it does not keep
   R.live_n -= *iso;
   const uint32_t u = c->h_cnt[C_RES_N];
   if (c->h_cnt[C_ERR]) {
