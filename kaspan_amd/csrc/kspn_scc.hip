@@ -3370,7 +3370,7 @@ int fwbw(Run& R, uint32_t pivot, uint32_t* decided, bool publish = true, bool ta
   {
     Prof p(c, R.s, K_BITMAP, 12.0 * R.words);
     Zero3 z{};
-    if (tail) z = Zero3{{&c->cnt[C_RES_N], &c->cnt[C_ISO], &c->cnt[C_ROOM]}};
+    if (tail) z = Zero3{{&c->cnt[C_RES_N], &c->cnt[C_ROOM], nullptr}};
     k_fwbw_finish<<<nparts, WG, 0, R.s>>>(c->live, c->bwv, R.words, c->part, tail ? c->qa : nullptr,
                                           &c->cnt[C_LIVE], z);
   }
