@@ -2031,34 +2031,31 @@ __global__ void k_tarjan(uint32_t n, const uint64_t* __restrict__ off, const uin
 // backward over out-edges within each colour, then writes the minimum
 // member as the label of every vertex decided in the round. state: 0 live,
 // 2 decided this round, 1 decided. Rounds are capped by max_rounds (0: none);
-// *status = 1 if the cap was hit.
-__global__ __launch_bounds__(1024) void k_tail_solve(const uint32_t* sn_ptr, const uint32_t* __restrict__ super,
-                                                     const uint32_t* __restrict__ subid,
-                                                     const uint64_t* __restrict__ fo, const uint64_t* __restrict__ bo,
-                                                     const uint32_t* __restrict__ sub, uint64_t cap,
-                                                     uint32_t* col, uint32_t* state, uint32_t* minv,
-                                                     uint32_t* __restrict__ labels, uint32_t r0, uint32_t max_rounds,
-                                                     uint32_t* status, uint32_t* live, const uint32_t* skip,
-                                                     const uint32_t* room) {
-  __shared__ uint32_t s_flag, s_live;
+// *status = 1 if the cap was hit. A subgraph of at most TAIL_LDS_V vertices
+// and TAIL_LDS_E edges per direction runs from LDS (its CSRs as u32, the
+// colours, states and minima), the rest from the global buffers.
+constexpr uint32_t TAIL_LDS_V = 4096;  // every tail (at most 4095 vertices)
+constexpr uint32_t TAIL_LDS_E = 7168;  // edges per direction staged in LDS
+
+// the tail solver's rounds over CSRs fo/fa (out) and bo/ba (in), with the
+// per-vertex colours, states and minima at col/state/minv (LDS or global)
+template <typename Off, typename St>
+__device__ void tail_rounds(uint32_t sn, const uint32_t* __restrict__ super, const uint32_t* __restrict__ subid,
+                            const Off* fo, const Off* bo, const uint32_t* fa, const uint32_t* ba, uint32_t* col,
+                            St* state, uint32_t* minv, uint32_t* __restrict__ labels, uint32_t r0,
+                            uint32_t max_rounds, uint32_t* status, uint32_t* live, uint32_t* s_flag,
+                            uint32_t* s_live) {
   const uint32_t tid = threadIdx.x, T = blockDim.x;
-  // the sizes and the sub-CSR come from the launches before (k_tail_fill)
-  const uint32_t sn = *sn_ptr;
-  if (*skip || *room) return;
-  const uint64_t sm = fo[sn];
-  if (2 * sm + 2 > cap) return;
-  const uint32_t* __restrict__ fa = sub;
-  const uint32_t* __restrict__ ba = sub + sm + 1;
   for (uint32_t i = tid; i < sn; i += T) state[i] = 0;
   uint32_t r = r0;
   for (uint32_t round = 0;; ++round) {
     __syncthreads();
-    if (tid == 0) s_live = 0;
+    if (tid == 0) *s_live = 0;
     __syncthreads();
     for (uint32_t i = tid; i < sn; i += T)
-      if (state[i] == 0) { s_live = 1; break; }
+      if (state[i] == 0) { *s_live = 1; break; }
     __syncthreads();
-    if (!s_live) break;
+    if (!*s_live) break;
     if (max_rounds && round >= max_rounds) {
       if (tid == 0) *status = 1;
       break;
@@ -2068,7 +2065,7 @@ __global__ __launch_bounds__(1024) void k_tail_solve(const uint32_t* sn_ptr, con
     // forward: max colour over in-neighbours, in place, to a fixpoint
     for (;;) {
       __syncthreads();
-      if (tid == 0) s_flag = 0;
+      if (tid == 0) *s_flag = 0;
       __syncthreads();
       for (uint32_t i = tid; i < sn; i += T) {
         if (state[i] != 0) continue;
@@ -2076,11 +2073,11 @@ __global__ __launch_bounds__(1024) void k_tail_solve(const uint32_t* sn_ptr, con
         for (uint64_t it = bo[i], end = bo[i + 1]; it < end; ++it) c = max(c, col[ba[it]]);
         if (c > c0) {
           col[i] = c;
-          s_flag = 1;
+          *s_flag = 1;
         }
       }
       __syncthreads();
-      if (!s_flag) break;
+      if (!*s_flag) break;
     }
     // roots, then backward closure within a colour over out-edges
     for (uint32_t i = tid; i < sn; i += T)
@@ -2090,7 +2087,7 @@ __global__ __launch_bounds__(1024) void k_tail_solve(const uint32_t* sn_ptr, con
       }
     for (;;) {
       __syncthreads();
-      if (tid == 0) s_flag = 0;
+      if (tid == 0) *s_flag = 0;
       __syncthreads();
       for (uint32_t i = tid; i < sn; i += T) {
         if (state[i] != 0) continue;
@@ -2099,13 +2096,13 @@ __global__ __launch_bounds__(1024) void k_tail_solve(const uint32_t* sn_ptr, con
           uint32_t j = fa[it];
           if (state[j] == 2 && col[j] == ci) {
             state[i] = 2;
-            s_flag = 1;
+            *s_flag = 1;
             break;
           }
         }
       }
       __syncthreads();
-      if (!s_flag) break;
+      if (!*s_flag) break;
     }
     // minimum member per colour: the root of colour c is the vertex whose
     // key is c, at compact index subid[root]
@@ -2126,6 +2123,45 @@ __global__ __launch_bounds__(1024) void k_tail_solve(const uint32_t* sn_ptr, con
   __syncthreads();
   for (uint32_t i = tid; i < sn; i += T)
     if (state[i] == 1) atomicAnd(&live[super[i] >> 5], ~(1u << (super[i] & 31)));
+}
+
+__global__ __launch_bounds__(1024) void k_tail_solve(const uint32_t* sn_ptr, const uint32_t* __restrict__ super,
+                                                     const uint32_t* __restrict__ subid,
+                                                     const uint64_t* __restrict__ fo, const uint64_t* __restrict__ bo,
+                                                     const uint32_t* __restrict__ sub, uint64_t cap,
+                                                     uint32_t* col, uint32_t* state, uint32_t* minv,
+                                                     uint32_t* __restrict__ labels, uint32_t r0, uint32_t max_rounds,
+                                                     uint32_t* status, uint32_t* live, const uint32_t* skip,
+                                                     const uint32_t* room) {
+  __shared__ uint32_t s_flag, s_live;
+  const uint32_t tid = threadIdx.x, T = blockDim.x;
+  // the sizes and the sub-CSR come from the launches before (k_tail_fill)
+  const uint32_t sn = *sn_ptr;
+  if (*skip || *room) return;
+  const uint64_t sm = fo[sn];
+  if (2 * sm + 2 > cap) return;
+  const uint32_t* __restrict__ fa = sub;
+  const uint32_t* __restrict__ ba = sub + sm + 1;
+  if (sn <= TAIL_LDS_V && sm <= TAIL_LDS_E) {
+    __shared__ uint32_t s_fo[TAIL_LDS_V + 1], s_bo[TAIL_LDS_V + 1];
+    __shared__ uint32_t s_fa[TAIL_LDS_E], s_ba[TAIL_LDS_E];
+    __shared__ uint32_t s_col[TAIL_LDS_V], s_minv[TAIL_LDS_V];
+    __shared__ uint8_t s_state[TAIL_LDS_V];
+    for (uint32_t i = tid; i <= sn; i += T) {
+      s_fo[i] = (uint32_t)fo[i];
+      s_bo[i] = (uint32_t)bo[i];
+    }
+    for (uint32_t e = tid; e < (uint32_t)sm; e += T) {
+      s_fa[e] = fa[e];
+      s_ba[e] = ba[e];
+    }
+    // the rounds start with a block barrier
+    tail_rounds(sn, super, subid, s_fo, s_bo, s_fa, s_ba, s_col, s_state, s_minv, labels, r0, max_rounds, status,
+                live, &s_flag, &s_live);
+  } else {
+    tail_rounds(sn, super, subid, fo, bo, fa, ba, col, state, minv, labels, r0, max_rounds, status, live, &s_flag,
+                &s_live);
+  }
 }
 
 // tail subgraph, one pass: compact ids, live out- and in-degrees (entry u = 0)
