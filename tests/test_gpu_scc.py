@@ -341,6 +341,35 @@ def test_repeated_sweeps(ka, O, shape, orient):
         assert np.array_equal(lab.cpu().numpy().view(np.uint32), tar)
 
 
+@pytest.mark.parametrize("dag_edges", [500, 8000], ids=["lds", "global"])
+def test_tail_solver_sizes(ka, O, dag_edges):
+    """A dense 5000-vertex SCC takes the pivot; what FW-BW leaves is 1000
+    three-cycles joined by forward DAG edges between cycles, all for the tail
+    solver. With 500 DAG edges its sub-CSR (3500 edges per direction) runs
+    from LDS, with 8000 (11 000) from the global buffers. Labels equal
+    Tarjan's and the tail took all 3000 cycle vertices."""
+    rng = np.random.default_rng(17)
+    big, cyc = 5000, 1000
+    src = [np.arange(big), rng.integers(0, big, 20 * big)]
+    dst = [(np.arange(big) + 1) % big, rng.integers(0, big, 20 * big)]
+    base = big + 3 * np.arange(cyc)
+    for k in range(3):
+        src.append(base + k)
+        dst.append(base + (k + 1) % 3)
+    a = rng.integers(0, cyc, 2 * dag_edges)
+    b = rng.integers(0, cyc, 2 * dag_edges)
+    keep = a < b
+    a, b = a[keep][:dag_edges], b[keep][:dag_edges]
+    src.append(big + 3 * a + rng.integers(0, 3, a.size))
+    dst.append(big + 3 * b + rng.integers(0, 3, b.size))
+    n = big + 3 * cyc
+    g = ka.BidiGraph.from_edges(n, np.concatenate(src).astype(np.int64), np.concatenate(dst).astype(np.int64))
+    tar = O.tarjan(host_graph(O, g))
+    lab, st = gpu_labels(ka, g)
+    assert np.array_equal(lab, tar)
+    assert st.residual_n == 3 * cyc
+
+
 def test_c2_rmat24_full(ka, O):
     """config 2 at full size: R-MAT scale 24, edge factor 16, permuted ids;
     labels equal the oracle's Tarjan labels bit for bit."""
