@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # library profile names -> kernel symbols as rocprofv3 reports them
 ROCPROF_NAMES = {"pull_fw": ["kspn::k_pull<0>"], "pull_bw": ["kspn::k_pull<1>"], "trim_first": ["kspn::k_trim_first"],
                  "expand_fw": ["kspn::k_expand<0>"], "expand_bw": ["kspn::k_expand<1>", "kspn::k_expand<5>"],
-                 "label_scc": ["kspn::k_label_bits", "kspn::k_part_apply"], "tail_solve": ["kspn::k_tail_solve"],
+                 "label_scc": ["kspn::k_label_bits", "kspn::k_label_prep", "kspn::k_part_apply"], "tail_solve": ["kspn::k_tail_solve"],
                  "trim_sweep": ["kspn::k_trim_sweep<1>"],
                  "reach_coop": ["kspn::k_reach_coop<0>", "kspn::k_reach_coop<1>", "kspn::k_reach_coop<5>"],
                  "bitmap_ops": ["kspn::k_part_classify"], "pivot_reduce": ["kspn::k_part_key"],

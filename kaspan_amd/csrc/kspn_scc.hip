@@ -1558,11 +1558,11 @@ __global__ __launch_bounds__(WG) void k_fwbw_finish(uint32_t* live, const uint32
 // after a first trim that wrote no labels) every label is written, in full
 // lines: m in the SCC, UNDEC for live vertices, and their own id for the
 // vertices the first trim decided (neither live nor in the SCC).
-__global__ __launch_bounds__(WG) void k_label_bits(uint32_t n, const uint32_t* __restrict__ bm,
+__device__ void label_bits(uint32_t bid, uint32_t nb, uint32_t n, const uint32_t* __restrict__ bm,
                                                    const uint64_t* __restrict__ part, uint32_t nparts,
                                                    uint32_t* minv_out, uint32_t* count_out,
                                                    uint32_t* __restrict__ labels,
-                                                   const uint32_t* __restrict__ live = nullptr) {
+                                                   const uint32_t* __restrict__ live) {
   __shared__ uint32_t s_red[WG / WAVE];
   __shared__ uint32_t s_m;
   {
@@ -1573,7 +1573,7 @@ __global__ __launch_bounds__(WG) void k_label_bits(uint32_t n, const uint32_t* _
     x = block_min(x, s_red);
     if (threadIdx.x == 0) {
       s_m = x;
-      if (blockIdx.x == 0) {
+      if (bid == 0) {
         *minv_out = x;
         *count_out = c;
       }
@@ -1587,7 +1587,7 @@ __global__ __launch_bounds__(WG) void k_label_bits(uint32_t n, const uint32_t* _
     constexpr int LQ = 4;
     const uint64_t quads = (uint64_t)n / 4;
     uint4* lab4 = reinterpret_cast<uint4*>(labels);
-    for (uint64_t base = blockIdx.x * (uint64_t)WG * LQ; base < quads; base += (uint64_t)gridDim.x * WG * LQ) {
+    for (uint64_t base = bid * (uint64_t)WG * LQ; base < quads; base += (uint64_t)nb * WG * LQ) {
       uint32_t bw[LQ], lw[LQ];
 #pragma unroll
       for (int k = 0; k < LQ; ++k) {
@@ -1610,14 +1610,14 @@ __global__ __launch_bounds__(WG) void k_label_bits(uint32_t n, const uint32_t* _
       }
     }
     // the last n % 4 vertices
-    if (blockIdx.x == 0 && threadIdx.x < (n & 3u)) {
+    if (bid == 0 && threadIdx.x < (n & 3u)) {
       const uint32_t v = (uint32_t)(quads * 4 + threadIdx.x);
       labels[v] = bm_get(bm, v) ? m : (bm_get(live, v) ? UNDEC : v);
     }
     return;
   }
   constexpr int LU = 8;  // vertices per thread and step (one per 256-vertex tile), word loads issued together
-  for (uint64_t base = blockIdx.x * (uint64_t)WG * LU; base < n; base += (uint64_t)gridDim.x * WG * LU) {
+  for (uint64_t base = bid * (uint64_t)WG * LU; base < n; base += (uint64_t)nb * WG * LU) {
     uint32_t bw[LU], lw[LU];
 #pragma unroll
     for (int k = 0; k < LU; ++k) {
@@ -1636,6 +1636,14 @@ __global__ __launch_bounds__(WG) void k_label_bits(uint32_t n, const uint32_t* _
         labels[v] = m;
     }
   }
+}
+
+__global__ __launch_bounds__(WG) void k_label_bits(uint32_t n, const uint32_t* __restrict__ bm,
+                                                   const uint64_t* __restrict__ part, uint32_t nparts,
+                                                   uint32_t* minv_out, uint32_t* count_out,
+                                                   uint32_t* __restrict__ labels,
+                                                   const uint32_t* __restrict__ live = nullptr) {
+  label_bits(blockIdx.x, gridDim.x, n, bm, part, nparts, minv_out, count_out, labels, live);
 }
 
 // =====================================================================
@@ -2169,7 +2177,7 @@ __global__ __launch_bounds__(1024) void k_tail_solve(const uint32_t* sn_ptr, con
 // tail scan reports that case). *iso counts live vertices without a live
 // neighbour in either direction: trim_1_normal's decisions. One wavefront
 // per vertex: tail vertices can have long rows.
-__global__ __launch_bounds__(WG) void k_tail_prep(const uint32_t* __restrict__ U, const uint32_t* ucount,
+__device__ void tail_prep(uint32_t bid, uint32_t nb, const uint32_t* __restrict__ U, const uint32_t* ucount,
                                                   uint32_t limit, const uint64_t* __restrict__ fw_off,
                                                   const uint32_t* __restrict__ fw_adj,
                                                   const uint64_t* __restrict__ bw_off,
@@ -2184,11 +2192,11 @@ __global__ __launch_bounds__(WG) void k_tail_prep(const uint32_t* __restrict__ U
   const uint32_t wpb = WG / WAVE;
   if (threadIdx.x == 0) s_iso = 0;
   __syncthreads();
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (bid == 0 && threadIdx.x == 0) {
     degf[u] = 0;
     degb[u] = 0;
   }
-  for (uint32_t i = blockIdx.x * wpb + threadIdx.x / WAVE; i < u; i += gridDim.x * wpb) {
+  for (uint32_t i = bid * wpb + threadIdx.x / WAVE; i < u; i += nb * wpb) {
     const uint32_t v = U[i];
     uint32_t df = 0, db = 0;
     for (uint64_t it = fw_off[v] + lane, end = fw_off[v + 1]; it < end; it += WAVE) df += bm_get(live, fw_adj[it]);
@@ -2204,6 +2212,42 @@ __global__ __launch_bounds__(WG) void k_tail_prep(const uint32_t* __restrict__ U
   }
   __syncthreads();
   if (iso && threadIdx.x == 0 && s_iso) atomicAdd(iso, s_iso);  // one global add per workgroup
+}
+
+__global__ __launch_bounds__(WG) void k_tail_prep(const uint32_t* __restrict__ U, const uint32_t* ucount,
+                                                  uint32_t limit, const uint64_t* __restrict__ fw_off,
+                                                  const uint32_t* __restrict__ fw_adj,
+                                                  const uint64_t* __restrict__ bw_off,
+                                                  const uint32_t* __restrict__ bw_adj,
+                                                  const uint32_t* __restrict__ live, uint32_t* __restrict__ subid,
+                                                  uint64_t* __restrict__ degf, uint64_t* __restrict__ degb,
+                                                  uint32_t* iso) {
+  tail_prep(blockIdx.x, gridDim.x, U, ucount, limit, fw_off, fw_adj, bw_off, bw_adj, live, subid, degf, degb, iso);
+}
+
+// The label pass of the last FW-BW (label_bits, blocks [0, lblocks); its
+// bitmap bm, partials, outputs and lab_live as for k_label_bits) and the
+// tail's degree pass (tail_prep, the other blocks; nothing counted as
+// isolated) in one launch. Both need only the tail split before them: the
+// split labels the vertices it decides with their own id, which is what
+// the label pass writes for vertices neither live nor in the SCC.
+__global__ __launch_bounds__(WG) void k_label_prep(uint32_t lblocks, uint32_t n, const uint32_t* __restrict__ bm,
+                                                   const uint64_t* __restrict__ part, uint32_t nparts,
+                                                   uint32_t* minv_out, uint32_t* count_out,
+                                                   uint32_t* __restrict__ labels,
+                                                   const uint32_t* __restrict__ lab_live,
+                                                   const uint32_t* __restrict__ U, const uint32_t* ucount,
+                                                   uint32_t limit, const uint64_t* __restrict__ fw_off,
+                                                   const uint32_t* __restrict__ fw_adj,
+                                                   const uint64_t* __restrict__ bw_off,
+                                                   const uint32_t* __restrict__ bw_adj,
+                                                   const uint32_t* __restrict__ live, uint32_t* __restrict__ subid,
+                                                   uint64_t* __restrict__ degf, uint64_t* __restrict__ degb) {
+  if (blockIdx.x < lblocks)
+    label_bits(blockIdx.x, lblocks, n, bm, part, nparts, minv_out, count_out, labels, lab_live);
+  else
+    tail_prep(blockIdx.x - lblocks, gridDim.x - lblocks, U, ucount, limit, fw_off, fw_adj, bw_off, bw_adj, live,
+              subid, degf, degb, nullptr);
 }
 
 // trim_1_normal over a list of live vertices, for the tail: a vertex with
@@ -3012,6 +3056,10 @@ struct Run {
   // (cnt[C_DECIDED], cnt[C_FLAG]); the first publication of the forward
   // reach brings them to the host
   bool live_pending = false;
+  // fwbw(tail = true) leaves its label pass to tail_spec (k_label_prep)
+  bool label_pending = false;
+  uint32_t label_nparts = 0;  // k_fwbw_finish's partials
+  bool label_dense = false;   // write every label (the first trim wrote none)
   uint32_t trim_decided = 0, first_pivot = 0;
 };
 
@@ -3410,7 +3458,12 @@ int fwbw(Run& R, uint32_t pivot, uint32_t* decided, bool publish = true, bool ta
     k_fwbw_finish<<<nparts, WG, 0, R.s>>>(c->live, c->bwv, R.words, c->part, tail ? c->qa : nullptr,
                                           &c->cnt[C_LIVE], z);
   }
-  {
+  if (tail) {
+    // tail_spec runs it beside the tail's degree pass (k_label_prep)
+    R.label_pending = true;
+    R.label_nparts = nparts;
+    R.label_dense = seeded;
+  } else {
     Prof p(c, R.s, K_LABEL, seeded ? R.n / 4.0 + 4.0 * R.n : R.n / 8.0);
     k_label_bits<<<stream_grid(c, R.n / 8 + 1), WG, 0, R.s>>>(R.n, c->bwv, c->part, nparts, &c->cnt[C_MIN],
                                                               &c->cnt[C_DECIDED], R.labels, seeded ? c->live : nullptr);
@@ -3915,10 +3968,23 @@ int tail_spec(Run& R, uint32_t limit, uint32_t* fwbw_decided, uint32_t* decided,
                                                           g->bw_adj, c->live, R.labels, c->ulist, &c->cnt[C_RES_N]);
   }
   {
-    Prof p(c, R.s, K_RESIDUAL, 24.0 * limit);
-    k_tail_prep<<<stream_grid(c, (uint64_t)limit * WAVE), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], limit, g->fw_off,
-                                                                    g->fw_adj, g->bw_off, g->bw_adj, c->live, subid,
-                                                                    c->scan, c->sub_off, nullptr);
+    if (!R.label_pending) {
+      note_error("tail_spec", "called without fwbw's deferred label pass", __FILE__, __LINE__);
+      return KSPN_E_INVALID;
+    }
+    R.label_pending = false;
+    // fwbw's label pass (its SCC's labels, and with label_dense every other
+    // label) in the same launch as the tail's degree pass
+    const uint32_t lblocks = stream_grid(c, R.n / 8 + 1);
+    const uint32_t pblocks = stream_grid(c, (uint64_t)limit * WAVE);
+    Prof p(c, R.s, K_LABEL, (R.label_dense ? R.n / 4.0 + 4.0 * R.n : R.n / 8.0) + 24.0 * limit);
+    k_label_prep<<<lblocks + pblocks, WG, 0, R.s>>>(lblocks, R.n, c->bwv, c->part, R.label_nparts, &c->cnt[C_MIN],
+                                                    &c->cnt[C_DECIDED], R.labels, R.label_dense ? c->live : nullptr,
+                                                    c->ulist, &c->cnt[C_RES_N], limit, g->fw_off, g->fw_adj,
+                                                    g->bw_off, g->bw_adj, c->live, subid, c->scan, c->sub_off);
+  }
+  {
+    Prof p(c, R.s, K_RESIDUAL, 0.0);
     k_tail_scan<<<1, 1024, 0, R.s>>>(&c->cnt[C_RES_N], limit, c->scan, c->sub_off, &c->cnt[C_FLAG], &c->cnt[C_ERR]);
     k_tail_fill<<<stream_grid(c, (uint64_t)limit * WAVE), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, g->bw_off,
                                                        g->bw_adj, c->live, subid, c->scan, c->sub_off, c->sub_adj,
