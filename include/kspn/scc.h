@@ -117,6 +117,8 @@ typedef struct kspn_phase {
   uint64_t memory;     /* device bytes in use when the scope ended (hipMemGetInfo), if has_memory */
   uint32_t entries;    /* times the scope was entered */
   uint32_t has_memory; /* the reference records "memory" for this scope */
+  double bytes;        /* KSPN_FLAG_PROFILE: algorithmic bytes of the scope's kernel launches */
+  double kernel_ms;    /* KSPN_FLAG_PROFILE: summed durations of those launches */
 } kspn_phase;
 
 typedef struct kspn_stats {
@@ -132,7 +134,8 @@ typedef struct kspn_stats {
   uint32_t fwbw_levels_coop;  /* FW/BW push levels run inside the persistent multi-level kernel */
   uint64_t residual_m;
   uint64_t edges_scanned;  /* adjacency entries read, all phases */
-  double bytes_trim, bytes_fwbw, bytes_color, bytes_residual, bytes_canon; /* algorithmic bytes */
+  /* KSPN_FLAG_PROFILE: algorithmic bytes, summed over the scopes as ms_* are */
+  double bytes_trim, bytes_fwbw, bytes_color, bytes_residual, bytes_canon;
   uint32_t n_kern;
   kspn_kern_stat kern[KSPN_KERN_MAX];
   uint32_t n_phase;                   /* scopes in order of first entry */

@@ -48,13 +48,14 @@ GEN_GNM, GEN_RMAT, GEN_LATTICE, GEN_PLANTED = 0, 1, 2, 3
 MLP_PARAMS = 1313
 UNDECIDED = 0xFFFFFFFF
 
-# every symbol include/kspn/scc.h declares
+# every symbol include/kspn/*.h declares
 EXPORTS = (
     "kspn_opts_default", "kspn_strerror", "kspn_last_error", "kspn_ctx_create", "kspn_ctx_destroy", "kspn_ctx_reserve",
     "kspn_scc", "kspn_scc_host", "kspn_transpose", "kspn_canonicalize", "kspn_subgraph", "kspn_tarjan",
     "kspn_generate", "kspn_build_csr", "kspn_free", "kspn_planted_labels", "kspn_pivot_features",
     "kspn_mlp_forward", "kspn_mlp_grad", "kspn_mlp_adam", "kspn_scc_sizes", "kspn_scc_batch", "kspn_batch_plan",
     "kspn_generate_batch", "kspn_manifest_parse", "kspn_manifest_load", "kspn_manifest_graph", "kspn_unpack_dense",
+    "kspn_test_tail_guard",
 )
 
 
@@ -131,7 +132,7 @@ class KernStat(C.Structure):
 
 class Phase(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("ms", C.c_double), ("decided", C.c_uint64), ("memory", C.c_uint64),
-                ("entries", C.c_uint32), ("has_memory", C.c_uint32)]
+                ("entries", C.c_uint32), ("has_memory", C.c_uint32), ("bytes", C.c_double), ("kernel_ms", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -159,7 +160,7 @@ class Stats(C.Structure):
     def phases(self):
         """the statistics scopes in order of first entry"""
         return [{"name": p.name.decode(), "ms": p.ms, "decided": p.decided, "memory": p.memory,
-                 "entries": p.entries, "has_memory": bool(p.has_memory)}
+                 "entries": p.entries, "has_memory": bool(p.has_memory), "bytes": p.bytes, "kernel_ms": p.kernel_ms}
                 for p in self.phase[: self.n_phase]]
 
     def as_dict(self):
@@ -247,6 +248,7 @@ def lib():
         "kspn_manifest_load": (i32, [vp, P(Manifest), vp, vp, vp, vp, vp]),
         "kspn_manifest_graph": (i32, [vp, P(Manifest), P(u32), P(u64), P(vp), P(vp), P(vp), P(vp), vp]),
         "kspn_unpack_dense": (i32, [vp, vp, u64, u32, u32, u64, vp, u32, vp]),
+        "kspn_test_tail_guard": (i32, [vp, P(Graph), u32, vp, u32, P(u32), vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

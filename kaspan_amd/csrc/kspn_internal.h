@@ -232,6 +232,7 @@ enum Counter : int {
   C_LREC,    // claims of the last pull list level of a batch
   C_LIVE,    // vertices left live after the last FW-BW, listed by k_fwbw_finish for the tail solver
   C_SPEC,    // k_spec_gate: a speculatively launched pull sweep is to run
+  C_BADID,   // a tail kernel met a listed vertex id >= n and skipped it
   C_COUNT_ = 24
 };
 
@@ -244,6 +245,7 @@ struct KernTimer {
 struct EventPair {
   hipEvent_t a, b;
   int kern;
+  int phase;  // statistics scope open at the launch (-1: none)
   double bytes;
 };
 
@@ -296,6 +298,9 @@ struct Ctx {
   std::vector<KernTimer> kt;
   std::vector<EventPair> pending;
   std::vector<hipEvent_t> free_events;
+  int cur_phase = -1;                     // statistics scope open now (kspn_stats.phase index), -1: none
+  double phase_bytes[KSPN_PHASE_MAX] = {};  // profile mode: algorithmic bytes of the launches in each scope
+  double phase_kms[KSPN_PHASE_MAX] = {};    // profile mode: summed launch durations in each scope
   hipStream_t stream = nullptr;
   int num_cu = 256;
   uint32_t pub_seq = 0;  // sequence number of the last counter publication (h_map[63])

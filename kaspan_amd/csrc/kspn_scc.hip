@@ -2177,14 +2177,12 @@ __global__ __launch_bounds__(1024) void k_tail_solve(const uint32_t* sn_ptr, con
 // tail scan reports that case). *iso counts live vertices without a live
 // neighbour in either direction: trim_1_normal's decisions. One wavefront
 // per vertex: tail vertices can have long rows.
-__device__ void tail_prep(uint32_t bid, uint32_t nb, const uint32_t* __restrict__ U, const uint32_t* ucount,
-                                                  uint32_t limit, const uint64_t* __restrict__ fw_off,
-                                                  const uint32_t* __restrict__ fw_adj,
-                                                  const uint64_t* __restrict__ bw_off,
-                                                  const uint32_t* __restrict__ bw_adj,
-                                                  const uint32_t* __restrict__ live, uint32_t* __restrict__ subid,
-                                                  uint64_t* __restrict__ degf, uint64_t* __restrict__ degb,
-                                                  uint32_t* iso) {
+__device__ void tail_prep(uint32_t bid, uint32_t nb, uint32_t n, const uint32_t* __restrict__ U,
+                          const uint32_t* ucount, uint32_t limit, const uint64_t* __restrict__ fw_off,
+                          const uint32_t* __restrict__ fw_adj, const uint64_t* __restrict__ bw_off,
+                          const uint32_t* __restrict__ bw_adj, const uint32_t* __restrict__ live,
+                          uint32_t* __restrict__ subid, uint64_t* __restrict__ degf, uint64_t* __restrict__ degb,
+                          uint32_t* iso, uint32_t* bad) {
   __shared__ uint32_t s_iso;
   const uint32_t u = *ucount;
   if (u > limit) return;
@@ -2198,6 +2196,10 @@ __device__ void tail_prep(uint32_t bid, uint32_t nb, const uint32_t* __restrict_
   }
   for (uint32_t i = bid * wpb + threadIdx.x / WAVE; i < u; i += nb * wpb) {
     const uint32_t v = U[i];
+    if (v >= n) {  // not a vertex: reported, never used as an index
+      if (lane == 0) *bad = 1;
+      continue;
+    }
     uint32_t df = 0, db = 0;
     for (uint64_t it = fw_off[v] + lane, end = fw_off[v + 1]; it < end; it += WAVE) df += bm_get(live, fw_adj[it]);
     for (uint64_t it = bw_off[v] + lane, end = bw_off[v + 1]; it < end; it += WAVE) db += bm_get(live, bw_adj[it]);
@@ -2214,15 +2216,16 @@ __device__ void tail_prep(uint32_t bid, uint32_t nb, const uint32_t* __restrict_
   if (iso && threadIdx.x == 0 && s_iso) atomicAdd(iso, s_iso);  // one global add per workgroup
 }
 
-__global__ __launch_bounds__(WG) void k_tail_prep(const uint32_t* __restrict__ U, const uint32_t* ucount,
+__global__ __launch_bounds__(WG) void k_tail_prep(uint32_t n, const uint32_t* __restrict__ U, const uint32_t* ucount,
                                                   uint32_t limit, const uint64_t* __restrict__ fw_off,
                                                   const uint32_t* __restrict__ fw_adj,
                                                   const uint64_t* __restrict__ bw_off,
                                                   const uint32_t* __restrict__ bw_adj,
                                                   const uint32_t* __restrict__ live, uint32_t* __restrict__ subid,
                                                   uint64_t* __restrict__ degf, uint64_t* __restrict__ degb,
-                                                  uint32_t* iso) {
-  tail_prep(blockIdx.x, gridDim.x, U, ucount, limit, fw_off, fw_adj, bw_off, bw_adj, live, subid, degf, degb, iso);
+                                                  uint32_t* iso, uint32_t* bad) {
+  tail_prep(blockIdx.x, gridDim.x, n, U, ucount, limit, fw_off, fw_adj, bw_off, bw_adj, live, subid, degf, degb, iso,
+            bad);
 }
 
 // The label pass of the last FW-BW (label_bits, blocks [0, lblocks); its
@@ -2242,12 +2245,13 @@ __global__ __launch_bounds__(WG) void k_label_prep(uint32_t lblocks, uint32_t n,
                                                    const uint64_t* __restrict__ bw_off,
                                                    const uint32_t* __restrict__ bw_adj,
                                                    const uint32_t* __restrict__ live, uint32_t* __restrict__ subid,
-                                                   uint64_t* __restrict__ degf, uint64_t* __restrict__ degb) {
+                                                   uint64_t* __restrict__ degf, uint64_t* __restrict__ degb,
+                                                   uint32_t* bad) {
   if (blockIdx.x < lblocks)
     label_bits(blockIdx.x, lblocks, n, bm, part, nparts, minv_out, count_out, labels, lab_live);
   else
-    tail_prep(blockIdx.x - lblocks, gridDim.x - lblocks, U, ucount, limit, fw_off, fw_adj, bw_off, bw_adj, live,
-              subid, degf, degb, nullptr);
+    tail_prep(blockIdx.x - lblocks, gridDim.x - lblocks, n, U, ucount, limit, fw_off, fw_adj, bw_off, bw_adj, live,
+              subid, degf, degb, nullptr, bad);
 }
 
 // trim_1_normal over a list of live vertices, for the tail: a vertex with
@@ -2259,13 +2263,13 @@ __global__ __launch_bounds__(WG) void k_label_prep(uint32_t lblocks, uint32_t n,
 // neighbour. A group of 16 lanes per vertex (four per wavefront) reads 16
 // entries of its out-row and of its in-row per step, stopping at the first
 // live neighbour.
-__global__ __launch_bounds__(WG) void k_tail_split(const uint32_t* __restrict__ L, const uint32_t* lcount,
+__global__ __launch_bounds__(WG) void k_tail_split(uint32_t n, const uint32_t* __restrict__ L, const uint32_t* lcount,
                                                    const uint64_t* __restrict__ fw_off,
                                                    const uint32_t* __restrict__ fw_adj,
                                                    const uint64_t* __restrict__ bw_off,
                                                    const uint32_t* __restrict__ bw_adj, uint32_t* live,
                                                    uint32_t* __restrict__ labels, uint32_t* __restrict__ rest,
-                                                   uint32_t* rest_cnt) {
+                                                   uint32_t* rest_cnt, uint32_t* bad) {
   constexpr uint32_t GL = 16;              // lanes per vertex
   constexpr uint32_t VPW = WAVE / GL;      // vertices per wavefront
   constexpr uint32_t VPB = VPW * (WG / WAVE);  // vertices per workgroup and step
@@ -2280,8 +2284,10 @@ __global__ __launch_bounds__(WG) void k_tail_split(const uint32_t* __restrict__ 
   const uint32_t cnt_pad = (cnt + VPB - 1) / VPB * VPB;
   for (uint32_t base = blockIdx.x * VPB; base < cnt_pad; base += gridDim.x * VPB) {
     const uint32_t i = base + threadIdx.x / GL;
-    const bool valid = i < cnt;
-    const uint32_t v = valid ? L[i] : 0u;
+    const uint32_t li = i < cnt ? L[i] : 0u;
+    if (i < cnt && li >= n && gl == 0) *bad = 1;  // not a vertex: reported, never used as an index
+    const bool valid = i < cnt && li < n;
+    const uint32_t v = valid ? li : 0u;
     uint64_t fb = 0, fe = 0, bb = 0, be = 0;
     if (valid) {
       fb = fw_off[v];
@@ -2363,7 +2369,7 @@ __global__ __launch_bounds__(1024) void k_tail_scan(const uint32_t* ucount, uint
 // The sub-CSR's two adjacency arrays share `sub` (cap entries): forward
 // at 0, backward after the forward total fo[u] and one spare entry. Nothing
 // is done when *skip is set; when they do not fit, *room is set instead.
-__global__ __launch_bounds__(WG) void k_tail_fill(const uint32_t* __restrict__ U, const uint32_t* ucount,
+__global__ __launch_bounds__(WG) void k_tail_fill(uint32_t n, const uint32_t* __restrict__ U, const uint32_t* ucount,
                                                   const uint64_t* __restrict__ fw_off,
                                                   const uint32_t* __restrict__ fw_adj,
                                                   const uint64_t* __restrict__ bw_off,
@@ -2371,7 +2377,7 @@ __global__ __launch_bounds__(WG) void k_tail_fill(const uint32_t* __restrict__ U
                                                   const uint32_t* __restrict__ live,
                                                   const uint32_t* __restrict__ subid, const uint64_t* __restrict__ fo,
                                                   const uint64_t* __restrict__ bo, uint32_t* __restrict__ sub,
-                                                  uint64_t cap, const uint32_t* skip, uint32_t* room) {
+                                                  uint64_t cap, const uint32_t* skip, uint32_t* room, uint32_t* bad) {
   const uint32_t u = *ucount;
   if (*skip) return;
   const uint64_t sm = fo[u];
@@ -2387,6 +2393,10 @@ __global__ __launch_bounds__(WG) void k_tail_fill(const uint32_t* __restrict__ U
   // one wavefront per vertex; live neighbours in row order
   for (uint32_t i = blockIdx.x * wpb + threadIdx.x / WAVE; i < u; i += gridDim.x * wpb) {
     const uint32_t v = U[i];
+    if (v >= n) {  // not a vertex: reported, never used as an index
+      if (lane == 0) *bad = 1;
+      continue;
+    }
     uint64_t p = fo[i];
     for (uint64_t b = fw_off[v], end = fw_off[v + 1]; b < end; b += WAVE) {
       const uint64_t it = b + lane;
@@ -2824,6 +2834,7 @@ struct Prof {
       if (hipEventCreate(&ep.a) != hipSuccess || hipEventCreate(&ep.b) != hipSuccess) return;
     }
     ep.kern = kern;
+    ep.phase = c->cur_phase;
     ep.bytes = bytes;
     (void)hipEventRecord(ep.a, s);
     c->pending.push_back(ep);
@@ -2844,6 +2855,10 @@ void prof_collect(Ctx* c) {
       c->kt[ep.kern].launches++;
       c->kt[ep.kern].ms += ms;
       c->kt[ep.kern].bytes += ep.bytes;
+      if (ep.phase >= 0 && ep.phase < KSPN_PHASE_MAX) {
+        c->phase_bytes[ep.phase] += ep.bytes;
+        c->phase_kms[ep.phase] += ms;
+      }
     }
     c->free_events.push_back(ep.a);
     c->free_events.push_back(ep.b);
@@ -3220,7 +3235,9 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
     const uint32_t grid =
         std::min<uint32_t>(stream_grid(c, R.n), (uint32_t)c->num_cu * (R.n <= (1u << 25) ? 8 : 16));
     {
-      Prof p(c, R.s, dir == 0 ? K_PULL_FW : K_PULL_BW, (double)R.n / 4.0 + 16.0 * R.live_n);
+      // candidate and visited bits of every vertex; the offsets of each
+      // candidate and the scanned edges are added once they are published
+      Prof p(c, R.s, dir == 0 ? K_PULL_FW : K_PULL_BW, (double)R.n / 4.0);
       if (prof) *prof = p.idx;
       // no out-degree sum of the claimed vertices: it would read a third
       // offset array at scattered rows, and the first push level after the
@@ -3350,8 +3367,10 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       if (swept && left > list_max) batch = false;
       list_level = batch;
       if (c->profile) {
-        // algorithmic bytes: per scanned edge an id and a bitmap bit; per list entry its id, offsets, out entry
-        if (sweep_prof >= 0) c->pending[sweep_prof].bytes += 4.125 * (double)c->h_u64[1];
+        // algorithmic bytes: per scanned edge an id and a bitmap bit; per sweep candidate (claimed or
+        // left) its two row offsets; per list entry its id, offsets, out entry
+        if (sweep_prof >= 0)
+          c->pending[sweep_prof].bytes += 4.125 * (double)c->h_u64[1] + 16.0 * ((double)c->h_cnt[nc] + (double)left);
         if (list_prof >= 0)
           c->pending[list_prof].bytes += 4.125 * (double)c->h_u64[3] + (batch ? 24.0 * (swept ? left : list_in) : 0.0);
       }
@@ -3859,15 +3878,26 @@ int residual(Run& R, uint32_t* decided) {
 // compact the live subgraph (both directions) and colour it in one workgroup
 // the sub-CSR's adjacency and the solver, for u live vertices whose compact
 // ids, offsets (scan, sub_off) and live list (ulist) are built; publishes
+// A tail kernel met a listed id >= n (cnt[C_BADID], published): the lists
+// that feed the tail are broken, so the solve stops with an error instead
+// of indexing with the id. The flag is cleared for the next solve.
+int tail_check_ids(Run& R) {
+  if (!R.c->h_cnt[C_BADID]) return KSPN_OK;
+  k_set_u32<<<1, 64, 0, R.s>>>(&R.c->cnt[C_BADID], 0u);
+  note_error("tail", "a listed vertex id is not below n", __FILE__, __LINE__);
+  return KSPN_E_INVALID;
+}
+
 int tail_finish(Run& R, uint32_t u) {
   Ctx* c = R.c;
   const kspn_graph* g = R.g;
   uint32_t* subid = c->col;
   {
     Prof p(c, R.s, K_RESIDUAL, 32.0 * u);
-    k_tail_fill<<<stream_grid(c, (uint64_t)u * WAVE), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, g->bw_off,
-                                                   g->bw_adj, c->live, subid, c->scan, c->sub_off, c->sub_adj,
-                                                   c->cap_sub_m, &c->cnt[C_ERR], &c->cnt[C_ROOM]);
+    k_tail_fill<<<stream_grid(c, (uint64_t)u * WAVE), WG, 0, R.s>>>(R.n, c->ulist, &c->cnt[C_RES_N], g->fw_off,
+                                                                   g->fw_adj, g->bw_off, g->bw_adj, c->live, subid,
+                                                                   c->scan, c->sub_off, c->sub_adj, c->cap_sub_m,
+                                                                   &c->cnt[C_ERR], &c->cnt[C_ROOM], &c->cnt[C_BADID]);
   }
   uint32_t cap = R.o.max_rounds ? R.o.max_rounds - R.rounds : 0;  // > 0: checked by the caller
   {
@@ -3897,9 +3927,10 @@ int tail_solve(Run& R, uint32_t* decided) {
   if (u < 4096) {
     // compact ids, degrees, both scans and the solver's flag: two launches
     Prof p(c, R.s, K_RESIDUAL, 24.0 * u);
-    k_tail_prep<<<stream_grid(c, (uint64_t)u * WAVE), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], 4095u, g->fw_off,
-                                                                g->fw_adj, g->bw_off, g->bw_adj, c->live, subid,
-                                                                c->scan, c->sub_off, &c->cnt[C_ISO]);
+    k_tail_prep<<<stream_grid(c, (uint64_t)u * WAVE), WG, 0, R.s>>>(R.n, c->ulist, &c->cnt[C_RES_N], 4095u,
+                                                                   g->fw_off, g->fw_adj, g->bw_off, g->bw_adj,
+                                                                   c->live, subid, c->scan, c->sub_off,
+                                                                   &c->cnt[C_ISO], &c->cnt[C_BADID]);
     k_tail_scan<<<1, 1024, 0, R.s>>>(&c->cnt[C_RES_N], 4095u, c->scan, c->sub_off, &c->cnt[C_FLAG], &c->cnt[C_ERR]);
   } else {
     {
@@ -3918,6 +3949,7 @@ int tail_solve(Run& R, uint32_t* decided) {
     k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_FLAG], 0u);
   }
   KSPN_TRY(publish(c, R.s, c->scan + u));
+  KSPN_TRY(tail_check_ids(R));
   // the compaction's own count (the host's live count should agree)
   if (c->h_cnt[C_ERR] || c->h_cnt[C_RES_N] != u) {
     note_error("tail_solve", "live count differs from the live bitmap", __FILE__, __LINE__);
@@ -3929,6 +3961,7 @@ int tail_solve(Run& R, uint32_t* decided) {
     c->cap_sub_m = 2 * sm + 2;
   }
   KSPN_TRY(tail_finish(R, u));
+  KSPN_TRY(tail_check_ids(R));
   if (c->h_cnt[C_FLAG]) return KSPN_E_ITER_CAP;
   if (R.st) {
     R.st->residual_n = u;
@@ -3954,6 +3987,7 @@ int tail_spec(Run& R, uint32_t limit, uint32_t* fwbw_decided, uint32_t* decided,
   *done = false;
   *decided = *iso = 0;
   uint32_t* subid = c->col;
+  int prof_split = -1, prof_prep = -1, prof_fill = -1, prof_tail = -1;
   if (!c->sub_off) KSPN_TRY(dalloc(&c->sub_off, (uint64_t)c->cap_n + 64));
   if (c->cap_sub_m < 2) {
     KSPN_TRY(dalloc(&c->sub_adj, 1u << 16));
@@ -3963,9 +3997,12 @@ int tail_spec(Run& R, uint32_t limit, uint32_t* fwbw_decided, uint32_t* decided,
     // every live vertex (qa, listed by fwbw's k_fwbw_finish, which also
     // zeroed the counters below), then trim_1_normal keeps those with a live
     // neighbour (ulist)
-    Prof p(c, R.s, K_TRIM_SWEEP, 20.0 * limit);
-    k_tail_split<<<(uint32_t)c->num_cu * 8, WG, 0, R.s>>>(c->qa, &c->cnt[C_LIVE], g->fw_off, g->fw_adj, g->bw_off,
-                                                          g->bw_adj, c->live, R.labels, c->ulist, &c->cnt[C_RES_N]);
+    // 20 bytes per listed vertex (id, two offset pairs); set once the list's length is known
+    Prof p(c, R.s, K_TRIM_SWEEP, 0.0);
+    prof_split = p.idx;
+    k_tail_split<<<(uint32_t)c->num_cu * 8, WG, 0, R.s>>>(R.n, c->qa, &c->cnt[C_LIVE], g->fw_off, g->fw_adj,
+                                                          g->bw_off, g->bw_adj, c->live, R.labels, c->ulist,
+                                                          &c->cnt[C_RES_N], &c->cnt[C_BADID]);
   }
   {
     if (!R.label_pending) {
@@ -3977,32 +4014,49 @@ int tail_spec(Run& R, uint32_t limit, uint32_t* fwbw_decided, uint32_t* decided,
     // label) in the same launch as the tail's degree pass
     const uint32_t lblocks = stream_grid(c, R.n / 8 + 1);
     const uint32_t pblocks = stream_grid(c, (uint64_t)limit * WAVE);
-    Prof p(c, R.s, K_LABEL, (R.label_dense ? R.n / 4.0 + 4.0 * R.n : R.n / 8.0) + 24.0 * limit);
+    Prof p(c, R.s, K_LABEL, R.label_dense ? R.n / 4.0 + 4.0 * R.n : R.n / 8.0);
+    prof_prep = p.idx;  // + 24 bytes per tail vertex, added below
     k_label_prep<<<lblocks + pblocks, WG, 0, R.s>>>(lblocks, R.n, c->bwv, c->part, R.label_nparts, &c->cnt[C_MIN],
                                                     &c->cnt[C_DECIDED], R.labels, R.label_dense ? c->live : nullptr,
                                                     c->ulist, &c->cnt[C_RES_N], limit, g->fw_off, g->fw_adj,
-                                                    g->bw_off, g->bw_adj, c->live, subid, c->scan, c->sub_off);
+                                                    g->bw_off, g->bw_adj, c->live, subid, c->scan, c->sub_off,
+                                                    &c->cnt[C_BADID]);
   }
   {
     Prof p(c, R.s, K_RESIDUAL, 0.0);
+    prof_fill = p.idx;
     k_tail_scan<<<1, 1024, 0, R.s>>>(&c->cnt[C_RES_N], limit, c->scan, c->sub_off, &c->cnt[C_FLAG], &c->cnt[C_ERR]);
-    k_tail_fill<<<stream_grid(c, (uint64_t)limit * WAVE), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->fw_off, g->fw_adj, g->bw_off,
-                                                       g->bw_adj, c->live, subid, c->scan, c->sub_off, c->sub_adj,
-                                                       c->cap_sub_m, &c->cnt[C_ERR], &c->cnt[C_ROOM]);
+    k_tail_fill<<<stream_grid(c, (uint64_t)limit * WAVE), WG, 0, R.s>>>(R.n, c->ulist, &c->cnt[C_RES_N], g->fw_off,
+                                                                       g->fw_adj, g->bw_off, g->bw_adj, c->live,
+                                                                       subid, c->scan, c->sub_off, c->sub_adj,
+                                                                       c->cap_sub_m, &c->cnt[C_ERR], &c->cnt[C_ROOM],
+                                                                       &c->cnt[C_BADID]);
   }
   const uint32_t cap = R.o.max_rounds ? R.o.max_rounds - R.rounds : 0;
   {
-    Prof p(c, R.s, K_TAIL, 24.0 * limit);
+    Prof p(c, R.s, K_TAIL, 0.0);
+    prof_tail = p.idx;
     k_tail_solve<<<1, 1024, 0, R.s>>>(&c->cnt[C_RES_N], c->ulist, subid, c->scan, c->sub_off, c->sub_adj, c->cap_sub_m,
                                       c->qa, c->qb, c->qc, R.labels, R.rotation, cap, &c->cnt[C_FLAG], c->live,
                                       &c->cnt[C_ERR], &c->cnt[C_ROOM]);
   }
   KSPN_TRY(read_counters(c, R.s, C_COUNT_));
+  KSPN_TRY(tail_check_ids(R));
   *fwbw_decided = c->h_cnt[C_DECIDED];
   R.live_n -= *fwbw_decided;
   *iso = c->h_cnt[C_LIVE] - c->h_cnt[C_RES_N];  // the split decides every listed vertex it does not keep
   R.live_n -= *iso;
   const uint32_t u = c->h_cnt[C_RES_N];
+  if (c->profile) {
+    // algorithmic bytes once the sizes are known: the split reads each listed vertex's id and offsets;
+    // the degree pass each tail vertex's id and offsets and writes its sub id; the scans read and write
+    // two arrays of u+1 offsets and the fill reads them again; the solver reads each vertex's state
+    const double listed = c->h_cnt[C_LIVE], tail = c->h_cnt[C_ERR] ? 0.0 : (double)u;
+    if (prof_split >= 0) c->pending[prof_split].bytes += 20.0 * listed;
+    if (prof_prep >= 0) c->pending[prof_prep].bytes += 24.0 * tail;
+    if (prof_fill >= 0) c->pending[prof_fill].bytes += 48.0 * (tail + 1.0);
+    if (prof_tail >= 0) c->pending[prof_tail].bytes += 24.0 * tail;
+  }
   if (c->h_cnt[C_ERR]) {
     // more than `limit` left after trim_1_normal: nothing else was solved
     k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_ERR], 0u);
@@ -4020,6 +4074,7 @@ int tail_spec(Run& R, uint32_t limit, uint32_t* fwbw_decided, uint32_t* decided,
     c->cap_sub_m = 2 * sm + 2;
     k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_ROOM], 0u);
     KSPN_TRY(tail_finish(R, u));
+    KSPN_TRY(tail_check_ids(R));
   }
   if (c->h_cnt[C_FLAG]) return KSPN_E_ITER_CAP;
   if (R.st) R.st->residual_n = u;
@@ -4113,6 +4168,7 @@ struct Phases {
     }
     if (idx >= 0) st->phase[idx].entries++;
     cur = idx;
+    c->cur_phase = idx;
     return mark(idx);
   }
   void decided(uint64_t d) {
@@ -4139,8 +4195,11 @@ struct Phases {
       }
     for (auto& m : marks)
       if (m.first == cur) m.first = prev;
+    for (auto& ep : c->pending)
+      if (ep.phase == cur) ep.phase = prev;
     --st->n_phase;
     cur = prev;
+    c->cur_phase = prev;
   }
   // closes the open scope; `hidden` starts an interval outside every scope
   // (the batch's label localisation), timed into ms_canon
@@ -4148,6 +4207,7 @@ struct Phases {
     if (!st) return KSPN_OK;
     close_scope();
     cur = -1;
+    c->cur_phase = -1;
     return mark(hidden ? -2 : -1);
   }
   // scope times, and the coarse ms_* fields as sums of scopes
@@ -4585,8 +4645,11 @@ int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* o
   KSPN_TRY(ctx_reserve(c, g->n, g->m));
   hipStream_t s = (hipStream_t)stream;
   c->profile = stats && (o.flags & KSPN_FLAG_PROFILE);
-  if (c->profile)
+  c->cur_phase = -1;
+  if (c->profile) {
     for (auto& t : c->kt) { t.launches = 0; t.ms = 0; t.bytes = 0; }
+    for (int i = 0; i < KSPN_PHASE_MAX; ++i) c->phase_bytes[i] = c->phase_kms[i] = 0;
+  }
   Run R{c, g, labels, o, stats, s, g->n, bm_words(g->n)};
   R.coop_edges = o.coop_edges ? o.coop_edges : kCoopEdges;
   R.coop_deg = o.coop_degree ? o.coop_degree : kCoopDeg;
@@ -4614,6 +4677,21 @@ int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* o
       ks.bytes = c->kt[i].bytes;
     }
     stats->n_kern = nk;
+    // algorithmic bytes and launch time per scope, and the coarse bytes_* as sums of scopes
+    for (uint32_t i = 0; i < stats->n_phase && i < (uint32_t)KSPN_PHASE_MAX; ++i) {
+      kspn_phase& ph = stats->phase[i];
+      ph.bytes = c->phase_bytes[i];
+      ph.kernel_ms = c->phase_kms[i];
+      const char* nm = ph.name;
+      if (!std::strcmp(nm, "trim_1_first") || !std::strcmp(nm, "trim_ex_first") || !std::strcmp(nm, "trim_2"))
+        stats->bytes_trim += ph.bytes;
+      else if (!std::strcmp(nm, "color") || !std::strcmp(nm, "color_trim_ex"))
+        stats->bytes_color += ph.bytes;
+      else if (!std::strcmp(nm, "residual"))
+        stats->bytes_residual += ph.bytes;
+      else
+        stats->bytes_fwbw += ph.bytes;
+    }
     c->profile = false;
   }
   return rc;
@@ -4729,3 +4807,58 @@ int kspn_subgraph(kspn_ctx* k, uint32_t n, const uint64_t* off, const uint32_t* 
 
 }  // extern "C"
 
+
+// ---------------------------------------------------------------- test hook
+extern "C" int kspn_test_tail_guard(kspn_ctx* k, const kspn_graph* g, uint32_t n_limit, const uint32_t* list,
+                                    uint32_t count, uint32_t* flagged, void* stream) {
+  if (!k || !g || !list || !flagged || n_limit > g->n || count == 0 || count > g->n || count > 4095)
+    return KSPN_E_INVALID;
+  *flagged = 0;
+  Ctx* c = &k->c;
+  if (hipSetDevice(c->device) != hipSuccess) return KSPN_E_HIP;
+  KSPN_TRY(ctx_reserve(c, g->n, g->m));
+  hipStream_t s = (hipStream_t)stream;
+  if (!c->sub_off) KSPN_TRY(dalloc(&c->sub_off, (uint64_t)c->cap_n + 64));
+  if (c->cap_sub_m < 2 * g->m + 2) {
+    KSPN_TRY(dalloc(&c->sub_adj, 2 * g->m + 2));
+    c->cap_sub_m = 2 * g->m + 2;
+  }
+  uint32_t* labels = nullptr;
+  KSPN_TRY(dalloc(&labels, (uint64_t)g->n));
+  const uint64_t words = bm_words(g->n);
+  int rc = KSPN_OK;
+  // every vertex live; the list in qa with its count in C_LIVE, as the tail split finds them
+  auto run = [&]() -> int {
+    KSPN_HIP(hipMemsetAsync(c->live, 0xFF, words * 4, s));
+    KSPN_HIP(hipMemcpyAsync(c->qa, list, sizeof(uint32_t) * count, hipMemcpyDeviceToDevice, s));
+    KSPN_HIP(hipMemcpyAsync(c->ulist, list, sizeof(uint32_t) * count, hipMemcpyDeviceToDevice, s));
+    k_set_u32<<<1, 64, 0, s>>>(&c->cnt[C_LIVE], count);
+    k_set_u32<<<1, 64, 0, s>>>(&c->cnt[C_AUX], count);
+    const uint32_t kernels[3] = {1, 2, 4};
+    for (uint32_t which : kernels) {
+      k_zero3<<<1, 64, 0, s>>>(&c->cnt[C_BADID], &c->cnt[C_RES_N], &c->cnt[C_ROOM]);
+      k_set_u32<<<1, 64, 0, s>>>(&c->cnt[C_ERR], 0u);
+      if (which == 1)
+        k_tail_split<<<8, WG, 0, s>>>(n_limit, c->qa, &c->cnt[C_LIVE], g->fw_off, g->fw_adj, g->bw_off, g->bw_adj,
+                                      c->live, labels, c->qb, &c->cnt[C_RES_N], &c->cnt[C_BADID]);
+      else if (which == 2)
+        k_tail_prep<<<8, WG, 0, s>>>(n_limit, c->ulist, &c->cnt[C_AUX], 4095u, g->fw_off, g->fw_adj, g->bw_off,
+                                     g->bw_adj, c->live, c->col, c->scan, c->sub_off, nullptr, &c->cnt[C_BADID]);
+      else {
+        KSPN_HIP(hipMemsetAsync(c->scan, 0, sizeof(uint64_t) * (count + 1), s));
+        KSPN_HIP(hipMemsetAsync(c->sub_off, 0, sizeof(uint64_t) * (count + 1), s));
+        k_tail_fill<<<8, WG, 0, s>>>(n_limit, c->ulist, &c->cnt[C_AUX], g->fw_off, g->fw_adj, g->bw_off, g->bw_adj,
+                                     c->live, c->col, c->scan, c->sub_off, c->sub_adj, c->cap_sub_m, &c->cnt[C_ERR],
+                                     &c->cnt[C_ROOM], &c->cnt[C_BADID]);
+      }
+      KSPN_TRY(publish(c, s, nullptr));
+      if (c->h_cnt[C_BADID]) *flagged |= which;
+    }
+    k_set_u32<<<1, 64, 0, s>>>(&c->cnt[C_BADID], 0u);
+    KSPN_HIP(hipStreamSynchronize(s));
+    return KSPN_OK;
+  };
+  rc = run();
+  (void)hipFree(labels);
+  return rc;
+}

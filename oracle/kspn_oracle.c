@@ -12,6 +12,8 @@
  * Parity pins (what checks each function; see tests/test_oracle.py):
  *   orc_tarjan          golden vectors (iSpan tiny graph), planted labels,
  *                       brute force on small graphs, closed forms
+ *   orc_tarjan_batch    equal to orc_tarjan on the union, made local per graph
+ *   orc_planted_labels  equal to the labels orc_gen_planted writes
  *   orc_canonicalize    hand-made cases; equality with Tarjan's min ids
  *   orc_brute_force     closed forms (cycle, DAG, empty graph, n=1)
  *   orc_transpose       golden vector test_allgather_sub_graph.cpp:33-72,
@@ -28,6 +30,7 @@
 #include "kspn_oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -408,6 +411,28 @@ uint64_t orc_gen_planted(uint64_t seed, uint32_t n, double expo, uint32_t max_si
   return k;
 }
 
+/* The labels orc_gen_planted writes, without its edges: positions laid out
+ * component by component, the same bijection of positions to ids, and the
+ * minimum id of each component as its label. */
+void orc_planted_labels(uint64_t seed, uint32_t n, double expo, uint32_t max_size, uint32_t* labels) {
+  uint32_t* sizes = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint64_t ncomp = orc_planted_sizes(seed, n, expo, max_size, sizes, n);
+  uint32_t bits = 0;
+  while (bits < 32 && (1ull << bits) < n) ++bits;
+  uint64_t pseed = mix64(seed ^ 0x544F504F4C4F4759ull);
+  uint64_t s = 0;
+  for (uint64_t c = 0; c < ncomp; ++c) {
+    uint32_t len = sizes[c], minid = 0xFFFFFFFFu;
+    for (uint32_t q = 0; q < len; ++q) {
+      uint32_t id = perm_n(pseed, bits, n, (uint32_t)(s + q));
+      if (id < minid) minid = id;
+    }
+    for (uint32_t q = 0; q < len; ++q) labels[perm_n(pseed, bits, n, (uint32_t)(s + q))] = minid;
+    s += len;
+  }
+  free(sizes);
+}
+
 /* ======================================================================== */
 /* CSR                                                                      */
 /* ======================================================================== */
@@ -501,7 +526,17 @@ uint64_t orc_subgraph(uint32_t n, const uint64_t* off, const uint32_t* adj, cons
  * each SCC is labelled with its minimum member (the callers' convention,
  * scc_trim_ex_residual.hpp:115-123, test_tarjan_vs_kaspan.cpp:37-45).
  * Returns the number of components. */
+/* Tarjan over rows off[0..n] whose entries are ids shifted by `base` (the
+ * rows of one graph of a batch union: off may point into the union's
+ * offsets, adj is the union's adjacency, and an entry x names vertex
+ * x - base). labels are the graph's own ids. */
+static uint32_t tarjan_impl(uint32_t n, const uint64_t* off, const uint32_t* adj, uint32_t base, uint32_t* labels);
+
 uint32_t orc_tarjan(uint32_t n, const uint64_t* off, const uint32_t* adj, uint32_t* labels) {
+  return tarjan_impl(n, off, adj, 0, labels);
+}
+
+static uint32_t tarjan_impl(uint32_t n, const uint64_t* off, const uint32_t* adj, uint32_t base, uint32_t* labels) {
   uint32_t* index = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
   uint32_t* low = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
   uint8_t* on_stack = (uint8_t*)calloc(n ? n : 1, 1);
@@ -519,7 +554,7 @@ uint32_t orc_tarjan(uint32_t n, const uint64_t* off, const uint32_t* adj, uint32
       uint32_t u = fu[fp - 1];
       uint64_t it = fit[fp - 1];
       if (it < off[u + 1]) {
-        uint32_t v = adj[it];
+        uint32_t v = adj[it] - base;
         fit[fp - 1] = it + 1;
         if (index[v] == ORC_UNDECIDED) {
           index[v] = low[v] = count++;
@@ -560,6 +595,43 @@ void orc_canonicalize(uint32_t n, uint32_t* labels) {
     labels[u] = first[l];
   }
   free(first);
+}
+
+/* Tarjan for every graph of a batch union (graph i owns vertices
+ * [vtx_off[i], vtx_off[i+1]) and its rows point inside that range), on
+ * `threads` POSIX threads that take graphs from a shared counter. labels[v]
+ * is the minimum local id of v's SCC inside its own graph. */
+struct batch_job {
+  uint32_t count;
+  const uint32_t* vtx_off;
+  const uint64_t* off;
+  const uint32_t* adj;
+  uint32_t* labels;
+  uint32_t next;
+};
+
+static void* batch_worker(void* arg) {
+  struct batch_job* j = (struct batch_job*)arg;
+  for (;;) {
+    uint32_t i = __atomic_fetch_add(&j->next, 1u, __ATOMIC_RELAXED);
+    if (i >= j->count) break;
+    uint32_t b = j->vtx_off[i], e = j->vtx_off[i + 1];
+    tarjan_impl(e - b, j->off + b, j->adj, b, j->labels + b);
+  }
+  return NULL;
+}
+
+void orc_tarjan_batch(uint32_t count, const uint32_t* vtx_off, const uint64_t* off, const uint32_t* adj,
+                      uint32_t* labels, uint32_t threads) {
+  struct batch_job job = {count, vtx_off, off, adj, labels, 0};
+  if (threads < 1) threads = 1;
+  if (threads > 64) threads = 64;
+  pthread_t th[64];
+  uint32_t started = 0;
+  for (uint32_t t = 1; t < threads; ++t)
+    if (pthread_create(&th[started], NULL, batch_worker, &job) == 0) ++started;
+  batch_worker(&job);
+  for (uint32_t t = 0; t < started; ++t) pthread_join(th[t], NULL);
 }
 
 /* u ~ v iff v in reach(u) and u in reach(v); reach sets by BFS from every

@@ -48,6 +48,8 @@ void orc_batch_plan(uint64_t seed, uint32_t count, uint32_t n_min, uint32_t n_ma
 uint64_t orc_gen_planted(uint64_t seed, uint32_t n, double expo, uint32_t max_size, double intra_deg,
                          double inter_deg, uint32_t* src, uint32_t* dst, uint64_t cap,
                          uint32_t* labels);
+/* labels of orc_gen_planted without generating its edges */
+void orc_planted_labels(uint64_t seed, uint32_t n, double expo, uint32_t max_size, uint32_t* labels);
 uint64_t orc_planted_sizes(uint64_t seed, uint32_t n, double expo, uint32_t max_size, uint32_t* sizes,
                            uint64_t cap);
 
@@ -65,6 +67,9 @@ uint64_t orc_subgraph(uint32_t n, const uint64_t* off, const uint32_t* adj, cons
 /* ---- SCC ------------------------------------------------------------------ */
 /* iterative Tarjan, scc/tarjan.hpp:20-122; labels = minimum member id */
 uint32_t orc_tarjan(uint32_t n, const uint64_t* off, const uint32_t* adj, uint32_t* labels);
+/* orc_tarjan for each graph of a batch union (vtx_off[count+1]); local labels; on `threads` threads */
+void orc_tarjan_batch(uint32_t count, const uint32_t* vtx_off, const uint64_t* off, const uint32_t* adj,
+                      uint32_t* labels, uint32_t threads);
 /* normalise_scc_id, test/normalise_scc.hpp:10-25 */
 void orc_canonicalize(uint32_t n, uint32_t* labels);
 /* mutual reachability by BFS from every vertex (n <= 4096) */

@@ -70,6 +70,8 @@ def lib():
             "orc_transpose": (None, [u32, _u64p, _u32p, _u64p, _u32p]),
             "orc_subgraph": (u64, [u32, _u64p, _u32p, _u8p, _u32p, C.POINTER(u32), _u64p, _u32p]),
             "orc_tarjan": (u32, [u32, _u64p, _u32p, _u32p]),
+            "orc_tarjan_batch": (None, [u32, _u32p, _u64p, _u32p, _u32p, u32]),
+            "orc_planted_labels": (None, [u64, u32, dbl, u32, _u32p]),
             "orc_canonicalize": (None, [u32, _u32p]),
             "orc_brute_force": (i32, [u32, _u64p, _u32p, _u32p]),
             "orc_trim_1_first": (u32, [u32, _u64p, _u64p, _u8p, _u32p]),
@@ -236,6 +238,24 @@ def tarjan(g: HostGraph) -> np.ndarray:
     labels = np.zeros(max(g.n, 1), np.uint32)
     lib().orc_tarjan(g.n, g.fw_off, _nonempty(g.fw_adj, np.uint32), labels)
     return labels[: g.n].copy()
+
+
+def tarjan_batch(vtx_off, fw_off, fw_adj, threads=16) -> np.ndarray:
+    """Tarjan labels of every graph of a batch union, local to each graph
+    (graph i owns vertices [vtx_off[i], vtx_off[i+1]))."""
+    vtx_off = np.ascontiguousarray(vtx_off, np.uint32)
+    n = int(vtx_off[-1])
+    labels = np.zeros(max(n, 1), np.uint32)
+    lib().orc_tarjan_batch(len(vtx_off) - 1, vtx_off, np.ascontiguousarray(fw_off, np.uint64),
+                           _nonempty(np.ascontiguousarray(fw_adj, np.uint32), np.uint32), labels, threads)
+    return labels[:n]
+
+
+def planted_labels(seed, n, expo=2.1, max_size=10_000_000) -> np.ndarray:
+    """the labels gen_planted returns, without generating the edges"""
+    labels = np.zeros(max(n, 1), np.uint32)
+    lib().orc_planted_labels(seed, n, expo, max_size, labels)
+    return labels[:n]
 
 
 def canonicalize(labels) -> np.ndarray:

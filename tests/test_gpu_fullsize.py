@@ -1,0 +1,146 @@
+"""GPU parity at the full sizes of the benchmark workloads.
+
+* planted SCCs, n = 5e7, component sizes up to 1e7: plain and learned
+  pivots at the benchmark's 32 FW-BW rounds (and the single-pivot method),
+  labels equal to the planted labels (known by construction, computed by
+  the oracle without the edges) and to the oracle's Tarjan;
+* the 256^3 lattice: persistent multi-level kernel on and off, plain and
+  learned pivots, labels equal to the oracle's Tarjan;
+* the batch of 4096 graphs (4.0e9 edges), and its plan extended to 4600
+  graphs (4.5e9 edges, so edge offsets pass 2^32): every graph's labels
+  equal the oracle's Tarjan of that graph alone.
+
+The reference's own procedure is test_mpi/test_tarjan_vs_kaspan.cpp:26-53:
+generate, solve with KaSpan, solve with Tarjan, compare normalised ids.
+"""
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
+
+C3_SEED = 0x6B5350AA00000002
+C4_SEED = 0x6B5350AA00000003
+C5_SEED = 0x6B5350AA00000004
+C3 = dict(n=50_000_000, max_size=10_000_000, expo=2.1, intra_deg=4.0, inter_deg=3.0)
+C5 = dict(n_min=1000, n_max=1_000_000, deg_min=2.0, deg_max=16.0, p_rmat=0.5)
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def progress(msg):
+    """a line on the process's own stderr (not captured), so long tests show their steps"""
+    sys.__stderr__.write(f"[fullsize {time.strftime('%H:%M:%S')}] {msg}\n")
+    sys.__stderr__.flush()
+
+
+def labels_np(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+def fw_tarjan(O, g):
+    """Tarjan on the forward CSR only (the reverse CSR stays on the device)"""
+    fo = g.fw_off.cpu().numpy().view(np.uint64)
+    fa = g.fw_adj.cpu().numpy().view(np.uint32)
+    return O.tarjan(O.HostGraph(g.n, fo, fa, fo, fa))
+
+
+@pytest.fixture
+def release():
+    import torch
+
+    yield
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+
+def test_c3_planted50m_full(ka, O, release):
+    from kaspan_amd import pivot as P
+
+    progress("C3: generating planted 5e7")
+    g = ka.generate("planted", C3_SEED, **C3)
+    assert g.n == 50_000_000 and g.m > 3 * g.n
+    want = O.planted_labels(C3_SEED, C3["n"], expo=C3["expo"], max_size=C3["max_size"])
+    assert np.array_equal(labels_np(ka.planted_labels(C3_SEED, C3["n"], max_size=C3["max_size"])), want)
+    mlp = P.load_params()
+    assert mlp is not None, "the trained scorer kaspan_amd/pivot_mlp_params.npy is part of the tree"
+    runs = [dict(fwbw_rounds=32), dict(fwbw_rounds=32, pivot=1, mlp_params=mlp), dict(fwbw_rounds=1)]
+    decided = []
+    for opts in runs:
+        progress(f"C3: solve {opts.get('fwbw_rounds')} rounds, pivot {opts.get('pivot', 0)}")
+        lab, st = ka.scc(g, **opts)
+        assert np.array_equal(labels_np(lab), want), {k: v for k, v in opts.items() if k != "mlp_params"}
+        assert st.decided_trim + st.decided_trim2 + st.decided_fwbw + st.decided_color + st.decided_residual == g.n
+        decided.append(st.decided_fwbw)
+    # the planted graph's own structure: Tarjan on the generated edges finds the planted components
+    progress("C3: oracle Tarjan")
+    assert np.array_equal(fw_tarjan(O, g), want)
+
+
+def test_c4_lattice256_full(ka, O, release):
+    from kaspan_amd import pivot as P
+
+    progress("C4: generating lattice 256^3")
+    g = ka.generate("lattice", C4_SEED, side=256, p_bidir=0.1)
+    assert g.n == 256 ** 3
+    tar = fw_tarjan(O, g)
+    progress("C4: oracle Tarjan done")
+    mlp = P.load_params()
+    coop_levels = []
+    for opts in (dict(), dict(flags=16), dict(pivot=1, mlp_params=mlp), dict(pivot=1, mlp_params=mlp, flags=16),
+                 dict(fwbw_rounds=16), dict(fwbw_rounds=16, pivot=1, mlp_params=mlp)):
+        lab, st = ka.scc(g, **opts)
+        assert np.array_equal(labels_np(lab), tar), {k: v for k, v in opts.items() if k != "mlp_params"}
+        coop_levels.append(st.fwbw_levels_coop)
+    # the persistent kernel ran where it was allowed and never where it was switched off
+    assert coop_levels[0] > 0 and coop_levels[1] == 0 and coop_levels[3] == 0
+
+
+def test_c5_batch4096_full(ka, O, release):
+    """The 4096-graph workload, and the same plan extended to 4600 graphs,
+    whose union holds more than 2^32 edges (4.48e9): the per-graph plan does
+    not depend on the count, so the 4096-graph workload is the prefix of
+    the larger batch (its arrays are views of the larger union's). Every
+    graph's labels equal the oracle's Tarjan of that graph alone, in both
+    solves."""
+    import torch
+
+    count = 4600
+    progress(f"C5: generating {count} graphs")
+    b = ka.generate_batch(C5_SEED, count, **C5)
+    vo = b.vtx_off.cpu().numpy().view(np.uint32)
+    eo = b.edge_off.cpu().numpy().view(np.uint64)
+    assert b.count == count and int(eo[-1]) == b.graph.m
+    # edge offsets cross 2^32 inside the union
+    assert int(eo[-1]) > 2 ** 32 and int(eo[4096]) < 2 ** 32
+    kind, n, draws, _ = O.batch_plan(C5_SEED, count, **C5)
+    assert np.array_equal(np.diff(vo.astype(np.int64)), n.astype(np.int64))
+    assert np.all(np.diff(eo.astype(np.int64)) <= draws.astype(np.int64))
+    # the 4096-graph workload as its prefix
+    u = b.graph
+    p = ka.BidiGraph.__new__(ka.BidiGraph)
+    p.n, p.m, p.device = int(vo[4096]), int(eo[4096]), u.device
+    p.fw_off, p.bw_off = u.fw_off[: p.n + 1], u.bw_off[: p.n + 1]
+    p.fw_adj, p.bw_adj = u.fw_adj[: p.m], u.bw_adj[: p.m]
+    c5 = ka.Batch(p, b.vtx_off[:4097], b.edge_off[:4097])
+    progress(f"C5: solving 4096 graphs (n={p.n}, m={p.m})")
+    lab5, st5 = ka.scc_batch(c5)
+    got5 = labels_np(lab5)
+    del lab5, c5, p
+    progress(f"C5: solving {count} graphs (n={u.n}, m={u.m})")
+    lab, st = ka.scc_batch(b)
+    got = labels_np(lab)
+    assert st.fwbw_rounds_done >= 1 and st5.fwbw_rounds_done >= 1
+    del lab, u
+    fo = b.graph.fw_off.cpu().numpy().view(np.uint64)
+    fa = b.graph.fw_adj.cpu().numpy().view(np.uint32)
+    del b
+    torch.cuda.empty_cache()
+    progress(f"C5: oracle Tarjan per graph on {THREADS} threads")
+    ref = O.tarjan_batch(vo, fo, fa, threads=THREADS)
+    bad = np.nonzero(got != ref)[0]
+    assert bad.size == 0, f"{bad.size} labels differ, first at union vertex {bad[0]}"
+    bad5 = np.nonzero(got5 != ref[: vo[4096]])[0]
+    assert bad5.size == 0, f"4096-graph batch: {bad5.size} labels differ, first at union vertex {bad5[0]}"

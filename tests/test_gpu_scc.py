@@ -395,3 +395,32 @@ def test_cpp_mirror(ka, tmp_path):
     out = subprocess.run([exe, mf], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "cpp ok" in out.stdout
+
+
+def test_tail_guard_reports_bad_ids(ka, O):
+    """The tail kernels check every listed id against n: an id >= n is
+    reported and skipped, never used as an index. The graph's arrays hold all
+    ids below 4096 and the kernels are told n = 2048, so the ids 2048..4095
+    in the list stay inside the arrays whatever the kernels do."""
+    import ctypes as C
+
+    import torch
+
+    from kaspan_amd import _lib
+
+    g = ka.generate("gnm", 5, n=4096, draws=16384)
+    ctx = ka.Context.default()
+
+    def flagged(ids):
+        lst = torch.tensor(np.array(ids, np.uint32).view(np.int32), device="cuda")
+        out = C.c_uint32(0)
+        gv = g.view()
+        _lib.check(_lib.lib().kspn_test_tail_guard(ctx.ptr, C.byref(gv), 2048, C.c_void_p(lst.data_ptr()), len(ids),
+                                                   C.byref(out), None), "kspn_test_tail_guard")
+        return out.value
+
+    assert flagged(list(range(0, 2048, 7))) == 0
+    assert flagged([3, 2048, 9, 4095, 11]) == 7
+    # the flag is cleared: the next solve works and its labels are right
+    lab, _ = gpu_labels(ka, g)
+    assert np.array_equal(lab, O.tarjan(host_graph(O, g)))

@@ -1,6 +1,7 @@
 """CPU checks of the C ABI: libkspn loads, exports every function the header
 declares, and the ctypes structures match the C layout. No GPU calls."""
 import ctypes as C
+import glob
 import os
 import re
 import subprocess
@@ -10,10 +11,11 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "kspn", "scc.h")
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "kspn", "*.h")))
 
 
 def declared_functions():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(kspn_\w+)\s*\(", text, re.M)))
 
 

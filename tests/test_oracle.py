@@ -310,3 +310,96 @@ def test_mlp_gradients_finite_differences(O):
     assert abs((lp - lm) / (2 * eps) - grads[5]) < 1e-6
     out = O.mlp_forward(x, *p, b3)
     assert abs(0.5 * np.mean((out - t) ** 2) - loss) < 1e-9
+
+
+def np_features(n, src, dst):
+    """The six pivot features from an edge list (duplicates and self-loops
+    kept, as rows of the CSR hold them), written as degree counts and
+    sparse matrix-vector products instead of the oracle's per-vertex loops:
+    log1p(in), log1p(out), log1p(in)*log1p(out)/8, log1p(A @ out),
+    log1p(A^T @ in), log1p(min(in, out)) with A[u, v] = #edges u -> v."""
+    src = np.asarray(src, np.int64)
+    dst = np.asarray(dst, np.int64)
+    outd = np.bincount(src, minlength=n).astype(np.float64)
+    ind = np.bincount(dst, minlength=n).astype(np.float64)
+    s2o = np.bincount(src, weights=outd[dst], minlength=n)  # (A @ out)[u] = sum over u -> w of out(w)
+    s2i = np.bincount(dst, weights=ind[src], minlength=n)   # (A^T @ in)[v] = sum over w -> v of in(w)
+    li, lo = np.log1p(ind), np.log1p(outd)
+    return np.stack([li, lo, li * lo / 8, np.log1p(s2o), np.log1p(s2i), np.log1p(np.minimum(ind, outd))], axis=1)
+
+
+def csr_with_multiedges(n, src, dst):
+    """CSR + reverse CSR that keep every listed edge (duplicates, self-loops)"""
+    fo, fa = H.csr_from_pairs(n, list(zip(src.tolist(), dst.tolist())))
+    bo, ba = H.csr_from_pairs(n, list(zip(dst.tolist(), src.tolist())))
+    return fo, fa, bo, ba
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_pivot_features_against_numpy(O, seed):
+    """orc_pivot_features == a NumPy restatement on graphs with self-loops and
+    duplicate edges, to float32 rounding (the oracle sums in integers and
+    takes log1pf in float32)"""
+    rng = np.random.default_rng(seed)
+    n = 300
+    src = rng.integers(0, n, 2000)
+    dst = rng.integers(0, n, 2000)
+    src = np.concatenate([src, src[:50], np.arange(20)])  # 50 duplicates, 20 self-loops
+    dst = np.concatenate([dst, dst[:50], np.arange(20)])
+    fo, fa, bo, ba = csr_with_multiedges(n, src, dst)
+    got = O.pivot_features(O.HostGraph(n, fo, fa, bo, ba)).astype(np.float64)
+    want = np_features(n, src, dst)
+    assert np.allclose(got, want, rtol=2e-6, atol=1e-6)
+    # the two 2-hop sums must not be interchangeable: swapping directions is caught
+    swapped = np_features(n, dst, src)
+    assert not np.allclose(got[:, 3:5], swapped[:, 3:5], rtol=1e-3)
+
+
+def test_pivot_features_closed_forms(O):
+    """out-star 0 -> 1..k and a 2-cycle 10 <-> 11: every feature by hand"""
+    k = 5
+    src = np.array([0] * k + [10, 11])
+    dst = np.array(list(range(1, k + 1)) + [11, 10])
+    fo, fa, bo, ba = csr_with_multiedges(12, src, dst)
+    f = O.pivot_features(O.HostGraph(12, fo, fa, bo, ba)).astype(np.float64)
+    l = np.log1p
+    assert np.allclose(f[0], [0, l(k), 0, 0, 0, 0], atol=1e-6)              # leaves have no out-edges
+    assert np.allclose(f[1], [l(1), 0, 0, 0, l(0), 0], atol=1e-6)           # the centre has in-degree 0
+    assert np.allclose(f[10], [l(1), l(1), l(1) ** 2 / 8, l(1), l(1), l(1)], atol=1e-6)
+    assert np.allclose(f[6], 0, atol=0)                                       # isolated vertex
+
+
+def test_mlp_forward_against_numpy(O):
+    """orc_mlp_forward == relu(relu(x W1^T + b1) W2^T + b2) w3 + b3 in float64 matrix form"""
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(257, 6)).astype(np.float32)
+    w1, b1, w2, b2, w3 = (rng.normal(scale=0.5, size=k) for k in (192, 32, 1024, 32, 32))
+    b3 = -0.3
+    got = O.mlp_forward(x, w1, b1, w2, b2, w3, b3)
+    h1 = np.maximum(x.astype(np.float64) @ w1.reshape(32, 6).T + b1, 0)
+    h2 = np.maximum(h1 @ w2.reshape(32, 32).T + b2, 0)
+    want = h2 @ w3 + b3
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+    # a transposed W2 is caught
+    h2t = np.maximum(h1 @ w2.reshape(32, 32) + b2, 0)
+    assert not np.allclose(got, h2t @ w3 + b3, rtol=1e-6)
+
+
+# ---------------------------------------------------------------- full-size helpers
+@pytest.mark.parametrize("seed,n,max_size", [(5, 1000, 100), (6, 20000, 5000), (0x6B5350AA00000002, 50000, 10000)])
+def test_planted_labels_without_edges(O, seed, n, max_size):
+    """planted_labels (no edges) == gen_planted's labels == Tarjan on its graph"""
+    g, lab = O.gen_planted(seed, n, max_size=max_size)
+    only = O.planted_labels(seed, n, max_size=max_size)
+    assert np.array_equal(only, lab)
+    assert np.array_equal(only, O.tarjan(g))
+
+
+def test_tarjan_batch_equals_tarjan_per_graph(O):
+    u, vo, eo, graphs = O.gen_batch(11, 40, n_min=20, n_max=3000, deg_min=1.0, deg_max=8.0)
+    got = O.tarjan_batch(vo, u.fw_off, u.fw_adj, threads=4)
+    want = np.concatenate([O.tarjan(g) for g in graphs])
+    assert np.array_equal(got, want)
+    lab_u = O.tarjan(u)
+    for i in range(40):
+        assert np.array_equal(got[vo[i]: vo[i + 1]], lab_u[vo[i]: vo[i + 1]] - vo[i])
