@@ -221,6 +221,19 @@ int kspn_subgraph(kspn_ctx* ctx, uint32_t n, const uint64_t* off, const uint32_t
 int kspn_tarjan(kspn_ctx* ctx, uint32_t n, uint64_t m, const uint64_t* off, const uint32_t* adj,
                 uint32_t* labels, void* stream);
 
+/* kspn_tarjan without any allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc). The work area
+ * (6 words per vertex) is a fixed device array of the library,
+ * 24 MiB + 256 B on every GPU it runs on, held until the process ends, so
+ * n may not exceed KSPN_TARJAN_NOALLOC_MAX_N (KSPN_E_INVALID otherwise).
+ * Runs on the calling thread's current device; stream-ordered, it returns
+ * after the launch without synchronising. Calls that overlap on one GPU
+ * share the work area and race: order them on one stream. */
+#define KSPN_TARJAN_NOALLOC_MAX_N (1u << 20)
+int kspn_tarjan_noalloc(uint32_t n, uint64_t m, const uint64_t* off, const uint32_t* adj, uint32_t* labels,
+                        void* stream);
+
 /* ---- learned pivot scorer (no reference counterpart; see KSPN_PIVOT_LEARNED) ----
  * MLP 6 -> 32 -> 32 -> 1 (ReLU). Parameter vector of KSPN_MLP_PARAMS floats:
  * W1[32][6], b1[32], W2[32][32], b2[32], w3[32], b3. Features per vertex:

@@ -47,6 +47,7 @@ MANIFEST_PATH = 4096
 GEN_GNM, GEN_RMAT, GEN_LATTICE, GEN_PLANTED = 0, 1, 2, 3
 MLP_PARAMS = 1313
 UNDECIDED = 0xFFFFFFFF
+TARJAN_NOALLOC_MAX_N = 1 << 20
 
 # every symbol include/kspn/*.h declares
 EXPORTS = (
@@ -55,7 +56,7 @@ EXPORTS = (
     "kspn_generate", "kspn_build_csr", "kspn_free", "kspn_planted_labels", "kspn_pivot_features",
     "kspn_mlp_forward", "kspn_mlp_grad", "kspn_mlp_adam", "kspn_scc_sizes", "kspn_scc_batch", "kspn_batch_plan",
     "kspn_generate_batch", "kspn_manifest_parse", "kspn_manifest_load", "kspn_manifest_graph", "kspn_unpack_dense",
-    "kspn_test_tail_guard",
+    "kspn_test_tail_guard", "kspn_tarjan_noalloc",
 )
 
 
@@ -249,6 +250,7 @@ def lib():
         "kspn_manifest_graph": (i32, [vp, P(Manifest), P(u32), P(u64), P(vp), P(vp), P(vp), P(vp), vp]),
         "kspn_unpack_dense": (i32, [vp, vp, u64, u32, u32, u64, vp, u32, vp]),
         "kspn_test_tail_guard": (i32, [vp, P(Graph), u32, vp, u32, P(u32), vp]),
+        "kspn_tarjan_noalloc": (i32, [u32, u64, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

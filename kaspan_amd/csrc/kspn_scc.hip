@@ -21,6 +21,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <mutex>
 #include <new>
 
 #include "kspn_internal.h"
@@ -4861,4 +4862,55 @@ extern "C" int kspn_test_tail_guard(kspn_ctx* k, const kspn_graph* g, uint32_t n
   rc = run();
   (void)hipFree(labels);
   return rc;
+}
+
+// ---------------------------------------------------------------- Tarjan without allocation
+// kspn_tarjan_noalloc keeps k_tarjan's work area (index, low, stack, frame
+// vertices and 64-bit frame iterators: 6 words per vertex and 2 of slack)
+// in one file-scope device array instead of context memory. Every device
+// that runs it holds its own copy for the life of the process. Its address
+// is taken once per device with that device current; host code passes only
+// that address, never the array's name (a host-side stand-in).
+namespace {
+constexpr uint32_t kTarjanStaticMaxN = KSPN_TARJAN_NOALLOC_MAX_N;
+constexpr uint64_t kTarjanStaticWords = 6ull * kTarjanStaticMaxN + 64;
+__device__ __align__(256) uint32_t scratch[kTarjanStaticWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
+// the calling device's copy of scratch (taken once per device)
+int scratch_for_device(int dev, uint32_t** out) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    note_error("kspn_tarjan_noalloc", "device index beyond the library's table", __FILE__, __LINE__);
+    return KSPN_E_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    KSPN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return KSPN_OK;
+}
+}  // namespace
+
+extern "C" int kspn_tarjan_noalloc(uint32_t n, uint64_t m, const uint64_t* off, const uint32_t* adj,
+                                   uint32_t* labels, void* stream) {
+  if ((n > 0 && (!off || !labels)) || (m > 0 && !adj)) return KSPN_E_INVALID;
+  if (n > kTarjanStaticMaxN) {
+    note_error("kspn_tarjan_noalloc", "n exceeds KSPN_TARJAN_NOALLOC_MAX_N", __FILE__, __LINE__);
+    return KSPN_E_INVALID;
+  }
+  if (n == 0) return KSPN_OK;
+  int dev = 0;
+  KSPN_HIP(hipGetDevice(&dev));
+  uint32_t* work = nullptr;
+  KSPN_TRY(scratch_for_device(dev, &work));
+  // k_tarjan initialises index[] itself and writes every other word before reading it
+  k_tarjan<<<1, 64, 0, (hipStream_t)stream>>>(n, off, adj, work, labels);
+  KSPN_HIP(hipGetLastError());
+  return KSPN_OK;
 }
