@@ -1,22 +1,26 @@
-"""Benchmark: edges/sec to full SCC labels on one MI355X.
+"""Benchmark: edges/sec to full SCC labels on one MI355X, learned vs plain pivots.
 
-Workload (default): directed R-MAT scale 24, edge factor 16 (268,435,456
-draws), a=0.57 b=0.19 c=0.19 d=0.05, ids randomly permuted, self-loops and
-duplicates removed, uint32 ids / uint64 offsets, CSR + reverse CSR resident
-in HBM. A step is one full solve (CSR in HBM -> canonical labels in HBM).
-value = m_final / seconds per step.
+Default workload (the metric's own comparison): planted SCCs with n = 5e7,
+component sizes from a power law (exponent 2.1, 1..1e7), each component a
+Hamiltonian cycle plus chords to out-degree 4, three forward inter-component
+edges per vertex; CSR + reverse CSR (uint32 ids, uint64 offsets) resident in
+HBM. A step is one full solve (CSR in HBM -> canonical labels in HBM). Both
+pivot rules are timed on the same graph, their labels must be identical, and
+value = m / seconds per step with the learned pivot; `ab` holds both times.
 
-The batch workload (--config batch4096) packs 4096 independent graphs (G(n,m)
-or R-MAT with probability 1/2, n log-uniform in [1e3, 1e6], average
-out-degree log-uniform in [2, 16]) into one union graph solved by one
-kspn_scc_batch call per step; it also reports graphs/sec and the p50/p99
-latency of solving sampled graphs one at a time.
+Other workloads (--config): gnm64k (G(n,m), n=65536), rmat24 (R-MAT scale
+24), lattice256 (3-D lattice 256^3), batch4096 (4096 graphs in one batch;
+also graphs/sec and the p50/p99 latency of sampled graphs solved alone).
+The default run also measures R-MAT scale 24 (plain pivot) under "extra".
 
-Prints one JSON line. `roofline` is measured by timing every kernel launch
-of the solve with HIP events recorded on the solve's stream (the library's
-profile mode) over --steps further solves; `cpu_baseline` is the oracle's
-single-thread restatement of kaspan::scc on the same graph, copied to the
-host.
+Prints one JSON line. `roofline` times every kernel launch of further solves
+with HIP events on the solve's stream (the library's profile mode); its
+`traffic` is the HBM bytes per launch of that kernel from the committed
+rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the workload. `phases` gives each
+statistics scope's algorithmic bytes (profile mode) over its time (HIP
+events per scope, no profiling). `cpu_baseline` is the oracle's
+single-thread restatement of kaspan::scc (lean form, and faithful form with
+the ghost-cache work of scc/scc.hpp:53,77,82) on a bounded sample.
 """
 import argparse
 import json
@@ -27,15 +31,30 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from kaspan_amd.workloads import CONFIGS, Workload  # noqa: E402  (no GPU work at import)
+
+METRIC = "edges/sec to full SCC labels on 1 MI355X; learned vs plain pivots, same labels"
 HBM_PEAK_GBS = 8000.0
 # library profile names -> kernel symbols as rocprofv3 reports them
 ROCPROF_NAMES = {"pull_fw": ["kspn::k_pull<0>"], "pull_bw": ["kspn::k_pull<1>"], "trim_first": ["kspn::k_trim_first"],
                  "expand_fw": ["kspn::k_expand<0>"], "expand_bw": ["kspn::k_expand<1>", "kspn::k_expand<5>"],
-                 "label_scc": ["kspn::k_label_bits", "kspn::k_label_prep", "kspn::k_part_apply"], "tail_solve": ["kspn::k_tail_solve"],
-                 "trim_sweep": ["kspn::k_trim_sweep<1>"],
+                 "label_scc": ["kspn::k_label_bits", "kspn::k_label_prep", "kspn::k_part_apply"],
+                 "tail_solve": ["kspn::k_tail_solve"], "trim_sweep": ["kspn::k_trim_sweep<1>"],
                  "reach_coop": ["kspn::k_reach_coop<0>", "kspn::k_reach_coop<1>", "kspn::k_reach_coop<5>"],
                  "bitmap_ops": ["kspn::k_part_classify"], "pivot_reduce": ["kspn::k_part_key"],
                  "pull_list": ["kspn::k_pull_list<0>", "kspn::k_pull_list<1>"]}
+
+METHOD_NAMES = ["kaspan", "trim_ex", "hpc_like", "ispan_like", "hpc_trim_ex", "trim_ex_residual",
+                "trim_ex_light_residual"]
+LATENCY_SAMPLE = 128  # graphs of the batch solved one at a time for p50/p99
+# FW-BW rounds when neither --fwbw-rounds nor a tuned file gives them: the planted workload's many
+# large SCCs want the partitioned multi-pivot rounds
+DEFAULT_ROUNDS = {"planted50m": 48}
+# bounded CPU-baseline samples for workloads whose full size takes the serial restatement too long
+CPU_SAMPLE = {
+    "planted50m": ("planted", dict(n=500_000, max_size=100_000, expo=2.1, intra_deg=4.0, inter_deg=3.0)),
+}
+CPU_MIN_S = 6.0  # each CPU form repeats its solve until this much time has passed (at most 1000 times)
 
 
 def pmc_traffic(config, kernel):
@@ -53,35 +72,187 @@ def pmc_traffic(config, kernel):
     return sum(k["bytes_per_launch"] * k["launches"] for k in found) / launches
 
 
-CONFIGS = {
-    # name: (generator kind, kwargs, seed, description)
-    "gnm64k": ("gnm", dict(n=65536, draws=524288), 0x6B5350AA00000000,
-               "G(n,m) n=65536 m=524288 draws, self-loops/duplicates removed"),
-    "rmat24": ("rmat", dict(scale=24, draws=16 << 24, a=0.57, b=0.19, c=0.19, permute=True), 0x6B5350AA00000001,
-               "R-MAT scale 24, edge factor 16, a=.57 b=.19 c=.19 d=.05, permuted ids, self-loops/duplicates removed"),
-    "planted50m": ("planted", dict(n=50_000_000, max_size=10_000_000, expo=2.1, intra_deg=4.0, inter_deg=3.0),
-                   0x6B5350AA00000002,
-                   "planted SCCs, n=5e7, power-law sizes (exponent 2.1, 1..1e7), Hamiltonian cycle + chords to "
-                   "out-degree 4 inside, 3 forward inter-component edges per vertex"),
-    "lattice256": ("lattice", dict(side=256, p_bidir=0.1), 0x6B5350AA00000003,
-                   "3-D lattice 256^3, each edge one random direction, 10% bidirectional"),
-    "batch4096": ("batch", dict(count=4096, n_min=1000, n_max=1_000_000, deg_min=2.0, deg_max=16.0, p_rmat=0.5),
-                  0x6B5350AA00000004,
-                  "4096 independent graphs in one batch: G(n,m) or R-MAT(.57/.19/.19/.05) with p=.5, n log-uniform "
-                  "in [1e3, 1e6], average out-degree log-uniform in [2, 16], deduplicated, concatenated CSR with "
-                  "per-graph vertex/edge offsets"),
-}
-METHOD_NAMES = ["kaspan", "trim_ex", "hpc_like", "ispan_like", "hpc_trim_ex", "trim_ex_residual",
-                "trim_ex_light_residual"]
-LATENCY_SAMPLE = 128  # graphs of the batch solved one at a time for p50/p99
-# FW-BW rounds when --fwbw-rounds is not given: the planted workload's many large SCCs want the
-# partitioned multi-pivot rounds (32 is the plain rule's best in a sweep of 1..64)
-DEFAULT_ROUNDS = {"planted50m": 32}
-# bounded CPU-baseline samples for workloads whose full size takes the serial restatement too long
-CPU_SAMPLE = {
-    "planted50m": ("planted", dict(n=500_000, max_size=100_000, expo=2.1, intra_deg=4.0, inter_deg=3.0)),
-}
-CPU_MIN_S = 10.0  # the CPU baseline repeats a solve until this much time has passed (at most 1000 times)
+def host_cpu():
+    """cores and model of the host the benchmark runs on"""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"nproc": avail, "cpu_count": os.cpu_count(), "model": model}
+
+
+def timed(wl, labels, opts, steps, warmup):
+    """ms per solve over `steps` solves after `warmup` (HIP events on the stream), and
+    the statistics of one further solve"""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        wl.solve(labels, **opts)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        wl.solve(labels, stats=False, **opts)
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / steps
+    _, st = wl.solve(labels, **opts)
+    return ms, st
+
+
+def profile(wl, labels, opts, steps):
+    """per-kernel and per-scope profile over `steps` solves in profile mode"""
+    agg, scopes = {}, {}
+    for _ in range(steps):
+        _, pst = wl.solve(labels, profile=True, **opts)
+        d = pst.as_dict()
+        for k in d["kernels"]:
+            a = agg.setdefault(k["name"], {"launches": 0, "ms": 0.0, "bytes": 0.0})
+            a["launches"] += k["launches"]
+            a["ms"] += k["ms"]
+            a["bytes"] += k["bytes"]
+        for ph in d["phases"]:
+            s = scopes.setdefault(ph["name"], {"bytes": 0.0, "kernel_ms": 0.0})
+            s["bytes"] += ph["bytes"]
+            s["kernel_ms"] += ph["kernel_ms"]
+    kernels = [{"name": name, "launches_per_step": a["launches"] / steps, "ms_per_step": a["ms"] / steps,
+                "avg_launch_us": 1e3 * a["ms"] / max(a["launches"], 1),
+                "gbs": a["bytes"] / max(a["ms"], 1e-12) / 1e6}
+               for name, a in sorted(agg.items(), key=lambda kv: -kv[1]["ms"])]
+    dom_name, dom = max(agg.items(), key=lambda kv: kv[1]["ms"])
+    per_launch_bytes = dom["bytes"] / dom["launches"]
+    per_launch_s = dom["ms"] / dom["launches"] / 1e3
+    achieved = per_launch_bytes / per_launch_s / 1e9
+    roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": pmc_traffic(wl.name, dom_name), "bytes_per_launch": per_launch_bytes,
+                "avg_launch_us": per_launch_s * 1e6, "launches_per_step": dom["launches"] / steps}
+    per_scope = {k: {"bytes": v["bytes"] / steps, "kernel_ms": v["kernel_ms"] / steps} for k, v in scopes.items()}
+    return kernels, roofline, per_scope
+
+
+def phase_table(stats, per_scope):
+    """each statistics scope: time (no profiling), algorithmic bytes (profile mode), GB/s and its
+    fraction of the HBM peak, and the same over the scope's summed kernel time"""
+    out = []
+    for ph in stats.phases():
+        b = per_scope.get(ph["name"], {}).get("bytes") if per_scope else None
+        kms = per_scope.get(ph["name"], {}).get("kernel_ms") if per_scope else None
+        row = {"scope": ph["name"], "ms": ph["ms"], "decided": ph["decided"]}
+        if b is not None:
+            gbs = b / max(ph["ms"], 1e-12) / 1e6
+            row.update(bytes=b, gbs=round(gbs, 2), frac=round(gbs / HBM_PEAK_GBS, 4), kernel_ms=kms,
+                       gbs_kernels=round(b / max(kms, 1e-12) / 1e6, 2))
+        out.append(row)
+    return out
+
+
+def tuned_options(name):
+    """the autotuner's stored options for this workload (kaspan_amd/tuned/<name>.json), or None"""
+    from kaspan_amd import autotune
+
+    return autotune.load(name)
+
+
+def cpu_baseline(wl, labels_gpu, what_pivot):
+    """the oracle's single-thread restatement of kaspan::scc, lean and faithful forms, on a bounded
+    sample of the workload; canonical labels checked against the GPU's"""
+    import numpy as np
+
+    from oracle import oracle as O  # checker / CPU baseline only
+
+    ka = wl.ka
+    forms = {}
+    if wl.batch is not None:
+        order = np.random.default_rng(0).permutation(wl.batch.count)
+        lab_h = labels_gpu.cpu().numpy().view(np.uint32)
+        vo = wl.batch.vtx_off.cpu().numpy().view(np.uint32)
+        graphs, m_total = [], 0
+        for i in order:
+            fo, fa, bo, ba = wl.batch.subgraph(int(i)).to_host()
+            graphs.append((int(i), O.HostGraph(wl.batch.n_of(int(i)), fo, fa, bo, ba)))
+            m_total += graphs[-1][1].m
+            if m_total > 30_000_000:
+                break
+        for form in ("lean", "faithful"):
+            cpu_s, m_done, same, done = 0.0, 0, True, 0
+            for i, hg in graphs:
+                t = time.perf_counter()
+                ref, _ = O.kaspan_scc(hg, faithful=form == "faithful")
+                cpu_s += time.perf_counter() - t
+                m_done += hg.m
+                done += 1
+                same &= bool(np.array_equal(O.canonicalize(ref), lab_h[vo[i]: vo[i + 1]]))
+                if cpu_s > CPU_MIN_S:
+                    break
+            forms[form] = {"value": m_done / cpu_s, "seconds": cpu_s, "graphs": done, "edges": m_done,
+                           "labels_equal_gpu": same}
+        sample = (f"random graphs of the batch solved one by one ({forms['lean']['graphs']} lean, "
+                  f"{forms['faithful']['graphs']} faithful)")
+    else:
+        if wl.name in CPU_SAMPLE:
+            skind, skw = CPU_SAMPLE[wl.name]
+            sg = ka.generate(skind, wl.seed, **skw)
+            slab, _ = ka.scc(sg)
+            sample = f"{skind} sample {skw} (n={sg.n}, m={sg.m}) of the {wl.name} workload, same seed"
+        else:
+            sg, slab = wl.g, labels_gpu
+            sample = f"whole {wl.name} graph (n={sg.n}, m={sg.m})"
+        fo, fa, bo, ba = sg.to_host()
+        hg = O.HostGraph(sg.n, fo, fa, bo, ba)
+        want = slab.cpu().numpy().view(np.uint32)
+        for form in ("lean", "faithful"):
+            reps, cpu_s = 0, 0.0
+            while (cpu_s < CPU_MIN_S and reps < 1000) or reps == 0:
+                t = time.perf_counter()
+                ref, _ = O.kaspan_scc(hg, faithful=form == "faithful")
+                cpu_s += time.perf_counter() - t
+                reps += 1
+            forms[form] = {"value": sg.m * reps / cpu_s, "seconds": cpu_s, "solves": reps,
+                           "labels_equal_gpu": bool(np.array_equal(O.canonicalize(ref), want))}
+    cpu = host_cpu()
+    return {"value": forms["lean"]["value"], "unit": "edges/s", "cores": 1, "kind": "port",
+            "sample": (f"{sample}; single-thread C restatement of kaspan::scc (scc/scc.hpp:21-87), plain pivot "
+                       f"(the reference has no learned pivot); value = lean form (no ghost caches); faithful form "
+                       f"(ghost-cache hash set/map allocated and cleared as scc.hpp:53,77,82 do) in forms; host "
+                       f"{cpu['model']}, {cpu['nproc']} cores available, 1 used (KaSpan has no intra-rank "
+                       f"threading)"),
+            "forms": forms, "host": cpu}
+
+
+def latency_sample(wl, opts):
+    """p50/p99 of sampled graphs of the batch solved one at a time"""
+    import numpy as np
+    import torch
+
+    ka, batch = wl.ka, wl.batch
+    stream = torch.cuda.current_stream()
+    idx = sorted(set(np.linspace(0, batch.count - 1, LATENCY_SAMPLE).astype(int).tolist()))
+    lat = []
+    for i in idx:
+        sg = batch.subgraph(i)
+        slab = torch.empty(sg.n, dtype=torch.int32, device="cuda")
+        ka.scc(sg, slab, **opts)  # warm-up of this size
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ka.scc(sg, slab, **opts)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        lat.append(e0.elapsed_time(e1))
+    lat = np.array(lat)
+    return {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)), "mean": float(lat.mean()),
+            "sample_graphs": len(idx)}
 
 
 def main():
@@ -89,24 +260,29 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="rmat24", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="planted50m", choices=sorted(CONFIGS))
     ap.add_argument("--manifest", default=None, metavar="FILE",
                     help="solve the graph of a manifest file (convert_graph's format) instead of --config")
-    ap.add_argument("--method", type=int, default=None, help="0 kaspan, 1 trim_ex (default: fastest known)")
+    ap.add_argument("--method", type=int, default=None, help="0 kaspan, 1 trim_ex (default: tuned or 0)")
     ap.add_argument("--variant", default=None,
                     help="one of the reference driver's variant names (sets --method): async, hpc_like, "
                          "hpc_trim_ex, ispan_like, trim_ex, trim_ex_light_residual, trim_ex_residual")
-    ap.add_argument("--pivot", choices=["plain", "learned"], default="plain")
+    ap.add_argument("--pivot", choices=["plain", "learned"], default=None,
+                    help="pivot rule of the measured solve (default: learned on planted50m with --ab, else plain)")
     ap.add_argument("--fwbw-rounds", type=int, default=None)
-    ap.add_argument("--coop-edges", type=int, default=0, help="persistent-kernel frontier edge bound (0: library default)")
-    ap.add_argument("--coop-workgroups", type=int, default=0, help="persistent-kernel workgroups (0: library default)")
-    ap.add_argument("--alpha", type=float, default=14.0, help="push->pull switch (Beamer alpha)")
-    ap.add_argument("--beta", type=float, default=24.0, help="pull->push switch (Beamer beta)")
-    ap.add_argument("--autotune", action="store_true",
-                    help="tune alpha/beta/persistent-kernel/round/tail/method options on this workload first")
-    ap.add_argument("--ab", action="store_true", help="also time the other pivot rule; report both and the speedup")
+    ap.add_argument("--coop-edges", type=int, default=None, help="persistent-kernel frontier edge bound")
+    ap.add_argument("--coop-workgroups", type=int, default=None, help="persistent-kernel workgroups")
+    ap.add_argument("--alpha", type=float, default=None, help="push->pull switch (Beamer alpha)")
+    ap.add_argument("--beta", type=float, default=None, help="pull->push switch (Beamer beta)")
+    ap.add_argument("--flags", type=int, default=None, help="KSPN_FLAG_* bits for the solves")
+    ap.add_argument("--no-tuned", action="store_true", help="ignore kaspan_amd/tuned/<config>.json")
+    ap.add_argument("--autotune", action="store_true", help="tune the options on this workload first")
+    ap.add_argument("--ab", action="store_true", help="time both pivot rules (on by default for planted50m)")
+    ap.add_argument("--no-ab", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-latency", action="store_true", help="batch: skip the one-graph-at-a-time latency sample")
+    ap.add_argument("--no-extra", action="store_true", help="skip the R-MAT 24 line of the default run")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes 3 times (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -120,219 +296,132 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy}))
         return
-    if args.fwbw_rounds is None:
-        args.fwbw_rounds = DEFAULT_ROUNDS.get(args.config, 1)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (multi-GPU sharding is a design, not built)"}))
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (multi-GPU sharding is a "
+                                   "design, not built)"}))
         sys.exit(2)
+    ab = (args.ab or (args.config == "planted50m" and not args.manifest)) and not args.no_ab
+    pivot = args.pivot or ("learned" if ab else "plain")
 
     import numpy as np
     import torch
 
     import kaspan_amd as ka
+    from kaspan_amd import autotune
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    print(f"[bench] generating {args.manifest or args.config}", file=sys.stderr, flush=True)
+    t0 = time.time()
+    wl = Workload(ka, args.config, args.manifest)
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+    g = wl.g
+    desc, kind = wl.desc, wl.kind
+    print(f"[bench] n={g.n} m={g.m} generated in {gen_s:.1f}s", file=sys.stderr, flush=True)
+
+    # options: library defaults < tuned file < command line
+    opts = dict(autotune.DEFAULTS)
+    opts["fwbw_rounds"] = DEFAULT_ROUNDS.get(args.config, 1)
+    tuned = None if (args.no_tuned or args.manifest) else tuned_options(wl.name)
+    if tuned:
+        opts.update(tuned["opts"])
+    for k, v in (("method", args.method), ("fwbw_rounds", args.fwbw_rounds), ("coop_edges", args.coop_edges),
+                 ("coop_workgroups", args.coop_workgroups), ("alpha", args.alpha), ("beta", args.beta),
+                 ("flags", args.flags)):
+        if v is not None:
+            opts[k] = v
     if args.variant is not None:
         if args.variant not in ka.VARIANTS:
             raise SystemExit(f"unknown variant {args.variant!r}")
-        args.method = ka.VARIANTS[args.variant]
-    if args.manifest:
-        from kaspan_amd import manifest as M
-
-        mf = M.parse_manifest(args.manifest)
-        args.config = f"manifest:{mf.name}"
-        kind, kw, seed = "manifest", {}, None
-        desc = (f"manifest graph {mf.name} (n={mf.node_count}, m={mf.edge_count}, {mf.head_bytes}/{mf.csr_bytes}-byte "
-                f"{mf.endian}-endian arrays, self-loops: {mf.contains_self_loops}, duplicate edges: "
-                f"{mf.contains_duplicate_edges})")
-    else:
-        kind, kw, seed, desc = CONFIGS[args.config]
-    print(f"[bench] generating {args.config}", file=sys.stderr, flush=True)
-    t0 = time.time()
-    batch = None
-    if kind == "manifest":
-        g = M.load_manifest(mf)
-    elif kind == "batch":
-        bkw = dict(kw)
-        count = bkw.pop("count")
-        batch = ka.generate_batch(seed, count, **bkw)
-        g = batch.graph
-    else:
-        g = ka.generate(kind, seed, **kw)
-    torch.cuda.synchronize()
-    gen_s = time.time() - t0
-    print(f"[bench] n={g.n} m={g.m} generated in {gen_s:.1f}s", file=sys.stderr, flush=True)
-
-    def solve(lab, **o):
-        if batch is not None:
-            return ka.scc_batch(batch, lab, **o)
-        return ka.scc(g, lab, **o)
-    method = args.method if args.method is not None else 0
-    labels = torch.empty(g.n, dtype=torch.int32, device="cuda")
-    stream = torch.cuda.current_stream()
+        opts["method"] = ka.VARIANTS[args.variant]
     mlp = None
-    if args.pivot == "learned" or args.ab:
+    if pivot == "learned" or ab:
         from kaspan_amd import pivot as P
 
         mlp = P.load_params()
         if mlp is None:
             raise SystemExit("no trained scorer: run python -m kaspan_amd.train_pivot first")
-    opts = dict(method=method, fwbw_rounds=args.fwbw_rounds, coop_edges=args.coop_edges,
-                coop_workgroups=args.coop_workgroups, alpha=args.alpha, beta=args.beta)
-    if args.pivot == "learned":
-        opts.update(pivot=1, mlp_params=mlp)
+    plain_opts = {k: v for k, v in opts.items() if k not in ("pivot", "mlp_params")}
+    learned_opts = dict(plain_opts, pivot=1, mlp_params=mlp)
+    if tuned and tuned.get("learned_opts") and mlp is not None:
+        learned_opts.update(tuned["learned_opts"])
+    main_opts = learned_opts if pivot == "learned" else plain_opts
 
-    tuned = None
+    tune_info = {"source": tuned["path"], "opts": tuned["opts"]} if tuned else None
     if args.autotune:
-        from kaspan_amd import autotune
-
-        space = dict(autotune.SPACE)
-        if args.pivot == "learned":
-            space.pop("method")
         t0 = time.time()
-        best, best_ms, trials = autotune.tune(batch if batch is not None else g, space=space,
-                                              start={**autotune.DEFAULTS, **{k: v for k, v in opts.items()
-                                                                             if k in autotune.DEFAULTS}},
+        space = dict(autotune.SPACE)
+        if pivot == "learned":
+            space.pop("method")
+        best, best_ms, trials = autotune.tune(wl.batch if wl.batch is not None else g, space=space,
+                                              start={k: v for k, v in main_opts.items() if k in autotune.DEFAULTS},
+                                              fixed={k: v for k, v in main_opts.items() if k not in autotune.DEFAULTS},
                                               log=lambda m: print(m, file=sys.stderr, flush=True))
-        opts.update(best)
-        tuned = {"opts": best, "ms": best_ms, "trials": len(trials), "seconds": time.time() - t0}
+        main_opts.update(best)
+        tune_info = {"opts": best, "ms": best_ms, "trials": len(trials), "seconds": time.time() - t0}
 
-    def timed(o):
-        for _ in range(args.warmup):
-            solve(labels, **o)
-        torch.cuda.synchronize()
-        start = torch.cuda.Event(enable_timing=True)
-        end = torch.cuda.Event(enable_timing=True)
-        start.record(stream)
-        for _ in range(args.steps):
-            solve(labels, stats=False, **o)
-        end.record(stream)
-        torch.cuda.synchronize()
-        ms = start.elapsed_time(end) / args.steps
-        # statistics (per-scope HIP events) from one further, untimed solve
-        _, st = solve(labels, **o)
-        return ms, st
-
-    ms_per_step, stats = timed(opts)
+    labels = torch.empty(g.n, dtype=torch.int32, device="cuda")
+    ms_per_step, stats = timed(wl, labels, main_opts, args.steps, args.warmup)
+    print(f"[bench] {pivot}: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
     value = g.m / (ms_per_step / 1e3)
-    print(f"[bench] {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
-    ab = None
-    if args.ab:
-        import numpy as np
-
+    ab_out = None
+    if ab:
         first = labels.clone()
-        other = {k: v for k, v in opts.items() if k not in ("pivot", "mlp_params")}
-        if args.pivot == "plain":
-            other.update(pivot=1, mlp_params=mlp)
-        ms_other, st_other = timed(other)
+        other_opts = plain_opts if pivot == "learned" else learned_opts
+        ms_other, st_other = timed(wl, labels, other_opts, args.steps, args.warmup)
         same = bool(torch.equal(first, labels))
-        plain_ms, learned_ms = (ms_per_step, ms_other) if args.pivot == "plain" else (ms_other, ms_per_step)
-        ab = {"plain_ms": plain_ms, "learned_ms": learned_ms, "speedup_learned": plain_ms / learned_ms,
-              "labels_identical": same, "fwbw_rounds": args.fwbw_rounds,
-              "fwbw_decided_plain": (stats if args.pivot == "plain" else st_other).decided_fwbw,
-              "fwbw_decided_learned": (st_other if args.pivot == "plain" else stats).decided_fwbw}
+        print(f"[bench] other rule: {ms_other:.3f} ms/step, labels identical: {same}", file=sys.stderr, flush=True)
+        plain_ms, learned_ms = (ms_other, ms_per_step) if pivot == "learned" else (ms_per_step, ms_other)
+        st_plain, st_learned = (st_other, stats) if pivot == "learned" else (stats, st_other)
+        ab_out = {"plain_ms": plain_ms, "learned_ms": learned_ms, "speedup_learned": plain_ms / learned_ms,
+                  "labels_identical": same, "fwbw_rounds_plain": plain_opts.get("fwbw_rounds"),
+                  "fwbw_rounds_learned": learned_opts.get("fwbw_rounds"),
+                  "fwbw_decided_plain": st_plain.decided_fwbw, "fwbw_decided_learned": st_learned.decided_fwbw,
+                  "plain_edges_per_s": g.m / (plain_ms / 1e3), "learned_edges_per_s": g.m / (learned_ms / 1e3)}
+        if kind == "planted":
+            planted = ka.planted_labels(wl.seed, wl.kw["n"], max_size=wl.kw["max_size"], expo=wl.kw["expo"])
+            ab_out["labels_equal_planted"] = bool(torch.equal(first, planted))
         labels.copy_(first)
 
-    # per-kernel profile over further steps (HIP events around every launch)
-    roofline = None
-    kernels = []
+    roofline, kernels, phases = None, [], phase_table(stats, None)
     if not args.no_profile:
-        agg = {}
-        for _ in range(args.steps):
-            _, pst = solve(labels, profile=True, **opts)
-            for k in pst.as_dict()["kernels"]:
-                a = agg.setdefault(k["name"], {"launches": 0, "ms": 0.0, "bytes": 0.0})
-                a["launches"] += k["launches"]
-                a["ms"] += k["ms"]
-                a["bytes"] += k["bytes"]
-        for name, a in sorted(agg.items(), key=lambda kv: -kv[1]["ms"]):
-            kernels.append({"name": name, "launches_per_step": a["launches"] / args.steps,
-                            "ms_per_step": a["ms"] / args.steps,
-                            "avg_launch_us": 1e3 * a["ms"] / max(a["launches"], 1),
-                            "gbs": a["bytes"] / max(a["ms"], 1e-12) / 1e6})
-        dom_name, dom = max(agg.items(), key=lambda kv: kv[1]["ms"])
-        per_launch_bytes = dom["bytes"] / dom["launches"]
-        per_launch_s = dom["ms"] / dom["launches"] / 1e3
-        achieved = per_launch_bytes / per_launch_s / 1e9
-        roofline = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": pmc_traffic(args.config, dom_name),
-                    "bytes_per_launch": per_launch_bytes, "avg_launch_us": per_launch_s * 1e6}
+        psteps = max(1, min(args.steps, 5 if g.m > 1e8 else args.steps))
+        kernels, roofline, per_scope = profile(wl, labels, main_opts, psteps)
+        phases = phase_table(stats, per_scope)
 
     batch_info = None
-    if batch is not None:
-        # serving one graph at a time: latency of each sampled graph alone (same options)
-        idx = sorted(set(np.linspace(0, batch.count - 1, LATENCY_SAMPLE).astype(int).tolist()))
-        lat = []
-        for i in idx:
-            sg = batch.subgraph(i)
-            slab = torch.empty(sg.n, dtype=torch.int32, device="cuda")
-            ka.scc(sg, slab, **opts)  # warm-up of this size
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            ka.scc(sg, slab, **opts)
-            e1.record(stream)
-            torch.cuda.synchronize()
-            lat.append(e0.elapsed_time(e1))
-        lat = np.array(lat)
-        batch_info = {"graphs": batch.count, "graphs_per_s": batch.count / (ms_per_step / 1e3),
-                      "batch_latency_ms": ms_per_step,
-                      "solo_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                                          "mean": float(lat.mean()), "sample_graphs": len(idx)},
-                      "solo_graphs_per_s_est": 1e3 / float(lat.mean()), "generate_s": gen_s}
+    if wl.batch is not None:
+        batch_info = {"graphs": wl.batch.count, "graphs_per_s": wl.batch.count / (ms_per_step / 1e3),
+                      "batch_latency_ms": ms_per_step, "generate_s": gen_s}
+        if not args.no_latency:
+            lat = latency_sample(wl, main_opts)
+            batch_info["solo_latency_ms"] = lat
+            batch_info["solo_graphs_per_s_est"] = 1e3 / lat["mean"]
 
-    cpu = None
-    if not args.no_cpu_baseline and batch is not None:
-        from oracle import oracle as O  # checker / CPU baseline only
+    cpu = None if args.no_cpu_baseline else cpu_baseline(wl, labels, pivot)
 
-        # graphs of the batch in a fixed order until about 15 s of CPU time
-        order = np.random.default_rng(0).permutation(batch.count)
-        cpu_s, cpu_m, done, same = 0.0, 0, 0, True
-        lab_h = labels.cpu().numpy().view(np.uint32)
-        vo = batch.vtx_off.cpu().numpy().view(np.uint32)
-        for i in order:
-            fo, fa, bo, ba = batch.subgraph(int(i)).to_host()
-            hg = O.HostGraph(batch.n_of(int(i)), fo, fa, bo, ba)
-            t = time.perf_counter()
-            ref, _ = O.kaspan_scc(hg)
-            cpu_s += time.perf_counter() - t
-            cpu_m += hg.m
-            done += 1
-            same &= bool(np.array_equal(O.canonicalize(ref), lab_h[vo[i]: vo[i + 1]]))
-            if cpu_s > 15.0:
-                break
-        cpu = {"value": cpu_m / cpu_s, "unit": "edges/s", "cores": 1, "kind": "port",
-               "sample": f"{done} random graphs of the batch ({cpu_m} edges) solved one by one; single-thread C "
-                         f"restatement of kaspan::scc (scc/scc.hpp:21-87), lean form; {cpu_s:.2f}s; canonical "
-                         f"labels equal to the GPU batch's: {same}"}
-    elif not args.no_cpu_baseline:
-        from oracle import oracle as O  # checker / CPU baseline only
-
-        if args.config in CPU_SAMPLE:
-            skind, skw = CPU_SAMPLE[args.config]
-            sg = ka.generate(skind, seed, **skw)
-            slab, _ = ka.scc(sg)
-            what = f"{skind} sample {skw} (n={sg.n}, m={sg.m}) of the {args.config} workload"
-        else:
-            sg, slab = g, labels
-            what = f"whole {args.config} graph (n={g.n}, m={g.m})"
-        fo, fa, bo, ba = sg.to_host()
-        hg = O.HostGraph(sg.n, fo, fa, bo, ba)
-        reps, cpu_s = 0, 0.0
-        while (cpu_s < CPU_MIN_S and reps < 1000) or reps == 0:
-            t = time.perf_counter()
-            ref, _ = O.kaspan_scc(hg)
-            cpu_s += time.perf_counter() - t
-            reps += 1
-        same = bool(np.array_equal(O.canonicalize(ref), slab.cpu().numpy().view(np.uint32)))
-        cpu = {"value": sg.m * reps / cpu_s, "unit": "edges/s", "cores": 1, "kind": "port",
-               "sample": f"{what}, solved {reps} times; single-thread C restatement of kaspan::scc "
-                         f"(scc/scc.hpp:21-87), lean form; {cpu_s:.2f}s in total; canonical labels equal to "
-                         f"the GPU's: {same}"}
+    extra = None
+    if args.config == "planted50m" and not args.no_extra and not args.manifest:
+        # R-MAT scale 24 with the plain rule: per-phase GB/s on the giant-SCC workload
+        del wl, labels
+        torch.cuda.empty_cache()
+        wx = Workload(ka, "rmat24")
+        xopts = dict(autotune.DEFAULTS)
+        xt = None if args.no_tuned else tuned_options("rmat24")
+        if xt:
+            xopts.update(xt["opts"])
+        lx = torch.empty(wx.g.n, dtype=torch.int32, device="cuda")
+        xms, xst = timed(wx, lx, xopts, args.steps, args.warmup)
+        extra = {"rmat24": {"workload": CONFIGS["rmat24"][3], "n": wx.g.n, "m": wx.g.m, "ms_per_step": xms,
+                            "value": wx.g.m / (xms / 1e3), "unit": "edges/s", "pivot": "plain",
+                            "options": xopts}}
+        if not args.no_profile:
+            _, xroof, xscope = profile(wx, lx, xopts, args.steps)
+            extra["rmat24"]["roofline"] = xroof
+            extra["rmat24"]["phases"] = phase_table(xst, xscope)
 
     out = {
-        "metric": "edges/sec to full SCC labels on 1 MI355X",
+        "metric": METRIC,
         "value": value,
         "unit": "edges/s",
         "n_gpus": 1,
@@ -344,24 +433,28 @@ def main():
         "vs_baseline": None,
         "dtype": "uint32",
         "data": "synthetic",
-        "config": {"workload": desc, "name": args.config, "n": g.n, "m": g.m,
-                   "method": METHOD_NAMES[opts.get("method", method)], "pivot": args.pivot,
-                   "fwbw_rounds": opts.get("fwbw_rounds", args.fwbw_rounds), "seed": seed,
-                   "options": {k: v for k, v in opts.items() if k != "mlp_params"}},
-        "ab": ab,
-        "autotune": tuned,
+        "config": {"workload": desc, "name": args.manifest or
+                   args.config, "n": g.n, "m": g.m, "method": METHOD_NAMES[main_opts.get("method", 0)],
+                   "pivot": pivot, "fwbw_rounds": main_opts.get("fwbw_rounds"), "seed": CONFIGS[args.config][2]
+                   if not args.manifest else None,
+                   "options": {k: v for k, v in main_opts.items() if k != "mlp_params"}},
+        "ab": ab_out,
+        "autotune": tune_info,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "phases": phases,
         "phases_ms": {k: getattr(stats, k) for k in ("ms_trim", "ms_fwbw", "ms_color", "ms_residual", "ms_canon")},
         "decided": {k: getattr(stats, k) for k in ("decided_trim", "decided_trim2", "decided_fwbw", "decided_color",
                                                    "decided_residual")},
-        "solve": {k: getattr(stats, k) for k in ("pivot", "fwbw_levels_push", "fwbw_levels_pull", "color_rounds",
-                                                 "color_iters", "edges_scanned", "residual_n")},
-        "kernels": kernels[:8],
-        "scopes": stats.phases(),
+        "solve": {k: getattr(stats, k) for k in ("pivot", "fwbw_levels_push", "fwbw_levels_pull", "fwbw_levels_coop",
+                                                 "fwbw_rounds_done", "color_rounds", "color_iters", "edges_scanned",
+                                                 "residual_n")},
+        "kernels": kernels[:10],
     }
     if batch_info is not None:
         out["batch"] = batch_info
+    if extra is not None:
+        out["extra"] = extra
     print(json.dumps(out))
 
 

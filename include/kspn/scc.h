@@ -71,6 +71,10 @@ typedef struct kspn_graph {
 #define KSPN_FLAG_NO_REFINE 8u /* colour rounds over the whole live graph, without partition refinement */
 #define KSPN_FLAG_NO_COOP 16u  /* every FW/BW level as separate launches (no persistent multi-level kernel) */
 #define KSPN_FLAG_STATS_MEMORY 32u /* fill kspn_phase.memory (one hipMemGetInfo per scope that carries it) */
+/* multi-pivot FW-BW rounds search backward over the whole partition and split it in three (forward-only,
+ * backward-only, unreached; Fleischer, Hendrickson and Pinar). Default: backward only within the forward
+ * set, which is the pivot's SCC plus what it reaches, and a two-way split (forward-only, the rest). */
+#define KSPN_FLAG_FULL_BW 64u
 
 typedef struct kspn_opts {
   uint32_t method;             /* KSPN_METHOD_* */

@@ -14,11 +14,15 @@ check of the option space on the caller's own graph.
 """
 from __future__ import annotations
 
+import json
+import os
 import statistics
 
 import torch
 
 import kaspan_amd as ka
+
+TUNED_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned")
 
 # candidate values per knob, in the order they are tried
 SPACE = {
@@ -26,12 +30,15 @@ SPACE = {
     "beta": [24.0, 6.0, 96.0],
     "coop_edges": [1 << 20, 1 << 16, 1 << 23, 1],
     "coop_workgroups": [128, 64, 256],
-    "fwbw_rounds": [1, 4, 16, 64],
+    "fwbw_rounds": [1, 4, 16, 32, 48, 64],
     "tail_threshold": [4096, 1024, 16384],
     "method": [0, 1],
+    "flags": [0, 64],  # KSPN_FLAG_FULL_BW: three-way split of the multi-pivot rounds
 }
+# launch shapes (compiled instantiations the library selects at run time); empty until the library has them
+SHAPES = {}
 DEFAULTS = {"alpha": 14.0, "beta": 24.0, "coop_edges": 1 << 20, "coop_workgroups": 128, "fwbw_rounds": 1,
-            "tail_threshold": 4096, "method": 0}
+            "tail_threshold": 4096, "method": 0, "flags": 0}
 
 
 def _solver(target):
@@ -58,15 +65,17 @@ def time_opts(target, opts, reps=3, labels=None):
     return statistics.median(times), lab
 
 
-def tune(target, space=None, start=None, reps=3, passes=1, log=None):
+def tune(target, space=None, start=None, reps=3, passes=1, log=None, fixed=None):
     """Coordinate descent from ``start`` (library defaults): returns (best opts, best ms, trials).
 
-    ``target`` is a BidiGraph or a Batch. ``trials`` lists (opts, ms) for every
-    candidate timed.
+    ``target`` is a BidiGraph or a Batch. ``fixed`` options (the pivot rule and
+    its parameters) go into every solve unchanged. ``trials`` lists (opts, ms)
+    for every candidate timed.
     """
     space = space or SPACE
+    fixed = dict(fixed or {})
     best = dict(DEFAULTS if start is None else start)
-    best_ms, ref = time_opts(target, best, reps)
+    best_ms, ref = time_opts(target, {**best, **fixed}, reps)
     ref = ref.clone()
     trials = [(dict(best), best_ms)]
     for _ in range(passes):
@@ -76,7 +85,7 @@ def tune(target, space=None, start=None, reps=3, passes=1, log=None):
                     continue
                 cand = dict(best)
                 cand[knob] = v
-                ms, lab = time_opts(target, cand, reps)
+                ms, lab = time_opts(target, {**cand, **fixed}, reps)
                 if not torch.equal(lab, ref):
                     raise ka.KspnError(-1, f"autotune: labels differ with {cand}")
                 trials.append((cand, ms))
@@ -85,3 +94,76 @@ def tune(target, space=None, start=None, reps=3, passes=1, log=None):
                 if ms < best_ms:
                     best, best_ms = cand, ms
     return best, best_ms, trials
+
+
+def path_for(name):
+    return os.path.join(TUNED_DIR, f"{name}.json")
+
+
+def save(name, opts, ms=None, learned_opts=None, note=None):
+    """store tuned options for a workload (kaspan_amd/tuned/<name>.json)"""
+    os.makedirs(TUNED_DIR, exist_ok=True)
+    rec = {"workload": name, "opts": {k: v for k, v in opts.items() if k in DEFAULTS or k in SHAPES}}
+    if learned_opts:
+        rec["learned_opts"] = learned_opts
+    if ms is not None:
+        rec["ms"] = ms
+    if note:
+        rec["note"] = note
+    with open(path_for(name), "w") as f:
+        json.dump(rec, f, indent=1, sort_keys=True)
+    return path_for(name)
+
+
+def load(name):
+    """the stored record for a workload with its path, or None"""
+    p = path_for(name)
+    if not os.path.exists(p):
+        return None
+    rec = json.load(open(p))
+    rec["path"] = os.path.relpath(p, os.path.dirname(TUNED_DIR))
+    return rec
+
+
+def main():
+    """python -m kaspan_amd.autotune --config NAME: tune the plain rule (and, on the planted workload,
+    the learned one) on the GPU and store the result in kaspan_amd/tuned/NAME.json, which bench.py
+    reads."""
+    import argparse
+    import time
+
+    from . import pivot as P
+    from .workloads import CONFIGS, Workload
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", required=True, choices=sorted(CONFIGS))
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--passes", type=int, default=1)
+    ap.add_argument("--learned", action="store_true", help="also tune the learned pivot rule")
+    args = ap.parse_args()
+    t0 = time.time()
+    wl = Workload(ka, args.config)
+    target = wl.batch if wl.batch is not None else wl.g
+    log = lambda m: print(m, flush=True)  # noqa: E731
+    space = dict(SPACE)
+    if wl.batch is not None:
+        space.pop("method")  # the batch runs the multi-pivot method
+    best, best_ms, trials = tune(target, space=space, reps=args.reps, passes=args.passes, log=log)
+    learned, learned_ms = None, None
+    if args.learned:
+        mlp = P.load_params()
+        if mlp is None:
+            raise SystemExit("no trained scorer (python -m kaspan_amd.train_pivot)")
+        lspace = {k: v for k, v in space.items() if k != "method"}
+        learned, learned_ms, ltrials = tune(target, space=lspace, start=dict(best), reps=args.reps,
+                                            passes=args.passes, log=log, fixed={"pivot": 1, "mlp_params": mlp})
+        trials += ltrials
+    path = save(args.config, best, ms=best_ms, learned_opts=learned,
+                note=f"coordinate descent, {len(trials)} trials, median of {args.reps} solves each, "
+                     f"{time.time() - t0:.0f}s" + (f"; learned rule {learned_ms:.3f} ms" if learned else ""))
+    print(json.dumps({"config": args.config, "plain_ms": best_ms, "opts": best, "learned_ms": learned_ms,
+                      "learned_opts": learned, "path": path}))
+
+
+if __name__ == "__main__":
+    main()

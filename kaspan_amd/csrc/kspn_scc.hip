@@ -336,6 +336,7 @@ struct OpArgs {
   const uint64_t* noff;          // OP_FW/OP_BW: offsets whose degree the next level expands
   unsigned long long* nsum;      // OP_FW/OP_BW: sum of those degrees over claimed targets; nsum[1]: their maximum
   const uint32_t* part;          // OP_FW/OP_PBW: partition of every vertex (nullptr: one partition)
+  const uint32_t* pcand;         // OP_PBW: the vertices it may claim (live, or fwv for a search within FW sets)
 };
 
 // reachability claim of edge u -> w: true for the one thread that claims w
@@ -346,7 +347,7 @@ __device__ inline bool reach_claim(uint32_t u, uint32_t w, const OpArgs& a) {
   } else if constexpr (OP == OP_BW) {
     return bm_get(a.fwv, w) && bm_claim(a.bwv, w);
   } else {
-    return bm_get(a.live, w) && a.part[w] == a.part[u] && bm_claim(a.bwv, w);
+    return bm_get(a.pcand, w) && a.part[w] == a.part[u] && bm_claim(a.bwv, w);
   }
 }
 
@@ -389,7 +390,7 @@ __device__ inline void expand_apply(uint32_t u, uint32_t w, const OpArgs& a, Blo
 template <int OP, int K>
 __device__ inline void reach_batch(const uint32_t (&u)[K], const uint32_t (&x)[K], const bool (&ok)[K],
                                    const OpArgs& a, BlockQueue& q, uint64_t& nsum, uint32_t& nmax) {
-  const uint32_t* cand = OP == OP_BW ? a.fwv : a.live;
+  const uint32_t* cand = OP == OP_BW ? a.fwv : (OP == OP_PBW ? a.pcand : a.live);
   uint32_t* vis = OP == OP_FW ? a.fwv : a.bwv;
   const bool part = OP == OP_PBW || (OP == OP_FW && a.part);
   uint32_t cw[K], vw[K];
@@ -3192,8 +3193,10 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   const uint64_t* roff = dir == 0 ? g->bw_off : g->fw_off;
   const uint32_t* radj = dir == 0 ? g->bw_adj : g->fw_adj;
   uint32_t* vis = dir == 0 ? c->fwv : c->bwv;
-  const uint32_t* cand = (dir == 0 || pid) ? c->live : c->fwv;
-  OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, 0, off, c->cnt64 + 1, pid};
+  // backward: within the forward sets, or (partitioned, KSPN_FLAG_FULL_BW) over every live vertex of the partition
+  const bool full_bw = pid && (R.o.flags & KSPN_FLAG_FULL_BW);
+  const uint32_t* cand = (dir == 0 || full_bw) ? c->live : c->fwv;
+  OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, 0, off, c->cnt64 + 1, pid, cand};
   uint32_t* F = c->qa;
   uint32_t* N = c->qb;
   int fc = C_Q0, nc = C_Q1;
