@@ -275,6 +275,7 @@ def main():
     ap.add_argument("--alpha", type=float, default=None, help="push->pull switch (Beamer alpha)")
     ap.add_argument("--beta", type=float, default=None, help="pull->push switch (Beamer beta)")
     ap.add_argument("--flags", type=int, default=None, help="KSPN_FLAG_* bits for the solves")
+    ap.add_argument("--graph", action="store_true", help="the device-driven solve, captured into a HIP graph")
     ap.add_argument("--no-tuned", action="store_true", help="ignore kaspan_amd/tuned/<config>.json")
     ap.add_argument("--autotune", action="store_true", help="tune the options on this workload first")
     ap.add_argument("--ab", action="store_true", help="time both pivot rules (on by default for planted50m)")
@@ -346,6 +347,8 @@ def main():
     if tuned and tuned.get("learned_opts") and mlp is not None:
         learned_opts.update(tuned["learned_opts"])
     main_opts = learned_opts if pivot == "learned" else plain_opts
+    if args.graph:
+        main_opts = dict(main_opts, flags=main_opts.get("flags", 0) | ka._lib.FLAG_GRAPH)
 
     tune_info = {"source": tuned["path"], "opts": tuned["opts"]} if tuned else None
     if args.autotune:
@@ -397,6 +400,18 @@ def main():
             lat = latency_sample(wl, main_opts)
             batch_info["solo_latency_ms"] = lat
             batch_info["solo_graphs_per_s_est"] = 1e3 / lat["mean"]
+        if pivot == "plain" and not (main_opts.get("flags", 0) & ka._lib.FLAG_GRAPH):
+            # the device-driven solve, captured once into a HIP graph and replayed
+            gopts = dict(main_opts, flags=main_opts.get("flags", 0) | ka._lib.FLAG_GRAPH)
+            glab = torch.empty_like(labels)
+            gms, gst = timed(wl, glab, gopts, args.steps, args.warmup)
+            batch_info["captured"] = {"ms": gms, "graphs_per_s": wl.batch.count / (gms / 1e3),
+                                      "edges_per_s": g.m / (gms / 1e3), "labels_identical": bool(torch.equal(glab, labels)),
+                                      "colour_rounds": gst.color_rounds, "decided_fwbw": gst.decided_fwbw,
+                                      "levels": gst.fwbw_levels_push + gst.fwbw_levels_pull}
+            if not args.no_latency:
+                batch_info["captured"]["solo_latency_ms"] = latency_sample(wl, gopts)
+            del glab
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(wl, labels, pivot)
 

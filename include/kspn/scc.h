@@ -62,8 +62,10 @@ typedef struct kspn_graph {
 #define KSPN_METHOD_TRIM_EX_LIGHT_RESIDUAL 6 /* variant/scc_trim_ex_light_residual.hpp: exhaustive trim-1, trim after decisions, residual */
 
 #define KSPN_PIVOT_PLAIN 0   /* max in*out degree product, smallest id on ties (trim_1_local.hpp:32-45) */
-#define KSPN_PIVOT_LEARNED 1 /* MLP score (kspn_mlp_*), plain rule as fallback */
-#define KSPN_PIVOT_RANDOM 2  /* seeded uniform choice per partition (for collecting pivot outcomes) */
+/* multi-pivot rounds: per partition, the vertices in the top half-octave of in*out are the candidates, and
+ * the MLP score (kspn_mlp_*) picks among them; single pivot: the MLP picks among the top in*out vertices */
+#define KSPN_PIVOT_LEARNED 1
+#define KSPN_PIVOT_RANDOM 2  /* a seeded uniform choice among the same candidates (for collecting pivot outcomes) */
 
 #define KSPN_FLAG_PROFILE 1u  /* time every kernel launch with HIP events (slower; fills kspn_stats.kern) */
 #define KSPN_FLAG_NO_TRIM2 2u /* KSPN_METHOD_TRIM_EX without the trim-2 pass */
@@ -75,6 +77,13 @@ typedef struct kspn_graph {
  * backward-only, unreached; Fleischer, Hendrickson and Pinar). Default: backward only within the forward
  * set, which is the pivot's SCC plus what it reaches, and a two-way split (forward-only, the rest). */
 #define KSPN_FLAG_FULL_BW 64u
+/* The device-driven solve: a fixed sequence of kernels (trim-1 first pass, one FW-BW from the plain pivot of
+ * every graph, colour rounds to completion) whose persistent kernels loop over levels and rounds on the
+ * device, with no host round trip until the labels are complete. It is captured into a HIP graph on the
+ * first call and replayed while the graph, labels, batch offsets, alpha, beta and max_rounds stay the same.
+ * Plain pivot and the KASPAN or HPC_LIKE method only (KSPN_E_UNIMPLEMENTED otherwise); kspn_stats then
+ * holds ms_total and the decided counts per scope, not per-scope times or kernel profiles. */
+#define KSPN_FLAG_GRAPH 128u
 
 typedef struct kspn_opts {
   uint32_t method;             /* KSPN_METHOD_* */
@@ -250,7 +259,9 @@ int kspn_pivot_features(kspn_ctx* ctx, const kspn_graph* g, const uint32_t* vert
                         void* stream);
 /* scores out[rows] = MLP(x[rows][6]); weights rounded to BF16, FP32 accumulation (MFMA) */
 int kspn_mlp_forward(kspn_ctx* ctx, const float* params, const float* x, uint32_t rows, float* out, void* stream);
-/* FP32 gradient of 0.5*mean((MLP(x) - target)^2): grad[KSPN_MLP_PARAMS], *loss (device) */
+/* gradient of 0.5*mean((MLP(x) - target)^2): grad[KSPN_MLP_PARAMS], *loss = 0.5*sum of squared errors
+ * (device). Matrix products on MFMA with BF16 operands (X, W1, relu(A1), W2 and the pre-activation
+ * gradients) and FP32 accumulation; biases, w3 and the loss in FP32. */
 int kspn_mlp_grad(kspn_ctx* ctx, const float* params, const float* x, const float* target, uint32_t rows, float* grad,
                   float* loss, void* stream);
 /* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8); state[2*KSPN_MLP_PARAMS] starts at 0; step >= 1 */

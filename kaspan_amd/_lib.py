@@ -42,6 +42,7 @@ FLAG_NO_REFINE = 8
 FLAG_NO_COOP = 16
 FLAG_STATS_MEMORY = 32
 FLAG_FULL_BW = 64
+FLAG_GRAPH = 128
 PHASE_MAX = 12
 MANIFEST_TEXT = 256
 MANIFEST_PATH = 4096

@@ -250,6 +250,29 @@ struct EventPair {
 };
 
 // ---------------------------------------------------------------- context
+// The device-driven solve's buffers and its captured HIP graph
+// (kspn_graphsolve.hip). The graph is kept while the inputs, outputs,
+// options and buffers it was captured with stay the same.
+struct GraphMode {
+  void* ctl = nullptr;                     // control block (gs::GCtl)
+  unsigned long long* pc = nullptr;        // per vertex: partition << 32 | colour
+  unsigned long long* q0 = nullptr;        // frontier row slices (two queues)
+  unsigned long long* q1 = nullptr;
+  uint64_t qcap = 0;                       // entries per queue
+  unsigned long long* pkey = nullptr;      // per graph: pivot key
+  uint32_t* gmin = nullptr;                // per graph: smallest member of the pivot's SCC
+  uint32_t cap_n = 0, cap_count = 0;
+  uint64_t cap_m = 0;
+  uint32_t grid = 0;                       // persistent workgroups, all resident at once (0: not checked yet)
+  hipStream_t cap_stream = nullptr;        // capture stream (default priority)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t key[16] = {};                   // what the captured graph was built for
+  void* h_status = nullptr;                // pinned copy of the control block
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  uint32_t replays = 0;                    // launches of the current captured graph
+};
+
 struct Ctx {
   int device = 0;
   uint32_t cap_n = 0;
@@ -305,6 +328,7 @@ struct Ctx {
   int num_cu = 256;
   uint32_t pub_seq = 0;  // sequence number of the last counter publication (h_map[63])
   // persistent multi-level reachability (cooperative launch)
+  GraphMode gm;                    // KSPN_FLAG_GRAPH
   struct CoopCtl* coop = nullptr;  // device control block
   uint32_t coop_grid = 0;          // co-resident workgroups (0: cooperative launch unavailable)
 };
@@ -338,6 +362,11 @@ inline uint32_t grid_for(uint64_t items, uint32_t per_block = WG, uint32_t cap =
   if (g > cap) g = cap;
   return (uint32_t)g;
 }
+
+// the device-driven solve (KSPN_FLAG_GRAPH): captured once per shape, replayed
+int graph_solve(Ctx* c, const kspn_graph* g, uint32_t* labels, const kspn_opts& o, const uint32_t* vtx_off,
+                uint32_t count, kspn_stats* st, hipStream_t s);
+void graph_release(Ctx* c);
 
 int select_pivot_learned_impl(Ctx* c, const kspn_graph* g, const float* params, hipStream_t s, uint32_t* pivot);
 int mlp_features(Ctx* c, const kspn_graph* g, const uint32_t* list, uint32_t count, float* feat, hipStream_t s);

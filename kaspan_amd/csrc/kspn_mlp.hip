@@ -1,5 +1,5 @@
-// kspn_mlp.hip -- learned pivot scorer: features, MLP inference (BF16
-// MFMA, FP32 accumulation), FP32 training kernels, Adam, outcome logging.
+// kspn_mlp.hip -- learned pivot scorer: features, MLP inference and
+// training (BF16 MFMA, FP32 accumulation), Adam, outcome logging.
 //
 // No counterpart in the reference: KaSpan picks the pivot with the plain
 // degree-product rule (scc/trim_1_local.hpp:32-45, scc/pivot.hpp:53-74),
@@ -148,116 +148,183 @@ __global__ __launch_bounds__(WG) void k_mlp_forward(uint32_t rows, const float* 
 }
 
 // ---------------------------------------------------------------- training
-// FP32 forward + backward for loss = 0.5 * mean((y - t)^2) over `rows`.
-// Each workgroup takes batches of 128 rows: activations in LDS, weight
-// gradients reduced over the batch by thread-per-output loops, then one
-// atomicAdd per parameter per batch into grad[]. loss_sum += sum of 0.5 e^2.
-constexpr int TB = 128;
+// Forward and backward for loss = 0.5 * mean((y - t)^2) over `rows`, every
+// matrix product on v_mfma_f32_16x16x16bf16_1k: BF16 operands, FP32
+// accumulation. Operands rounded to BF16: X, W1, H1 = relu(A1), W2 (as in
+// k_mlp_forward), G2 and G1 (the gradients at the pre-activations A2, A1).
+// A2, H2 = relu(A2), y and the bias / w3 gradients stay FP32.
+//   A1 = X W1^T + b1        A2 = H1 W2^T + b2       y = H2 . w3 + b3
+//   dy = (y - t) / rows     G2 = [A2 > 0] * (dy w3^T)
+//   dW2 = G2^T H1           G1 = [A1 > 0] * (G2 W2)  dW1 = G1^T X
+// Each wavefront takes 16 rows per step (a workgroup 64, all wavefronts in
+// step): per row block the weight gradients are MFMAs with the rows as the
+// reduction dimension, accumulated in registers across the steps. The
+// workgroup sums its wavefronts' gradients in LDS and adds each parameter
+// to grad[] once. MFMA lane layouts as in k_mlp_forward.
 __global__ __launch_bounds__(WG) void k_mlp_grad(uint32_t rows, const float* __restrict__ x,
                                                  const float* __restrict__ target, const float* __restrict__ params,
                                                  float inv_rows, float* __restrict__ grad, float* loss_sum) {
-  __shared__ float s_w1[192], s_w2[1024], s_b1[32], s_b2[32], s_w3[32];
-  __shared__ float s_x[TB * 6];
-  __shared__ float s_a1[TB * 33], s_a2[TB * 33];  // pre-activations (+pad)
-  __shared__ float s_g[TB * 33];                  // dH2 then dH1 (on pre-activations)
-  __shared__ float s_dy[TB];
-  __shared__ float s_loss[TB];
+  constexpr int NW = WG / WAVE;
+  __shared__ short s_w1[32 * 16];  // W1 as [n][k], k padded 6 -> 16
+  __shared__ short s_w2[32 * 32];  // W2 as [n][k]
+  __shared__ float s_b1[32], s_b2[32], s_w3[32];
+  __shared__ float s_a1[NW][16 * 33], s_a2[NW][16 * 33];  // per wavefront: 16 rows of pre-activations
+  __shared__ float s_g[NW][16 * 33];                      // G2, then G1
+  __shared__ float s_x[NW][16 * 8];
+  __shared__ float s_dy[NW][16];
+  __shared__ float s_sum[P_COUNT + 1];  // the workgroup's gradient, then its loss in the last slot
   const uint32_t t = threadIdx.x;
-  for (uint32_t i = t; i < 192; i += WG) s_w1[i] = params[P_W1 + i];
-  for (uint32_t i = t; i < 1024; i += WG) s_w2[i] = params[P_W2 + i];
+  for (uint32_t i = t; i < 32 * 16; i += WG) {
+    const uint32_t n = i / 16, k = i % 16;
+    s_w1[i] = k < 6 ? to_bf16(params[P_W1 + n * 6 + k]) : (short)0;
+  }
+  for (uint32_t i = t; i < 32 * 32; i += WG) s_w2[i] = to_bf16(params[P_W2 + i]);
   if (t < 32) {
     s_b1[t] = params[P_B1 + t];
     s_b2[t] = params[P_B2 + t];
     s_w3[t] = params[P_W3 + t];
   }
-  const float b3 = params[P_B3];
+  for (uint32_t i = t; i <= P_COUNT; i += WG) s_sum[i] = 0.f;
   __syncthreads();
-  const uint32_t batches = (rows + TB - 1) / TB;
-  for (uint32_t bi = blockIdx.x; bi < batches; bi += gridDim.x) {
-    const uint32_t r0 = bi * TB;
-    const uint32_t nb = min((uint32_t)TB, rows - r0);
-    for (uint32_t i = t; i < TB * 6; i += WG) s_x[i] = (i / 6 < nb) ? x[(uint64_t)r0 * 6 + i] : 0.f;
-    __syncthreads();
-    // layer 1 pre-activations
-    for (uint32_t i = t; i < TB * 32; i += WG) {
-      uint32_t r = i / 32, o = i % 32;
-      float acc = s_b1[o];
-      for (int k = 0; k < 6; ++k) acc += s_w1[o * 6 + k] * s_x[r * 6 + k];
-      s_a1[r * 33 + o] = acc;
+  const float b3 = params[P_B3];
+  const uint32_t lane = lane_id(), wv = t / WAVE;
+  const uint32_t col = lane % 16, kq = lane / 16;
+  float* a1 = s_a1[wv];
+  float* a2 = s_a2[wv];
+  float* g = s_g[wv];
+  float* xr = s_x[wv];
+  float* dy = s_dy[wv];
+  v4f acc_w2[2][2], acc_w1[2];
+  for (int i = 0; i < 2; ++i) {
+    acc_w1[i] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 2; ++j) acc_w2[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+  }
+  float db1 = 0.f, db2 = 0.f, dw3 = 0.f, db3 = 0.f, loss = 0.f;  // lane o < 32 owns output o
+  const uint64_t chunks = ((uint64_t)rows + 63) / 64;
+  for (uint64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+    const uint64_t r0 = ch * 64 + wv * 16;
+    // inputs of the 16 rows (FP32 copy for dW1's B operand)
+    for (uint32_t i = lane; i < 16 * 8; i += WAVE) {
+      const uint32_t r = i / 8, k = i % 8;
+      xr[i] = (k < 6 && r0 + r < rows) ? x[(r0 + r) * 6 + k] : 0.f;
     }
     __syncthreads();
-    for (uint32_t i = t; i < TB * 32; i += WG) {
-      uint32_t r = i / 32, o = i % 32;
-      float acc = s_b2[o];
-      for (int k = 0; k < 32; ++k) acc += s_w2[o * 32 + k] * fmaxf(s_a1[r * 33 + k], 0.f);
-      s_a2[r * 33 + o] = acc;
+    // ---- forward
+    v4s a;
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t k = 4 * kq + j;
+      a[j] = k < 6 ? to_bf16(xr[col * 8 + k]) : (short)0;
+    }
+    for (int nt = 0; nt < 2; ++nt) {
+      v4s b;
+      for (int j = 0; j < 4; ++j) b[j] = s_w1[(nt * 16 + col) * 16 + 4 * kq + j];
+      v4f d = {0.f, 0.f, 0.f, 0.f};
+      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, d, 0, 0, 0);
+      for (int j = 0; j < 4; ++j) a1[(4 * kq + j) * 33 + nt * 16 + col] = d[j] + s_b1[nt * 16 + col];
     }
     __syncthreads();
-    float lpart = 0.f;
-    if (t < TB) {
+    v4s h1[2];  // H1 in the A layout (row col, k = ks*16 + 4kq + j)
+    for (int ks = 0; ks < 2; ++ks)
+      for (int j = 0; j < 4; ++j) h1[ks][j] = to_bf16(fmaxf(a1[col * 33 + ks * 16 + 4 * kq + j], 0.f));
+    for (int nt = 0; nt < 2; ++nt) {
+      v4f d = {0.f, 0.f, 0.f, 0.f};
+      for (int ks = 0; ks < 2; ++ks) {
+        v4s b;
+        for (int j = 0; j < 4; ++j) b[j] = s_w2[(nt * 16 + col) * 32 + ks * 16 + 4 * kq + j];
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h1[ks], b, d, 0, 0, 0);
+      }
+      for (int j = 0; j < 4; ++j) a2[(4 * kq + j) * 33 + nt * 16 + col] = d[j] + s_b2[nt * 16 + col];
+    }
+    __syncthreads();
+    if (lane < 16) {
+      const uint64_t r = r0 + lane;
       float y = b3;
-      for (int k = 0; k < 32; ++k) y += s_w3[k] * fmaxf(s_a2[t * 33 + k], 0.f);
-      float e = (t < nb) ? y - target[r0 + t] : 0.f;
-      s_dy[t] = e * inv_rows;
-      lpart = 0.5f * e * e;
+      for (int k = 0; k < 32; ++k) y += s_w3[k] * fmaxf(a2[lane * 33 + k], 0.f);
+      const float e = r < rows ? y - target[r] : 0.f;
+      dy[lane] = e * inv_rows;
+      loss += 0.5f * e * e;
     }
     __syncthreads();
-    // dH2 (pre-activation) = dy * w3 * [a2 > 0]
-    for (uint32_t i = t; i < TB * 32; i += WG) {
-      uint32_t r = i / 32, o = i % 32;
-      s_g[r * 33 + o] = (s_a2[r * 33 + o] > 0.f) ? s_dy[r] * s_w3[o] : 0.f;
+    // ---- backward: G2, and the FP32 gradients of w3, b3, b2
+    for (uint32_t i = lane; i < 16 * 32; i += WAVE) {
+      const uint32_t r = i / 32, o = i % 32;
+      g[r * 33 + o] = a2[r * 33 + o] > 0.f ? dy[r] * s_w3[o] : 0.f;
     }
     __syncthreads();
-    // dW3, db3, dW2, db2 (one parameter per thread, summed over the batch)
-    for (uint32_t p = t; p < 32 + 1 + 1024 + 32; p += WG) {
-      float acc = 0.f;
-      if (p < 32) {
-        for (uint32_t r = 0; r < nb; ++r) acc += s_dy[r] * fmaxf(s_a2[r * 33 + p], 0.f);
-        atomicAdd(&grad[P_W3 + p], acc);
-      } else if (p == 32) {
-        for (uint32_t r = 0; r < nb; ++r) acc += s_dy[r];
-        atomicAdd(&grad[P_B3], acc);
-      } else if (p < 33 + 1024) {
-        uint32_t q = p - 33, o = q / 32, k = q % 32;
-        for (uint32_t r = 0; r < nb; ++r) acc += s_g[r * 33 + o] * fmaxf(s_a1[r * 33 + k], 0.f);
-        atomicAdd(&grad[P_W2 + q], acc);
-      } else {
-        uint32_t o = p - 33 - 1024;
-        for (uint32_t r = 0; r < nb; ++r) acc += s_g[r * 33 + o];
-        atomicAdd(&grad[P_B2 + o], acc);
+    if (lane < 32) {
+      for (int r = 0; r < 16; ++r) {
+        db2 += g[r * 33 + lane];
+        dw3 += dy[r] * fmaxf(a2[r * 33 + lane], 0.f);
       }
     }
-    __syncthreads();
-    // dH1 (pre-activation) = (W2^T dH2) * [a1 > 0], written over s_a1's sign
-    for (uint32_t i = t; i < TB * 32; i += WG) {
-      uint32_t r = i / 32, k = i % 32;
-      float acc = 0.f;
-      for (int o = 0; o < 32; ++o) acc += s_w2[o * 32 + k] * s_g[r * 33 + o];
-      s_a2[r * 33 + k] = (s_a1[r * 33 + k] > 0.f) ? acc : 0.f;  // s_a2 now holds dH1
-    }
-    __syncthreads();
-    for (uint32_t p = t; p < 192 + 32; p += WG) {
-      float acc = 0.f;
-      if (p < 192) {
-        uint32_t o = p / 6, k = p % 6;
-        for (uint32_t r = 0; r < nb; ++r) acc += s_a2[r * 33 + o] * s_x[r * 6 + k];
-        atomicAdd(&grad[P_W1 + p], acc);
-      } else {
-        uint32_t o = p - 192;
-        for (uint32_t r = 0; r < nb; ++r) acc += s_a2[r * 33 + o];
-        atomicAdd(&grad[P_B1 + o], acc);
+    if (lane == 0)
+      for (int r = 0; r < 16; ++r) db3 += dy[r];
+    // dW2[o][k] += sum_r G2[r][o] H1[r][k]: M = o, N = k, K = the 16 rows
+    for (int ot = 0; ot < 2; ++ot) {
+      v4s ga;
+      for (int j = 0; j < 4; ++j) ga[j] = to_bf16(g[(4 * kq + j) * 33 + ot * 16 + col]);
+      for (int kt = 0; kt < 2; ++kt) {
+        v4s hb;
+        for (int j = 0; j < 4; ++j) hb[j] = to_bf16(fmaxf(a1[(4 * kq + j) * 33 + kt * 16 + col], 0.f));
+        acc_w2[ot][kt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ga, hb, acc_w2[ot][kt], 0, 0, 0);
       }
     }
-    // batch loss
-    if (t < TB) s_loss[t] = lpart;
+    // G1 = [A1 > 0] * (G2 W2): M = rows, K = o, N = k
+    v4f g1t[2];
+    {
+      v4s ga2[2];
+      for (int ks = 0; ks < 2; ++ks)
+        for (int j = 0; j < 4; ++j) ga2[ks][j] = to_bf16(g[col * 33 + ks * 16 + 4 * kq + j]);
+      for (int nt = 0; nt < 2; ++nt) {
+        v4f d = {0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < 2; ++ks) {
+          v4s b;
+          for (int j = 0; j < 4; ++j) b[j] = s_w2[(ks * 16 + 4 * kq + j) * 32 + nt * 16 + col];
+          d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ga2[ks], b, d, 0, 0, 0);
+        }
+        g1t[nt] = d;
+      }
+    }
+    __syncthreads();  // every lane has read G2
+    for (int nt = 0; nt < 2; ++nt)
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t r = 4 * kq + j, k = nt * 16 + col;
+        g[r * 33 + k] = a1[r * 33 + k] > 0.f ? g1t[nt][j] : 0.f;
+      }
     __syncthreads();
-    if (t == 0) {
-      float l = 0.f;
-      for (uint32_t r = 0; r < TB; ++r) l += s_loss[r];
-      atomicAdd(loss_sum, l);
+    if (lane < 32)
+      for (int r = 0; r < 16; ++r) db1 += g[r * 33 + lane];
+    // dW1[o][i] += sum_r G1[r][o] X[r][i]: M = o, N = i (6 of 16), K = the 16 rows
+    {
+      v4s xb;
+      for (int j = 0; j < 4; ++j) xb[j] = col < 6 ? to_bf16(xr[(4 * kq + j) * 8 + col]) : (short)0;
+      for (int ot = 0; ot < 2; ++ot) {
+        v4s ga;
+        for (int j = 0; j < 4; ++j) ga[j] = to_bf16(g[(4 * kq + j) * 33 + ot * 16 + col]);
+        acc_w1[ot] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ga, xb, acc_w1[ot], 0, 0, 0);
+      }
     }
     __syncthreads();
   }
+  // the workgroup's sums in LDS, then one global add per parameter
+  for (int ot = 0; ot < 2; ++ot)
+    for (int kt = 0; kt < 2; ++kt)
+      for (int j = 0; j < 4; ++j)
+        atomicAdd(&s_sum[P_W2 + (ot * 16 + 4 * kq + j) * 32 + kt * 16 + col], acc_w2[ot][kt][j]);
+  if (col < 6)
+    for (int ot = 0; ot < 2; ++ot)
+      for (int j = 0; j < 4; ++j) atomicAdd(&s_sum[P_W1 + (ot * 16 + 4 * kq + j) * 6 + col], acc_w1[ot][j]);
+  if (lane < 32) {
+    atomicAdd(&s_sum[P_B1 + lane], db1);
+    atomicAdd(&s_sum[P_B2 + lane], db2);
+    atomicAdd(&s_sum[P_W3 + lane], dw3);
+  }
+  if (lane == 0) atomicAdd(&s_sum[P_B3], db3);
+  if (lane < 16) atomicAdd(&s_sum[P_COUNT], loss);
+  __syncthreads();
+  for (uint32_t i = t; i < P_COUNT; i += WG)
+    if (s_sum[i] != 0.f) atomicAdd(&grad[i], s_sum[i]);
+  if (t == 0) atomicAdd(loss_sum, s_sum[P_COUNT]);
 }
 
 // Adam (Kingma & Ba 2015) on the parameter vector; state = [m | v]
@@ -452,8 +519,8 @@ int kspn_mlp_grad(kspn_ctx* k, const float* params, const float* x, const float*
   KSPN_HIP(hipMemsetAsync(grad, 0, KSPN_MLP_PARAMS * sizeof(float), s));
   KSPN_HIP(hipMemsetAsync(loss, 0, sizeof(float), s));
   if (rows == 0) return KSPN_OK;
-  uint32_t batches = (rows + TB - 1) / TB;
-  uint32_t grid = std::min<uint32_t>(batches, (uint32_t)k->c.num_cu * 4);
+  const uint32_t chunks = (rows + 63) / 64;  // 64 rows per workgroup step
+  const uint32_t grid = std::min<uint32_t>(chunks, (uint32_t)k->c.num_cu * 4);
   k_mlp_grad<<<grid, WG, 0, s>>>(rows, x, target, params, 1.0f / rows, grad, loss);
   KSPN_HIP(hipGetLastError());
   return KSPN_OK;

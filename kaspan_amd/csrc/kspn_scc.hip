@@ -585,8 +585,9 @@ constexpr uint32_t COOP_SPIN_LIMIT = 1u << 21;  // barrier polls before giving u
 constexpr uint32_t COOP_LOCAL = 8;              // rounds a wavefront expands its own claims within a level
 enum CoopReason : uint32_t { COOP_EMPTY = 1, COOP_PULL = 2, COOP_BIG = 3, COOP_ERR = 4 };
 
-// relaxed device-scope loads: coherent across XCDs without a cache
-// invalidation (ordering comes from the one fence pair per barrier)
+// relaxed device-scope loads of values other workgroups change within a
+// level (counters, claims); what a level wrote is ordered before the next
+// level's reads by the release/acquire pair of coop_barrier
 __device__ inline uint32_t ld_relaxed(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -615,17 +616,20 @@ __device__ inline void coop_read_level(const CoopCtl* ctl, uint32_t slot, CoopLe
 // an otherwise idle device). Everything one workgroup writes and another
 // reads inside the launch goes through device-scope atomics (counters,
 // bitmap claims) or device-scope relaxed stores and loads (the frontier
-// queues), which are coherent across XCDs without cache maintenance; the
-// workgroup barrier before arrival waits for the workgroup's stores to
-// complete. So the barrier needs no L2 writeback or invalidation, and the
-// read-only graph arrays stay cached from level to level. The arrival
-// counter only grows: barrier k of a launch waits for k * workgroups
-// arrivals. Thread 0 arrives, polls, then reads the next level's values
-// into *lv. Returns false if the others did not arrive within the spin limit
-// or one of them gave up: a launch that is not fully resident ends with an
-// error instead of waiting forever.
+// queues). The HIP memory model orders them across the barrier: each wave
+// releases at agent scope before the workgroup barrier, thread 0 arrives
+// with a relaxed atomic add, polls with relaxed loads and acquires at agent
+// scope once all have arrived. The arrival counter only grows: barrier k of
+// a launch waits for k * workgroups arrivals. Thread 0 then reads the next
+// level's values into *lv. Returns false if the others did not arrive within
+// the spin limit or one of them gave up: a launch that is not fully
+// resident ends with an error instead of waiting forever.
 __device__ bool coop_barrier(CoopCtl* ctl, uint32_t k, uint32_t next_slot, CoopLevel* lv) {
   __shared__ uint32_t s_ok;
+  // every wave releases its writes at agent scope before the workgroup
+  // arrives (the workgroup barrier alone does not wait for other waves'
+  // stores), and thread 0 acquires after the others arrived
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t ok = 1;
@@ -641,6 +645,7 @@ __device__ bool coop_barrier(CoopCtl* ctl, uint32_t k, uint32_t next_slot, CoopL
         __builtin_amdgcn_s_sleep(1);
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     coop_read_level(ctl, next_slot, lv);
     s_ok = ok;
   }
@@ -2499,6 +2504,13 @@ __global__ void k_mask_to_bitmap(uint32_t n, const uint8_t* mask, uint32_t* bm) 
 // partition), decides FW & BW (the pivots' SCCs) and splits every partition
 // into FW-only, BW-only and neither (Fleischer, Hendrickson, Pinar 2000).
 
+// half-octave of a positive product: 2 * floor(log2 p) + the bit after the leading one (0 for p = 0)
+__device__ inline uint32_t prod_bucket(uint64_t p) {
+  if (p == 0) return 0;
+  const uint32_t b = 63 - __clzll(p);
+  return 1 + 2 * b + (b ? (uint32_t)((p >> (b - 1)) & 1u) : 0u);
+}
+
 __device__ inline uint32_t float_order(float f) {
   uint32_t u = __float_as_uint(f);
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
@@ -2585,14 +2597,21 @@ __global__ __launch_bounds__(WG) void k_part_key(uint32_t n, const uint32_t* __r
     if (valid) {
       p = pid[v];
       uint32_t pri;
-      if (score) {
-        pri = float_order(score[v]);
-      } else if (rnd_seed) {
-        uint64_t z = ((uint64_t)rnd_seed << 32 | (uint32_t)v) * 0x9E3779B97F4A7C15ull;
-        z = (z ^ (z >> 31)) * 0xBF58476D1CE4E5B9ull;
-        pri = (uint32_t)(z >> 32) | 1u;
+      const uint64_t prod = (fw_off[v + 1] - fw_off[v]) * (bw_off[v + 1] - bw_off[v]);
+      if (score || rnd_seed) {
+        // the plain rule's half-octave of in*out first, then the score (or a
+        // seeded hash) inside it: the pivot is always one of the plain rule's
+        // top candidates of its partition
+        uint32_t low;
+        if (score) {
+          low = float_order(score[v]) >> 8;
+        } else {
+          uint64_t z = ((uint64_t)rnd_seed << 32 | (uint32_t)v) * 0x9E3779B97F4A7C15ull;
+          z = (z ^ (z >> 31)) * 0xBF58476D1CE4E5B9ull;
+          low = (uint32_t)(z >> 40);
+        }
+        pri = (prod_bucket(prod) << 24) | low | 1u;
       } else {
-        uint64_t prod = (fw_off[v + 1] - fw_off[v]) * (bw_off[v + 1] - bw_off[v]);
         pri = prod > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)prod;
       }
       key = ((unsigned long long)pri << 32) | (uint32_t)~(uint32_t)v;
@@ -4601,6 +4620,7 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   if (!k) return KSPN_OK;
   Ctx* c = &k->c;
   (void)hipSetDevice(c->device);
+  graph_release(c);
   void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->sbm, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
                   c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf, c->pid, c->pbest,
                   c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt};
@@ -4654,6 +4674,7 @@ int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* o
     for (auto& t : c->kt) { t.launches = 0; t.ms = 0; t.bytes = 0; }
     for (int i = 0; i < KSPN_PHASE_MAX; ++i) c->phase_bytes[i] = c->phase_kms[i] = 0;
   }
+  if (o.flags & KSPN_FLAG_GRAPH) return graph_solve(c, g, labels, o, vtx_off, count, stats, s);
   Run R{c, g, labels, o, stats, s, g->n, bm_words(g->n)};
   R.coop_edges = o.coop_edges ? o.coop_edges : kCoopEdges;
   R.coop_deg = o.coop_degree ? o.coop_degree : kCoopDeg;

@@ -7,11 +7,13 @@ candidates and the best score seeds FW-BW; the plain rule's pivot is always
 among the candidates and remains the fallback. Pivot choice changes time,
 never labels.
 
-In the multi-pivot rounds the score replaces in*out as each partition's
-pivot priority. The scorer is trained on pivot outcomes: by default the SCC
-a FW-BW from the vertex decides (log1p of its size; one solve gives it for
-every vertex), or, from the rounds' own outcome log (opts.pivot_log), the
-partition's residual shrinkage. Forward (BF16 MFMA), gradient (FP32) and
+In the multi-pivot rounds the candidates of a partition are its vertices
+in the plain rule's top half-octave of in*out, and the score picks among
+them. The scorer is trained on pivot outcomes the rounds themselves log
+(opts.pivot_log): by default log1p of the SCC each pivot decided, with
+pivots drawn at random among the same candidates; optionally the SCC a
+FW-BW from every vertex decides (one solve gives all) or the partition's
+shrinkage. Forward (BF16 MFMA), gradient (FP32) and
 Adam run as libkspn kernels on the GPU; inputs and target are standardised
 during training and the scaling is folded back into W1, b1, w3 and b3.
 """
@@ -101,16 +103,23 @@ def pivot_log(g: BidiGraph, rounds=24, pivot=_lib.PIVOT_RANDOM, seed=0, cap=1 <<
     return rec.to(torch.int64) & 0xFFFFFFFF
 
 
-def outcome_samples(g: BidiGraph, rounds=24, seed=0, min_size=32, target="size", sample=1 << 19):
+def outcome_samples(g: BidiGraph, rounds=24, seed=0, min_size=32, target="log", sample=1 << 19):
     """(features, target) pairs from pivot outcomes on g.
 
+    target="log" (default): the multi-pivot rounds' own outcome log, with
+    pivots drawn at random among each partition's candidates (the plain
+    rule's top half-octave of in*out, the set the learned rule picks from);
+    the target is log1p of the SCC each logged pivot decided.
     target="size": every live vertex is a candidate pivot whose outcome is the
     SCC a FW-BW from it decides; one solve gives all of them (kspn_scc_sizes)
     and the target is log1p(SCC size), for a seeded sample of ``sample`` live
-    vertices. target="shrinkage": the multi-pivot rounds' own log with random
-    pivots; per partition of at least ``min_size`` vertices the target is
-    1 - max(forward-only, backward-only, unreached) / partition size.
+    vertices. target="shrinkage": the log as for "log"; per partition of at
+    least ``min_size`` vertices the target is 1 - (largest part) / partition size.
     """
+    if target == "log":
+        rec = pivot_log(g, rounds=rounds, pivot=_lib.PIVOT_RANDOM, seed=seed)
+        x = features(g, rec[:, 0].to(torch.int32).contiguous())
+        return x, torch.log1p(rec[:, 3].float())
     if target == "size":
         labels, _ = scc(g)
         sizes = scc_sizes(labels)

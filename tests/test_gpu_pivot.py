@@ -1,9 +1,10 @@
 """GPU tests of the learned pivot scorer and the planted generator.
 
-Floating point: features and gradients are FP32 on the GPU against the FP64
-oracle; the forward pass rounds weights, inputs and the first hidden layer
-to BF16 (FP32 accumulation) and is compared with a float64 model that rounds
-at the same points. Tolerances are written next to each comparison."""
+Floating point: features are FP32 on the GPU against the FP64 oracle; the
+forward and training passes run their matrix products on MFMA with BF16
+operands (FP32 accumulation) and are compared with float64 models that round
+at the same points, and with the unrounded FP64 oracle. Tolerances are
+written next to each comparison."""
 import numpy as np
 import pytest
 
@@ -28,6 +29,26 @@ def forward_bf16_model(x, p):
     h1 = np.maximum(xb @ bf16_round(w1).reshape(32, 6).T + b1, 0)
     h2 = np.maximum(bf16_round(h1) @ bf16_round(w2).reshape(32, 32).T + b2, 0)
     return h2 @ w3 + b3
+
+
+def grad_bf16_model(x, t, p):
+    """float64 gradient of 0.5*mean((y - t)^2) with the training kernel's BF16
+    rounding points: X, W1, relu(A1), W2, G2 and G1 are rounded; the rest is
+    exact. Returns (0.5*sum of squared errors, gradient in parameter order)."""
+    w1, b1, w2, b2, w3, b3 = split(p)
+    W1, W2 = w1.reshape(32, 6), w2.reshape(32, 32)
+    xb = bf16_round(x)
+    a1 = xb @ bf16_round(W1).T + b1
+    h1 = bf16_round(np.maximum(a1, 0))
+    a2 = h1 @ bf16_round(W2).T + b2
+    h2 = np.maximum(a2, 0)
+    e = h2 @ w3 + b3 - np.asarray(t, np.float64)
+    dy = e / len(x)
+    g2 = (a2 > 0) * dy[:, None] * w3[None, :]
+    g1 = (a1 > 0) * (bf16_round(g2) @ bf16_round(W2))
+    grad = np.concatenate([(bf16_round(g1).T @ xb).ravel(), g1.sum(0), (bf16_round(g2).T @ h1).ravel(), g2.sum(0),
+                           h2.T @ dy, [dy.sum()]])
+    return 0.5 * float((e ** 2).sum()), grad
 
 
 @pytest.fixture(scope="module")
@@ -88,9 +109,15 @@ def test_gradients_match_oracle(ka, O, piv):
         ref_loss, ref = O.mlp_loss_grad(x, t, *split(p))
         g = g.cpu().numpy().astype(np.float64)
         ref_flat = np.concatenate([ref[0], ref[1], ref[2], ref[3], ref[4], [ref[5]]])
-        # FP32 sums over the batch vs FP64: 1e-4 relative to the largest gradient entry
-        assert np.abs(g - ref_flat).max() <= 1e-4 * max(1.0, np.abs(ref_flat).max()), rows
-        assert abs(float(loss.item()) / rows - ref_loss) <= 1e-4 * max(1.0, ref_loss)
+        model_loss, model = grad_bf16_model(x, t, p)
+        scale = max(1.0, np.abs(model).max())
+        # same BF16 rounding points, FP32 accumulation vs float64 (and the rare double rounding of a
+        # gradient through FP32 before BF16): 1e-3 relative to the largest gradient entry
+        assert np.abs(g - model).max() <= 1e-3 * scale, rows
+        # against the unrounded FP64 oracle: BF16 operands, ~2^-8 relative per term
+        assert np.abs(g - ref_flat).max() <= 3e-2 * max(1.0, np.abs(ref_flat).max()), rows
+        assert abs(float(loss.item()) - model_loss) <= 1e-4 * max(1.0, model_loss)
+        assert abs(float(loss.item()) / rows - ref_loss) <= 3e-2 * max(1.0, ref_loss)
 
 
 def test_fold_scaling_is_exact(ka, O, piv):
