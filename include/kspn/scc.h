@@ -84,6 +84,10 @@ typedef struct kspn_graph {
  * Plain pivot and the KASPAN or HPC_LIKE method only (KSPN_E_UNIMPLEMENTED otherwise); kspn_stats then
  * holds ms_total and the decided counts per scope, not per-scope times or kernel profiles. */
 #define KSPN_FLAG_GRAPH 128u
+/* Every FW / BW reachability of the host-driven solve as one asynchronous launch: no levels; wavefronts
+ * take the claimed vertices' row slices from a queue as they appear (coop_workgroups workgroups of 256
+ * threads, 0 -> 512). The reached sets, and so the labels, are the same as level by level. */
+#define KSPN_FLAG_ASYNC 256u
 
 typedef struct kspn_opts {
   uint32_t method;             /* KSPN_METHOD_* */
@@ -102,6 +106,11 @@ typedef struct kspn_opts {
   uint32_t coop_edges;         /* persistent multi-level kernel takes frontiers with at most this many out-edges (0 -> 2^20) */
   uint32_t coop_degree;        /* ... and no vertex of out-degree above this (0 -> 2048) */
   uint32_t coop_workgroups;    /* its workgroups, at most one per CU (0 -> 128) */
+  /* launch shapes (the autotuner's; 0 -> the default) */
+  uint32_t pull_ilp;           /* pull sweep: vertices in flight per thread, 1, 2 or 4 (0 -> 2) */
+  uint32_t expand_ept;         /* push expansion: edges per thread of a workgroup tile, 4, 8 or 16 (0 -> 8) */
+  uint32_t sweep_wgs;          /* pull sweep: workgroups per CU (0 -> 8, 16 above 2^25 vertices) */
+  uint32_t list_wgs;           /* pull list levels: workgroups per CU at most (0 -> 4) */
 } kspn_opts;
 
 /* pivot outcome record: pivot, round, partition size, SCC size (decided),

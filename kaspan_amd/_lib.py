@@ -43,6 +43,7 @@ FLAG_NO_COOP = 16
 FLAG_STATS_MEMORY = 32
 FLAG_FULL_BW = 64
 FLAG_GRAPH = 128
+FLAG_ASYNC = 256
 PHASE_MAX = 12
 MANIFEST_TEXT = 256
 MANIFEST_PATH = 4096
@@ -126,6 +127,10 @@ class Opts(C.Structure):
         ("coop_edges", C.c_uint32),
         ("coop_degree", C.c_uint32),
         ("coop_workgroups", C.c_uint32),
+        ("pull_ilp", C.c_uint32),
+        ("expand_ept", C.c_uint32),
+        ("sweep_wgs", C.c_uint32),
+        ("list_wgs", C.c_uint32),
     ]
 
 

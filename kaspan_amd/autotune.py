@@ -2,6 +2,9 @@
 
 Coordinate descent over the options that change time but never labels:
 
+* launch shapes: vertices per thread of the pull sweep and edges per thread
+  of the push expansion (compiled instantiations), and the workgroups per
+  CU of pull sweeps and list levels;
 * the push/pull switch (Beamer alpha, beta);
 * the persistent multi-level kernel's frontier bound and workgroup count;
 * FW-BW rounds and the tail-solver threshold;
@@ -35,9 +38,16 @@ SPACE = {
     "method": [0, 1],
     "flags": [0, 64],  # KSPN_FLAG_FULL_BW: three-way split of the multi-pivot rounds
 }
-# launch shapes (compiled instantiations the library selects at run time); empty until the library has them
-SHAPES = {}
-DEFAULTS = {"alpha": 14.0, "beta": 24.0, "coop_edges": 1 << 20, "coop_workgroups": 128, "fwbw_rounds": 1,
+# launch shapes: compiled instantiations (pull sweep vertices per thread, push tile edges per thread) and
+# grid sizes (workgroups per CU) the library selects at run time; 0 is the library's default
+SHAPES = {
+    "pull_ilp": [2, 1, 4],
+    "expand_ept": [8, 4, 16],
+    "sweep_wgs": [0, 4, 16],
+    "list_wgs": [0, 2, 8],
+}
+SPACE.update(SHAPES)
+DEFAULTS = {"pull_ilp": 2, "expand_ept": 8, "sweep_wgs": 0, "list_wgs": 0, "alpha": 14.0, "beta": 24.0, "coop_edges": 1 << 20, "coop_workgroups": 128, "fwbw_rounds": 1,
             "tail_threshold": 4096, "method": 0, "flags": 0}
 
 

@@ -233,6 +233,7 @@ enum Counter : int {
   C_LIVE,    // vertices left live after the last FW-BW, listed by k_fwbw_finish for the tail solver
   C_SPEC,    // k_spec_gate: a speculatively launched pull sweep is to run
   C_BADID,   // a tail kernel met a listed vertex id >= n and skipped it
+  C_AERR,    // the asynchronous reach gave up (1: no progress, 2: queue overflow)
   C_COUNT_ = 24
 };
 
@@ -329,6 +330,9 @@ struct Ctx {
   uint32_t pub_seq = 0;  // sequence number of the last counter publication (h_map[63])
   // persistent multi-level reachability (cooperative launch)
   GraphMode gm;                    // KSPN_FLAG_GRAPH
+  unsigned long long* aq = nullptr;  // asynchronous reach: slot queue (all empty between reaches)
+  uint64_t aq_cap = 0;
+  void* actl = nullptr;              // its control block
   struct CoopCtl* coop = nullptr;  // device control block
   uint32_t coop_grid = 0;          // co-resident workgroups (0: cooperative launch unavailable)
 };

@@ -440,7 +440,7 @@ __device__ inline void reach_batch(const uint32_t (&u)[K], const uint32_t (&x)[K
   }
 }
 
-constexpr int EXP_TILE = WG * 8;  // edges per workgroup tile
+constexpr int EXP_EPT = 8;        // edges per thread of a tile (default; 4 and 16 are also built)
 constexpr int EXP_WIN = 1024;     // frontier entries staged in LDS per tile
 
 // Each workgroup takes 2048-edge tiles of the concatenated adjacency of the
@@ -449,7 +449,7 @@ constexpr int EXP_WIN = 1024;     // frontier entries staged in LDS per tile
 // scan value) is staged in LDS, and each thread maps its edges to their
 // source by a forward-only LDS search. Consecutive threads read
 // consecutive adjacency entries.
-template <int OP>
+template <int OP, int EPT_ = EXP_EPT>
 __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, const uint32_t* fcount,
                                                const uint64_t* __restrict__ S, const uint64_t* __restrict__ off,
                                                const uint32_t* __restrict__ adj, OpArgs a, uint32_t* __restrict__ next,
@@ -458,6 +458,7 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
   __shared__ uint64_t s_beg[EXP_WIN];
   __shared__ uint32_t s_v[EXP_WIN];
   __shared__ uint32_t s_i0, s_w;
+  constexpr int EXP_TILE = WG * EPT_;  // edges per workgroup tile
   __shared__ uint32_t s_out[2 * EXP_TILE];
   __shared__ uint32_t s_on, s_ob;
   __shared__ unsigned long long s_nsum;
@@ -879,6 +880,300 @@ __global__ __launch_bounds__(COOP_WG) void k_reach_coop(uint32_t n, uint32_t* F0
 
 // control block for a launch: level 0 is the host's frontier (count at
 // *fcount, degree sum and maximum known)
+// =====================================================================
+// asynchronous reachability
+// =====================================================================
+// One launch runs a whole reach without levels. Work items are row slices
+// (AS_SLICE edges) of claimed vertices in one linear queue: every vertex is
+// claimed once, so a reach never appends more than n + m / AS_SLICE entries.
+// A wavefront takes 64 queue slots at a time and waits for them to be
+// written; whatever entries are ready it expands as one batch (their edges
+// concatenated, four per lane, the loads of the four issued together),
+// claims targets with the bitmap atomics of the level kernels, and stages
+// the claimed vertices' slices in LDS before reserving queue slots for them.
+// Slots are written once with agent-scope stores and polled with
+// agent-scope loads; the consumer writes them back to AS_EMPTY, so the
+// queue is empty again when the reach ends. One counter, `pending`, holds
+// the entries reserved and not yet expanded. A producer adds its entries to
+// it, and waits for that atomic to return, before it writes their slots; a
+// batch subtracts itself only after its children are counted. So a consumer
+// can subtract an entry only after the producer's addition for it, and the
+// counter reaches 0 exactly when the reach is over: a wavefront waiting on
+// slots that will never be written sees that and leaves. Waits are bounded:
+// after AS_SPIN polls without progress the kernel sets `err` and every
+// wavefront leaves; a reach that ends with `pending` above 0 sets it too.
+constexpr uint32_t AS_SLICE = 256;
+constexpr unsigned long long AS_EMPTY = ~0ull;
+constexpr int AS_WG = 256;
+constexpr int AS_WAVES = AS_WG / WAVE;
+constexpr uint32_t AS_STAGE = 256;
+constexpr uint32_t AS_SPIN = 1u << 24;
+
+struct AsyncCtl {
+  uint32_t head, tail, err, pad;
+  unsigned long long pending;  // entries reserved and not yet expanded
+  unsigned long long edges;    // edges expanded (statistics)
+  unsigned long long verts;    // entries expanded
+  unsigned long long claims;   // vertices claimed
+};
+
+__device__ inline unsigned long long ld_agent64(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_agent64(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_async_init(AsyncCtl* ctl) {
+  if (threadIdx.x == 0) *ctl = AsyncCtl{};
+}
+
+// n queue slots reserved and counted in `pending`; the slot index is
+// returned only after both atomics have returned, so the count is in place
+// before any of the slots is written
+__device__ inline uint32_t as_reserve(uint32_t n, AsyncCtl* ctl) {
+  uint32_t base = 0;
+  if (lane_id() == 0) {
+    base = atomicAdd(&ctl->tail, n);
+    const unsigned long long before = atomicAdd(&ctl->pending, (unsigned long long)n);
+    if (before == AS_EMPTY) base = 0xFFFFFFFFu;  // never true; keeps the wait for the return
+  }
+  return __shfl(base, 0);
+}
+
+// Slots for the staged entries, written with agent-scope stores; returns how
+// many were reserved. A queue that would overflow sets err instead.
+__device__ inline uint32_t as_flush(unsigned long long* st, uint32_t n, unsigned long long* Q, uint32_t qcap,
+                                    AsyncCtl* ctl) {
+  if (n == 0) return 0;
+  const uint32_t lane = lane_id();
+  const uint32_t base = as_reserve(n, ctl);
+  if ((uint64_t)base + n > qcap) {
+    if (lane == 0) atomicOr(&ctl->err, 2u);
+  } else {
+    for (uint32_t i = lane; i < n; i += WAVE) st_agent64(&Q[base + i], st[i]);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  return n;
+}
+
+// the slices of the lanes' claimed vertices (take, deg) into the stage;
+// every lane calls together. *made counts the slots reserved by flushes.
+__device__ inline void as_stage(unsigned long long* st, uint32_t& sn, bool take, uint32_t v, uint64_t deg,
+                                unsigned long long* Q, uint32_t qcap, AsyncCtl* ctl, uint32_t& made) {
+  const bool any = take && deg;
+  if (!__ballot(any)) return;
+  const uint32_t lane = lane_id();
+  const uint32_t k = any ? (uint32_t)((deg + AS_SLICE - 1) / AS_SLICE) : 0u;
+  uint32_t incl = k;
+  for (uint32_t o = 1; o < WAVE; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  const uint32_t total = __shfl(incl, WAVE - 1);
+  if (sn + total > AS_STAGE) {
+    made += as_flush(st, sn, Q, qcap, ctl);
+    sn = 0;
+  }
+  if (total > AS_STAGE) {  // a hub's many slices: straight to the queue
+    const uint32_t base = as_reserve(total, ctl);
+    if ((uint64_t)base + total > qcap) {
+      if (lane == 0) atomicOr(&ctl->err, 2u);
+    } else {
+      for (uint32_t s = 0; s < k; ++s) st_agent64(&Q[base + incl - k + s], ((unsigned long long)v << 32) | s);
+    }
+    made += total;
+    return;
+  }
+  for (uint32_t s = 0; s < k; ++s) st[sn + incl - k + s] = ((unsigned long long)v << 32) | s;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  sn += total;
+}
+
+// the frontier F[0..*fcount) (vis bits already set) as the queue's first entries
+__global__ __launch_bounds__(WG) void k_async_seed(const uint32_t* __restrict__ F, const uint32_t* fcount,
+                                                   const uint64_t* __restrict__ off, unsigned long long* Q,
+                                                   uint32_t qcap, AsyncCtl* ctl) {
+  __shared__ unsigned long long s_st[WG / WAVE][AS_STAGE];
+  const uint32_t f = *fcount;
+  const uint32_t wid = threadIdx.x / WAVE;
+  uint32_t sn = 0, made = 0;
+  const uint32_t pad = (f + WG - 1) / WG * WG;
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < pad; i += gridDim.x * WG) {
+    const bool ok = i < f;
+    const uint32_t v = ok ? F[i] : 0u;
+    const uint64_t deg = ok ? off[v + 1] - off[v] : 0;
+    as_stage(s_st[wid], sn, ok, v, deg, Q, qcap, ctl, made);
+  }
+  made += as_flush(s_st[wid], sn, Q, qcap, ctl);
+}
+
+template <int OP>
+__global__ __launch_bounds__(AS_WG) void k_reach_async(const uint64_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ adj, OpArgs a,
+                                                       unsigned long long* Q, uint32_t qcap, AsyncCtl* ctl) {
+  __shared__ uint64_t s_beg[AS_WAVES][WAVE];
+  __shared__ uint32_t s_ex[AS_WAVES][WAVE], s_src[AS_WAVES][WAVE];
+  __shared__ unsigned long long s_st[AS_WAVES][AS_STAGE];
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  const uint32_t* cand = OP == OP_BW ? a.fwv : (OP == OP_PBW ? a.pcand : a.live);
+  uint32_t* vis = OP == OP_FW ? a.fwv : a.bwv;
+  const bool part = OP == OP_PBW || (OP == OP_FW && a.part != nullptr);
+  uint64_t* sb = s_beg[wid];
+  uint32_t* sx = s_ex[wid];
+  uint32_t* su = s_src[wid];
+  unsigned long long* st = s_st[wid];
+  unsigned long long edges = 0, verts = 0, claims = 0;
+  bool over = false;
+  while (!over) {
+    uint32_t u0 = 0;
+    if (lane == 0) u0 = atomicAdd(&ctl->head, (uint32_t)WAVE);
+    u0 = __shfl(u0, 0);
+    if (u0 >= qcap) break;  // never written: the queue cannot hold that many entries
+    const uint32_t slot = u0 + lane;
+    bool mine = slot < qcap;  // slots of this range not yet expanded
+    uint32_t spins = 0;
+    while (__ballot(mine)) {
+      const unsigned long long e = mine ? ld_agent64(&Q[slot]) : AS_EMPTY;
+      const bool ready = mine && e != AS_EMPTY;
+      if (!__ballot(ready)) {
+        unsigned long long pend = 0;
+        uint32_t err = 0;
+        if (lane == 0) {
+          pend = ld_agent64(&ctl->pending);
+          err = __hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        pend = __shfl(pend, 0);
+        err = __shfl(err, 0);
+        if (pend == 0 || err) {  // nothing outstanding anywhere: these slots stay empty
+          over = true;
+          break;
+        }
+        if (++spins > AS_SPIN) {
+          if (lane == 0) atomicOr(&ctl->err, 1u);
+          over = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+        continue;
+      }
+      spins = 0;
+      // the ready entries as one batch
+      uint32_t v = 0, cnt = 0;
+      uint64_t b = 0;
+      if (ready) {
+        v = (uint32_t)(e >> 32);
+        const uint32_t sl = (uint32_t)e;
+        const uint64_t re = off[v + 1];
+        b = off[v] + (uint64_t)sl * AS_SLICE;
+        cnt = (uint32_t)(min(b + (uint64_t)AS_SLICE, re) - b);
+      }
+      uint32_t incl = cnt;
+      for (uint32_t o = 1; o < WAVE; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+      }
+      const uint32_t total = __shfl(incl, WAVE - 1);
+      sb[lane] = b;
+      sx[lane] = incl - cnt;
+      su[lane] = v;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      uint32_t sn = 0, made = 0;
+      for (uint32_t base = 0; base < total; base += 4 * WAVE) {
+        uint32_t w[4], u[4];
+        bool ok[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t x = base + q * WAVE + lane;
+          ok[q] = x < total;
+          w[q] = u[q] = 0;
+          if (ok[q]) {
+            uint32_t lo = 0, hi = WAVE;  // last entry whose scan is <= x
+            while (hi - lo > 1) {
+              const uint32_t mid = (lo + hi) >> 1;
+              if (sx[mid] <= x) lo = mid; else hi = mid;
+            }
+            u[q] = su[lo];
+            w[q] = adj[sb[lo] + (x - sx[lo])];
+          }
+        }
+        uint32_t cw[4], vw[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          cw[q] = ok[q] ? cand[w[q] >> 5] : 0u;
+          vw[q] = ok[q] ? vis[w[q] >> 5] : 0xFFFFFFFFu;
+        }
+        bool c[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t bit = 1u << (w[q] & 31);
+          c[q] = (cw[q] & bit) && !(vw[q] & bit);
+        }
+        if (part) {
+          uint32_t pu[4], pw[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            pu[q] = c[q] ? a.part[u[q]] : 0u;
+            pw[q] = c[q] ? a.part[w[q]] : 0u;
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) c[q] = c[q] && pu[q] == pw[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t bit = 1u << (w[q] & 31);
+          if (c[q]) c[q] = (atomicOr(&vis[w[q] >> 5], bit) & bit) == 0;
+        }
+        uint64_t d0[4], d1[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          d0[q] = c[q] ? off[w[q]] : 0;
+          d1[q] = c[q] ? off[w[q] + 1] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          claims += c[q];
+          as_stage(st, sn, c[q], w[q], d1[q] - d0[q], Q, qcap, ctl, made);
+        }
+      }
+      made += as_flush(st, sn, Q, qcap, ctl);
+      if (ready) st_agent64(&Q[slot], AS_EMPTY);
+      const uint32_t nready = (uint32_t)__popcll(__ballot(ready));
+      // this batch out, after its children were counted in (as_reserve)
+      if (lane == 0) atomicAdd(&ctl->pending, 0ull - (unsigned long long)nready);
+      if (lane == 0) {
+        edges += total;
+        verts += nready;
+      }
+      mine = mine && !ready;
+    }
+  }
+  claims = wave_sum64(claims);
+  if (lane == 0) {
+    if (edges) atomicAdd(&ctl->edges, edges);
+    if (verts) atomicAdd(&ctl->verts, verts);
+    if (claims) atomicAdd(&ctl->claims, claims);
+  }
+}
+
+// after an asynchronous reach: its statistics into cnt64[0..1], its error
+// into cnt[C_AERR]; both frontier counters zeroed (as a level-by-level
+// reach leaves them)
+__global__ void k_async_done(const AsyncCtl* ctl, uint32_t* cnt, unsigned long long* cnt64) {
+  if (threadIdx.x == 0) {
+    cnt64[0] = ctl->edges;
+    cnt64[1] = ctl->claims;
+    cnt[C_AERR] = ctl->err | (ctl->pending ? 4u : 0u);
+    cnt[C_Q0] = 0;
+    cnt[C_Q1] = 0;
+  }
+}
+
 __global__ void k_coop_init(CoopCtl* ctl, const uint32_t* fcount, unsigned long long edges, uint32_t maxd) {
   if (threadIdx.x == 0) {
     for (int k = 0; k < 4; ++k) {
@@ -1125,7 +1420,7 @@ constexpr int PULL_ILP = 2;  // vertices in flight per thread (2 beat 1 and 4 on
 constexpr uint32_t PULL_FLUSH = 4;                     // k_pull_list: iterations between checks of its LDS queues
 constexpr uint32_t PULL_QCAP = PULL_FLUSH * WG * PULL_ILP;  // entries of each LDS queue (8 KiB)
 
-template <int DIR>
+template <int DIR, int ILP>
 __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restrict__ cand, uint32_t* vis,
                                              const uint64_t* __restrict__ roff, const uint32_t* __restrict__ radj,
                                              const uint64_t* __restrict__ noff, uint32_t* __restrict__ fnew,
@@ -1135,19 +1430,19 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
   if (gate && *gate == 0) return;  // a speculative sweep that lost (SpecGate)
   // A candidate costs a chain of dependent loads (bitmap word, row offsets,
   // in-neighbour, its visited bit, its partition). Each thread carries
-  // PULL_ILP vertices (one per 256-vertex tile of its workgroup's span)
+  // ILP vertices (one per 256-vertex tile of its workgroup's span)
   // through every step together, so their loads overlap.
   __shared__ uint64_t s_red[WG / WAVE];
   uint64_t claimed = 0, edges = 0, nd = 0;
-  constexpr uint64_t SPAN = (uint64_t)WG * PULL_ILP;
+  constexpr uint64_t SPAN = (uint64_t)WG * ILP;
   const uint64_t n_pad = ((uint64_t)n + SPAN - 1) / SPAN * SPAN;
   const uint32_t lane = lane_id();
   for (uint64_t base = (uint64_t)blockIdx.x * SPAN; base < n_pad; base += (uint64_t)gridDim.x * SPAN) {
-    uint32_t vw[PULL_ILP], pv[PULL_ILP];
-    uint64_t it[PULL_ILP], end[PULL_ILP];
-    bool act[PULL_ILP], hit[PULL_ILP], cnd[PULL_ILP];
+    uint32_t vw[ILP], pv[ILP];
+    uint64_t it[ILP], end[ILP];
+    bool act[ILP], hit[ILP], cnd[ILP];
 #pragma unroll
-    for (int k = 0; k < PULL_ILP; ++k) {
+    for (int k = 0; k < ILP; ++k) {
       const uint64_t v = base + (uint64_t)k * WG + threadIdx.x;
       const uint64_t word = v >> 5;
       vw[k] = (v < n) ? vis[word] : 0xFFFFFFFFu;
@@ -1160,7 +1455,7 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
       pv[k] = 0;
     }
 #pragma unroll
-    for (int k = 0; k < PULL_ILP; ++k) {
+    for (int k = 0; k < ILP; ++k) {
       if (act[k]) {
         const uint64_t v = base + (uint64_t)k * WG + threadIdx.x;
         it[k] = roff[v];
@@ -1171,19 +1466,19 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
     // scan the rows together, one in-neighbour of every unfinished row per step
     for (;;) {
       bool more = false;
-      uint32_t u[PULL_ILP];
+      uint32_t u[ILP];
 #pragma unroll
-      for (int k = 0; k < PULL_ILP; ++k) {
+      for (int k = 0; k < ILP; ++k) {
         act[k] = act[k] && it[k] < end[k];
         more |= act[k];
         u[k] = act[k] ? radj[it[k]] : 0u;
       }
       if (!more) break;
-      uint32_t uw[PULL_ILP];
+      uint32_t uw[ILP];
 #pragma unroll
-      for (int k = 0; k < PULL_ILP; ++k) uw[k] = act[k] ? vis[u[k] >> 5] : 0u;
+      for (int k = 0; k < ILP; ++k) uw[k] = act[k] ? vis[u[k] >> 5] : 0u;
 #pragma unroll
-      for (int k = 0; k < PULL_ILP; ++k) {
+      for (int k = 0; k < ILP; ++k) {
         if (!act[k]) continue;
         ++edges;
         ++it[k];
@@ -1196,7 +1491,7 @@ __global__ __launch_bounds__(WG) void k_pull(uint32_t n, const uint32_t* __restr
       }
     }
 #pragma unroll
-    for (int k = 0; k < PULL_ILP; ++k) {
+    for (int k = 0; k < ILP; ++k) {
       const uint64_t v = base + (uint64_t)k * WG + threadIdx.x;
       if (hit[k]) {
         ++claimed;
@@ -2887,8 +3182,9 @@ void prof_collect(Ctx* c) {
   c->pending.clear();
 }
 
-inline uint32_t expand_grid(Ctx* c, uint64_t total) {
-  uint64_t tiles = (total + EXP_TILE - 1) / EXP_TILE;
+inline uint32_t expand_grid(Ctx* c, uint64_t total, uint32_t ept = EXP_EPT) {
+  const uint64_t tile = (uint64_t)WG * ept;
+  uint64_t tiles = (total + tile - 1) / tile;
   uint64_t cap = (uint64_t)c->num_cu * 16;
   if (tiles < 1) tiles = 1;
   return (uint32_t)std::min<uint64_t>(tiles, cap);
@@ -3087,6 +3383,8 @@ struct Run {
   unsigned long long coop_edges = 0;  // frontiers the persistent kernel takes (degree sum, largest degree)
   uint32_t coop_deg = 0;
   uint32_t coop_wg = 0;               // its workgroups
+  uint32_t pull_ilp = PULL_ILP;       // launch shapes (kspn_opts)
+  uint32_t ept = EXP_EPT;
   bool pivot_live_deg = false;        // pivots by live degree product (after an exhaustive trim)
   // the first trim's decided count and pivot are still on the device
   // (cnt[C_DECIDED], cnt[C_FLAG]); the first publication of the forward
@@ -3203,6 +3501,99 @@ int run_coop(Run& R, int op, const OpArgs& a, const uint64_t* off, const uint32_
 // With `seeded`, the frontier queue, the level counters and the first
 // level's degree scan are already on the device (k_pivot_reduce's FwbwSeed,
 // k_reach_start).
+// A whole reach as one asynchronous launch (KSPN_FLAG_ASYNC): the
+// frontier F (count cnt[fc], vis bits set) seeds the slot queue.
+int reach_async(Run& R, int op, const OpArgs& a, const uint64_t* off, const uint32_t* adj, const uint32_t* F, int fc,
+                uint32_t f_hint) {
+  Ctx* c = R.c;
+  const uint64_t qcap = (uint64_t)R.n + R.g->m / AS_SLICE + 64;
+  if (qcap > 0xFFFFFFF0ull) {
+    note_error("reach_async", "graph too large for the 32-bit slot queue", __FILE__, __LINE__);
+    return KSPN_E_UNIMPLEMENTED;
+  }
+  if (!c->actl) KSPN_HIP(hipMalloc(&c->actl, sizeof(AsyncCtl)));
+  if (qcap > c->aq_cap) {
+    if (c->aq) (void)hipFree(c->aq);
+    c->aq = nullptr;
+    c->aq_cap = 0;
+    KSPN_TRY(dalloc(&c->aq, qcap));
+    KSPN_HIP(hipMemsetAsync(c->aq, 0xFF, qcap * sizeof(unsigned long long), R.s));  // every slot empty
+    c->aq_cap = qcap;
+  }
+  AsyncCtl* ctl = static_cast<AsyncCtl*>(c->actl);
+  const uint32_t grid = R.o.coop_workgroups ? R.o.coop_workgroups : 512u;
+  {
+    Prof p(c, R.s, K_COOP, 0.0);
+    k_async_init<<<1, 64, 0, R.s>>>(ctl);
+    k_async_seed<<<stream_grid(c, std::max<uint32_t>(f_hint, 1)), WG, 0, R.s>>>(F, &c->cnt[fc], off, c->aq,
+                                                                              (uint32_t)qcap, ctl);
+    if (op == OP_FW) k_reach_async<OP_FW><<<grid, AS_WG, 0, R.s>>>(off, adj, a, c->aq, (uint32_t)qcap, ctl);
+    else if (op == OP_BW) k_reach_async<OP_BW><<<grid, AS_WG, 0, R.s>>>(off, adj, a, c->aq, (uint32_t)qcap, ctl);
+    else k_reach_async<OP_PBW><<<grid, AS_WG, 0, R.s>>>(off, adj, a, c->aq, (uint32_t)qcap, ctl);
+    k_async_done<<<1, 64, 0, R.s>>>(ctl, c->cnt, c->cnt64);
+  }
+  // published, then C_AERR and cnt64[0..2] are zeroed again
+  KSPN_TRY(publish(c, R.s, nullptr, nullptr, 0, nullptr, &c->cnt[C_AERR]));
+  if (c->h_cnt[C_AERR]) {
+    // the queue may hold entries: empty it for the next reach
+    KSPN_HIP(hipMemsetAsync(c->aq, 0xFF, c->aq_cap * sizeof(unsigned long long), R.s));
+    note_error("reach_async",
+               c->h_cnt[C_AERR] & 2u   ? "slot queue overflow"
+               : c->h_cnt[C_AERR] & 1u ? "no progress within the spin limit"
+                                       : "the reach ended with entries not expanded",
+               __FILE__, __LINE__);
+    return KSPN_E_HIP;
+  }
+  const uint64_t edges = c->h_u64[1];
+  if (R.st) {
+    R.st->edges_scanned += edges;
+    R.st->fwbw_levels_coop += 1;
+  }
+  if (c->profile && !c->pending.empty()) {
+    // the reach's profile entry: per claimed vertex its id, offsets and queue slot; per edge an id and a bit
+    for (size_t i = c->pending.size(); i-- > 0;)
+      if (c->pending[i].kern == K_COOP) {
+        c->pending[i].bytes = 28.0 * (double)c->h_u64[2] + 4.125 * (double)edges;
+        break;
+      }
+  }
+  if (R.live_pending) {
+    R.trim_decided = c->h_cnt[C_DECIDED];
+    R.first_pivot = c->h_cnt[C_FLAG];
+    R.live_n = R.n - R.trim_decided;
+    R.live_pending = false;
+  }
+  return KSPN_OK;
+}
+
+// a push level of a reach with the tile width the options ask for
+template <int OP>
+void launch_expand_ept(uint32_t ept, uint32_t grid, hipStream_t s, const uint32_t* F, const uint32_t* fc,
+                       const uint64_t* S, const uint64_t* off, const uint32_t* adj, const OpArgs& a, uint32_t* N,
+                       uint32_t* nc) {
+  if (ept == 4) k_expand<OP, 4><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc);
+  else if (ept == 16) k_expand<OP, 16><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc);
+  else k_expand<OP, EXP_EPT><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc);
+}
+
+void launch_reach_expand(int op, uint32_t ept, uint32_t grid, hipStream_t s, const uint32_t* F, const uint32_t* fc,
+                         const uint64_t* S, const uint64_t* off, const uint32_t* adj, const OpArgs& a, uint32_t* N,
+                         uint32_t* nc) {
+  if (op == OP_FW) launch_expand_ept<OP_FW>(ept, grid, s, F, fc, S, off, adj, a, N, nc);
+  else if (op == OP_PBW) launch_expand_ept<OP_PBW>(ept, grid, s, F, fc, S, off, adj, a, N, nc);
+  else launch_expand_ept<OP_BW>(ept, grid, s, F, fc, S, off, adj, a, N, nc);
+}
+
+// a pull sweep with the width the options ask for
+template <int DIR>
+void launch_pull_ilp(uint32_t ilp, uint32_t grid, hipStream_t s, uint32_t n, const uint32_t* cand, uint32_t* vis,
+                     const uint64_t* roff, const uint32_t* radj, uint32_t* fnew, uint64_t* part, const uint32_t* pid,
+                     uint32_t* cursor, uint32_t* sbm, uint32_t* lcnt, const uint32_t* gate) {
+  if (ilp == 1) k_pull<DIR, 1><<<grid, WG, 0, s>>>(n, cand, vis, roff, radj, nullptr, fnew, part, pid, cursor, sbm, lcnt, gate);
+  else if (ilp == 4) k_pull<DIR, 4><<<grid, WG, 0, s>>>(n, cand, vis, roff, radj, nullptr, fnew, part, pid, cursor, sbm, lcnt, gate);
+  else k_pull<DIR, PULL_ILP><<<grid, WG, 0, s>>>(n, cand, vis, roff, radj, nullptr, fnew, part, pid, cursor, sbm, lcnt, gate);
+}
+
 int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels_push, uint32_t* fw_levels_pull,
           bool seeded = false) {
   Ctx* c = R.c;
@@ -3230,6 +3621,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   uint32_t maxd = 0xFFFFFFFFu;   // largest out-degree in the frontier (unknown)
   const bool coop = R.coop_wg > 0 && !(R.o.flags & KSPN_FLAG_NO_COOP);
   const int op = dir == 0 ? OP_FW : (pid ? OP_PBW : OP_BW);
+  if (R.o.flags & KSPN_FLAG_ASYNC) return reach_async(R, op, a, off, adj, F, fc, f0);
   // A pull sweep leaves its unreached candidates in the bitmap sbm. When
   // they are few, the following pull levels visit only them: they are
   // compacted into a list, and list levels run in batches of LIST_BATCH
@@ -3255,8 +3647,8 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
     // up to 2^25 vertices one resident round of workgroups (2048 beat 1024,
     // 4096 and 16384 on R-MAT 24), above it two (the 4096-graph batch, n =
     // 6e8, lost 2.4 % with one)
-    const uint32_t grid =
-        std::min<uint32_t>(stream_grid(c, R.n), (uint32_t)c->num_cu * (R.n <= (1u << 25) ? 8 : 16));
+    const uint32_t per_cu = R.o.sweep_wgs ? R.o.sweep_wgs : (R.n <= (1u << 25) ? 8u : 16u);
+    const uint32_t grid = std::min<uint32_t>(stream_grid(c, R.n), (uint32_t)c->num_cu * per_cu);
     {
       // candidate and visited bits of every vertex; the offsets of each
       // candidate and the scanned edges are added once they are published
@@ -3266,11 +3658,11 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       // offset array at scattered rows, and the first push level after the
       // pull levels sums its frontier's degrees itself
       if (dir == 0)
-        k_pull<0><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
-                                        &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0], gate);
+        launch_pull_ilp<0>(R.pull_ilp, grid, R.s, R.n, cand, vis, roff, radj, c->fnew, c->part, pid,
+                           &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0], gate);
       else
-        k_pull<1><<<grid, WG, 0, R.s>>>(R.n, cand, vis, roff, radj, nullptr, c->fnew, c->part, pid,
-                                        &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0], gate);
+        launch_pull_ilp<1>(R.pull_ilp, grid, R.s, R.n, cand, vis, roff, radj, c->fnew, c->part, pid,
+                           &c->cnt[C_CURSOR], c->sbm, &c->cnt[C_SURV0], gate);
     }
     Prof p(c, R.s, K_LIVE_LIST, 4.0 * R.words);
     k_sweep_list<<<b2q_grid(R.words), WG, 0, R.s>>>(c->part, grid, &c->cnt[claim_slot], c->cnt64, c->sbm, R.words,
@@ -3281,7 +3673,8 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   auto launch_batch = [&](uint32_t* rec, uint32_t n_in, int* prof) {
     // the levels' scanned edges, for statistics and profiles only
     unsigned long long* esum = R.st || c->profile ? c->cnt64 + 2 : nullptr;
-    const uint32_t lgrid = std::min<uint32_t>(stream_grid(c, n_in / PULL_ILP + 1), (uint32_t)c->num_cu * 4);
+    const uint32_t lgrid = std::min<uint32_t>(stream_grid(c, n_in / PULL_ILP + 1),
+                                              (uint32_t)c->num_cu * (R.o.list_wgs ? R.o.list_wgs : 4u));
     for (uint32_t j = 0; j < LIST_BATCH; ++j, ++L) {
       // the first level zeroes the claim count the last one records
       uint32_t* q = j + 1 == LIST_BATCH ? rec : nullptr;
@@ -3354,14 +3747,10 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       if (f >= 4096 && !scanned) KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)f + 1, R.s));
       scanned = false;
       int kern = dir == 0 ? K_EXPAND_FW : K_EXPAND_BW;
-      uint32_t grid = known ? expand_grid(c, edges) : (uint32_t)c->num_cu * 8;
+      uint32_t grid = known ? expand_grid(c, edges, R.ept) : (uint32_t)c->num_cu * 8;
       Prof p(c, R.s, kern, 20.0 * f + (4.0 + 0.125) * (double)edges);
-      if (dir == 0)
-        k_expand<OP_FW><<<grid, WG, 0, R.s>>>(F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
-      else if (pid)
-        k_expand<OP_PBW><<<grid, WG, 0, R.s>>>(F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
-      else
-        k_expand<OP_BW><<<grid, WG, 0, R.s>>>(F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
+      const int rop = dir == 0 ? OP_FW : (pid ? OP_PBW : OP_BW);
+      launch_reach_expand(rop, R.ept, grid, R.s, F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
       if (fw_levels_push) (*fw_levels_push)++;
     } else {
       bool batch = list_mode;  // a batch of list levels follows (after a sweep: when the remainder is short)
@@ -4621,6 +5010,8 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   Ctx* c = &k->c;
   (void)hipSetDevice(c->device);
   graph_release(c);
+  if (c->aq) (void)hipFree(c->aq);
+  if (c->actl) (void)hipFree(c->actl);
   void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->sbm, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
                   c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf, c->pid, c->pbest,
                   c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt};
@@ -4661,6 +5052,10 @@ int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* o
   if (o.method > KSPN_METHOD_TRIM_EX_LIGHT_RESIDUAL) return KSPN_E_INVALID;
   if (o.pivot > KSPN_PIVOT_RANDOM) return KSPN_E_INVALID;
   if (o.pivot == KSPN_PIVOT_LEARNED && !o.mlp_params) return KSPN_E_INVALID;
+  if ((o.pull_ilp && o.pull_ilp != 1 && o.pull_ilp != 2 && o.pull_ilp != 4) ||
+      (o.expand_ept && o.expand_ept != 4 && o.expand_ept != 8 && o.expand_ept != 16) || o.sweep_wgs > 64 ||
+      o.list_wgs > 64)
+    return KSPN_E_INVALID;
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (g->n == 0) return KSPN_OK;
   if (vtx_off && (count == 0 || count > g->n)) return KSPN_E_INVALID;
@@ -4679,6 +5074,8 @@ int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* o
   R.coop_edges = o.coop_edges ? o.coop_edges : kCoopEdges;
   R.coop_deg = o.coop_degree ? o.coop_degree : kCoopDeg;
   R.coop_wg = std::min<uint32_t>(o.coop_workgroups ? o.coop_workgroups : kCoopWorkgroups, c->coop_grid);
+  R.pull_ilp = o.pull_ilp ? o.pull_ilp : (uint32_t)PULL_ILP;
+  R.ept = o.expand_ept ? o.expand_ept : (uint32_t)EXP_EPT;
   if (vtx_off) {
     // graph i is preset partition i of the multi-pivot FW-BW rounds
     if (!c->pid) KSPN_TRY(dalloc(&c->pid, (uint64_t)c->cap_n + 64));
