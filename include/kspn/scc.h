@@ -88,6 +88,12 @@ typedef struct kspn_graph {
  * take the claimed vertices' row slices from a queue as they appear (coop_workgroups workgroups of 256
  * threads, 0 -> 512). The reached sets, and so the labels, are the same as level by level. */
 #define KSPN_FLAG_ASYNC 256u
+/* The push levels the persistent multi-level kernel would take (frontier degree sum <= coop_edges, largest
+ * degree <= coop_degree) run instead as chains of chain_levels launches, one level each, enqueued back to back
+ * with a single host round trip per chain: no grid barrier, and the coop_workgroups workgroups need not be
+ * resident together. A level that ends the run (empty frontier, pull switch, frontier too large) makes the
+ * rest of its chain return at once. Same reached sets and labels. */
+#define KSPN_FLAG_CHAIN 512u
 
 typedef struct kspn_opts {
   uint32_t method;             /* KSPN_METHOD_* */
@@ -111,6 +117,7 @@ typedef struct kspn_opts {
   uint32_t expand_ept;         /* push expansion: edges per thread of a workgroup tile, 4, 8 or 16 (0 -> 8) */
   uint32_t sweep_wgs;          /* pull sweep: workgroups per CU (0 -> 8, 16 above 2^25 vertices) */
   uint32_t list_wgs;           /* pull list levels: workgroups per CU at most (0 -> 4) */
+  uint32_t chain_levels;       /* KSPN_FLAG_CHAIN: push levels enqueued per host round trip (0 -> 16, at most 256) */
 } kspn_opts;
 
 /* pivot outcome record: pivot, round, partition size, SCC size (decided),

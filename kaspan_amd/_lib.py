@@ -44,6 +44,7 @@ FLAG_STATS_MEMORY = 32
 FLAG_FULL_BW = 64
 FLAG_GRAPH = 128
 FLAG_ASYNC = 256
+FLAG_CHAIN = 512
 PHASE_MAX = 12
 MANIFEST_TEXT = 256
 MANIFEST_PATH = 4096
@@ -130,7 +131,7 @@ class Opts(C.Structure):
         ("pull_ilp", C.c_uint32),
         ("expand_ept", C.c_uint32),
         ("sweep_wgs", C.c_uint32),
-        ("list_wgs", C.c_uint32),
+        ("list_wgs", C.c_uint32), ("chain_levels", C.c_uint32),
     ]
 
 

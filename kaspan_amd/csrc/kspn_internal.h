@@ -351,6 +351,7 @@ struct CoopCtl {
   uint32_t levels, reason, err, pad;
   unsigned long long lexp[4];   // edges expanded by the wavefronts' local rounds of a level
   unsigned long long lvert[4];  // vertices expanded by them
+  double mrest[4];              // chained launches: edges not yet expanded at the level of each slot
 };
 
 // host helpers implemented in kspn_scc.hip

@@ -675,25 +675,191 @@ __device__ inline void coop_flush(uint32_t* sq, uint32_t& qn, uint32_t* q, uint3
   qn = 0;
 }
 
+// LDS of a workgroup expanding one level (persistent or chained launches)
+struct CoopLds {
+  uint64_t ex[COOP_WAVES][WAVE];
+  uint64_t beg[COOP_WAVES][WAVE];
+  uint32_t v[COOP_WAVES][WAVE];
+  uint32_t q[COOP_WAVES][COOP_QCAP];
+  uint32_t qn[COOP_WAVES];
+  unsigned long long sum[COOP_WAVES];
+  unsigned long long lexp[COOP_WAVES], lvert[COOP_WAVES];
+  uint32_t max[COOP_WAVES];
+  uint32_t base;
+};
+
+// One level of a reach by every workgroup of the launch: the frontier F
+// (f vertices, out-degree sum fe) into N, whose count, degree sum, largest
+// degree and local-round totals go to ring slot s1; chunks beyond the
+// first pass come from ctl->work[s0]. `waves` is the launch's wavefront
+// count, the same in every workgroup.
+template <int OP>
+__device__ void coop_level(CoopLds& S, uint32_t n, const uint32_t* F, uint32_t* N, uint32_t f,
+                           unsigned long long fe, const uint64_t* __restrict__ off,
+                           const uint32_t* __restrict__ adj, const OpArgs& a, CoopCtl* ctl, uint32_t s0,
+                           uint32_t s1, uint32_t d_max) {
+  const uint32_t wid = threadIdx.x / WAVE, lane = lane_id();
+  const uint64_t below = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+  const uint32_t waves = gridDim.x * COOP_WAVES;
+  const uint32_t gw = blockIdx.x * COOP_WAVES + wid;
+  uint64_t my_sum = 0;
+  uint32_t my_max = 0, qn = 0;
+  // edge-balanced expansion of up to 64 sources (one per lane: v, row
+  // start b, degree d); claims go to the wavefront's LDS queue, a full
+  // queue out to the next frontier
+  auto expand = [&](uint32_t v, uint64_t b, uint64_t d) {
+    const uint64_t incl = wave_incl_scan64(d);
+    const uint64_t total = __shfl(incl, WAVE - 1);
+    S.ex[wid][lane] = incl - d;
+    S.beg[wid][lane] = b;
+    S.v[wid][lane] = v;
+    __builtin_amdgcn_wave_barrier();
+    uint32_t lo = 0;
+    for (uint64_t e0 = 0; e0 < total; e0 += WAVE) {
+      const uint64_t e = e0 + lane;
+      bool push = false;
+      uint32_t w = 0;
+      if (e < total) {
+        uint32_t hi = WAVE;  // last source whose edges start at or before e
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (S.ex[wid][mid] <= e) lo = mid; else hi = mid;
+        }
+        w = adj[S.beg[wid][lo] + (e - S.ex[wid][lo])];
+        push = reach_claim<OP>(S.v[wid][lo], w, a);
+        if (push) {
+          const uint64_t dd = a.noff[w + 1] - a.noff[w];
+          my_sum += dd;
+          my_max = max(my_max, (uint32_t)min(dd, (uint64_t)0xFFFFFFFFu));
+        }
+      }
+      const uint64_t mask = __ballot(push);
+      if (mask) {
+        if (push) S.q[wid][qn + __popcll(mask & below)] = w;
+        qn += (uint32_t)__popcll(mask);
+        if (qn > COOP_QCAP - WAVE) coop_flush(S.q[wid], qn, N, &ctl->cnt[s1]);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    return total;
+  };
+  // chunk gw first (no atomic); beyond the first pass chunks come from a counter
+  const uint64_t nchunks = ((uint64_t)f + WAVE - 1) / WAVE;
+  uint64_t chunk = gw;
+  while (chunk < nchunks) {
+    const uint64_t i = chunk * WAVE + lane;
+    uint32_t v = 0;
+    uint64_t b = 0, d = 0;
+    if (i < f) {
+      v = __hip_atomic_load(&F[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (v < n) {  // always true; an id out of range would be skipped, not used as an index
+        b = off[v];
+        d = off[v + 1] - b;
+      }
+    }
+    expand(v, b, d);
+    if (nchunks <= waves) break;
+    uint32_t k = 0;
+    if (lane == 0) k = atomicAdd(&ctl->work[s0], 1u);
+    chunk = (uint64_t)waves + __shfl(k, 0);
+  }
+  // Reachability needs each claimed vertex expanded once, not level by
+  // level. On a thin frontier of low degree (at most 16 vertices per
+  // wavefront, at most 4 edges per vertex: long-diameter graphs) a
+  // wavefront goes on with its own claims for a few rounds while they fit
+  // one round, so such a frontier takes fewer levels; other frontiers
+  // wait for the next level, where all wavefronts share them. A claim of
+  // degree above d_max stays for the next level (its degree was counted
+  // for it).
+  uint64_t loc_e = 0, loc_v = 0;
+  const bool thin = f <= 16ull * waves && fe <= 4ull * f;
+  for (uint32_t r = 0; thin && r < COOP_LOCAL && qn > 0 && qn <= WAVE; ++r) {
+    const uint32_t take = min(qn, (uint32_t)WAVE);
+    qn -= take;
+    const bool have = lane < take;
+    uint32_t v = have ? S.q[wid][qn + lane] : 0u;
+    uint64_t b = 0, d = 0;
+    if (have) {
+      b = off[v];
+      d = off[v + 1] - b;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const bool big = have && d > d_max;
+    const uint64_t bm = __ballot(big);
+    if (bm) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(&ctl->cnt[s1], (uint32_t)__popcll(bm));
+      base = __shfl(base, 0);
+      if (big) __hip_atomic_store(&N[base + __popcll(bm & below)], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (have && !big) my_sum -= d;  // not in the next frontier after all (a.noff is off)
+    loc_v += (uint32_t)__popcll(__ballot(have && !big));
+    loc_e += expand(v, b, big ? 0 : d);
+  }
+  // the workgroup's staged entries, degree sum and maximum: one atomic each
+  my_sum = wave_sum64(my_sum);
+  for (int o = 32; o > 0; o >>= 1) my_max = max(my_max, __shfl_down(my_max, o));
+  if (lane == 0) {
+    S.sum[wid] = my_sum;
+    S.max[wid] = my_max;
+    S.qn[wid] = qn;
+    S.lexp[wid] = loc_e;
+    S.lvert[wid] = loc_v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long bs = 0, le = 0, lv = 0;
+    uint32_t bm = 0, bq = 0;
+    for (int k = 0; k < COOP_WAVES; ++k) {
+      bs += S.sum[k];
+      bm = max(bm, S.max[k]);
+      le += S.lexp[k];
+      lv += S.lvert[k];
+      const uint32_t q = S.qn[k];
+      S.qn[k] = bq;  // exclusive prefix
+      bq += q;
+    }
+    if (bs) atomicAdd(&ctl->nsum[s1], bs);
+    if (bm) atomicMax(&ctl->maxd[s1], bm);
+    if (le) atomicAdd(&ctl->lexp[s1], le);
+    if (lv) atomicAdd(&ctl->lvert[s1], lv);
+    S.base = bq ? atomicAdd(&ctl->cnt[s1], bq) : 0u;
+  }
+  __syncthreads();
+  {
+    const uint32_t dst = S.base + S.qn[wid];
+    for (uint32_t k = lane; k < qn; k += WAVE)
+      __hip_atomic_store(&N[dst + k], S.q[wid][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// why a run of push levels on the device ends before the next level (the
+// same in every workgroup: computed from values written before the level)
+__device__ inline uint32_t coop_reason(uint32_t f, unsigned long long fe, uint32_t fmax, double m_rest, float alpha,
+                                       unsigned long long e_max, uint32_t d_max) {
+  if (f == 0) return COOP_EMPTY;
+  if (f > 64 && (double)fe * alpha > m_rest) return COOP_PULL;
+  if (fe > e_max || fmax > d_max) return COOP_BIG;
+  return 0;
+}
+
+// the ring slot after next (last read before this level) cleared for reuse
+__device__ inline void coop_clear_slot(CoopCtl* ctl, uint32_t s2) {
+  __hip_atomic_store(&ctl->cnt[s2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&ctl->work[s2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&ctl->nsum[s2], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&ctl->maxd[s2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&ctl->lexp[s2], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&ctl->lvert[s2], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int OP>
 __global__ __launch_bounds__(COOP_WG) void k_reach_coop(uint32_t n, uint32_t* F0, uint32_t* F1, const uint64_t* __restrict__ off,
                                                         const uint32_t* __restrict__ adj, OpArgs a, CoopCtl* ctl,
                                                         double m_rest, float alpha, unsigned long long e_max,
                                                         uint32_t d_max) {
-  __shared__ uint64_t s_ex[COOP_WAVES][WAVE];
-  __shared__ uint64_t s_beg[COOP_WAVES][WAVE];
-  __shared__ uint32_t s_v[COOP_WAVES][WAVE];
-  __shared__ uint32_t s_q[COOP_WAVES][COOP_QCAP];
-  __shared__ uint32_t s_qn[COOP_WAVES];
-  __shared__ unsigned long long s_sum[COOP_WAVES];
-  __shared__ unsigned long long s_lexp[COOP_WAVES], s_lvert[COOP_WAVES];
-  __shared__ uint32_t s_max[COOP_WAVES];
-  __shared__ uint32_t s_base;
+  __shared__ CoopLds S;
   __shared__ CoopLevel s_lv;
-  const uint32_t wid = threadIdx.x / WAVE, lane = lane_id();
-  const uint64_t below = lane ? ((~0ull) >> (64 - lane)) : 0ull;
-  const uint32_t waves = gridDim.x * COOP_WAVES;          // wavefronts of the launch
-  const uint32_t gw = blockIdx.x * COOP_WAVES + wid;      // this wavefront's index
   unsigned long long expanded = 0, verts = 0;
   uint32_t L = 0;
   if (threadIdx.x == 0) coop_read_level(ctl, 0, &s_lv);  // written by the init kernel before the launch
@@ -707,10 +873,7 @@ __global__ __launch_bounds__(COOP_WG) void k_reach_coop(uint32_t n, uint32_t* F0
     verts += s_lv.lvert;
     m_rest -= (double)s_lv.lexp;
     if (s_lv.err) return;  // a workgroup timed out at a barrier
-    uint32_t reason = 0;   // identical in every workgroup: read after the same barrier
-    if (f == 0) reason = COOP_EMPTY;
-    else if (f > 64 && (double)fe * alpha > m_rest) reason = COOP_PULL;
-    else if (fe > e_max || fmax > d_max) reason = COOP_BIG;
+    const uint32_t reason = coop_reason(f, fe, fmax, m_rest, alpha, e_max, d_max);
     if (reason) {
       if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->levels = L;
@@ -720,145 +883,8 @@ __global__ __launch_bounds__(COOP_WG) void k_reach_coop(uint32_t n, uint32_t* F0
       }
       return;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {  // the ring slot after next: last read before the previous barrier
-      __hip_atomic_store(&ctl->cnt[s2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctl->work[s2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctl->nsum[s2], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctl->maxd[s2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctl->lexp[s2], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctl->lvert[s2], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    uint32_t* F = (L & 1) ? F1 : F0;
-    uint32_t* N = (L & 1) ? F0 : F1;
-    uint64_t my_sum = 0;
-    uint32_t my_max = 0, qn = 0;
-    // edge-balanced expansion of up to 64 sources (one per lane: v, row
-    // start b, degree d); claims go to the wavefront's LDS queue, a full
-    // queue out to the next frontier
-    auto expand = [&](uint32_t v, uint64_t b, uint64_t d) {
-      const uint64_t incl = wave_incl_scan64(d);
-      const uint64_t total = __shfl(incl, WAVE - 1);
-      s_ex[wid][lane] = incl - d;
-      s_beg[wid][lane] = b;
-      s_v[wid][lane] = v;
-      __builtin_amdgcn_wave_barrier();
-      uint32_t lo = 0;
-      for (uint64_t e0 = 0; e0 < total; e0 += WAVE) {
-        const uint64_t e = e0 + lane;
-        bool push = false;
-        uint32_t w = 0;
-        if (e < total) {
-          uint32_t hi = WAVE;  // last source whose edges start at or before e
-          while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (s_ex[wid][mid] <= e) lo = mid; else hi = mid;
-          }
-          w = adj[s_beg[wid][lo] + (e - s_ex[wid][lo])];
-          push = reach_claim<OP>(s_v[wid][lo], w, a);
-          if (push) {
-            const uint64_t dd = a.noff[w + 1] - a.noff[w];
-            my_sum += dd;
-            my_max = max(my_max, (uint32_t)min(dd, (uint64_t)0xFFFFFFFFu));
-          }
-        }
-        const uint64_t mask = __ballot(push);
-        if (mask) {
-          if (push) s_q[wid][qn + __popcll(mask & below)] = w;
-          qn += (uint32_t)__popcll(mask);
-          if (qn > COOP_QCAP - WAVE) coop_flush(s_q[wid], qn, N, &ctl->cnt[s1]);
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-      return total;
-    };
-    // chunk gw first (no atomic); beyond the first pass chunks come from a counter
-    const uint64_t nchunks = ((uint64_t)f + WAVE - 1) / WAVE;
-    uint64_t chunk = gw;
-    while (chunk < nchunks) {
-      const uint64_t i = chunk * WAVE + lane;
-      uint32_t v = 0;
-      uint64_t b = 0, d = 0;
-      if (i < f) {
-        v = __hip_atomic_load(&F[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (v < n) {  // always true; an id out of range would be skipped, not used as an index
-          b = off[v];
-          d = off[v + 1] - b;
-        }
-      }
-      expand(v, b, d);
-      if (nchunks <= waves) break;
-      uint32_t k = 0;
-      if (lane == 0) k = atomicAdd(&ctl->work[s0], 1u);
-      chunk = (uint64_t)waves + __shfl(k, 0);
-    }
-    // Reachability needs each claimed vertex expanded once, not level by
-    // level. On a thin frontier of low degree (at most 16 vertices per
-    // wavefront, at most 4 edges per vertex: long-diameter graphs) a
-    // wavefront goes on with its own claims for a few rounds while they fit
-    // one round, so such a frontier takes fewer barriers; other frontiers
-    // wait for the next level, where all wavefronts share them. A claim of
-    // degree above d_max stays for the next level (its degree was counted
-    // for it).
-    uint64_t loc_e = 0, loc_v = 0;
-    const bool thin = f <= 16ull * waves && fe <= 4ull * f;
-    for (uint32_t r = 0; thin && r < COOP_LOCAL && qn > 0 && qn <= WAVE; ++r) {
-      const uint32_t take = min(qn, (uint32_t)WAVE);
-      qn -= take;
-      const bool have = lane < take;
-      uint32_t v = have ? s_q[wid][qn + lane] : 0u;
-      uint64_t b = 0, d = 0;
-      if (have) {
-        b = off[v];
-        d = off[v + 1] - b;
-      }
-      __builtin_amdgcn_wave_barrier();
-      const bool big = have && d > d_max;
-      const uint64_t bm = __ballot(big);
-      if (bm) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&ctl->cnt[s1], (uint32_t)__popcll(bm));
-        base = __shfl(base, 0);
-        if (big) __hip_atomic_store(&N[base + __popcll(bm & below)], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      if (have && !big) my_sum -= d;  // not in the next frontier after all (a.noff is off)
-      loc_v += (uint32_t)__popcll(__ballot(have && !big));
-      loc_e += expand(v, b, big ? 0 : d);
-    }
-    // the workgroup's staged entries, degree sum and maximum: one atomic each
-    my_sum = wave_sum64(my_sum);
-    for (int o = 32; o > 0; o >>= 1) my_max = max(my_max, __shfl_down(my_max, o));
-    if (lane == 0) {
-      s_sum[wid] = my_sum;
-      s_max[wid] = my_max;
-      s_qn[wid] = qn;
-      s_lexp[wid] = loc_e;
-      s_lvert[wid] = loc_v;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long bs = 0, le = 0, lv = 0;
-      uint32_t bm = 0, bq = 0;
-      for (int k = 0; k < COOP_WAVES; ++k) {
-        bs += s_sum[k];
-        bm = max(bm, s_max[k]);
-        le += s_lexp[k];
-        lv += s_lvert[k];
-        const uint32_t q = s_qn[k];
-        s_qn[k] = bq;  // exclusive prefix
-        bq += q;
-      }
-      if (bs) atomicAdd(&ctl->nsum[s1], bs);
-      if (bm) atomicMax(&ctl->maxd[s1], bm);
-      if (le) atomicAdd(&ctl->lexp[s1], le);
-      if (lv) atomicAdd(&ctl->lvert[s1], lv);
-      s_base = bq ? atomicAdd(&ctl->cnt[s1], bq) : 0u;
-    }
-    __syncthreads();
-    {
-      const uint32_t dst = s_base + s_qn[wid];
-      for (uint32_t k = lane; k < qn; k += WAVE)
-        __hip_atomic_store(&N[dst + k], s_q[wid][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) coop_clear_slot(ctl, s2);
+    coop_level<OP>(S, n, (L & 1) ? F1 : F0, (L & 1) ? F0 : F1, f, fe, off, adj, a, ctl, s0, s1, d_max);
     expanded += fe;
     verts += f;
     m_rest -= (double)fe;
@@ -875,6 +901,61 @@ __global__ __launch_bounds__(COOP_WG) void k_reach_coop(uint32_t n, uint32_t* F0
       }
       return;
     }
+  }
+}
+
+// Chained push levels: level L of a run as a launch of its own, enqueued
+// back to back with the others of the chain and no host round trip in
+// between; the kernel boundaries order one level's writes before the next
+// level's reads, so no grid barrier or fence is needed and the grid need
+// not be resident at once. Each launch reads its level's values from ring
+// slot L % 3 and the remaining-edge estimate from ctl->mrest[L % 3]; the
+// first launch whose level ends the run records levels and reason, and the
+// launches after it return at once. Block 0 keeps the run's totals.
+template <int OP>
+__global__ __launch_bounds__(COOP_WG) void k_chain_level(uint32_t n, uint32_t* F0, uint32_t* F1,
+                                                         const uint64_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ adj, OpArgs a, CoopCtl* ctl,
+                                                         float alpha, unsigned long long e_max, uint32_t d_max,
+                                                         uint32_t L) {
+  __shared__ CoopLds S;
+  __shared__ CoopLevel s_lv;
+  __shared__ double s_m;
+  if (threadIdx.x == 0) {
+    const uint32_t ended = ctl->reason;  // set by an earlier launch of the chain
+    coop_read_level(ctl, L % 3, &s_lv);
+    s_lv.err = (ended || s_lv.err) ? 1u : 0u;
+    s_m = ctl->mrest[L % 3] - (double)s_lv.lexp;
+  }
+  __syncthreads();
+  if (s_lv.err) return;
+  const uint32_t s0 = L % 3, s1 = (L + 1) % 3, s2 = (L + 2) % 3;
+  const uint32_t f = s_lv.f;
+  const unsigned long long fe = s_lv.fe;
+  const uint32_t reason = coop_reason(f, fe, s_lv.fmax, s_m, alpha, e_max, d_max);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // the local rounds of the level before, then (unless the run ends here) this level
+    ctl->expanded += s_lv.lexp + (reason ? 0ull : fe);
+    ctl->verts += s_lv.lvert + (reason ? 0ull : f);
+    if (reason) {
+      ctl->levels = L;
+      ctl->reason = reason;
+    } else {
+      ctl->mrest[s1] = s_m - (double)fe;
+      coop_clear_slot(ctl, s2);
+    }
+  }
+  if (reason) return;
+  coop_level<OP>(S, n, (L & 1) ? F1 : F0, (L & 1) ? F0 : F1, f, fe, off, adj, a, ctl, s0, s1, d_max);
+}
+
+// after a chain of `len` launches that did not end the run: its levels,
+// and the local rounds of its last level in the totals
+__global__ void k_chain_end(CoopCtl* ctl, uint32_t len) {
+  if (threadIdx.x == 0 && ctl->reason == 0) {
+    ctl->levels = len;
+    ctl->expanded += ctl->lexp[len % 3];
+    ctl->verts += ctl->lvert[len % 3];
   }
 }
 
@@ -1174,7 +1255,8 @@ __global__ void k_async_done(const AsyncCtl* ctl, uint32_t* cnt, unsigned long l
   }
 }
 
-__global__ void k_coop_init(CoopCtl* ctl, const uint32_t* fcount, unsigned long long edges, uint32_t maxd) {
+__global__ void k_coop_init(CoopCtl* ctl, const uint32_t* fcount, unsigned long long edges, uint32_t maxd,
+                            double m_rest = 0.0) {
   if (threadIdx.x == 0) {
     for (int k = 0; k < 4; ++k) {
       ctl->cnt[k] = 0;
@@ -1191,6 +1273,7 @@ __global__ void k_coop_init(CoopCtl* ctl, const uint32_t* fcount, unsigned long 
     ctl->verts = 0;
     ctl->bar_count = 0;
     ctl->levels = ctl->reason = ctl->err = 0;
+    ctl->mrest[0] = m_rest;
   }
 }
 
@@ -3385,6 +3468,7 @@ struct Run {
   uint32_t coop_wg = 0;               // its workgroups
   uint32_t pull_ilp = PULL_ILP;       // launch shapes (kspn_opts)
   uint32_t ept = EXP_EPT;
+  uint32_t chain = 0;                 // KSPN_FLAG_CHAIN: launches per chain of push levels (0: persistent kernel)
   bool pivot_live_deg = false;        // pivots by live degree product (after an exhaustive trim)
   // the first trim's decided count and pivot are still on the device
   // (cnt[C_DECIDED], cnt[C_FLAG]); the first publication of the forward
@@ -3453,6 +3537,7 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b) {
 constexpr uint64_t kCoopEdges = 1ull << 20;
 constexpr uint32_t kCoopDeg = 2048;
 constexpr uint32_t kCoopWorkgroups = 128;
+constexpr uint32_t kChainLevels = 16;  // launches per chain of push levels (KSPN_FLAG_CHAIN)
 
 struct CoopOut {
   uint32_t levels, reason, f, maxd;
@@ -3461,13 +3546,28 @@ struct CoopOut {
 
 // push levels of one reachability in one persistent launch, starting from
 // the frontier F (count at cnt[fc]) whose degree sum and maximum are known
+// (KSPN_FLAG_CHAIN: up to R.chain levels as launches of their own, enqueued
+// together, then one publication)
 int run_coop(Run& R, int op, const OpArgs& a, const uint64_t* off, const uint32_t* adj, uint32_t* F, uint32_t* N,
              int fc, int nc, uint64_t edges, uint32_t maxd, double m_rest, double alpha, CoopOut* out) {
   Ctx* c = R.c;
-  k_coop_init<<<1, 64, 0, R.s>>>(c->coop, &c->cnt[fc], edges, maxd);
+  k_coop_init<<<1, 64, 0, R.s>>>(c->coop, &c->cnt[fc], edges, maxd, m_rest);
   KSPN_HIP(hipGetLastError());
   const float al = (float)alpha;
-  {
+  if (R.chain) {
+    Prof p(c, R.s, K_COOP, 0.0);
+    const dim3 grid(R.coop_wg), block(COOP_WG);
+    for (uint32_t L = 0; L < R.chain; ++L) {
+      if (op == OP_FW)
+        k_chain_level<OP_FW><<<grid, block, 0, R.s>>>(R.n, F, N, off, adj, a, c->coop, al, R.coop_edges, R.coop_deg, L);
+      else if (op == OP_BW)
+        k_chain_level<OP_BW><<<grid, block, 0, R.s>>>(R.n, F, N, off, adj, a, c->coop, al, R.coop_edges, R.coop_deg, L);
+      else
+        k_chain_level<OP_PBW><<<grid, block, 0, R.s>>>(R.n, F, N, off, adj, a, c->coop, al, R.coop_edges, R.coop_deg, L);
+    }
+    k_chain_end<<<1, 64, 0, R.s>>>(c->coop, R.chain);
+    KSPN_HIP(hipGetLastError());
+  } else {
     Prof p(c, R.s, K_COOP, 0.0);
     const dim3 grid(R.coop_wg), block(COOP_WG);
     if (op == OP_FW)
@@ -3619,7 +3719,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   const double beta = R.o.beta > 0 ? R.o.beta : 24.0;
   double m_rest = (double)g->m;  // edges not yet expanded or reached
   uint32_t maxd = 0xFFFFFFFFu;   // largest out-degree in the frontier (unknown)
-  const bool coop = R.coop_wg > 0 && !(R.o.flags & KSPN_FLAG_NO_COOP);
+  const bool coop = (R.coop_wg > 0 || R.chain) && !(R.o.flags & KSPN_FLAG_NO_COOP);
   const int op = dir == 0 ? OP_FW : (pid ? OP_PBW : OP_BW);
   if (R.o.flags & KSPN_FLAG_ASYNC) return reach_async(R, op, a, off, adj, F, fc, f0);
   // A pull sweep leaves its unreached candidates in the bitmap sbm. When
@@ -5054,7 +5154,7 @@ int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* o
   if (o.pivot == KSPN_PIVOT_LEARNED && !o.mlp_params) return KSPN_E_INVALID;
   if ((o.pull_ilp && o.pull_ilp != 1 && o.pull_ilp != 2 && o.pull_ilp != 4) ||
       (o.expand_ept && o.expand_ept != 4 && o.expand_ept != 8 && o.expand_ept != 16) || o.sweep_wgs > 64 ||
-      o.list_wgs > 64)
+      o.list_wgs > 64 || o.chain_levels > 256)
     return KSPN_E_INVALID;
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (g->n == 0) return KSPN_OK;
@@ -5076,6 +5176,11 @@ int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* o
   R.coop_wg = std::min<uint32_t>(o.coop_workgroups ? o.coop_workgroups : kCoopWorkgroups, c->coop_grid);
   R.pull_ilp = o.pull_ilp ? o.pull_ilp : (uint32_t)PULL_ILP;
   R.ept = o.expand_ept ? o.expand_ept : (uint32_t)EXP_EPT;
+  if (o.flags & KSPN_FLAG_CHAIN) {
+    // chained launches need no co-residency: the grid is not capped by it
+    R.chain = o.chain_levels ? o.chain_levels : kChainLevels;
+    R.coop_wg = o.coop_workgroups ? o.coop_workgroups : kCoopWorkgroups;
+  }
   if (vtx_off) {
     // graph i is preset partition i of the multi-pivot FW-BW rounds
     if (!c->pid) KSPN_TRY(dalloc(&c->pid, (uint64_t)c->cap_n + 64));

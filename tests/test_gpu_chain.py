@@ -1,0 +1,86 @@
+"""Chained push levels (KSPN_FLAG_CHAIN): the push levels the persistent
+kernel would take run as chains of single-level launches with one host
+round trip per chain. Labels must equal the oracle's Tarjan labels (planted
+labels on the fuzzy grid) for every chain length, including chains that end
+early and chains of one level, single graphs and batches."""
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+CHAIN = 512  # KSPN_FLAG_CHAIN
+
+
+def host(O, g):
+    fo, fa, bo, ba = g.to_host()
+    return O.HostGraph(g.n, fo, fa, bo, ba)
+
+
+def lab_np(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+def test_chain_fuzzy_grid(ka, O):
+    """the reference's fuzzy grid (test_scc_fuzzy.cpp:31-80): planted labels"""
+    for n in (1, 6, 8, 16, 33, 64, 300):
+        for run in range(6):
+            hg, planted = O.gen_fuzzy(3000 * n + run, n)
+            g = ka.BidiGraph.from_host(hg.n, hg.fw_off, hg.fw_adj, hg.bw_off, hg.bw_adj)
+            for rounds, chain in ((1, 0), (4, 1), (4, 3)):
+                lab, st = ka.scc(g, flags=CHAIN, fwbw_rounds=rounds, chain_levels=chain)
+                assert np.array_equal(lab_np(lab), planted), (n, run, rounds, chain)
+
+
+@pytest.mark.parametrize("kind", ["gnm", "rmat", "lattice", "planted", "cycle", "dag"])
+@pytest.mark.parametrize("opts", [dict(), dict(fwbw_rounds=8), dict(method=1), dict(chain_levels=1),
+                                  dict(chain_levels=64, coop_edges=1 << 12), dict(fwbw_rounds=4, flags=64),
+                                  dict(coop_workgroups=7)])
+def test_chain_single_graphs(ka, O, kind, opts):
+    if kind == "gnm":
+        g = ka.generate("gnm", 7, n=20000, draws=60000)
+    elif kind == "rmat":
+        g = ka.generate("rmat", 8, scale=15, draws=16 << 15)
+    elif kind == "lattice":
+        g = ka.generate("lattice", 9, side=24, p_bidir=0.1)
+    elif kind == "planted":
+        hg0, _ = O.gen_planted(16, 40000, max_size=8000)
+        g = ka.BidiGraph.from_host(hg0.n, hg0.fw_off, hg0.fw_adj, hg0.bw_off, hg0.bw_adj)
+    elif kind == "cycle":
+        off, adj = H.cycle(5000)
+        g = ka.BidiGraph.from_host(5000, off, adj)
+    else:
+        off, adj = H.forward_dag(3000, 9000, np.random.default_rng(3))
+        g = ka.BidiGraph.from_host(3000, off, adj)
+    ref = O.tarjan(host(O, g))
+    o = dict(opts)
+    flags = CHAIN | o.pop("flags", 0)
+    lab, st = ka.scc(g, flags=flags, **o)
+    assert np.array_equal(lab_np(lab), ref)
+    assert st.decided_trim + st.decided_trim2 + st.decided_fwbw + st.decided_color + st.decided_residual == g.n
+    plain, pst = ka.scc(g, **{k: v for k, v in opts.items() if k not in ("chain_levels",)})
+    assert np.array_equal(lab_np(plain), ref)
+
+
+def test_chain_levels_counted_and_batches(ka, O):
+    """a long-diameter graph runs its levels in chains (counted as device
+    levels); batches keep labels local per graph"""
+    off, adj = H.cycle(20000)
+    g = ka.BidiGraph.from_host(20000, off, adj)
+    lab, st = ka.scc(g, flags=CHAIN, chain_levels=8)
+    assert np.array_equal(lab_np(lab), np.zeros(20000, np.uint32))
+    assert st.fwbw_levels_coop > 0
+    kw = dict(n_min=1000, n_max=60_000, deg_min=2.0, deg_max=16.0)
+    gb = ka.generate_batch(0x6B5350AA00000004, 128, **kw)
+    u, vo, eo, graphs = O.gen_batch(0x6B5350AA00000004, 128, **kw)
+    ref = np.concatenate([O.tarjan(x) for x in graphs])
+    lab, st = ka.scc_batch(gb, flags=CHAIN)
+    assert np.array_equal(lab_np(lab), ref)
+
+
+def test_chain_option_bounds(ka):
+    g = ka.generate("gnm", 5, n=4000, draws=12000)
+    with pytest.raises(ka.KspnError) as ei:
+        ka.scc(g, flags=CHAIN, chain_levels=257)
+    assert ei.value.code == -1
