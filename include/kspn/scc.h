@@ -114,7 +114,7 @@ typedef struct kspn_opts {
   uint32_t coop_workgroups;    /* its workgroups, at most one per CU (0 -> 128) */
   /* launch shapes (the autotuner's; 0 -> the default) */
   uint32_t pull_ilp;           /* pull sweep: vertices in flight per thread, 1, 2 or 4 (0 -> 2) */
-  uint32_t expand_ept;         /* push expansion: edges per thread of a workgroup tile, 4, 8 or 16 (0 -> 8) */
+  uint32_t expand_ept;         /* push expansion: edges per thread of a workgroup tile, 2, 4, 8 or 16 (0 -> per level) */
   uint32_t sweep_wgs;          /* pull sweep: workgroups per CU (0 -> 8, 16 above 2^25 vertices) */
   uint32_t list_wgs;           /* pull list levels: workgroups per CU at most (0 -> 4) */
   uint32_t chain_levels;       /* KSPN_FLAG_CHAIN: push levels enqueued per host round trip (0 -> 16, at most 256) */

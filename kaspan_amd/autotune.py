@@ -42,12 +42,12 @@ SPACE = {
 # grid sizes (workgroups per CU) the library selects at run time; 0 is the library's default
 SHAPES = {
     "pull_ilp": [2, 1, 4],
-    "expand_ept": [8, 4, 16],
+    "expand_ept": [0, 8, 4, 2, 16],
     "sweep_wgs": [0, 4, 16],
     "list_wgs": [0, 2, 8],
 }
 SPACE.update(SHAPES)
-DEFAULTS = {"pull_ilp": 2, "expand_ept": 8, "sweep_wgs": 0, "list_wgs": 0, "alpha": 14.0, "beta": 24.0, "coop_edges": 1 << 20, "coop_workgroups": 128, "fwbw_rounds": 1,
+DEFAULTS = {"pull_ilp": 2, "expand_ept": 0, "sweep_wgs": 0, "list_wgs": 0, "alpha": 14.0, "beta": 24.0, "coop_edges": 1 << 20, "coop_workgroups": 128, "fwbw_rounds": 1,
             "tail_threshold": 4096, "method": 0, "flags": 0}
 
 

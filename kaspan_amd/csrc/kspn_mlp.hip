@@ -19,9 +19,32 @@ constexpr int P_W1 = 0, P_B1 = 192, P_W2 = 224, P_B2 = 1248, P_W3 = 1280, P_B3 =
 static_assert(P_COUNT == KSPN_MLP_PARAMS, "parameter layout");
 
 // ---------------------------------------------------------------- features
-// One wavefront per vertex: in/out degree, their logs, and the 2-hop sums
-// (out-degrees of out-neighbours, in-degrees of in-neighbours) reduced
-// across lanes, so hub rows are split over 64 lanes.
+// In/out degree, their logs, and the 2-hop sums (out-degrees of
+// out-neighbours, in-degrees of in-neighbours). One lane per vertex: a
+// wavefront takes 64 consecutive list entries, each lane sums its own rows
+// (four neighbour offsets in flight at a time), so the rows of neighbouring
+// lanes are read from neighbouring memory. Vertices with a row longer than
+// FEAT_LANE_MAX are summed afterwards by the whole wavefront, one at a
+// time, so a hub costs its degree / 64 steps, not its degree.
+constexpr uint64_t FEAT_LANE_MAX = 64;
+
+__device__ inline uint64_t row_degree_sum(const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                                          uint64_t b, uint64_t e) {
+  uint64_t sum = 0;
+  uint64_t it = b;
+  for (; it + 4 <= e; it += 4) {
+    const uint32_t w0 = adj[it], w1 = adj[it + 1], w2 = adj[it + 2], w3 = adj[it + 3];
+    const uint64_t a0 = off[w0], a1 = off[w1], a2 = off[w2], a3 = off[w3];
+    const uint64_t c0 = off[w0 + 1], c1 = off[w1 + 1], c2 = off[w2 + 1], c3 = off[w3 + 1];
+    sum += (c0 - a0) + (c1 - a1) + (c2 - a2) + (c3 - a3);
+  }
+  for (; it < e; ++it) {
+    const uint32_t w = adj[it];
+    sum += off[w + 1] - off[w];
+  }
+  return sum;
+}
+
 __global__ __launch_bounds__(WG) void k_features(uint32_t count, const uint32_t* __restrict__ list,
                                                  const uint64_t* __restrict__ fw_off, const uint32_t* __restrict__ fw_adj,
                                                  const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
@@ -29,21 +52,46 @@ __global__ __launch_bounds__(WG) void k_features(uint32_t count, const uint32_t*
   const uint32_t lane = lane_id();
   const uint64_t wave = (blockIdx.x * (uint64_t)WG + threadIdx.x) / WAVE;
   const uint64_t waves = (uint64_t)gridDim.x * WG / WAVE;
-  for (uint64_t i = wave; i < count; i += waves) {
-    const uint32_t v = list ? list[i] : (uint32_t)i;
-    const uint64_t fb = fw_off[v], fe = fw_off[v + 1], bb = bw_off[v], be = bw_off[v + 1];
+  for (uint64_t base = wave * WAVE; base < count; base += waves * WAVE) {
+    const uint64_t i = base + lane;
+    const bool ok = i < count;
+    const uint32_t v = ok ? (list ? list[i] : (uint32_t)i) : 0u;
+    uint64_t fb = 0, fe = 0, bb = 0, be = 0;
+    if (ok) {
+      fb = fw_off[v];
+      fe = fw_off[v + 1];
+      bb = bw_off[v];
+      be = bw_off[v + 1];
+    }
+    const bool own = ok && fe - fb <= FEAT_LANE_MAX && be - bb <= FEAT_LANE_MAX;
     uint64_t s2o = 0, s2i = 0;
-    for (uint64_t it = fb + lane; it < fe; it += WAVE) {
-      uint32_t w = fw_adj[it];
-      s2o += fw_off[w + 1] - fw_off[w];
+    if (own) {
+      s2o = row_degree_sum(fw_off, fw_adj, fb, fe);
+      s2i = row_degree_sum(bw_off, bw_adj, bb, be);
     }
-    for (uint64_t it = bb + lane; it < be; it += WAVE) {
-      uint32_t w = bw_adj[it];
-      s2i += bw_off[w + 1] - bw_off[w];
+    // the long rows: the whole wavefront, one vertex at a time
+    uint64_t big = __ballot(ok && !own);
+    while (big) {
+      const uint32_t j = __ffsll((unsigned long long)big) - 1;
+      big &= big - 1;
+      const uint64_t jfb = __shfl(fb, j), jfe = __shfl(fe, j), jbb = __shfl(bb, j), jbe = __shfl(be, j);
+      uint64_t so = 0, si = 0;
+      for (uint64_t it = jfb + lane; it < jfe; it += WAVE) {
+        const uint32_t w = fw_adj[it];
+        so += fw_off[w + 1] - fw_off[w];
+      }
+      for (uint64_t it = jbb + lane; it < jbe; it += WAVE) {
+        const uint32_t w = bw_adj[it];
+        si += bw_off[w + 1] - bw_off[w];
+      }
+      so = __shfl(wave_sum64(so), 0);  // the sum is complete in lane 0
+      si = __shfl(wave_sum64(si), 0);
+      if (lane == j) {
+        s2o = so;
+        s2i = si;
+      }
     }
-    s2o = wave_sum64(s2o);
-    s2i = wave_sum64(s2i);
-    if (lane == 0) {
+    if (ok) {
       const uint64_t out = fe - fb, in = be - bb;
       const float li = log1pf((float)in), lo = log1pf((float)out);
       float* f = feat + i * 6;
@@ -434,8 +482,7 @@ __global__ __launch_bounds__(1024) void k_argmax_score(uint32_t count, const uin
 // host helpers used by the pipeline (kspn_scc.hip)
 int mlp_features(Ctx* c, const kspn_graph* g, const uint32_t* list, uint32_t count, float* feat, hipStream_t s) {
   if (count == 0) return KSPN_OK;
-  uint64_t waves_needed = count;
-  uint32_t grid = (uint32_t)std::min<uint64_t>((waves_needed * WAVE + WG - 1) / WG, (uint64_t)c->num_cu * 16);
+  uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)count + WG - 1) / WG, (uint64_t)c->num_cu * 16);
   k_features<<<grid, WG, 0, s>>>(count, list, g->fw_off, g->fw_adj, g->bw_off, g->bw_adj, feat);
   KSPN_HIP(hipGetLastError());
   return KSPN_OK;

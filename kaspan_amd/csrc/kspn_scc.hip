@@ -480,23 +480,36 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const uint64_t e0 = t * EXP_TILE;
     const uint64_t e1 = min(e0 + (uint64_t)EXP_TILE, total);
-    if (threadIdx.x == 0) {
+    uint32_t i0, w;
+    {
+      // the last frontier entries whose edges start at or before e0 and at
+      // or before e1 - 1: two 128-way searches side by side (threads 0..127
+      // and 128..255). Each round probes 128 evenly spaced scan values at
+      // once, so a tile waits for about log_128(f) loads instead of
+      // 2 log_2(f) dependent ones. Invariant: S[lo] <= e, and hi == f or
+      // S[hi] > e; the scan is non-decreasing, so the probes that pass form
+      // a prefix.
+      const uint32_t half = threadIdx.x >> 7, t = threadIdx.x & 127;
+      const uint64_t e = half ? e1 - 1 : e0;
       uint32_t lo = 0, hi = f;
-      while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (S[mid] <= e0) lo = mid; else hi = mid;
+      while (__syncthreads_or(hi - lo > 1)) {
+        const uint32_t step = (hi - lo + 127) / 128;
+        const uint64_t p = (uint64_t)lo + (uint64_t)t * step;
+        const bool pred = hi - lo > 1 && p < hi && S[p] <= e;
+        const uint32_t c0 = __syncthreads_count(half == 0 && pred);
+        const uint32_t c1 = __syncthreads_count(half == 1 && pred);
+        const uint32_t c = half ? c1 : c0;  // >= 1 while the half is still searching (its probe 0 is lo)
+        if (hi - lo > 1) {
+          hi = (uint32_t)min((uint64_t)lo + (uint64_t)c * step, (uint64_t)hi);
+          lo += (c - 1) * step;
+        }
       }
-      uint32_t i0 = lo;
-      hi = f;
-      while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (S[mid] <= e1 - 1) lo = mid; else hi = mid;
-      }
-      s_i0 = i0;
-      s_w = lo - i0 + 1;
+      if (threadIdx.x == 0) s_i0 = lo;
+      if (threadIdx.x == 128) s_w = lo;
+      __syncthreads();
+      i0 = s_i0;
+      w = s_w - i0 + 1;
     }
-    __syncthreads();
-    const uint32_t i0 = s_i0, w = s_w;
     if (w <= (uint32_t)EXP_WIN) {
       for (uint32_t j = threadIdx.x; j < w; j += WG) {
         uint32_t v = F[i0 + j];
@@ -2902,13 +2915,34 @@ __device__ inline unsigned long long wave_max64(unsigned long long v) {
   return v;
 }
 
-// pbest[p] = max over live v in p of (priority << 32 | ~v): priority is
-// min(in*out, 2^32-1) (plain rule, smallest id on ties) or the ordered MLP
-// score; lanes are grouped by partition before the 64-bit atomicMax
-__global__ __launch_bounds__(WG) void k_part_key(uint32_t n, const uint32_t* __restrict__ live,
-                                                 const uint32_t* __restrict__ pid, const uint64_t* __restrict__ fw_off,
-                                                 const uint64_t* __restrict__ bw_off, const float* __restrict__ score,
-                                                 uint64_t rnd_seed, unsigned long long* pbest);
+// Pivot priority of every vertex for a whole solve (the degrees and the
+// MLP score do not change within it): min(in*out, 2^32-1) for the plain
+// rule (smallest id on ties), else the plain rule's half-octave of in*out in
+// bits 24..31, then the ordered score (learned) or nothing (random: a
+// seeded hash of the round fills bits 0..23 when the key is formed), and
+// bit 0 set so a vertex of product 0 still has a priority.
+__global__ void k_part_prio(uint32_t n, const uint64_t* __restrict__ fw_off, const uint64_t* __restrict__ bw_off,
+                            const float* __restrict__ score, bool bucketed, uint32_t* __restrict__ prio) {
+  for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG) {
+    const uint64_t prod = (fw_off[v + 1] - fw_off[v]) * (bw_off[v + 1] - bw_off[v]);
+    uint32_t pri;
+    if (bucketed) pri = (prod_bucket(prod) << 24) | (score ? float_order(score[v]) >> 8 : 0u) | 1u;
+    else pri = prod > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)prod;
+    prio[v] = pri;
+  }
+}
+
+// the pivot key of vertex v: (priority << 32) | ~v, so the largest key is
+// the highest priority and, among equals, the smallest id
+__device__ inline unsigned long long part_key(const uint32_t* __restrict__ prio, uint32_t v, uint64_t rnd_seed) {
+  uint32_t pri = prio[v];
+  if (rnd_seed) {
+    uint64_t z = ((uint64_t)rnd_seed << 32 | v) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 31)) * 0xBF58476D1CE4E5B9ull;
+    pri |= (uint32_t)(z >> 40);
+  }
+  return ((unsigned long long)pri << 32) | (uint32_t)~v;
+}
 
 // Per-workgroup aggregation of (slot, value) updates in an LDS hash table
 // (linear probing over AGG_SLOTS entries); a full table falls back to the
@@ -2954,19 +2988,45 @@ __device__ inline void agg_flush(const uint32_t* key, const T* val, T* global) {
   }
 }
 
-// pbest[p] = max over live v in p of (priority << 32 | ~v): priority is
-// min(in*out, 2^32-1) (plain rule, smallest id on ties), the ordered MLP
-// score, or a hash (random pivots). The wavefront's largest group makes one
-// update; updates are aggregated per workgroup in LDS.
+// One update per lane (slot p, value x; lanes with ok false have none): the
+// wavefront's largest group (the first lane's slot) makes one update with
+// its extreme when it has at least 8 members; the rest go to the
+// workgroup's LDS table one by one.
+template <typename T, bool MAX>
+__device__ inline void agg_wave_put(uint32_t* key, T* val, T* global, bool ok, uint32_t p, T x) {
+  const uint64_t pending = __ballot(ok);
+  if (!pending) return;
+  bool left = ok;
+  const uint32_t leader = __ffsll((unsigned long long)pending) - 1;
+  const uint32_t r = __shfl(p, leader);
+  const bool mine = ok && p == r;
+  if (__popcll(__ballot(mine)) >= 8) {
+    T m = mine ? x : (MAX ? (T)0 : (T)~(T)0);
+    for (int o = 32; o > 0; o >>= 1) {
+      const T t = __shfl_down(m, o);
+      m = MAX ? (t > m ? t : m) : (t < m ? t : m);
+    }
+    if (lane_id() == 0 && !agg_put<T, MAX>(key, val, r, m)) {
+      if constexpr (MAX) atomicMax(&global[r], m);
+      else atomicMin(&global[r], m);
+    }
+    if (mine) left = false;
+  }
+  if (left && !agg_put<T, MAX>(key, val, p, x)) {
+    if constexpr (MAX) atomicMax(&global[p], x);
+    else atomicMin(&global[p], x);
+  }
+}
+
+// pbest[p] = max over live v in p of part_key(v): the first round's keys
+// (later rounds form them in k_part_split)
 __global__ __launch_bounds__(WG) void k_part_key(uint32_t n, const uint32_t* __restrict__ live,
-                                                 const uint32_t* __restrict__ pid, const uint64_t* __restrict__ fw_off,
-                                                 const uint64_t* __restrict__ bw_off, const float* __restrict__ score,
+                                                 const uint32_t* __restrict__ pid, const uint32_t* __restrict__ prio,
                                                  uint64_t rnd_seed, unsigned long long* pbest) {
   __shared__ uint32_t s_key[AGG_SLOTS];
   __shared__ unsigned long long s_val[AGG_SLOTS];
   agg_init<unsigned long long>(s_key, s_val, 0ull);
   const uint64_t pad = ((uint64_t)n + WG - 1) / WG * WG;
-  const uint32_t lane = lane_id();
   for (uint64_t base = (uint64_t)blockIdx.x * WG; base < pad; base += (uint64_t)gridDim.x * WG) {
     const uint64_t v = base + threadIdx.x;
     const bool valid = v < n && bm_get(live, (uint32_t)v);
@@ -2974,39 +3034,9 @@ __global__ __launch_bounds__(WG) void k_part_key(uint32_t n, const uint32_t* __r
     unsigned long long key = 0;
     if (valid) {
       p = pid[v];
-      uint32_t pri;
-      const uint64_t prod = (fw_off[v + 1] - fw_off[v]) * (bw_off[v + 1] - bw_off[v]);
-      if (score || rnd_seed) {
-        // the plain rule's half-octave of in*out first, then the score (or a
-        // seeded hash) inside it: the pivot is always one of the plain rule's
-        // top candidates of its partition
-        uint32_t low;
-        if (score) {
-          low = float_order(score[v]) >> 8;
-        } else {
-          uint64_t z = ((uint64_t)rnd_seed << 32 | (uint32_t)v) * 0x9E3779B97F4A7C15ull;
-          z = (z ^ (z >> 31)) * 0xBF58476D1CE4E5B9ull;
-          low = (uint32_t)(z >> 40);
-        }
-        pri = (prod_bucket(prod) << 24) | low | 1u;
-      } else {
-        pri = prod > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)prod;
-      }
-      key = ((unsigned long long)pri << 32) | (uint32_t)~(uint32_t)v;
+      key = part_key(prio, (uint32_t)v, rnd_seed);
     }
-    const uint64_t pending = __ballot(valid);
-    bool left = valid;
-    if (pending) {
-      const uint32_t leader = __ffsll((unsigned long long)pending) - 1;
-      const uint32_t r = __shfl(p, leader);
-      const bool mine = valid && p == r;
-      if (__popcll(__ballot(mine)) >= 8) {
-        const unsigned long long m = wave_max64(mine ? key : 0ull);  // valid in lane 0
-        if (lane == 0 && !agg_put<unsigned long long, true>(s_key, s_val, r, m)) atomicMax(&pbest[r], m);
-        if (mine) left = false;
-      }
-    }
-    if (left && !agg_put<unsigned long long, true>(s_key, s_val, p, key)) atomicMax(&pbest[p], key);
+    agg_wave_put<unsigned long long, true>(s_key, s_val, pbest, valid, p, key);
   }
   agg_flush<unsigned long long, true>(s_key, s_val, pbest);
 }
@@ -3121,6 +3151,81 @@ __global__ __launch_bounds__(WG) void k_part_apply(uint32_t n, uint32_t* live, c
     partials[3 * blockIdx.x] = count;
     partials[3 * blockIdx.x + 1] = 0;
     partials[3 * blockIdx.x + 2] = 0;
+  }
+}
+
+// After a round (without outcome records): classification and
+// relabelling in one pass over the vertices. A live vertex reached both
+// forward and backward is decided: it leaves live (each wavefront owns its
+// two words), its id goes to pmin[newid[pid]] (minimum, aggregated) and it
+// is appended to dlist (count *dcount). Any other live vertex gets pid =
+// 3 * newid[pid] + class (0 unreached, 1 forward only, 2 backward only)
+// and, with `keys`, its pivot key goes to pbest[new pid] (maximum,
+// aggregated): the next round's keys, over pbest zeroed for its ids.
+// Per-workgroup decided counts into partials. (The decided vertices are
+// not listed here: one counter for the whole device would take an atomic
+// per wavefront; k_part_label finds them in the bitmaps instead.)
+__global__ __launch_bounds__(WG) void k_part_split(uint32_t n, uint32_t* live, const uint32_t* __restrict__ fwv,
+                                                   const uint32_t* __restrict__ bwv, uint32_t* __restrict__ pid,
+                                                   const uint32_t* __restrict__ newid, uint32_t* pmin,
+                                                   const uint32_t* __restrict__ prio, uint64_t rnd_seed, bool keys,
+                                                   unsigned long long* pbest, uint64_t* __restrict__ partials) {
+  __shared__ uint32_t s_mk[AGG_SLOTS];
+  __shared__ uint32_t s_mv[AGG_SLOTS];
+  __shared__ uint32_t s_bk[AGG_SLOTS];
+  __shared__ unsigned long long s_bv[AGG_SLOTS];
+  __shared__ uint64_t s_red[WG / WAVE];
+  agg_init<uint32_t>(s_mk, s_mv, 0xFFFFFFFFu);
+  agg_init<unsigned long long>(s_bk, s_bv, 0ull);
+  const uint64_t pad = ((uint64_t)n + WG - 1) / WG * WG;
+  const uint32_t lane = lane_id();
+  uint64_t count = 0;
+  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < pad; base += (uint64_t)gridDim.x * WG) {
+    const uint64_t v = base + threadIdx.x;
+    const uint32_t lw = v < n ? live[v >> 5] : 0u;
+    const bool is_live = v < n && ((lw >> (v & 31)) & 1u);
+    bool dec = false;
+    uint32_t k = 0, np = 0;
+    unsigned long long key = 0;
+    if (is_live) {
+      const bool f = bm_get(fwv, (uint32_t)v), b = bm_get(bwv, (uint32_t)v);
+      k = newid[pid[v]];
+      dec = f && b;
+      if (!dec) {
+        np = 3u * k + (f ? 1u : (b ? 2u : 0u));
+        pid[v] = np;
+        if (keys) key = part_key(prio, (uint32_t)v, rnd_seed);
+      }
+    }
+    agg_wave_put<uint32_t, false>(s_mk, s_mv, pmin, dec, k, (uint32_t)v);
+    if (keys) agg_wave_put<unsigned long long, true>(s_bk, s_bv, pbest, is_live && !dec, np, key);
+    const uint64_t mask = __ballot(dec);
+    if (mask) {
+      const uint64_t word0 = (base + threadIdx.x - lane) >> 5;
+      if (lane == 0 && (uint32_t)mask) live[word0] = lw & ~(uint32_t)mask;
+      if (lane == 32 && (uint32_t)(mask >> 32)) live[word0 + 1] = lw & ~(uint32_t)(mask >> 32);
+      count += dec;
+    }
+  }
+  agg_flush<uint32_t, false>(s_mk, s_mv, pmin);
+  if (keys) agg_flush<unsigned long long, true>(s_bk, s_bv, pbest);
+  count = block_sum64(count, s_red);
+  if (threadIdx.x == 0) {
+    partials[3 * blockIdx.x] = count;
+    partials[3 * blockIdx.x + 1] = 0;
+    partials[3 * blockIdx.x + 2] = 0;
+  }
+}
+
+// the vertices k_part_split decided (forward and backward reached: every
+// reached vertex was live when the round began): label = the smallest
+// decided id of their partition (pid still names the round's partition)
+__global__ void k_part_label(uint32_t n, const uint32_t* __restrict__ fwv, const uint32_t* __restrict__ bwv,
+                             const uint32_t* __restrict__ pid, const uint32_t* __restrict__ newid,
+                             const uint32_t* __restrict__ pmin, uint32_t* __restrict__ labels) {
+  for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG) {
+    const uint32_t w = fwv[v >> 5] & bwv[v >> 5];
+    if ((w >> (v & 31)) & 1u) labels[v] = pmin[newid[pid[v]]];
   }
 }
 
@@ -3330,7 +3435,7 @@ int ctx_reserve(Ctx* c, uint32_t n, uint64_t m) {
     c->cub_bytes = need;
   }
   // buffers sized by cap_n and allocated on first use: drop them so they are reallocated
-  for (uint32_t** p : {&c->tj, &c->pid, &c->pmin, &c->pflags, &c->pnew, &c->pcnt})
+  for (uint32_t** p : {&c->tj, &c->pid, &c->pmin, &c->pflags, &c->pnew, &c->pcnt, &c->vprio})
     if (*p) {
       (void)hipFree(*p);
       *p = nullptr;
@@ -3671,7 +3776,8 @@ template <int OP>
 void launch_expand_ept(uint32_t ept, uint32_t grid, hipStream_t s, const uint32_t* F, const uint32_t* fc,
                        const uint64_t* S, const uint64_t* off, const uint32_t* adj, const OpArgs& a, uint32_t* N,
                        uint32_t* nc) {
-  if (ept == 4) k_expand<OP, 4><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc);
+  if (ept == 2) k_expand<OP, 2><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc);
+  else if (ept == 4) k_expand<OP, 4><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc);
   else if (ept == 16) k_expand<OP, 16><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc);
   else k_expand<OP, EXP_EPT><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc);
 }
@@ -3847,10 +3953,18 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       if (f >= 4096 && !scanned) KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)f + 1, R.s));
       scanned = false;
       int kern = dir == 0 ? K_EXPAND_FW : K_EXPAND_BW;
-      uint32_t grid = known ? expand_grid(c, edges, R.ept) : (uint32_t)c->num_cu * 8;
+      // tile width: the options' or, by default, the widest that still
+      // gives every CU a few tiles (narrow tiles fill the device on
+      // mid-sized frontiers, wide ones keep more loads in flight per thread)
+      uint32_t ept = R.ept;
+      if (!ept) {
+        const uint64_t cu = c->num_cu;
+        ept = !known || edges >= cu * WG * 8 * 4 ? 8u : (edges >= cu * WG * 4 * 2 ? 4u : 2u);
+      }
+      uint32_t grid = known ? expand_grid(c, edges, ept) : (uint32_t)c->num_cu * 8;
       Prof p(c, R.s, kern, 20.0 * f + (4.0 + 0.125) * (double)edges);
       const int rop = dir == 0 ? OP_FW : (pid ? OP_PBW : OP_BW);
-      launch_reach_expand(rop, R.ept, grid, R.s, F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
+      launch_reach_expand(rop, ept, grid, R.s, F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
       if (fw_levels_push) (*fw_levels_push)++;
     } else {
       bool batch = list_mode;  // a batch of list levels follows (after a sweep: when the remainder is short)
@@ -4015,13 +4129,16 @@ int fwbw(Run& R, uint32_t pivot, uint32_t* decided, bool publish = true, bool ta
 
 // Pivot keys of the next round over the S sparse partition ids and the
 // "used" flag of every id (pflags)
-void part_keys(Run& R, uint32_t S, const float* score, uint32_t round) {
+// the random rule's seed of a round (0: not the random rule)
+inline uint64_t part_seed(const Run& R, uint32_t round) {
+  return R.o.pivot == KSPN_PIVOT_RANDOM ? ((uint64_t)R.o.seed * 0x10001ull + round + 1) : 0;
+}
+
+void part_keys(Run& R, uint32_t S, uint32_t round) {
   Ctx* c = R.c;
-  const kspn_graph* g = R.g;
-  Prof p(c, R.s, K_PIVOT, 20.0 * R.live_n + 12.0 * S);
+  Prof p(c, R.s, K_PIVOT, R.n / 8.0 + 8.0 * R.live_n + 12.0 * S);
   k_fill_u64<<<stream_grid(c, S), WG, 0, R.s>>>((unsigned long long*)c->pbest, S, 0ull);
-  const uint64_t rs = R.o.pivot == KSPN_PIVOT_RANDOM ? ((uint64_t)R.o.seed * 0x10001ull + round + 1) : 0;
-  k_part_key<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->live, c->pid, g->fw_off, g->bw_off, score, rs,
+  k_part_key<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->live, c->pid, c->vprio, part_seed(R, round),
                                                   (unsigned long long*)c->pbest);
   k_part_used<<<stream_grid(c, (uint64_t)S + 1), WG, 0, R.s>>>(S, (unsigned long long*)c->pbest, c->pflags);
 }
@@ -4067,13 +4184,20 @@ int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
     }
     score = c->vscore;
   }
+  // every vertex's pivot priority, once for the solve
+  if (!c->vprio) KSPN_TRY(dalloc(&c->vprio, (uint64_t)c->cap_n + 64));
+  {
+    Prof p(c, R.s, K_PIVOT, 36.0 * n);
+    k_part_prio<<<stream_grid(c, n), WG, 0, R.s>>>(R.n, g->fw_off, g->bw_off, score,
+                                                   R.o.pivot != KSPN_PIVOT_PLAIN, c->vprio);
+  }
   uint32_t* pcount = nullptr;
   if (R.o.pivot_log) {
     if (!c->pcnt) KSPN_TRY(dalloc(&c->pcnt, 4ull * c->cap_n + 64));
     pcount = c->pcnt;
   }
   // the first round's keys and compaction
-  part_keys(R, S, score, 0);
+  part_keys(R, S, 0);
   KSPN_TRY(part_compact(R, S));
   KSPN_TRY(publish(c, R.s, nullptr));
   np = c->h_cnt[C_AUX];
@@ -4098,10 +4222,29 @@ int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
       k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_Q0], np);
     }
     KSPN_TRY(reach(R, 1, np, c->pid, &lp, &lq));
-    {
+    const uint32_t S_next = 3u * np;
+    const bool more = round + 1 < K;
+    if (!pcount) {
+      // classification, relabelling and the next round's keys in one pass,
+      // then the labels of the decided vertices from their list
+      {
+        Prof p(c, R.s, K_BITMAP, 0.5 * R.n + 16.0 * R.live_n + 12.0 * np + 8.0 * S_next);
+        k_fill_u32<<<stream_grid(c, np), WG, 0, R.s>>>(c->pmin, np, 0xFFFFFFFFu);
+        if (more) k_fill_u64<<<stream_grid(c, S_next), WG, 0, R.s>>>((unsigned long long*)c->pbest, S_next, 0ull);
+        k_part_split<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->fwv, c->bwv, c->pid, c->pnew, c->pmin, c->vprio,
+                                            part_seed(R, round + 1), more, (unsigned long long*)c->pbest, c->part);
+      }
+      {
+        Prof p(c, R.s, K_LABEL, R.n / 4.0 + 16.0 * np);
+        k_part_label<<<gridn, WG, 0, R.s>>>(R.n, c->fwv, c->bwv, c->pid, c->pnew, c->pmin, R.labels);
+        if (more)
+          k_part_used<<<stream_grid(c, (uint64_t)S_next + 1), WG, 0, R.s>>>(S_next, (unsigned long long*)c->pbest,
+                                                                             c->pflags);
+      }
+    } else {
       Prof p(c, R.s, K_BITMAP, 0.375 * R.n + 12.0 * np);
       k_fill_u32<<<stream_grid(c, np), WG, 0, R.s>>>(c->pmin, np, 0xFFFFFFFFu);
-      if (pcount) k_fill_u32<<<stream_grid(c, 4ull * np), WG, 0, R.s>>>(pcount, 4ull * np, 0u);
+      k_fill_u32<<<stream_grid(c, 4ull * np), WG, 0, R.s>>>(pcount, 4ull * np, 0u);
       k_part_classify<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->fwv, c->bwv, c->pid, c->pnew, c->pmin, pcount);
       if (pcount && R.log_count < R.o.pivot_log_cap) {
         k_part_log<<<stream_grid(c, S), WG, 0, R.s>>>(S, (unsigned long long*)c->pbest, c->pnew, pcount, round,
@@ -4110,14 +4253,14 @@ int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
         if (R.st) R.st->pivot_log_count = R.log_count;
       }
     }
-    {
-      Prof p(c, R.s, K_LABEL, 16.0 * R.live_n);
-      k_part_apply<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->fwv, c->bwv, c->pid, c->pmin, c->pnew, R.labels, c->part);
+    if (pcount) {
+      {
+        Prof p(c, R.s, K_LABEL, 16.0 * R.live_n);
+        k_part_apply<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->fwv, c->bwv, c->pid, c->pmin, c->pnew, R.labels, c->part);
+      }
+      if (more) part_keys(R, S_next, round + 1);
     }
-    // the next round's keys and compaction ride on this round's publication
-    const uint32_t S_next = 3u * np;
-    const bool more = round + 1 < K;
-    if (more) part_keys(R, S_next, score, round + 1);
+    // the next round's compaction rides on this round's publication
     k_reduce3<<<1, WG, 0, R.s>>>(c->part, gridn, &c->cnt[C_DECIDED], c->cnt64);
     if (more) KSPN_TRY(part_compact(R, S_next));
     KSPN_TRY(publish(c, R.s, nullptr));
@@ -5114,7 +5257,7 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   if (c->actl) (void)hipFree(c->actl);
   void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->sbm, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
                   c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf, c->pid, c->pbest,
-                  c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt};
+                  c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt, c->vprio};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
@@ -5153,7 +5296,8 @@ int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* o
   if (o.pivot > KSPN_PIVOT_RANDOM) return KSPN_E_INVALID;
   if (o.pivot == KSPN_PIVOT_LEARNED && !o.mlp_params) return KSPN_E_INVALID;
   if ((o.pull_ilp && o.pull_ilp != 1 && o.pull_ilp != 2 && o.pull_ilp != 4) ||
-      (o.expand_ept && o.expand_ept != 4 && o.expand_ept != 8 && o.expand_ept != 16) || o.sweep_wgs > 64 ||
+      (o.expand_ept && o.expand_ept != 2 && o.expand_ept != 4 && o.expand_ept != 8 && o.expand_ept != 16) ||
+      o.sweep_wgs > 64 ||
       o.list_wgs > 64 || o.chain_levels > 256)
     return KSPN_E_INVALID;
   if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -5175,7 +5319,7 @@ int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* o
   R.coop_deg = o.coop_degree ? o.coop_degree : kCoopDeg;
   R.coop_wg = std::min<uint32_t>(o.coop_workgroups ? o.coop_workgroups : kCoopWorkgroups, c->coop_grid);
   R.pull_ilp = o.pull_ilp ? o.pull_ilp : (uint32_t)PULL_ILP;
-  R.ept = o.expand_ept ? o.expand_ept : (uint32_t)EXP_EPT;
+  R.ept = o.expand_ept;  // 0: chosen per level
   if (o.flags & KSPN_FLAG_CHAIN) {
     // chained launches need no co-residency: the grid is not capped by it
     R.chain = o.chain_levels ? o.chain_levels : kChainLevels;
