@@ -337,16 +337,25 @@ struct OpArgs {
   unsigned long long* nsum;      // OP_FW/OP_BW: sum of those degrees over claimed targets; nsum[1]: their maximum
   const uint32_t* part;          // OP_FW/OP_PBW: partition of every vertex (nullptr: one partition)
   const uint32_t* pcand;         // OP_PBW: the vertices it may claim (live, or fwv for a search within FW sets)
+  uint32_t dead_pid;             // part holds DEAD_PID for every vertex that is not live, so a partition match
+                                 // implies live and the live bitmap need not be read
 };
+
+// partition id of a vertex that is not live during the multi-pivot rounds
+// (above every id they use: 3 * partitions + 2 < 2^32 - 1 for n <= DEAD_PID_MAX_N)
+constexpr uint32_t DEAD_PID = 0xFFFFFFFFu;
+constexpr uint64_t DEAD_PID_MAX_N = 0x55555554ull;
 
 // reachability claim of edge u -> w: true for the one thread that claims w
 template <int OP>
 __device__ inline bool reach_claim(uint32_t u, uint32_t w, const OpArgs& a) {
   if constexpr (OP == OP_FW) {
+    if (a.part && a.dead_pid) return a.part[w] == a.part[u] && bm_claim(a.fwv, w);
     return bm_get(a.live, w) && (!a.part || a.part[w] == a.part[u]) && bm_claim(a.fwv, w);
   } else if constexpr (OP == OP_BW) {
     return bm_get(a.fwv, w) && bm_claim(a.bwv, w);
   } else {
+    if (a.dead_pid && a.pcand == a.live) return a.part[w] == a.part[u] && bm_claim(a.bwv, w);
     return bm_get(a.pcand, w) && a.part[w] == a.part[u] && bm_claim(a.bwv, w);
   }
 }
@@ -393,10 +402,12 @@ __device__ inline void reach_batch(const uint32_t (&u)[K], const uint32_t (&x)[K
   const uint32_t* cand = OP == OP_BW ? a.fwv : (OP == OP_PBW ? a.pcand : a.live);
   uint32_t* vis = OP == OP_FW ? a.fwv : a.bwv;
   const bool part = OP == OP_PBW || (OP == OP_FW && a.part);
+  // candidates = live and partition ids mark the dead: the id test alone decides
+  const bool any_live = part && a.dead_pid && cand == a.live;
   uint32_t cw[K], vw[K];
 #pragma unroll
   for (int r = 0; r < K; ++r) {
-    cw[r] = ok[r] ? cand[x[r] >> 5] : 0u;
+    cw[r] = ok[r] ? (any_live ? 0xFFFFFFFFu : cand[x[r] >> 5]) : 0u;
     vw[r] = ok[r] ? vis[x[r] >> 5] : 0xFFFFFFFFu;
   }
   bool c[K];
@@ -3136,6 +3147,7 @@ __global__ __launch_bounds__(WG) void k_part_apply(uint32_t n, uint32_t* live, c
       if (f && b) {
         dec = true;
         labels[v] = pmin[k];
+        pid[v] = DEAD_PID;  // no longer live
         ++count;
       } else {
         pid[v] = 3u * k + (f ? 1u : (b ? 2u : 0u));
@@ -3221,11 +3233,24 @@ __global__ __launch_bounds__(WG) void k_part_split(uint32_t n, uint32_t* live, c
 // reached vertex was live when the round began): label = the smallest
 // decided id of their partition (pid still names the round's partition)
 __global__ void k_part_label(uint32_t n, const uint32_t* __restrict__ fwv, const uint32_t* __restrict__ bwv,
-                             const uint32_t* __restrict__ pid, const uint32_t* __restrict__ newid,
+                             uint32_t* __restrict__ pid, const uint32_t* __restrict__ newid,
                              const uint32_t* __restrict__ pmin, uint32_t* __restrict__ labels) {
   for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG) {
     const uint32_t w = fwv[v >> 5] & bwv[v >> 5];
-    if ((w >> (v & 31)) & 1u) labels[v] = pmin[newid[pid[v]]];
+    if ((w >> (v & 31)) & 1u) {
+      labels[v] = pmin[newid[pid[v]]];
+      pid[v] = DEAD_PID;  // no longer live
+    }
+  }
+}
+
+// partition ids at the start of the multi-pivot rounds: DEAD_PID for the
+// vertices that are not live; the live keep theirs (preset: the graph of a
+// batch) or get 0
+__global__ void k_pid_init(uint32_t n, const uint32_t* __restrict__ live, uint32_t* __restrict__ pid, bool keep) {
+  for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG) {
+    if (!bm_get(live, (uint32_t)v)) pid[v] = DEAD_PID;
+    else if (!keep) pid[v] = 0u;
   }
 }
 
@@ -3574,6 +3599,7 @@ struct Run {
   uint32_t pull_ilp = PULL_ILP;       // launch shapes (kspn_opts)
   uint32_t ept = EXP_EPT;
   uint32_t chain = 0;                 // KSPN_FLAG_CHAIN: launches per chain of push levels (0: persistent kernel)
+  bool dead_pid = false;              // pid holds DEAD_PID for every vertex that is not live (multi-pivot rounds)
   bool pivot_live_deg = false;        // pivots by live degree product (after an exhaustive trim)
   // the first trim's decided count and pivot are still on the device
   // (cnt[C_DECIDED], cnt[C_FLAG]); the first publication of the forward
@@ -3812,7 +3838,8 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   // backward: within the forward sets, or (partitioned, KSPN_FLAG_FULL_BW) over every live vertex of the partition
   const bool full_bw = pid && (R.o.flags & KSPN_FLAG_FULL_BW);
   const uint32_t* cand = (dir == 0 || full_bw) ? c->live : c->fwv;
-  OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, 0, off, c->cnt64 + 1, pid, cand};
+  OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, 0, off, c->cnt64 + 1, pid, cand,
+           pid && R.dead_pid ? 1u : 0u};
   uint32_t* F = c->qa;
   uint32_t* N = c->qb;
   int fc = C_Q0, nc = C_Q1;
@@ -4162,11 +4189,10 @@ int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
   if (!c->pnew) KSPN_TRY(dalloc(&c->pnew, 3ull * c->cap_n + 64));
   // sparse ids of the first round: 0 for every vertex, or the preset ids
   // (the graphs of a batch; those the first trim emptied are not used)
-  uint32_t S = np;
-  if (!R.pid_preset) {
-    k_fill_u32<<<stream_grid(c, n), WG, 0, R.s>>>(c->pid, n, 0u);
-    S = 1;
-  }
+  uint32_t S = R.pid_preset ? np : 1u;
+  k_pid_init<<<stream_grid(c, n), WG, 0, R.s>>>(R.n, c->live, c->pid, R.pid_preset);
+  // the reaches of the rounds test partition ids instead of the live bitmap
+  R.dead_pid = n <= DEAD_PID_MAX_N;
   const float* score = nullptr;
   if (R.o.pivot == KSPN_PIVOT_LEARNED) {
     // one MLP score per live vertex for the whole solve
@@ -4279,6 +4305,7 @@ int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
     np = c->h_cnt[C_AUX];
   }
   *decided_total = total;
+  R.dead_pid = false;
   return KSPN_OK;
 }
 

@@ -103,13 +103,16 @@ def pivot_log(g: BidiGraph, rounds=24, pivot=_lib.PIVOT_RANDOM, seed=0, cap=1 <<
     return rec.to(torch.int64) & 0xFFFFFFFF
 
 
-def outcome_samples(g: BidiGraph, rounds=24, seed=0, min_size=32, target="log", sample=1 << 19):
+def outcome_samples(g: BidiGraph, rounds=24, seed=0, min_size=32, target="log", sample=1 << 19, gamma=0.0):
     """(features, target) pairs from pivot outcomes on g.
 
     target="log" (default): the multi-pivot rounds' own outcome log, with
     pivots drawn at random among each partition's candidates (the plain
     rule's top half-octave of in*out, the set the learned rule picks from);
-    the target is log1p of the SCC each logged pivot decided.
+    the target is log1p of the SCC each logged pivot decided, less
+    ``gamma`` * log1p of its forward set (the SCC and what it reaches in the
+    partition), so gamma > 0 prefers pivots whose search decides much of
+    what it traverses.
     target="size": every live vertex is a candidate pivot whose outcome is the
     SCC a FW-BW from it decides; one solve gives all of them (kspn_scc_sizes)
     and the target is log1p(SCC size), for a seeded sample of ``sample`` live
@@ -119,7 +122,10 @@ def outcome_samples(g: BidiGraph, rounds=24, seed=0, min_size=32, target="log", 
     if target == "log":
         rec = pivot_log(g, rounds=rounds, pivot=_lib.PIVOT_RANDOM, seed=seed)
         x = features(g, rec[:, 0].to(torch.int32).contiguous())
-        return x, torch.log1p(rec[:, 3].float())
+        t = torch.log1p(rec[:, 3].float())
+        if gamma:
+            t = t - gamma * torch.log1p((rec[:, 3] + rec[:, 4]).float())
+        return x, t
     if target == "size":
         labels, _ = scc(g)
         sizes = scc_sizes(labels)

@@ -35,13 +35,15 @@ def main():
     ap.add_argument("--epochs", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=1e-2)
     ap.add_argument("--target", choices=["log", "size", "shrinkage"], default="log")
+    ap.add_argument("--gamma", type=float, default=0.0,
+                    help="log target: minus gamma * log1p(forward set), favouring searches that decide what they traverse")
     args = ap.parse_args()
     t0 = time.time()
     xs, ts = [], []
     for i in range(args.graphs):
         g = generate("planted", 0x7452414E00000000 + i, n=args.n, max_size=args.max_size)
         for rep in range(1 if args.target == "size" else 4):
-            x, t = P.outcome_samples(g, rounds=args.rounds, seed=17 * i + rep + 1, target=args.target)
+            x, t = P.outcome_samples(g, rounds=args.rounds, seed=17 * i + rep + 1, target=args.target, gamma=args.gamma)
             xs.append(x)
             ts.append(t)
         del g
@@ -49,7 +51,7 @@ def main():
     params, losses = P.train(x, t, epochs=args.epochs, lr=args.lr)
     # fit on a held-out graph: R^2 of the (BF16 MFMA) forward pass against the target
     gh = generate("planted", 0x7452414E00000000 + 1000, n=args.n, max_size=args.max_size)
-    xh, th = P.outcome_samples(gh, seed=999, target=args.target)
+    xh, th = P.outcome_samples(gh, seed=999, target=args.target, gamma=args.gamma)
     ph = P.forward(params, xh)
     r2 = 1.0 - float(((ph - th) ** 2).mean()) / float(th.var())
     np.save(args.out, params.cpu().numpy().astype(np.float32), allow_pickle=False)
@@ -62,7 +64,7 @@ def main():
         _, sp = scc(gh, lab, fwbw_rounds=k)
         _, sl = scc(gh, lab, fwbw_rounds=k, pivot=1, mlp_params=params)
         gain[f"decided_{k}_rounds"] = {"plain": sp.decided_fwbw, "learned": sl.decided_fwbw}
-    print(json.dumps({"target": args.target, "heldout_r2": round(r2, 4), "samples": int(x.shape[0]),
+    print(json.dumps({"target": args.target, "gamma": args.gamma, "heldout_r2": round(r2, 4), "samples": int(x.shape[0]),
                       "graphs": args.graphs, "n": args.n, "epochs": args.epochs,
                       "loss_first": losses[0], "loss_last": losses[-1], "seconds": round(time.time() - t0, 1),
                       "heldout": gain, "out": args.out}))
