@@ -234,6 +234,7 @@ enum Counter : int {
   C_SPEC,    // k_spec_gate: a speculatively launched pull sweep is to run
   C_BADID,   // a tail kernel met a listed vertex id >= n and skipped it
   C_AERR,    // the asynchronous reach gave up (1: no progress, 2: queue overflow)
+  C_SERR,    // the look-back degree scan met a count or ticket it could not use
   C_COUNT_ = 28  // at most 32: the publication map holds the persistent kernel's results from word 32 on
 };
 
@@ -317,6 +318,9 @@ struct Ctx {
   uint32_t* pcnt = nullptr;    // per partition: outcome counters for the pivot log
   float* vscore = nullptr;     // per vertex: MLP score
   uint32_t* vprio = nullptr;   // per vertex: pivot priority of the multi-pivot rounds (fixed within a solve)
+  unsigned long long* ds_st = nullptr;  // look-back scan: status word per tile
+  uint32_t* ds_ticket = nullptr;        // its ticket counter
+  uint32_t ds_epoch = 0;                // epoch of its last launch
   float* vfeat = nullptr;      // features of live vertices (6 per vertex) + scores
   // profiling
   bool profile = false;

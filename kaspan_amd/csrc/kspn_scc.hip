@@ -1869,6 +1869,113 @@ __global__ __launch_bounds__(1024) void k_deg_scan_small(const uint32_t* __restr
   }
 }
 
+// Exclusive scan of the frontier's out-degrees, S[0..f] (S[f] = total), in
+// one pass with decoupled look-back (Merrill and Garland, "Single-pass
+// parallel prefix scan with decoupled look-back", 2016). Workgroups take
+// tiles of DS_TILE entries in ticket order, so every earlier tile belongs
+// to a workgroup that is already running. A tile publishes its aggregate,
+// then its inclusive prefix once the earlier tiles' values are known; one
+// wavefront reads 64 earlier status words at a time. Status words carry
+// the launch's epoch (bits 50..63), so they need no clearing between
+// launches; values are below 2^48. The last ticket of a launch resets the
+// ticket counter for the next one. The launch has exactly `tiles`
+// workgroups, ceil((f + 1) / DS_TILE) for the f the host knows; a device
+// count that does not fit sets *err.
+constexpr uint32_t DS_ITEMS = 8;
+constexpr uint32_t DS_TILE = WG * DS_ITEMS;
+constexpr unsigned long long DS_AGG = 1ull << 48, DS_INC = 2ull << 48, DS_FLAGS = 3ull << 48;
+constexpr unsigned long long DS_VALUE = (1ull << 48) - 1;
+constexpr uint32_t DS_EPOCHS = 1u << 14;
+constexpr uint32_t DS_SPIN = 1u << 22;
+
+__global__ __launch_bounds__(WG) void k_deg_scan(const uint32_t* __restrict__ F, const uint32_t* fcount,
+                                                 const uint64_t* __restrict__ off, uint64_t* __restrict__ S,
+                                                 unsigned long long* st, uint32_t* ticket, uint32_t tiles,
+                                                 uint32_t epoch, uint32_t* err) {
+  __shared__ uint32_t s_t;
+  __shared__ uint64_t s_w[WG / WAVE];
+  __shared__ uint64_t s_excl;
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  if (threadIdx.x == 0) {
+    const uint32_t t = atomicAdd(ticket, 1u);
+    if (t + 1 == tiles) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_t = t;
+  }
+  __syncthreads();
+  const uint32_t t = s_t;
+  if (t >= tiles) {  // a ticket counter out of step (not expected): report, write nothing
+    if (threadIdx.x == 0) *err = 1u;
+    return;
+  }
+  const uint32_t f = *fcount;
+  if (t == 0 && threadIdx.x == 0 && (uint64_t)f + 1 > (uint64_t)tiles * DS_TILE) *err = 1u;
+  const uint64_t i0 = (uint64_t)t * DS_TILE + (uint64_t)threadIdx.x * DS_ITEMS;
+  uint32_t v[DS_ITEMS];
+  uint64_t d[DS_ITEMS];
+  uint64_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < DS_ITEMS; ++k) v[k] = i0 + k < f ? F[i0 + k] : 0u;
+#pragma unroll
+  for (uint32_t k = 0; k < DS_ITEMS; ++k) {
+    d[k] = i0 + k < f ? off[v[k] + 1] - off[v[k]] : 0;
+    sum += d[k];
+  }
+  // workgroup scan of the threads' sums
+  const uint64_t incl = wave_incl_scan64(sum);
+  if (lane == WAVE - 1) s_w[wid] = incl;
+  __syncthreads();
+  uint64_t wave_off = 0, agg = 0;
+  for (uint32_t k = 0; k < WG / WAVE; ++k) {
+    wave_off += k < wid ? s_w[k] : 0;
+    agg += s_w[k];
+  }
+  const unsigned long long tag = (unsigned long long)epoch << 50;
+  if (wid == 0) {
+    uint64_t excl = 0;
+    if (t == 0) {
+      if (lane == 0) __hip_atomic_store(&st[0], tag | DS_INC | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      if (lane == 0) __hip_atomic_store(&st[t], tag | DS_AGG | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int64_t p = (int64_t)t - 1;  // nearest predecessor not yet summed
+      bool failed = false;
+      for (;;) {
+        const int64_t q = p - (int64_t)lane;
+        unsigned long long w = tag | DS_INC;  // before tile 0: an inclusive prefix of 0
+        if (q >= 0) {
+          uint32_t spins = 0;
+          for (;;) {
+            w = __hip_atomic_load(&st[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((w & ~(DS_FLAGS | DS_VALUE)) == tag && (w & DS_FLAGS)) break;
+            if (++spins > DS_SPIN) {
+              failed = true;
+              w = tag | DS_INC;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+        }
+        const uint64_t incm = __ballot((w & DS_FLAGS) == DS_INC);
+        const uint32_t j = incm ? (uint32_t)__ffsll((unsigned long long)incm) - 1 : WAVE - 1;
+        uint64_t x = lane <= j ? (w & DS_VALUE) : 0;
+        x = __shfl(wave_sum64(x), 0);
+        excl += x;
+        if (incm) break;
+        p -= WAVE;
+      }
+      if (__ballot(failed) && lane == 0) *err = 1u;
+      if (lane == 0) __hip_atomic_store(&st[t], tag | DS_INC | (excl + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (lane == 0) s_excl = excl;
+  }
+  __syncthreads();
+  uint64_t run = s_excl + wave_off + incl - sum;
+#pragma unroll
+  for (uint32_t k = 0; k < DS_ITEMS; ++k) {
+    if (i0 + k <= f) S[i0 + k] = run;
+    run += d[k];
+  }
+}
+
 // zero a queue counter and the 64-bit level counters
 __global__ void k_level_reset(uint32_t* cnt_slot, unsigned long long* cnt64, uint32_t* s0 = nullptr,
                               uint32_t* s1 = nullptr) {
@@ -2195,6 +2302,8 @@ __global__ void k_color_init_list(const uint32_t* __restrict__ U, const uint32_t
 
 // pull sweep: v takes the max colour over in-neighbours (decided ones hold
 // 0); only v's thread writes col[v]; changed vertices go to `out`
+constexpr int CP_BATCH = 8;  // in-neighbours a colour pull reads per step
+
 __global__ __launch_bounds__(WG) void k_color_pull(const uint32_t* __restrict__ U, const uint32_t* ucount,
                                                    const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
                                                    uint32_t* col, uint32_t* __restrict__ out, uint32_t* out_cnt,
@@ -2214,14 +2323,22 @@ __global__ __launch_bounds__(WG) void k_color_pull(const uint32_t* __restrict__ 
       v = U[i];
       uint32_t c0 = col[v];
       uint32_t c = c0;
-      if (pid) {
-        const uint32_t pv = pid[v];
-        for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it) {
-          const uint32_t u = bw_adj[it];
-          if (pid[u] == pv) c = max(c, col[u]);
+      const uint32_t pv = pid ? pid[v] : 0u;
+      // CP_BATCH in-neighbours per step, their ids, then their colours and
+      // partitions, each group of loads in flight together
+      for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; it += CP_BATCH) {
+        uint32_t w[CP_BATCH], cw[CP_BATCH], pw[CP_BATCH];
+#pragma unroll
+        for (int k = 0; k < CP_BATCH; ++k) w[k] = it + k < end ? bw_adj[it + k] : v;
+#pragma unroll
+        for (int k = 0; k < CP_BATCH; ++k) {
+          const bool ok = it + k < end;
+          cw[k] = ok ? col[w[k]] : 0u;
+          pw[k] = ok && pid ? pid[w[k]] : pv;
         }
-      } else {
-        for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; ++it) c = max(c, col[bw_adj[it]]);
+#pragma unroll
+        for (int k = 0; k < CP_BATCH; ++k)
+          if (pw[k] == pv) c = max(c, cw[k]);
       }
       if (c > c0 && c0 != 0u) {
         atomicMax(&col[v], c);
@@ -3441,6 +3558,14 @@ int ctx_reserve(Ctx* c, uint32_t n, uint64_t m) {
   KSPN_TRY(dalloc(&c->col, (uint64_t)nn + 64));
   KSPN_TRY(dalloc(&c->deg, (uint64_t)nn + 64));
   KSPN_TRY(dalloc(&c->scan, (uint64_t)nn + 64));
+  {
+    // look-back scan: one status word per tile, then its ticket counter
+    const uint64_t tiles = ((uint64_t)nn + DS_TILE) / DS_TILE + 1;
+    KSPN_TRY(dalloc(&c->ds_st, tiles + 2));
+    KSPN_HIP(hipMemset(c->ds_st, 0, (tiles + 2) * sizeof(unsigned long long)));
+    c->ds_ticket = reinterpret_cast<uint32_t*>(c->ds_st + tiles + 1);
+    c->ds_epoch = 0;
+  }
   KSPN_TRY(dalloc(&c->part, (uint64_t)c->num_cu * 64 * 2 + 64));
   size_t b1 = 0, b2 = 0;
   if (hipcub::DeviceScan::ExclusiveSum(nullptr, b1, (uint64_t*)nullptr, (uint64_t*)nullptr, (int)(nn + 1)) != hipSuccess)
@@ -3614,6 +3739,33 @@ struct Run {
 
 int select_pivot_live(Run& R, uint32_t* pivot);
 
+// S = c->scan[0..f]: exclusive scan of the out-degrees of queue F (count on
+// device at fcount, host copy f); S[f] is the total. One workgroup below
+// 4096 entries, else the single-pass look-back scan (hipCUB's device scan
+// where a total could reach 2^48).
+int frontier_scan(Run& R, const uint32_t* F, const uint32_t* fcount_dev, uint32_t f, const uint64_t* off) {
+  Ctx* c = R.c;
+  Prof p(c, R.s, K_FRONTIER_DEG, 28.0 * f);
+  if (f < 4096) {  // S[0..f] fits the 4096 slots of one workgroup
+    k_deg_scan_small<<<1, 1024, 0, R.s>>>(F, fcount_dev, off, c->scan);
+    return KSPN_OK;
+  }
+  if (R.g->m >= DS_VALUE) {
+    k_frontier_deg<<<stream_grid(c, (uint64_t)f + 1), WG, 0, R.s>>>(F, fcount_dev, off, c->deg);
+    return exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)f + 1, R.s);
+  }
+  if (++c->ds_epoch >= DS_EPOCHS) {
+    // status words of this epoch may remain from DS_EPOCHS launches ago
+    KSPN_HIP(hipMemsetAsync(c->ds_st, 0, ((uint64_t)(c->cap_n + DS_TILE) / DS_TILE + 1) * sizeof(unsigned long long),
+                            R.s));
+    c->ds_epoch = 1;
+  }
+  const uint32_t tiles = (uint32_t)(((uint64_t)f + DS_TILE) / DS_TILE);
+  k_deg_scan<<<tiles, WG, 0, R.s>>>(F, fcount_dev, off, c->scan, c->ds_st, c->ds_ticket, tiles, c->ds_epoch,
+                                    &c->cnt[C_SERR]);
+  return KSPN_OK;
+}
+
 // Edge-balanced expansion of queue F (count on device at fcount, host copy
 // f) over CSR (off, adj). Appends to `next` with its counter at next_cnt.
 // Returns the expanded edge count through *edges.
@@ -3626,14 +3778,14 @@ int expand(Run& R, const uint32_t* F, const uint32_t* fcount_dev, uint32_t f, co
     if (edges) *edges = 0;
     return KSPN_OK;
   }
-  {
-    Prof p(c, R.s, K_FRONTIER_DEG, 4.0 * f + 16.0 * f + 8.0 * f);
-    k_frontier_deg<<<stream_grid(c, (uint64_t)f + 1), WG, 0, R.s>>>(F, fcount_dev, off, c->deg);
-  }
-  KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)f + 1, R.s));
+  KSPN_TRY(frontier_scan(R, F, fcount_dev, f, off));
   uint64_t total = total_hint;
   if (!known_total) {
     KSPN_TRY(publish(c, R.s, c->scan + f));
+    if (c->h_cnt[C_SERR]) {
+      note_error("k_deg_scan", "frontier count or ticket out of step with the launch", __FILE__, __LINE__);
+      return KSPN_E_HIP;
+    }
     total = c->h_u64[0];
   }
   if (edges) *edges = total;
@@ -3969,15 +4121,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       // nothing expanded (not expected: the bounds were checked): one level on the host
     }
     if (!pull) {
-      if (!scanned) {
-        Prof p(c, R.s, K_FRONTIER_DEG, 28.0 * f);
-        if (f < 4096) {  // S[0..f] fits the 4096 slots of one workgroup
-          k_deg_scan_small<<<1, 1024, 0, R.s>>>(F, &c->cnt[fc], off, c->scan);
-        } else {
-          k_frontier_deg<<<stream_grid(c, (uint64_t)f + 1), WG, 0, R.s>>>(F, &c->cnt[fc], off, c->deg);
-        }
-      }
-      if (f >= 4096 && !scanned) KSPN_TRY(exclusive_scan_u64(c, c->deg, c->scan, (uint64_t)f + 1, R.s));
+      if (!scanned) KSPN_TRY(frontier_scan(R, F, &c->cnt[fc], f, off));
       scanned = false;
       int kern = dir == 0 ? K_EXPAND_FW : K_EXPAND_BW;
       // tile width: the options' or, by default, the widest that still
@@ -4051,6 +4195,10 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
         spec_seq = publish_launch(c, R.s, nullptr, nullptr, 0, nullptr, &c->cnt[nc], nullptr, 1, &c->cnt[C_SPEC]);
       }
       KSPN_TRY(publish_collect(c, R.s, seq0));
+      if (c->h_cnt[C_SERR]) {
+        note_error("k_deg_scan", "frontier count or ticket out of step with the launch", __FILE__, __LINE__);
+        return KSPN_E_HIP;
+      }
     }
     first_level = false;
     if (!pull) {
@@ -5284,7 +5432,7 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   if (c->actl) (void)hipFree(c->actl);
   void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->sbm, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
                   c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf, c->pid, c->pbest,
-                  c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt, c->vprio};
+                  c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt, c->vprio, c->ds_st};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
