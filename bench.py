@@ -35,14 +35,17 @@ from kaspan_amd.workloads import CONFIGS, Workload  # noqa: E402  (no GPU work a
 
 METRIC = "edges/sec to full SCC labels on 1 MI355X; learned vs plain pivots, same labels"
 HBM_PEAK_GBS = 8000.0
-# library profile names -> kernel symbols as rocprofv3 reports them
-ROCPROF_NAMES = {"pull_fw": ["kspn::k_pull<0>"], "pull_bw": ["kspn::k_pull<1>"], "trim_first": ["kspn::k_trim_first"],
-                 "expand_fw": ["kspn::k_expand<0>"], "expand_bw": ["kspn::k_expand<1>", "kspn::k_expand<5>"],
-                 "label_scc": ["kspn::k_label_bits", "kspn::k_label_prep", "kspn::k_part_apply"],
-                 "tail_solve": ["kspn::k_tail_solve"], "trim_sweep": ["kspn::k_trim_sweep<1>"],
-                 "reach_coop": ["kspn::k_reach_coop<0>", "kspn::k_reach_coop<1>", "kspn::k_reach_coop<5>"],
-                 "bitmap_ops": ["kspn::k_part_classify"], "pivot_reduce": ["kspn::k_part_key"],
-                 "pull_list": ["kspn::k_pull_list<0>", "kspn::k_pull_list<1>"]}
+# library profile names -> prefixes of the kernel symbols rocprofv3 reports for them (every instantiation)
+ROCPROF_NAMES = {"pull_fw": ["kspn::k_pull<0,"], "pull_bw": ["kspn::k_pull<1,"], "trim_first": ["kspn::k_trim_first"],
+                 "expand_fw": ["kspn::k_expand<0,"], "expand_bw": ["kspn::k_expand<1,", "kspn::k_expand<5,"],
+                 "expand_color": ["kspn::k_expand<3,"], "expand_color_bw": ["kspn::k_expand<4,"],
+                 "label_scc": ["kspn::k_label_bits", "kspn::k_label_prep", "kspn::k_part_apply", "kspn::k_part_label"],
+                 "tail_solve": ["kspn::k_tail_solve"], "trim_sweep": ["kspn::k_trim_sweep<"],
+                 "reach_coop": ["kspn::k_reach_coop<", "kspn::k_chain_level<"],
+                 "bitmap_ops": ["kspn::k_part_split", "kspn::k_part_classify"],
+                 "pivot_reduce": ["kspn::k_part_key", "kspn::k_part_prio"],
+                 "frontier_deg": ["kspn::k_deg_scan", "kspn::k_frontier_deg"], "color_pull": ["kspn::k_color_pull"],
+                 "pull_list": ["kspn::k_pull_list<"]}
 
 METHOD_NAMES = ["kaspan", "trim_ex", "hpc_like", "ispan_like", "hpc_trim_ex", "trim_ex_residual",
                 "trim_ex_light_residual"]
@@ -65,7 +68,8 @@ def pmc_traffic(config, kernel):
     if not os.path.exists(path):
         return None
     data = json.load(open(path)).get("kernels", {})
-    found = [data[k] for k in ROCPROF_NAMES.get(kernel, []) if k in data]
+    prefixes = ROCPROF_NAMES.get(kernel, [])
+    found = [v for k, v in data.items() if any(k.startswith(p) for p in prefixes)]
     launches = sum(k["launches"] for k in found)
     if not launches:
         return None
@@ -381,6 +385,21 @@ def main():
                   "fwbw_rounds_learned": learned_opts.get("fwbw_rounds"),
                   "fwbw_decided_plain": st_plain.decided_fwbw, "fwbw_decided_learned": st_learned.decided_fwbw,
                   "plain_edges_per_s": g.m / (plain_ms / 1e3), "learned_edges_per_s": g.m / (learned_ms / 1e3)}
+        # the two rules at the same options (each rule's tuned options in turn): the pivot rule is then
+        # the only difference
+        knobs_l = {k: v for k, v in learned_opts.items() if k not in ("pivot", "mlp_params")}
+        if knobs_l != plain_opts:
+            same = {}
+            for name, knobs in (("plain_options", plain_opts), ("learned_options", knobs_l)):
+                pl = plain_ms if knobs is plain_opts else None
+                ll = learned_ms if knobs is knobs_l else None
+                if pl is None:
+                    pl, _ = timed(wl, labels, knobs, args.steps, args.warmup)
+                if ll is None:
+                    ll, _ = timed(wl, labels, dict(knobs, pivot=1, mlp_params=mlp), args.steps, args.warmup)
+                same[name] = {"fwbw_rounds": knobs.get("fwbw_rounds"), "plain_ms": pl, "learned_ms": ll,
+                              "speedup_learned": pl / ll, "labels_identical": bool(torch.equal(first, labels))}
+            ab_out["same_options"] = same
         if kind == "planted":
             planted = ka.planted_labels(wl.seed, wl.kw["n"], max_size=wl.kw["max_size"], expo=wl.kw["expo"])
             ab_out["labels_equal_planted"] = bool(torch.equal(first, planted))

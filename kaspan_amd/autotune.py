@@ -6,7 +6,8 @@ Coordinate descent over the options that change time but never labels:
   of the push expansion (compiled instantiations), and the workgroups per
   CU of pull sweeps and list levels;
 * the push/pull switch (Beamer alpha, beta);
-* the persistent multi-level kernel's frontier bound and workgroup count;
+* the persistent multi-level kernel's frontier bound and workgroup count,
+  and whether its levels run as chained launches instead;
 * FW-BW rounds and the tail-solver threshold;
 * the pipeline method.
 
@@ -29,14 +30,17 @@ TUNED_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned")
 
 # candidate values per knob, in the order they are tried
 SPACE = {
+    # KSPN_FLAG_FULL_BW (64): three-way split of the multi-pivot rounds; KSPN_FLAG_CHAIN (512): the
+    # persistent kernel's push levels as chained single-level launches. First: they change what the
+    # frontier bounds below mean.
+    "flags": [0, 64, 512, 576],
+    "fwbw_rounds": [1, 4, 16, 24, 32, 48, 64],
     "alpha": [14.0, 4.0, 40.0, 1e-9],
     "beta": [24.0, 6.0, 96.0],
     "coop_edges": [1 << 20, 1 << 16, 1 << 23, 1],
     "coop_workgroups": [128, 64, 256],
-    "fwbw_rounds": [1, 4, 16, 32, 48, 64],
     "tail_threshold": [4096, 1024, 16384],
     "method": [0, 1],
-    "flags": [0, 64],  # KSPN_FLAG_FULL_BW: three-way split of the multi-pivot rounds
 }
 # launch shapes: compiled instantiations (pull sweep vertices per thread, push tile edges per thread) and
 # grid sizes (workgroups per CU) the library selects at run time; 0 is the library's default

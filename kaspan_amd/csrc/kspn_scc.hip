@@ -451,6 +451,68 @@ __device__ inline void reach_batch(const uint32_t (&u)[K], const uint32_t (&x)[K
   }
 }
 
+// The colour ops of a thread's K edges u[r] -> x[r], loads of each step
+// issued for all K together. OP_COLOR: a live target of smaller colour
+// (same partition) takes the source's colour by atomicMax and is queued
+// once (inq). OP_COLOR_BW: a live in-neighbour of the same colour is
+// decided (removed from live, labelled with the colour's root) and queued.
+// Same effect as expand_apply on each edge in turn.
+template <int OP, int K>
+__device__ inline void color_batch(const uint32_t (&u)[K], const uint32_t (&x)[K], const bool (&ok)[K],
+                                   const OpArgs& a, BlockQueue& q) {
+  uint32_t cu[K], cw[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    cu[r] = ok[r] ? a.col[u[r]] : 0u;
+    cw[r] = ok[r] ? a.col[x[r]] : 0u;
+  }
+  bool c[K];
+  if constexpr (OP == OP_COLOR) {
+#pragma unroll
+    for (int r = 0; r < K; ++r) c[r] = ok[r] && cw[r] != 0u && cw[r] < cu[r];
+    if (a.part) {
+      uint32_t pu[K], pw[K];
+#pragma unroll
+      for (int r = 0; r < K; ++r) {
+        pu[r] = c[r] ? a.part[u[r]] : 0u;
+        pw[r] = c[r] ? a.part[x[r]] : 0u;
+      }
+#pragma unroll
+      for (int r = 0; r < K; ++r) c[r] = c[r] && pu[r] == pw[r];
+    }
+    uint32_t old[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) old[r] = c[r] ? atomicMax(&a.col[x[r]], cu[r]) : 0xFFFFFFFFu;
+    uint32_t qw[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      c[r] = c[r] && old[r] < cu[r];
+      qw[r] = c[r] ? atomicOr(&a.inq[x[r] >> 5], 1u << (x[r] & 31)) : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r) bq_push(q, c[r] && !((qw[r] >> (x[r] & 31)) & 1u), x[r]);
+  } else {
+#pragma unroll
+    for (int r = 0; r < K; ++r) c[r] = ok[r] && cw[r] == cu[r];
+    uint32_t lw[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) lw[r] = c[r] ? a.live[x[r] >> 5] : 0u;
+    uint32_t old[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      const uint32_t bit = 1u << (x[r] & 31);
+      c[r] = c[r] && (lw[r] & bit);
+      old[r] = c[r] ? atomicAnd(&a.live[x[r] >> 5], ~bit) : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      const bool push = c[r] && ((old[r] >> (x[r] & 31)) & 1u);
+      if (push) a.labels[x[r]] = rotl32(~cu[r], a.value);
+      bq_push(q, push, x[r]);
+    }
+  }
+}
+
 constexpr int EXP_EPT = 8;        // edges per thread of a tile (default; 4 and 16 are also built)
 constexpr int EXP_WIN = 1024;     // frontier entries staged in LDS per tile
 
@@ -530,7 +592,7 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
       }
       __syncthreads();
       uint32_t lo = 0;
-      if constexpr (OP == OP_FW || OP == OP_BW || OP == OP_PBW) {
+      if constexpr (OP == OP_FW || OP == OP_BW || OP == OP_PBW || OP == OP_COLOR || OP == OP_COLOR_BW) {
         // the thread's edges of the tile, their claims in flight together
         constexpr int EPT = EXP_TILE / WG;
         uint32_t uu[EPT], xx[EPT];
@@ -550,7 +612,8 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
             xx[r] = adj[s_beg[lo] + (e - s_S[lo])];
           }
         }
-        reach_batch<OP, EPT>(uu, xx, ok, a, q, nsum, nmax);
+        if constexpr (OP == OP_COLOR || OP == OP_COLOR_BW) color_batch<OP, EPT>(uu, xx, ok, a, q);
+        else reach_batch<OP, EPT>(uu, xx, ok, a, q, nsum, nmax);
       } else {
         for (uint64_t e = e0 + threadIdx.x; e < e1; e += WG) {
           uint32_t hi = w;
