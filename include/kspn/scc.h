@@ -71,7 +71,7 @@ typedef struct kspn_graph {
 #define KSPN_FLAG_NO_TRIM2 2u /* KSPN_METHOD_TRIM_EX without the trim-2 pass */
 #define KSPN_FLAG_NO_TAIL 4u  /* never hand the rest to the single-workgroup tail solver */
 #define KSPN_FLAG_NO_REFINE 8u /* colour rounds over the whole live graph, without partition refinement */
-#define KSPN_FLAG_NO_COOP 16u  /* every FW/BW level as separate launches (no persistent multi-level kernel) */
+#define KSPN_FLAG_NO_COOP 16u  /* every FW/BW level as separate launches with a host round trip each (no chained or persistent levels) */
 #define KSPN_FLAG_STATS_MEMORY 32u /* fill kspn_phase.memory (one hipMemGetInfo per scope that carries it) */
 /* multi-pivot FW-BW rounds search backward over the whole partition and split it in three (forward-only,
  * backward-only, unreached; Fleischer, Hendrickson and Pinar). Default: backward only within the forward
@@ -88,12 +88,15 @@ typedef struct kspn_graph {
  * take the claimed vertices' row slices from a queue as they appear (coop_workgroups workgroups of 256
  * threads, 0 -> 512). The reached sets, and so the labels, are the same as level by level. */
 #define KSPN_FLAG_ASYNC 256u
-/* The push levels the persistent multi-level kernel would take (frontier degree sum <= coop_edges, largest
- * degree <= coop_degree) run instead as chains of chain_levels launches, one level each, enqueued back to back
- * with a single host round trip per chain: no grid barrier, and the coop_workgroups workgroups need not be
- * resident together. A level that ends the run (empty frontier, pull switch, frontier too large) makes the
- * rest of its chain return at once. Same reached sets and labels. */
+/* Push levels on the device (frontier degree sum <= coop_edges, largest degree <= coop_degree) run as chains
+ * of chain_levels launches, one level each, enqueued back to back with a single host round trip per chain: no
+ * grid barrier, and the coop_workgroups workgroups need not be resident together. A level that ends the run
+ * (empty frontier, pull switch, frontier too large) makes the rest of its chain return at once. This is the
+ * default; KSPN_FLAG_CHAIN asks for it explicitly (it wins over KSPN_FLAG_PERSISTENT). */
 #define KSPN_FLAG_CHAIN 512u
+/* Those push levels in one persistent launch instead, with a grid barrier (agent-scope release and acquire)
+ * between levels; coop_workgroups must be resident together (capped at what the device holds). Same labels. */
+#define KSPN_FLAG_PERSISTENT 1024u
 
 typedef struct kspn_opts {
   uint32_t method;             /* KSPN_METHOD_* */

@@ -45,6 +45,7 @@ FLAG_FULL_BW = 64
 FLAG_GRAPH = 128
 FLAG_ASYNC = 256
 FLAG_CHAIN = 512
+FLAG_PERSISTENT = 1024
 PHASE_MAX = 12
 MANIFEST_TEXT = 256
 MANIFEST_PATH = 4096

@@ -6,8 +6,8 @@ Coordinate descent over the options that change time but never labels:
   of the push expansion (compiled instantiations), and the workgroups per
   CU of pull sweeps and list levels;
 * the push/pull switch (Beamer alpha, beta);
-* the persistent multi-level kernel's frontier bound and workgroup count,
-  and whether its levels run as chained launches instead;
+* the frontier bound and workgroup count of the push levels run on the
+  device, and whether they run as chained launches or one persistent launch;
 * FW-BW rounds and the tail-solver threshold;
 * the pipeline method.
 
@@ -30,10 +30,10 @@ TUNED_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned")
 
 # candidate values per knob, in the order they are tried
 SPACE = {
-    # KSPN_FLAG_FULL_BW (64): three-way split of the multi-pivot rounds; KSPN_FLAG_CHAIN (512): the
-    # persistent kernel's push levels as chained single-level launches. First: they change what the
+    # KSPN_FLAG_FULL_BW (64): three-way split of the multi-pivot rounds; KSPN_FLAG_PERSISTENT (1024): device
+    # push levels in one persistent launch instead of chained launches. First: they change what the
     # frontier bounds below mean.
-    "flags": [0, 64, 512, 576],
+    "flags": [0, 64, 1024, 1088],
     "fwbw_rounds": [1, 4, 16, 24, 32, 48, 64],
     "alpha": [14.0, 4.0, 40.0, 1e-9],
     "beta": [24.0, 6.0, 96.0],

@@ -5558,8 +5558,8 @@ int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* o
   R.coop_wg = std::min<uint32_t>(o.coop_workgroups ? o.coop_workgroups : kCoopWorkgroups, c->coop_grid);
   R.pull_ilp = o.pull_ilp ? o.pull_ilp : (uint32_t)PULL_ILP;
   R.ept = o.expand_ept;  // 0: chosen per level
-  if (o.flags & KSPN_FLAG_CHAIN) {
-    // chained launches need no co-residency: the grid is not capped by it
+  if (!(o.flags & KSPN_FLAG_PERSISTENT) || (o.flags & KSPN_FLAG_CHAIN)) {
+    // chained launches (the default) need no co-residency: the grid is not capped by it
     R.chain = o.chain_levels ? o.chain_levels : kChainLevels;
     R.coop_wg = o.coop_workgroups ? o.coop_workgroups : kCoopWorkgroups;
   }

@@ -31,10 +31,12 @@ METHODS = [
     dict(method=0, flags=4 | 8),               # the reference's colour rounds: no tail, no refinement
     dict(method=0, flags=16),                  # every FW/BW level as separate launches
     dict(method=0, fwbw_rounds=8, flags=16),   # partitioned rounds, separate launches
+    dict(method=0, flags=1024),                # device push levels in one persistent launch
+    dict(method=0, fwbw_rounds=8, flags=1024),
 ]
 MIDS = ["kaspan", "trim_ex", "trim_ex_no_trim2", "fwbw4", "residual300", "kaspan_residual_all", "pull", "push",
         "kaspan_no_tail", "trim_ex_no_tail", "tail_all", "multi_pivot64", "multi_pivot3_pull", "kaspan_plain_colour",
-        "no_coop", "multi_pivot8_no_coop"]
+        "no_coop", "multi_pivot8_no_coop", "persistent", "multi_pivot8_persistent"]
 
 
 def host_graph(O, g):
@@ -163,19 +165,21 @@ def test_random_graphs(ka, O, kind, opts):
 
 
 def test_persistent_levels(ka, O):
-    """long-diameter lattice: the persistent kernel runs the push levels (and
-    expands its own claims for a few rounds per level, so it needs fewer
-    levels); the same labels and the same FW-BW decisions as separate
-    launches per level"""
+    """long-diameter lattice: the device runs the push levels, as chained
+    launches (default) or in one persistent launch, and expands its own claims
+    for a few rounds per level, so it needs fewer levels; the same labels and
+    the same FW-BW decisions as separate launches per level"""
     g = ka.generate("lattice", 31, side=40, p_bidir=0.05)
     ref = O.tarjan(host_graph(O, g))
     for rounds in (1, 16):
-        lab, st = gpu_labels(ka, g, fwbw_rounds=rounds)
         lab2, st2 = gpu_labels(ka, g, fwbw_rounds=rounds, flags=16)
-        assert np.array_equal(lab, ref) and np.array_equal(lab2, ref)
-        assert st.fwbw_levels_coop > 0 and st2.fwbw_levels_coop == 0
-        assert st.fwbw_levels_coop + st.fwbw_levels_push <= st2.fwbw_levels_push
-        assert st.decided_fwbw == st2.decided_fwbw
+        assert np.array_equal(lab2, ref) and st2.fwbw_levels_coop == 0
+        for flags in (0, 1024):
+            lab, st = gpu_labels(ka, g, fwbw_rounds=rounds, flags=flags)
+            assert np.array_equal(lab, ref)
+            assert st.fwbw_levels_coop > 0
+            assert st.fwbw_levels_coop + st.fwbw_levels_push <= st2.fwbw_levels_push
+            assert st.decided_fwbw == st2.decided_fwbw
 
 
 def test_c1_gnm_full(ka, O):
