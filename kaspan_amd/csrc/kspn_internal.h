@@ -318,6 +318,9 @@ struct Ctx {
   uint32_t* pcnt = nullptr;    // per partition: outcome counters for the pivot log
   float* vscore = nullptr;     // per vertex: MLP score
   uint32_t* vprio = nullptr;   // per vertex: pivot priority of the multi-pivot rounds (fixed within a solve)
+  unsigned long long* cpw = nullptr;    // colour rounds: (partition << 32 | colour) per vertex
+  uint8_t* deg8 = nullptr;              // features: min(degree, 255) per vertex, out then in
+  uint32_t deg8_cap = 0;
   unsigned long long* ds_st = nullptr;  // look-back scan: status word per tile
   uint32_t* ds_ticket = nullptr;        // its ticket counter
   uint32_t ds_epoch = 0;                // epoch of its last launch

@@ -22,25 +22,42 @@ static_assert(P_COUNT == KSPN_MLP_PARAMS, "parameter layout");
 // In/out degree, their logs, and the 2-hop sums (out-degrees of
 // out-neighbours, in-degrees of in-neighbours). One lane per vertex: a
 // wavefront takes 64 consecutive list entries, each lane sums its own rows
-// (four neighbour offsets in flight at a time), so the rows of neighbouring
-// lanes are read from neighbouring memory. Vertices with a row longer than
+// (four neighbours in flight at a time), so the rows of neighbouring lanes
+// are read from neighbouring memory. Vertices with a row longer than
 // FEAT_LANE_MAX are summed afterwards by the whole wavefront, one at a
 // time, so a hub costs its degree / 64 steps, not its degree.
+// The neighbours' degrees come from byte arrays (k_deg8: min(degree, 255)),
+// n bytes per direction instead of 8n of offsets, so the random reads stay
+// in the Infinity Cache; a saturated entry is read exactly from the offsets.
 constexpr uint64_t FEAT_LANE_MAX = 64;
+constexpr uint32_t DEG8_SAT = 255;
 
-__device__ inline uint64_t row_degree_sum(const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj,
-                                          uint64_t b, uint64_t e) {
+__global__ void k_deg8(uint32_t n, const uint64_t* __restrict__ fw_off, const uint64_t* __restrict__ bw_off,
+                       uint8_t* __restrict__ out8, uint8_t* __restrict__ in8) {
+  for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG) {
+    const uint64_t o = fw_off[v + 1] - fw_off[v], i = bw_off[v + 1] - bw_off[v];
+    out8[v] = (uint8_t)(o < DEG8_SAT ? o : DEG8_SAT);
+    in8[v] = (uint8_t)(i < DEG8_SAT ? i : DEG8_SAT);
+  }
+}
+
+__device__ inline uint64_t deg_of(const uint8_t* __restrict__ d8, const uint64_t* __restrict__ off, uint32_t w,
+                                  uint32_t d) {
+  return d < DEG8_SAT ? (uint64_t)d : off[w + 1] - off[w];
+}
+
+__device__ inline uint64_t row_degree_sum(const uint8_t* __restrict__ d8, const uint64_t* __restrict__ off,
+                                          const uint32_t* __restrict__ adj, uint64_t b, uint64_t e) {
   uint64_t sum = 0;
   uint64_t it = b;
   for (; it + 4 <= e; it += 4) {
     const uint32_t w0 = adj[it], w1 = adj[it + 1], w2 = adj[it + 2], w3 = adj[it + 3];
-    const uint64_t a0 = off[w0], a1 = off[w1], a2 = off[w2], a3 = off[w3];
-    const uint64_t c0 = off[w0 + 1], c1 = off[w1 + 1], c2 = off[w2 + 1], c3 = off[w3 + 1];
-    sum += (c0 - a0) + (c1 - a1) + (c2 - a2) + (c3 - a3);
+    const uint32_t d0 = d8[w0], d1 = d8[w1], d2 = d8[w2], d3 = d8[w3];
+    sum += deg_of(d8, off, w0, d0) + deg_of(d8, off, w1, d1) + deg_of(d8, off, w2, d2) + deg_of(d8, off, w3, d3);
   }
   for (; it < e; ++it) {
     const uint32_t w = adj[it];
-    sum += off[w + 1] - off[w];
+    sum += deg_of(d8, off, w, d8[w]);
   }
   return sum;
 }
@@ -48,6 +65,7 @@ __device__ inline uint64_t row_degree_sum(const uint64_t* __restrict__ off, cons
 __global__ __launch_bounds__(WG) void k_features(uint32_t count, const uint32_t* __restrict__ list,
                                                  const uint64_t* __restrict__ fw_off, const uint32_t* __restrict__ fw_adj,
                                                  const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
+                                                 const uint8_t* __restrict__ out8, const uint8_t* __restrict__ in8,
                                                  float* __restrict__ feat) {
   const uint32_t lane = lane_id();
   const uint64_t wave = (blockIdx.x * (uint64_t)WG + threadIdx.x) / WAVE;
@@ -66,8 +84,8 @@ __global__ __launch_bounds__(WG) void k_features(uint32_t count, const uint32_t*
     const bool own = ok && fe - fb <= FEAT_LANE_MAX && be - bb <= FEAT_LANE_MAX;
     uint64_t s2o = 0, s2i = 0;
     if (own) {
-      s2o = row_degree_sum(fw_off, fw_adj, fb, fe);
-      s2i = row_degree_sum(bw_off, bw_adj, bb, be);
+      s2o = row_degree_sum(out8, fw_off, fw_adj, fb, fe);
+      s2i = row_degree_sum(in8, bw_off, bw_adj, bb, be);
     }
     // the long rows: the whole wavefront, one vertex at a time
     uint64_t big = __ballot(ok && !own);
@@ -78,11 +96,11 @@ __global__ __launch_bounds__(WG) void k_features(uint32_t count, const uint32_t*
       uint64_t so = 0, si = 0;
       for (uint64_t it = jfb + lane; it < jfe; it += WAVE) {
         const uint32_t w = fw_adj[it];
-        so += fw_off[w + 1] - fw_off[w];
+        so += deg_of(out8, fw_off, w, out8[w]);
       }
       for (uint64_t it = jbb + lane; it < jbe; it += WAVE) {
         const uint32_t w = bw_adj[it];
-        si += bw_off[w + 1] - bw_off[w];
+        si += deg_of(in8, bw_off, w, in8[w]);
       }
       so = __shfl(wave_sum64(so), 0);  // the sum is complete in lane 0
       si = __shfl(wave_sum64(si), 0);
@@ -482,8 +500,19 @@ __global__ __launch_bounds__(1024) void k_argmax_score(uint32_t count, const uin
 // host helpers used by the pipeline (kspn_scc.hip)
 int mlp_features(Ctx* c, const kspn_graph* g, const uint32_t* list, uint32_t count, float* feat, hipStream_t s) {
   if (count == 0) return KSPN_OK;
+  if (c->deg8_cap < g->n) {
+    if (c->deg8) (void)hipFree(c->deg8);
+    c->deg8 = nullptr;
+    c->deg8_cap = 0;
+    KSPN_HIP(hipMalloc((void**)&c->deg8, 2ull * g->n + 64));
+    c->deg8_cap = g->n;
+  }
+  uint8_t* out8 = c->deg8;
+  uint8_t* in8 = c->deg8 + g->n;
+  k_deg8<<<(uint32_t)std::min<uint64_t>(((uint64_t)g->n + WG - 1) / WG, (uint64_t)c->num_cu * 16), WG, 0, s>>>(
+      g->n, g->fw_off, g->bw_off, out8, in8);
   uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)count + WG - 1) / WG, (uint64_t)c->num_cu * 16);
-  k_features<<<grid, WG, 0, s>>>(count, list, g->fw_off, g->fw_adj, g->bw_off, g->bw_adj, feat);
+  k_features<<<grid, WG, 0, s>>>(count, list, g->fw_off, g->fw_adj, g->bw_off, g->bw_adj, out8, in8, feat);
   KSPN_HIP(hipGetLastError());
   return KSPN_OK;
 }

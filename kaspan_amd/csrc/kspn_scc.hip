@@ -339,6 +339,7 @@ struct OpArgs {
   const uint32_t* pcand;         // OP_PBW: the vertices it may claim (live, or fwv for a search within FW sets)
   uint32_t dead_pid;             // part holds DEAD_PID for every vertex that is not live, so a partition match
                                  // implies live and the live bitmap need not be read
+  unsigned long long* cp;        // OP_COLOR/OP_COLOR_BW: per vertex (partition << 32 | colour)
 };
 
 // partition id of a vertex that is not live during the multi-pivot rounds
@@ -369,16 +370,15 @@ __device__ inline void expand_apply(uint32_t u, uint32_t w, const OpArgs& a, Blo
   } else if constexpr (OP == OP_NEIGH) {
     push = bm_get(a.live, w) && bm_claim(a.inq, w);
   } else if constexpr (OP == OP_COLOR) {
-    uint32_t c = a.col[u];
-    uint32_t cw = a.col[w];
-    if (cw != 0u && cw < c && (!a.part || a.part[w] == a.part[u])) {
-      uint32_t old = atomicMax(&a.col[w], c);
-      push = (old < c) && bm_claim(a.inq, w);
+    const unsigned long long xu = a.cp[u], xw = a.cp[w];
+    if ((uint32_t)xw != 0u && (uint32_t)xw < (uint32_t)xu && (xw >> 32) == (xu >> 32)) {
+      const unsigned long long old = atomicMax(&a.cp[w], xu);
+      push = ((uint32_t)old < (uint32_t)xu) && bm_claim(a.inq, w);
     }
   } else if constexpr (OP == OP_COLOR_BW) {
-    uint32_t c = a.col[u];
-    if (a.col[w] == c && bm_unclaim(a.live, w)) {
-      a.labels[w] = rotl32(~c, a.value);
+    const unsigned long long xu = a.cp[u];
+    if (a.cp[w] == xu && bm_unclaim(a.live, w)) {
+      a.labels[w] = rotl32(~(uint32_t)xu, a.value);
       push = true;
     }
   }
@@ -452,48 +452,40 @@ __device__ inline void reach_batch(const uint32_t (&u)[K], const uint32_t (&x)[K
 }
 
 // The colour ops of a thread's K edges u[r] -> x[r], loads of each step
-// issued for all K together. OP_COLOR: a live target of smaller colour
-// (same partition) takes the source's colour by atomicMax and is queued
-// once (inq). OP_COLOR_BW: a live in-neighbour of the same colour is
-// decided (removed from live, labelled with the colour's root) and queued.
+// issued for all K together. OP_COLOR: a live target of smaller colour in
+// the same partition takes the source's colour by atomicMax on its
+// (partition, colour) word and is queued once (inq). OP_COLOR_BW: a live
+// in-neighbour with the same word is decided (removed from live, labelled
+// with the colour's root) and queued.
 // Same effect as expand_apply on each edge in turn.
 template <int OP, int K>
 __device__ inline void color_batch(const uint32_t (&u)[K], const uint32_t (&x)[K], const bool (&ok)[K],
                                    const OpArgs& a, BlockQueue& q) {
-  uint32_t cu[K], cw[K];
+  unsigned long long xu[K], xw[K];
 #pragma unroll
   for (int r = 0; r < K; ++r) {
-    cu[r] = ok[r] ? a.col[u[r]] : 0u;
-    cw[r] = ok[r] ? a.col[x[r]] : 0u;
+    xu[r] = ok[r] ? a.cp[u[r]] : 0ull;
+    xw[r] = ok[r] ? a.cp[x[r]] : 0ull;
   }
   bool c[K];
   if constexpr (OP == OP_COLOR) {
 #pragma unroll
-    for (int r = 0; r < K; ++r) c[r] = ok[r] && cw[r] != 0u && cw[r] < cu[r];
-    if (a.part) {
-      uint32_t pu[K], pw[K];
+    for (int r = 0; r < K; ++r)
+      c[r] = ok[r] && (uint32_t)xw[r] != 0u && (uint32_t)xw[r] < (uint32_t)xu[r] && (xw[r] >> 32) == (xu[r] >> 32);
+    unsigned long long old[K];
 #pragma unroll
-      for (int r = 0; r < K; ++r) {
-        pu[r] = c[r] ? a.part[u[r]] : 0u;
-        pw[r] = c[r] ? a.part[x[r]] : 0u;
-      }
-#pragma unroll
-      for (int r = 0; r < K; ++r) c[r] = c[r] && pu[r] == pw[r];
-    }
-    uint32_t old[K];
-#pragma unroll
-    for (int r = 0; r < K; ++r) old[r] = c[r] ? atomicMax(&a.col[x[r]], cu[r]) : 0xFFFFFFFFu;
+    for (int r = 0; r < K; ++r) old[r] = c[r] ? atomicMax(&a.cp[x[r]], xu[r]) : ~0ull;
     uint32_t qw[K];
 #pragma unroll
     for (int r = 0; r < K; ++r) {
-      c[r] = c[r] && old[r] < cu[r];
+      c[r] = c[r] && (uint32_t)old[r] < (uint32_t)xu[r];
       qw[r] = c[r] ? atomicOr(&a.inq[x[r] >> 5], 1u << (x[r] & 31)) : 0xFFFFFFFFu;
     }
 #pragma unroll
     for (int r = 0; r < K; ++r) bq_push(q, c[r] && !((qw[r] >> (x[r] & 31)) & 1u), x[r]);
   } else {
 #pragma unroll
-    for (int r = 0; r < K; ++r) c[r] = ok[r] && cw[r] == cu[r];
+    for (int r = 0; r < K; ++r) c[r] = ok[r] && xw[r] == xu[r];
     uint32_t lw[K];
 #pragma unroll
     for (int r = 0; r < K; ++r) lw[r] = c[r] ? a.live[x[r] >> 5] : 0u;
@@ -507,7 +499,7 @@ __device__ inline void color_batch(const uint32_t (&u)[K], const uint32_t (&x)[K
 #pragma unroll
     for (int r = 0; r < K; ++r) {
       const bool push = c[r] && ((old[r] >> (x[r] & 31)) & 1u);
-      if (push) a.labels[x[r]] = rotl32(~cu[r], a.value);
+      if (push) a.labels[x[r]] = rotl32(~(uint32_t)xu[r], a.value);
       bq_push(q, push, x[r]);
     }
   }
@@ -2348,29 +2340,38 @@ __global__ __launch_bounds__(WG) void k_trim2(const uint32_t* __restrict__ Q, co
 // same root as the reference's min of rotr(v, r); 0 marks decided vertices.
 __device__ inline uint32_t color_key(uint32_t v, uint32_t r) { return ~rotr32(v, r); }
 
-__global__ void k_color_init_all(uint32_t n, const uint32_t* __restrict__ live, uint32_t* __restrict__ col, uint32_t r) {
+// Colour rounds keep, per vertex, one 64-bit word: the partition id in the
+// high half (0 without refinement) and the colour in the low half (0 for a
+// decided vertex). A neighbour's partition and colour then come from one
+// random read, and atomicMax on the word raises the colour of a vertex whose
+// partition never changes within a round.
+__device__ inline unsigned long long color_word(uint32_t p, uint32_t c) { return ((unsigned long long)p << 32) | c; }
+
+__global__ void k_color_init_all(uint32_t n, const uint32_t* __restrict__ live, const uint32_t* __restrict__ pid,
+                                 unsigned long long* __restrict__ cp, uint32_t r) {
   for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG)
-    col[v] = bm_get(live, (uint32_t)v) ? color_key((uint32_t)v, r) : 0u;
+    cp[v] = bm_get(live, (uint32_t)v) ? color_word(pid ? pid[v] : 0u, color_key((uint32_t)v, r)) : 0ull;
 }
 
-// re-key the previous round's live list: still live -> new key, decided -> 0
+// re-key the previous round's live list: still live -> new key (and current
+// partition), decided -> 0
 __global__ void k_color_init_list(const uint32_t* __restrict__ U, const uint32_t* ucount, const uint32_t* __restrict__ live,
-                                  uint32_t* __restrict__ col, uint32_t r) {
+                                  const uint32_t* __restrict__ pid, unsigned long long* __restrict__ cp, uint32_t r) {
   uint32_t u = *ucount;
   for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < u; i += gridDim.x * WG) {
     uint32_t v = U[i];
-    col[v] = bm_get(live, v) ? color_key(v, r) : 0u;
+    cp[v] = bm_get(live, v) ? color_word(pid ? pid[v] : 0u, color_key(v, r)) : 0ull;
   }
 }
 
-// pull sweep: v takes the max colour over in-neighbours (decided ones hold
-// 0); only v's thread writes col[v]; changed vertices go to `out`
+// pull sweep: v takes the max colour over its in-neighbours of the same
+// partition (decided ones hold 0); only v's thread raises cp[v]; changed
+// vertices go to `out`
 constexpr int CP_BATCH = 8;  // in-neighbours a colour pull reads per step
 
 __global__ __launch_bounds__(WG) void k_color_pull(const uint32_t* __restrict__ U, const uint32_t* ucount,
                                                    const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
-                                                   uint32_t* col, uint32_t* __restrict__ out, uint32_t* out_cnt,
-                                                   const uint32_t* __restrict__ pid) {
+                                                   unsigned long long* cp, uint32_t* __restrict__ out, uint32_t* out_cnt) {
   __shared__ uint32_t s_out[8 * WG];
   __shared__ uint32_t s_on, s_ob;
   BlockQueue q{s_out, &s_on, &s_ob, 8 * WG};
@@ -2384,27 +2385,24 @@ __global__ __launch_bounds__(WG) void k_color_pull(const uint32_t* __restrict__ 
     uint32_t v = 0;
     if (i < u) {
       v = U[i];
-      uint32_t c0 = col[v];
+      const unsigned long long w0 = cp[v];
+      const uint32_t c0 = (uint32_t)w0, pv = (uint32_t)(w0 >> 32);
       uint32_t c = c0;
-      const uint32_t pv = pid ? pid[v] : 0u;
-      // CP_BATCH in-neighbours per step, their ids, then their colours and
-      // partitions, each group of loads in flight together
+      // CP_BATCH in-neighbours per step: their ids, then their words, each
+      // group of loads in flight together
       for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; it += CP_BATCH) {
-        uint32_t w[CP_BATCH], cw[CP_BATCH], pw[CP_BATCH];
+        uint32_t w[CP_BATCH];
+        unsigned long long x[CP_BATCH];
 #pragma unroll
         for (int k = 0; k < CP_BATCH; ++k) w[k] = it + k < end ? bw_adj[it + k] : v;
 #pragma unroll
-        for (int k = 0; k < CP_BATCH; ++k) {
-          const bool ok = it + k < end;
-          cw[k] = ok ? col[w[k]] : 0u;
-          pw[k] = ok && pid ? pid[w[k]] : pv;
-        }
+        for (int k = 0; k < CP_BATCH; ++k) x[k] = it + k < end ? cp[w[k]] : 0ull;
 #pragma unroll
         for (int k = 0; k < CP_BATCH; ++k)
-          if (pw[k] == pv) c = max(c, cw[k]);
+          if ((uint32_t)(x[k] >> 32) == pv) c = max(c, (uint32_t)x[k]);
       }
       if (c > c0 && c0 != 0u) {
-        atomicMax(&col[v], c);
+        atomicMax(&cp[v], color_word(pv, c));
         changed = true;
       }
     }
@@ -2418,17 +2416,17 @@ __global__ __launch_bounds__(WG) void k_color_pull(const uint32_t* __restrict__ 
 // their ancestors and hence their colour, so (partition, colour) refines the
 // partition; the colour's root id names the new partition
 __global__ void k_refine_pid(const uint32_t* __restrict__ U, const uint32_t* ucount, const uint32_t* __restrict__ live,
-                             const uint32_t* __restrict__ col, uint32_t r, uint32_t* __restrict__ pid) {
+                             const unsigned long long* __restrict__ cp, uint32_t r, uint32_t* __restrict__ pid) {
   uint32_t u = *ucount;
   for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < u; i += gridDim.x * WG) {
     uint32_t v = U[i];
-    if (bm_get(live, v)) pid[v] = rotl32(~col[v], r);
+    if (bm_get(live, v)) pid[v] = rotl32(~(uint32_t)cp[v], r);
   }
 }
 
 // roots: live v whose colour is its own key; decided with id v
 __global__ __launch_bounds__(WG) void k_color_roots(const uint32_t* __restrict__ U, const uint32_t* ucount, uint32_t* live,
-                                                    const uint32_t* __restrict__ col, uint32_t* __restrict__ labels,
+                                                    const unsigned long long* __restrict__ cp, uint32_t* __restrict__ labels,
                                                     uint32_t r, uint32_t* __restrict__ out, uint32_t* out_cnt) {
   __shared__ uint32_t s_out[8 * WG];
   __shared__ uint32_t s_on, s_ob;
@@ -2443,7 +2441,7 @@ __global__ __launch_bounds__(WG) void k_color_roots(const uint32_t* __restrict__
     uint32_t v = 0;
     if (i < u) {
       v = U[i];
-      if (col[v] == color_key(v, r) && bm_unclaim(live, v)) {
+      if ((uint32_t)cp[v] == color_key(v, r) && bm_unclaim(live, v)) {
         labels[v] = v;
         root = true;
       }
@@ -3653,7 +3651,7 @@ int ctx_reserve(Ctx* c, uint32_t n, uint64_t m) {
       (void)hipFree(*p);
       *p = nullptr;
     }
-  for (void** p : {(void**)&c->pbest, (void**)&c->vscore, (void**)&c->vfeat, (void**)&c->sub_off})
+  for (void** p : {(void**)&c->pbest, (void**)&c->vscore, (void**)&c->vfeat, (void**)&c->sub_off, (void**)&c->cpw})
     if (*p) {
       (void)hipFree(*p);
       *p = nullptr;
@@ -4627,25 +4625,29 @@ int color_round(Run& R, uint32_t* decided) {
   uint32_t r = ++R.rotation;
   // keys on live vertices, 0 elsewhere: a full pass the first time, then
   // only over the previous round's live list (everything else is already 0)
+  const uint32_t* pid = R.pid_active ? c->pid : nullptr;
+  if (!c->cpw) {
+    KSPN_TRY(dalloc(&c->cpw, (uint64_t)c->cap_n + 64));
+    R.col_valid = false;
+  }
   if (!R.col_valid) {
-    Prof p(c, R.s, K_COLOR_INIT, 4.0 * R.n + R.n / 8.0);
-    k_color_init_all<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->live, c->col, r);
+    Prof p(c, R.s, K_COLOR_INIT, 8.0 * R.n + R.n / 8.0);
+    k_color_init_all<<<stream_grid(c, R.n), WG, 0, R.s>>>(R.n, c->live, pid, c->cpw, r);
   } else if (R.prev_u) {
-    Prof p(c, R.s, K_COLOR_INIT, 8.0 * R.prev_u);
-    k_color_init_list<<<stream_grid(c, R.prev_u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], c->live, c->col, r);
+    Prof p(c, R.s, K_COLOR_INIT, 16.0 * R.prev_u);
+    k_color_init_list<<<stream_grid(c, R.prev_u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], c->live, pid, c->cpw, r);
   }
   uint32_t u = 0;
   KSPN_TRY(build_live_list(c, R.n, c->live, c->ulist, &u, R.s));
   k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_RES_N], &c->cnt[C_AUX]);
   R.prev_u = u;
-  const uint32_t* pid = R.pid_active ? c->pid : nullptr;
-  OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, r, nullptr, nullptr, pid};
+  OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, r, nullptr, nullptr, pid, nullptr, 0u, c->cpw};
   // forward: pull sweep over all live vertices
   KSPN_HIP(hipMemsetAsync(&c->cnt[C_Q0], 0, sizeof(uint32_t), R.s));
   {
     Prof p(c, R.s, K_COLOR_PULL, 24.0 * u);
-    k_color_pull<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->col, c->qa,
-                                                    &c->cnt[C_Q0], pid);
+    k_color_pull<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->cpw, c->qa,
+                                                    &c->cnt[C_Q0]);
   }
   KSPN_TRY(read_counters(c, R.s, 1));
   uint32_t f = c->h_cnt[C_Q0];
@@ -4660,8 +4662,8 @@ int color_round(Run& R, uint32_t* decided) {
     if ((double)f * 4.0 > (double)u) {
       // large frontier: another pull sweep over every live vertex
       Prof p(c, R.s, K_COLOR_PULL, 24.0 * u);
-      k_color_pull<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->col, N,
-                                                      &c->cnt[nc], pid);
+      k_color_pull<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->cpw, N,
+                                                      &c->cnt[nc]);
     } else {
       uint64_t e = 0;
       KSPN_TRY(expand<OP_COLOR>(R, F, &c->cnt[fc], f, g->fw_off, g->fw_adj, a, N, &c->cnt[nc], K_EXPAND_COLOR, 8.0, &e));
@@ -4680,7 +4682,7 @@ int color_round(Run& R, uint32_t* decided) {
   KSPN_HIP(hipMemsetAsync(&c->cnt[C_DTOT], 0, sizeof(uint32_t), R.s));
   {
     Prof p(c, R.s, K_COLOR_ROOTS, 12.0 * u);
-    k_color_roots<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], c->live, c->col, R.labels, r, D,
+    k_color_roots<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], c->live, c->cpw, R.labels, r, D,
                                                      &c->cnt[C_DTOT]);
   }
   KSPN_TRY(read_counters(c, R.s, C_DTOT + 1));
@@ -4699,7 +4701,7 @@ int color_round(Run& R, uint32_t* decided) {
   const uint32_t total = seg_end;
   if (pid) {
     Prof p(c, R.s, K_COLOR_INIT, 12.0 * u);
-    k_refine_pid<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], c->live, c->col, r, c->pid);
+    k_refine_pid<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], c->live, c->cpw, r, c->pid);
   }
   // canonical ids: every decision of this round carries its root's id;
   // replace it by the smallest member (col[root] is the min table)
@@ -5495,7 +5497,7 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   if (c->actl) (void)hipFree(c->actl);
   void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->sbm, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
                   c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf, c->pid, c->pbest,
-                  c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt, c->vprio, c->ds_st};
+                  c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt, c->vprio, c->ds_st, c->deg8, c->cpw};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
