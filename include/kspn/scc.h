@@ -97,6 +97,11 @@ typedef struct kspn_graph {
 /* Those push levels in one persistent launch instead, with a grid barrier (agent-scope release and acquire)
  * between levels; coop_workgroups must be resident together (capped at what the device holds). Same labels. */
 #define KSPN_FLAG_PERSISTENT 1024u
+/* Push levels whose frontier has at most 2^19 out-edges, none of its vertices more than 64, run on the device
+ * as chains of chain_levels launches (0 -> 8) with no degree scan: eight lanes read one row, each lane claims
+ * eight edges together. A larger frontier, a vertex of higher degree, the pull switch or an empty frontier
+ * ends a chain. Taken before the chains of KSPN_FLAG_CHAIN / KSPN_FLAG_PERSISTENT. Same labels. */
+#define KSPN_FLAG_NARROW 2048u
 
 typedef struct kspn_opts {
   uint32_t method;             /* KSPN_METHOD_* */

@@ -46,6 +46,7 @@ FLAG_GRAPH = 128
 FLAG_ASYNC = 256
 FLAG_CHAIN = 512
 FLAG_PERSISTENT = 1024
+FLAG_NARROW = 2048
 PHASE_MAX = 12
 MANIFEST_TEXT = 256
 MANIFEST_PATH = 4096

@@ -31,9 +31,9 @@ TUNED_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned")
 # candidate values per knob, in the order they are tried
 SPACE = {
     # KSPN_FLAG_FULL_BW (64): three-way split of the multi-pivot rounds; KSPN_FLAG_PERSISTENT (1024): device
-    # push levels in one persistent launch instead of chained launches. First: they change what the
-    # frontier bounds below mean.
-    "flags": [0, 64, 1024, 1088],
+    # push levels in one persistent launch instead of chained launches; KSPN_FLAG_NARROW (2048): chained
+    # narrow levels for frontiers of degree <= 64. First: they change what the frontier bounds below mean.
+    "flags": [0, 64, 1024, 1088, 2048, 2112],
     "fwbw_rounds": [1, 4, 16, 24, 32, 48, 64],
     "alpha": [14.0, 4.0, 40.0, 1e-9],
     "beta": [24.0, 6.0, 96.0],
@@ -49,10 +49,13 @@ SHAPES = {
     "expand_ept": [0, 8, 4, 2, 16],
     "sweep_wgs": [0, 4, 16],
     "list_wgs": [0, 2, 8],
+    # launches per chain of device push levels (0: the library's default, 16, or 8 for narrow levels)
+    "chain_levels": [0, 4, 8, 16],
 }
 SPACE.update(SHAPES)
-DEFAULTS = {"pull_ilp": 2, "expand_ept": 0, "sweep_wgs": 0, "list_wgs": 0, "alpha": 14.0, "beta": 24.0, "coop_edges": 1 << 20, "coop_workgroups": 128, "fwbw_rounds": 1,
-            "tail_threshold": 4096, "method": 0, "flags": 0}
+DEFAULTS = {"pull_ilp": 2, "expand_ept": 0, "sweep_wgs": 0, "list_wgs": 0, "chain_levels": 0, "alpha": 14.0, "beta": 24.0,
+            "coop_edges": 1 << 20, "coop_workgroups": 128, "fwbw_rounds": 1, "tail_threshold": 4096, "method": 0,
+            "flags": 0}
 
 
 def _solver(target):
@@ -142,7 +145,7 @@ def load(name):
 def main():
     """python -m kaspan_amd.autotune --config NAME: tune the plain rule (and, on the planted workload,
     the learned one) on the GPU and store the result in kaspan_amd/tuned/NAME.json, which bench.py
-    reads."""
+    reads. Each rule starts from its stored options (--fresh: from the library defaults)."""
     import argparse
     import time
 
@@ -154,7 +157,9 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--passes", type=int, default=1)
     ap.add_argument("--learned", action="store_true", help="also tune the learned pivot rule")
+    ap.add_argument("--fresh", action="store_true", help="start from the library defaults, not the stored options")
     args = ap.parse_args()
+    stored = None if args.fresh else load(args.config)
     t0 = time.time()
     wl = Workload(ka, args.config)
     target = wl.batch if wl.batch is not None else wl.g
@@ -162,14 +167,16 @@ def main():
     space = dict(SPACE)
     if wl.batch is not None:
         space.pop("method")  # the batch runs the multi-pivot method
-    best, best_ms, trials = tune(target, space=space, reps=args.reps, passes=args.passes, log=log)
+    start = dict(DEFAULTS, **stored["opts"]) if stored else None
+    best, best_ms, trials = tune(target, space=space, start=start, reps=args.reps, passes=args.passes, log=log)
     learned, learned_ms = None, None
     if args.learned:
         mlp = P.load_params()
         if mlp is None:
             raise SystemExit("no trained scorer (python -m kaspan_amd.train_pivot)")
         lspace = {k: v for k, v in space.items() if k != "method"}
-        learned, learned_ms, ltrials = tune(target, space=lspace, start=dict(best), reps=args.reps,
+        lstart = dict(DEFAULTS, **stored["learned_opts"]) if stored and stored.get("learned_opts") else dict(best)
+        learned, learned_ms, ltrials = tune(target, space=lspace, start=lstart, reps=args.reps,
                                             passes=args.passes, log=log, fixed={"pivot": 1, "mlp_params": mlp})
         trials += ltrials
     path = save(args.config, best, ms=best_ms, learned_opts=learned,

@@ -392,13 +392,44 @@ __device__ inline void expand_apply(uint32_t u, uint32_t w, const OpArgs& a, Blo
   bq_push(q, push, w);
 }
 
+// A wavefront-private LDS queue of claimed vertices: entries are staged
+// without block barriers, so a wavefront may loop on its own trip count.
+struct WaveQueue {
+  uint32_t* buf;  // __shared__, this wavefront's entries
+  uint32_t n;     // wave-uniform
+};
+
+__device__ inline void bq_push(WaveQueue& q, bool pred, uint32_t val) {
+  const uint64_t mask = __ballot(pred);
+  if (pred) {
+    const uint32_t lane = lane_id();
+    const uint64_t below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
+    q.buf[q.n + __popcll(below)] = val;
+  }
+  q.n += (uint32_t)__popcll(mask);
+}
+
+// the staged entries to queue[*count ...], one atomic per flush
+__device__ inline void wq_flush(WaveQueue& q, uint32_t* queue, uint32_t* count) {
+  if (q.n == 0) return;
+  const uint32_t lane = lane_id();
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(count, q.n);
+  base = __shfl(base, 0);
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t k = lane; k < q.n; k += WAVE) queue[base + k] = q.buf[k];
+  __builtin_amdgcn_wave_barrier();
+  q.n = 0;
+}
+
 // The reach claims of a thread's K edges u[r] -> x[r] (ok[r]: the edge
 // exists), with the loads of each step issued for all K together: candidate
 // and visited words, partitions, the claiming atomics, the claimed
 // vertices' degrees. Same claims as expand_apply on each edge in turn.
-template <int OP, int K>
+// Claimed vertices go to q (a BlockQueue or a WaveQueue).
+template <int OP, int K, class Q>
 __device__ inline void reach_batch(const uint32_t (&u)[K], const uint32_t (&x)[K], const bool (&ok)[K],
-                                   const OpArgs& a, BlockQueue& q, uint64_t& nsum, uint32_t& nmax) {
+                                   const OpArgs& a, Q& q, uint64_t& nsum, uint32_t& nmax) {
   const uint32_t* cand = OP == OP_BW ? a.fwv : (OP == OP_PBW ? a.pcand : a.live);
   uint32_t* vis = OP == OP_FW ? a.fwv : a.bwv;
   const bool part = OP == OP_PBW || (OP == OP_FW && a.part);
@@ -605,7 +636,7 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
           }
         }
         if constexpr (OP == OP_COLOR || OP == OP_COLOR_BW) color_batch<OP, EPT>(uu, xx, ok, a, q);
-        else reach_batch<OP, EPT>(uu, xx, ok, a, q, nsum, nmax);
+        else reach_batch<OP, EPT, BlockQueue>(uu, xx, ok, a, q, nsum, nmax);
       } else {
         for (uint64_t e = e0 + threadIdx.x; e < e1; e += WG) {
           uint32_t hi = w;
@@ -1035,6 +1066,105 @@ __global__ void k_chain_end(CoopCtl* ctl, uint32_t len) {
     ctl->levels = len;
     ctl->expanded += ctl->lexp[len % 3];
     ctl->verts += ctl->lvert[len % 3];
+  }
+}
+
+// Narrow push levels: frontiers whose vertices have at most NARROW_DEG
+// out-edges each (planted-SCC graphs, lattices), chained like
+// k_chain_level but with no degree scan and no edge-balanced tiling. A
+// wavefront takes 64 frontier vertices, lane L loading the row of vertex
+// L. Lanes form eight groups of eight; in each step lane k of group g
+// takes edge e0 + k of the rows of vertices g, g + 8, ..., g + 56, so
+// eight lanes read eight consecutive entries of one row, and each lane has
+// eight claims in flight together (reach_batch). A wavefront steps to the
+// largest degree among its 64 rows. Claims are staged in the wavefront's
+// LDS queue, one global atomic per flush; the next frontier's degree sum
+// and maximum take one atomic each per wavefront. The chain ends, as
+// k_chain_level's does, at an empty frontier, at the pull switch, or at a
+// frontier holding a vertex of degree above NARROW_DEG (the host expands
+// that level edge-balanced), and at a frontier of more than e_max out-edges
+// (NARROW_EDGES: the edge-balanced expansion is faster on larger frontiers;
+// planted 5e7 was fastest at 2^19 among 2^16 .. 2^20).
+constexpr uint32_t NARROW_DEG = 64;
+constexpr uint64_t NARROW_EDGES = 1ull << 19;
+constexpr int NARROW_WG = 256;
+constexpr uint32_t NARROW_QCAP = 1024;  // staged claims per wavefront (a step adds at most 512)
+constexpr uint32_t kNarrowChain = 8;    // launches per chain (opts.chain_levels overrides)
+
+template <int OP>
+__global__ __launch_bounds__(NARROW_WG) void k_narrow_level(uint32_t* F0, uint32_t* F1,
+                                                            const uint64_t* __restrict__ off,
+                                                            const uint32_t* __restrict__ adj, OpArgs a,
+                                                            CoopCtl* ctl, float alpha, unsigned long long e_max,
+                                                            uint32_t L) {
+  __shared__ uint32_t s_q[NARROW_WG / WAVE][NARROW_QCAP];
+  __shared__ CoopLevel s_lv;
+  __shared__ double s_m;
+  if (threadIdx.x == 0) {
+    const uint32_t ended = ctl->reason;  // set by an earlier launch of the chain
+    coop_read_level(ctl, L % 3, &s_lv);
+    s_lv.err = (ended || s_lv.err) ? 1u : 0u;
+    s_m = ctl->mrest[L % 3];
+  }
+  __syncthreads();
+  if (s_lv.err) return;
+  const uint32_t s1 = (L + 1) % 3, s2 = (L + 2) % 3;
+  const uint32_t f = s_lv.f;
+  const unsigned long long fe = s_lv.fe;
+  const uint32_t reason = coop_reason(f, fe, s_lv.fmax, s_m, alpha, e_max, NARROW_DEG);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (reason) {
+      ctl->levels = L;
+      ctl->reason = reason;
+    } else {
+      ctl->expanded += fe;
+      ctl->verts += f;
+      ctl->mrest[s1] = s_m - (double)fe;
+      coop_clear_slot(ctl, s2);
+    }
+  }
+  if (reason) return;
+  const uint32_t* F = (L & 1) ? F1 : F0;
+  uint32_t* N = (L & 1) ? F0 : F1;
+  const uint32_t wid = threadIdx.x / WAVE, lane = lane_id(), k = lane & 7, grp = lane >> 3;
+  WaveQueue q{s_q[wid], 0};
+  uint64_t nsum = 0;
+  uint32_t nmax = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * NARROW_WG;
+  for (uint64_t wb = (uint64_t)blockIdx.x * NARROW_WG + (uint64_t)wid * WAVE; wb < f; wb += stride) {
+    const uint64_t i = wb + lane;
+    uint32_t v = 0, d = 0;
+    uint64_t b = 0;
+    if (i < f) {
+      v = F[i];
+      b = off[v];
+      d = (uint32_t)min(off[v + 1] - b, (uint64_t)0xFFFFFFFFu);
+    }
+    uint32_t dm = d;
+    for (int o = 32; o > 0; o >>= 1) dm = max(dm, (uint32_t)__shfl_xor(dm, o));
+    for (uint32_t e0 = 0; e0 < dm; e0 += 8) {
+      uint32_t uu[8], xx[8];
+      bool ok[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int src = grp + 8 * r;
+        const uint32_t vr = __shfl(v, src), dr = __shfl(d, src);
+        const uint64_t br = __shfl((unsigned long long)b, src);
+        const uint32_t e = e0 + k;
+        ok[r] = e < dr;
+        uu[r] = vr;
+        xx[r] = ok[r] ? adj[br + e] : 0u;
+      }
+      if (q.n > NARROW_QCAP - 8 * WAVE) wq_flush(q, N, &ctl->cnt[s1]);
+      reach_batch<OP, 8, WaveQueue>(uu, xx, ok, a, q, nsum, nmax);
+    }
+  }
+  wq_flush(q, N, &ctl->cnt[s1]);
+  nsum = wave_sum64(nsum);
+  for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, (uint32_t)__shfl_xor(nmax, o));
+  if (lane == 0) {
+    if (nsum) atomicAdd(&ctl->nsum[s1], (unsigned long long)nsum);
+    if (nmax) atomicMax(&ctl->maxd[s1], nmax);
   }
 }
 
@@ -3785,6 +3915,7 @@ struct Run {
   uint32_t pull_ilp = PULL_ILP;       // launch shapes (kspn_opts)
   uint32_t ept = EXP_EPT;
   uint32_t chain = 0;                 // KSPN_FLAG_CHAIN: launches per chain of push levels (0: persistent kernel)
+  uint32_t narrow_len = 0;            // KSPN_FLAG_NARROW: launches per chain of narrow levels (0: off)
   bool dead_pid = false;              // pid holds DEAD_PID for every vertex that is not live (multi-pivot rounds)
   bool pivot_live_deg = false;        // pivots by live degree product (after an exhaustive trim)
   // the first trim's decided count and pivot are still on the device
@@ -3893,12 +4024,28 @@ struct CoopOut {
 // (KSPN_FLAG_CHAIN: up to R.chain levels as launches of their own, enqueued
 // together, then one publication)
 int run_coop(Run& R, int op, const OpArgs& a, const uint64_t* off, const uint32_t* adj, uint32_t* F, uint32_t* N,
-             int fc, int nc, uint64_t edges, uint32_t maxd, double m_rest, double alpha, CoopOut* out) {
+             int fc, int nc, uint64_t edges, uint32_t maxd, double m_rest, double alpha, CoopOut* out,
+             bool narrow = false) {
   Ctx* c = R.c;
   k_coop_init<<<1, 64, 0, R.s>>>(c->coop, &c->cnt[fc], edges, maxd, m_rest);
   KSPN_HIP(hipGetLastError());
   const float al = (float)alpha;
-  if (R.chain) {
+  if (narrow) {
+    // narrow levels (every frontier degree <= NARROW_DEG): a chain of
+    // R.narrow_len launches, enough workgroups for a few per CU
+    Prof p(c, R.s, K_COOP, 0.0);
+    const uint32_t grid = (uint32_t)c->num_cu * 8;
+    for (uint32_t L = 0; L < R.narrow_len; ++L) {
+      if (op == OP_FW)
+        k_narrow_level<OP_FW><<<grid, NARROW_WG, 0, R.s>>>(F, N, off, adj, a, c->coop, al, NARROW_EDGES, L);
+      else if (op == OP_BW)
+        k_narrow_level<OP_BW><<<grid, NARROW_WG, 0, R.s>>>(F, N, off, adj, a, c->coop, al, NARROW_EDGES, L);
+      else
+        k_narrow_level<OP_PBW><<<grid, NARROW_WG, 0, R.s>>>(F, N, off, adj, a, c->coop, al, NARROW_EDGES, L);
+    }
+    k_chain_end<<<1, 64, 0, R.s>>>(c->coop, R.narrow_len);
+    KSPN_HIP(hipGetLastError());
+  } else if (R.chain) {
     Prof p(c, R.s, K_COOP, 0.0);
     const dim3 grid(R.coop_wg), block(COOP_WG);
     for (uint32_t L = 0; L < R.chain; ++L) {
@@ -4162,10 +4309,13 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       c->pub_seq = spec_seq - 1;  // the gated publication did not happen
       spec_seq = 0;
     }
-    if (!pull && known && coop && edges <= R.coop_edges && maxd <= R.coop_deg) {
-      // the following push levels in one persistent launch
+    const bool narrow = !pull && known && R.narrow_len && maxd <= NARROW_DEG && edges <= NARROW_EDGES &&
+                        !(R.o.flags & KSPN_FLAG_NO_COOP);
+    if (narrow || (!pull && known && coop && edges <= R.coop_edges && maxd <= R.coop_deg)) {
+      // the following push levels on the device: narrow chained launches,
+      // chained launches or one persistent launch
       CoopOut co{};
-      KSPN_TRY(run_coop(R, op, a, off, adj, F, N, fc, nc, edges, maxd, m_rest, alpha, &co));
+      KSPN_TRY(run_coop(R, op, a, off, adj, F, N, fc, nc, edges, maxd, m_rest, alpha, &co, narrow));
       if (co.levels & 1) {
         std::swap(F, N);
         std::swap(fc, nc);
@@ -5565,6 +5715,7 @@ int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* o
     R.chain = o.chain_levels ? o.chain_levels : kChainLevels;
     R.coop_wg = o.coop_workgroups ? o.coop_workgroups : kCoopWorkgroups;
   }
+  if (o.flags & KSPN_FLAG_NARROW) R.narrow_len = o.chain_levels ? o.chain_levels : kNarrowChain;
   if (vtx_off) {
     // graph i is preset partition i of the multi-pivot FW-BW rounds
     if (!c->pid) KSPN_TRY(dalloc(&c->pid, (uint64_t)c->cap_n + 64));
