@@ -41,7 +41,7 @@ ROCPROF_NAMES = {"pull_fw": ["kspn::k_pull<0,"], "pull_bw": ["kspn::k_pull<1,"],
                  "expand_color": ["kspn::k_expand<3,"], "expand_color_bw": ["kspn::k_expand<4,"],
                  "label_scc": ["kspn::k_label_bits", "kspn::k_label_prep", "kspn::k_part_apply", "kspn::k_part_label"],
                  "tail_solve": ["kspn::k_tail_solve"], "trim_sweep": ["kspn::k_trim_sweep<"],
-                 "reach_coop": ["kspn::k_reach_coop<", "kspn::k_chain_level<"],
+                 "reach_coop": ["kspn::k_reach_coop<", "kspn::k_chain_level<", "kspn::k_narrow_level<"],
                  "bitmap_ops": ["kspn::k_part_split", "kspn::k_part_classify"],
                  "pivot_reduce": ["kspn::k_part_key", "kspn::k_part_prio"],
                  "frontier_deg": ["kspn::k_deg_scan", "kspn::k_frontier_deg"], "color_pull": ["kspn::k_color_pull"],

@@ -13,9 +13,10 @@ them. The scorer is trained on pivot outcomes the rounds themselves log
 (opts.pivot_log): by default log1p of the SCC each pivot decided, with
 pivots drawn at random among the same candidates; optionally the SCC a
 FW-BW from every vertex decides (one solve gives all) or the partition's
-shrinkage. Forward (BF16 MFMA), gradient (FP32) and
-Adam run as libkspn kernels on the GPU; inputs and target are standardised
-during training and the scaling is folded back into W1, b1, w3 and b3.
+shrinkage. Forward and gradient (both BF16 MFMA with FP32 accumulation)
+and Adam run as libkspn kernels on the GPU; inputs and target are
+standardised during training and the scaling is folded back into W1, b1, w3
+and b3.
 """
 from __future__ import annotations
 
@@ -67,7 +68,8 @@ def forward(params: torch.Tensor, x: torch.Tensor, ctx: Context | None = None) -
 
 
 def grad(params: torch.Tensor, x: torch.Tensor, target: torch.Tensor, ctx: Context | None = None):
-    """(loss, gradient) of 0.5*mean((MLP(x) - target)^2), FP32 kernels."""
+    """(loss, gradient) of 0.5*mean((MLP(x) - target)^2): matrix products on MFMA with BF16 operands and
+    FP32 accumulation (the sum of squared errors and the gradient come back in FP32)."""
     ctx = ctx or Context.default()
     g = torch.empty(N_PARAMS, dtype=torch.float32, device="cuda")
     loss = torch.empty(1, dtype=torch.float32, device="cuda")
