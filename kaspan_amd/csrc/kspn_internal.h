@@ -384,6 +384,8 @@ void graph_release(Ctx* c);
 int select_pivot_learned_impl(Ctx* c, const kspn_graph* g, const float* params, hipStream_t s, uint32_t* pivot);
 int mlp_features(Ctx* c, const kspn_graph* g, const uint32_t* list, uint32_t count, float* feat, hipStream_t s);
 int mlp_forward(Ctx* c, const float* params, const float* x, uint32_t rows, float* out, hipStream_t s);
+int mlp_score_top(Ctx* c, const kspn_graph* g, const float* params, uint32_t live_n, uint32_t div, uint32_t* list,
+                  uint32_t* count_dev, float* feat, float* sc, float* vscore, hipStream_t s, uint32_t* scored);
 
 }  // namespace kspn
 

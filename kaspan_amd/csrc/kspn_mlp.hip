@@ -497,6 +497,17 @@ __global__ __launch_bounds__(1024) void k_argmax_score(uint32_t count, const uin
   if (threadIdx.x == 0) *out = s_v[0];
 }
 
+__global__ void k_fill_score(uint32_t n, float v, float* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG) out[i] = v;
+}
+
+__global__ void k_scatter_score(const uint32_t* __restrict__ list, const uint32_t* count, const float* __restrict__ sc,
+                                float* __restrict__ out) {
+  const uint32_t cnt = *count;
+  for (uint64_t i = blockIdx.x * (uint64_t)WG + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * WG)
+    out[list[i]] = sc[i];
+}
+
 // host helpers used by the pipeline (kspn_scc.hip)
 int mlp_features(Ctx* c, const kspn_graph* g, const uint32_t* list, uint32_t count, float* feat, hipStream_t s) {
   if (count == 0) return KSPN_OK;
@@ -529,6 +540,45 @@ int mlp_forward(Ctx* c, const float* params, const float* x, uint32_t rows, floa
 // Learned pivot among live vertices: candidates = live vertices in the top
 // log2 buckets of in*out (at least `want`, at most `cap`), scored by the
 // MLP; the best score wins (smallest id on ties). Returns n if none.
+// Scores of the multi-pivot rounds. The learned rule only breaks ties
+// among a partition's top half-octave of in*out, so a vertex of low in*out
+// rarely needs a score: the live vertices of the highest log2 buckets of
+// in*out, at least live_n / div of them, get the MLP score, every other
+// vertex -inf (a partition whose top candidates have no score takes the
+// smallest id among them, as the plain rule's ties do). `list`, `feat` and
+// `sc` hold up to n entries; *scored returns how many were scored.
+int mlp_score_top(Ctx* c, const kspn_graph* g, const float* params, uint32_t live_n, uint32_t div, uint32_t* list,
+                  uint32_t* count_dev, float* feat, float* sc, float* vscore, hipStream_t s, uint32_t* scored) {
+  if (!c->mlp_buf) {
+    if (hipMalloc((void**)&c->mlp_buf, sizeof(uint32_t) * (4096 + 128 + 4096 * 6 + 4096)) != hipSuccess) return KSPN_E_OOM;
+  }
+  uint32_t* hist = c->mlp_buf + 4096;
+  KSPN_HIP(hipMemsetAsync(hist, 0, 65 * sizeof(uint32_t), s));
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)g->n + WG - 1) / WG, (uint64_t)c->num_cu * 16);
+  k_prod_hist<<<grid, WG, 0, s>>>(g->n, c->live, g->fw_off, g->bw_off, hist);
+  uint32_t h[65];
+  KSPN_HIP(hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, s));
+  KSPN_HIP(hipStreamSynchronize(s));
+  const uint64_t want = std::max<uint64_t>(1, live_n / std::max<uint32_t>(div, 1u));
+  uint32_t b_min = 65;
+  uint64_t acc = 0;
+  while (b_min > 0 && acc < want) acc += h[--b_min];
+  k_fill_score<<<grid, WG, 0, s>>>(g->n, -INFINITY, vscore);
+  KSPN_HIP(hipMemsetAsync(count_dev, 0, sizeof(uint32_t), s));
+  k_collect_cand<<<grid, WG, 0, s>>>(g->n, c->live, g->fw_off, g->bw_off, b_min, g->n, list, count_dev);
+  uint32_t nc = 0;
+  KSPN_HIP(hipMemcpyAsync(&nc, count_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  KSPN_HIP(hipStreamSynchronize(s));
+  nc = std::min(nc, g->n);
+  KSPN_TRY(mlp_features(c, g, list, nc, feat, s));
+  KSPN_TRY(mlp_forward(c, params, feat, nc, sc, s));
+  k_scatter_score<<<(uint32_t)std::min<uint64_t>(((uint64_t)nc + WG - 1) / WG + 1, (uint64_t)c->num_cu * 16), WG, 0, s>>>(
+      list, count_dev, sc, vscore);
+  KSPN_HIP(hipGetLastError());
+  *scored = nc;
+  return KSPN_OK;
+}
+
 int select_pivot_learned_impl(Ctx* c, const kspn_graph* g, const float* params, hipStream_t s, uint32_t* pivot) {
   const uint32_t want = 1024, cap = 4096;
   if (!c->mlp_buf) {

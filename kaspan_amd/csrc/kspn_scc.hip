@@ -4537,6 +4537,9 @@ int part_compact(Run& R, uint32_t S) {
   return KSPN_OK;
 }
 
+// live vertices scored for the learned rule: at least live / kScoreTopDiv, those of highest in*out
+constexpr uint32_t kScoreTopDiv = 8;
+
 int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
   Ctx* c = R.c;
   const kspn_graph* g = R.g;
@@ -4554,18 +4557,19 @@ int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
   R.dead_pid = n <= DEAD_PID_MAX_N;
   const float* score = nullptr;
   if (R.o.pivot == KSPN_PIVOT_LEARNED) {
-    // one MLP score per live vertex for the whole solve
-    uint32_t u = 0;
-    KSPN_TRY(build_live_list(c, R.n, c->live, c->ulist, &u, R.s));
+    // MLP scores, fixed for the whole solve, of the live vertices with the
+    // highest in*out (the only ones likely to be a partition's candidates)
     if (!c->vscore) KSPN_TRY(dalloc(&c->vscore, (uint64_t)c->cap_n + 64));
     if (!c->vfeat) KSPN_TRY(dalloc(&c->vfeat, 7ull * c->cap_n + 64));
     float* feat = c->vfeat;
     float* sc = c->vfeat + 6ull * c->cap_n;
     {
-      Prof p(c, R.s, K_PIVOT, 24.0 * u);
-      KSPN_TRY(mlp_features(c, g, c->ulist, u, feat, R.s));
-      KSPN_TRY(mlp_forward(c, R.o.mlp_params, feat, u, sc, R.s));
-      k_scatter_f32<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_AUX], sc, c->vscore);
+      Prof p(c, R.s, K_PIVOT, 0.0);
+      uint32_t scored = 0;
+      KSPN_TRY(mlp_score_top(c, g, R.o.mlp_params, R.live_n, kScoreTopDiv, c->ulist, &c->cnt[C_AUX], feat, sc,
+                             c->vscore, R.s, &scored));
+      // histogram and list passes over the vertices, features and scores of the scored ones
+      p.add_bytes(36.0 * R.n + 64.0 * scored);
     }
     score = c->vscore;
   }
@@ -4809,8 +4813,9 @@ int color_round(Run& R, uint32_t* decided) {
     // drop F's queue bits so its members can be queued again
     k_bm_clear_list<<<stream_grid(c, f), WG, 0, R.s>>>(c->inq, F, &c->cnt[fc]);
     KSPN_HIP(hipMemsetAsync(&c->cnt[nc], 0, sizeof(uint32_t), R.s));
-    if ((double)f * 4.0 > (double)u) {
-      // large frontier: another pull sweep over every live vertex
+    if ((double)f * 2.0 > (double)u) {
+      // large frontier: another pull sweep over every live vertex (half the
+      // live list was faster than a quarter or all of it on planted 5e7)
       Prof p(c, R.s, K_COLOR_PULL, 24.0 * u);
       k_color_pull<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->cpw, N,
                                                       &c->cnt[nc]);
