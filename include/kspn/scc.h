@@ -126,6 +126,8 @@ typedef struct kspn_opts {
   uint32_t sweep_wgs;          /* pull sweep: workgroups per CU (0 -> 8, 16 above 2^25 vertices) */
   uint32_t list_wgs;           /* pull list levels: workgroups per CU at most (0 -> 4) */
   uint32_t chain_levels;       /* KSPN_FLAG_CHAIN: push levels enqueued per host round trip (0 -> 16, at most 256) */
+  uint32_t learned_rounds;     /* KSPN_PIVOT_LEARNED, multi-pivot rounds: the first learned_rounds rounds take the
+                                  learned pivot, the later ones the plain rule (0 -> every round) */
 } kspn_opts;
 
 /* pivot outcome record: pivot, round, partition size, SCC size (decided),

@@ -240,7 +240,7 @@ def planted_labels(seed, n, device="cuda", ctx: Context | None = None, stream=No
 
 def _opts(method=METHOD_KASPAN, pivot=0, fwbw_rounds=1, residual_threshold=0, max_rounds=0, flags=0, alpha=14.0,
           beta=24.0, mlp_params=None, tail_threshold=0, pivot_log=None, seed=0, coop_edges=0, coop_degree=0,
-          coop_workgroups=0, pull_ilp=0, expand_ept=0, sweep_wgs=0, list_wgs=0, chain_levels=0):
+          coop_workgroups=0, pull_ilp=0, expand_ept=0, sweep_wgs=0, list_wgs=0, chain_levels=0, learned_rounds=0):
     o = _lib.Opts()
     _lib.lib().kspn_opts_default(C.byref(o))
     o.method, o.pivot, o.fwbw_rounds = method, pivot, fwbw_rounds
@@ -249,6 +249,7 @@ def _opts(method=METHOD_KASPAN, pivot=0, fwbw_rounds=1, residual_threshold=0, ma
     o.coop_edges, o.coop_degree, o.coop_workgroups = coop_edges, coop_degree, coop_workgroups
     o.pull_ilp, o.expand_ept, o.sweep_wgs, o.list_wgs = pull_ilp, expand_ept, sweep_wgs, list_wgs
     o.chain_levels = chain_levels
+    o.learned_rounds = learned_rounds
     if pivot_log is not None:
         o.pivot_log = pivot_log.data_ptr()
         o.pivot_log_cap = pivot_log.numel() // _lib.LOG_WORDS

@@ -133,7 +133,7 @@ class Opts(C.Structure):
         ("pull_ilp", C.c_uint32),
         ("expand_ept", C.c_uint32),
         ("sweep_wgs", C.c_uint32),
-        ("list_wgs", C.c_uint32), ("chain_levels", C.c_uint32),
+        ("list_wgs", C.c_uint32), ("chain_levels", C.c_uint32), ("learned_rounds", C.c_uint32),
     ]
 
 

@@ -55,7 +55,9 @@ SHAPES = {
 SPACE.update(SHAPES)
 DEFAULTS = {"pull_ilp": 2, "expand_ept": 0, "sweep_wgs": 0, "list_wgs": 0, "chain_levels": 0, "alpha": 14.0, "beta": 24.0,
             "coop_edges": 1 << 20, "coop_workgroups": 128, "fwbw_rounds": 1, "tail_threshold": 4096, "method": 0,
-            "flags": 0}
+            "flags": 0, "learned_rounds": 0}
+# the learned rule's schedule: rounds with the learned pivot before the plain rule takes over (0: all)
+LEARNED_SPACE = {"learned_rounds": [0, 2, 4, 6, 8, 12, 16], "fwbw_rounds": [20, 24, 28, 32, 40, 48]}
 
 
 def _solver(target):
@@ -175,6 +177,7 @@ def main():
         if mlp is None:
             raise SystemExit("no trained scorer (python -m kaspan_amd.train_pivot)")
         lspace = {k: v for k, v in space.items() if k != "method"}
+        lspace.update(LEARNED_SPACE)
         lstart = dict(DEFAULTS, **stored["learned_opts"]) if stored and stored.get("learned_opts") else dict(best)
         learned, learned_ms, ltrials = tune(target, space=lspace, start=lstart, reps=args.reps,
                                             passes=args.passes, log=log, fixed={"pivot": 1, "mlp_params": mlp})

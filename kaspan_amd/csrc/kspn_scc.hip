@@ -4613,6 +4613,11 @@ int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
     KSPN_TRY(reach(R, 1, np, c->pid, &lp, &lq));
     const uint32_t S_next = 3u * np;
     const bool more = round + 1 < K;
+    if (more && score && R.o.learned_rounds && round + 1 == R.o.learned_rounds) {
+      // the next round's keys (formed below) take the plain rule's priorities
+      Prof p(c, R.s, K_PIVOT, 36.0 * n);
+      k_part_prio<<<stream_grid(c, n), WG, 0, R.s>>>(R.n, g->fw_off, g->bw_off, nullptr, false, c->vprio);
+    }
     if (!pcount) {
       // classification, relabelling and the next round's keys in one pass,
       // then the labels of the decided vertices from their list

@@ -233,3 +233,22 @@ def test_pivot_log_records(ka, O):
     assert np.array_equal(lab.cpu().numpy(), planted)
     with pytest.raises(ka.KspnError):
         ka.scc(g, pivot=1)  # learned pivot without parameters
+
+
+def test_learned_then_plain_schedule(ka, piv):
+    """opts.learned_rounds: the first rounds take the learned pivot, the rest
+    the plain rule; planted labels for every schedule, and a schedule as long
+    as the rounds is the learned rule throughout (same FW-BW decisions)"""
+    params = piv.load_params()
+    seed = 0x7452414E00000000 + 2002
+    g = ka.generate("planted", seed, n=300_000, max_size=60_000)
+    planted = ka.planted_labels(seed, 300_000, max_size=60_000)
+    decided = {}
+    for lr in (0, 1, 3, 8):
+        lab, st = ka.scc(g, fwbw_rounds=8, pivot=1, mlp_params=params, learned_rounds=lr)
+        assert bool((lab == planted).all()), lr
+        decided[lr] = st.decided_fwbw
+    assert decided[0] == decided[8]
+    lab, st = ka.scc_batch(ka.generate_batch(0x6B5350AA00000004, 32, n_min=1000, n_max=20_000), fwbw_rounds=6,
+                           pivot=1, mlp_params=params, learned_rounds=2)
+    assert st.decided_trim + st.decided_trim2 + st.decided_fwbw + st.decided_color + st.decided_residual == lab.numel()
