@@ -550,16 +550,21 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
                                                const uint64_t* __restrict__ S, const uint64_t* __restrict__ off,
                                                const uint32_t* __restrict__ adj, OpArgs a, uint32_t* __restrict__ next,
                                                uint32_t* next_cnt) {
-  __shared__ uint64_t s_S[EXP_WIN];
-  __shared__ uint64_t s_beg[EXP_WIN];
+  // the window's entries: scan value relative to the tile start (0 for the
+  // first, whose edges may begin before it), row start minus scan value (so
+  // edge e of entry j is adj[s_base[j] + e]) and vertex; 16 bytes per entry
+  // and a one-tile queue keep the workgroup's LDS at 24 KB with EPT 8 (six
+  // workgroups per CU)
+  __shared__ int32_t s_S[EXP_WIN];
+  __shared__ uint64_t s_base[EXP_WIN];
   __shared__ uint32_t s_v[EXP_WIN];
   __shared__ uint32_t s_i0, s_w;
   constexpr int EXP_TILE = WG * EPT_;  // edges per workgroup tile
-  __shared__ uint32_t s_out[2 * EXP_TILE];
+  __shared__ uint32_t s_out[EXP_TILE];  // a tile claims at most one vertex per edge; flushed after every tile
   __shared__ uint32_t s_on, s_ob;
   __shared__ unsigned long long s_nsum;
   __shared__ uint32_t s_nmax;
-  BlockQueue q{s_out, &s_on, &s_ob, 2 * EXP_TILE};
+  BlockQueue q{s_out, &s_on, &s_ob, EXP_TILE};
   if (threadIdx.x == 0) {
     s_nsum = 0;
     s_nmax = 0;
@@ -610,8 +615,10 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
       for (uint32_t j = threadIdx.x; j < w; j += WG) {
         uint32_t v = F[i0 + j];
         s_v[j] = v;
-        s_S[j] = S[i0 + j];
-        s_beg[j] = off[v];
+        const uint64_t sj = S[i0 + j];
+        // entries after the first start inside the tile: 0 < sj - e0 < EXP_TILE
+        s_S[j] = j ? (int32_t)(sj - e0) : 0;
+        s_base[j] = off[v] - sj;
       }
       __syncthreads();
       uint32_t lo = 0;
@@ -626,26 +633,28 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
           ok[r] = e < e1;
           uu[r] = xx[r] = 0;
           if (ok[r]) {
+            const int32_t el = (int32_t)(e - e0);
             uint32_t hi = w;
             while (hi - lo > 1) {
               uint32_t mid = (lo + hi) >> 1;
-              if (s_S[mid] <= e) lo = mid; else hi = mid;
+              if (s_S[mid] <= el) lo = mid; else hi = mid;
             }
             uu[r] = s_v[lo];
-            xx[r] = adj[s_beg[lo] + (e - s_S[lo])];
+            xx[r] = adj[s_base[lo] + e];
           }
         }
         if constexpr (OP == OP_COLOR || OP == OP_COLOR_BW) color_batch<OP, EPT>(uu, xx, ok, a, q);
         else reach_batch<OP, EPT, BlockQueue>(uu, xx, ok, a, q, nsum, nmax);
       } else {
         for (uint64_t e = e0 + threadIdx.x; e < e1; e += WG) {
+          const int32_t el = (int32_t)(e - e0);
           uint32_t hi = w;
           while (hi - lo > 1) {
             uint32_t mid = (lo + hi) >> 1;
-            if (s_S[mid] <= e) lo = mid; else hi = mid;
+            if (s_S[mid] <= el) lo = mid; else hi = mid;
           }
           uint32_t u = s_v[lo];
-          uint32_t x = adj[s_beg[lo] + (e - s_S[lo])];
+          uint32_t x = adj[s_base[lo] + e];
           expand_apply<OP>(u, x, a, q, nsum, nmax);
         }
       }
