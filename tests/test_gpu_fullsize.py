@@ -66,7 +66,12 @@ def test_c3_planted50m_full(ka, O, release):
     assert np.array_equal(labels_np(ka.planted_labels(C3_SEED, C3["n"], max_size=C3["max_size"])), want)
     mlp = P.load_params()
     assert mlp is not None, "the trained scorer kaspan_amd/pivot_mlp_params.npy is part of the tree"
-    runs = [dict(fwbw_rounds=32), dict(fwbw_rounds=32, pivot=1, mlp_params=mlp), dict(fwbw_rounds=1)]
+    from kaspan_amd import autotune
+
+    tuned = autotune.load("planted50m")
+    runs = [dict(fwbw_rounds=32), dict(fwbw_rounds=32, pivot=1, mlp_params=mlp), dict(fwbw_rounds=1),
+            # the benchmark's options for both rules (narrow levels, the learned-then-plain schedule)
+            dict(tuned["opts"]), dict(tuned["learned_opts"], pivot=1, mlp_params=mlp)]
     decided = []
     for opts in runs:
         progress(f"C3: solve {opts.get('fwbw_rounds')} rounds, pivot {opts.get('pivot', 0)}")
@@ -89,8 +94,11 @@ def test_c4_lattice256_full(ka, O, release):
     progress("C4: oracle Tarjan done")
     mlp = P.load_params()
     coop_levels = []
+    from kaspan_amd import autotune
+
     for opts in (dict(), dict(flags=16), dict(pivot=1, mlp_params=mlp), dict(pivot=1, mlp_params=mlp, flags=16),
-                 dict(fwbw_rounds=16), dict(fwbw_rounds=16, pivot=1, mlp_params=mlp)):
+                 dict(fwbw_rounds=16), dict(fwbw_rounds=16, pivot=1, mlp_params=mlp),
+                 dict(autotune.load("lattice256")["opts"])):
         lab, st = ka.scc(g, **opts)
         assert np.array_equal(labels_np(lab), tar), {k: v for k, v in opts.items() if k != "mlp_params"}
         coop_levels.append(st.fwbw_levels_coop)
@@ -128,6 +136,12 @@ def test_c5_batch4096_full(ka, O, release):
     progress(f"C5: solving 4096 graphs (n={p.n}, m={p.m})")
     lab5, st5 = ka.scc_batch(c5)
     got5 = labels_np(lab5)
+    # the benchmark's tuned options give the same labels
+    from kaspan_amd import autotune
+
+    lab5t, _ = ka.scc_batch(c5, **autotune.load("batch4096")["opts"])
+    assert np.array_equal(labels_np(lab5t), got5)
+    del lab5t
     del lab5, c5, p
     progress(f"C5: solving {count} graphs (n={u.n}, m={u.m})")
     lab, st = ka.scc_batch(b)
