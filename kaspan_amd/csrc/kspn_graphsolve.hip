@@ -109,6 +109,39 @@ __device__ inline uint32_t wscan_incl(uint32_t v) {
   return v;
 }
 
+// Concurrent atomics on one address serialise (about 11 ns each), so counters
+// and flags every wavefront updates are combined per workgroup first: one
+// atomic per workgroup. Every thread of the workgroup calls these.
+__device__ inline void block_add(uint32_t v, uint32_t* dst) {
+  __shared__ uint32_t s_sum;
+  if (threadIdx.x == 0) s_sum = 0;
+  __syncthreads();
+  v = wave_sum(v);
+  if (lane_id() == 0 && v) atomicAdd(&s_sum, v);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_sum) atomicAdd(dst, s_sum);
+}
+__device__ inline void block_add2(uint32_t v, unsigned long long w, uint32_t* dst, unsigned long long* dst64) {
+  __shared__ uint32_t s_sum;
+  __shared__ unsigned long long s_sum64;
+  if (threadIdx.x == 0) {
+    s_sum = 0;
+    s_sum64 = 0;
+  }
+  __syncthreads();
+  v = wave_sum(v);
+  w = wave_sum64(w);
+  if (lane_id() == 0) {
+    if (v) atomicAdd(&s_sum, v);
+    if (w) atomicAdd(&s_sum64, w);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_sum) atomicAdd(dst, s_sum);
+    if (s_sum64) atomicAdd(dst64, s_sum64);
+  }
+}
+
 constexpr uint32_t HEAVY = 64;  // rows longer than this are scanned by the whole wavefront
 
 // The lanes with `heavy` set own rows [it, end) longer than HEAVY: each such
@@ -168,9 +201,15 @@ __global__ __launch_bounds__(WG) void k_g_init(uint32_t n, const uint64_t* __res
                                                unsigned long long* __restrict__ pc, unsigned long long* pkey,
                                                GCtl* ctl) {
   __shared__ uint32_t s_g0, s_g1;
+  __shared__ uint32_t s_rg[WG / WAVE];
+  __shared__ unsigned long long s_rm[WG / WAVE];
   const uint32_t lane = lane_id();
   const uint64_t words64 = ((uint64_t)n + 63) / 64;  // 64-vertex groups
   uint32_t dead = 0, alive = 0;
+  // the wavefront's running maximum key of one graph (wave-uniform), written
+  // out when the graph changes and, combined per workgroup, at the end
+  uint32_t run_g = 0xFFFFFFFFu;
+  unsigned long long run_m = 0;
   for (uint64_t gbase = (uint64_t)blockIdx.x * (WG / WAVE); gbase < words64; gbase += (uint64_t)gridDim.x * (WG / WAVE)) {
     // graphs covering this workgroup's 256 vertices
     const uint64_t first = gbase * 64, last = std::min<uint64_t>(first + WG, n) - 1;
@@ -215,19 +254,39 @@ __global__ __launch_bounds__(WG) void k_g_init(uint32_t n, const uint64_t* __res
           const unsigned long long t = __shfl_xor(m, o);
           m = t > m ? t : m;
         }
-        if (lane == 0 && m) atomicMax(&pkey[g0], m);
+        if (g0 != run_g) {
+          if (lane == 0 && run_m) atomicMax(&pkey[run_g], run_m);
+          run_g = g0;
+          run_m = 0;
+        }
+        run_m = m > run_m ? m : run_m;
       } else if (key) {
         atomicMax(&pkey[g], key);
       }
     }
     __syncthreads();
   }
-  dead = wave_sum(dead);
-  alive = wave_sum(alive);
+  const uint32_t wid = threadIdx.x / WAVE;
   if (lane == 0) {
-    if (dead) atomicAdd(&ctl->dec_trim, dead);
-    if (alive) atomicAdd(&ctl->live_after_trim, alive);
+    s_rg[wid] = run_m ? run_g : 0xFFFFFFFFu;
+    s_rm[wid] = run_m;
   }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // the waves' maxima, one atomic per distinct graph
+    for (uint32_t a = 0; a < WG / WAVE; ++a) {
+      if (s_rg[a] == 0xFFFFFFFFu) continue;
+      unsigned long long m = s_rm[a];
+      for (uint32_t b = a + 1; b < WG / WAVE; ++b)
+        if (s_rg[b] == s_rg[a]) {
+          m = s_rm[b] > m ? s_rm[b] : m;
+          s_rg[b] = 0xFFFFFFFFu;
+        }
+      atomicMax(&pkey[s_rg[a]], m);
+    }
+  }
+  block_add(dead, &ctl->dec_trim);
+  block_add(alive, &ctl->live_after_trim);
 }
 
 // appends the row slices of v (deg edges) to queue q: every lane with `take`
@@ -378,12 +437,10 @@ __global__ __launch_bounds__(GWG) void k_g_reach(uint32_t n, uint64_t m, const u
       // a binary search over the wavefront's scan in LDS
       uint64_t* sb = s_beg[wid];
       uint32_t* sx = s_ex[wid];
-      for (;;) {
-        uint32_t u0 = 0;
-        if (lane == 0) u0 = atomicAdd(&ctl->cur[s0], (uint32_t)WAVE);
-        u0 = __shfl(u0, 0);
-        if (u0 >= fn) break;
-        const uint32_t i = u0 + lane;
+      // 64-entry chunks in a fixed order (entries are row slices of at most
+      // SLICE edges, so chunks cost about the same)
+      for (uint64_t u0 = (uint64_t)gw * WAVE; u0 < fn; u0 += (uint64_t)waves * WAVE) {
+        const uint64_t i = u0 + lane;
         uint64_t b = 0;
         uint32_t cnt = 0;
         if (i < fn) {
@@ -512,12 +569,7 @@ __global__ __launch_bounds__(GWG) void k_g_reach(uint32_t n, uint64_t m, const u
       }
     }
     stage_flush(st, N, &ctl->qn[s1], qcap, &ctl->err);
-    claimed = wave_sum(claimed);
-    cdeg = wave_sum64(cdeg);
-    if (lane == 0 && claimed) {
-      atomicAdd(&ctl->qv[s1], claimed);
-      atomicAdd(&ctl->qe[s1], cdeg);
-    }
+    block_add2(claimed, cdeg, &ctl->qv[s1], &ctl->qe[s1]);
     if (!gbar(bar, &ctl->err, ++k)) break;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl->expanded[DIR] = expanded;
@@ -527,9 +579,12 @@ __global__ __launch_bounds__(GWG) void k_g_reach(uint32_t n, uint64_t m, const u
 __global__ __launch_bounds__(WG) void k_g_scc_min(uint32_t n, const uint32_t* __restrict__ fwv,
                                                   const uint32_t* __restrict__ bwv,
                                                   const unsigned long long* __restrict__ pc, uint32_t* gmin) {
-  // lanes of one graph take their minimum first: one atomic per wavefront and graph
+  // lanes of one graph take their minimum first, and a wavefront keeps a
+  // running minimum while its graph stays the same: one atomic per wavefront
+  // and graph change
   const uint64_t pad = ((uint64_t)n + WG - 1) / WG * WG;
   const uint32_t lane = lane_id();
+  uint32_t run_g = 0xFFFFFFFFu, run_m = 0xFFFFFFFFu;  // wave-uniform
   for (uint64_t v64 = blockIdx.x * (uint64_t)WG + threadIdx.x; v64 < pad; v64 += (uint64_t)gridDim.x * WG) {
     const uint32_t v = (uint32_t)v64, bit = 1u << (v & 31);
     bool in = v64 < n && (fwv[v >> 5] & bwv[v >> 5] & bit) != 0;
@@ -539,12 +594,18 @@ __global__ __launch_bounds__(WG) void k_g_scc_min(uint32_t n, const uint32_t* __
       const uint32_t leader = __ffsll((unsigned long long)pend) - 1;
       const uint32_t gl = __shfl(g, leader);
       const bool mine = in && g == gl;
-      const uint32_t m = wave_min(mine ? v : 0xFFFFFFFFu);  // valid in lane 0
-      if (lane == 0) atomicMin(&gmin[gl], m);
+      const uint32_t m = __shfl(wave_min(mine ? v : 0xFFFFFFFFu), 0);
+      if (gl != run_g) {
+        if (lane == 0 && run_m != 0xFFFFFFFFu) atomicMin(&gmin[run_g], run_m);
+        run_g = gl;
+        run_m = 0xFFFFFFFFu;
+      }
+      run_m = m < run_m ? m : run_m;
       pend &= ~__ballot(mine);
       in = in && !mine;
     }
   }
+  if (lane == 0 && run_m != 0xFFFFFFFFu) atomicMin(&gmin[run_g], run_m);
 }
 
 __global__ __launch_bounds__(WG) void k_g_scc_label(uint32_t n, const uint32_t* __restrict__ fwv,
@@ -573,8 +634,7 @@ __global__ __launch_bounds__(WG) void k_g_scc_label(uint32_t n, const uint32_t* 
     }
     dec += in;
   }
-  dec = wave_sum(dec);
-  if (lane == 0 && dec) atomicAdd(&ctl->dec_fwbw, dec);
+  block_add(dec, &ctl->dec_fwbw);
 }
 
 // Persistent colour rounds until no vertex is live (or max_rounds). pc[v]
@@ -693,7 +753,7 @@ __global__ __launch_bounds__(GWG) void k_g_colour(uint32_t n, const uint64_t* __
           ch = true;
         }
       }
-      if (__ballot(ch) && lane == 0) atomicOr(fl, 1u);
+      if (__syncthreads_or(ch) && threadIdx.x == 0) atomicOr(fl, 1u);
       if (!gbar(bar, &ctl->err, ++k)) return;
       const bool again = *fl != 0;
       ++fix;
@@ -747,7 +807,7 @@ __global__ __launch_bounds__(GWG) void k_g_colour(uint32_t n, const uint64_t* __
           ch = true;
         }
       }
-      if (__ballot(ch) && lane == 0) atomicOr(fl, 1u);
+      if (__syncthreads_or(ch) && threadIdx.x == 0) atomicOr(fl, 1u);
       if (!gbar(bar, &ctl->err, ++k)) return;
       const bool again = *fl != 0;
       ++fix;
@@ -773,8 +833,7 @@ __global__ __launch_bounds__(GWG) void k_g_colour(uint32_t n, const uint64_t* __
         pc[v] = (unsigned long long)root << 32;
       }
     }
-    d = wave_sum(d);
-    if (lane == 0 && d) atomicAdd(&ctl->dec_color, d);
+    block_add(d, &ctl->dec_color);
   }
 }
 
