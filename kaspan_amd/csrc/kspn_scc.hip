@@ -1169,11 +1169,27 @@ __global__ __launch_bounds__(NARROW_WG) void k_narrow_level(uint32_t* F0, uint32
     }
   }
   wq_flush(q, N, &ctl->cnt[s1]);
+  // the next frontier's degree sum and maximum: one atomic each per
+  // workgroup (the wavefronts of a launch finish together, and adds to one
+  // address from all of them would queue)
+  __shared__ unsigned long long s_sum[NARROW_WG / WAVE];
+  __shared__ uint32_t s_max[NARROW_WG / WAVE];
   nsum = wave_sum64(nsum);
   for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, (uint32_t)__shfl_xor(nmax, o));
   if (lane == 0) {
-    if (nsum) atomicAdd(&ctl->nsum[s1], (unsigned long long)nsum);
-    if (nmax) atomicMax(&ctl->maxd[s1], nmax);
+    s_sum[wid] = nsum;
+    s_max[wid] = nmax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long bs = 0;
+    uint32_t bm = 0;
+    for (int w = 0; w < NARROW_WG / WAVE; ++w) {
+      bs += s_sum[w];
+      bm = max(bm, s_max[w]);
+    }
+    if (bs) atomicAdd(&ctl->nsum[s1], bs);
+    if (bm) atomicMax(&ctl->maxd[s1], bm);
   }
 }
 
