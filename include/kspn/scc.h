@@ -102,6 +102,12 @@ typedef struct kspn_graph {
  * eight edges together. A larger frontier, a vertex of higher degree, the pull switch or an empty frontier
  * ends a chain. Taken before the chains of KSPN_FLAG_CHAIN / KSPN_FLAG_PERSISTENT. Same labels. */
 #define KSPN_FLAG_NARROW 2048u
+/* An edge that a forward push level of the multi-pivot rounds, a colour pull sweep or a colour push finds joining
+ * two partitions (or reaching a decided vertex) is marked in a bitmap of one bit per adjacency entry, and later
+ * levels and sweeps of the solve skip it before reading its other end's state (partitions only split and decided
+ * vertices stay decided, so such an edge never again joins two vertices of one partition). The context keeps
+ * m / 8 bytes per direction for it, cleared at the start of each solve. Same labels. */
+#define KSPN_FLAG_PRUNE 4096u
 
 typedef struct kspn_opts {
   uint32_t method;             /* KSPN_METHOD_* */

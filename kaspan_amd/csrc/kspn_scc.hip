@@ -340,6 +340,10 @@ struct OpArgs {
   uint32_t dead_pid;             // part holds DEAD_PID for every vertex that is not live, so a partition match
                                  // implies live and the live bitmap need not be read
   unsigned long long* cp;        // OP_COLOR/OP_COLOR_BW: per vertex (partition << 32 | colour)
+  uint32_t* prune;               // OP_FW/OP_PBW with part: one bit per entry of the adjacency being expanded, set
+                                 // once the edge is seen to join two partitions (or reach a decided vertex);
+                                 // partitions only split and decided vertices stay decided, so a set edge is
+                                 // of no use to any later level of the solve (nullptr: not kept)
 };
 
 // partition id of a vertex that is not live during the multi-pivot rounds
@@ -426,15 +430,28 @@ __device__ inline void wq_flush(WaveQueue& q, uint32_t* queue, uint32_t* count) 
 // exists), with the loads of each step issued for all K together: candidate
 // and visited words, partitions, the claiming atomics, the claimed
 // vertices' degrees. Same claims as expand_apply on each edge in turn.
-// Claimed vertices go to q (a BlockQueue or a WaveQueue).
+// Claimed vertices go to q (a BlockQueue or a WaveQueue). p[r] is the
+// edge's position in the adjacency: with a.prune, an edge whose bit is set
+// is skipped before its target is read, and an edge found to join two
+// partitions gets its bit set.
 template <int OP, int K, class Q>
-__device__ inline void reach_batch(const uint32_t (&u)[K], const uint32_t (&x)[K], const bool (&ok)[K],
-                                   const OpArgs& a, Q& q, uint64_t& nsum, uint32_t& nmax) {
+__device__ inline void reach_batch(const uint32_t (&u)[K], const uint32_t (&x)[K], const bool (&ok_in)[K],
+                                   const uint64_t (&p)[K], const OpArgs& a, Q& q, uint64_t& nsum, uint32_t& nmax) {
   const uint32_t* cand = OP == OP_BW ? a.fwv : (OP == OP_PBW ? a.pcand : a.live);
   uint32_t* vis = OP == OP_FW ? a.fwv : a.bwv;
   const bool part = OP == OP_PBW || (OP == OP_FW && a.part);
   // candidates = live and partition ids mark the dead: the id test alone decides
   const bool any_live = part && a.dead_pid && cand == a.live;
+  bool ok[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) ok[r] = ok_in[r];
+  if (part && a.prune) {
+    uint32_t pw[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) pw[r] = ok[r] ? a.prune[p[r] >> 5] : 0u;
+#pragma unroll
+    for (int r = 0; r < K; ++r) ok[r] = ok[r] && !((pw[r] >> (p[r] & 31)) & 1u);
+  }
   uint32_t cw[K], vw[K];
 #pragma unroll
   for (int r = 0; r < K; ++r) {
@@ -455,7 +472,10 @@ __device__ inline void reach_batch(const uint32_t (&u)[K], const uint32_t (&x)[K
       px[r] = c[r] ? a.part[x[r]] : 0u;
     }
 #pragma unroll
-    for (int r = 0; r < K; ++r) c[r] = c[r] && pu[r] == px[r];
+    for (int r = 0; r < K; ++r) {
+      if (a.prune && c[r] && pu[r] != px[r]) atomicOr(&a.prune[p[r] >> 5], 1u << (p[r] & 31));
+      c[r] = c[r] && pu[r] == px[r];
+    }
   }
   uint32_t old[K];
 #pragma unroll
@@ -488,15 +508,32 @@ __device__ inline void reach_batch(const uint32_t (&u)[K], const uint32_t (&x)[K
 // (partition, colour) word and is queued once (inq). OP_COLOR_BW: a live
 // in-neighbour with the same word is decided (removed from live, labelled
 // with the colour's root) and queued.
-// Same effect as expand_apply on each edge in turn.
+// Same effect as expand_apply on each edge in turn. With a.prune (p[r]:
+// the edge's position), a marked edge is skipped, and an edge whose ends
+// are in different partitions (a decided end holds partition 0) is marked.
 template <int OP, int K>
-__device__ inline void color_batch(const uint32_t (&u)[K], const uint32_t (&x)[K], const bool (&ok)[K],
-                                   const OpArgs& a, BlockQueue& q) {
+__device__ inline void color_batch(const uint32_t (&u)[K], const uint32_t (&x)[K], const bool (&ok_in)[K],
+                                   const uint64_t (&p)[K], const OpArgs& a, BlockQueue& q) {
+  bool ok[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) ok[r] = ok_in[r];
+  if (a.prune) {
+    uint32_t pw[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) pw[r] = ok[r] ? a.prune[p[r] >> 5] : 0u;
+#pragma unroll
+    for (int r = 0; r < K; ++r) ok[r] = ok[r] && !((pw[r] >> (p[r] & 31)) & 1u);
+  }
   unsigned long long xu[K], xw[K];
 #pragma unroll
   for (int r = 0; r < K; ++r) {
     xu[r] = ok[r] ? a.cp[u[r]] : 0ull;
     xw[r] = ok[r] ? a.cp[x[r]] : 0ull;
+  }
+  if (a.prune) {
+#pragma unroll
+    for (int r = 0; r < K; ++r)
+      if (ok[r] && (xw[r] >> 32) != (xu[r] >> 32)) atomicOr(&a.prune[p[r] >> 5], 1u << (p[r] & 31));
   }
   bool c[K];
   if constexpr (OP == OP_COLOR) {
@@ -626,12 +663,14 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
         // the thread's edges of the tile, their claims in flight together
         constexpr int EPT = EXP_TILE / WG;
         uint32_t uu[EPT], xx[EPT];
+        uint64_t pp[EPT];
         bool ok[EPT];
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
           const uint64_t e = e0 + threadIdx.x + (uint64_t)r * WG;
           ok[r] = e < e1;
           uu[r] = xx[r] = 0;
+          pp[r] = 0;
           if (ok[r]) {
             const int32_t el = (int32_t)(e - e0);
             uint32_t hi = w;
@@ -640,11 +679,12 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
               if (s_S[mid] <= el) lo = mid; else hi = mid;
             }
             uu[r] = s_v[lo];
-            xx[r] = adj[s_base[lo] + e];
+            pp[r] = s_base[lo] + e;
+            xx[r] = adj[pp[r]];
           }
         }
-        if constexpr (OP == OP_COLOR || OP == OP_COLOR_BW) color_batch<OP, EPT>(uu, xx, ok, a, q);
-        else reach_batch<OP, EPT, BlockQueue>(uu, xx, ok, a, q, nsum, nmax);
+        if constexpr (OP == OP_COLOR || OP == OP_COLOR_BW) color_batch<OP, EPT>(uu, xx, ok, pp, a, q);
+        else reach_batch<OP, EPT, BlockQueue>(uu, xx, ok, pp, a, q, nsum, nmax);
       } else {
         for (uint64_t e = e0 + threadIdx.x; e < e1; e += WG) {
           const int32_t el = (int32_t)(e - e0);
@@ -1153,6 +1193,7 @@ __global__ __launch_bounds__(NARROW_WG) void k_narrow_level(uint32_t* F0, uint32
     for (int o = 32; o > 0; o >>= 1) dm = max(dm, (uint32_t)__shfl_xor(dm, o));
     for (uint32_t e0 = 0; e0 < dm; e0 += 8) {
       uint32_t uu[8], xx[8];
+      uint64_t pp[8];
       bool ok[8];
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
@@ -1162,10 +1203,11 @@ __global__ __launch_bounds__(NARROW_WG) void k_narrow_level(uint32_t* F0, uint32
         const uint32_t e = e0 + k;
         ok[r] = e < dr;
         uu[r] = vr;
-        xx[r] = ok[r] ? adj[br + e] : 0u;
+        pp[r] = br + e;
+        xx[r] = ok[r] ? adj[pp[r]] : 0u;
       }
       if (q.n > NARROW_QCAP - 8 * WAVE) wq_flush(q, N, &ctl->cnt[s1]);
-      reach_batch<OP, 8, WaveQueue>(uu, xx, ok, a, q, nsum, nmax);
+      reach_batch<OP, 8, WaveQueue>(uu, xx, ok, pp, a, q, nsum, nmax);
     }
   }
   wq_flush(q, N, &ctl->cnt[s1]);
@@ -2521,12 +2563,15 @@ __global__ void k_color_init_list(const uint32_t* __restrict__ U, const uint32_t
 
 // pull sweep: v takes the max colour over its in-neighbours of the same
 // partition (decided ones hold 0); only v's thread raises cp[v]; changed
-// vertices go to `out`
+// vertices go to `out`. With `prune` (one bit per bw_adj entry), marked
+// in-edges are skipped and an in-neighbour of another partition gets its
+// edge marked (partitions only split, decided vertices stay decided).
 constexpr int CP_BATCH = 8;  // in-neighbours a colour pull reads per step
 
 __global__ __launch_bounds__(WG) void k_color_pull(const uint32_t* __restrict__ U, const uint32_t* ucount,
                                                    const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
-                                                   unsigned long long* cp, uint32_t* __restrict__ out, uint32_t* out_cnt) {
+                                                   unsigned long long* cp, uint32_t* __restrict__ out, uint32_t* out_cnt,
+                                                   uint32_t* prune) {
   __shared__ uint32_t s_out[8 * WG];
   __shared__ uint32_t s_on, s_ob;
   BlockQueue q{s_out, &s_on, &s_ob, 8 * WG};
@@ -2548,13 +2593,24 @@ __global__ __launch_bounds__(WG) void k_color_pull(const uint32_t* __restrict__ 
       for (uint64_t it = bw_off[v], end = bw_off[v + 1]; it < end; it += CP_BATCH) {
         uint32_t w[CP_BATCH];
         unsigned long long x[CP_BATCH];
+        bool in[CP_BATCH];
+        uint32_t pw[CP_BATCH];
 #pragma unroll
-        for (int k = 0; k < CP_BATCH; ++k) w[k] = it + k < end ? bw_adj[it + k] : v;
+        for (int k = 0; k < CP_BATCH; ++k) {
+          in[k] = it + k < end;
+          pw[k] = prune && in[k] ? prune[(it + k) >> 5] : 0u;
+          w[k] = in[k] ? bw_adj[it + k] : v;
+        }
 #pragma unroll
-        for (int k = 0; k < CP_BATCH; ++k) x[k] = it + k < end ? cp[w[k]] : 0ull;
+        for (int k = 0; k < CP_BATCH; ++k) {
+          in[k] = in[k] && !((pw[k] >> ((it + k) & 31)) & 1u);
+          x[k] = in[k] ? cp[w[k]] : 0ull;
+        }
 #pragma unroll
-        for (int k = 0; k < CP_BATCH; ++k)
+        for (int k = 0; k < CP_BATCH; ++k) {
           if ((uint32_t)(x[k] >> 32) == pv) c = max(c, (uint32_t)x[k]);
+          else if (prune && in[k]) atomicOr(&prune[(it + k) >> 5], 1u << ((it + k) & 31));
+        }
       }
       if (c > c0 && c0 != 0u) {
         atomicMax(&cp[v], color_word(pv, c));
@@ -3941,6 +3997,7 @@ struct Run {
   uint32_t ept = EXP_EPT;
   uint32_t chain = 0;                 // KSPN_FLAG_CHAIN: launches per chain of push levels (0: persistent kernel)
   uint32_t narrow_len = 0;            // KSPN_FLAG_NARROW: launches per chain of narrow levels (0: off)
+  bool prune = false;                 // multi-pivot rounds with KSPN_FLAG_PRUNE: the edge bitmaps are current
   bool dead_pid = false;              // pid holds DEAD_PID for every vertex that is not live (multi-pivot rounds)
   bool pivot_live_deg = false;        // pivots by live degree product (after an exhaustive trim)
   // the first trim's decided count and pivot are still on the device
@@ -4225,6 +4282,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
   const uint32_t* cand = (dir == 0 || full_bw) ? c->live : c->fwv;
   OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, 0, off, c->cnt64 + 1, pid, cand,
            pid && R.dead_pid ? 1u : 0u};
+  if (pid && R.prune && dir == 0) a.prune = c->prune_fw;
   uint32_t* F = c->qa;
   uint32_t* N = c->qb;
   int fc = C_Q0, nc = C_Q1;
@@ -4580,6 +4638,7 @@ int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
   k_pid_init<<<stream_grid(c, n), WG, 0, R.s>>>(R.n, c->live, c->pid, R.pid_preset);
   // the reaches of the rounds test partition ids instead of the live bitmap
   R.dead_pid = n <= DEAD_PID_MAX_N;
+
   const float* score = nullptr;
   if (R.o.pivot == KSPN_PIVOT_LEARNED) {
     // MLP scores, fixed for the whole solve, of the live vertices with the
@@ -4831,7 +4890,7 @@ int color_round(Run& R, uint32_t* decided) {
   {
     Prof p(c, R.s, K_COLOR_PULL, 24.0 * u);
     k_color_pull<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->cpw, c->qa,
-                                                    &c->cnt[C_Q0]);
+                                                    &c->cnt[C_Q0], R.prune ? c->prune_bw : nullptr);
   }
   KSPN_TRY(read_counters(c, R.s, 1));
   uint32_t f = c->h_cnt[C_Q0];
@@ -4848,10 +4907,12 @@ int color_round(Run& R, uint32_t* decided) {
       // live list was faster than a quarter or all of it on planted 5e7)
       Prof p(c, R.s, K_COLOR_PULL, 24.0 * u);
       k_color_pull<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->cpw, N,
-                                                      &c->cnt[nc]);
+                                                      &c->cnt[nc], R.prune ? c->prune_bw : nullptr);
     } else {
       uint64_t e = 0;
-      KSPN_TRY(expand<OP_COLOR>(R, F, &c->cnt[fc], f, g->fw_off, g->fw_adj, a, N, &c->cnt[nc], K_EXPAND_COLOR, 8.0, &e));
+      OpArgs af = a;
+      if (R.prune) af.prune = c->prune_fw;
+      KSPN_TRY(expand<OP_COLOR>(R, F, &c->cnt[fc], f, g->fw_off, g->fw_adj, af, N, &c->cnt[nc], K_EXPAND_COLOR, 8.0, &e));
     }
     KSPN_TRY(read_counters(c, R.s, 2));
     f = c->h_cnt[nc];
@@ -4873,6 +4934,7 @@ int color_round(Run& R, uint32_t* decided) {
   KSPN_TRY(read_counters(c, R.s, C_DTOT + 1));
   const uint32_t roots = c->h_cnt[C_DTOT];
   uint32_t seg_beg = 0, seg_end = roots;
+  if (R.prune) a.prune = c->prune_bw;
   while (seg_end > seg_beg) {
     uint32_t fseg = seg_end - seg_beg;
     k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_FSEG], fseg);
@@ -5682,7 +5744,8 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   if (c->actl) (void)hipFree(c->actl);
   void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->sbm, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
                   c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf, c->pid, c->pbest,
-                  c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt, c->vprio, c->ds_st, c->deg8, c->cpw};
+                  c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt, c->vprio, c->ds_st, c->deg8, c->cpw,
+                  c->prune_fw, c->prune_bw};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
@@ -5759,6 +5822,20 @@ int solve(kspn_ctx* k, const kspn_graph* g, uint32_t* labels, const kspn_opts* o
     R.pid_preset = true;
     R.np0 = count;
     R.vtx_off = vtx_off;
+  }
+  if (o.flags & KSPN_FLAG_PRUNE) {
+    // one bit per adjacency entry in each direction, all clear: no edge is
+    // known yet to join two partitions
+    const uint64_t words = R.g->m / 32 + 1;
+    if (c->prune_words < words) {
+      c->prune_words = 0;
+      KSPN_TRY(dalloc(&c->prune_fw, words));
+      KSPN_TRY(dalloc(&c->prune_bw, words));
+      c->prune_words = words;
+    }
+    k_fill_u32<<<stream_grid(c, words), WG, 0, s>>>(c->prune_fw, words, 0u);
+    k_fill_u32<<<stream_grid(c, words), WG, 0, s>>>(c->prune_bw, words, 0u);
+    R.prune = true;
   }
   int rc = run_pipeline(R);
   if (c->profile) {
