@@ -324,6 +324,8 @@ struct Ctx {
   unsigned long long* ds_st = nullptr;  // look-back scan: status word per tile
   uint32_t* ds_ticket = nullptr;        // its ticket counter
   uint32_t ds_epoch = 0;                // epoch of its last launch
+  uint32_t* tstart = nullptr;           // per expansion tile of the last look-back scan: its first frontier entry
+  uint64_t tstart_cap = 0;              // entries tstart holds
   uint32_t* prune_fw = nullptr;         // KSPN_FLAG_PRUNE: one bit per fw_adj entry (edges joining two partitions)
   uint32_t* prune_bw = nullptr;         // ... and per bw_adj entry
   uint64_t prune_words = 0;             // words each holds

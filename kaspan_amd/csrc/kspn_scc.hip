@@ -586,7 +586,8 @@ template <int OP, int EPT_ = EXP_EPT>
 __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, const uint32_t* fcount,
                                                const uint64_t* __restrict__ S, const uint64_t* __restrict__ off,
                                                const uint32_t* __restrict__ adj, OpArgs a, uint32_t* __restrict__ next,
-                                               uint32_t* next_cnt) {
+                                               uint32_t* next_cnt, const uint32_t* __restrict__ tstart,
+                                               uint64_t tcap) {
   // the window's entries: scan value relative to the tile start (0 for the
   // first, whose edges may begin before it), row start minus scan value (so
   // edge e of entry j is adj[s_base[j] + e]) and vertex; 16 bytes per entry
@@ -615,11 +616,18 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
   const uint64_t ntiles = (total + EXP_TILE - 1) / EXP_TILE;
   // a grid sized before the edge total was known leaves blocks without a tile
   if (blockIdx.x >= ntiles) return;
+  if (ntiles > tcap) tstart = nullptr;  // not from the scan of this frontier
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const uint64_t e0 = t * EXP_TILE;
     const uint64_t e1 = min(e0 + (uint64_t)EXP_TILE, total);
     uint32_t i0, w;
-    {
+    if (tstart) {
+      // the scan recorded the entry holding each tile's first edge; the
+      // window ends with the next tile's (whose edges may all lie beyond
+      // this tile: the LDS search never selects it then)
+      i0 = tstart[t];
+      w = (t + 1 < ntiles ? tstart[t + 1] : f - 1) - i0 + 1;
+    } else {
       // the last frontier entries whose edges start at or before e0 and at
       // or before e1 - 1: two 128-way searches side by side (threads 0..127
       // and 128..255). Each round probes 128 evenly spaced scan values at
@@ -2132,7 +2140,10 @@ __global__ __launch_bounds__(1024) void k_deg_scan_small(const uint32_t* __restr
 // launches; values are below 2^48. The last ticket of a launch resets the
 // ticket counter for the next one. The launch has exactly `tiles`
 // workgroups, ceil((f + 1) / DS_TILE) for the f the host knows; a device
-// count that does not fit sets *err.
+// count that does not fit sets *err. With tstart, each entry i of non-zero
+// degree also writes tstart[t] = i for every expansion tile t < tcap (tiles
+// of 2^tile_shift edges) whose first edge is one of its own, so k_expand
+// reads a tile's frontier window instead of searching the scan for it.
 constexpr uint32_t DS_ITEMS = 8;
 constexpr uint32_t DS_TILE = WG * DS_ITEMS;
 constexpr unsigned long long DS_AGG = 1ull << 48, DS_INC = 2ull << 48, DS_FLAGS = 3ull << 48;
@@ -2143,7 +2154,8 @@ constexpr uint32_t DS_SPIN = 1u << 22;
 __global__ __launch_bounds__(WG) void k_deg_scan(const uint32_t* __restrict__ F, const uint32_t* fcount,
                                                  const uint64_t* __restrict__ off, uint64_t* __restrict__ S,
                                                  unsigned long long* st, uint32_t* ticket, uint32_t tiles,
-                                                 uint32_t epoch, uint32_t* err) {
+                                                 uint32_t epoch, uint32_t* err, uint32_t* __restrict__ tstart,
+                                                 uint32_t tile_shift, uint64_t tcap) {
   __shared__ uint32_t s_t;
   __shared__ uint64_t s_w[WG / WAVE];
   __shared__ uint64_t s_excl;
@@ -2224,6 +2236,12 @@ __global__ __launch_bounds__(WG) void k_deg_scan(const uint32_t* __restrict__ F,
 #pragma unroll
   for (uint32_t k = 0; k < DS_ITEMS; ++k) {
     if (i0 + k <= f) S[i0 + k] = run;
+    if (tstart && d[k]) {
+      // the tiles whose first edge lies in [run, run + d)
+      const uint64_t mask = (1ull << tile_shift) - 1;
+      const uint64_t t1 = min((run + d[k] - 1) >> tile_shift, tcap - 1);
+      for (uint64_t t = (run + mask) >> tile_shift; t <= t1; ++t) tstart[t] = (uint32_t)(i0 + k);
+    }
     run += d[k];
   }
 }
@@ -4017,9 +4035,14 @@ int select_pivot_live(Run& R, uint32_t* pivot);
 // device at fcount, host copy f); S[f] is the total. One workgroup below
 // 4096 entries, else the single-pass look-back scan (hipCUB's device scan
 // where a total could reach 2^48).
-int frontier_scan(Run& R, const uint32_t* F, const uint32_t* fcount_dev, uint32_t f, const uint64_t* off) {
+// With `tiles` (expansion tiles of 2^tile_shift edges), the look-back scan
+// also writes their first frontier entries to c->tstart and *tiles is set;
+// the other scans leave *tiles false.
+int frontier_scan(Run& R, const uint32_t* F, const uint32_t* fcount_dev, uint32_t f, const uint64_t* off,
+                  bool* tiles_out = nullptr, uint32_t tile_shift = 11) {
   Ctx* c = R.c;
   Prof p(c, R.s, K_FRONTIER_DEG, 28.0 * f);
+  if (tiles_out) *tiles_out = false;
   if (f < 4096) {  // S[0..f] fits the 4096 slots of one workgroup
     k_deg_scan_small<<<1, 1024, 0, R.s>>>(F, fcount_dev, off, c->scan);
     return KSPN_OK;
@@ -4035,8 +4058,22 @@ int frontier_scan(Run& R, const uint32_t* F, const uint32_t* fcount_dev, uint32_
     c->ds_epoch = 1;
   }
   const uint32_t tiles = (uint32_t)(((uint64_t)f + DS_TILE) / DS_TILE);
+  uint32_t* tstart = nullptr;
+  if (tiles_out) {
+    // every tile of the largest level: m edges, tiles of at least 2^9 edges
+    const uint64_t need = (R.g->m >> 9) + 2;
+    if (c->tstart_cap < need) {
+      c->tstart_cap = 0;
+      KSPN_TRY(dalloc(&c->tstart, need));
+      c->tstart_cap = need;
+    }
+    if (tile_shift >= 9) {
+      tstart = c->tstart;
+      *tiles_out = true;
+    }
+  }
   k_deg_scan<<<tiles, WG, 0, R.s>>>(F, fcount_dev, off, c->scan, c->ds_st, c->ds_ticket, tiles, c->ds_epoch,
-                                    &c->cnt[C_SERR]);
+                                    &c->cnt[C_SERR], tstart, tile_shift, c->tstart_cap);
   return KSPN_OK;
 }
 
@@ -4052,7 +4089,8 @@ int expand(Run& R, const uint32_t* F, const uint32_t* fcount_dev, uint32_t f, co
     if (edges) *edges = 0;
     return KSPN_OK;
   }
-  KSPN_TRY(frontier_scan(R, F, fcount_dev, f, off));
+  bool tiles = false;
+  KSPN_TRY(frontier_scan(R, F, fcount_dev, f, off, &tiles, 11));  // k_expand<OP>: tiles of WG * 8 edges
   uint64_t total = total_hint;
   if (!known_total) {
     KSPN_TRY(publish(c, R.s, c->scan + f));
@@ -4066,7 +4104,8 @@ int expand(Run& R, const uint32_t* F, const uint32_t* fcount_dev, uint32_t f, co
   if (total == 0) return KSPN_OK;
   {
     Prof p(c, R.s, kern, 4.0 * f + 8.0 * f + 8.0 * f + (4.0 + state_bytes_per_edge) * total);
-    k_expand<OP><<<expand_grid(c, total), WG, 0, R.s>>>(F, fcount_dev, c->scan, off, adj, a, next, next_cnt);
+    k_expand<OP><<<expand_grid(c, total), WG, 0, R.s>>>(F, fcount_dev, c->scan, off, adj, a, next, next_cnt,
+                                                         tiles ? c->tstart : nullptr, c->tstart_cap);
   }
   if (R.st) R.st->edges_scanned += total;
   return KSPN_OK;
@@ -4240,22 +4279,23 @@ int reach_async(Run& R, int op, const OpArgs& a, const uint64_t* off, const uint
 }
 
 // a push level of a reach with the tile width the options ask for
+// (tstart: the scan's tile starts for tiles of WG * ept edges, or nullptr)
 template <int OP>
 void launch_expand_ept(uint32_t ept, uint32_t grid, hipStream_t s, const uint32_t* F, const uint32_t* fc,
                        const uint64_t* S, const uint64_t* off, const uint32_t* adj, const OpArgs& a, uint32_t* N,
-                       uint32_t* nc) {
-  if (ept == 2) k_expand<OP, 2><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc);
-  else if (ept == 4) k_expand<OP, 4><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc);
-  else if (ept == 16) k_expand<OP, 16><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc);
-  else k_expand<OP, EXP_EPT><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc);
+                       uint32_t* nc, const uint32_t* tstart, uint64_t tcap) {
+  if (ept == 2) k_expand<OP, 2><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc, tstart, tcap);
+  else if (ept == 4) k_expand<OP, 4><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc, tstart, tcap);
+  else if (ept == 16) k_expand<OP, 16><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc, tstart, tcap);
+  else k_expand<OP, EXP_EPT><<<grid, WG, 0, s>>>(F, fc, S, off, adj, a, N, nc, tstart, tcap);
 }
 
 void launch_reach_expand(int op, uint32_t ept, uint32_t grid, hipStream_t s, const uint32_t* F, const uint32_t* fc,
                          const uint64_t* S, const uint64_t* off, const uint32_t* adj, const OpArgs& a, uint32_t* N,
-                         uint32_t* nc) {
-  if (op == OP_FW) launch_expand_ept<OP_FW>(ept, grid, s, F, fc, S, off, adj, a, N, nc);
-  else if (op == OP_PBW) launch_expand_ept<OP_PBW>(ept, grid, s, F, fc, S, off, adj, a, N, nc);
-  else launch_expand_ept<OP_BW>(ept, grid, s, F, fc, S, off, adj, a, N, nc);
+                         uint32_t* nc, const uint32_t* tstart, uint64_t tcap) {
+  if (op == OP_FW) launch_expand_ept<OP_FW>(ept, grid, s, F, fc, S, off, adj, a, N, nc, tstart, tcap);
+  else if (op == OP_PBW) launch_expand_ept<OP_PBW>(ept, grid, s, F, fc, S, off, adj, a, N, nc, tstart, tcap);
+  else launch_expand_ept<OP_BW>(ept, grid, s, F, fc, S, off, adj, a, N, nc, tstart, tcap);
 }
 
 // a pull sweep with the width the options ask for
@@ -4415,8 +4455,6 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       // nothing expanded (not expected: the bounds were checked): one level on the host
     }
     if (!pull) {
-      if (!scanned) KSPN_TRY(frontier_scan(R, F, &c->cnt[fc], f, off));
-      scanned = false;
       int kern = dir == 0 ? K_EXPAND_FW : K_EXPAND_BW;
       // tile width: the options' or, by default, the widest that still
       // gives every CU a few tiles (narrow tiles fill the device on
@@ -4426,10 +4464,17 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
         const uint64_t cu = c->num_cu;
         ept = !known || edges >= cu * WG * 8 * 4 ? 8u : (edges >= cu * WG * 4 * 2 ? 4u : 2u);
       }
+      bool tiles = false;  // the scan recorded the tiles' frontier windows
+      if (!scanned) {
+        const uint32_t shift = ept == 2 ? 9u : (ept == 4 ? 10u : (ept == 16 ? 12u : 11u));
+        KSPN_TRY(frontier_scan(R, F, &c->cnt[fc], f, off, &tiles, shift));
+      }
+      scanned = false;
       uint32_t grid = known ? expand_grid(c, edges, ept) : (uint32_t)c->num_cu * 8;
       Prof p(c, R.s, kern, 20.0 * f + (4.0 + 0.125) * (double)edges);
       const int rop = dir == 0 ? OP_FW : (pid ? OP_PBW : OP_BW);
-      launch_reach_expand(rop, ept, grid, R.s, F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc]);
+      launch_reach_expand(rop, ept, grid, R.s, F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc],
+                          tiles ? c->tstart : nullptr, c->tstart_cap);
       if (fw_levels_push) (*fw_levels_push)++;
     } else {
       bool batch = list_mode;  // a batch of list levels follows (after a sweep: when the remainder is short)
@@ -5745,7 +5790,7 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->sbm, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
                   c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf, c->pid, c->pbest,
                   c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt, c->vprio, c->ds_st, c->deg8, c->cpw,
-                  c->prune_fw, c->prune_bw};
+                  c->prune_fw, c->prune_bw, c->tstart};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
