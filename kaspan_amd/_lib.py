@@ -48,6 +48,7 @@ FLAG_CHAIN = 512
 FLAG_PERSISTENT = 1024
 FLAG_NARROW = 2048
 FLAG_PRUNE = 4096
+FLAG_CHUNKED = 8192
 PHASE_MAX = 12
 MANIFEST_TEXT = 256
 MANIFEST_PATH = 4096

@@ -33,8 +33,9 @@ SPACE = {
     # KSPN_FLAG_FULL_BW (64): three-way split of the multi-pivot rounds; KSPN_FLAG_PERSISTENT (1024): device
     # push levels in one persistent launch instead of chained launches; KSPN_FLAG_NARROW (2048): chained
     # narrow levels for frontiers of degree <= 64; KSPN_FLAG_PRUNE (4096): edges found to join two partitions
-    # are skipped from then on. First: they change what the frontier bounds below mean.
-    "flags": [0, 64, 1024, 1088, 2048, 2112, 4096, 6144, 6208],
+    # are skipped from then on; KSPN_FLAG_CHUNKED (8192): frontiers of degree <= 64 without a degree scan.
+    # First: they change what the frontier bounds below mean.
+    "flags": [0, 64, 1024, 1088, 2048, 2112, 4096, 6144, 6208, 8192, 14336, 14400],
     "fwbw_rounds": [1, 4, 16, 24, 32, 48, 64],
     "alpha": [14.0, 4.0, 40.0, 1e-9],
     "beta": [24.0, 6.0, 96.0],

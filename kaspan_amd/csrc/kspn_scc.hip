@@ -573,6 +573,16 @@ __device__ inline void color_batch(const uint32_t (&u)[K], const uint32_t (&x)[K
   }
 }
 
+// inclusive prefix sum over the wavefront
+__device__ inline uint64_t wave_incl_scan64(uint64_t v) {
+  const uint32_t lane = lane_id();
+  for (uint32_t o = 1; o < WAVE; o <<= 1) {
+    const uint64_t t = __shfl_up(v, o);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
 constexpr int EXP_EPT = 8;        // edges per thread of a tile (default; 4 and 16 are also built)
 constexpr int EXP_WIN = 1024;     // frontier entries staged in LDS per tile
 
@@ -732,6 +742,115 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
   }
 }
 
+// Push levels over frontiers of bounded out-degree, without a degree scan:
+// a workgroup takes CHUNK_N consecutive frontier entries, scans their
+// out-degrees in LDS and claims the chunk's edges WG * 8 at a time (lane l
+// of step r takes edge base + r * WG + l of the chunk, so consecutive lanes
+// read consecutive entries of a row) with reach_batch. A chunk holds at
+// most CHUNK_N * CHUNK_MAXD edges; on frontiers whose degrees vary little
+// (planted SCC graphs, lattices) the chunks carry about the same edge
+// count. Claimed vertices, the next frontier's degree sum and maximum go
+// out as in k_expand. The frontier's edge count, which the host knows,
+// is stored to *stot, where a degree scan would have left its total.
+constexpr uint32_t CHUNK_N = 2 * WG;      // frontier entries per chunk, two per thread
+constexpr uint32_t CHUNK_MAXD = 64;       // largest out-degree of a frontier the host sends here
+constexpr uint32_t CHUNK_QCAP = 2 * WG * 8;
+
+template <int OP>
+__global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict__ F, const uint32_t* fcount,
+                                                     const uint64_t* __restrict__ off,
+                                                     const uint32_t* __restrict__ adj, OpArgs a,
+                                                     uint32_t* __restrict__ next, uint32_t* next_cnt, uint32_t* err,
+                                                     uint64_t* stot, uint64_t etot) {
+  __shared__ uint32_t s_v[CHUNK_N];
+  __shared__ uint64_t s_b[CHUNK_N];       // row starts
+  __shared__ uint32_t s_S[CHUNK_N + 1];   // exclusive prefix of the degrees within the chunk
+  __shared__ uint64_t s_w[WG / WAVE];
+  __shared__ uint32_t s_out[CHUNK_QCAP];
+  __shared__ uint32_t s_on, s_ob;
+  __shared__ unsigned long long s_nsum;
+  __shared__ uint32_t s_nmax;
+  BlockQueue q{s_out, &s_on, &s_ob, CHUNK_QCAP};
+  if (threadIdx.x == 0) {
+    s_nsum = 0;
+    s_nmax = 0;
+  }
+  bq_init(q);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *stot = etot;
+  uint64_t nsum = 0;
+  uint32_t nmax = 0;
+  const uint32_t f = *fcount;
+  const uint32_t nchunks = (uint32_t)(((uint64_t)f + CHUNK_N - 1) / CHUNK_N);
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const uint64_t i0 = (uint64_t)ch * CHUNK_N;
+    uint32_t d[2];
+#pragma unroll
+    for (uint32_t k = 0; k < 2; ++k) {
+      const uint32_t j = 2 * threadIdx.x + k;
+      uint32_t v = 0;
+      uint64_t b = 0;
+      d[k] = 0;
+      if (i0 + j < f) {
+        v = F[i0 + j];
+        b = off[v];
+        d[k] = (uint32_t)min(off[v + 1] - b, (uint64_t)0xFFFFFFFFu);
+      }
+      s_v[j] = v;
+      s_b[j] = b;
+    }
+    const uint64_t sum = (uint64_t)d[0] + d[1];
+    const uint64_t incl = wave_incl_scan64(sum);
+    if (lane == WAVE - 1) s_w[wid] = incl;
+    __syncthreads();
+    uint64_t woff = 0, T = 0;
+    for (uint32_t w = 0; w < WG / WAVE; ++w) {
+      woff += w < wid ? s_w[w] : 0;
+      T += s_w[w];
+    }
+    const uint64_t ex = woff + incl - sum;
+    s_S[2 * threadIdx.x] = (uint32_t)ex;
+    s_S[2 * threadIdx.x + 1] = (uint32_t)(ex + d[0]);
+    if (threadIdx.x == 0) s_S[CHUNK_N] = (uint32_t)T;
+    __syncthreads();
+    // the host sends frontiers of degree <= CHUNK_MAXD only; a larger chunk is reported, not expanded past it
+    if (T > (uint64_t)CHUNK_N * CHUNK_MAXD && threadIdx.x == 0) *err = 1u;
+    const uint32_t total = (uint32_t)min(T, (uint64_t)CHUNK_N * CHUNK_MAXD);
+    uint32_t lo = 0;  // each thread's edges come in increasing order: the search only moves forward
+    for (uint32_t base = 0; base < total; base += WG * 8) {
+      uint32_t uu[8], xx[8];
+      uint64_t pp[8];
+      bool ok[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const uint32_t e = base + threadIdx.x + (uint32_t)r * WG;
+        ok[r] = e < total;
+        uu[r] = xx[r] = 0;
+        pp[r] = 0;
+        if (ok[r]) {
+          uint32_t hi = CHUNK_N;
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_S[mid] <= e) lo = mid; else hi = mid;
+          }
+          uu[r] = s_v[lo];
+          pp[r] = s_b[lo] + (e - s_S[lo]);
+          xx[r] = adj[pp[r]];
+        }
+      }
+      reach_batch<OP, 8, BlockQueue>(uu, xx, ok, pp, a, q, nsum, nmax);
+      bq_maybe_flush(q, next, next_cnt, WG * 8);  // a step appends at most WG * 8 entries
+    }
+    __syncthreads();  // the next chunk rewrites the LDS tables
+  }
+  bq_flush(q, next, next_cnt);
+  if (nsum) atomicAdd(&s_nsum, (unsigned long long)nsum);
+  if (nmax) atomicMax(&s_nmax, nmax);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_nsum) atomicAdd(a.nsum, s_nsum);
+  if (threadIdx.x == 0 && s_nmax) atomicMax(&a.nsum[1], (unsigned long long)s_nmax);
+}
+
 // =====================================================================
 // persistent multi-level reachability
 // =====================================================================
@@ -819,15 +938,6 @@ __device__ bool coop_barrier(CoopCtl* ctl, uint32_t k, uint32_t next_slot, CoopL
   }
   __syncthreads();
   return s_ok != 0;
-}
-
-__device__ inline uint64_t wave_incl_scan64(uint64_t v) {
-  const uint32_t lane = lane_id();
-  for (uint32_t o = 1; o < WAVE; o <<= 1) {
-    const uint64_t t = __shfl_up(v, o);
-    if (lane >= o) v += t;
-  }
-  return v;
 }
 
 // a full wavefront queue goes out on its own (one atomic per flush)
@@ -4464,8 +4574,10 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
         const uint64_t cu = c->num_cu;
         ept = !known || edges >= cu * WG * 8 * 4 ? 8u : (edges >= cu * WG * 4 * 2 ? 4u : 2u);
       }
+      // a frontier of small degrees: chunks of entries, no degree scan
+      const bool chunked = (R.o.flags & KSPN_FLAG_CHUNKED) && known && !scanned && maxd <= CHUNK_MAXD;
       bool tiles = false;  // the scan recorded the tiles' frontier windows
-      if (!scanned) {
+      if (!scanned && !chunked) {
         const uint32_t shift = ept == 2 ? 9u : (ept == 4 ? 10u : (ept == 16 ? 12u : 11u));
         KSPN_TRY(frontier_scan(R, F, &c->cnt[fc], f, off, &tiles, shift));
       }
@@ -4473,8 +4585,22 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       uint32_t grid = known ? expand_grid(c, edges, ept) : (uint32_t)c->num_cu * 8;
       Prof p(c, R.s, kern, 20.0 * f + (4.0 + 0.125) * (double)edges);
       const int rop = dir == 0 ? OP_FW : (pid ? OP_PBW : OP_BW);
-      launch_reach_expand(rop, ept, grid, R.s, F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc],
-                          tiles ? c->tstart : nullptr, c->tstart_cap);
+      if (chunked) {
+        const uint32_t cgrid = (uint32_t)std::min<uint64_t>(((uint64_t)f + CHUNK_N - 1) / CHUNK_N,
+                                                           (uint64_t)c->num_cu * 16);
+        if (rop == OP_FW)
+          k_expand_chunk<OP_FW><<<cgrid, WG, 0, R.s>>>(F, &c->cnt[fc], off, adj, a, N, &c->cnt[nc], &c->cnt[C_ERR],
+                                                        c->scan + f, edges);
+        else if (rop == OP_PBW)
+          k_expand_chunk<OP_PBW><<<cgrid, WG, 0, R.s>>>(F, &c->cnt[fc], off, adj, a, N, &c->cnt[nc], &c->cnt[C_ERR],
+                                                        c->scan + f, edges);
+        else
+          k_expand_chunk<OP_BW><<<cgrid, WG, 0, R.s>>>(F, &c->cnt[fc], off, adj, a, N, &c->cnt[nc], &c->cnt[C_ERR],
+                                                        c->scan + f, edges);
+      } else {
+        launch_reach_expand(rop, ept, grid, R.s, F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc],
+                            tiles ? c->tstart : nullptr, c->tstart_cap);
+      }
       if (fw_levels_push) (*fw_levels_push)++;
     } else {
       bool batch = list_mode;  // a batch of list levels follows (after a sweep: when the remainder is short)
