@@ -742,18 +742,22 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
   }
 }
 
-// Push levels over frontiers of bounded out-degree, without a degree scan:
+// Push levels over frontiers of small out-degrees, without a degree scan:
 // a workgroup takes CHUNK_N consecutive frontier entries, scans their
-// out-degrees in LDS and claims the chunk's edges WG * 8 at a time (lane l
-// of step r takes edge base + r * WG + l of the chunk, so consecutive lanes
-// read consecutive entries of a row) with reach_batch. A chunk holds at
-// most CHUNK_N * CHUNK_MAXD edges; on frontiers whose degrees vary little
-// (planted SCC graphs, lattices) the chunks carry about the same edge
-// count. Claimed vertices, the next frontier's degree sum and maximum go
-// out as in k_expand. The frontier's edge count, which the host knows,
-// is stored to *stot, where a degree scan would have left its total.
+// out-degrees in LDS and applies OP to the chunk's edges WG * 8 at a time
+// (lane l of step r takes edge base + r * WG + l of the chunk, so
+// consecutive lanes read consecutive entries of a row) with reach_batch or
+// color_batch. The reach levels the host sends here have no vertex of
+// out-degree above CHUNK_MAXD, so the chunks of a planted-SCC or lattice
+// frontier carry about the same edge count (the colour levels, whose
+// degrees the host does not know, take the same path when the options ask
+// for it). Claimed vertices, and for the reach ops the next frontier's
+// degree sum and maximum, go out as in k_expand. For the reach ops the
+// frontier's edge count, which the host knows, is stored to *stot, where a
+// degree scan would have left its total; with esum, every chunk adds its
+// edge count there (statistics of the colour levels).
 constexpr uint32_t CHUNK_N = 2 * WG;      // frontier entries per chunk, two per thread
-constexpr uint32_t CHUNK_MAXD = 64;       // largest out-degree of a frontier the host sends here
+constexpr uint32_t CHUNK_MAXD = 64;       // largest out-degree of a reach frontier the host sends here
 constexpr uint32_t CHUNK_QCAP = 2 * WG * 8;
 
 template <int OP>
@@ -761,7 +765,7 @@ __global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict_
                                                      const uint64_t* __restrict__ off,
                                                      const uint32_t* __restrict__ adj, OpArgs a,
                                                      uint32_t* __restrict__ next, uint32_t* next_cnt, uint32_t* err,
-                                                     uint64_t* stot, uint64_t etot) {
+                                                     uint64_t* stot, uint64_t etot, unsigned long long* esum) {
   __shared__ uint32_t s_v[CHUNK_N];
   __shared__ uint64_t s_b[CHUNK_N];       // row starts
   __shared__ uint32_t s_S[CHUNK_N + 1];   // exclusive prefix of the degrees within the chunk
@@ -776,7 +780,7 @@ __global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict_
     s_nmax = 0;
   }
   bq_init(q);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *stot = etot;
+  if (stot && blockIdx.x == 0 && threadIdx.x == 0) *stot = etot;
   uint64_t nsum = 0;
   uint32_t nmax = 0;
   const uint32_t f = *fcount;
@@ -813,9 +817,10 @@ __global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict_
     s_S[2 * threadIdx.x + 1] = (uint32_t)(ex + d[0]);
     if (threadIdx.x == 0) s_S[CHUNK_N] = (uint32_t)T;
     __syncthreads();
-    // the host sends frontiers of degree <= CHUNK_MAXD only; a larger chunk is reported, not expanded past it
-    if (T > (uint64_t)CHUNK_N * CHUNK_MAXD && threadIdx.x == 0) *err = 1u;
-    const uint32_t total = (uint32_t)min(T, (uint64_t)CHUNK_N * CHUNK_MAXD);
+    // the in-chunk scan is 32-bit: a chunk of more edges is reported, not expanded past them
+    if (T > 0xFFFFFFFFull && threadIdx.x == 0) *err = 1u;
+    const uint32_t total = (uint32_t)min(T, 0xFFFFFFFFull);
+    if (esum && threadIdx.x == 0 && total) atomicAdd(esum, (unsigned long long)total);
     uint32_t lo = 0;  // each thread's edges come in increasing order: the search only moves forward
     for (uint32_t base = 0; base < total; base += WG * 8) {
       uint32_t uu[8], xx[8];
@@ -838,17 +843,20 @@ __global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict_
           xx[r] = adj[pp[r]];
         }
       }
-      reach_batch<OP, 8, BlockQueue>(uu, xx, ok, pp, a, q, nsum, nmax);
+      if constexpr (OP == OP_COLOR || OP == OP_COLOR_BW) color_batch<OP, 8>(uu, xx, ok, pp, a, q);
+      else reach_batch<OP, 8, BlockQueue>(uu, xx, ok, pp, a, q, nsum, nmax);
       bq_maybe_flush(q, next, next_cnt, WG * 8);  // a step appends at most WG * 8 entries
     }
     __syncthreads();  // the next chunk rewrites the LDS tables
   }
   bq_flush(q, next, next_cnt);
-  if (nsum) atomicAdd(&s_nsum, (unsigned long long)nsum);
-  if (nmax) atomicMax(&s_nmax, nmax);
-  __syncthreads();
-  if (threadIdx.x == 0 && s_nsum) atomicAdd(a.nsum, s_nsum);
-  if (threadIdx.x == 0 && s_nmax) atomicMax(&a.nsum[1], (unsigned long long)s_nmax);
+  if constexpr (OP == OP_FW || OP == OP_BW || OP == OP_PBW) {
+    if (nsum) atomicAdd(&s_nsum, (unsigned long long)nsum);
+    if (nmax) atomicMax(&s_nmax, nmax);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_nsum) atomicAdd(a.nsum, s_nsum);
+    if (threadIdx.x == 0 && s_nmax) atomicMax(&a.nsum[1], (unsigned long long)s_nmax);
+  }
 }
 
 // =====================================================================
@@ -4199,6 +4207,19 @@ int expand(Run& R, const uint32_t* F, const uint32_t* fcount_dev, uint32_t f, co
     if (edges) *edges = 0;
     return KSPN_OK;
   }
+  if constexpr (OP == OP_COLOR || OP == OP_COLOR_BW) {
+    if (R.o.flags & KSPN_FLAG_CHUNKED) {
+      // chunks of the frontier, no degree scan and no publication of its
+      // edge count (statistics: summed into cnt64[3], read by color_round)
+      Prof p(c, R.s, kern, 20.0 * f);
+      const uint32_t cgrid = (uint32_t)std::min<uint64_t>(((uint64_t)f + CHUNK_N - 1) / CHUNK_N,
+                                                         (uint64_t)c->num_cu * 16);
+      k_expand_chunk<OP><<<cgrid, WG, 0, R.s>>>(F, fcount_dev, off, adj, a, next, next_cnt, &c->cnt[C_ERR], nullptr,
+                                                 0, R.st ? c->cnt64 + 3 : nullptr);
+      if (edges) *edges = 0;
+      return KSPN_OK;
+    }
+  }
   bool tiles = false;
   KSPN_TRY(frontier_scan(R, F, fcount_dev, f, off, &tiles, 11));  // k_expand<OP>: tiles of WG * 8 edges
   uint64_t total = total_hint;
@@ -4590,13 +4611,13 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
                                                            (uint64_t)c->num_cu * 16);
         if (rop == OP_FW)
           k_expand_chunk<OP_FW><<<cgrid, WG, 0, R.s>>>(F, &c->cnt[fc], off, adj, a, N, &c->cnt[nc], &c->cnt[C_ERR],
-                                                        c->scan + f, edges);
+                                                        c->scan + f, edges, nullptr);
         else if (rop == OP_PBW)
           k_expand_chunk<OP_PBW><<<cgrid, WG, 0, R.s>>>(F, &c->cnt[fc], off, adj, a, N, &c->cnt[nc], &c->cnt[C_ERR],
-                                                        c->scan + f, edges);
+                                                        c->scan + f, edges, nullptr);
         else
           k_expand_chunk<OP_BW><<<cgrid, WG, 0, R.s>>>(F, &c->cnt[fc], off, adj, a, N, &c->cnt[nc], &c->cnt[C_ERR],
-                                                        c->scan + f, edges);
+                                                        c->scan + f, edges, nullptr);
       } else {
         launch_reach_expand(rop, ept, grid, R.s, F, &c->cnt[fc], c->scan, off, adj, a, N, &c->cnt[nc],
                             tiles ? c->tstart : nullptr, c->tstart_cap);
@@ -5056,6 +5077,9 @@ int color_round(Run& R, uint32_t* decided) {
   k_copy_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_RES_N], &c->cnt[C_AUX]);
   R.prev_u = u;
   OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, r, nullptr, nullptr, pid, nullptr, 0u, c->cpw};
+  // chunked push levels publish no edge count: they sum it into cnt64[3]
+  const bool chunk_stats = R.st && (R.o.flags & KSPN_FLAG_CHUNKED);
+  if (chunk_stats) k_fill_u64<<<1, 64, 0, R.s>>>(c->cnt64 + 3, 1, 0ull);
   // forward: pull sweep over all live vertices
   KSPN_HIP(hipMemsetAsync(&c->cnt[C_Q0], 0, sizeof(uint32_t), R.s));
   {
@@ -5117,6 +5141,12 @@ int color_round(Run& R, uint32_t* decided) {
     seg_end = c->h_cnt[C_DTOT];
   }
   const uint32_t total = seg_end;
+  if (chunk_stats) {
+    unsigned long long e = 0;
+    KSPN_HIP(hipMemcpyAsync(&e, c->cnt64 + 3, sizeof(e), hipMemcpyDeviceToHost, R.s));
+    KSPN_HIP(hipStreamSynchronize(R.s));
+    R.st->edges_scanned += e;
+  }
   if (pid) {
     Prof p(c, R.s, K_COLOR_INIT, 12.0 * u);
     k_refine_pid<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], c->live, c->cpw, r, c->pid);

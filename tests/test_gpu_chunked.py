@@ -1,6 +1,7 @@
-"""Chunked push levels (KSPN_FLAG_CHUNKED): a frontier whose vertices have
-at most 64 out-edges each is expanded by chunks of 512 frontier entries
-with their degrees scanned in LDS, without the device-wide degree scan.
+"""Chunked push levels (KSPN_FLAG_CHUNKED): a reach frontier whose vertices
+have at most 64 out-edges each, and every colour push level, is expanded by
+chunks of 512 frontier entries with their degrees scanned in LDS, without
+the device-wide degree scan.
 Labels must equal the oracle's Tarjan labels (the planted labels on the
 reference's fuzzy grid), and a reach must claim the same vertices level by
 level as the edge-balanced expansion: equal level counts and decided
@@ -104,6 +105,8 @@ def test_chunked_levels_equal_edge_balanced(ka, O):
         assert st_c.fwbw_levels_pull == st_e.fwbw_levels_pull
         assert st_c.fwbw_levels_coop == st_e.fwbw_levels_coop
         assert st_c.decided_fwbw == st_e.decided_fwbw
+        assert st_c.decided_color == st_e.decided_color
+        assert st_c.color_rounds == st_e.color_rounds
 
 
 def test_chunked_batch(ka, O):
