@@ -109,7 +109,7 @@ typedef struct kspn_graph {
  * m / 8 bytes per direction for it, cleared at the start of each solve. Same labels. */
 #define KSPN_FLAG_PRUNE 4096u
 /* Push levels of a reach whose frontier has no vertex of out-degree above 64 (known from the previous level),
- * and every push level of the colour rounds, run without a degree scan: each workgroup takes 512 consecutive
+ * and every push level of the colour rounds, run without a degree scan: each workgroup takes 256 consecutive
  * frontier entries, scans their degrees in LDS and processes their edges 2048 at a time. Taken after the narrow
  * chains of KSPN_FLAG_NARROW. Suits graphs without high-degree vertices (a chunk holding a hub is one
  * workgroup's work). Same labels. */

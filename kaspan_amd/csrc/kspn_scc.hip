@@ -744,7 +744,7 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
 
 // Push levels over frontiers of small out-degrees, without a degree scan:
 // a workgroup takes CHUNK_N consecutive frontier entries, scans their
-// out-degrees in LDS and applies OP to the chunk's edges WG * 8 at a time
+// out-degrees in LDS and applies OP to the chunk's edges WG * CHUNK_EPT at a time
 // (lane l of step r takes edge base + r * WG + l of the chunk, so
 // consecutive lanes read consecutive entries of a row) with reach_batch or
 // color_batch. The reach levels the host sends here have no vertex of
@@ -756,9 +756,11 @@ __global__ __launch_bounds__(WG) void k_expand(const uint32_t* __restrict__ F, c
 // frontier's edge count, which the host knows, is stored to *stot, where a
 // degree scan would have left its total; with esum, every chunk adds its
 // edge count there (statistics of the colour levels).
-constexpr uint32_t CHUNK_N = 2 * WG;      // frontier entries per chunk, two per thread
-constexpr uint32_t CHUNK_MAXD = 64;       // largest out-degree of a reach frontier the host sends here
-constexpr uint32_t CHUNK_QCAP = 2 * WG * 8;
+constexpr uint32_t CHUNK_PER = 1;                 // frontier entries per thread (1 beat 2 and 4 on planted 5e7)
+constexpr uint32_t CHUNK_N = CHUNK_PER * WG;      // frontier entries per chunk
+constexpr int CHUNK_EPT = 8;                      // edges per thread of a step
+constexpr uint32_t CHUNK_MAXD = 64;               // largest out-degree of a reach frontier the host sends here
+constexpr uint32_t CHUNK_QCAP = 2 * WG * CHUNK_EPT;
 
 template <int OP>
 __global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict__ F, const uint32_t* fcount,
@@ -788,10 +790,11 @@ __global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict_
   const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
   for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
     const uint64_t i0 = (uint64_t)ch * CHUNK_N;
-    uint32_t d[2];
+    uint32_t d[CHUNK_PER];
+    uint64_t sum = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < 2; ++k) {
-      const uint32_t j = 2 * threadIdx.x + k;
+    for (uint32_t k = 0; k < CHUNK_PER; ++k) {
+      const uint32_t j = CHUNK_PER * threadIdx.x + k;
       uint32_t v = 0;
       uint64_t b = 0;
       d[k] = 0;
@@ -802,8 +805,8 @@ __global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict_
       }
       s_v[j] = v;
       s_b[j] = b;
+      sum += d[k];
     }
-    const uint64_t sum = (uint64_t)d[0] + d[1];
     const uint64_t incl = wave_incl_scan64(sum);
     if (lane == WAVE - 1) s_w[wid] = incl;
     __syncthreads();
@@ -812,9 +815,12 @@ __global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict_
       woff += w < wid ? s_w[w] : 0;
       T += s_w[w];
     }
-    const uint64_t ex = woff + incl - sum;
-    s_S[2 * threadIdx.x] = (uint32_t)ex;
-    s_S[2 * threadIdx.x + 1] = (uint32_t)(ex + d[0]);
+    uint64_t ex = woff + incl - sum;
+#pragma unroll
+    for (uint32_t k = 0; k < CHUNK_PER; ++k) {
+      s_S[CHUNK_PER * threadIdx.x + k] = (uint32_t)ex;
+      ex += d[k];
+    }
     if (threadIdx.x == 0) s_S[CHUNK_N] = (uint32_t)T;
     __syncthreads();
     // the in-chunk scan is 32-bit: a chunk of more edges is reported, not expanded past them
@@ -822,12 +828,12 @@ __global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict_
     const uint32_t total = (uint32_t)min(T, 0xFFFFFFFFull);
     if (esum && threadIdx.x == 0 && total) atomicAdd(esum, (unsigned long long)total);
     uint32_t lo = 0;  // each thread's edges come in increasing order: the search only moves forward
-    for (uint32_t base = 0; base < total; base += WG * 8) {
-      uint32_t uu[8], xx[8];
-      uint64_t pp[8];
-      bool ok[8];
+    for (uint32_t base = 0; base < total; base += WG * CHUNK_EPT) {
+      uint32_t uu[CHUNK_EPT], xx[CHUNK_EPT];
+      uint64_t pp[CHUNK_EPT];
+      bool ok[CHUNK_EPT];
 #pragma unroll
-      for (int r = 0; r < 8; ++r) {
+      for (int r = 0; r < CHUNK_EPT; ++r) {
         const uint32_t e = base + threadIdx.x + (uint32_t)r * WG;
         ok[r] = e < total;
         uu[r] = xx[r] = 0;
@@ -843,9 +849,9 @@ __global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict_
           xx[r] = adj[pp[r]];
         }
       }
-      if constexpr (OP == OP_COLOR || OP == OP_COLOR_BW) color_batch<OP, 8>(uu, xx, ok, pp, a, q);
-      else reach_batch<OP, 8, BlockQueue>(uu, xx, ok, pp, a, q, nsum, nmax);
-      bq_maybe_flush(q, next, next_cnt, WG * 8);  // a step appends at most WG * 8 entries
+      if constexpr (OP == OP_COLOR || OP == OP_COLOR_BW) color_batch<OP, CHUNK_EPT>(uu, xx, ok, pp, a, q);
+      else reach_batch<OP, CHUNK_EPT, BlockQueue>(uu, xx, ok, pp, a, q, nsum, nmax);
+      bq_maybe_flush(q, next, next_cnt, WG * CHUNK_EPT);  // a step appends at most WG * CHUNK_EPT entries
     }
     __syncthreads();  // the next chunk rewrites the LDS tables
   }

@@ -1,6 +1,6 @@
 """Chunked push levels (KSPN_FLAG_CHUNKED): a reach frontier whose vertices
 have at most 64 out-edges each, and every colour push level, is expanded by
-chunks of 512 frontier entries with their degrees scanned in LDS, without
+chunks of 256 frontier entries with their degrees scanned in LDS, without
 the device-wide degree scan.
 Labels must equal the oracle's Tarjan labels (the planted labels on the
 reference's fuzzy grid), and a reach must claim the same vertices level by
