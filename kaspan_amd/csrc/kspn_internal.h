@@ -96,6 +96,16 @@ __device__ inline uint64_t wave_sum64(uint64_t v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
   return v;
 }
+// inclusive prefix sum over the wavefront
+__device__ inline uint64_t wave_incl_scan64(uint64_t v) {
+  const uint32_t lane = lane_id();
+  for (uint32_t o = 1; o < WAVE; o <<= 1) {
+    const uint64_t t = __shfl_up(v, o);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
 // block-wide 64-bit sum; every thread calls; result in thread 0
 __device__ inline uint64_t block_sum64(uint64_t v, uint64_t* s_red) {
   v = wave_sum64(v);

@@ -20,16 +20,10 @@ static_assert(P_COUNT == KSPN_MLP_PARAMS, "parameter layout");
 
 // ---------------------------------------------------------------- features
 // In/out degree, their logs, and the 2-hop sums (out-degrees of
-// out-neighbours, in-degrees of in-neighbours). One lane per vertex: a
-// wavefront takes 64 consecutive list entries, each lane sums its own rows
-// (four neighbours in flight at a time), so the rows of neighbouring lanes
-// are read from neighbouring memory. Vertices with a row longer than
-// FEAT_LANE_MAX are summed afterwards by the whole wavefront, one at a
-// time, so a hub costs its degree / 64 steps, not its degree.
+// out-neighbours, in-degrees of in-neighbours), computed by k_features.
 // The neighbours' degrees come from byte arrays (k_deg8: min(degree, 255)),
 // n bytes per direction instead of 8n of offsets, so the random reads stay
 // in the Infinity Cache; a saturated entry is read exactly from the offsets.
-constexpr uint64_t FEAT_LANE_MAX = 64;
 constexpr uint32_t DEG8_SAT = 255;
 
 __global__ void k_deg8(uint32_t n, const uint64_t* __restrict__ fw_off, const uint64_t* __restrict__ bw_off,
@@ -46,32 +40,28 @@ __device__ inline uint64_t deg_of(const uint8_t* __restrict__ d8, const uint64_t
   return d < DEG8_SAT ? (uint64_t)d : off[w + 1] - off[w];
 }
 
-__device__ inline uint64_t row_degree_sum(const uint8_t* __restrict__ d8, const uint64_t* __restrict__ off,
-                                          const uint32_t* __restrict__ adj, uint64_t b, uint64_t e) {
-  uint64_t sum = 0;
-  uint64_t it = b;
-  for (; it + 4 <= e; it += 4) {
-    const uint32_t w0 = adj[it], w1 = adj[it + 1], w2 = adj[it + 2], w3 = adj[it + 3];
-    const uint32_t d0 = d8[w0], d1 = d8[w1], d2 = d8[w2], d3 = d8[w3];
-    sum += deg_of(d8, off, w0, d0) + deg_of(d8, off, w1, d1) + deg_of(d8, off, w2, d2) + deg_of(d8, off, w3, d3);
-  }
-  for (; it < e; ++it) {
-    const uint32_t w = adj[it];
-    sum += deg_of(d8, off, w, d8[w]);
-  }
-  return sum;
-}
+// Features of `count` vertices (list[i], or i without a list), 256 per
+// workgroup step: each thread loads its vertex's two rows, the rows'
+// lengths are scanned in LDS, and the concatenated edges (out-row, then
+// in-row, vertex by vertex) are read FEAT_EPT per thread at a time, their
+// neighbours' degrees summed into the vertex's LDS slot with an LDS atomic.
+// Consecutive lanes read consecutive entries of a row; a long row is
+// shared by many lanes instead of one.
+constexpr int FEAT_EPT = 8;
 
 __global__ __launch_bounds__(WG) void k_features(uint32_t count, const uint32_t* __restrict__ list,
                                                  const uint64_t* __restrict__ fw_off, const uint32_t* __restrict__ fw_adj,
                                                  const uint64_t* __restrict__ bw_off, const uint32_t* __restrict__ bw_adj,
                                                  const uint8_t* __restrict__ out8, const uint8_t* __restrict__ in8,
                                                  float* __restrict__ feat) {
-  const uint32_t lane = lane_id();
-  const uint64_t wave = (blockIdx.x * (uint64_t)WG + threadIdx.x) / WAVE;
-  const uint64_t waves = (uint64_t)gridDim.x * WG / WAVE;
-  for (uint64_t base = wave * WAVE; base < count; base += waves * WAVE) {
-    const uint64_t i = base + lane;
+  __shared__ uint64_t s_fb[WG], s_bb[WG];  // row starts
+  __shared__ uint64_t s_od[WG];            // out-row length (the vertex's edges past it are its in-row)
+  __shared__ uint64_t s_S[WG + 1];         // exclusive prefix of the two rows' lengths
+  __shared__ unsigned long long s_so[WG], s_si[WG];
+  __shared__ uint64_t s_w[WG / WAVE];
+  const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < count; base += (uint64_t)gridDim.x * WG) {
+    const uint64_t i = base + threadIdx.x;
     const bool ok = i < count;
     const uint32_t v = ok ? (list ? list[i] : (uint32_t)i) : 0u;
     uint64_t fb = 0, fe = 0, bb = 0, be = 0;
@@ -81,45 +71,70 @@ __global__ __launch_bounds__(WG) void k_features(uint32_t count, const uint32_t*
       bb = bw_off[v];
       be = bw_off[v + 1];
     }
-    const bool own = ok && fe - fb <= FEAT_LANE_MAX && be - bb <= FEAT_LANE_MAX;
-    uint64_t s2o = 0, s2i = 0;
-    if (own) {
-      s2o = row_degree_sum(out8, fw_off, fw_adj, fb, fe);
-      s2i = row_degree_sum(in8, bw_off, bw_adj, bb, be);
+    const uint64_t d = (fe - fb) + (be - bb);
+    s_fb[threadIdx.x] = fb;
+    s_bb[threadIdx.x] = bb;
+    s_od[threadIdx.x] = fe - fb;
+    s_so[threadIdx.x] = 0;
+    s_si[threadIdx.x] = 0;
+    const uint64_t incl = wave_incl_scan64(d);
+    if (lane == WAVE - 1) s_w[wid] = incl;
+    __syncthreads();
+    uint64_t woff = 0, T = 0;
+    for (uint32_t w = 0; w < WG / WAVE; ++w) {
+      woff += w < wid ? s_w[w] : 0;
+      T += s_w[w];
     }
-    // the long rows: the whole wavefront, one vertex at a time
-    uint64_t big = __ballot(ok && !own);
-    while (big) {
-      const uint32_t j = __ffsll((unsigned long long)big) - 1;
-      big &= big - 1;
-      const uint64_t jfb = __shfl(fb, j), jfe = __shfl(fe, j), jbb = __shfl(bb, j), jbe = __shfl(be, j);
-      uint64_t so = 0, si = 0;
-      for (uint64_t it = jfb + lane; it < jfe; it += WAVE) {
-        const uint32_t w = fw_adj[it];
-        so += deg_of(out8, fw_off, w, out8[w]);
+    s_S[threadIdx.x] = woff + incl - d;
+    if (threadIdx.x == 0) s_S[WG] = T;
+    __syncthreads();
+    uint32_t lo = 0;  // each thread's edges come in increasing order: the search only moves forward
+    for (uint64_t e0 = 0; e0 < T; e0 += (uint64_t)WG * FEAT_EPT) {
+      uint32_t w[FEAT_EPT], src[FEAT_EPT];
+      bool in_row[FEAT_EPT], has[FEAT_EPT];
+#pragma unroll
+      for (int r = 0; r < FEAT_EPT; ++r) {
+        const uint64_t e = e0 + threadIdx.x + (uint64_t)r * WG;
+        has[r] = e < T;
+        w[r] = 0;
+        src[r] = 0;
+        in_row[r] = false;
+        if (has[r]) {
+          uint32_t hi = WG;
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_S[mid] <= e) lo = mid; else hi = mid;
+          }
+          const uint64_t k = e - s_S[lo];
+          src[r] = lo;
+          in_row[r] = k >= s_od[lo];
+          w[r] = in_row[r] ? bw_adj[s_bb[lo] + (k - s_od[lo])] : fw_adj[s_fb[lo] + k];
+        }
       }
-      for (uint64_t it = jbb + lane; it < jbe; it += WAVE) {
-        const uint32_t w = bw_adj[it];
-        si += deg_of(in8, bw_off, w, in8[w]);
-      }
-      so = __shfl(wave_sum64(so), 0);  // the sum is complete in lane 0
-      si = __shfl(wave_sum64(si), 0);
-      if (lane == j) {
-        s2o = so;
-        s2i = si;
+      uint32_t d8[FEAT_EPT];
+#pragma unroll
+      for (int r = 0; r < FEAT_EPT; ++r) d8[r] = has[r] ? (in_row[r] ? in8[w[r]] : out8[w[r]]) : 0u;
+#pragma unroll
+      for (int r = 0; r < FEAT_EPT; ++r) {
+        if (!has[r]) continue;
+        const uint64_t dw = in_row[r] ? deg_of(in8, bw_off, w[r], d8[r]) : deg_of(out8, fw_off, w[r], d8[r]);
+        if (dw) atomicAdd(in_row[r] ? &s_si[src[r]] : &s_so[src[r]], (unsigned long long)dw);
       }
     }
+    __syncthreads();
     if (ok) {
       const uint64_t out = fe - fb, in = be - bb;
-      const float li = log1pf((float)in), lo = log1pf((float)out);
+      const uint64_t s2o = s_so[threadIdx.x], s2i = s_si[threadIdx.x];
+      const float li = log1pf((float)in), lo_ = log1pf((float)out);
       float* f = feat + i * 6;
       f[0] = li;
-      f[1] = lo;
-      f[2] = li * lo * 0.125f;
+      f[1] = lo_;
+      f[2] = li * lo_ * 0.125f;
       f[3] = log1pf((float)s2o);
       f[4] = log1pf((float)s2i);
       f[5] = log1pf((float)(in < out ? in : out));
     }
+    __syncthreads();  // the next step rewrites the LDS tables
   }
 }
 

@@ -573,16 +573,6 @@ __device__ inline void color_batch(const uint32_t (&u)[K], const uint32_t (&x)[K
   }
 }
 
-// inclusive prefix sum over the wavefront
-__device__ inline uint64_t wave_incl_scan64(uint64_t v) {
-  const uint32_t lane = lane_id();
-  for (uint32_t o = 1; o < WAVE; o <<= 1) {
-    const uint64_t t = __shfl_up(v, o);
-    if (lane >= o) v += t;
-  }
-  return v;
-}
-
 constexpr int EXP_EPT = 8;        // edges per thread of a tile (default; 4 and 16 are also built)
 constexpr int EXP_WIN = 1024;     // frontier entries staged in LDS per tile
 
