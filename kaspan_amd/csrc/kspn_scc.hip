@@ -4206,12 +4206,26 @@ int expand(Run& R, const uint32_t* F, const uint32_t* fcount_dev, uint32_t f, co
   if constexpr (OP == OP_COLOR || OP == OP_COLOR_BW) {
     if (R.o.flags & KSPN_FLAG_CHUNKED) {
       // chunks of the frontier, no degree scan and no publication of its
-      // edge count (statistics: summed into cnt64[3], read by color_round)
-      Prof p(c, R.s, kern, 20.0 * f);
-      const uint32_t cgrid = (uint32_t)std::min<uint64_t>(((uint64_t)f + CHUNK_N - 1) / CHUNK_N,
-                                                         (uint64_t)c->num_cu * 16);
-      k_expand_chunk<OP><<<cgrid, WG, 0, R.s>>>(F, fcount_dev, off, adj, a, next, next_cnt, &c->cnt[C_ERR], nullptr,
-                                                 0, R.st ? c->cnt64 + 3 : nullptr);
+      // edge count (statistics: summed into cnt64[3], read by color_round;
+      // profile mode reads it around the launch for the level's bytes)
+      unsigned long long e_before = 0, e_after = 0;
+      if (c->profile) {
+        KSPN_HIP(hipMemcpyAsync(&e_before, c->cnt64 + 3, sizeof(e_before), hipMemcpyDeviceToHost, R.s));
+        KSPN_HIP(hipStreamSynchronize(R.s));
+      }
+      {
+        Prof p(c, R.s, kern, 20.0 * f);
+        const uint32_t cgrid = (uint32_t)std::min<uint64_t>(((uint64_t)f + CHUNK_N - 1) / CHUNK_N,
+                                                           (uint64_t)c->num_cu * 16);
+        k_expand_chunk<OP><<<cgrid, WG, 0, R.s>>>(F, fcount_dev, off, adj, a, next, next_cnt, &c->cnt[C_ERR],
+                                                   nullptr, 0, R.st || c->profile ? c->cnt64 + 3 : nullptr);
+      }
+      if (c->profile) {
+        KSPN_HIP(hipMemcpyAsync(&e_after, c->cnt64 + 3, sizeof(e_after), hipMemcpyDeviceToHost, R.s));
+        KSPN_HIP(hipStreamSynchronize(R.s));
+        if (!c->pending.empty())
+          c->pending.back().bytes = 20.0 * f + (4.0 + state_bytes_per_edge) * (double)(e_after - e_before);
+      }
       if (edges) *edges = 0;
       return KSPN_OK;
     }
@@ -5075,7 +5089,7 @@ int color_round(Run& R, uint32_t* decided) {
   OpArgs a{c->live, c->fwv, c->bwv, c->inq, c->col, R.labels, r, nullptr, nullptr, pid, nullptr, 0u, c->cpw};
   // chunked push levels publish no edge count: they sum it into cnt64[3]
   const bool chunk_stats = R.st && (R.o.flags & KSPN_FLAG_CHUNKED);
-  if (chunk_stats) k_fill_u64<<<1, 64, 0, R.s>>>(c->cnt64 + 3, 1, 0ull);
+  if (chunk_stats || c->profile) k_fill_u64<<<1, 64, 0, R.s>>>(c->cnt64 + 3, 1, 0ull);
   // forward: pull sweep over all live vertices
   KSPN_HIP(hipMemsetAsync(&c->cnt[C_Q0], 0, sizeof(uint32_t), R.s));
   {
