@@ -344,6 +344,8 @@ struct OpArgs {
                                  // once the edge is seen to join two partitions (or reach a decided vertex);
                                  // partitions only split and decided vertices stay decided, so a set edge is
                                  // of no use to any later level of the solve (nullptr: not kept)
+  uint32_t* inq_clear;           // k_expand_chunk<OP_COLOR>: the queue bitmap holding the frontier's bits, cleared
+                                 // as the frontier is read (the level queues into inq, the other bitmap)
 };
 
 // partition id of a vertex that is not live during the multi-pivot rounds
@@ -792,6 +794,9 @@ __global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict_
         v = F[i0 + j];
         b = off[v];
         d[k] = (uint32_t)min(off[v + 1] - b, (uint64_t)0xFFFFFFFFu);
+        if constexpr (OP == OP_COLOR) {
+          if (a.inq_clear) atomicAnd(&a.inq_clear[v >> 5], ~(1u << (v & 31)));
+        }
       }
       s_v[j] = v;
       s_b[j] = b;
@@ -3953,6 +3958,7 @@ int ctx_reserve(Ctx* c, uint32_t n, uint64_t m) {
   KSPN_TRY(dalloc(&c->fwv, words));
   KSPN_TRY(dalloc(&c->bwv, words));
   KSPN_TRY(dalloc(&c->inq, words));
+  KSPN_TRY(dalloc(&c->inq2, words));
   KSPN_TRY(dalloc(&c->fnew, words));
   KSPN_TRY(dalloc(&c->sbm, words));
   KSPN_TRY(dalloc(&c->qa, (uint64_t)nn + 64));
@@ -5103,13 +5109,24 @@ int color_round(Run& R, uint32_t* decided) {
   uint32_t* N = c->qb;
   int fc = C_Q0, nc = C_Q1;
   uint32_t iters = 1;
+  // Chunked push levels alternate between the queue bitmaps inq and inq2:
+  // a level queues into one and clears its frontier's bits in the other as
+  // it reads the frontier, so no clearing launch is needed; inq_f is the
+  // bitmap holding F's bits (nullptr after a pull sweep, whose output sets
+  // none). Otherwise F's bits are cleared before each level.
+  const bool chunk = R.o.flags & KSPN_FLAG_CHUNKED;
+  uint32_t* inq_f = nullptr;
+  if (chunk) k_fill_u32<<<stream_grid(c, R.words), WG, 0, R.s>>>(c->inq2, R.words, 0u);
   while (f > 0) {
+    const bool sweep = (double)f * 2.0 > (double)u;
     // drop F's queue bits so its members can be queued again
-    k_bm_clear_list<<<stream_grid(c, f), WG, 0, R.s>>>(c->inq, F, &c->cnt[fc]);
+    if (!chunk) k_bm_clear_list<<<stream_grid(c, f), WG, 0, R.s>>>(c->inq, F, &c->cnt[fc]);
+    else if (sweep && inq_f) k_bm_clear_list<<<stream_grid(c, f), WG, 0, R.s>>>(inq_f, F, &c->cnt[fc]);
     KSPN_HIP(hipMemsetAsync(&c->cnt[nc], 0, sizeof(uint32_t), R.s));
-    if ((double)f * 2.0 > (double)u) {
+    if (sweep) {
       // large frontier: another pull sweep over every live vertex (half the
       // live list was faster than a quarter or all of it on planted 5e7)
+      inq_f = nullptr;
       Prof p(c, R.s, K_COLOR_PULL, 24.0 * u);
       k_color_pull<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->cpw, N,
                                                       &c->cnt[nc], R.prune ? c->prune_bw : nullptr);
@@ -5117,6 +5134,11 @@ int color_round(Run& R, uint32_t* decided) {
       uint64_t e = 0;
       OpArgs af = a;
       if (R.prune) af.prune = c->prune_fw;
+      if (chunk) {
+        af.inq = inq_f == c->inq ? c->inq2 : c->inq;
+        af.inq_clear = inq_f;
+        inq_f = af.inq;
+      }
       KSPN_TRY(expand<OP_COLOR>(R, F, &c->cnt[fc], f, g->fw_off, g->fw_adj, af, N, &c->cnt[nc], K_EXPAND_COLOR, 8.0, &e));
     }
     KSPN_TRY(read_counters(c, R.s, 2));
@@ -5953,7 +5975,7 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   graph_release(c);
   if (c->aq) (void)hipFree(c->aq);
   if (c->actl) (void)hipFree(c->actl);
-  void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->fnew, c->sbm, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
+  void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->inq2, c->fnew, c->sbm, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
                   c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf, c->pid, c->pbest,
                   c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt, c->vprio, c->ds_st, c->deg8, c->cpw,
                   c->prune_fw, c->prune_bw, c->tstart};
