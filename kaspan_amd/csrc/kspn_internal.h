@@ -295,6 +295,7 @@ struct Ctx {
   uint32_t* bwv = nullptr;   // backward-reached (inside fwv)
   uint32_t* inq = nullptr;   // queue membership
   uint32_t* inq2 = nullptr;  // chunked colour levels: the other queue bitmap (levels alternate between the two)
+  uint32_t* cchain = nullptr;  // chunked colour levels: control words of a device chain of push levels
   uint32_t* fnew = nullptr;  // vertices reached by the last pull level
   uint32_t* sbm = nullptr;   // candidates the last pull sweep left unreached
   // vertex arrays

@@ -754,30 +754,14 @@ constexpr int CHUNK_EPT = 8;                      // edges per thread of a step
 constexpr uint32_t CHUNK_MAXD = 64;               // largest out-degree of a reach frontier the host sends here
 constexpr uint32_t CHUNK_QCAP = 2 * WG * CHUNK_EPT;
 
+// The chunks blockIdx.x, blockIdx.x + gridDim.x, ... of frontier F (f
+// entries); every thread of the workgroup calls. LDS: s_v, s_b, s_S
+// (CHUNK_N, CHUNK_N, CHUNK_N + 1 entries) and s_w (a word per wavefront).
 template <int OP>
-__global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict__ F, const uint32_t* fcount,
-                                                     const uint64_t* __restrict__ off,
-                                                     const uint32_t* __restrict__ adj, OpArgs a,
-                                                     uint32_t* __restrict__ next, uint32_t* next_cnt, uint32_t* err,
-                                                     uint64_t* stot, uint64_t etot, unsigned long long* esum) {
-  __shared__ uint32_t s_v[CHUNK_N];
-  __shared__ uint64_t s_b[CHUNK_N];       // row starts
-  __shared__ uint32_t s_S[CHUNK_N + 1];   // exclusive prefix of the degrees within the chunk
-  __shared__ uint64_t s_w[WG / WAVE];
-  __shared__ uint32_t s_out[CHUNK_QCAP];
-  __shared__ uint32_t s_on, s_ob;
-  __shared__ unsigned long long s_nsum;
-  __shared__ uint32_t s_nmax;
-  BlockQueue q{s_out, &s_on, &s_ob, CHUNK_QCAP};
-  if (threadIdx.x == 0) {
-    s_nsum = 0;
-    s_nmax = 0;
-  }
-  bq_init(q);
-  if (stot && blockIdx.x == 0 && threadIdx.x == 0) *stot = etot;
-  uint64_t nsum = 0;
-  uint32_t nmax = 0;
-  const uint32_t f = *fcount;
+__device__ inline void chunk_body(const uint32_t* __restrict__ F, uint32_t f, const uint64_t* __restrict__ off,
+                                  const uint32_t* __restrict__ adj, const OpArgs& a, BlockQueue& q, uint32_t* next,
+                                  uint32_t* next_cnt, uint32_t* err, unsigned long long* esum, uint32_t* s_v,
+                                  uint64_t* s_b, uint32_t* s_S, uint64_t* s_w, uint64_t& nsum, uint32_t& nmax) {
   const uint32_t nchunks = (uint32_t)(((uint64_t)f + CHUNK_N - 1) / CHUNK_N);
   const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
   for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
@@ -850,6 +834,32 @@ __global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict_
     }
     __syncthreads();  // the next chunk rewrites the LDS tables
   }
+}
+
+template <int OP>
+__global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict__ F, const uint32_t* fcount,
+                                                     const uint64_t* __restrict__ off,
+                                                     const uint32_t* __restrict__ adj, OpArgs a,
+                                                     uint32_t* __restrict__ next, uint32_t* next_cnt, uint32_t* err,
+                                                     uint64_t* stot, uint64_t etot, unsigned long long* esum) {
+  __shared__ uint32_t s_v[CHUNK_N];
+  __shared__ uint64_t s_b[CHUNK_N];       // row starts
+  __shared__ uint32_t s_S[CHUNK_N + 1];   // exclusive prefix of the degrees within the chunk
+  __shared__ uint64_t s_w[WG / WAVE];
+  __shared__ uint32_t s_out[CHUNK_QCAP];
+  __shared__ uint32_t s_on, s_ob;
+  __shared__ unsigned long long s_nsum;
+  __shared__ uint32_t s_nmax;
+  BlockQueue q{s_out, &s_on, &s_ob, CHUNK_QCAP};
+  if (threadIdx.x == 0) {
+    s_nsum = 0;
+    s_nmax = 0;
+  }
+  bq_init(q);
+  if (stot && blockIdx.x == 0 && threadIdx.x == 0) *stot = etot;
+  uint64_t nsum = 0;
+  uint32_t nmax = 0;
+  chunk_body<OP>(F, *fcount, off, adj, a, q, next, next_cnt, err, esum, s_v, s_b, s_S, s_w, nsum, nmax);
   bq_flush(q, next, next_cnt);
   if constexpr (OP == OP_FW || OP == OP_BW || OP == OP_PBW) {
     if (nsum) atomicAdd(&s_nsum, (unsigned long long)nsum);
@@ -857,6 +867,81 @@ __global__ __launch_bounds__(WG) void k_expand_chunk(const uint32_t* __restrict_
     __syncthreads();
     if (threadIdx.x == 0 && s_nsum) atomicAdd(a.nsum, s_nsum);
     if (threadIdx.x == 0 && s_nmax) atomicMax(&a.nsum[1], (unsigned long long)s_nmax);
+  }
+}
+
+// Colour push levels chained on the device (KSPN_FLAG_CHUNKED): launch L
+// of a chain expands the frontier of level L (count ring[L % 3], buffer
+// F0 for even L, F1 for odd) into the other buffer (count ring[(L + 1) % 3])
+// with the chunk body, queueing into inq0 (even L) or inq1 (odd L) and
+// clearing its frontier's bits in the other (for L = 0 only with
+// first_clear), and zeroes ring[(L + 2) % 3] for the launch after it. A
+// level whose frontier is empty or more than half of the u live vertices
+// (the host's sweep rule) ends the chain: it records its index in
+// end_level, and the launches after it return at once.
+struct ColorChain {
+  uint32_t ring[3];
+  uint32_t ended;
+  uint32_t end_level;
+  uint32_t pad[3];
+};
+
+__global__ __launch_bounds__(WG) void k_color_chain(uint32_t* F0, uint32_t* F1, const uint64_t* __restrict__ off,
+                                                    const uint32_t* __restrict__ adj, OpArgs a, uint32_t* inq0,
+                                                    uint32_t* inq1, bool first_clear, ColorChain* ctl, uint32_t u,
+                                                    uint32_t L, uint32_t* err, unsigned long long* esum) {
+  __shared__ uint32_t s_v[CHUNK_N];
+  __shared__ uint64_t s_b[CHUNK_N];
+  __shared__ uint32_t s_S[CHUNK_N + 1];
+  __shared__ uint64_t s_w[WG / WAVE];
+  __shared__ uint32_t s_out[CHUNK_QCAP];
+  __shared__ uint32_t s_on, s_ob, s_f, s_stop;
+  if (threadIdx.x == 0) {
+    const uint32_t ended = ctl->ended;
+    const uint32_t f = ctl->ring[L % 3];
+    s_f = f;
+    s_stop = ended || f == 0 || (double)f * 2.0 > (double)u;
+    if (!ended && s_stop && blockIdx.x == 0) {
+      ctl->ended = 1u;
+      ctl->end_level = L;
+    }
+  }
+  __syncthreads();
+  if (s_stop) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->ring[(L + 2) % 3] = 0u;
+  BlockQueue q{s_out, &s_on, &s_ob, CHUNK_QCAP};
+  bq_init(q);
+  OpArgs b = a;
+  b.inq = (L & 1) ? inq1 : inq0;
+  b.inq_clear = (L == 0 && !first_clear) ? nullptr : ((L & 1) ? inq0 : inq1);
+  uint64_t nsum = 0;
+  uint32_t nmax = 0;
+  uint32_t* N = (L & 1) ? F0 : F1;
+  uint32_t* nc = &ctl->ring[(L + 1) % 3];
+  chunk_body<OP_COLOR>((L & 1) ? F1 : F0, s_f, off, adj, b, q, N, nc, err, esum, s_v, s_b, s_S, s_w, nsum, nmax);
+  bq_flush(q, N, nc);
+}
+
+constexpr uint32_t kColorChain = 8;  // launches per chain of colour push levels
+
+// start of a chain: ring[0] = *fcount, the other slots and the end record zeroed
+__global__ void k_color_chain_init(ColorChain* ctl, const uint32_t* fcount) {
+  if (threadIdx.x == 0) {
+    ctl->ring[0] = *fcount;
+    ctl->ring[1] = 0u;
+    ctl->ring[2] = 0u;
+    ctl->ended = 0u;
+    ctl->end_level = 0u;
+  }
+}
+
+// end of a chain of `len` launches: the level it stopped at (len if none
+// stopped) to *level, that level's frontier count to *fcount
+__global__ void k_color_chain_end(const ColorChain* ctl, uint32_t len, uint32_t* fcount, uint32_t* level) {
+  if (threadIdx.x == 0) {
+    const uint32_t e = ctl->ended ? ctl->end_level : len;
+    *level = e;
+    *fcount = ctl->ring[e % 3];
   }
 }
 
@@ -3959,6 +4044,7 @@ int ctx_reserve(Ctx* c, uint32_t n, uint64_t m) {
   KSPN_TRY(dalloc(&c->bwv, words));
   KSPN_TRY(dalloc(&c->inq, words));
   KSPN_TRY(dalloc(&c->inq2, words));
+  if (!c->cchain) KSPN_TRY(dalloc(&c->cchain, 16));
   KSPN_TRY(dalloc(&c->fnew, words));
   KSPN_TRY(dalloc(&c->sbm, words));
   KSPN_TRY(dalloc(&c->qa, (uint64_t)nn + 64));
@@ -5130,15 +5216,48 @@ int color_round(Run& R, uint32_t* decided) {
       Prof p(c, R.s, K_COLOR_PULL, 24.0 * u);
       k_color_pull<<<stream_grid(c, u), WG, 0, R.s>>>(c->ulist, &c->cnt[C_RES_N], g->bw_off, g->bw_adj, c->cpw, N,
                                                       &c->cnt[nc], R.prune ? c->prune_bw : nullptr);
+    } else if (chunk) {
+      // up to kColorChain push levels chained on the device, one round trip
+      ColorChain* ch = reinterpret_cast<ColorChain*>(c->cchain);
+      uint32_t* inq0 = inq_f == c->inq ? c->inq2 : c->inq;  // level 0 queues into the bitmap without F's bits
+      uint32_t* inq1 = inq0 == c->inq ? c->inq2 : c->inq;
+      OpArgs af = a;
+      if (R.prune) af.prune = c->prune_fw;
+      unsigned long long* esum = R.st || c->profile ? c->cnt64 + 3 : nullptr;
+      unsigned long long e_before = 0, e_after = 0;
+      if (c->profile) {
+        KSPN_HIP(hipMemcpyAsync(&e_before, c->cnt64 + 3, sizeof(e_before), hipMemcpyDeviceToHost, R.s));
+        KSPN_HIP(hipStreamSynchronize(R.s));
+      }
+      k_color_chain_init<<<1, 64, 0, R.s>>>(ch, &c->cnt[fc]);
+      {
+        Prof p(c, R.s, K_EXPAND_COLOR, 20.0 * f);
+        const uint32_t cgrid = (uint32_t)c->num_cu * 8;
+        for (uint32_t L = 0; L < kColorChain; ++L)
+          k_color_chain<<<cgrid, WG, 0, R.s>>>(F, N, g->fw_off, g->fw_adj, af, inq0, inq1, inq_f != nullptr, ch, u, L,
+                                               &c->cnt[C_ERR], esum);
+      }
+      if (c->profile) {
+        KSPN_HIP(hipMemcpyAsync(&e_after, c->cnt64 + 3, sizeof(e_after), hipMemcpyDeviceToHost, R.s));
+        KSPN_HIP(hipStreamSynchronize(R.s));
+        if (!c->pending.empty()) c->pending.back().bytes = 20.0 * f + 12.0 * (double)(e_after - e_before);
+      }
+      k_color_chain_end<<<1, 64, 0, R.s>>>(ch, kColorChain, &c->cnt[fc], &c->cnt[C_AUX]);
+      KSPN_TRY(read_counters(c, R.s, 2));
+      const uint32_t E = c->h_cnt[C_AUX];
+      if (E == 0 || E > kColorChain) {
+        note_error("k_color_chain", "a chain of colour levels ran no level", __FILE__, __LINE__);
+        return KSPN_E_HIP;
+      }
+      f = c->h_cnt[fc];
+      if (E & 1) std::swap(F, N);  // the frontier of level E and its count (now in cnt[fc])
+      inq_f = ((E - 1) & 1) ? inq1 : inq0;
+      iters += E;
+      continue;
     } else {
       uint64_t e = 0;
       OpArgs af = a;
       if (R.prune) af.prune = c->prune_fw;
-      if (chunk) {
-        af.inq = inq_f == c->inq ? c->inq2 : c->inq;
-        af.inq_clear = inq_f;
-        inq_f = af.inq;
-      }
       KSPN_TRY(expand<OP_COLOR>(R, F, &c->cnt[fc], f, g->fw_off, g->fw_adj, af, N, &c->cnt[nc], K_EXPAND_COLOR, 8.0, &e));
     }
     KSPN_TRY(read_counters(c, R.s, 2));
@@ -5975,7 +6094,7 @@ int kspn_ctx_destroy(kspn_ctx* k) {
   graph_release(c);
   if (c->aq) (void)hipFree(c->aq);
   if (c->actl) (void)hipFree(c->actl);
-  void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->inq2, c->fnew, c->sbm, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
+  void* ptrs[] = {c->live, c->fwv, c->bwv, c->inq, c->inq2, c->cchain, c->fnew, c->sbm, c->qa, c->qb, c->qc, c->ulist, c->col, c->deg, c->scan,
                   c->part, c->cnt, c->cub_tmp, c->sub_off, c->sub_adj, c->tj, c->mlp_buf, c->pid, c->pbest,
                   c->pmin, c->pflags, c->pnew, c->vscore, c->vfeat, c->coop, c->pcnt, c->vprio, c->ds_st, c->deg8, c->cpw,
                   c->prune_fw, c->prune_bw, c->tstart};
