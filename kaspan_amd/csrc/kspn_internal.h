@@ -245,6 +245,7 @@ enum Counter : int {
   C_BADID,   // a tail kernel met a listed vertex id >= n and skipped it
   C_AERR,    // the asynchronous reach gave up (1: no progress, 2: queue overflow)
   C_SERR,    // the look-back degree scan met a count or ticket it could not use
+  C_DLIST,   // length of the list of vertices a multi-pivot round decided (k_part_split)
   C_COUNT_ = 28  // at most 32: the publication map holds the persistent kernel's results from word 32 on
 };
 

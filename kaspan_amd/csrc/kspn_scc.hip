@@ -3554,16 +3554,20 @@ __global__ void k_part_prio(uint32_t n, const uint64_t* __restrict__ fw_off, con
   }
 }
 
-// the pivot key of vertex v: (priority << 32) | ~v, so the largest key is
-// the highest priority and, among equals, the smallest id
-__device__ inline unsigned long long part_key(const uint32_t* __restrict__ prio, uint32_t v, uint64_t rnd_seed) {
-  uint32_t pri = prio[v];
+// the pivot key of vertex v of priority pri (prio[v]): (priority << 32) |
+// ~v, so the largest key is the highest priority and, among equals, the
+// smallest id
+__device__ inline unsigned long long part_key_of(uint32_t pri, uint32_t v, uint64_t rnd_seed) {
   if (rnd_seed) {
     uint64_t z = ((uint64_t)rnd_seed << 32 | v) * 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 31)) * 0xBF58476D1CE4E5B9ull;
     pri |= (uint32_t)(z >> 40);
   }
   return ((unsigned long long)pri << 32) | (uint32_t)~v;
+}
+
+__device__ inline unsigned long long part_key(const uint32_t* __restrict__ prio, uint32_t v, uint64_t rnd_seed) {
+  return part_key_of(prio[v], v, rnd_seed);
 }
 
 // Per-workgroup aggregation of (slot, value) updates in an LDS hash table
@@ -3785,51 +3789,82 @@ __global__ __launch_bounds__(WG) void k_part_apply(uint32_t n, uint32_t* live, c
 // 3 * newid[pid] + class (0 unreached, 1 forward only, 2 backward only)
 // and, with `keys`, its pivot key goes to pbest[new pid] (maximum,
 // aggregated): the next round's keys, over pbest zeroed for its ids.
-// Per-workgroup decided counts into partials. (The decided vertices are
-// not listed here: one counter for the whole device would take an atomic
-// per wavefront; k_part_label finds them in the bitmaps instead.)
+// Per-workgroup decided counts into partials; the decided vertices are
+// listed in dlist (count *dcount), staged in LDS with one atomic per flush.
+constexpr int PS_U = 4;                               // k_part_split: vertices per thread and step
+constexpr uint32_t PS_SPAN = (uint32_t)WG * PS_U;     // vertices per workgroup and step
+
 __global__ __launch_bounds__(WG) void k_part_split(uint32_t n, uint32_t* live, const uint32_t* __restrict__ fwv,
                                                    const uint32_t* __restrict__ bwv, uint32_t* __restrict__ pid,
                                                    const uint32_t* __restrict__ newid, uint32_t* pmin,
                                                    const uint32_t* __restrict__ prio, uint64_t rnd_seed, bool keys,
-                                                   unsigned long long* pbest, uint64_t* __restrict__ partials) {
+                                                   unsigned long long* pbest, uint64_t* __restrict__ partials,
+                                                   uint32_t* __restrict__ dlist, uint32_t* dcount) {
   __shared__ uint32_t s_mk[AGG_SLOTS];
   __shared__ uint32_t s_mv[AGG_SLOTS];
   __shared__ uint32_t s_bk[AGG_SLOTS];
   __shared__ unsigned long long s_bv[AGG_SLOTS];
   __shared__ uint64_t s_red[WG / WAVE];
+  __shared__ uint32_t s_dq[2 * PS_SPAN];
+  __shared__ uint32_t s_dn, s_db;
+  BlockQueue dq{s_dq, &s_dn, &s_db, 2 * PS_SPAN};
   agg_init<uint32_t>(s_mk, s_mv, 0xFFFFFFFFu);
   agg_init<unsigned long long>(s_bk, s_bv, 0ull);
-  const uint64_t pad = ((uint64_t)n + WG - 1) / WG * WG;
+  bq_init(dq);
+  // PS_U vertices per thread and step (one per 256-vertex tile): the
+  // loads of each kind are issued for all of them before any is used
+  constexpr uint64_t SPAN = PS_SPAN;
+  const uint64_t pad = ((uint64_t)n + SPAN - 1) / SPAN * SPAN;
   const uint32_t lane = lane_id();
   uint64_t count = 0;
-  for (uint64_t base = (uint64_t)blockIdx.x * WG; base < pad; base += (uint64_t)gridDim.x * WG) {
-    const uint64_t v = base + threadIdx.x;
-    const uint32_t lw = v < n ? live[v >> 5] : 0u;
-    const bool is_live = v < n && ((lw >> (v & 31)) & 1u);
-    bool dec = false;
-    uint32_t k = 0, np = 0;
-    unsigned long long key = 0;
-    if (is_live) {
-      const bool f = bm_get(fwv, (uint32_t)v), b = bm_get(bwv, (uint32_t)v);
-      k = newid[pid[v]];
-      dec = f && b;
-      if (!dec) {
-        np = 3u * k + (f ? 1u : (b ? 2u : 0u));
-        pid[v] = np;
-        if (keys) key = part_key(prio, (uint32_t)v, rnd_seed);
+  for (uint64_t base = (uint64_t)blockIdx.x * SPAN; base < pad; base += (uint64_t)gridDim.x * SPAN) {
+    uint32_t lw[PS_U], fw[PS_U], bw[PS_U], p[PS_U], k[PS_U], pr[PS_U];
+    bool is_live[PS_U];
+#pragma unroll
+    for (int u = 0; u < PS_U; ++u) {
+      const uint64_t v = base + (uint64_t)u * WG + threadIdx.x;
+      lw[u] = v < n ? live[v >> 5] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < PS_U; ++u) {
+      const uint64_t v = base + (uint64_t)u * WG + threadIdx.x;
+      is_live[u] = v < n && ((lw[u] >> (v & 31)) & 1u);
+      fw[u] = is_live[u] ? fwv[v >> 5] : 0u;
+      bw[u] = is_live[u] ? bwv[v >> 5] : 0u;
+      p[u] = is_live[u] ? pid[v] : 0u;
+      pr[u] = is_live[u] && keys ? prio[v] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < PS_U; ++u) k[u] = is_live[u] ? newid[p[u]] : 0u;
+#pragma unroll
+    for (int u = 0; u < PS_U; ++u) {
+      const uint64_t v = base + (uint64_t)u * WG + threadIdx.x;
+      bool dec = false;
+      uint32_t np = 0;
+      unsigned long long key = 0;
+      if (is_live[u]) {
+        const bool f = (fw[u] >> (v & 31)) & 1u, b = (bw[u] >> (v & 31)) & 1u;
+        dec = f && b;
+        if (!dec) {
+          np = 3u * k[u] + (f ? 1u : (b ? 2u : 0u));
+          pid[v] = np;
+          if (keys) key = part_key_of(pr[u], (uint32_t)v, rnd_seed);
+        }
+      }
+      agg_wave_put<uint32_t, false>(s_mk, s_mv, pmin, dec, k[u], (uint32_t)v);
+      if (keys) agg_wave_put<unsigned long long, true>(s_bk, s_bv, pbest, is_live[u] && !dec, np, key);
+      bq_push(dq, dec, (uint32_t)v);
+      const uint64_t mask = __ballot(dec);
+      if (mask) {
+        const uint64_t word0 = (v - lane) >> 5;
+        if (lane == 0 && (uint32_t)mask) live[word0] = lw[u] & ~(uint32_t)mask;
+        if (lane == 32 && (uint32_t)(mask >> 32)) live[word0 + 1] = lw[u] & ~(uint32_t)(mask >> 32);
+        count += dec;
       }
     }
-    agg_wave_put<uint32_t, false>(s_mk, s_mv, pmin, dec, k, (uint32_t)v);
-    if (keys) agg_wave_put<unsigned long long, true>(s_bk, s_bv, pbest, is_live && !dec, np, key);
-    const uint64_t mask = __ballot(dec);
-    if (mask) {
-      const uint64_t word0 = (base + threadIdx.x - lane) >> 5;
-      if (lane == 0 && (uint32_t)mask) live[word0] = lw & ~(uint32_t)mask;
-      if (lane == 32 && (uint32_t)(mask >> 32)) live[word0 + 1] = lw & ~(uint32_t)(mask >> 32);
-      count += dec;
-    }
+    bq_maybe_flush(dq, dlist, dcount, PS_SPAN);  // a step lists at most PS_SPAN vertices
   }
+  bq_flush(dq, dlist, dcount);
   agg_flush<uint32_t, false>(s_mk, s_mv, pmin);
   if (keys) agg_flush<unsigned long long, true>(s_bk, s_bv, pbest);
   count = block_sum64(count, s_red);
@@ -3841,16 +3876,33 @@ __global__ __launch_bounds__(WG) void k_part_split(uint32_t n, uint32_t* live, c
 }
 
 // the vertices k_part_split decided (forward and backward reached: every
-// reached vertex was live when the round began): label = the smallest
-// decided id of their partition (pid still names the round's partition)
-__global__ void k_part_label(uint32_t n, const uint32_t* __restrict__ fwv, const uint32_t* __restrict__ bwv,
-                             uint32_t* __restrict__ pid, const uint32_t* __restrict__ newid,
-                             const uint32_t* __restrict__ pmin, uint32_t* __restrict__ labels) {
-  for (uint64_t v = blockIdx.x * (uint64_t)WG + threadIdx.x; v < n; v += (uint64_t)gridDim.x * WG) {
-    const uint32_t w = fwv[v >> 5] & bwv[v >> 5];
-    if ((w >> (v & 31)) & 1u) {
-      labels[v] = pmin[newid[pid[v]]];
-      pid[v] = DEAD_PID;  // no longer live
+// reached vertex was live when the round began), listed in dlist: label =
+// the smallest decided id of their partition (pid still names the round's
+// partition); PL_U per thread and step, each kind of load issued for all
+__global__ void k_part_label(const uint32_t* __restrict__ dlist, const uint32_t* dcount, uint32_t* __restrict__ pid,
+                             const uint32_t* __restrict__ newid, const uint32_t* __restrict__ pmin,
+                             uint32_t* __restrict__ labels) {
+  constexpr int PL_U = 4;
+  constexpr uint64_t SPAN = (uint64_t)WG * PL_U;
+  const uint32_t cnt = *dcount;
+  for (uint64_t base = blockIdx.x * SPAN; base < cnt; base += (uint64_t)gridDim.x * SPAN) {
+    uint32_t v[PL_U], k[PL_U];
+    bool ok[PL_U];
+#pragma unroll
+    for (int u = 0; u < PL_U; ++u) {
+      const uint64_t i = base + (uint64_t)u * WG + threadIdx.x;
+      ok[u] = i < cnt;
+      v[u] = ok[u] ? dlist[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < PL_U; ++u) k[u] = ok[u] ? pid[v[u]] : 0u;
+#pragma unroll
+    for (int u = 0; u < PL_U; ++u) k[u] = ok[u] ? newid[k[u]] : 0u;
+#pragma unroll
+    for (int u = 0; u < PL_U; ++u) {
+      if (!ok[u]) continue;
+      labels[v[u]] = pmin[k[u]];
+      pid[v[u]] = DEAD_PID;  // no longer live
     }
   }
 }
@@ -5003,12 +5055,15 @@ int pfwbw_rounds(Run& R, uint32_t K, uint32_t np, uint32_t* decided_total) {
         Prof p(c, R.s, K_BITMAP, 0.5 * R.n + 16.0 * R.live_n + 12.0 * np + 8.0 * S_next);
         k_fill_u32<<<stream_grid(c, np), WG, 0, R.s>>>(c->pmin, np, 0xFFFFFFFFu);
         if (more) k_fill_u64<<<stream_grid(c, S_next), WG, 0, R.s>>>((unsigned long long*)c->pbest, S_next, 0ull);
+        k_set_u32<<<1, 64, 0, R.s>>>(&c->cnt[C_DLIST], 0u);
+        // the decided list goes to qb, free between the backward reach and the next round's
         k_part_split<<<gridn, WG, 0, R.s>>>(R.n, c->live, c->fwv, c->bwv, c->pid, c->pnew, c->pmin, c->vprio,
-                                            part_seed(R, round + 1), more, (unsigned long long*)c->pbest, c->part);
+                                            part_seed(R, round + 1), more, (unsigned long long*)c->pbest, c->part,
+                                            c->qb, &c->cnt[C_DLIST]);
       }
       {
-        Prof p(c, R.s, K_LABEL, R.n / 4.0 + 16.0 * np);
-        k_part_label<<<gridn, WG, 0, R.s>>>(R.n, c->fwv, c->bwv, c->pid, c->pnew, c->pmin, R.labels);
+        Prof p(c, R.s, K_LABEL, 16.0 * np);
+        k_part_label<<<gridn, WG, 0, R.s>>>(c->qb, &c->cnt[C_DLIST], c->pid, c->pnew, c->pmin, R.labels);
         if (more)
           k_part_used<<<stream_grid(c, (uint64_t)S_next + 1), WG, 0, R.s>>>(S_next, (unsigned long long*)c->pbest,
                                                                              c->pflags);
