@@ -37,10 +37,11 @@ METRIC = "edges/sec to full SCC labels on 1 MI355X; learned vs plain pivots, sam
 HBM_PEAK_GBS = 8000.0
 # library profile names -> prefixes of the kernel symbols rocprofv3 reports for them (every instantiation)
 ROCPROF_NAMES = {"pull_fw": ["kspn::k_pull<0,"], "pull_bw": ["kspn::k_pull<1,"], "trim_first": ["kspn::k_trim_first"],
-                 "expand_fw": ["kspn::k_expand<0,", "kspn::k_expand_chunk<0>"],
-                 "expand_bw": ["kspn::k_expand<1,", "kspn::k_expand<5,", "kspn::k_expand_chunk<1>",
-                               "kspn::k_expand_chunk<5>"],
-                 "expand_color": ["kspn::k_expand<3,"], "expand_color_bw": ["kspn::k_expand<4,"],
+                 "expand_fw": ["kspn::k_expand<0,"], "expand_bw": ["kspn::k_expand<1,", "kspn::k_expand<5,"],
+                 "expand_chunk_fw": ["kspn::k_expand_chunk<0>"],
+                 "expand_chunk_bw": ["kspn::k_expand_chunk<1>", "kspn::k_expand_chunk<5>"],
+                 "expand_color": ["kspn::k_expand<3,", "kspn::k_expand_chunk<3>"],
+                 "expand_color_bw": ["kspn::k_expand<4,", "kspn::k_expand_chunk<4>"],
                  "label_scc": ["kspn::k_label_bits", "kspn::k_label_prep", "kspn::k_part_apply", "kspn::k_part_label"],
                  "tail_solve": ["kspn::k_tail_solve"], "trim_sweep": ["kspn::k_trim_sweep<"],
                  "reach_coop": ["kspn::k_reach_coop<", "kspn::k_chain_level<", "kspn::k_narrow_level<"],

@@ -4002,13 +4002,14 @@ namespace {
 enum KernId : int {
   K_TRIM_FIRST = 0, K_PIVOT, K_BITMAP, K_FRONTIER_DEG, K_SCAN, K_EXPAND_FW, K_EXPAND_BW, K_PULL_FW, K_PULL_BW,
   K_TRIM_SWEEP, K_TRIM2, K_EXPAND_NEIGH, K_COLOR_INIT, K_COLOR_PULL, K_EXPAND_COLOR, K_COLOR_ROOTS,
-  K_EXPAND_COLOR_BW, K_LIVE_LIST, K_CANON, K_RESIDUAL, K_TARJAN, K_LABEL, K_TAIL, K_COOP, K_PULL_LIST, K_COUNT
+  K_EXPAND_COLOR_BW, K_LIVE_LIST, K_CANON, K_RESIDUAL, K_TARJAN, K_LABEL, K_TAIL, K_COOP, K_PULL_LIST,
+  K_CHUNK_FW, K_CHUNK_BW, K_COUNT
 };
 const char* kKernNames[K_COUNT] = {"trim_first", "pivot_reduce", "bitmap_ops", "frontier_deg", "scan", "expand_fw",
                                    "expand_bw", "pull_fw", "pull_bw", "trim_sweep", "trim2", "expand_neigh",
                                    "color_init", "color_pull", "expand_color", "color_roots", "expand_color_bw",
                                    "live_list", "canonicalize", "residual_build", "tarjan", "label_scc", "tail_solve",
-                                   "reach_coop", "pull_list"};
+                                   "reach_coop", "pull_list", "expand_chunk_fw", "expand_chunk_bw"};
 
 struct Prof {
   Ctx* c;
@@ -4743,7 +4744,6 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       // nothing expanded (not expected: the bounds were checked): one level on the host
     }
     if (!pull) {
-      int kern = dir == 0 ? K_EXPAND_FW : K_EXPAND_BW;
       // tile width: the options' or, by default, the widest that still
       // gives every CU a few tiles (narrow tiles fill the device on
       // mid-sized frontiers, wide ones keep more loads in flight per thread)
@@ -4754,6 +4754,7 @@ int reach(Run& R, int dir, uint32_t f0, const uint32_t* pid, uint32_t* fw_levels
       }
       // a frontier of small degrees: chunks of entries, no degree scan
       const bool chunked = (R.o.flags & KSPN_FLAG_CHUNKED) && known && !scanned && maxd <= CHUNK_MAXD;
+      const int kern = chunked ? (dir == 0 ? K_CHUNK_FW : K_CHUNK_BW) : (dir == 0 ? K_EXPAND_FW : K_EXPAND_BW);
       bool tiles = false;  // the scan recorded the tiles' frontier windows
       if (!scanned && !chunked) {
         const uint32_t shift = ept == 2 ? 9u : (ept == 4 ? 10u : (ept == 16 ? 12u : 11u));

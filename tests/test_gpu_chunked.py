@@ -118,3 +118,15 @@ def test_chunked_batch(ka, O):
     for flags in (CHUNKED, CHUNKED | NARROW | PRUNE):
         lab, st = ka.scc_batch(gb, flags=flags, alpha=1e-9)
         assert np.array_equal(lab_np(lab), ref), flags
+
+
+def test_chunked_profile_names(ka, O):
+    """chunked levels are timed under their own profile names, so a level
+    count and an average launch time name one kernel each"""
+    g = ka.generate("gnm", 61, n=200000, draws=1000000)
+    lab, st = ka.scc(g, flags=CHUNKED | NO_COOP, alpha=1e-9, profile=True)
+    k = {x["name"]: x["launches"] for x in st.as_dict()["kernels"]}
+    assert k.get("expand_chunk_fw", 0) > 0 and k.get("expand_chunk_bw", 0) > 0, k
+    lab, st = ka.scc(g, flags=NO_COOP, alpha=1e-9, profile=True)
+    k = {x["name"]: x["launches"] for x in st.as_dict()["kernels"]}
+    assert k.get("expand_fw", 0) > 0 and "expand_chunk_fw" not in k and "expand_chunk_bw" not in k, k
